@@ -1,0 +1,95 @@
+// Shared device/host helpers for libhfgpu (gfx950, FP64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace hfgpu {
+
+// Interleaved complex double, layout-compatible with numpy complex128.
+struct cd {
+    double x, y;
+};
+
+__host__ __device__ __forceinline__ cd mk(double x, double y) { return cd{x, y}; }
+__host__ __device__ __forceinline__ cd cadd(cd a, cd b) { return cd{a.x + b.x, a.y + b.y}; }
+__host__ __device__ __forceinline__ cd csub(cd a, cd b) { return cd{a.x - b.x, a.y - b.y}; }
+__host__ __device__ __forceinline__ cd cmul(cd a, cd b) {
+    return cd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__host__ __device__ __forceinline__ cd cconj(cd a) { return cd{a.x, -a.y}; }
+__host__ __device__ __forceinline__ cd cscale(cd a, double s) { return cd{a.x * s, a.y * s}; }
+__host__ __device__ __forceinline__ cd cneg(cd a) { return cd{-a.x, -a.y}; }
+// a * conj(b)
+__host__ __device__ __forceinline__ cd cmulc(cd a, cd b) {
+    return cd{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+__host__ __device__ __forceinline__ double cnorm(cd a) { return a.x * a.x + a.y * a.y; }
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, const char* what, const char* file, int line) {
+    if (e != hipSuccess)
+        throw DeviceError(std::string(what) + " failed: " + hipGetErrorString(e) + " (" +
+                          file + ":" + std::to_string(line) + ")");
+}
+
+#define HFGPU_CHECK(expr) ::hfgpu::hip_check((expr), #expr, __FILE__, __LINE__)
+#define HFGPU_LAUNCH_CHECK() ::hfgpu::hip_check(hipGetLastError(), "kernel launch", __FILE__, __LINE__)
+
+inline unsigned cdiv(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+// Block-wide sum of one double, result valid in every thread.  Fixed
+// reduction tree, so the result is deterministic for a given launch shape.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double* red /* BLOCK/64 slots */) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double block_max(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = red[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) s = fmax(s, red[w]);
+    __syncthreads();
+    return s;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double block_min(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = red[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) s = fmin(s, red[w]);
+    __syncthreads();
+    return s;
+}
+
+// Wave-wide sum (64 lanes), result valid in every lane.
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace hfgpu

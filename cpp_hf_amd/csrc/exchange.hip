@@ -1,0 +1,531 @@
+// Exchange self-energy Sigma = -IFFT2[FFT2(P - refP) * VR] on gfx950, and the
+// Fock assembly that follows it (reference: selfenergy.hpp:91-171,
+// fock.hpp:34-299).
+//
+// Data layout in HBM is the reference's C order, (nk1, nk2, nb, nb) with the
+// band pair innermost.  The 2-D transform runs as three passes over a
+// channel-major workspace T(nk1, nk2, nch):
+//   A  rows:  load P - refP for a tile of C channels of one k1 row
+//             (C * 16 B contiguous per k2), FFT along k2 in LDS, store T;
+//             optionally the weighted row sums feeding Hartree/contact terms.
+//   B  cols:  load one k2 column of T, FFT along k1, multiply by VR, inverse
+//             FFT along k1, store T.  The V(q) product is fused between the
+//             forward and inverse transforms, so the k-space product never
+//             makes its own trip to HBM.
+//   C  rows:  inverse FFT along k2, scale by -1/(nk1 nk2), and write Sigma
+//             (plus F, for the fused Fock epilogue).
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "exchange.h"
+
+namespace hfgpu {
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+
+FftAxis AxisPlan::axis() const {
+    FftAxis a{};
+    a.n = n;
+    a.nstage = static_cast<int>(radices.size());
+    for (size_t i = 0; i < radices.size(); ++i) a.radix[i] = radices[i];
+    a.tw = d_tw;
+    return a;
+}
+
+void AxisPlan::init(int n_) {
+    release();
+    n = n_;
+    radices.clear();
+    int m = n;
+    while (m % 8 == 0) { radices.push_back(8); m /= 8; }
+    while (m % 4 == 0) { radices.push_back(4); m /= 4; }
+    while (m % 2 == 0) { radices.push_back(2); m /= 2; }
+    for (int f : {3, 5, 7, 11, 13})
+        while (m % f == 0) { radices.push_back(f); m /= f; }
+    if (m != 1)
+        throw std::invalid_argument("k-mesh size " + std::to_string(n) +
+                                    " has a prime factor above 13, which the GPU FFT does not support");
+    if (radices.size() > static_cast<size_t>(kMaxStages))
+        throw std::invalid_argument("k-mesh size too large for the GPU FFT");
+    std::vector<cd> tw(static_cast<size_t>(n));
+    for (int k = 0; k < n; ++k) {
+        const long double a = -2.0L * 3.141592653589793238462643383279502884L *
+                              static_cast<long double>(k) / static_cast<long double>(n);
+        tw[k] = cd{static_cast<double>(std::cos(a)), static_cast<double>(std::sin(a))};
+    }
+    HFGPU_CHECK(hipMalloc(&d_tw, sizeof(cd) * tw.size()));
+    HFGPU_CHECK(hipMemcpy(d_tw, tw.data(), sizeof(cd) * tw.size(), hipMemcpyHostToDevice));
+}
+
+void AxisPlan::release() {
+    if (d_tw) (void)hipFree(d_tw);
+    d_tw = nullptr;
+}
+
+static bool tile_fits(const AxisPlan& ax, int C, int threads) {
+    for (int r : ax.radices) {
+        const long need = static_cast<long>(ax.n / r) * C;
+        if (need > static_cast<long>(maxb_of(r)) * threads) return false;
+    }
+    return true;
+}
+
+void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
+    release();
+    nk1 = nk1_;
+    nk2 = nk2_;
+    nb = nb_;
+    packed = packed_;
+    std::vector<int> src, ca, cb;
+    for (int a = 0; a < nb; ++a)
+        for (int b = packed ? a : 0; b < nb; ++b) {
+            src.push_back(a * nb + b);
+            ca.push_back(a);
+            cb.push_back(b);
+        }
+    nch = static_cast<int>(src.size());
+    ax1.init(nk1);
+    ax2.init(nk2);
+    // Widest tile of <= 16 channels whose stages fit the per-thread butterfly
+    // budget at 256 threads (up to 1024 threads for very long axes).
+    int cmax = 1;
+    while (cmax < nch && cmax < 16) cmax <<= 1;
+    int C = cmax, threads = 256;
+    while (C > 1 && !(tile_fits(ax1, C, threads) && tile_fits(ax2, C, threads))) C >>= 1;
+    while (!(tile_fits(ax1, C, threads) && tile_fits(ax2, C, threads))) {
+        threads *= 2;
+        if (threads > 1024) throw std::invalid_argument("k-mesh too large for the GPU FFT tile");
+    }
+    clog = 0;
+    while ((1 << clog) < C) ++clog;
+    nblk = (nch + C - 1) / C;
+    threads_rows = threads_cols = threads;
+    lds_rows = sizeof(cd) * (static_cast<size_t>(nk2) * C + nk2);
+    lds_cols = sizeof(cd) * (static_cast<size_t>(nk1) * C + nk1);
+    if (lds_rows > 160 * 1024 || lds_cols > 160 * 1024)
+        throw std::invalid_argument("k-mesh too large for an LDS-resident FFT line");
+    HFGPU_CHECK(hipMalloc(&d_chan_src, sizeof(int) * nch));
+    HFGPU_CHECK(hipMalloc(&d_chan_a, sizeof(int) * nch));
+    HFGPU_CHECK(hipMalloc(&d_chan_b, sizeof(int) * nch));
+    HFGPU_CHECK(hipMemcpy(d_chan_src, src.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
+    HFGPU_CHECK(hipMemcpy(d_chan_a, ca.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
+    HFGPU_CHECK(hipMemcpy(d_chan_b, cb.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
+}
+
+void ExchangeGeom::release() {
+    if (d_chan_src) (void)hipFree(d_chan_src);
+    if (d_chan_a) (void)hipFree(d_chan_a);
+    if (d_chan_b) (void)hipFree(d_chan_b);
+    d_chan_src = d_chan_a = d_chan_b = nullptr;
+    ax1.release();
+    ax2.release();
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void load_twiddles(cd* s_tw, const cd* tw, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s_tw[i] = tw[i];
+}
+
+// Pass A.  grid = (nblk, nk1).
+__global__ void k_rows_fwd(FftAxis ax2, int nk2, int nb2, int nch, int clog,
+                           const int* __restrict__ chan_src, const cd* __restrict__ P,
+                           const cd* __restrict__ refP, cd* __restrict__ T,
+                           cd* __restrict__ partial, const double* __restrict__ w2d,
+                           int do_fft) {
+    extern __shared__ cd smem[];
+    const int C = 1 << clog;
+    cd* s = smem;
+    cd* s_tw = smem + nk2 * C;
+    const int k1 = blockIdx.y;
+    const int c0 = blockIdx.x * C;
+    if (do_fft) load_twiddles(s_tw, ax2.tw, nk2);
+    for (int e = threadIdx.x; e < nk2 * C; e += blockDim.x) {
+        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
+        cd v{0.0, 0.0};
+        if (ch < nch) {
+            const size_t off = static_cast<size_t>(k1 * nk2 + i) * nb2 + chan_src[ch];
+            v = P[off];
+            if (refP) v = csub(v, refP[off]);
+        }
+        s[e] = v;
+    }
+    __syncthreads();
+    if (partial && threadIdx.x < C) {
+        const int ch = c0 + threadIdx.x;
+        if (ch < nch) {
+            cd acc{0.0, 0.0};
+            for (int i = 0; i < nk2; ++i) {
+                const double w = w2d[k1 * nk2 + i];
+                const cd v = s[(i << clog) + threadIdx.x];
+                acc.x += w * v.x;
+                acc.y += w * v.y;
+            }
+            partial[static_cast<size_t>(k1) * nch + ch] = acc;
+        }
+    }
+    if (!do_fft) return;
+    __syncthreads();
+    fft_tile<false>(s, s_tw, ax2, clog);
+    for (int e = threadIdx.x; e < nk2 * C; e += blockDim.x) {
+        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
+        if (ch < nch) T[static_cast<size_t>(k1 * nk2 + i) * nch + ch] = s[e];
+    }
+}
+
+// Pass B.  grid = (nblk, nk2).
+__global__ void k_cols(FftAxis ax1, int nk1, int nk2, int nch, int clog,
+                       cd* __restrict__ T, const cd* __restrict__ VR, int nvr) {
+    extern __shared__ cd smem[];
+    const int C = 1 << clog;
+    cd* s = smem;
+    cd* s_tw = smem + nk1 * C;
+    const int k2 = blockIdx.y;
+    const int c0 = blockIdx.x * C;
+    load_twiddles(s_tw, ax1.tw, nk1);
+    for (int e = threadIdx.x; e < nk1 * C; e += blockDim.x) {
+        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
+        cd v{0.0, 0.0};
+        if (ch < nch) v = T[static_cast<size_t>(i * nk2 + k2) * nch + ch];
+        s[e] = v;
+    }
+    __syncthreads();
+    fft_tile<false>(s, s_tw, ax1, clog);
+    for (int e = threadIdx.x; e < nk1 * C; e += blockDim.x) {
+        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
+        if (ch < nch) {
+            const size_t q = static_cast<size_t>(i * nk2 + k2);
+            const cd v = nvr == 1 ? VR[q] : VR[q * nvr + ch];
+            s[e] = cmul(s[e], v);
+        }
+    }
+    __syncthreads();
+    fft_tile<true>(s, s_tw, ax1, clog);
+    for (int e = threadIdx.x; e < nk1 * C; e += blockDim.x) {
+        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
+        if (ch < nch) T[static_cast<size_t>(i * nk2 + k2) * nch + ch] = s[e];
+    }
+}
+
+// Pass C.  grid = (nblk, nk1).
+__global__ void k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog,
+                           const int* __restrict__ chan_src, const int* __restrict__ chan_a,
+                           const int* __restrict__ chan_b, const cd* __restrict__ T,
+                           int epilogue, int shift, double inv_n, cd* __restrict__ Sigma,
+                           cd* __restrict__ F, const cd* __restrict__ h,
+                           const cd* __restrict__ SC, const double* __restrict__ hdiag) {
+    extern __shared__ cd smem[];
+    const int C = 1 << clog;
+    cd* s = smem;
+    cd* s_tw = smem + nk2 * C;
+    const int k1 = blockIdx.y;
+    const int c0 = blockIdx.x * C;
+    const int nb2 = nb * nb;
+    load_twiddles(s_tw, ax2.tw, nk2);
+    for (int e = threadIdx.x; e < nk2 * C; e += blockDim.x) {
+        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
+        cd v{0.0, 0.0};
+        if (ch < nch) v = T[static_cast<size_t>(k1 * nk2 + i) * nch + ch];
+        s[e] = v;
+    }
+    __syncthreads();
+    fft_tile<true>(s, s_tw, ax2, clog);
+    // destination row (ifftshift remaps source (i, j) -> (i - n/2, j - n/2))
+    const int d1 = shift ? (k1 + nk1 - nk1 / 2) % nk1 : k1;
+    for (int e = threadIdx.x; e < nk2 * C; e += blockDim.x) {
+        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
+        if (ch >= nch) continue;
+        const cd y = s[e];
+        // reference order: ifftn = backward * (1/N), then negate
+        const cd sig{-(y.x * inv_n), -(y.y * inv_n)};
+        const int d2 = shift ? (i + nk2 - nk2 / 2) % nk2 : i;
+        const size_t kb = static_cast<size_t>(d1 * nk2 + d2) * nb2;
+        if (epilogue == EPI_RAW_FULL) {
+            Sigma[kb + chan_src[ch]] = sig;
+        } else {
+            const int a = chan_a[ch], b = chan_b[ch];
+            const size_t iab = kb + a * nb + b, iba = kb + b * nb + a;
+            if (epilogue == EPI_RAW_PACKED) {
+                Sigma[iab] = sig;
+                if (a != b) Sigma[iba] = cconj(sig);
+            } else {
+                cd s_ab = sig, s_ba = cconj(sig);
+                if (SC) {
+                    s_ab = cadd(s_ab, SC[a * nb + b]);
+                    s_ba = cadd(s_ba, SC[b * nb + a]);
+                }
+                cd x_ab = cadd(h[iab], s_ab);
+                cd x_ba = cadd(h[iba], s_ba);
+                if (hdiag && a == b) {
+                    x_ab.x += hdiag[a];
+                    x_ba.x += hdiag[a];
+                }
+                const cd f{0.5 * (x_ab.x + x_ba.x), 0.5 * (x_ab.y - x_ba.y)};
+                Sigma[iab] = s_ab;
+                F[iab] = f;
+                if (a != b) {
+                    Sigma[iba] = s_ba;
+                    F[iba] = cconj(f);
+                }
+            }
+        }
+    }
+}
+
+// rho_bar from row sums, then Hartree diagonal and contact self-energy.
+// One workgroup of 256 threads; nb <= 32.
+__global__ void k_small_terms(const cd* __restrict__ partial, int nrows, int nch,
+                              const int* __restrict__ chan_a, const int* __restrict__ chan_b,
+                              int nb, int packed, int include_hartree, int has_contact,
+                              const double* __restrict__ HH, int n_contact,
+                              const double* __restrict__ cg, const cd* __restrict__ Oi,
+                              const cd* __restrict__ Oj, double* __restrict__ hdiag,
+                              cd* __restrict__ SC) {
+    __shared__ cd rho[32 * 32];
+    __shared__ cd tmp[32 * 32];
+    __shared__ cd acc[32 * 32];
+    __shared__ cd trv;
+    const int nb2 = nb * nb;
+    for (int ch = threadIdx.x; ch < nch; ch += blockDim.x) {
+        cd s{0.0, 0.0};
+        for (int r = 0; r < nrows; ++r) s = cadd(s, partial[static_cast<size_t>(r) * nch + ch]);
+        const int a = chan_a[ch], b = chan_b[ch];
+        rho[a * nb + b] = s;
+        if (packed && a != b) rho[b * nb + a] = cconj(s);
+    }
+    for (int e = threadIdx.x; e < nb2; e += blockDim.x) acc[e] = cd{0.0, 0.0};
+    __syncthreads();
+    if (include_hartree) {
+        for (int a = threadIdx.x; a < nb; a += blockDim.x) {
+            double s = 0.0;
+            for (int b = 0; b < nb; ++b) s += HH[a * nb + b] * rho[b * nb + b].x;
+            hdiag[a] = s;
+        }
+    } else {
+        for (int a = threadIdx.x; a < nb; a += blockDim.x) hdiag[a] = 0.0;
+    }
+    if (has_contact) {
+        for (int t = 0; t < n_contact; ++t) {
+            const double g = cg[t];
+            if (g == 0.0) continue;  // uniform across the block
+            const cd* oi = Oi + static_cast<size_t>(t) * nb2;
+            const cd* oj = Oj + static_cast<size_t>(t) * nb2;
+            if (threadIdx.x == 0) {
+                cd tr{0.0, 0.0};
+                for (int i = 0; i < nb; ++i)
+                    for (int j = 0; j < nb; ++j) tr = cadd(tr, cmul(oj[i * nb + j], rho[j * nb + i]));
+                trv = tr;
+            }
+            // tmp = Oi * rho
+            for (int e = threadIdx.x; e < nb2; e += blockDim.x) {
+                const int i = e / nb, j = e % nb;
+                cd s{0.0, 0.0};
+                for (int k = 0; k < nb; ++k) s = cadd(s, cmul(oi[i * nb + k], rho[k * nb + j]));
+                tmp[e] = s;
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < nb2; e += blockDim.x) {
+                const int i = e / nb, j = e % nb;
+                cd s{0.0, 0.0};
+                for (int k = 0; k < nb; ++k) s = cadd(s, cmul(tmp[i * nb + k], oj[k * nb + j]));
+                // Oi * tr * g + (-g) * (Oi rho Oj)
+                const cd herm = cmul(cmul(oi[e], trv), cd{g, 0.0});
+                const cd fk = cmul(cd{-g, 0.0}, s);
+                acc[e] = cadd(acc[e], cadd(herm, fk));
+            }
+            __syncthreads();
+        }
+    }
+    for (int e = threadIdx.x; e < nb2; e += blockDim.x) SC[e] = acc[e];
+}
+
+// grid-stride over (k, a <= b) pairs.
+__global__ void k_fock_assemble(int nk, int nb, const cd* __restrict__ h,
+                                cd* __restrict__ Sigma, cd* __restrict__ F,
+                                const cd* __restrict__ SC, const double* __restrict__ hdiag) {
+    const int npair = nb * (nb + 1) / 2;
+    const size_t total = static_cast<size_t>(nk) * npair;
+    const int nb2 = nb * nb;
+    for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < total;
+         t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t k = t / npair;
+        int r = static_cast<int>(t % npair);
+        int a = 0;
+        while (r >= nb - a) { r -= nb - a; ++a; }
+        const int b = a + r;
+        const size_t kb = k * nb2;
+        const size_t iab = kb + a * nb + b, iba = kb + b * nb + a;
+        cd s_ab = Sigma[iab], s_ba = Sigma[iba];
+        if (SC) {
+            s_ab = cadd(s_ab, SC[a * nb + b]);
+            s_ba = cadd(s_ba, SC[b * nb + a]);
+            Sigma[iab] = s_ab;
+            if (a != b) Sigma[iba] = s_ba;
+        }
+        cd x_ab = cadd(h[iab], s_ab), x_ba = cadd(h[iba], s_ba);
+        if (hdiag && a == b) {
+            x_ab.x += hdiag[a];
+            x_ba.x += hdiag[a];
+        }
+        const cd f{0.5 * (x_ab.x + x_ba.x), 0.5 * (x_ab.y - x_ba.y)};
+        F[iab] = f;
+        if (a != b) F[iba] = cconj(f);
+    }
+}
+
+// One wave per k point, four per block; partial[block] = sum over its k.
+// Hm (nullable): full Hartree matrix with k stride hk_stride (0: broadcast).
+__global__ void k_energy_partial(int nk, int nb, const cd* __restrict__ h,
+                                 const cd* __restrict__ P, const cd* __restrict__ refP,
+                                 const cd* __restrict__ Sigma, const double* __restrict__ hdiag,
+                                 const cd* __restrict__ Hm, size_t hk_stride,
+                                 const double* __restrict__ w2d, double* __restrict__ partial) {
+    __shared__ double red[4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + wave;
+    const int nb2 = nb * nb;
+    double s = 0.0;
+    if (k < nk) {
+        const size_t kb = static_cast<size_t>(k) * nb2;
+        for (int e = lane; e < nb2; e += 64) {
+            const int i = e / nb, j = e % nb;
+            const cd p = P[kb + j * nb + i];
+            cd dp = p;
+            if (refP) dp = csub(dp, refP[kb + j * nb + i]);
+            cd inter;
+            if (Hm) {
+                inter = cscale(cadd(Sigma[kb + e], Hm[k * hk_stride + e]), 0.5);
+            } else {
+                inter = cscale(Sigma[kb + e], 0.5);
+                if (hdiag && i == j) inter.x += 0.5 * hdiag[i];
+            }
+            s += cmul(h[kb + e], p).x;
+            s += cmul(inter, dp).x;
+        }
+        s = wave_sum(s) * w2d[k];
+    }
+    if (lane == 0) red[wave] = (k < nk) ? s : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partial, int n, double* out) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+void launch_row_sums(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
+    dim3 grid(g.nblk, g.nk1);
+    hipLaunchKernelGGL(k_rows_fwd, grid, dim3(g.threads_rows), g.lds_rows, st, g.ax2.axis(),
+                       g.nk2, g.nb * g.nb, g.nch, g.clog, g.d_chan_src, a.P, a.refP, a.T,
+                       a.partial, a.w2d, 0);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_exchange_fwd(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_rows_fwd, dim3(g.nblk, g.nk1), dim3(g.threads_rows), g.lds_rows, st,
+                       g.ax2.axis(), g.nk2, g.nb * g.nb, g.nch, g.clog, g.d_chan_src, a.P,
+                       a.refP, a.T, a.partial, a.w2d, 1);
+    HFGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_cols, dim3(g.nblk, g.nk2), dim3(g.threads_cols), g.lds_cols, st,
+                       g.ax1.axis(), g.nk1, g.nk2, g.nch, g.clog, a.T, a.VR, a.nvr);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_exchange_inv(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
+    if (a.epilogue != EPI_RAW_FULL && !g.packed)
+        throw std::logic_error("packed epilogue on a full-channel geometry");
+    if (a.epilogue == EPI_RAW_FULL && g.packed)
+        throw std::logic_error("full epilogue on a packed geometry");
+    const double inv_n = 1.0 / static_cast<double>(static_cast<size_t>(g.nk1) * g.nk2);
+    hipLaunchKernelGGL(k_rows_inv, dim3(g.nblk, g.nk1), dim3(g.threads_rows), g.lds_rows, st,
+                       g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.d_chan_src,
+                       g.d_chan_a, g.d_chan_b, a.T, a.epilogue, a.ifftshift ? 1 : 0, inv_n,
+                       a.Sigma, a.F, a.h, a.SC, a.hdiag);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_small_terms(const ExchangeGeom& g, const SmallTermsArgs& a, hipStream_t st) {
+    if (g.nb > 32) throw std::invalid_argument("Hartree/contact terms support nb <= 32");
+    hipLaunchKernelGGL(k_small_terms, dim3(1), dim3(256), 0, st, a.partial, a.nrows, g.nch,
+                       g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0,
+                       a.include_hartree ? 1 : 0, a.has_contact ? 1 : 0, a.HH, a.n_contact,
+                       a.contact_g, a.Oi, a.Oj, a.hdiag, a.SC);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_fock_assemble(int nk, int nb, const cd* h, cd* Sigma, cd* F, const cd* SC,
+                          const double* hdiag, hipStream_t st) {
+    const size_t total = static_cast<size_t>(nk) * nb * (nb + 1) / 2;
+    const unsigned blocks = std::min<unsigned>(cdiv(total, 256), 4096u);
+    hipLaunchKernelGGL(k_fock_assemble, dim3(blocks), dim3(256), 0, st, nk, nb, h, Sigma, F,
+                       SC, hdiag);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_energy(int nk, int nb, const cd* h, const cd* P, const cd* refP, const cd* Sigma,
+                   const double* hdiag, const double* w2d, double* partial, double* E_out,
+                   hipStream_t st) {
+    const unsigned blocks = cdiv(nk, 4);
+    hipLaunchKernelGGL(k_energy_partial, dim3(blocks), dim3(256), 0, st, nk, nb, h, P, refP,
+                       Sigma, hdiag, static_cast<const cd*>(nullptr), size_t(0), w2d, partial);
+    HFGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partial,
+                       static_cast<int>(blocks), E_out);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_energy_fullH(int nk, int nb, const cd* h, const cd* P, const cd* refP,
+                         const cd* Sigma, const cd* Hm, size_t hk_stride, const double* w2d,
+                         double* partial, double* E_out, hipStream_t st) {
+    const unsigned blocks = cdiv(nk, 4);
+    hipLaunchKernelGGL(k_energy_partial, dim3(blocks), dim3(256), 0, st, nk, nb, h, P, refP,
+                       Sigma, static_cast<const double*>(nullptr), Hm, hk_stride, w2d, partial);
+    HFGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partial,
+                       static_cast<int>(blocks), E_out);
+    HFGPU_LAUNCH_CHECK();
+}
+
+namespace {
+__global__ void k_entropy(const double* __restrict__ p, const double* __restrict__ w, int nb,
+                          size_t n, double* partial) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        double pv = p[i];
+        if (pv < 1.0e-14) pv = 1.0e-14;
+        if (pv > 1.0 - 1.0e-14) pv = 1.0 - 1.0e-14;
+        s -= w[i / nb] * (pv * log(pv) + (1.0 - pv) * log1p(-pv));
+    }
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+}  // namespace
+
+void launch_entropy(const double* p, const double* w, int nk, int nb, double* partial,
+                    double* S_out, hipStream_t st) {
+    const size_t n = static_cast<size_t>(nk) * nb;
+    const unsigned blocks = std::max(1u, std::min<unsigned>(cdiv(n, 256), 1024u));
+    hipLaunchKernelGGL(k_entropy, dim3(blocks), dim3(256), 0, st, p, w, nb, n, partial);
+    HFGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partial,
+                       static_cast<int>(blocks), S_out);
+    HFGPU_LAUNCH_CHECK();
+}
+
+}  // namespace hfgpu

@@ -1,0 +1,127 @@
+// Host-side core of libhfgpu: device buffers and the device-resident HF
+// kernel (the GPU counterpart of the reference's HFKernel, kernel.hpp:10-84).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hf_desc.h"
+#include "common.h"
+#include "exchange.h"
+#include "kernels.h"
+
+namespace hfgpu {
+
+// Owning device allocation.
+class DevBuf {
+public:
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) { alloc(bytes); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    void alloc(size_t bytes) {
+        release();
+        if (bytes) HFGPU_CHECK(hipMalloc(&p_, bytes));
+        n_ = bytes;
+    }
+    void ensure(size_t bytes) { if (bytes > n_) alloc(bytes); }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p_); }
+    void* get() const { return p_; }
+    size_t size() const { return n_; }
+
+private:
+    void* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+inline void h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (bytes) HFGPU_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+}
+inline void d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (bytes) HFGPU_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+}
+inline void d2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (bytes) HFGPU_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+}
+
+// Zero-fill through a kernel (safe under graph capture, unlike memset nodes).
+void fill_zero(void* p, size_t bytes, hipStream_t st);
+// Device-side elementwise helpers used by the solvers.
+void launch_axpy_c(size_t n, double alpha, const cd* x, cd* y, hipStream_t st);  // y += a x
+void launch_sub_c(size_t n, const cd* a, const cd* b, cd* out, hipStream_t st);    // a - b
+
+// Device-resident HF kernel.
+struct GpuKernel {
+    int nk1 = 0, nk2 = 0, nb = 0, dv = 1, n_contact = 0;
+    size_t nk = 0, nb2 = 0, nd = 0;
+    bool hartree = false, exchange = true, hcp = false, has_refP = false, contact = false;
+    bool dense_packable = false;  // (nb, nb) V with a real, band-symmetric kernel
+    double T = 0.0, weight_sum = 0.0;
+    std::vector<double> w_host;  // (nk) weights, host copy
+    hipStream_t stream = nullptr;
+
+    DevBuf h, refP, w2d, HH, cg, Oi, Oj;
+    DevBuf VR_full;    // (nk, dv*dv) as given
+    DevBuf VR_packed;  // (nk, nb(nb+1)/2) for dense V, when packable
+    std::unique_ptr<ExchangeGeom> geom_full, geom_packed;
+
+    // Fock workspaces
+    DevBuf T_ws, partial, SC, hdiag, e_partial, e_out;
+
+    explicit GpuKernel(const hf_kernel_desc& d);
+    ~GpuKernel();
+
+    bool packed_for(bool p_hermitian) const { return hcp || (dense_packable && p_hermitian); }
+    ExchangeGeom& geom(bool packed);
+
+    // build_fock_compact on device: Sigma, F, hdiag (nb) from P.
+    void build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, bool p_hermitian,
+                    hipStream_t st);
+    // hf_energy_with_hartree_diag on device into E (one double).
+    void energy(const cd* P, const cd* Sigma, const double* hdiag_in, double* E, hipStream_t st);
+};
+
+// Last error message of the calling thread (hfgpu_last_error).
+extern thread_local std::string g_last_error;
+
+// Run fn, mapping exceptions to hf_status codes: std::invalid_argument ->
+// HF_EINVAL, DeviceError -> HF_EDEVICE, anything else -> HF_ERUNTIME.
+template <class Fn>
+int guarded(Fn&& fn) {
+    try {
+        fn();
+        return HF_OK;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return HF_EINVAL;
+    } catch (const DeviceError& e) {
+        g_last_error = e.what();
+        return HF_EDEVICE;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return HF_ERUNTIME;
+    }
+}
+
+// True when M (nk, nb, nb) equals its conjugate transpose bit for bit.
+bool host_is_hermitian(const double* M, size_t nk, size_t nb);
+// In-place hermitisation on the host (linalg.hpp:20-33).
+void host_hermitize(double* M, size_t nk, size_t nb);
+
+}  // namespace hfgpu
+
+struct hfgpu_kernel {
+    std::unique_ptr<hfgpu::GpuKernel> impl;
+};

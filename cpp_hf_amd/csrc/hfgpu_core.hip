@@ -1,0 +1,689 @@
+// libhfgpu core: device-resident HF kernel, Fock build orchestration and the
+// C ABI entry points that are not solver loops.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/hfgpu.h"
+#include "hfgpu_core.h"
+#include "solver_kernels.h"
+
+namespace hfgpu {
+
+thread_local std::string g_last_error;
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+
+namespace {
+
+__global__ void k_fill_zero(unsigned long long* p, size_t n8) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n8;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        p[i] = 0ull;
+}
+
+__global__ void k_axpy_c(size_t n, double alpha, const cd* __restrict__ x, cd* __restrict__ y) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const cd a = y[i], b = x[i];
+        y[i] = cd{a.x + alpha * b.x, a.y + alpha * b.y};
+    }
+}
+
+__global__ void k_sub_c(size_t n, const cd* __restrict__ a, const cd* __restrict__ b,
+                        cd* __restrict__ out) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        out[i] = csub(a[i], b[i]);
+}
+
+unsigned grid_for(size_t n) { return std::max(1u, std::min<unsigned>(cdiv(n, 256), 8192u)); }
+
+}  // namespace
+
+void fill_zero(void* p, size_t bytes, hipStream_t st) {
+    if (!bytes) return;
+    if (bytes % 8) throw std::logic_error("fill_zero needs a multiple of 8 bytes");
+    const size_t n8 = bytes / 8;
+    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n8)), dim3(256), 0, st,
+                       static_cast<unsigned long long*>(p), n8);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_axpy_c(size_t n, double alpha, const cd* x, cd* y, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_axpy_c, dim3(grid_for(n)), dim3(256), 0, st, n, alpha, x, y);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_sub_c(size_t n, const cd* a, const cd* b, cd* out, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_sub_c, dim3(grid_for(n)), dim3(256), 0, st, n, a, b, out);
+    HFGPU_LAUNCH_CHECK();
+}
+
+bool host_is_hermitian(const double* M, size_t nk, size_t nb) {
+    const size_t nb2 = nb * nb;
+    for (size_t k = 0; k < nk; ++k) {
+        const double* m = M + 2 * k * nb2;
+        for (size_t i = 0; i < nb; ++i)
+            for (size_t j = i; j < nb; ++j) {
+                const double* a = m + 2 * (i * nb + j);
+                const double* b = m + 2 * (j * nb + i);
+                if (a[0] != b[0] || a[1] != -b[1]) return false;
+            }
+    }
+    return true;
+}
+
+void host_hermitize(double* M, size_t nk, size_t nb) {
+    const size_t nb2 = nb * nb;
+    for (size_t k = 0; k < nk; ++k) {
+        double* m = M + 2 * k * nb2;
+        for (size_t i = 0; i < nb; ++i)
+            for (size_t j = i; j < nb; ++j) {
+                double* a = m + 2 * (i * nb + j);
+                double* b = m + 2 * (j * nb + i);
+                const double hr = 0.5 * (a[0] + b[0]);
+                const double hi = 0.5 * (a[1] - b[1]);
+                a[0] = hr;
+                a[1] = hi;
+                b[0] = hr;
+                b[1] = -hi;
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// GpuKernel
+// ---------------------------------------------------------------------------
+
+static bool dense_vr_packable(const double* VR, size_t nk1, size_t nk2, size_t nb) {
+    // Packed exchange for (nb, nb) V needs the index-space kernel W_ab =
+    // IFFT(VR_ab) to be real and symmetric in (a, b):
+    //   VR_ab(q) == VR_ba(q)  and  VR_ab(-q) == conj(VR_ab(q)).
+    const size_t nb2 = nb * nb;
+    double vmax = 0.0, err = 0.0;
+    for (size_t q1 = 0; q1 < nk1; ++q1)
+        for (size_t q2 = 0; q2 < nk2; ++q2) {
+            const size_t q = q1 * nk2 + q2;
+            const size_t mq = ((nk1 - q1) % nk1) * nk2 + (nk2 - q2) % nk2;
+            for (size_t a = 0; a < nb; ++a)
+                for (size_t b = 0; b < nb; ++b) {
+                    const double* v = VR + 2 * (q * nb2 + a * nb + b);
+                    const double* vt = VR + 2 * (q * nb2 + b * nb + a);
+                    const double* vm = VR + 2 * (mq * nb2 + a * nb + b);
+                    vmax = std::max(vmax, std::hypot(v[0], v[1]));
+                    err = std::max(err, std::hypot(v[0] - vt[0], v[1] - vt[1]));
+                    err = std::max(err, std::hypot(vm[0] - v[0], vm[1] + v[1]));
+                }
+        }
+    return err <= 64.0 * 2.220446049250313e-16 * vmax;
+}
+
+GpuKernel::GpuKernel(const hf_kernel_desc& d) {
+    if (d.nk1 == 0 || d.nk2 == 0 || d.nb == 0)
+        throw std::invalid_argument("h must be (nk1,nk2,nb,nb) with non-zero sizes");
+    if (!((d.dv1 == 1 && d.dv2 == 1) || (d.dv1 == d.nb && d.dv2 == d.nb)))
+        throw std::invalid_argument("VR last two dims must be (1,1) or (nb,nb)");
+    if (d.has_refP && !d.refP)
+        throw std::invalid_argument("refP must match h shape when has_refP is true");
+    if (!d.h || !d.VR || !d.w2d || !d.HH || !d.contact_g || !d.contact_Oi || !d.contact_Oj)
+        throw std::invalid_argument("kernel description has a null array");
+    nk1 = static_cast<int>(d.nk1);
+    nk2 = static_cast<int>(d.nk2);
+    nb = static_cast<int>(d.nb);
+    dv = static_cast<int>(d.dv1);
+    n_contact = static_cast<int>(d.n_contact);
+    nk = d.nk1 * d.nk2;
+    nb2 = d.nb * d.nb;
+    nd = nk * nb2;
+    hartree = d.include_hartree != 0;
+    exchange = d.include_exchange != 0;
+    hcp = d.exchange_hcp != 0 && dv == 1;
+    has_refP = d.has_refP != 0;
+    T = d.T;
+    weight_sum = d.weight_sum;
+    w_host.assign(d.w2d, d.w2d + nk);
+    // fock.hpp:25-29
+    contact = n_contact > 1;
+    for (int t = 0; t < n_contact; ++t)
+        if (d.contact_g[t] != 0.0) contact = true;
+
+    HFGPU_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    const size_t cz = sizeof(cd);
+    h.alloc(nd * cz);
+    h2d(h.get(), d.h, nd * cz, stream);
+    if (has_refP) {
+        refP.alloc(nd * cz);
+        h2d(refP.get(), d.refP, nd * cz, stream);
+    }
+    w2d.alloc(nk * sizeof(double));
+    h2d(w2d.get(), d.w2d, nk * sizeof(double), stream);
+    HH.alloc(nb2 * sizeof(double));
+    h2d(HH.get(), d.HH, nb2 * sizeof(double), stream);
+    if (n_contact > 0) {
+        cg.alloc(n_contact * sizeof(double));
+        h2d(cg.get(), d.contact_g, n_contact * sizeof(double), stream);
+        Oi.alloc(n_contact * nb2 * cz);
+        Oj.alloc(n_contact * nb2 * cz);
+        h2d(Oi.get(), d.contact_Oi, n_contact * nb2 * cz, stream);
+        h2d(Oj.get(), d.contact_Oj, n_contact * nb2 * cz, stream);
+    }
+    const size_t nvr = static_cast<size_t>(dv) * dv;
+    VR_full.alloc(nk * nvr * cz);
+    h2d(VR_full.get(), d.VR, nk * nvr * cz, stream);
+    if (exchange && dv > 1 && nb > 1) {
+        dense_packable = dense_vr_packable(d.VR, d.nk1, d.nk2, d.nb);
+        if (dense_packable) {
+            const size_t np = d.nb * (d.nb + 1) / 2;
+            std::vector<double> vp(2 * nk * np);
+            for (size_t q = 0; q < nk; ++q) {
+                size_t c = 0;
+                for (size_t a = 0; a < d.nb; ++a)
+                    for (size_t b = a; b < d.nb; ++b, ++c) {
+                        vp[2 * (q * np + c)] = d.VR[2 * (q * nb2 + a * d.nb + b)];
+                        vp[2 * (q * np + c) + 1] = d.VR[2 * (q * nb2 + a * d.nb + b) + 1];
+                    }
+            }
+            VR_packed.alloc(vp.size() * sizeof(double));
+            h2d(VR_packed.get(), vp.data(), vp.size() * sizeof(double), stream);
+            HFGPU_CHECK(hipStreamSynchronize(stream));
+        }
+    }
+    T_ws.alloc(nd * cz);
+    partial.alloc(static_cast<size_t>(nk1) * nb2 * cz);
+    SC.alloc(nb2 * cz);
+    hdiag.alloc(std::max<size_t>(nb, 1) * sizeof(double));
+    e_partial.alloc((cdiv(nk, 4) + 1) * sizeof(double));
+    e_out.alloc(sizeof(double));
+    fill_zero(SC.get(), nb2 * cz, stream);
+    fill_zero(hdiag.get(), nb * sizeof(double), stream);
+    HFGPU_CHECK(hipStreamSynchronize(stream));
+}
+
+GpuKernel::~GpuKernel() {
+    if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+    }
+    if (geom_full) geom_full->release();
+    if (geom_packed) geom_packed->release();
+}
+
+ExchangeGeom& GpuKernel::geom(bool packed) {
+    auto& g = packed ? geom_packed : geom_full;
+    if (!g) {
+        g = std::make_unique<ExchangeGeom>();
+        g->init(nk1, nk2, nb, packed);
+    }
+    return *g;
+}
+
+void GpuKernel::build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, bool p_hermitian,
+                           hipStream_t st) {
+    const bool need_partials = hartree || contact;
+    const bool packed = exchange && packed_for(p_hermitian);
+    ExchangeGeom& g = geom(packed);
+    ExchangeArgs a;
+    a.P = P;
+    a.refP = has_refP ? refP.as<cd>() : nullptr;
+    a.T = T_ws.as<cd>();
+    a.partial = need_partials ? partial.as<cd>() : nullptr;
+    a.w2d = w2d.as<double>();
+    if (packed && dv > 1) {
+        a.VR = VR_packed.as<cd>();
+        a.nvr = g.nch;
+    } else {
+        a.VR = VR_full.as<cd>();
+        a.nvr = dv * dv;
+    }
+    if (exchange)
+        launch_exchange_fwd(g, a, st);
+    else if (need_partials)
+        launch_row_sums(g, a, st);
+    double* hd = hdiag_out ? hdiag_out : hdiag.as<double>();
+    if (need_partials) {
+        SmallTermsArgs s;
+        s.partial = partial.as<cd>();
+        s.nrows = nk1;
+        s.include_hartree = hartree;
+        s.has_contact = contact;
+        s.HH = HH.as<double>();
+        s.n_contact = n_contact;
+        s.contact_g = cg.as<double>();
+        s.Oi = Oi.as<cd>();
+        s.Oj = Oj.as<cd>();
+        s.hdiag = hd;
+        s.SC = SC.as<cd>();
+        launch_small_terms(g, s, st);
+    } else {
+        fill_zero(hd, nb * sizeof(double), st);
+    }
+    const cd* sc = contact ? SC.as<cd>() : nullptr;
+    const double* hdp = hartree ? hd : nullptr;
+    if (exchange) {
+        a.Sigma = Sigma;
+        a.ifftshift = false;
+        if (packed) {
+            a.epilogue = EPI_FOCK_PACKED;
+            a.F = F;
+            a.h = h.as<cd>();
+            a.SC = sc;
+            a.hdiag = hdp;
+            launch_exchange_inv(g, a, st);
+        } else {
+            a.epilogue = EPI_RAW_FULL;
+            launch_exchange_inv(g, a, st);
+            launch_fock_assemble(static_cast<int>(nk), nb, h.as<cd>(), Sigma, F, sc, hdp, st);
+        }
+    } else {
+        fill_zero(Sigma, nd * sizeof(cd), st);
+        launch_fock_assemble(static_cast<int>(nk), nb, h.as<cd>(), Sigma, F, sc, hdp, st);
+    }
+}
+
+void GpuKernel::energy(const cd* P, const cd* Sigma, const double* hdiag_in, double* E,
+                       hipStream_t st) {
+    launch_energy(static_cast<int>(nk), nb, h.as<cd>(), P, has_refP ? refP.as<cd>() : nullptr,
+                  Sigma, hartree ? hdiag_in : nullptr, w2d.as<double>(), e_partial.as<double>(),
+                  E, st);
+}
+
+// ---------------------------------------------------------------------------
+// geometry cache for kernel-less self-energy calls (the counterpart of the
+// reference's FftPlanCache, selfenergy.hpp:20-55: 4 entries, FIFO)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct GeomCache {
+    std::mutex mu;
+    std::vector<std::pair<std::tuple<int, int, int, bool>, std::unique_ptr<ExchangeGeom>>> entries;
+
+    ExchangeGeom& get(int nk1, int nk2, int nb, bool packed) {
+        std::lock_guard<std::mutex> lock(mu);
+        const auto key = std::make_tuple(nk1, nk2, nb, packed);
+        for (auto& e : entries)
+            if (e.first == key) return *e.second;
+        while (entries.size() >= 4) {
+            entries.front().second->release();
+            entries.erase(entries.begin());
+        }
+        auto g = std::make_unique<ExchangeGeom>();
+        g->init(nk1, nk2, nb, packed);
+        entries.emplace_back(key, std::move(g));
+        return *entries.back().second;
+    }
+    void clear() {
+        std::lock_guard<std::mutex> lock(mu);
+        for (auto& e : entries) e.second->release();
+        entries.clear();
+    }
+};
+
+GeomCache& geom_cache() {
+    static GeomCache c;
+    return c;
+}
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { HFGPU_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~Stream() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+    void sync() { HFGPU_CHECK(hipStreamSynchronize(s)); }
+};
+
+}  // namespace
+
+void clear_geometry_cache() { geom_cache().clear(); }
+
+}  // namespace hfgpu
+
+using namespace hfgpu;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char* hfgpu_last_error(void) { return g_last_error.c_str(); }
+
+int hfgpu_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void hfgpu_clear_plan_cache(void) { clear_geometry_cache(); }
+
+int hfgpu_kernel_create(const hf_kernel_desc* desc, hfgpu_kernel** out) {
+    return guarded([&] {
+        if (!desc || !out) throw std::invalid_argument("null argument");
+        auto* k = new hfgpu_kernel;
+        try {
+            k->impl = std::make_unique<GpuKernel>(*desc);
+        } catch (...) {
+            delete k;
+            throw;
+        }
+        *out = k;
+    });
+}
+
+int hfgpu_kernel_destroy(hfgpu_kernel* k) {
+    return guarded([&] { delete k; });
+}
+
+int hfgpu_kernel_packable(const hfgpu_kernel* k) {
+    if (!k) return 0;
+    return (k->impl->hcp || k->impl->dense_packable) ? 1 : 0;
+}
+
+int hfgpu_selfenergy_fft_full(const double* VR, const double* P, double* sigma, size_t nk1,
+                              size_t nk2, size_t nb, size_t dv1, size_t dv2,
+                              int apply_ifftshift, int hcp) {
+    return guarded([&] {
+        if (hcp && (dv1 != 1 || dv2 != 1))
+            throw std::invalid_argument(
+                "hermitian_channel_packing requires VR with shape (...,1,1)");
+        if (!((dv1 == 1 && dv2 == 1) || (dv1 == nb && dv2 == nb)))
+            throw std::invalid_argument("VR shape must be (..., 1, 1) or (..., nb, nb).");
+        const size_t nk = nk1 * nk2, nd = nk * nb * nb;
+        if (nd == 0) return;
+        ExchangeGeom& g = geom_cache().get(static_cast<int>(nk1), static_cast<int>(nk2),
+                                           static_cast<int>(nb), hcp != 0);
+        Stream st;
+        DevBuf dP(nd * sizeof(cd)), dS(nd * sizeof(cd)), dT(nk * g.nch * sizeof(cd)),
+            dV(nk * dv1 * dv2 * sizeof(cd));
+        h2d(dP.get(), P, nd * sizeof(cd), st.s);
+        h2d(dV.get(), VR, nk * dv1 * dv2 * sizeof(cd), st.s);
+        ExchangeArgs a;
+        a.P = dP.as<cd>();
+        a.VR = dV.as<cd>();
+        a.nvr = static_cast<int>(dv1 * dv2);
+        a.T = dT.as<cd>();
+        a.epilogue = hcp ? EPI_RAW_PACKED : EPI_RAW_FULL;
+        a.ifftshift = apply_ifftshift != 0;
+        a.Sigma = dS.as<cd>();
+        launch_exchange_fwd(g, a, st.s);
+        launch_exchange_inv(g, a, st.s);
+        d2h(sigma, dS.get(), nd * sizeof(cd), st.s);
+        st.sync();
+    });
+}
+
+int hfgpu_eigh_batched(const double* M, double* w, double* V, size_t nmat, size_t nb) {
+    return guarded([&] {
+        if (nmat == 0 || nb == 0) return;
+        Stream st;
+        const size_t n = nmat * nb * nb;
+        DevBuf dM(n * sizeof(cd)), dV(n * sizeof(cd)), dw(nmat * nb * sizeof(double));
+        h2d(dM.get(), M, n * sizeof(cd), st.s);
+        launch_eigh(static_cast<int>(nmat), static_cast<int>(nb), 0, dM.as<cd>(), dw.as<double>(),
+                    dV.as<cd>(), st.s);
+        d2h(w, dw.get(), nmat * nb * sizeof(double), st.s);
+        d2h(V, dV.get(), n * sizeof(cd), st.s);
+        st.sync();
+    });
+}
+
+int hfgpu_find_mu_bisection(const double* bands, const double* weights, size_t nk, size_t nb,
+                            double n_e, double T, int maxiter, double* mu) {
+    return guarded([&] {
+        if (nk * nb == 0) throw std::invalid_argument("bands must not be empty");
+        Stream st;
+        DevBuf de(nk * nb * sizeof(double)), dw(nk * sizeof(double)), ds(sizeof(MuState)),
+            dp(kMuPartialDoubles * sizeof(double));
+        h2d(de.get(), bands, nk * nb * sizeof(double), st.s);
+        h2d(dw.get(), weights, nk * sizeof(double), st.s);
+        fill_zero(ds.get(), sizeof(MuState), st.s);
+        launch_mu_bisection(de.as<double>(), dw.as<double>(), static_cast<int>(nk),
+                            static_cast<int>(nb), n_e, T, maxiter, ds.as<MuState>(),
+                            dp.as<double>(), st.s);
+        MuState hs;
+        d2h(&hs, ds.get(), sizeof(MuState), st.s);
+        st.sync();
+        *mu = hs.mu;
+    });
+}
+
+int hfgpu_solve_mu_inloop(const double* eps, const double* w_norm, size_t nk, size_t nb,
+                          double n_target_norm, double mu0, double T, int maxiter, double* mu) {
+    return guarded([&] {
+        if (nk * nb == 0) throw std::invalid_argument("eps must not be empty");
+        Stream st;
+        DevBuf de(nk * nb * sizeof(double)), dw(nk * sizeof(double)), ds(sizeof(MuState)),
+            dp(kMuPartialDoubles * sizeof(double));
+        h2d(de.get(), eps, nk * nb * sizeof(double), st.s);
+        h2d(dw.get(), w_norm, nk * sizeof(double), st.s);
+        fill_zero(ds.get(), sizeof(MuState), st.s);
+        launch_mu_newton(de.as<double>(), dw.as<double>(), static_cast<int>(nk),
+                         static_cast<int>(nb), n_target_norm, mu0, false, T, maxiter,
+                         ds.as<MuState>(), dp.as<double>(), st.s);
+        MuState hs;
+        d2h(&hs, ds.get(), sizeof(MuState), st.s);
+        st.sync();
+        *mu = hs.mu;
+    });
+}
+
+int hfgpu_build_fock_apply(hfgpu_kernel* k, const double* P, double* Sigma, double* Hh,
+                           double* F, double* E, hf_project_fn project, void* user) {
+    return guarded([&] {
+        if (!k) throw std::invalid_argument("null kernel");
+        GpuKernel& K = *k->impl;
+        hipStream_t s = K.stream;
+        const size_t bytes = K.nd * sizeof(cd);
+        DevBuf dP(bytes), dS(bytes), dF(bytes), dH(K.nb * sizeof(double)), dE(sizeof(double));
+        h2d(dP.get(), P, bytes, s);
+        const bool herm = K.dense_packable ? host_is_hermitian(P, K.nk, K.nb) : false;
+        K.build_fock(dP.as<cd>(), dS.as<cd>(), dF.as<cd>(), dH.as<double>(), herm, s);
+        // hf_energy (fock.hpp:324-351) with the dense Hartree array equals the
+        // compact-diagonal form exactly: 0.5 * (s + h) == 0.5 * s + 0.5 * h.
+        K.energy(dP.as<cd>(), dS.as<cd>(), dH.as<double>(), dE.as<double>(), s);
+        std::vector<double> hd(K.nb);
+        d2h(Sigma, dS.get(), bytes, s);
+        d2h(F, dF.get(), bytes, s);
+        d2h(hd.data(), dH.get(), K.nb * sizeof(double), s);
+        d2h(E, dE.get(), sizeof(double), s);
+        HFGPU_CHECK(hipStreamSynchronize(s));
+        std::memset(Hh, 0, bytes);
+        if (K.hartree)
+            for (size_t kk = 0; kk < K.nk; ++kk)
+                for (int i = 0; i < K.nb; ++i) Hh[2 * (kk * K.nb2 + i * K.nb + i)] = hd[i];
+        if (project) {
+            if (project(user, F, K.nk1, K.nk2, K.nb) != 0)
+                throw std::runtime_error("project_fn failed");
+            host_hermitize(F, K.nk, K.nb);
+        }
+    });
+}
+
+int hfgpu_build_fock_device(hfgpu_kernel* k, const void* dP, void* dSigma, void* dF,
+                            void* dHdiag, int p_hermitian, void* stream) {
+    return guarded([&] {
+        if (!k) throw std::invalid_argument("null kernel");
+        GpuKernel& K = *k->impl;
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : K.stream;
+        K.build_fock(static_cast<const cd*>(dP), static_cast<cd*>(dSigma), static_cast<cd*>(dF),
+                     static_cast<double*>(dHdiag), p_hermitian != 0, s);
+    });
+}
+
+int hfgpu_energy_device(hfgpu_kernel* k, const void* dP, const void* dSigma,
+                        const void* dHdiag, void* dE, void* stream) {
+    return guarded([&] {
+        if (!k) throw std::invalid_argument("null kernel");
+        GpuKernel& K = *k->impl;
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : K.stream;
+        K.energy(static_cast<const cd*>(dP), static_cast<const cd*>(dSigma),
+                 static_cast<const double*>(dHdiag), static_cast<double*>(dE), s);
+    });
+}
+
+int hfgpu_hf_energy(const double* h, const double* P, const double* Sigma, const double* H,
+                    size_t n_H, const double* w, const double* refP, size_t nk, size_t nb,
+                    double* E) {
+    return guarded([&] {
+        if (n_H != 1 && n_H != nk) throw std::invalid_argument("H must be broadcast (1) or per k");
+        if (nk == 0 || nb == 0) {
+            *E = 0.0;
+            return;
+        }
+        Stream st;
+        const size_t nd = nk * nb * nb, hb = n_H * nb * nb;
+        DevBuf dh(nd * sizeof(cd)), dP(nd * sizeof(cd)), dS(nd * sizeof(cd)),
+            dH(hb * sizeof(cd)), dw(nk * sizeof(double)), dr(refP ? nd * sizeof(cd) : 0),
+            dpart((cdiv(nk, 4) + 1) * sizeof(double)), dE(sizeof(double));
+        h2d(dh.get(), h, nd * sizeof(cd), st.s);
+        h2d(dP.get(), P, nd * sizeof(cd), st.s);
+        h2d(dS.get(), Sigma, nd * sizeof(cd), st.s);
+        h2d(dH.get(), H, hb * sizeof(cd), st.s);
+        h2d(dw.get(), w, nk * sizeof(double), st.s);
+        if (refP) h2d(dr.get(), refP, nd * sizeof(cd), st.s);
+        launch_energy_fullH(static_cast<int>(nk), static_cast<int>(nb), dh.as<cd>(), dP.as<cd>(),
+                            refP ? dr.as<cd>() : nullptr, dS.as<cd>(), dH.as<cd>(),
+                            n_H == 1 ? 0 : nb * nb, dw.as<double>(), dpart.as<double>(),
+                            dE.as<double>(), st.s);
+        d2h(E, dE.get(), sizeof(double), st.s);
+        st.sync();
+    });
+}
+
+int hfgpu_occupation_entropy(const double* p, const double* w_norm, size_t nk, size_t nb,
+                             double* S) {
+    return guarded([&] {
+        if (nk * nb == 0) {
+            *S = 0.0;
+            return;
+        }
+        Stream st;
+        DevBuf dp(nk * nb * sizeof(double)), dw(nk * sizeof(double)),
+            dpart(1025 * sizeof(double)), dS(sizeof(double));
+        h2d(dp.get(), p, nk * nb * sizeof(double), st.s);
+        h2d(dw.get(), w_norm, nk * sizeof(double), st.s);
+        launch_entropy(dp.as<double>(), dw.as<double>(), static_cast<int>(nk),
+                       static_cast<int>(nb), dpart.as<double>(), dS.as<double>(), st.s);
+        d2h(S, dS.get(), sizeof(double), st.s);
+        st.sync();
+    });
+}
+
+int hfgpu_density_from_fock(const double* F, const double* weights, size_t nk, size_t nb,
+                            double n_e, double T, int maxiter, double* P, double* mu) {
+    return guarded([&] {
+        if (nk * nb == 0) throw std::invalid_argument("F must not be empty");
+        Stream st;
+        const size_t nd = nk * nb * nb, nkb = nk * nb;
+        DevBuf dF(nd * sizeof(cd)), dV(nd * sizeof(cd)), dP(nd * sizeof(cd)),
+            de(nkb * sizeof(double)), docc(nkb * sizeof(double)), dw(nk * sizeof(double)),
+            ds(sizeof(MuState)), dpart(kMuPartialDoubles * sizeof(double));
+        h2d(dF.get(), F, nd * sizeof(cd), st.s);
+        h2d(dw.get(), weights, nk * sizeof(double), st.s);
+        fill_zero(ds.get(), sizeof(MuState), st.s);
+        launch_eigh(static_cast<int>(nk), static_cast<int>(nb), 0, dF.as<cd>(), de.as<double>(),
+                    dV.as<cd>(), st.s);
+        launch_mu_bisection(de.as<double>(), dw.as<double>(), static_cast<int>(nk),
+                            static_cast<int>(nb), n_e, T, maxiter, ds.as<MuState>(),
+                            dpart.as<double>(), st.s);
+        launch_occupations(de.as<double>(), nkb, ds.as<MuState>(), T, 0, docc.as<double>(), st.s);
+        launch_density(static_cast<int>(nk), static_cast<int>(nb), dV.as<cd>(), docc.as<double>(),
+                       dP.as<cd>(), true, st.s);
+        MuState hs;
+        d2h(P, dP.get(), nd * sizeof(cd), st.s);
+        d2h(&hs, ds.get(), sizeof(MuState), st.s);
+        st.sync();
+        *mu = hs.mu;
+    });
+}
+
+int hfgpu_cayley_spectral_setup(const double* d, double* V, double* lam, size_t nk, size_t nb) {
+    return guarded([&] {
+        if (nk == 0 || nb == 0) return;
+        Stream st;
+        const size_t n = nk * nb * nb;
+        DevBuf dd(n * sizeof(cd)), dV(n * sizeof(cd)), dl(nk * nb * sizeof(double));
+        h2d(dd.get(), d, n * sizeof(cd), st.s);
+        launch_eigh(static_cast<int>(nk), static_cast<int>(nb), 1, dd.as<cd>(), dl.as<double>(),
+                    dV.as<cd>(), st.s);
+        d2h(V, dV.get(), n * sizeof(cd), st.s);
+        d2h(lam, dl.get(), nk * nb * sizeof(double), st.s);
+        st.sync();
+    });
+}
+
+int hfgpu_cayley_unitary_from_spectrum(const double* V, const double* lam, double tau,
+                                       double* U, size_t nk, size_t nb) {
+    return guarded([&] {
+        if (nk == 0 || nb == 0) return;
+        Stream st;
+        const size_t n = nk * nb * nb;
+        DevBuf dV(n * sizeof(cd)), dU(n * sizeof(cd)), dl(nk * nb * sizeof(double));
+        h2d(dV.get(), V, n * sizeof(cd), st.s);
+        h2d(dl.get(), lam, nk * nb * sizeof(double), st.s);
+        launch_cayley_unitary(static_cast<int>(nk), static_cast<int>(nb), dV.as<cd>(),
+                              dl.as<double>(), tau, dU.as<cd>(), st.s);
+        d2h(U, dU.get(), n * sizeof(cd), st.s);
+        st.sync();
+    });
+}
+
+int hfgpu_diag_UFU_from_spectrum(const double* V, const double* Ft_eig, const double* lam,
+                                 double tau, double* diag, size_t nk, size_t nb) {
+    return guarded([&] {
+        if (nk == 0 || nb == 0) return;
+        Stream st;
+        const size_t n = nk * nb * nb;
+        DevBuf dV(n * sizeof(cd)), dF(n * sizeof(cd)), dl(nk * nb * sizeof(double)),
+            dd(nk * nb * sizeof(double));
+        h2d(dV.get(), V, n * sizeof(cd), st.s);
+        h2d(dF.get(), Ft_eig, n * sizeof(cd), st.s);
+        h2d(dl.get(), lam, nk * nb * sizeof(double), st.s);
+        launch_diag_ufu(static_cast<int>(nk), static_cast<int>(nb), dV.as<cd>(), dF.as<cd>(),
+                        dl.as<double>(), tau, dd.as<double>(), st.s);
+        d2h(diag, dd.get(), nk * nb * sizeof(double), st.s);
+        st.sync();
+    });
+}
+
+int hfgpu_resample_kgrid_2d(const double* x, double* out, size_t nk_old, size_t nk_new,
+                            size_t inner) {
+    return guarded([&] {
+        const size_t n_in = nk_old * nk_old * inner, n_out = nk_new * nk_new * inner;
+        if (nk_old == nk_new) {
+            std::memcpy(out, x, n_in * sizeof(cd));
+            return;
+        }
+        if (n_out == 0) return;
+        if (nk_old == 0) throw std::invalid_argument("resample_kgrid_2d: empty input grid");
+        Stream st;
+        DevBuf dx(n_in * sizeof(cd)), dt(nk_new * nk_old * inner * sizeof(cd)),
+            dy(n_out * sizeof(cd));
+        h2d(dx.get(), x, n_in * sizeof(cd), st.s);
+        // axis 0: (nk_old, nk_old*inner) -> (nk_new, nk_old*inner)
+        launch_resample_axis(dx.as<cd>(), dt.as<cd>(), static_cast<int>(nk_old),
+                             static_cast<int>(nk_new), nk_old * inner, 1, st.s);
+        // axis 1 of each of the nk_new rows
+        launch_resample_axis(dt.as<cd>(), dy.as<cd>(), static_cast<int>(nk_old),
+                             static_cast<int>(nk_new), inner, nk_new, st.s);
+        d2h(out, dy.get(), n_out * sizeof(cd), st.s);
+        st.sync();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,55 @@
+// Launchers of the per-k dense kernels, the eigensolver and the chemical
+// potential solvers.  All take device pointers and a stream.
+#pragma once
+
+#include "common.h"
+
+namespace hfgpu {
+
+// ---- eigh (eigh.hip) -------------------------------------------------------
+// in_mode 0: eigh(herm(M)); in_mode 1: eigh(herm(i M)) (Cayley setup).
+void launch_eigh(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V, hipStream_t st);
+
+// ---- chemical potential (mu.hip) -------------------------------------------
+// Device-side state shared by the mu kernels of one solve.
+struct MuState {
+    double lo, hi, mu;
+    double emin, emax;
+    int done;
+    int pad;
+    unsigned counter;
+    unsigned pad2;
+};
+
+// Workspace for the two-stage reductions: >= kMuPartials * 32 doubles.
+constexpr int kMuBlocks = 512;
+constexpr int kMuPartialDoubles = kMuBlocks * 32;
+
+// find_mu_bisection (utils.hpp:49-75): result mu = (lo+hi)/2 left in st->mu.
+void launch_mu_bisection(const double* eps, const double* w, int nk, int nb, double n_e,
+                         double T, int maxiter, MuState* state, double* partial,
+                         hipStream_t st);
+// solve_mu_inloop (utils.hpp:79-116); warm start read from state->mu when
+// mu0_from_state, else mu0.
+void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
+                      double n_target_norm, double mu0, bool mu0_from_state, double T,
+                      int maxiter, MuState* state, double* partial, hipStream_t st);
+
+// occupations from state->mu.
+//   mode 0 (SCF): occ = expit(-(eps - mu) / (T + 1e-12))
+//   mode 1 (DM):  p   = expit((mu - eps) / T)
+void launch_occupations(const double* eps, size_t n, const MuState* state, double T, int mode,
+                        double* out, hipStream_t st);
+
+// level shift: out = eps + shift * (1 - expit(-(eps - mu)/(T + 1e-12)))
+void launch_level_shift(const double* eps, size_t n, const MuState* state, double T,
+                        double shift, double* out, hipStream_t st);
+
+// ---- dense per-k products (dense.hip) --------------------------------------
+// P_k = herm(V_k diag(f_k) V_k^H) when hermitize, else V diag(f) V^H.
+void launch_density(int nk, int nb, const cd* V, const double* f, cd* P, bool hermitize,
+                    hipStream_t st);
+// in-place hermitisation of (nk, nb, nb)
+void launch_hermitize(int nk, int nb, cd* M, hipStream_t st);
+
+}  // namespace hfgpu

@@ -1,0 +1,655 @@
+// Fused per-k kernels of the two solver loops.
+//
+// Direct minimisation (solver_dm.hpp:474-613): each iteration after the Fock
+// build is five passes over the k points:
+//   orbital_grad  Ft = Q^H F Q, eps, G, g_p          (+ |G|_w^2, sum eps^2)
+//   precond       d_Q = G / sqrt(gap^2 + lam^2), h_p (+ the six PR+ products)
+//   cg_combine    d += beta d_prev                   (+ |d_Q|_w^2, Omega_0)
+//   [eigh of herm(i d_Q)]
+//   linesearch    Ft_eig = V^H Ft V, diag(U^H Ft U) at every backtracking
+//                 step tau_j at once, frozen free energies (+ pick tau)
+//   retract       Q <- Q V c(tau) V^H, eps_work = diag(U^H Ft U)
+// The SCF loop (solver_scf.hpp:220-402) adds the residual pass
+// (delta = P_new - P, [F, P], both weighted norms).
+//
+// One wavefront per k point with the k point's nb x nb blocks in LDS; W
+// wavefronts per workgroup, grid-stride over k.  Scalar results go through
+// per-block partials and a last-block reduction in block order into the
+// solver's device scalar array, so no host round trip is needed inside an
+// iteration and every sum is deterministic for a fixed launch shape.
+#include <cmath>
+
+#include "loops.h"
+
+namespace hfgpu {
+
+namespace {
+
+constexpr double TINY_REAL = 1.0e-30;
+constexpr double LOGIT_CLIP = 1.0e-6;
+constexpr double OCC_CLIP = 1.0e-8;
+
+__device__ __forceinline__ double clampd(double v, double lo, double hi) {
+    return v < lo ? lo : (hi < v ? hi : v);
+}
+
+// complex slots per wavefront of the line-search / retraction kernels:
+// 4 nb^2 + nb complex, then nb^2 + nb doubles
+__host__ __device__ __forceinline__ int ls_wave_cd(int nb) {
+    return 4 * nb * nb + nb + (nb * nb + nb + 1) / 2;
+}
+
+__device__ __forceinline__ cd cayley_c(double lam, double tau) {
+    const double a = 0.5 * tau * lam;
+    const double den = 1.0 + a * a;
+    return cd{(1.0 - a * a) / den, (a + a) / den};
+}
+
+// C = op(A) op(B) on nb x nb LDS blocks by one wavefront.
+__device__ __forceinline__ void wave_mm(const cd* A, const cd* B, cd* C, int nb, bool ta,
+                                        bool tb, int lane) {
+    const int nb2 = nb * nb;
+    for (int e = lane; e < nb2; e += 64) {
+        const int i = e / nb, j = e % nb;
+        cd s{0.0, 0.0};
+        for (int n = 0; n < nb; ++n) {
+            const cd a = ta ? cconj(A[n * nb + i]) : A[i * nb + n];
+            const cd b = tb ? cconj(B[j * nb + n]) : B[n * nb + j];
+            s = cadd(s, cmul(a, b));
+        }
+        C[e] = s;
+    }
+}
+
+// Sum NS per-thread values over the block and write partial[block * NS + s].
+template <int NS, int W>
+__device__ __forceinline__ void store_block_partials(double (&v)[NS], double* red,
+                                                     double* partial) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const double t = wave_sum(v[s]);
+        if (lane == 0) red[wave * NS + s] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < NS) {
+        double t = 0.0;
+        for (int w = 0; w < W; ++w) t += red[w * NS + threadIdx.x];
+        partial[blockIdx.x * NS + threadIdx.x] = t;
+    }
+}
+
+// Last block to arrive sums the partials slot by slot (fixed order) into
+// out[0..NS).  Returns true in that block only.
+template <int NS, int W>
+__device__ bool reduce_last_block(double* partial, unsigned* counter, double* out, double* red,
+                                  bool* is_last) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) *is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!*is_last) return false;
+    __threadfence();
+    for (int s = 0; s < NS; ++s) {
+        double t = 0.0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) t += partial[b * NS + s];
+        t = block_sum<W * 64>(t, red);
+        if (threadIdx.x == 0) out[s] = t;
+    }
+    if (threadIdx.x == 0) *counter = 0;
+    __syncthreads();
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// DM kernels
+// ---------------------------------------------------------------------------
+
+// Ft = Q^H F Q (written over F), eps = Re diag Ft, G, g_p.
+// Slots: [GN2, EPS_SQ].  grad=false: only the rotation (finalise step).
+template <int W>
+__global__ void __launch_bounds__(W * 64)
+k_orbital_grad(int nk, int nb, const cd* __restrict__ Q, cd* __restrict__ F,
+               const double* __restrict__ p, const double* __restrict__ w, const MuState* mus,
+               double T_r, int grad, double* __restrict__ eps, cd* __restrict__ G,
+               double* __restrict__ g_p, double* partial, unsigned* counter, double* sc) {
+    extern __shared__ cd smem[];
+    __shared__ double red[W * 2];
+    __shared__ bool is_last;
+    const int nb2 = nb * nb;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    cd* sQ = smem + wave * 3 * nb2;
+    cd* sF = sQ + nb2;
+    cd* sT = sF + nb2;
+    const double mu = grad ? mus->mu : 0.0;
+    double acc[2] = {0.0, 0.0};
+    const int nsteps = (nk + gridDim.x * W - 1) / (gridDim.x * W);
+    for (int step = 0; step < nsteps; ++step) {
+        const int k = (step * gridDim.x + blockIdx.x) * W + wave;
+        const bool act = k < nk;
+        const size_t kb = static_cast<size_t>(k) * nb2;
+        if (act)
+            for (int e = lane; e < nb2; e += 64) {
+                sQ[e] = Q[kb + e];
+                sF[e] = F[kb + e];
+            }
+        __syncthreads();
+        if (act) wave_mm(sQ, sF, sT, nb, true, false, lane);
+        __syncthreads();
+        if (act) wave_mm(sT, sQ, sF, nb, false, false, lane);
+        __syncthreads();
+        if (act) {
+            for (int e = lane; e < nb2; e += 64) F[kb + e] = sF[e];
+            const double* pk = p + static_cast<size_t>(k) * nb;
+            for (int i = lane; i < nb; i += 64) {
+                const double ei = sF[i * nb + i].x;
+                if (eps) eps[static_cast<size_t>(k) * nb + i] = ei;
+                if (grad) {
+                    const double pv = clampd(pk[i], LOGIT_CLIP, 1.0 - LOGIT_CLIP);
+                    g_p[static_cast<size_t>(k) * nb + i] = ei + T_r * (log(pv) - log1p(-pv)) - mu;
+                    acc[1] += ei * ei;
+                }
+            }
+            if (grad) {
+                const double wk = w[k];
+                double gs = 0.0;
+                for (int e = lane; e < nb2; e += 64) {
+                    const int i = e / nb, j = e % nb;
+                    cd g{0.0, 0.0};
+                    if (i != j) {
+                        const cd aij = cscale(sF[i * nb + j], pk[j] - pk[i]);
+                        const cd aji = cscale(sF[j * nb + i], pk[i] - pk[j]);
+                        g = cd{0.5 * (aij.x - aji.x), 0.5 * (aij.y + aji.y)};
+                    }
+                    G[kb + e] = g;
+                    gs += cnorm(g);
+                }
+                acc[0] += wk * gs;
+            }
+        }
+        __syncthreads();
+    }
+    if (!grad) return;
+    store_block_partials<2, W>(acc, red, partial);
+    reduce_last_block<2, W>(partial, counter, sc + SC_GN2, red, &is_last);
+}
+
+// d_Q = G / sqrt(gap^2 + lam^2), h_p = g_p / curv, plus the PR+ products.
+// Slots: [PR_S1 .. PR_S6].
+template <int W>
+__global__ void __launch_bounds__(W * 64)
+k_precond(int nk, int nb, const cd* __restrict__ G, const cd* __restrict__ Gp,
+          const double* __restrict__ eps, const double* __restrict__ p,
+          const double* __restrict__ g_p, const double* __restrict__ g_pp,
+          const double* __restrict__ w, double T_r, double denom_scale, cd* __restrict__ dQ,
+          double* __restrict__ h_p, double* partial, unsigned* counter, double* sc) {
+    __shared__ double red[W * 6];
+    __shared__ bool is_last;
+    const int nb2 = nb * nb;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t nkb = static_cast<size_t>(nk) * nb;
+    const double eps_scale = sqrt(sc[SC_EPS_SQ] / static_cast<double>(nkb > 0 ? nkb : 1) + TINY_REAL);
+    const double lam = fmax(T_r, denom_scale * eps_scale);
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = blockIdx.x * W + wave; k < nk; k += gridDim.x * W) {
+        const size_t kb = static_cast<size_t>(k) * nb2;
+        const double* ek = eps + static_cast<size_t>(k) * nb;
+        const double wk = w[k];
+        double s1 = 0, s2 = 0, s5 = 0;
+        for (int e = lane; e < nb2; e += 64) {
+            const int i = e / nb, j = e % nb;
+            const double gap = ek[i] - ek[j];
+            const double base = sqrt(gap * gap + lam * lam);
+            const cd g = G[kb + e];
+            const cd d{g.x / base, g.y / base};
+            dQ[kb + e] = d;
+            const cd gp = Gp[kb + e];
+            s1 += g.x * d.x + g.y * d.y;
+            s2 += gp.x * d.x + gp.y * d.y;
+            s5 += cnorm(gp);
+        }
+        double s3 = 0, s4 = 0, s6 = 0;
+        for (int i = lane; i < nb; i += 64) {
+            const size_t idx = static_cast<size_t>(k) * nb + i;
+            const double ps = clampd(p[idx], OCC_CLIP, 1.0 - OCC_CLIP);
+            const double curv = T_r / (ps * (1.0 - ps));
+            const double hp = g_p[idx] / curv;
+            h_p[idx] = hp;
+            s3 += g_p[idx] * hp;
+            s4 += g_pp[idx] * hp;
+            s6 += g_pp[idx] * g_pp[idx];
+        }
+        acc[0] += wk * s1;
+        acc[1] += wk * s2;
+        acc[2] += wk * s3;
+        acc[3] += wk * s4;
+        acc[4] += wk * s5;
+        acc[5] += wk * s6;
+    }
+    store_block_partials<6, W>(acc, red, partial);
+    reduce_last_block<6, W>(partial, counter, sc + SC_PR1, red, &is_last);
+}
+
+// beta from the PR+ products; d_Q += beta d_Q_prev, d_p = h_p + beta d_p_prev.
+// Slots: [DQN2, E0, S0].
+template <int W>
+__global__ void __launch_bounds__(W * 64)
+k_cg_combine(int nk, int nb, int restart, cd* __restrict__ dQ, const cd* __restrict__ dQp,
+             const double* __restrict__ h_p, const double* __restrict__ d_pp,
+             double* __restrict__ d_p, const double* __restrict__ eps,
+             const double* __restrict__ p, const double* __restrict__ w, double T_r,
+             double* partial, unsigned* counter, double* sc) {
+    __shared__ double red[W * 3];
+    __shared__ bool is_last;
+    const int nb2 = nb * nb;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const double num = ((sc[SC_PR1] - sc[SC_PR1 + 1]) + sc[SC_PR1 + 2]) - sc[SC_PR1 + 3];
+    const double den = sc[SC_PR1 + 4] + sc[SC_PR1 + 5];
+    double beta = (den > TINY_REAL) ? fmax(0.0, num / den) : 0.0;
+    beta = fmin(beta, 5.0);
+    if (restart) beta = 0.0;
+    double acc[3] = {0, 0, 0};
+    for (int k = blockIdx.x * W + wave; k < nk; k += gridDim.x * W) {
+        const size_t kb = static_cast<size_t>(k) * nb2;
+        const double wk = w[k];
+        double sq = 0.0;
+        for (int e = lane; e < nb2; e += 64) {
+            const cd a = dQ[kb + e], b = dQp[kb + e];
+            const cd d{a.x + beta * b.x, a.y + beta * b.y};
+            dQ[kb + e] = d;
+            sq += cnorm(d);
+        }
+        double e0 = 0.0, s0 = 0.0;
+        for (int i = lane; i < nb; i += 64) {
+            const size_t idx = static_cast<size_t>(k) * nb + i;
+            d_p[idx] = h_p[idx] + beta * d_pp[idx];
+            const double pv = clampd(p[idx], OCC_CLIP, 1.0 - OCC_CLIP);
+            e0 += wk * pv * eps[idx];
+            s0 -= wk * (pv * log(pv) + (1.0 - pv) * log1p(-pv));
+        }
+        acc[0] += wk * sq;
+        acc[1] += e0;
+        acc[2] += s0;
+    }
+    store_block_partials<3, W>(acc, red, partial);
+    if (reduce_last_block<3, W>(partial, counter, sc + SC_DQN2, red, &is_last) &&
+        threadIdx.x == 0)
+        sc[SC_BETA] = beta;
+}
+
+// Per-k: Fe = V^H Ft V (stored), then for each tau: diag(U^H Ft U) via the
+// spectrum, as in diag_UFU_from_spectrum.  Result for row i in out[i].
+__device__ __forceinline__ void diag_at_tau(const cd* sV, const cd* sFe, const double* slam,
+                                            cd* sVC, cd* sc_, double* srow, int nb, double tau,
+                                            int lane) {
+    const int nb2 = nb * nb;
+    for (int n = lane; n < nb; n += 64) sc_[n] = cayley_c(slam[n], tau);
+    __syncthreads();
+    for (int e = lane; e < nb2; e += 64) {
+        const int n = e % nb;
+        sVC[e] = cmulc(sV[e], sc_[n]);  // V_in conj(c_n)
+    }
+    __syncthreads();
+    for (int e = lane; e < nb2; e += 64) {
+        const int i = e / nb, j = e % nb;
+        cd a{0.0, 0.0};
+        for (int n = 0; n < nb; ++n) a = cadd(a, cmul(sVC[i * nb + n], sFe[n * nb + j]));
+        a = cmul(a, sc_[j]);
+        srow[e] = cmulc(a, sV[e]).x;
+    }
+    __syncthreads();
+}
+
+// Slots: [2 * nbt trial (E_j, S_j)]; the last block then selects tau.
+template <int W>
+__global__ void __launch_bounds__(W * 64)
+k_linesearch(int nk, int nb, const cd* __restrict__ V, const double* __restrict__ lam,
+             const cd* __restrict__ Ft, cd* __restrict__ Fe, const double* __restrict__ p,
+             const double* __restrict__ d_p, const double* __restrict__ w, double T_r,
+             double max_step, double shrink, int nbt, double* partial, unsigned* counter,
+             double* sc) {
+    extern __shared__ cd smem[];
+    __shared__ double red[W * 2 * kMaxBt];
+    __shared__ bool is_last;
+    const int nb2 = nb * nb;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    cd* sV = smem + wave * ls_wave_cd(nb);
+    cd* sF = sV + nb2;
+    cd* sT = sF + nb2;
+    cd* sVC = sT + nb2;
+    cd* scf = sVC + nb2;
+    double* srow = reinterpret_cast<double*>(scf + nb);
+    double* slam = srow + nb2;
+    const double dqn = sqrt(fmax(0.0, sc[SC_DQN2]));
+    const double tau0 = fmin(1.0, max_step / fmax(dqn, TINY_REAL));
+    double acc[2 * kMaxBt];
+#pragma unroll
+    for (int s = 0; s < 2 * kMaxBt; ++s) acc[s] = 0.0;
+    const int nsteps = (nk + gridDim.x * W - 1) / (gridDim.x * W);
+    for (int step = 0; step < nsteps; ++step) {
+        const int k = (step * gridDim.x + blockIdx.x) * W + wave;
+        const bool act = k < nk;
+        const size_t kb = static_cast<size_t>(k) * nb2;
+        if (act) {
+            for (int e = lane; e < nb2; e += 64) {
+                sV[e] = V[kb + e];
+                sF[e] = Ft[kb + e];
+            }
+            for (int i = lane; i < nb; i += 64) slam[i] = lam[static_cast<size_t>(k) * nb + i];
+        }
+        __syncthreads();
+        if (act) wave_mm(sV, sF, sT, nb, true, false, lane);
+        __syncthreads();
+        if (act) {
+            wave_mm(sT, sV, sF, nb, false, false, lane);  // sF <- Fe
+        }
+        __syncthreads();
+        if (act)
+            for (int e = lane; e < nb2; e += 64) Fe[kb + e] = sF[e];
+        double tau = tau0;
+        for (int j = 0; j < nbt; ++j) {
+            diag_at_tau(sV, sF, slam, sVC, scf, srow, nb, tau, lane);
+            if (act) {
+                const double wk = w[k];
+                for (int i = lane; i < nb; i += 64) {
+                    double di = 0.0;
+                    for (int jj = 0; jj < nb; ++jj) di += srow[i * nb + jj];
+                    const size_t idx = static_cast<size_t>(k) * nb + i;
+                    const double pv = clampd(p[idx] - tau * d_p[idx], OCC_CLIP, 1.0 - OCC_CLIP);
+                    acc[2 * j] += wk * pv * di;
+                    acc[2 * j + 1] -= wk * (pv * log(pv) + (1.0 - pv) * log1p(-pv));
+                }
+            }
+            __syncthreads();
+            tau *= shrink;
+        }
+    }
+    store_block_partials<2 * kMaxBt, W>(acc, red, partial);
+    if (reduce_last_block<2 * kMaxBt, W>(partial, counter, sc + SC_TRIAL, red, &is_last) &&
+        threadIdx.x == 0) {
+        const double Tv = fmax(T_r, 1.0e-14);
+        const double om0 = sc[SC_E0] - Tv * sc[SC_S0];
+        double t = tau0;
+        bool ok = false;
+        for (int j = 0; j < nbt; ++j) {
+            const double om = sc[SC_TRIAL + 2 * j] - Tv * sc[SC_TRIAL + 2 * j + 1];
+            if (om < om0) {
+                ok = true;
+                break;
+            }
+            t *= shrink;
+        }
+        if (!ok) t = tau0 * pow(shrink, static_cast<double>(nbt));
+        sc[SC_TAU0] = tau0;
+        sc[SC_OMEGA0] = om0;
+        sc[SC_TAU] = t;
+        sc[SC_ACCEPTED] = ok ? 1.0 : 0.0;
+    }
+}
+
+// Q <- Q V diag(c(tau)) V^H and eps_work = diag(U^H Ft U) at the chosen tau.
+// use_sc_tau = 0 with U = V (tau ignored): the finalise rotation Q <- Q V.
+template <int W>
+__global__ void __launch_bounds__(W * 64)
+k_retract(int nk, int nb, cd* __restrict__ Q, const cd* __restrict__ V,
+          const double* __restrict__ lam, const cd* __restrict__ Fe,
+          double* __restrict__ eps_work, int cayley, const double* sc) {
+    extern __shared__ cd smem[];
+    const int nb2 = nb * nb;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    cd* sV = smem + wave * ls_wave_cd(nb);
+    cd* sQ = sV + nb2;
+    cd* sU = sQ + nb2;
+    cd* sVC = sU + nb2;
+    cd* scf = sVC + nb2;
+    double* srow = reinterpret_cast<double*>(scf + nb);
+    double* slam = srow + nb2;
+    cd* sFe = sU;  // reused once U has been applied
+    const double tau = cayley ? sc[SC_TAU] : 0.0;
+    const int nsteps = (nk + gridDim.x * W - 1) / (gridDim.x * W);
+    for (int step = 0; step < nsteps; ++step) {
+        const int k = (step * gridDim.x + blockIdx.x) * W + wave;
+        const bool act = k < nk;
+        const size_t kb = static_cast<size_t>(k) * nb2;
+        if (act) {
+            for (int e = lane; e < nb2; e += 64) {
+                sV[e] = V[kb + e];
+                sQ[e] = Q[kb + e];
+            }
+            for (int i = lane; i < nb; i += 64) {
+                slam[i] = cayley ? lam[static_cast<size_t>(k) * nb + i] : 0.0;
+                scf[i] = cayley ? cayley_c(slam[i], tau) : cd{1.0, 0.0};
+            }
+        }
+        __syncthreads();
+        if (act) {
+            if (cayley) {
+                // U = (V diag(c)) V^H
+                for (int e = lane; e < nb2; e += 64) sVC[e] = cmul(sV[e], scf[e % nb]);
+            }
+        }
+        __syncthreads();
+        if (act) {
+            if (cayley) wave_mm(sVC, sV, sU, nb, false, true, lane);
+        }
+        __syncthreads();
+        if (act) {
+            // Q_new = Q U (cayley) or Q V (finalise)
+            for (int e = lane; e < nb2; e += 64) {
+                const int i = e / nb, j = e % nb;
+                cd s{0.0, 0.0};
+                const cd* R = cayley ? sU : sV;
+                for (int n = 0; n < nb; ++n) s = cadd(s, cmul(sQ[i * nb + n], R[n * nb + j]));
+                Q[kb + e] = s;
+            }
+        }
+        __syncthreads();
+        if (cayley) {
+            if (act)
+                for (int e = lane; e < nb2; e += 64) sFe[e] = Fe[kb + e];
+            __syncthreads();
+            diag_at_tau(sV, sFe, slam, sVC, scf, srow, nb, tau, lane);
+            if (act)
+                for (int i = lane; i < nb; i += 64) {
+                    double di = 0.0;
+                    for (int jj = 0; jj < nb; ++jj) di += srow[i * nb + jj];
+                    eps_work[static_cast<size_t>(k) * nb + i] = di;
+                }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SCF / generic reductions
+// ---------------------------------------------------------------------------
+
+// delta = Pn - D; comm = F D - D F (stored when comm != nullptr).
+// Slots: [sum w |delta|^2, sum w |comm|^2].
+template <int W>
+__global__ void __launch_bounds__(W * 64)
+k_scf_residuals(int nk, int nb, const cd* __restrict__ F, const cd* __restrict__ D,
+                const cd* __restrict__ Pn, cd* __restrict__ delta, cd* __restrict__ comm,
+                const double* __restrict__ w, double* partial, unsigned* counter, double* out) {
+    extern __shared__ cd smem[];
+    __shared__ double red[W * 2];
+    __shared__ bool is_last;
+    const int nb2 = nb * nb;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    cd* sF = smem + wave * 2 * nb2;
+    cd* sD = sF + nb2;
+    double acc[2] = {0.0, 0.0};
+    const int nsteps = (nk + gridDim.x * W - 1) / (gridDim.x * W);
+    for (int step = 0; step < nsteps; ++step) {
+        const int k = (step * gridDim.x + blockIdx.x) * W + wave;
+        const bool act = k < nk;
+        const size_t kb = static_cast<size_t>(k) * nb2;
+        if (act)
+            for (int e = lane; e < nb2; e += 64) {
+                sF[e] = F[kb + e];
+                sD[e] = D[kb + e];
+            }
+        __syncthreads();
+        if (act) {
+            const double wk = w[k];
+            double sd = 0.0, scm = 0.0;
+            for (int e = lane; e < nb2; e += 64) {
+                const int i = e / nb, j = e % nb;
+                const cd dl = csub(Pn[kb + e], sD[e]);
+                delta[kb + e] = dl;
+                sd += cnorm(dl);
+                cd fp{0.0, 0.0}, pf{0.0, 0.0};
+                for (int n = 0; n < nb; ++n) {
+                    fp = cadd(fp, cmul(sF[i * nb + n], sD[n * nb + j]));
+                    pf = cadd(pf, cmul(sD[i * nb + n], sF[n * nb + j]));
+                }
+                const cd c = csub(fp, pf);
+                if (comm) comm[kb + e] = c;
+                scm += cnorm(c);
+            }
+            acc[0] += wk * sd;
+            acc[1] += wk * scm;
+        }
+        __syncthreads();
+    }
+    store_block_partials<2, W>(acc, red, partial);
+    reduce_last_block<2, W>(partial, counter, out, red, &is_last);
+}
+
+// out = sum_k w_k sum_i Re(conj(X_ki) Y_ki)  (m = elements per k)
+__global__ void __launch_bounds__(256)
+k_wdot(int nk, int m, const cd* __restrict__ X, const cd* __restrict__ Y,
+       const double* __restrict__ w, double* partial, unsigned* counter, double* out) {
+    __shared__ double red[4];
+    __shared__ bool is_last;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double acc[1] = {0.0};
+    for (int k = blockIdx.x * 4 + wave; k < nk; k += gridDim.x * 4) {
+        const size_t kb = static_cast<size_t>(k) * m;
+        double s = 0.0;
+        for (int e = lane; e < m; e += 64) {
+            const cd x = X[kb + e], y = Y[kb + e];
+            s += x.x * y.x + x.y * y.y;
+        }
+        acc[0] += w[k] * s;
+    }
+    store_block_partials<1, 4>(acc, red, partial);
+    reduce_last_block<1, 4>(partial, counter, out, red, &is_last);
+}
+
+unsigned loop_blocks(int nk, int W) {
+    return std::max(1u, std::min<unsigned>(cdiv(nk, W), 1024u));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+#define HFGPU_DISPATCH_W(nb, CALL4, CALL1) \
+    do {                                   \
+        if ((nb) <= 16) { CALL4; } else { CALL1; } \
+    } while (0)
+
+static void check_nb(int nb) {
+    if (nb < 1 || nb > 32) throw std::invalid_argument("solver kernels support 1 <= nb <= 32");
+}
+
+void launch_orbital_grad(int nk, int nb, const cd* Q, cd* F, const double* p, const double* w,
+                         const MuState* mus, double T_r, bool grad, double* eps, cd* G,
+                         double* g_p, LoopScratch& ls, hipStream_t st) {
+    check_nb(nb);
+    const size_t per_wave = sizeof(cd) * 3 * nb * nb;
+    HFGPU_DISPATCH_W(nb,
+        hipLaunchKernelGGL(k_orbital_grad<4>, dim3(loop_blocks(nk, 4)), dim3(256), 4 * per_wave, st,
+                           nk, nb, Q, F, p, w, mus, T_r, grad ? 1 : 0, eps, G, g_p, ls.partial,
+                           ls.counter, ls.sc),
+        hipLaunchKernelGGL(k_orbital_grad<1>, dim3(loop_blocks(nk, 1)), dim3(64), per_wave, st,
+                           nk, nb, Q, F, p, w, mus, T_r, grad ? 1 : 0, eps, G, g_p, ls.partial,
+                           ls.counter, ls.sc));
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_precond(int nk, int nb, const cd* G, const cd* Gp, const double* eps,
+                    const double* p, const double* g_p, const double* g_pp, const double* w,
+                    double T_r, double denom_scale, cd* dQ, double* h_p, LoopScratch& ls,
+                    hipStream_t st) {
+    check_nb(nb);
+    HFGPU_DISPATCH_W(nb,
+        hipLaunchKernelGGL(k_precond<4>, dim3(loop_blocks(nk, 4)), dim3(256), 0, st, nk, nb, G, Gp,
+                           eps, p, g_p, g_pp, w, T_r, denom_scale, dQ, h_p, ls.partial,
+                           ls.counter, ls.sc),
+        hipLaunchKernelGGL(k_precond<1>, dim3(loop_blocks(nk, 1)), dim3(64), 0, st, nk, nb, G, Gp,
+                           eps, p, g_p, g_pp, w, T_r, denom_scale, dQ, h_p, ls.partial,
+                           ls.counter, ls.sc));
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_cg_combine(int nk, int nb, bool restart, cd* dQ, const cd* dQp, const double* h_p,
+                       const double* d_pp, double* d_p, const double* eps, const double* p,
+                       const double* w, double T_r, LoopScratch& ls, hipStream_t st) {
+    check_nb(nb);
+    HFGPU_DISPATCH_W(nb,
+        hipLaunchKernelGGL(k_cg_combine<4>, dim3(loop_blocks(nk, 4)), dim3(256), 0, st, nk, nb,
+                           restart ? 1 : 0, dQ, dQp, h_p, d_pp, d_p, eps, p, w, T_r, ls.partial,
+                           ls.counter, ls.sc),
+        hipLaunchKernelGGL(k_cg_combine<1>, dim3(loop_blocks(nk, 1)), dim3(64), 0, st, nk, nb,
+                           restart ? 1 : 0, dQ, dQp, h_p, d_pp, d_p, eps, p, w, T_r, ls.partial,
+                           ls.counter, ls.sc));
+    HFGPU_LAUNCH_CHECK();
+}
+
+static size_t ls_wave_bytes(int nb) { return sizeof(cd) * static_cast<size_t>(ls_wave_cd(nb)); }
+
+void launch_linesearch(int nk, int nb, const cd* V, const double* lam, const cd* Ft, cd* Fe,
+                       const double* p, const double* d_p, const double* w, double T_r,
+                       double max_step, double shrink, int nbt, LoopScratch& ls,
+                       hipStream_t st) {
+    check_nb(nb);
+    if (nbt < 0 || nbt > kMaxBt)
+        throw std::invalid_argument("bt_max above the GPU line-search limit of 16");
+    const size_t per_wave = ls_wave_bytes(nb);
+    HFGPU_DISPATCH_W(nb,
+        hipLaunchKernelGGL(k_linesearch<4>, dim3(loop_blocks(nk, 4)), dim3(256), 4 * per_wave, st,
+                           nk, nb, V, lam, Ft, Fe, p, d_p, w, T_r, max_step, shrink, nbt,
+                           ls.partial, ls.counter, ls.sc),
+        hipLaunchKernelGGL(k_linesearch<1>, dim3(loop_blocks(nk, 1)), dim3(64), per_wave, st,
+                           nk, nb, V, lam, Ft, Fe, p, d_p, w, T_r, max_step, shrink, nbt,
+                           ls.partial, ls.counter, ls.sc));
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_retract(int nk, int nb, cd* Q, const cd* V, const double* lam, const cd* Fe,
+                    double* eps_work, bool cayley, LoopScratch& ls, hipStream_t st) {
+    check_nb(nb);
+    const size_t per_wave = ls_wave_bytes(nb);
+    HFGPU_DISPATCH_W(nb,
+        hipLaunchKernelGGL(k_retract<4>, dim3(loop_blocks(nk, 4)), dim3(256), 4 * per_wave, st, nk,
+                           nb, Q, V, lam, Fe, eps_work, cayley ? 1 : 0, ls.sc),
+        hipLaunchKernelGGL(k_retract<1>, dim3(loop_blocks(nk, 1)), dim3(64), per_wave, st, nk, nb,
+                           Q, V, lam, Fe, eps_work, cayley ? 1 : 0, ls.sc));
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_scf_residuals(int nk, int nb, const cd* F, const cd* D, const cd* Pn, cd* delta,
+                          cd* comm, const double* w, LoopScratch& ls, double* out,
+                          hipStream_t st) {
+    check_nb(nb);
+    const size_t per_wave = sizeof(cd) * 2 * nb * nb;
+    HFGPU_DISPATCH_W(nb,
+        hipLaunchKernelGGL(k_scf_residuals<4>, dim3(loop_blocks(nk, 4)), dim3(256), 4 * per_wave,
+                           st, nk, nb, F, D, Pn, delta, comm, w, ls.partial, ls.counter, out),
+        hipLaunchKernelGGL(k_scf_residuals<1>, dim3(loop_blocks(nk, 1)), dim3(64), per_wave, st,
+                           nk, nb, F, D, Pn, delta, comm, w, ls.partial, ls.counter, out));
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_wdot(int nk, int m, const cd* X, const cd* Y, const double* w, LoopScratch& ls,
+                 double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_wdot, dim3(loop_blocks(nk, 4)), dim3(256), 0, st, nk, m, X, Y, w,
+                       ls.partial, ls.counter, out);
+    HFGPU_LAUNCH_CHECK();
+}
+
+}  // namespace hfgpu

@@ -1,0 +1,611 @@
+// pybind11 module `cpp_hf_amd._native`: the reference module's function
+// names and signatures (cpp/cpp_hf_native.cpp:242-841), bound to the C ABI of
+// libhfgpu (include/hfgpu.h).  Numpy arrays cross as raw pointers; outputs
+// are allocated here as numpy arrays and filled by the GPU library; the GIL
+// is released around every GPU call and re-acquired only by project_fn.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hfgpu.h"
+
+namespace py = pybind11;
+using c128 = std::complex<double>;
+using CArr = py::array_t<c128, py::array::c_style | py::array::forcecast>;
+using RArr = py::array_t<double, py::array::c_style | py::array::forcecast>;
+
+namespace {
+
+void check(int rc) {
+    if (rc == HF_OK) return;
+    const std::string msg = hfgpu_last_error();
+    if (rc == HF_EINVAL) throw py::value_error(msg);
+    throw std::runtime_error(msg);
+}
+
+double* dp(py::array& a) { return static_cast<double*>(a.mutable_data()); }
+const double* cdp(const py::array& a) { return static_cast<const double*>(a.data()); }
+
+std::vector<py::ssize_t> lead_shape(const py::array& a, int drop) {
+    std::vector<py::ssize_t> s;
+    for (int i = 0; i < a.ndim() - drop; ++i) s.push_back(a.shape(i));
+    return s;
+}
+
+// kernel_args dict -> hf_kernel_desc with the arrays kept alive.
+struct KernelArgs {
+    CArr h, VR, refP, Oi, Oj;
+    RArr w2d, HH, cg;
+    hf_kernel_desc d{};
+
+    explicit KernelArgs(py::object ka) {
+        h = py::cast<CArr>(ka["h"]);
+        VR = py::cast<CArr>(ka["VR"]);
+        refP = py::cast<CArr>(ka["refP"]);
+        w2d = py::cast<RArr>(ka["w2d"]);
+        HH = py::cast<RArr>(ka["HH"]);
+        cg = py::cast<RArr>(ka["contact_g"]);
+        Oi = py::cast<CArr>(ka["contact_Oi"]);
+        Oj = py::cast<CArr>(ka["contact_Oj"]);
+        // shape checks of make_kernel (cpp_hf_native.cpp:58-87)
+        if (h.ndim() != 4) throw py::value_error("h must be (nk1,nk2,nb,nb)");
+        if (h.shape(3) != h.shape(2))
+            throw py::value_error("h last two axes must be square (nb,nb)");
+        if (VR.ndim() != 4) throw py::value_error("VR must be (nk1,nk2,dv1,dv2)");
+        const bool has_refP = py::cast<bool>(ka["has_refP"]);
+        if (has_refP && (refP.ndim() != 4 || refP.shape(0) != h.shape(0) ||
+                         refP.shape(1) != h.shape(1) || refP.shape(2) != h.shape(2) ||
+                         refP.shape(3) != h.shape(3)))
+            throw py::value_error("refP must match h shape when has_refP is true");
+        if (cg.ndim() != 1) throw py::value_error("contact_g must be 1-D");
+        if (VR.shape(0) != h.shape(0) || VR.shape(1) != h.shape(1))
+            throw py::value_error("VR k-mesh must match h");
+        if (w2d.size() != h.shape(0) * h.shape(1))
+            throw py::value_error("w2d must be (nk1,nk2)");
+        d.nk1 = h.shape(0);
+        d.nk2 = h.shape(1);
+        d.nb = h.shape(2);
+        d.dv1 = VR.shape(2);
+        d.dv2 = VR.shape(3);
+        d.n_contact = cg.shape(0);
+        d.include_hartree = py::cast<bool>(ka["include_hartree"]);
+        d.include_exchange = py::cast<bool>(ka["include_exchange"]);
+        d.exchange_hcp = py::cast<bool>(ka["exchange_hcp"]);
+        d.has_refP = has_refP;
+        d.T = py::cast<double>(ka["T"]);
+        d.weight_sum = py::cast<double>(ka["weight_sum"]);
+        d.h = cdp(h);
+        d.VR = cdp(VR);
+        d.refP = has_refP ? cdp(refP) : nullptr;
+        d.w2d = cdp(w2d);
+        d.HH = cdp(HH);
+        d.contact_g = cdp(cg);
+        d.contact_Oi = cdp(Oi);
+        d.contact_Oj = cdp(Oj);
+    }
+};
+
+// RAII kernel handle.
+struct Handle {
+    hfgpu_kernel* k = nullptr;
+    explicit Handle(const hf_kernel_desc& d) {
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_kernel_create(&d, &k);
+        }
+        check(rc);
+    }
+    ~Handle() {
+        if (k) hfgpu_kernel_destroy(k);
+    }
+    Handle(const Handle&) = delete;
+    Handle& operator=(const Handle&) = delete;
+};
+
+// Python-owned device-resident kernel handle.
+struct PyKernel {
+    hfgpu_kernel* k = nullptr;
+    PyKernel() = default;
+    PyKernel(const PyKernel&) = delete;
+    PyKernel& operator=(const PyKernel&) = delete;
+    ~PyKernel() {
+        if (k) hfgpu_kernel_destroy(k);
+    }
+};
+
+// project_fn trampoline (cpp_hf_native.cpp:215-238): wraps the live buffer
+// without copying, calls the Python function, checks the shape, copies back.
+struct ProjectCtx {
+    py::object fn;
+    std::string error;
+    bool python_error = false;
+    py::object exc_type, exc_value, exc_tb;
+};
+
+int project_trampoline(void* user, double* M, size_t nk1, size_t nk2, size_t nb) {
+    auto* ctx = static_cast<ProjectCtx*>(user);
+    py::gil_scoped_acquire gil;
+    try {
+        std::vector<py::ssize_t> shape{(py::ssize_t)nk1, (py::ssize_t)nk2, (py::ssize_t)nb,
+                                       (py::ssize_t)nb};
+        py::array_t<c128> view(shape, reinterpret_cast<c128*>(M), py::none());
+        py::object result = ctx->fn(view);
+        CArr out = py::cast<CArr>(result);
+        if (out.ndim() != 4 || (size_t)out.shape(0) != nk1 || (size_t)out.shape(1) != nk2 ||
+            (size_t)out.shape(2) != nb || (size_t)out.shape(3) != nb) {
+            ctx->error = "project_fn returned an array of wrong shape";
+            return 1;
+        }
+        std::memmove(M, out.data(), nk1 * nk2 * nb * nb * sizeof(c128));
+        return 0;
+    } catch (py::error_already_set& e) {
+        ctx->python_error = true;
+        ctx->error = e.what();
+        return 1;
+    } catch (const std::exception& e) {
+        ctx->error = e.what();
+        return 1;
+    }
+}
+
+void check_project(int rc, const ProjectCtx* ctx) {
+    if (rc != HF_OK && ctx && !ctx->error.empty()) throw std::runtime_error(ctx->error);
+    check(rc);
+}
+
+py::array_t<c128> new_c(const std::vector<py::ssize_t>& shape) { return py::array_t<c128>(shape); }
+py::array_t<double> new_r(const std::vector<py::ssize_t>& shape) { return py::array_t<double>(shape); }
+
+}  // namespace
+
+PYBIND11_MODULE(_native, m) {
+    m.doc() = "cpp_hf_amd native core: HIP kernels for MI355X (gfx950) behind a C ABI";
+
+    m.def("device_count", []() { return hfgpu_device_count(); });
+
+    // --- Self-energy / FFT (cpp_hf_native.cpp:246-284) ---
+    m.def(
+        "selfenergy_fft_full",
+        [](CArr VR, CArr P, bool apply_ifftshift, bool hcp) {
+            if (P.ndim() != 4) throw py::value_error("P must be (nk1,nk2,nb,nb)");
+            if (VR.ndim() != 4) throw py::value_error("VR must be (nk1,nk2,dv1,dv2)");
+            if (VR.shape(0) != P.shape(0) || VR.shape(1) != P.shape(1))
+                throw py::value_error("VR k-mesh must match P");
+            if (P.shape(2) != P.shape(3)) throw py::value_error("P last two dims must be square");
+            const size_t nk1 = P.shape(0), nk2 = P.shape(1), nb = P.shape(2);
+            auto sigma = new_c({(py::ssize_t)nk1, (py::ssize_t)nk2, (py::ssize_t)nb, (py::ssize_t)nb});
+            py::array sg = sigma;
+            int rc;
+            {
+                py::gil_scoped_release rel;
+                rc = hfgpu_selfenergy_fft_full(cdp(VR), cdp(P), dp(sg), nk1, nk2, nb, VR.shape(2),
+                                               VR.shape(3), apply_ifftshift, hcp);
+            }
+            check(rc);
+            return sigma;
+        },
+        py::arg("VR"), py::arg("P"), py::arg("apply_ifftshift") = true,
+        py::arg("hermitian_channel_packing") = false);
+
+    m.def("clear_fft_plan_cache", []() {
+        py::gil_scoped_release rel;
+        hfgpu_clear_plan_cache();
+    });
+
+    // --- Hermitian eigh (cpp_hf_native.cpp:287-312) ---
+    m.def(
+        "eigh_batched",
+        [](CArr M) {
+            if (M.ndim() < 2) throw py::value_error("M needs at least 2 dims");
+            const size_t nb = M.shape(M.ndim() - 1);
+            if ((size_t)M.shape(M.ndim() - 2) != nb)
+                throw py::value_error("M last two dims must be square");
+            size_t nmat = 1;
+            for (int i = 0; i < M.ndim() - 2; ++i) nmat *= M.shape(i);
+            auto ws = lead_shape(M, 1);
+            auto w = new_r(ws);
+            auto V = new_c(lead_shape(M, 0));
+            py::array wa = w, va = V;
+            int rc;
+            {
+                py::gil_scoped_release rel;
+                rc = hfgpu_eigh_batched(cdp(M), dp(wa), dp(va), nmat, nb);
+            }
+            check(rc);
+            return py::make_tuple(w, V);
+        },
+        py::arg("M"));
+
+    // --- mu solvers (cpp_hf_native.cpp:361-377) ---
+    m.def(
+        "find_mu_bisection",
+        [](RArr bands, RArr weights, double n_e, double T, int maxiter) {
+            if (bands.ndim() < 1) throw py::value_error("bands must be at least 1-D");
+            size_t nk = 1;
+            for (int i = 0; i < bands.ndim() - 1; ++i) nk *= bands.shape(i);
+            const size_t nb = bands.shape(bands.ndim() - 1);
+            if ((size_t)weights.size() < nk) throw py::value_error("weights too short for bands");
+            double mu = 0.0;
+            int rc;
+            {
+                py::gil_scoped_release rel;
+                rc = hfgpu_find_mu_bisection(cdp(bands), cdp(weights), nk, nb, n_e, T, maxiter, &mu);
+            }
+            check(rc);
+            return mu;
+        },
+        py::arg("bands"), py::arg("weights"), py::arg("n_e"), py::arg("T"), py::arg("maxiter") = 0);
+
+    m.def(
+        "solve_mu_inloop",
+        [](RArr eps, RArr w_norm, double n_target_norm, double mu0, double T, int maxiter) {
+            size_t nk = 1;
+            for (int i = 0; i < eps.ndim() - 1; ++i) nk *= eps.shape(i);
+            const size_t nb = eps.shape(eps.ndim() - 1);
+            double mu = 0.0;
+            int rc;
+            {
+                py::gil_scoped_release rel;
+                rc = hfgpu_solve_mu_inloop(cdp(eps), cdp(w_norm), nk, nb, n_target_norm, mu0, T,
+                                           maxiter, &mu);
+            }
+            check(rc);
+            return mu;
+        },
+        py::arg("eps"), py::arg("w_norm"), py::arg("n_target_norm"), py::arg("mu0"),
+        py::arg("T"), py::arg("maxiter") = 25);
+
+    // --- build_fock (cpp_hf_native.cpp:380-415) ---
+    m.def("build_fock_apply", [](py::object kernel_args, CArr P, py::object project_fn) {
+        KernelArgs ka(kernel_args);
+        if (P.ndim() != 4 || (size_t)P.shape(0) != ka.d.nk1 || (size_t)P.shape(1) != ka.d.nk2 ||
+            (size_t)P.shape(2) != ka.d.nb || (size_t)P.shape(3) != ka.d.nb)
+            throw py::value_error("P must match h shape");
+        Handle H(ka.d);
+        std::vector<py::ssize_t> shape{(py::ssize_t)ka.d.nk1, (py::ssize_t)ka.d.nk2,
+                                       (py::ssize_t)ka.d.nb, (py::ssize_t)ka.d.nb};
+        auto S = new_c(shape), Hh = new_c(shape), F = new_c(shape);
+        py::array sa = S, ha = Hh, fa = F;
+        ProjectCtx ctx;
+        ctx.fn = project_fn;
+        const bool has_proj = !project_fn.is_none();
+        double E = 0.0;
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_build_fock_apply(H.k, cdp(P), dp(sa), dp(ha), dp(fa), &E,
+                                        has_proj ? project_trampoline : nullptr, &ctx);
+        }
+        check_project(rc, &ctx);
+        return py::make_tuple(S, Hh, F, E);
+    });
+
+    // --- SCF solver (cpp_hf_native.cpp:418-485) ---
+    m.def("solve_scf", [](py::object kernel_args, CArr P0, double n_e, size_t max_iter,
+                          double density_tol, double comm_tol, double mixing, double level_shift,
+                          py::object project_fn, std::vector<size_t> block_sizes,
+                          std::string acceleration, size_t diis_size, size_t diis_start,
+                          double diis_damping, double trust_radius, bool return_density,
+                          bool return_fock) {
+        KernelArgs ka(kernel_args);
+        if (P0.ndim() != 4 || (size_t)P0.shape(0) != ka.d.nk1 ||
+            (size_t)P0.shape(1) != ka.d.nk2 || (size_t)P0.shape(2) != ka.d.nb ||
+            (size_t)P0.shape(3) != ka.d.nb)
+            throw py::value_error("P0 must match h shape");
+        int accel;
+        if (acceleration == "linear") accel = 0;
+        else if (acceleration == "diis") accel = 1;
+        else if (acceleration == "oda") accel = 2;
+        else throw py::value_error("SCFConfig.acceleration must be 'linear', 'diis', or 'oda'");
+        hf_scf_config cfg{};
+        cfg.max_iter = max_iter;
+        cfg.density_tol = density_tol;
+        cfg.comm_tol = comm_tol;
+        cfg.mixing = mixing;
+        cfg.level_shift = level_shift;
+        cfg.acceleration = accel;
+        cfg.diis_size = diis_size;
+        cfg.diis_start = diis_start;
+        cfg.diis_damping = diis_damping;
+        cfg.trust_radius = trust_radius;
+        cfg.block_sizes = block_sizes.empty() ? nullptr : block_sizes.data();
+        cfg.n_blocks = block_sizes.size();
+        Handle H(ka.d);
+        std::vector<py::ssize_t> shape{(py::ssize_t)ka.d.nk1, (py::ssize_t)ka.d.nk2,
+                                       (py::ssize_t)ka.d.nb, (py::ssize_t)ka.d.nb};
+        auto dens = new_c(shape), fock = new_c(shape);
+        auto hE = new_r({(py::ssize_t)max_iter}), hD = new_r({(py::ssize_t)max_iter}),
+             hC = new_r({(py::ssize_t)max_iter});
+        py::array da = dens, fa = fock, ea = hE, ra = hD, ca = hC;
+        hf_scf_result res{};
+        res.density = dp(da);
+        res.fock = dp(fa);
+        res.hist_E = dp(ea);
+        res.hist_density = dp(ra);
+        res.hist_comm = dp(ca);
+        ProjectCtx ctx;
+        ctx.fn = project_fn;
+        const bool has_proj = !project_fn.is_none();
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_solve_scf(H.k, cdp(P0), n_e, &cfg, has_proj ? project_trampoline : nullptr,
+                                 &ctx, &res);
+        }
+        check_project(rc, &ctx);
+        const py::ssize_t n = (py::ssize_t)res.iterations;
+        py::object d_obj = py::none(), f_obj = py::none();
+        if (return_density) d_obj = dens;
+        if (return_fock) f_obj = fock;
+        return py::make_tuple(d_obj, f_obj, res.energy, res.mu, res.iterations,
+                              (bool)res.converged, hE[py::slice(0, n, 1)].attr("copy")(),
+                              hD[py::slice(0, n, 1)].attr("copy")(),
+                              hC[py::slice(0, n, 1)].attr("copy")());
+    });
+
+    // --- Direct minimisation (cpp_hf_native.cpp:488-582), optimizer "cg" ---
+    m.def("solve_dm", [](py::object kernel_args, CArr P0, double n_e, size_t max_iter,
+                         double tol_E, double tol_grad, double max_step, double bt_shrink,
+                         double denom_scale, size_t bt_max, size_t cg_restart,
+                         size_t plateau_window, int mu_maxiter, int optimizer, double tr_delta0,
+                         size_t tr_cg_max, CArr deflation_targets, double deflation_sigma,
+                         double deflation_length, std::vector<size_t> block_sizes,
+                         py::object project_fn, bool return_Q, bool return_density,
+                         bool return_fock) {
+        (void)tr_delta0;
+        (void)tr_cg_max;
+        (void)deflation_sigma;
+        (void)deflation_length;
+        if (optimizer != 0)
+            throw py::value_error(
+                "optimizer='newton' (trust-region Newton) is not available in cpp_hf_amd; "
+                "use optimizer='cg'");
+        if (deflation_targets.ndim() == 5 && deflation_targets.shape(0) > 0 && deflation_sigma > 0)
+            throw py::value_error("deflation is only supported with optimizer='newton'");
+        KernelArgs ka(kernel_args);
+        if (P0.ndim() != 4 || (size_t)P0.shape(0) != ka.d.nk1 ||
+            (size_t)P0.shape(1) != ka.d.nk2 || (size_t)P0.shape(2) != ka.d.nb ||
+            (size_t)P0.shape(3) != ka.d.nb)
+            throw py::value_error("P0 must match h shape");
+        hf_dm_config cfg{};
+        cfg.max_iter = max_iter;
+        cfg.tol_E = tol_E;
+        cfg.tol_grad = tol_grad;
+        cfg.plateau_window = plateau_window;
+        cfg.denom_scale = denom_scale;
+        cfg.max_step = max_step;
+        cfg.cg_restart = cg_restart;
+        cfg.bt_shrink = bt_shrink;
+        cfg.bt_max = bt_max;
+        cfg.mu_maxiter = mu_maxiter;
+        cfg.block_sizes = block_sizes.empty() ? nullptr : block_sizes.data();
+        cfg.n_blocks = block_sizes.size();
+        Handle H(ka.d);
+        std::vector<py::ssize_t> shape{(py::ssize_t)ka.d.nk1, (py::ssize_t)ka.d.nk2,
+                                       (py::ssize_t)ka.d.nb, (py::ssize_t)ka.d.nb};
+        std::vector<py::ssize_t> pshape{(py::ssize_t)ka.d.nk1, (py::ssize_t)ka.d.nk2,
+                                        (py::ssize_t)ka.d.nb};
+        auto Q = new_c(shape), dens = new_c(shape), fock = new_c(shape);
+        auto p = new_r(pshape);
+        const py::ssize_t mh = (py::ssize_t)std::max<size_t>(max_iter, 1);
+        auto hE = new_r({mh}), hG = new_r({mh});
+        py::array qa = Q, da = dens, fa = fock, pa = p, ea = hE, ga = hG;
+        hf_dm_result res{};
+        res.Q = dp(qa);
+        res.p = dp(pa);
+        res.density = dp(da);
+        res.fock = dp(fa);
+        res.hist_E = dp(ea);
+        res.hist_grad = dp(ga);
+        ProjectCtx ctx;
+        ctx.fn = project_fn;
+        const bool has_proj = !project_fn.is_none();
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_solve_dm(H.k, cdp(P0), n_e, &cfg, has_proj ? project_trampoline : nullptr,
+                                &ctx, &res);
+        }
+        check_project(rc, &ctx);
+        const py::ssize_t n = (py::ssize_t)res.n_iter;
+        py::object q_obj = py::none(), d_obj = py::none(), f_obj = py::none();
+        if (return_Q) q_obj = Q;
+        if (return_density) d_obj = dens;
+        if (return_fock) f_obj = fock;
+        return py::make_tuple(q_obj, p, d_obj, f_obj, res.mu, res.energy, res.n_iter,
+                              (bool)res.converged, hE[py::slice(0, n, 1)].attr("copy")(),
+                              hG[py::slice(0, n, 1)].attr("copy")());
+    });
+
+    // --- building blocks of the Python layer ---
+    m.def("hf_energy", [](CArr P, CArr h, CArr Sigma, CArr H, RArr w, py::object refP) {
+        if (P.ndim() != 4 || h.ndim() != 4 || Sigma.ndim() != 4)
+            throw py::value_error("P, h, Sigma must be (nk1,nk2,nb,nb)");
+        const size_t nk = P.shape(0) * P.shape(1), nb = P.shape(2);
+        const size_t nd = nk * nb * nb;
+        if ((size_t)h.size() != nd || (size_t)Sigma.size() != nd)
+            throw py::value_error("P, h, Sigma shapes differ");
+        const size_t nH = (size_t)H.size() / (nb * nb);
+        if (nH * nb * nb != (size_t)H.size() || (nH != 1 && nH != nk))
+            throw py::value_error("H must be (1,1,nb,nb) or (nk1,nk2,nb,nb)");
+        if ((size_t)w.size() != nk) throw py::value_error("weights must be (nk1,nk2)");
+        CArr r;
+        const double* rp = nullptr;
+        if (!refP.is_none()) {
+            r = py::cast<CArr>(refP);
+            if ((size_t)r.size() != nd) throw py::value_error("refP must match P");
+            rp = cdp(r);
+        }
+        double E = 0.0;
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_hf_energy(cdp(h), cdp(P), cdp(Sigma), cdp(H), nH, cdp(w), rp, nk, nb, &E);
+        }
+        check(rc);
+        return E;
+    });
+
+    m.def("occupation_entropy", [](RArr p, RArr w_norm) {
+        const size_t nb = p.ndim() ? p.shape(p.ndim() - 1) : 1;
+        const size_t nk = nb ? (size_t)p.size() / nb : 0;
+        if ((size_t)w_norm.size() != nk) throw py::value_error("w_norm must match p's k axes");
+        double S = 0.0;
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_occupation_entropy(cdp(p), cdp(w_norm), nk, nb, &S);
+        }
+        check(rc);
+        return S;
+    });
+
+    m.def(
+        "density_from_fock",
+        [](CArr F, RArr weights, double n_e, double T, int maxiter) {
+            if (F.ndim() < 2) throw py::value_error("F needs at least 2 dims");
+            const size_t nb = F.shape(F.ndim() - 1);
+            size_t nk = 1;
+            for (int i = 0; i < F.ndim() - 2; ++i) nk *= F.shape(i);
+            if ((size_t)weights.size() != nk) throw py::value_error("weights must match F's k axes");
+            auto P = new_c(lead_shape(F, 0));
+            py::array pa = P;
+            double mu = 0.0;
+            int rc;
+            {
+                py::gil_scoped_release rel;
+                rc = hfgpu_density_from_fock(cdp(F), cdp(weights), nk, nb, n_e, T, maxiter, dp(pa),
+                                             &mu);
+            }
+            check(rc);
+            return py::make_tuple(P, mu);
+        },
+        py::arg("F"), py::arg("weights"), py::arg("n_e"), py::arg("T"), py::arg("maxiter") = 0);
+
+    // --- Cayley test hooks (cpp_hf_native.cpp:585-651) ---
+    m.def("_cayley_spectral_setup", [](CArr d) {
+        if (d.ndim() < 2) throw py::value_error("d needs at least 2 dims");
+        const size_t nb = d.shape(d.ndim() - 1);
+        if ((size_t)d.shape(d.ndim() - 2) != nb) throw py::value_error("d last two dims must match");
+        size_t nk = 1;
+        for (int i = 0; i < d.ndim() - 2; ++i) nk *= d.shape(i);
+        auto V = new_c(lead_shape(d, 0));
+        auto lam = new_r(lead_shape(d, 1));
+        py::array va = V, la = lam;
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_cayley_spectral_setup(cdp(d), dp(va), dp(la), nk, nb);
+        }
+        check(rc);
+        return py::make_tuple(V, lam);
+    });
+
+    m.def("_cayley_unitary_from_spectrum", [](CArr V, RArr lam, double tau) {
+        if (V.ndim() < 2) throw py::value_error("V needs at least 2 dims");
+        const size_t nb = V.shape(V.ndim() - 1);
+        size_t nk = 1;
+        for (int i = 0; i < V.ndim() - 2; ++i) nk *= V.shape(i);
+        auto U = new_c(lead_shape(V, 0));
+        py::array ua = U;
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_cayley_unitary_from_spectrum(cdp(V), cdp(lam), tau, dp(ua), nk, nb);
+        }
+        check(rc);
+        return U;
+    });
+
+    m.def("_diag_UFU_from_spectrum", [](CArr V, CArr Ft_eig, RArr lam, double tau) {
+        if (V.ndim() < 2) throw py::value_error("V needs at least 2 dims");
+        const size_t nb = V.shape(V.ndim() - 1);
+        size_t nk = 1;
+        for (int i = 0; i < V.ndim() - 2; ++i) nk *= V.shape(i);
+        auto out = new_r(lead_shape(V, 1));
+        py::array oa = out;
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_diag_UFU_from_spectrum(cdp(V), cdp(Ft_eig), cdp(lam), tau, dp(oa), nk, nb);
+        }
+        check(rc);
+        return out;
+    });
+
+    // --- resample (cpp_hf_native.cpp:819-840) ---
+    m.def("resample_kgrid_2d", [](CArr values, size_t nk_new) {
+        if (values.ndim() < 2) throw py::value_error("need >= 2 dims");
+        const size_t nk_old = values.shape(0);
+        if ((size_t)values.shape(1) != nk_old)
+            throw py::value_error("First two axes must be square (nk_old, nk_old)");
+        size_t inner = 1;
+        std::vector<py::ssize_t> out_shape{(py::ssize_t)nk_new, (py::ssize_t)nk_new};
+        for (int i = 2; i < values.ndim(); ++i) {
+            inner *= values.shape(i);
+            out_shape.push_back(values.shape(i));
+        }
+        auto out = new_c(out_shape);
+        py::array oa = out;
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_resample_kgrid_2d(cdp(values), dp(oa), nk_old, nk_new, inner);
+        }
+        check(rc);
+        return out;
+    });
+
+    // --- device-resident handle (benchmarks, torch wrapper) ---
+    py::class_<PyKernel>(m, "GpuKernel")
+        .def(py::init([](py::object kernel_args) {
+                 KernelArgs ka(kernel_args);
+                 auto pk = std::make_unique<PyKernel>();
+                 int rc;
+                 {
+                     py::gil_scoped_release rel;
+                     rc = hfgpu_kernel_create(&ka.d, &pk->k);
+                 }
+                 check(rc);
+                 return pk;
+             }),
+             py::arg("kernel_args"))
+        .def_property_readonly("packable",
+                               [](const PyKernel& k) { return hfgpu_kernel_packable(k.k) != 0; })
+        .def(
+            "build_fock_device",
+            [](PyKernel& k, std::uintptr_t P, std::uintptr_t Sigma, std::uintptr_t F,
+               std::uintptr_t hdiag, bool p_hermitian, std::uintptr_t stream) {
+                int rc;
+                {
+                    py::gil_scoped_release rel;
+                    rc = hfgpu_build_fock_device(k.k, (const void*)P, (void*)Sigma, (void*)F,
+                                                 (void*)hdiag, p_hermitian ? 1 : 0, (void*)stream);
+                }
+                check(rc);
+            },
+            py::arg("P"), py::arg("Sigma"), py::arg("F"), py::arg("hdiag"),
+            py::arg("p_hermitian") = true, py::arg("stream") = 0)
+        .def(
+            "energy_device",
+            [](PyKernel& k, std::uintptr_t P, std::uintptr_t Sigma, std::uintptr_t hdiag,
+               std::uintptr_t E, std::uintptr_t stream) {
+                int rc;
+                {
+                    py::gil_scoped_release rel;
+                    rc = hfgpu_energy_device(k.k, (const void*)P, (const void*)Sigma,
+                                             (const void*)hdiag, (void*)E, (void*)stream);
+                }
+                check(rc);
+            },
+            py::arg("P"), py::arg("Sigma"), py::arg("hdiag"), py::arg("E"), py::arg("stream") = 0);
+}

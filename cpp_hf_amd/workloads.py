@@ -1,0 +1,140 @@
+"""Deterministic model problems: the QWZ and layered-QWZ lattice models.
+
+These build ``HartreeFockKernel`` inputs for the benchmark and test
+workloads:
+
+* ``qwz_kernel``      2-band Qi-Wu-Zhang model, d = (sin kx, sin ky,
+  m - cos kx - cos ky), scalar interaction V(q) = 2 pi s tanh(d_g |q|)/|q|
+  with V(0) = 0; exchange only.
+* ``layered_kernel``  nl layers x 2 sublattices (orbital 2 l + s), QWZ per
+  layer plus u (l - (nl - 1)/2) on site, interlayer hopping gamma1 between
+  B_l and A_{l+1}; layer-resolved V_ll'(q) = 2 pi s exp(-a |q| |l - l'|)
+  tanh(d_g |q|)/|q| with V(0) = 2 pi s d_g, Hartree on with reference
+  density I/2 and Hartree matrix V_ll'(0).
+
+Meshes are k_i = 2 pi i / n - pi (index n // 2 holds k = 0).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .problem import HartreeFockKernel
+
+
+@dataclass
+class Workload:
+    kernel: HartreeFockKernel
+    P0: np.ndarray
+    n_electrons: float
+    name: str
+
+
+def kgrid(n: int) -> np.ndarray:
+    return 2.0 * np.pi * np.arange(n) / n - np.pi
+
+
+def _qwz_block(kx, ky, mass):
+    dx, dy, dz = np.sin(kx), np.sin(ky), mass - np.cos(kx) - np.cos(ky)
+    h = np.zeros(kx.shape + (2, 2), dtype=np.complex128)
+    h[..., 0, 0] = dz
+    h[..., 1, 1] = -dz
+    h[..., 0, 1] = dx - 1j * dy
+    h[..., 1, 0] = dx + 1j * dy
+    return h
+
+
+def _v_of_q(absq, scale, gate):
+    out = np.empty_like(absq)
+    nz = absq > 0
+    out[nz] = 2.0 * np.pi * scale * np.tanh(gate * absq[nz]) / absq[nz]
+    return out, nz
+
+
+def initial_density(n: int, nb: int, seed: int = 0, amp: float = 0.05) -> np.ndarray:
+    """P0 = I/2 + amp * H with H a seeded Gaussian Hermitian array."""
+    rng = np.random.default_rng(seed)
+    s = (n, n, nb, nb)
+    X = rng.standard_normal(s) + 1j * rng.standard_normal(s)
+    H = 0.5 * (X + np.conj(np.swapaxes(X, -1, -2)))
+    return 0.5 * np.eye(nb)[None, None] + amp * H
+
+
+def qwz_kernel(n: int = 24, *, T: float = 0.1, mass: float = 0.5, scale: float = 0.8,
+               gate: float = 4.0, seed: int = 0) -> Workload:
+    """2-band QWZ, exchange only, scalar V (the Hermitian-packed path)."""
+    k = kgrid(n)
+    KX, KY = np.meshgrid(k, k, indexing="ij")
+    h = _qwz_block(KX, KY, mass)
+    w = np.full((n, n), 1.0 / (n * n))
+    absq = np.sqrt(KX ** 2 + KY ** 2)
+    V, nz = _v_of_q(absq, scale, gate)
+    V[~nz] = 0.0
+    kern = HartreeFockKernel(w, h, V[..., None, None], T=T)
+    return Workload(kern, initial_density(n, 2, seed), 1.0, f"qwz_n{n}_nb2")
+
+
+def layered_kernel(n: int = 96, n_layers: int = 4, *, T: float = 0.05, mass: float = 0.5,
+                   u: float = 0.1, gamma1: float = 0.4, scale: float = 0.8,
+                   gate: float = 4.0, alpha: float = 0.3, filling: float | None = None,
+                   hartree: bool = True, seed: int = 0) -> Workload:
+    """Layered QWZ with layer-resolved V (nb = 2 n_layers)."""
+    nb = 2 * n_layers
+    k = kgrid(n)
+    KX, KY = np.meshgrid(k, k, indexing="ij")
+    h = np.zeros((n, n, nb, nb), dtype=np.complex128)
+    blk = _qwz_block(KX, KY, mass)
+    for l in range(n_layers):
+        s = slice(2 * l, 2 * l + 2)
+        h[..., s, s] = blk
+        onsite = u * (l - 0.5 * (n_layers - 1))
+        h[..., 2 * l, 2 * l] += onsite
+        h[..., 2 * l + 1, 2 * l + 1] += onsite
+        if l + 1 < n_layers:
+            h[..., 2 * l + 1, 2 * (l + 1)] += gamma1
+            h[..., 2 * (l + 1), 2 * l + 1] += gamma1
+    layer = np.arange(nb) // 2
+    dl = np.abs(layer[:, None] - layer[None, :]).astype(np.float64)
+    absq = np.sqrt(KX ** 2 + KY ** 2)
+    base, nz = _v_of_q(absq, scale, gate)
+    Vq = base[..., None, None] * np.exp(-alpha * absq[..., None, None] * dl[None, None])
+    Vq[~nz] = 2.0 * np.pi * scale * gate
+    w = np.full((n, n), 1.0 / (n * n))
+    HH = np.full((nb, nb), 2.0 * np.pi * scale * gate)
+    refP = np.broadcast_to(0.5 * np.eye(nb, dtype=np.complex128), (n, n, nb, nb)).copy()
+    kern = HartreeFockKernel(
+        w, h, Vq, T=T, include_hartree=hartree, include_exchange=True,
+        reference_density=refP if hartree else None, hartree_matrix=HH if hartree else None,
+    )
+    n_e = (0.5 * nb + 0.1) if filling is None else filling
+    return Workload(kern, initial_density(n, nb, seed), float(n_e), f"layered_n{n}_nb{nb}")
+
+
+def config2(n: int = 96) -> Workload:
+    """8 bands (4 layers), T = 0.05, 4.1 electrons per k, Hartree on."""
+    return layered_kernel(n, 4, T=0.05, filling=4.1)
+
+
+def sweep_params(count: int = 48, seed: int = 1):
+    """Parameter draws of the batched sweep: mesh, bands and model knobs."""
+    rng = np.random.default_rng(seed)
+    meshes = np.array([36, 48, 60, 72, 96, 120])
+    bands = np.array([2, 4, 8, 12])
+    out = []
+    for _ in range(count):
+        n = int(rng.choice(meshes))
+        nb = int(rng.choice(bands))
+        u = float(rng.uniform(0.0, 0.3))
+        scale = float(rng.uniform(0.5, 1.5))
+        gate = float(rng.uniform(2.0, 8.0))
+        offset = float(rng.uniform(-0.25, 0.25))
+        out.append(dict(n=n, nb=nb, u=u, scale=scale, gate=gate, offset=offset))
+    return out
+
+
+def sweep_workload(p: dict, *, T: float = 0.05) -> Workload:
+    nl = p["nb"] // 2
+    return layered_kernel(p["n"], nl, T=T, u=p["u"], scale=0.8 * p["scale"], gate=p["gate"],
+                          filling=0.5 * p["nb"] + p["offset"])

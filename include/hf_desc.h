@@ -1,0 +1,130 @@
+/*
+ * Plain-C descriptors shared by the GPU library (libhfgpu) and the CPU oracle
+ * (liboracle_hfcpu).  They carry exactly the information of the reference's
+ * `kernel_args` dict (src/cpp_hf/solver.py:87-103) and of its two solver
+ * configs (cpp/include/cpp_hf/solver_dm.hpp:22-63,
+ * cpp/include/cpp_hf/solver_scf.hpp:20-50), as sizes, flags and raw pointers.
+ *
+ * Complex arrays are interleaved (re, im) doubles in C order, so a
+ * complex128 numpy array can be passed as `const double*` without a copy.
+ */
+#ifndef HF_DESC_H
+#define HF_DESC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes returned by every entry point. */
+enum hf_status {
+    HF_OK = 0,
+    HF_EINVAL = 1,   /* bad argument: maps to Python ValueError     */
+    HF_ERUNTIME = 2, /* runtime failure: maps to Python RuntimeError */
+    HF_EDEVICE = 3   /* HIP error: maps to Python RuntimeError       */
+};
+
+/*
+ * Hartree-Fock kernel description (reference: HFKernel, kernel.hpp:10-34,
+ * filled by make_kernel, cpp_hf_native.cpp:42-102).  All pointers are host
+ * pointers, borrowed for the duration of the call that receives them.
+ *   h, refP     : (nk1, nk2, nb, nb) complex
+ *   VR          : (nk1, nk2, dv1, dv2) complex, dv = 1 or nb
+ *   w2d         : (nk1, nk2) real
+ *   HH          : (nb, nb) real
+ *   contact_g   : (n_contact,) real; contact_Oi/Oj: (n_contact, nb, nb) complex
+ */
+typedef struct hf_kernel_desc {
+    size_t nk1, nk2, nb, dv1, dv2, n_contact;
+    int include_hartree;
+    int include_exchange;
+    int exchange_hcp;
+    int has_refP;
+    double T;
+    double weight_sum;
+    const double* h;
+    const double* VR;
+    const double* refP;
+    const double* w2d;
+    const double* HH;
+    const double* contact_g;
+    const double* contact_Oi;
+    const double* contact_Oj;
+} hf_kernel_desc;
+
+/* Reference: SCFConfig, solver_scf.hpp:20-50. acceleration: 0 linear, 1 diis, 2 oda. */
+typedef struct hf_scf_config {
+    size_t max_iter;
+    double density_tol;
+    double comm_tol;
+    double mixing;
+    double level_shift;
+    int acceleration;
+    size_t diis_size;
+    size_t diis_start;
+    double diis_damping;
+    double trust_radius;
+    const size_t* block_sizes; /* may be NULL when n_blocks == 0 */
+    size_t n_blocks;
+} hf_scf_config;
+
+/* Reference: SolverConfig, solver_dm.hpp:22-63 (optimizer "cg" only). */
+typedef struct hf_dm_config {
+    size_t max_iter;
+    double tol_E;
+    double tol_grad;
+    size_t plateau_window;
+    double denom_scale;
+    double max_step;
+    size_t cg_restart;
+    double bt_shrink;
+    size_t bt_max;
+    int mu_maxiter;
+    const size_t* block_sizes;
+    size_t n_blocks;
+} hf_dm_config;
+
+/*
+ * Host callback standing in for the reference's ProjectFn
+ * (fock.hpp:21-23; wrapped from Python in cpp_hf_native.cpp:215-238).
+ * It rewrites the (nk1, nk2, nb, nb) complex array M in place and returns 0,
+ * or returns non-zero to abort the caller with HF_ERUNTIME.
+ */
+typedef int (*hf_project_fn)(void* user, double* M, size_t nk1, size_t nk2,
+                             size_t nb);
+
+/* SCF outputs (reference: SCFResult, solver_scf.hpp:52-62).  density/fock may
+ * be NULL; hist_* hold max_iter entries, the first *iterations are valid. */
+typedef struct hf_scf_result {
+    double* density;
+    double* fock;
+    double energy;
+    double mu;
+    size_t iterations;
+    int converged;
+    double* hist_E;
+    double* hist_density;
+    double* hist_comm;
+} hf_scf_result;
+
+/* DM outputs (reference: DMResult, solver_dm.hpp:65-76).  Q/density/fock may
+ * be NULL; p is required; hist_* hold max_iter entries. */
+typedef struct hf_dm_result {
+    double* Q;
+    double* p;
+    double* density;
+    double* fock;
+    double mu;
+    double energy;
+    size_t n_iter;
+    int converged;
+    double* hist_E;
+    double* hist_grad;
+} hf_dm_result;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HF_DESC_H */

@@ -1,0 +1,180 @@
+/*
+ * libhfgpu — C ABI of the MI355X (gfx950) Hartree-Fock hot path.
+ *
+ * Every entry point here replaces one function of the reference's pybind11
+ * module `cpp_hf._native` (cpp/cpp_hf_native.cpp) or the C++ core it calls;
+ * the citation on each names it.  The pybind11 module
+ * `cpp_hf_amd._native` binds these with the reference's Python signatures,
+ * and any other host language (ctypes, cffi, JNI, cgo) can bind them too:
+ * plain pointers and sizes, no torch or HIP types in host-pointer calls.
+ *
+ * Conventions
+ *  - Complex arrays are interleaved (re, im) doubles in C order (numpy
+ *    complex128 memory).  Real arrays are doubles.
+ *  - Host-pointer entry points copy inputs to the GPU, run, copy outputs back
+ *    and synchronise before returning.  Output buffers are caller-allocated.
+ *  - Return value: HF_OK (0) or an hf_status code; hfgpu_last_error() gives
+ *    the message of the last failure on the calling thread.  HF_EINVAL maps
+ *    to Python ValueError (the reference's std::invalid_argument), the others
+ *    to RuntimeError.
+ *  - One GPU: the library runs on the current HIP device of the caller.
+ */
+#ifndef HFGPU_H
+#define HFGPU_H
+
+#include <stddef.h>
+
+#include "hf_desc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* hfgpu_last_error(void);
+
+/* Number of visible GPUs (0 when none); never fails. */
+int hfgpu_device_count(void);
+
+/* Drop the cached FFT geometries of kernel-less self-energy calls.
+ * Replaces: _native.clear_fft_plan_cache (cpp/cpp_hf_native.cpp:280-284) ->
+ * FftPlanCache::clear (cpp/include/cpp_hf/selfenergy.hpp:39-42). */
+void hfgpu_clear_plan_cache(void);
+
+/* ------------------------------------------------------------------------
+ * Device-resident kernel handle.
+ * Replaces: make_kernel (cpp/cpp_hf_native.cpp:42-102) building an HFKernel
+ * (cpp/include/cpp_hf/kernel.hpp:10-84).  Uploads h, VR, refP, w2d, HH and
+ * the contact terms once; validates shapes with the same messages.
+ * ---------------------------------------------------------------------- */
+typedef struct hfgpu_kernel hfgpu_kernel;
+
+int hfgpu_kernel_create(const hf_kernel_desc* desc, hfgpu_kernel** out);
+int hfgpu_kernel_destroy(hfgpu_kernel* k);
+/* 1 when the Hermitian-packed exchange (nb(nb+1)/2 channels) is usable for
+ * Hermitian densities: scalar V with exchange_hcp, or (nb, nb) V whose
+ * index-space kernel is real and symmetric in the band pair. */
+int hfgpu_kernel_packable(const hfgpu_kernel* k);
+
+/* ------------------------------------------------------------------------
+ * Exchange self-energy.
+ * Replaces: _native.selfenergy_fft_full (cpp/cpp_hf_native.cpp:246-278) ->
+ * selfenergy_fft_full / selfenergy_fft_full_hcp / ifftshift_2d_batch
+ * (cpp/include/cpp_hf/selfenergy.hpp:91-190).
+ *   VR: (nk1, nk2, dv1, dv2), P and sigma: (nk1, nk2, nb, nb).
+ * ---------------------------------------------------------------------- */
+int hfgpu_selfenergy_fft_full(const double* VR, const double* P, double* sigma, size_t nk1,
+                              size_t nk2, size_t nb, size_t dv1, size_t dv2,
+                              int apply_ifftshift, int hermitian_channel_packing);
+
+/* ------------------------------------------------------------------------
+ * Batched Hermitian eigendecomposition (hermitises its input first).
+ * Replaces: _native.eigh_batched (cpp/cpp_hf_native.cpp:287-312) ->
+ * eigh_batched / eigh_one (cpp/include/cpp_hf/linalg.hpp:39-66).
+ *   M, V: (nmat, nb, nb); w: (nmat, nb) ascending; V columns = eigenvectors.
+ * ---------------------------------------------------------------------- */
+int hfgpu_eigh_batched(const double* M, double* w, double* V, size_t nmat, size_t nb);
+
+/* ------------------------------------------------------------------------
+ * Chemical potential by bisection (maxiter <= 0 means 30).
+ * Replaces: _native.find_mu_bisection (cpp/cpp_hf_native.cpp:361-377) ->
+ * find_mu_bisection (cpp/include/cpp_hf/utils.hpp:49-75).
+ * ---------------------------------------------------------------------- */
+int hfgpu_find_mu_bisection(const double* bands, const double* weights, size_t nk, size_t nb,
+                            double n_e, double T, int maxiter, double* mu);
+
+/* In-loop Newton/bisection chemical potential on normalised weights.
+ * Replaces: solve_mu_inloop (cpp/include/cpp_hf/utils.hpp:79-116). */
+int hfgpu_solve_mu_inloop(const double* eps, const double* w_norm, size_t nk, size_t nb,
+                          double n_target_norm, double mu0, double T, int maxiter,
+                          double* mu);
+
+/* ------------------------------------------------------------------------
+ * Fock build + energy at density P.
+ * Replaces: _native.build_fock_apply (cpp/cpp_hf_native.cpp:380-415) ->
+ * build_fock + hf_energy (cpp/include/cpp_hf/fock.hpp:302-351).
+ *   Outputs Sigma, Hh (dense Hartree diagonal), F: (nk1, nk2, nb, nb); E.
+ *   project (nullable) is applied to F, followed by hermitisation, like the
+ *   reference's ProjectFn (fock.hpp:295-298).
+ * ---------------------------------------------------------------------- */
+int hfgpu_build_fock_apply(hfgpu_kernel* k, const double* P, double* Sigma, double* Hh,
+                           double* F, double* E, hf_project_fn project, void* user);
+
+/* Device-pointer Fock build for callers that keep P resident in HBM (torch
+ * tensors, benchmarks).  Replaces: build_fock_compact (fock.hpp:34-299).
+ *   dP, dSigma, dF: device (nk1, nk2, nb, nb) complex; dHdiag: device (nb).
+ *   p_hermitian != 0 asserts P == P^H (enables the packed exchange).
+ *   stream: a hipStream_t, or NULL for the handle's own stream.  Asynchronous. */
+int hfgpu_build_fock_device(hfgpu_kernel* k, const void* dP, void* dSigma, void* dF,
+                            void* dHdiag, int p_hermitian, void* stream);
+
+/* Device-pointer energy: E = hf_energy_with_hartree_diag (fock.hpp:355-383)
+ * written to device dE (one double).  Asynchronous. */
+int hfgpu_energy_device(hfgpu_kernel* k, const void* dP, const void* dSigma,
+                        const void* dHdiag, void* dE, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Public building blocks of the reference's Python layer, on the GPU.
+ * ---------------------------------------------------------------------- */
+
+/* E = sum_k w_k Re[Tr(h_k P_k) + 1/2 Tr((Sigma_k + H_k)(P_k - refP_k))].
+ * Replaces: fock.hf_energy (src/cpp_hf/fock.py:91-107) and hf_energy
+ * (cpp/include/cpp_hf/fock.hpp:324-351).  H has n_H k-blocks: 1 (broadcast,
+ * the (1, 1, nb, nb) Hartree matrix) or nk; refP may be NULL. */
+int hfgpu_hf_energy(const double* h, const double* P, const double* Sigma, const double* H,
+                    size_t n_H, const double* w, const double* refP, size_t nk, size_t nb,
+                    double* E);
+
+/* S = -sum_k w_k sum_b [p ln p + (1 - p) ln(1 - p)], p clipped to
+ * [1e-14, 1 - 1e-14].  Replaces: occupation_entropy
+ * (cpp/include/cpp_hf/fock.hpp:385-401; src/cpp_hf/fock.py:110-113). */
+int hfgpu_occupation_entropy(const double* p, const double* w_norm, size_t nk, size_t nb,
+                             double* S);
+
+/* P = herm(V diag(f) V^H) with V, eps = eigh(herm(F)), mu by bisection with
+ * maxiter steps (<= 0: 30) and f = fermidirac(eps - mu, T).
+ * Replaces: density_from_fock (cpp/include/cpp_hf/solver_scf.hpp:96-154,
+ * level_shift 0) and utils.density_matrix_from_fock
+ * (src/cpp_hf/utils.py:363-389). */
+int hfgpu_density_from_fock(const double* F, const double* weights, size_t nk, size_t nb,
+                            double n_e, double T, int maxiter, double* P, double* mu);
+
+/* ------------------------------------------------------------------------
+ * Solvers.
+ * Replaces: _native.solve_scf (cpp/cpp_hf_native.cpp:418-485) -> solve_scf
+ * (cpp/include/cpp_hf/solver_scf.hpp:156-444), and _native.solve_dm with
+ * optimizer "cg" (cpp/cpp_hf_native.cpp:488-582) -> solve_dm
+ * (cpp/include/cpp_hf/solver_dm.hpp:392-653).
+ * ---------------------------------------------------------------------- */
+int hfgpu_solve_scf(hfgpu_kernel* k, const double* P0, double n_e, const hf_scf_config* cfg,
+                    hf_project_fn project, void* user, hf_scf_result* out);
+int hfgpu_solve_dm(hfgpu_kernel* k, const double* P0, double n_e, const hf_dm_config* cfg,
+                   hf_project_fn project, void* user, hf_dm_result* out);
+
+/* ------------------------------------------------------------------------
+ * Cayley spectral helpers (test hooks of the reference).
+ * Replaces: _native._cayley_spectral_setup / _cayley_unitary_from_spectrum /
+ * _diag_UFU_from_spectrum (cpp/cpp_hf_native.cpp:585-651) ->
+ * cpp/include/cpp_hf/solver_dm.hpp:206-222, 258-272, 277-303.
+ * ---------------------------------------------------------------------- */
+int hfgpu_cayley_spectral_setup(const double* d, double* V, double* lam, size_t nk,
+                                size_t nb);
+int hfgpu_cayley_unitary_from_spectrum(const double* V, const double* lam, double tau,
+                                       double* U, size_t nk, size_t nb);
+int hfgpu_diag_UFU_from_spectrum(const double* V, const double* Ft_eig, const double* lam,
+                                 double tau, double* diag, size_t nk, size_t nb);
+
+/* ------------------------------------------------------------------------
+ * Periodic linear k-grid resampling (bit-exact index map).
+ * Replaces: _native.resample_kgrid_2d (cpp/cpp_hf_native.cpp:819-840) ->
+ * resample_kgrid_2d (cpp/include/cpp_hf/utils.hpp:120-176).
+ *   x: (nk_old, nk_old, inner), out: (nk_new, nk_new, inner) complex.
+ * ---------------------------------------------------------------------- */
+int hfgpu_resample_kgrid_2d(const double* x, double* out, size_t nk_old, size_t nk_new,
+                            size_t inner);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HFGPU_H */
