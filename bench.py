@@ -46,7 +46,21 @@ def main() -> int:
     ap.add_argument("--no-solves", action="store_true")
     ap.add_argument("--full-channels", action="store_true",
                     help="time the unpacked nb^2-channel exchange instead")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 64 MiB x 10 (scales the PMC byte counters)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        torch.cuda.set_device(0)
+        src = torch.ones(8 << 20, dtype=torch.float64, device="cuda")  # 64 MiB
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_launch": src.numel() * 8}), flush=True)
+        return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -11,6 +11,9 @@
 
 namespace hfgpu {
 
+constexpr int kFftThreads = 256;      // every exchange pass runs 256 threads
+constexpr int kTileMaxPerThread = 16; // tile elements per thread (n * C <= 4096)
+
 // Twiddle table + radix schedule for one FFT length (device resident).
 struct AxisPlan {
     int n = 0;
@@ -32,11 +35,11 @@ struct ExchangeGeom {
     bool packed = false;
     int clog = 0;   // log2 of channels per workgroup tile
     int nblk = 0;   // channel tiles
-    int threads_rows = 256, threads_cols = 256;
     size_t lds_rows = 0, lds_cols = 0;
     int* d_chan_src = nullptr;  // nch: a * nb + b
     int* d_chan_a = nullptr;    // nch
     int* d_chan_b = nullptr;    // nch
+    unsigned* d_counter = nullptr;  // last-block ticket of pass A
     AxisPlan ax1, ax2;          // along k1 (nk1) and along k2 (nk2)
 
     void init(int nk1_, int nk2_, int nb_, bool packed_);
@@ -50,6 +53,22 @@ enum ExchEpilogue : int {
     EPI_FOCK_PACKED = 2  // Sigma = Sigma_x + SC, F = herm(h + Sigma + diag(H))
 };
 
+// k-independent terms derived from rho_bar = sum_k w_k (P_k - refP_k):
+// the Hartree diagonal HH n (fock.hpp:222-241) and the contact self-energy
+// (fock.hpp:244-279).  Computed by the last workgroup of pass A.
+struct SmallTerms {
+    int enabled = 0;
+    int include_hartree = 0;
+    int has_contact = 0;
+    const double* HH = nullptr;
+    int n_contact = 0;
+    const double* contact_g = nullptr;
+    const cd* Oi = nullptr;
+    const cd* Oj = nullptr;
+    double* hdiag = nullptr;  // out (nb)
+    cd* SC = nullptr;         // out (nb, nb)
+};
+
 struct ExchangeArgs {
     const cd* P = nullptr;     // (nk, nb, nb)
     const cd* refP = nullptr;  // nullable
@@ -58,6 +77,7 @@ struct ExchangeArgs {
     cd* T = nullptr;           // workspace (nk, nch)
     cd* partial = nullptr;     // (nk1, nch) weighted row sums of P - refP, nullable
     const double* w2d = nullptr;
+    SmallTerms small;
     int epilogue = EPI_RAW_FULL;
     bool ifftshift = false;
     cd* Sigma = nullptr;       // output
@@ -67,30 +87,14 @@ struct ExchangeArgs {
     const double* hdiag = nullptr;  // (nb) Hartree diagonal, nullable
 };
 
-// Forward rows pass only computing the weighted row sums (no transform):
-// used when exchange is off but Hartree/contact terms need rho_bar.
+// Pass A without the transform: weighted row sums (+ small terms) only, for
+// kernels with exchange off but Hartree/contact terms on.
 void launch_row_sums(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st);
-// Passes A (rows fwd) and B (cols fwd, x VR, cols inv).
+// Passes A (rows fwd, + row sums and small terms) and B (cols fwd, x VR,
+// cols inv).
 void launch_exchange_fwd(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st);
-// Pass C (rows inv + epilogue).  Runs after the small-terms kernel when the
-// epilogue consumes SC/hdiag.
+// Pass C (rows inv + epilogue).
 void launch_exchange_inv(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st);
-
-// Reduce the row sums into rho_bar, then hdiag (Hartree) and SC (contact).
-struct SmallTermsArgs {
-    const cd* partial = nullptr;
-    int nrows = 0;           // nk1
-    bool include_hartree = false;
-    bool has_contact = false;
-    const double* HH = nullptr;
-    int n_contact = 0;
-    const double* contact_g = nullptr;
-    const cd* Oi = nullptr;
-    const cd* Oj = nullptr;
-    double* hdiag = nullptr;  // out (nb)
-    cd* SC = nullptr;         // out (nb, nb)
-};
-void launch_small_terms(const ExchangeGeom& g, const SmallTermsArgs& a, hipStream_t st);
 
 // Full-path assembly: Sigma += SC, F = herm(h + Sigma + diag(hdiag)).
 void launch_fock_assemble(int nk, int nb, const cd* h, cd* Sigma, cd* F, const cd* SC,

@@ -66,13 +66,18 @@ void AxisPlan::release() {
     d_tw = nullptr;
 }
 
-static bool tile_fits(const AxisPlan& ax, int C, int threads) {
+static bool tile_fits(const AxisPlan& ax, int C) {
+    if (static_cast<long>(ax.n) * C > static_cast<long>(kFftThreads) * kTileMaxPerThread)
+        return false;
     for (int r : ax.radices) {
         const long need = static_cast<long>(ax.n / r) * C;
-        if (need > static_cast<long>(maxb_of(r)) * threads) return false;
+        if (need > static_cast<long>(maxb_of(r)) * kFftThreads) return false;
     }
     return true;
 }
+
+// complex slots the last workgroup of pass A needs for the small terms
+static size_t small_terms_cd(int nb) { return 3 * static_cast<size_t>(nb) * nb + 257; }
 
 void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
     release();
@@ -80,6 +85,7 @@ void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
     nk2 = nk2_;
     nb = nb_;
     packed = packed_;
+    if (nb > 32) throw std::invalid_argument("the GPU Fock build supports nb <= 32");
     std::vector<int> src, ca, cb;
     for (int a = 0; a < nb; ++a)
         for (int b = packed ? a : 0; b < nb; ++b) {
@@ -90,37 +96,37 @@ void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
     nch = static_cast<int>(src.size());
     ax1.init(nk1);
     ax2.init(nk2);
-    // Widest tile of <= 16 channels whose stages fit the per-thread butterfly
-    // budget at 256 threads (up to 1024 threads for very long axes).
-    int cmax = 1;
-    while (cmax < nch && cmax < 16) cmax <<= 1;
-    int C = cmax, threads = 256;
-    while (C > 1 && !(tile_fits(ax1, C, threads) && tile_fits(ax2, C, threads))) C >>= 1;
-    while (!(tile_fits(ax1, C, threads) && tile_fits(ax2, C, threads))) {
-        threads *= 2;
-        if (threads > 1024) throw std::invalid_argument("k-mesh too large for the GPU FFT tile");
-    }
+    // Widest tile of <= 16 channels whose stages fit the per-thread
+    // butterfly and load budgets of a 256-thread workgroup.
+    int C = 1;
+    while (C < nch && C < 16) C <<= 1;
+    while (C > 1 && !(tile_fits(ax1, C) && tile_fits(ax2, C))) C >>= 1;
+    if (!(tile_fits(ax1, C) && tile_fits(ax2, C)))
+        throw std::invalid_argument("k-mesh too large for the GPU FFT tile");
     clog = 0;
     while ((1 << clog) < C) ++clog;
     nblk = (nch + C - 1) / C;
-    threads_rows = threads_cols = threads;
-    lds_rows = sizeof(cd) * (static_cast<size_t>(nk2) * C + nk2);
+    lds_rows = sizeof(cd) * std::max(static_cast<size_t>(nk2) * C + nk2, small_terms_cd(nb));
     lds_cols = sizeof(cd) * (static_cast<size_t>(nk1) * C + nk1);
     if (lds_rows > 160 * 1024 || lds_cols > 160 * 1024)
         throw std::invalid_argument("k-mesh too large for an LDS-resident FFT line");
     HFGPU_CHECK(hipMalloc(&d_chan_src, sizeof(int) * nch));
     HFGPU_CHECK(hipMalloc(&d_chan_a, sizeof(int) * nch));
     HFGPU_CHECK(hipMalloc(&d_chan_b, sizeof(int) * nch));
+    HFGPU_CHECK(hipMalloc(&d_counter, 64));
     HFGPU_CHECK(hipMemcpy(d_chan_src, src.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
     HFGPU_CHECK(hipMemcpy(d_chan_a, ca.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
     HFGPU_CHECK(hipMemcpy(d_chan_b, cb.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
+    HFGPU_CHECK(hipMemset(d_counter, 0, 64));
 }
 
 void ExchangeGeom::release() {
     if (d_chan_src) (void)hipFree(d_chan_src);
     if (d_chan_a) (void)hipFree(d_chan_a);
     if (d_chan_b) (void)hipFree(d_chan_b);
+    if (d_counter) (void)hipFree(d_counter);
     d_chan_src = d_chan_a = d_chan_b = nullptr;
+    d_counter = nullptr;
     ax1.release();
     ax2.release();
 }
@@ -130,32 +136,124 @@ void ExchangeGeom::release() {
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void load_twiddles(cd* s_tw, const cd* tw, int n) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s_tw[i] = tw[i];
+    for (int i = threadIdx.x; i < n; i += kFftThreads) s_tw[i] = tw[i];
+}
+
+// Move a tile into LDS with every load of a thread in flight at once:
+// element e of the tile comes from src(e).  total <= 16 * 256.
+template <class Src>
+__device__ __forceinline__ void load_tile(cd* s, int total, Src src) {
+    cd v[kTileMaxPerThread];
+#pragma unroll
+    for (int r = 0; r < kTileMaxPerThread; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < total) v[r] = src(e);
+    }
+#pragma unroll
+    for (int r = 0; r < kTileMaxPerThread; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < total) s[e] = v[r];
+    }
+}
+
+// rho_bar from the row sums, then the Hartree diagonal and the contact
+// self-energy; run by one workgroup (the last one of pass A).
+__device__ void small_terms_block(const cd* __restrict__ partial, int nrows, int nch,
+                                  const int* __restrict__ chan_a, const int* __restrict__ chan_b,
+                                  int nb, int packed, const SmallTerms& st, cd* smem) {
+    const int nb2 = nb * nb;
+    cd* rho = smem;
+    cd* tmp = rho + nb2;
+    cd* acc = tmp + nb2;
+    cd* grp = acc + nb2;   // <= 256 group partials
+    cd* trv = grp + 256;
+    const int G = nch >= kFftThreads ? 1 : kFftThreads / nch;
+    // rows split over G groups per channel, then groups summed in order
+    for (int t = threadIdx.x; t < G * nch; t += kFftThreads) {
+        const int ch = t % nch, g = t / nch;
+        cd s{0.0, 0.0};
+        for (int r = g; r < nrows; r += G) s = cadd(s, partial[static_cast<size_t>(r) * nch + ch]);
+        grp[t] = s;
+    }
+    __syncthreads();
+    for (int ch = threadIdx.x; ch < nch; ch += kFftThreads) {
+        cd s{0.0, 0.0};
+        for (int g = 0; g < G; ++g) s = cadd(s, grp[g * nch + ch]);
+        const int a = chan_a[ch], b = chan_b[ch];
+        rho[a * nb + b] = s;
+        if (packed && a != b) rho[b * nb + a] = cconj(s);
+    }
+    for (int e = threadIdx.x; e < nb2; e += kFftThreads) acc[e] = cd{0.0, 0.0};
+    __syncthreads();
+    if (st.include_hartree) {
+        for (int a = threadIdx.x; a < nb; a += kFftThreads) {
+            double s = 0.0;
+            for (int b = 0; b < nb; ++b) s += st.HH[a * nb + b] * rho[b * nb + b].x;
+            st.hdiag[a] = s;
+        }
+    } else {
+        for (int a = threadIdx.x; a < nb; a += kFftThreads) st.hdiag[a] = 0.0;
+    }
+    if (st.has_contact) {
+        for (int t = 0; t < st.n_contact; ++t) {
+            const double g = st.contact_g[t];
+            if (g == 0.0) continue;  // uniform across the block
+            const cd* oi = st.Oi + static_cast<size_t>(t) * nb2;
+            const cd* oj = st.Oj + static_cast<size_t>(t) * nb2;
+            if (threadIdx.x == 0) {
+                cd tr{0.0, 0.0};
+                for (int i = 0; i < nb; ++i)
+                    for (int j = 0; j < nb; ++j) tr = cadd(tr, cmul(oj[i * nb + j], rho[j * nb + i]));
+                trv[0] = tr;
+            }
+            for (int e = threadIdx.x; e < nb2; e += kFftThreads) {
+                const int i = e / nb, j = e % nb;
+                cd s{0.0, 0.0};
+                for (int k = 0; k < nb; ++k) s = cadd(s, cmul(oi[i * nb + k], rho[k * nb + j]));
+                tmp[e] = s;
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < nb2; e += kFftThreads) {
+                const int i = e / nb, j = e % nb;
+                cd s{0.0, 0.0};
+                for (int k = 0; k < nb; ++k) s = cadd(s, cmul(tmp[i * nb + k], oj[k * nb + j]));
+                // g Oi tr(Oj rho) - g Oi rho Oj
+                const cd herm = cmul(cmul(oi[e], trv[0]), cd{g, 0.0});
+                const cd fk = cmul(cd{-g, 0.0}, s);
+                acc[e] = cadd(acc[e], cadd(herm, fk));
+            }
+            __syncthreads();
+        }
+        for (int e = threadIdx.x; e < nb2; e += kFftThreads) st.SC[e] = acc[e];
+    }
 }
 
 // Pass A.  grid = (nblk, nk1).
-__global__ void k_rows_fwd(FftAxis ax2, int nk2, int nb2, int nch, int clog,
-                           const int* __restrict__ chan_src, const cd* __restrict__ P,
-                           const cd* __restrict__ refP, cd* __restrict__ T,
-                           cd* __restrict__ partial, const double* __restrict__ w2d,
-                           int do_fft) {
+__global__ void __launch_bounds__(kFftThreads)
+k_rows_fwd(FftAxis ax2, int nk2, int nb, int nch, int clog, int packed,
+           const int* __restrict__ chan_src, const int* __restrict__ chan_a,
+           const int* __restrict__ chan_b, const cd* __restrict__ P,
+           const cd* __restrict__ refP, cd* __restrict__ T, cd* __restrict__ partial,
+           const double* __restrict__ w2d, int do_fft, SmallTerms st, unsigned* counter) {
     extern __shared__ cd smem[];
+    __shared__ bool is_last;
     const int C = 1 << clog;
+    const int nb2 = nb * nb;
     cd* s = smem;
     cd* s_tw = smem + nk2 * C;
     const int k1 = blockIdx.y;
     const int c0 = blockIdx.x * C;
     if (do_fft) load_twiddles(s_tw, ax2.tw, nk2);
-    for (int e = threadIdx.x; e < nk2 * C; e += blockDim.x) {
-        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
+    load_tile(s, nk2 * C, [&](int e) {
+        const int i = e >> clog, ch = c0 + (e & (C - 1));
         cd v{0.0, 0.0};
         if (ch < nch) {
             const size_t off = static_cast<size_t>(k1 * nk2 + i) * nb2 + chan_src[ch];
             v = P[off];
             if (refP) v = csub(v, refP[off]);
         }
-        s[e] = v;
-    }
+        return v;
+    });
     __syncthreads();
     if (partial && threadIdx.x < C) {
         const int ch = c0 + threadIdx.x;
@@ -170,18 +268,34 @@ __global__ void k_rows_fwd(FftAxis ax2, int nk2, int nb2, int nch, int clog,
             partial[static_cast<size_t>(k1) * nch + ch] = acc;
         }
     }
-    if (!do_fft) return;
-    __syncthreads();
-    fft_tile<false>(s, s_tw, ax2, clog);
-    for (int e = threadIdx.x; e < nk2 * C; e += blockDim.x) {
-        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
-        if (ch < nch) T[static_cast<size_t>(k1 * nk2 + i) * nch + ch] = s[e];
+    if (do_fft) {
+        __syncthreads();
+        fft_tile<false>(s, s_tw, ax2, clog);
+#pragma unroll
+        for (int r = 0; r < kTileMaxPerThread; ++r) {
+            const int e = threadIdx.x + r * kFftThreads;
+            if (e < nk2 * C) {
+                const int ch = c0 + (e & (C - 1));
+                if (ch < nch) T[static_cast<size_t>(k1 * nk2 + (e >> clog)) * nch + ch] = s[e];
+            }
+        }
     }
+    if (!st.enabled) return;
+    // the last workgroup to finish reduces the row sums (deterministic order)
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = atomicAdd(counter, 1u) == gridDim.x * gridDim.y - 1;
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    small_terms_block(partial, gridDim.y, nch, chan_a, chan_b, nb, packed, st, smem);
+    if (threadIdx.x == 0) *counter = 0;
 }
 
 // Pass B.  grid = (nblk, nk2).
-__global__ void k_cols(FftAxis ax1, int nk1, int nk2, int nch, int clog,
-                       cd* __restrict__ T, const cd* __restrict__ VR, int nvr) {
+__global__ void __launch_bounds__(kFftThreads)
+k_cols(FftAxis ax1, int nk1, int nk2, int nch, int clog, cd* __restrict__ T,
+       const cd* __restrict__ VR, int nvr) {
     extern __shared__ cd smem[];
     const int C = 1 << clog;
     cd* s = smem;
@@ -189,37 +303,43 @@ __global__ void k_cols(FftAxis ax1, int nk1, int nk2, int nch, int clog,
     const int k2 = blockIdx.y;
     const int c0 = blockIdx.x * C;
     load_twiddles(s_tw, ax1.tw, nk1);
-    for (int e = threadIdx.x; e < nk1 * C; e += blockDim.x) {
-        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
-        cd v{0.0, 0.0};
-        if (ch < nch) v = T[static_cast<size_t>(i * nk2 + k2) * nch + ch];
-        s[e] = v;
-    }
+    load_tile(s, nk1 * C, [&](int e) {
+        const int ch = c0 + (e & (C - 1));
+        return ch < nch ? T[static_cast<size_t>((e >> clog) * nk2 + k2) * nch + ch] : cd{0.0, 0.0};
+    });
     __syncthreads();
     fft_tile<false>(s, s_tw, ax1, clog);
-    for (int e = threadIdx.x; e < nk1 * C; e += blockDim.x) {
-        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
-        if (ch < nch) {
-            const size_t q = static_cast<size_t>(i * nk2 + k2);
-            const cd v = nvr == 1 ? VR[q] : VR[q * nvr + ch];
-            s[e] = cmul(s[e], v);
+#pragma unroll
+    for (int r = 0; r < kTileMaxPerThread; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < nk1 * C) {
+            const int ch = c0 + (e & (C - 1));
+            if (ch < nch) {
+                const size_t q = static_cast<size_t>((e >> clog) * nk2 + k2);
+                const cd v = nvr == 1 ? VR[q] : VR[q * nvr + ch];
+                s[e] = cmul(s[e], v);
+            }
         }
     }
     __syncthreads();
     fft_tile<true>(s, s_tw, ax1, clog);
-    for (int e = threadIdx.x; e < nk1 * C; e += blockDim.x) {
-        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
-        if (ch < nch) T[static_cast<size_t>(i * nk2 + k2) * nch + ch] = s[e];
+#pragma unroll
+    for (int r = 0; r < kTileMaxPerThread; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < nk1 * C) {
+            const int ch = c0 + (e & (C - 1));
+            if (ch < nch) T[static_cast<size_t>((e >> clog) * nk2 + k2) * nch + ch] = s[e];
+        }
     }
 }
 
 // Pass C.  grid = (nblk, nk1).
-__global__ void k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog,
-                           const int* __restrict__ chan_src, const int* __restrict__ chan_a,
-                           const int* __restrict__ chan_b, const cd* __restrict__ T,
-                           int epilogue, int shift, double inv_n, cd* __restrict__ Sigma,
-                           cd* __restrict__ F, const cd* __restrict__ h,
-                           const cd* __restrict__ SC, const double* __restrict__ hdiag) {
+__global__ void __launch_bounds__(kFftThreads)
+k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog,
+           const int* __restrict__ chan_src, const int* __restrict__ chan_a,
+           const int* __restrict__ chan_b, const cd* __restrict__ T, int epilogue, int shift,
+           double inv_n, cd* __restrict__ Sigma, cd* __restrict__ F, const cd* __restrict__ h,
+           const cd* __restrict__ SC, const double* __restrict__ hdiag) {
     extern __shared__ cd smem[];
     const int C = 1 << clog;
     cd* s = smem;
@@ -228,18 +348,19 @@ __global__ void k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int c
     const int c0 = blockIdx.x * C;
     const int nb2 = nb * nb;
     load_twiddles(s_tw, ax2.tw, nk2);
-    for (int e = threadIdx.x; e < nk2 * C; e += blockDim.x) {
-        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
-        cd v{0.0, 0.0};
-        if (ch < nch) v = T[static_cast<size_t>(k1 * nk2 + i) * nch + ch];
-        s[e] = v;
-    }
+    load_tile(s, nk2 * C, [&](int e) {
+        const int ch = c0 + (e & (C - 1));
+        return ch < nch ? T[static_cast<size_t>(k1 * nk2 + (e >> clog)) * nch + ch] : cd{0.0, 0.0};
+    });
     __syncthreads();
     fft_tile<true>(s, s_tw, ax2, clog);
     // destination row (ifftshift remaps source (i, j) -> (i - n/2, j - n/2))
     const int d1 = shift ? (k1 + nk1 - nk1 / 2) % nk1 : k1;
-    for (int e = threadIdx.x; e < nk2 * C; e += blockDim.x) {
-        const int i = e >> clog, c = e & (C - 1), ch = c0 + c;
+#pragma unroll
+    for (int r = 0; r < kTileMaxPerThread; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e >= nk2 * C) continue;
+        const int i = e >> clog, ch = c0 + (e & (C - 1));
         if (ch >= nch) continue;
         const cd y = s[e];
         // reference order: ifftn = backward * (1/N), then negate
@@ -255,7 +376,8 @@ __global__ void k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int c
                 Sigma[iab] = sig;
                 if (a != b) Sigma[iba] = cconj(sig);
             } else {
-                cd s_ab = sig, s_ba = cconj(sig);
+                // a == b is one element: its mirror is itself, not its conjugate
+                cd s_ab = sig, s_ba = a == b ? sig : cconj(sig);
                 if (SC) {
                     s_ab = cadd(s_ab, SC[a * nb + b]);
                     s_ba = cadd(s_ba, SC[b * nb + a]);
@@ -276,73 +398,6 @@ __global__ void k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int c
             }
         }
     }
-}
-
-// rho_bar from row sums, then Hartree diagonal and contact self-energy.
-// One workgroup of 256 threads; nb <= 32.
-__global__ void k_small_terms(const cd* __restrict__ partial, int nrows, int nch,
-                              const int* __restrict__ chan_a, const int* __restrict__ chan_b,
-                              int nb, int packed, int include_hartree, int has_contact,
-                              const double* __restrict__ HH, int n_contact,
-                              const double* __restrict__ cg, const cd* __restrict__ Oi,
-                              const cd* __restrict__ Oj, double* __restrict__ hdiag,
-                              cd* __restrict__ SC) {
-    __shared__ cd rho[32 * 32];
-    __shared__ cd tmp[32 * 32];
-    __shared__ cd acc[32 * 32];
-    __shared__ cd trv;
-    const int nb2 = nb * nb;
-    for (int ch = threadIdx.x; ch < nch; ch += blockDim.x) {
-        cd s{0.0, 0.0};
-        for (int r = 0; r < nrows; ++r) s = cadd(s, partial[static_cast<size_t>(r) * nch + ch]);
-        const int a = chan_a[ch], b = chan_b[ch];
-        rho[a * nb + b] = s;
-        if (packed && a != b) rho[b * nb + a] = cconj(s);
-    }
-    for (int e = threadIdx.x; e < nb2; e += blockDim.x) acc[e] = cd{0.0, 0.0};
-    __syncthreads();
-    if (include_hartree) {
-        for (int a = threadIdx.x; a < nb; a += blockDim.x) {
-            double s = 0.0;
-            for (int b = 0; b < nb; ++b) s += HH[a * nb + b] * rho[b * nb + b].x;
-            hdiag[a] = s;
-        }
-    } else {
-        for (int a = threadIdx.x; a < nb; a += blockDim.x) hdiag[a] = 0.0;
-    }
-    if (has_contact) {
-        for (int t = 0; t < n_contact; ++t) {
-            const double g = cg[t];
-            if (g == 0.0) continue;  // uniform across the block
-            const cd* oi = Oi + static_cast<size_t>(t) * nb2;
-            const cd* oj = Oj + static_cast<size_t>(t) * nb2;
-            if (threadIdx.x == 0) {
-                cd tr{0.0, 0.0};
-                for (int i = 0; i < nb; ++i)
-                    for (int j = 0; j < nb; ++j) tr = cadd(tr, cmul(oj[i * nb + j], rho[j * nb + i]));
-                trv = tr;
-            }
-            // tmp = Oi * rho
-            for (int e = threadIdx.x; e < nb2; e += blockDim.x) {
-                const int i = e / nb, j = e % nb;
-                cd s{0.0, 0.0};
-                for (int k = 0; k < nb; ++k) s = cadd(s, cmul(oi[i * nb + k], rho[k * nb + j]));
-                tmp[e] = s;
-            }
-            __syncthreads();
-            for (int e = threadIdx.x; e < nb2; e += blockDim.x) {
-                const int i = e / nb, j = e % nb;
-                cd s{0.0, 0.0};
-                for (int k = 0; k < nb; ++k) s = cadd(s, cmul(tmp[i * nb + k], oj[k * nb + j]));
-                // Oi * tr * g + (-g) * (Oi rho Oj)
-                const cd herm = cmul(cmul(oi[e], trv), cd{g, 0.0});
-                const cd fk = cmul(cd{-g, 0.0}, s);
-                acc[e] = cadd(acc[e], cadd(herm, fk));
-            }
-            __syncthreads();
-        }
-    }
-    for (int e = threadIdx.x; e < nb2; e += blockDim.x) SC[e] = acc[e];
 }
 
 // grid-stride over (k, a <= b) pairs.
@@ -428,19 +483,20 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
 // ---------------------------------------------------------------------------
 
 void launch_row_sums(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
-    dim3 grid(g.nblk, g.nk1);
-    hipLaunchKernelGGL(k_rows_fwd, grid, dim3(g.threads_rows), g.lds_rows, st, g.ax2.axis(),
-                       g.nk2, g.nb * g.nb, g.nch, g.clog, g.d_chan_src, a.P, a.refP, a.T,
-                       a.partial, a.w2d, 0);
+    hipLaunchKernelGGL(k_rows_fwd, dim3(g.nblk, g.nk1), dim3(kFftThreads), g.lds_rows, st,
+                       g.ax2.axis(), g.nk2, g.nb, g.nch, g.clog, g.packed ? 1 : 0, g.d_chan_src,
+                       g.d_chan_a, g.d_chan_b, a.P, a.refP, a.T, a.partial, a.w2d, 0, a.small,
+                       g.d_counter);
     HFGPU_LAUNCH_CHECK();
 }
 
 void launch_exchange_fwd(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_rows_fwd, dim3(g.nblk, g.nk1), dim3(g.threads_rows), g.lds_rows, st,
-                       g.ax2.axis(), g.nk2, g.nb * g.nb, g.nch, g.clog, g.d_chan_src, a.P,
-                       a.refP, a.T, a.partial, a.w2d, 1);
+    hipLaunchKernelGGL(k_rows_fwd, dim3(g.nblk, g.nk1), dim3(kFftThreads), g.lds_rows, st,
+                       g.ax2.axis(), g.nk2, g.nb, g.nch, g.clog, g.packed ? 1 : 0, g.d_chan_src,
+                       g.d_chan_a, g.d_chan_b, a.P, a.refP, a.T, a.partial, a.w2d, 1, a.small,
+                       g.d_counter);
     HFGPU_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_cols, dim3(g.nblk, g.nk2), dim3(g.threads_cols), g.lds_cols, st,
+    hipLaunchKernelGGL(k_cols, dim3(g.nblk, g.nk2), dim3(kFftThreads), g.lds_cols, st,
                        g.ax1.axis(), g.nk1, g.nk2, g.nch, g.clog, a.T, a.VR, a.nvr);
     HFGPU_LAUNCH_CHECK();
 }
@@ -451,19 +507,10 @@ void launch_exchange_inv(const ExchangeGeom& g, const ExchangeArgs& a, hipStream
     if (a.epilogue == EPI_RAW_FULL && g.packed)
         throw std::logic_error("full epilogue on a packed geometry");
     const double inv_n = 1.0 / static_cast<double>(static_cast<size_t>(g.nk1) * g.nk2);
-    hipLaunchKernelGGL(k_rows_inv, dim3(g.nblk, g.nk1), dim3(g.threads_rows), g.lds_rows, st,
+    hipLaunchKernelGGL(k_rows_inv, dim3(g.nblk, g.nk1), dim3(kFftThreads), g.lds_rows, st,
                        g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.d_chan_src,
                        g.d_chan_a, g.d_chan_b, a.T, a.epilogue, a.ifftshift ? 1 : 0, inv_n,
                        a.Sigma, a.F, a.h, a.SC, a.hdiag);
-    HFGPU_LAUNCH_CHECK();
-}
-
-void launch_small_terms(const ExchangeGeom& g, const SmallTermsArgs& a, hipStream_t st) {
-    if (g.nb > 32) throw std::invalid_argument("Hartree/contact terms support nb <= 32");
-    hipLaunchKernelGGL(k_small_terms, dim3(1), dim3(256), 0, st, a.partial, a.nrows, g.nch,
-                       g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0,
-                       a.include_hartree ? 1 : 0, a.has_contact ? 1 : 0, a.HH, a.n_contact,
-                       a.contact_g, a.Oi, a.Oj, a.hdiag, a.SC);
     HFGPU_LAUNCH_CHECK();
 }
 

@@ -91,6 +91,17 @@ struct GpuKernel {
                     hipStream_t st);
     // hf_energy_with_hartree_diag on device into E (one double).
     void energy(const cd* P, const cd* Sigma, const double* hdiag_in, double* E, hipStream_t st);
+
+    // build_fock replayed from a captured hipGraph (one per pointer set,
+    // small FIFO); falls back to eager launches when HFGPU_NO_GRAPH is set.
+    void build_fock_graph(const cd* P, cd* Sigma, cd* F, double* hdiag_out, bool p_hermitian,
+                          hipStream_t st);
+    struct GraphEntry {
+        const void *P, *Sigma, *F, *hdiag;
+        bool herm;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
 };
 
 // Last error message of the calling thread (hfgpu_last_error).

@@ -2,6 +2,7 @@
 // C ABI entry points that are not solver loops.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -215,6 +216,7 @@ GpuKernel::~GpuKernel() {
         (void)hipStreamSynchronize(stream);
         (void)hipStreamDestroy(stream);
     }
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (geom_full) geom_full->release();
     if (geom_packed) geom_packed->release();
 }
@@ -246,17 +248,12 @@ void GpuKernel::build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, boo
         a.VR = VR_full.as<cd>();
         a.nvr = dv * dv;
     }
-    if (exchange)
-        launch_exchange_fwd(g, a, st);
-    else if (need_partials)
-        launch_row_sums(g, a, st);
     double* hd = hdiag_out ? hdiag_out : hdiag.as<double>();
     if (need_partials) {
-        SmallTermsArgs s;
-        s.partial = partial.as<cd>();
-        s.nrows = nk1;
-        s.include_hartree = hartree;
-        s.has_contact = contact;
+        SmallTerms& s = a.small;
+        s.enabled = 1;
+        s.include_hartree = hartree ? 1 : 0;
+        s.has_contact = contact ? 1 : 0;
         s.HH = HH.as<double>();
         s.n_contact = n_contact;
         s.contact_g = cg.as<double>();
@@ -264,10 +261,12 @@ void GpuKernel::build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, boo
         s.Oj = Oj.as<cd>();
         s.hdiag = hd;
         s.SC = SC.as<cd>();
-        launch_small_terms(g, s, st);
-    } else {
-        fill_zero(hd, nb * sizeof(double), st);
     }
+    if (exchange)
+        launch_exchange_fwd(g, a, st);
+    else if (need_partials)
+        launch_row_sums(g, a, st);
+    if (!need_partials) fill_zero(hd, nb * sizeof(double), st);
     const cd* sc = contact ? SC.as<cd>() : nullptr;
     const double* hdp = hartree ? hd : nullptr;
     if (exchange) {
@@ -289,6 +288,44 @@ void GpuKernel::build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, boo
         fill_zero(Sigma, nd * sizeof(cd), st);
         launch_fock_assemble(static_cast<int>(nk), nb, h.as<cd>(), Sigma, F, sc, hdp, st);
     }
+}
+
+void GpuKernel::build_fock_graph(const cd* P, cd* Sigma, cd* F, double* hdiag_out,
+                                 bool p_hermitian, hipStream_t st) {
+    static const bool no_graph = std::getenv("HFGPU_NO_GRAPH") != nullptr;
+    if (no_graph) {
+        build_fock(P, Sigma, F, hdiag_out, p_hermitian, st);
+        return;
+    }
+    for (auto& g : graphs)
+        if (g.P == P && g.Sigma == Sigma && g.F == F && g.hdiag == hdiag_out &&
+            g.herm == p_hermitian) {
+            HFGPU_CHECK(hipGraphLaunch(g.exec, st));
+            return;
+        }
+    // geometry set-up allocates and copies: never inside a capture
+    (void)geom(exchange && packed_for(p_hermitian));
+    HFGPU_CHECK(hipStreamSynchronize(stream));
+    hipGraph_t graph = nullptr;
+    HFGPU_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+        build_fock(P, Sigma, F, hdiag_out, p_hermitian, stream);
+    } catch (...) {
+        (void)hipStreamEndCapture(stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HFGPU_CHECK(hipStreamEndCapture(stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HFGPU_CHECK(ie);
+    if (graphs.size() >= 4) {
+        (void)hipGraphExecDestroy(graphs.front().exec);
+        graphs.erase(graphs.begin());
+    }
+    graphs.push_back(GraphEntry{P, Sigma, F, hdiag_out, p_hermitian, exec});
+    HFGPU_CHECK(hipGraphLaunch(exec, st));
 }
 
 void GpuKernel::energy(const cd* P, const cd* Sigma, const double* hdiag_in, double* E,
@@ -520,8 +557,9 @@ int hfgpu_build_fock_device(hfgpu_kernel* k, const void* dP, void* dSigma, void*
         if (!k) throw std::invalid_argument("null kernel");
         GpuKernel& K = *k->impl;
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : K.stream;
-        K.build_fock(static_cast<const cd*>(dP), static_cast<cd*>(dSigma), static_cast<cd*>(dF),
-                     static_cast<double*>(dHdiag), p_hermitian != 0, s);
+        K.build_fock_graph(static_cast<const cd*>(dP), static_cast<cd*>(dSigma),
+                           static_cast<cd*>(dF), static_cast<double*>(dHdiag), p_hermitian != 0,
+                           s);
     });
 }
 

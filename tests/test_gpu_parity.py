@@ -119,13 +119,36 @@ def test_eigh_through_ctypes_c_abi(gpu):
 def test_mu_bisection_matches_oracle(gpu, oracle):
     rng = np.random.RandomState(42)
     for shape, n_e, T in [((4, 4, 6), 3.0, 0.05), ((96, 96, 8), 4.1, 0.05), ((1, 1, 2), 1.0, 0.1),
-                          ((24, 24, 2), 1.0, 1e-5)]:
+                          ((24, 24, 2), 1.1, 0.02)]:
         bands = rng.randn(*shape)
         w = np.full(shape[:2], 1.0 / (shape[0] * shape[1]))
         for maxiter in (0, 7, 54):
             got = gpu.find_mu_bisection(bands, w, n_e, T, maxiter)
             ref = oracle.find_mu_bisection(bands, w, n_e, T, maxiter)
-            assert abs(got - ref) <= 1e-15 * max(1.0, abs(ref)), (shape, maxiter, got, ref)
+            # Up to the default 30 halvings the count differs from n_e by far
+            # more than summation rounding, so both take the same decisions and
+            # land on the same midpoint.  Deeper (54 halvings, the float32
+            # default) the bracket shrinks below the rounding of the count and
+            # the decisions may differ: agreement is then 1e-12.
+            tol = 1e-15 if maxiter <= 30 else 1e-12
+            assert abs(got - ref) <= tol * max(1.0, abs(ref)), (shape, maxiter, got, ref)
+
+
+def test_mu_bisection_cold_limit_hits_target(gpu, oracle):
+    """T -> 0 with an integer filling: the count is a step function equal to
+    n_e over a whole gap, so `count > n_e` compares equal numbers whose last
+    bits depend on the summation order; every mu in the gap is a root.  Both
+    implementations must land on a root (tests/test_main.py:237-244)."""
+    rng = np.random.RandomState(5)
+    bands = rng.randn(24, 24, 2)
+    w = np.full((24, 24), 1.0 / 576)
+    for mu in (gpu.find_mu_bisection(bands, w, 1.0, 1e-5, 0),
+               oracle.find_mu_bisection(bands, w, 1.0, 1e-5, 0)):
+        occ = 0.5 * (1.0 - np.tanh((bands - mu) / (2 * (1e-5 + 1e-12))))
+        assert abs(np.sum(w[..., None] * occ) - 1.0) < 1e-4
+    tiny = np.array([[[0.0, 1.0, 2.0]]])
+    mu = gpu.find_mu_bisection(tiny, np.ones((1, 1)), 2.0, 1e-5, 0)
+    assert 1.0 < mu < 2.0
 
 
 def test_mu_newton_matches_oracle(gpu, oracle):
@@ -289,7 +312,10 @@ def test_solve_scf_matches_oracle(gpu, oracle, accel):
     assert g[4] == r[4] and g[5] == r[5]
     assert abs(g[2] - r[2]) < 1e-10
     assert abs(g[3] - r[3]) < 1e-10
-    assert relerr(g[0], r[0]) < 1e-9
+    # ODA here ends stalled (lambda = 0 once c1 >= 0 with c2 <= 1e-12,
+    # solver_scf.hpp:381-388); which steps stall depends on the sign of c1 at
+    # rounding level, so the parked density agrees to 1e-7, energy to 1e-10.
+    assert relerr(g[0], r[0]) < (1e-7 if accel == "oda" else 1e-9)
     np.testing.assert_allclose(g[6], r[6], atol=1e-10)
 
 
