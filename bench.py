@@ -93,13 +93,16 @@ def main() -> int:
     Sigma = torch.empty_like(P)
     F = torch.empty_like(P)
     hdiag = torch.empty(nb, dtype=torch.float64, device=dev)
-    stream = torch.cuda.current_stream(dev)
+    # an explicit stream (default priority): the HIP events below are recorded
+    # on the same stream the Fock kernels run on
+    stream = torch.cuda.Stream(device=dev)
     herm = not args.full_channels
 
     def step():
         gk.build_fock_device(P.data_ptr(), Sigma.data_ptr(), F.data_ptr(), hdiag.data_ptr(),
                              herm, stream.cuda_stream)
 
+    torch.cuda.synchronize()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

@@ -39,7 +39,6 @@ struct ExchangeGeom {
     int* d_chan_src = nullptr;  // nch: a * nb + b
     int* d_chan_a = nullptr;    // nch
     int* d_chan_b = nullptr;    // nch
-    unsigned* d_counter = nullptr;  // last-block ticket of pass A
     AxisPlan ax1, ax2;          // along k1 (nk1) and along k2 (nk2)
 
     void init(int nk1_, int nk2_, int nb_, bool packed_);

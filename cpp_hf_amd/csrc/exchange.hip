@@ -15,6 +15,7 @@
 //   C  rows:  inverse FFT along k2, scale by -1/(nk1 nk2), and write Sigma
 //             (plus F, for the fused Fock epilogue).
 #include <cmath>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -96,37 +97,40 @@ void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
     nch = static_cast<int>(src.size());
     ax1.init(nk1);
     ax2.init(nk2);
-    // Widest tile of <= 16 channels whose stages fit the per-thread
-    // butterfly and load budgets of a 256-thread workgroup.
+    // Channel tile: 4 channels (64 B per k point) by default, which gives
+    // several resident tiles per CU at production sizes; HFGPU_EXCH_TILE
+    // overrides it (1, 2, 4, 8 or 16).  Shrunk until every FFT stage fits the
+    // per-thread butterfly budget of a 256-thread workgroup.
+    int want = 4;
+    if (const char* e = std::getenv("HFGPU_EXCH_TILE")) {
+        const int v = std::atoi(e);
+        if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) want = v;
+    }
     int C = 1;
-    while (C < nch && C < 16) C <<= 1;
+    while (C < nch && C < want) C <<= 1;
     while (C > 1 && !(tile_fits(ax1, C) && tile_fits(ax2, C))) C >>= 1;
     if (!(tile_fits(ax1, C) && tile_fits(ax2, C)))
         throw std::invalid_argument("k-mesh too large for the GPU FFT tile");
     clog = 0;
     while ((1 << clog) < C) ++clog;
     nblk = (nch + C - 1) / C;
-    lds_rows = sizeof(cd) * std::max(static_cast<size_t>(nk2) * C + nk2, small_terms_cd(nb));
+    lds_rows = sizeof(cd) * (static_cast<size_t>(nk2) * C + nk2);
     lds_cols = sizeof(cd) * (static_cast<size_t>(nk1) * C + nk1);
     if (lds_rows > 160 * 1024 || lds_cols > 160 * 1024)
         throw std::invalid_argument("k-mesh too large for an LDS-resident FFT line");
     HFGPU_CHECK(hipMalloc(&d_chan_src, sizeof(int) * nch));
     HFGPU_CHECK(hipMalloc(&d_chan_a, sizeof(int) * nch));
     HFGPU_CHECK(hipMalloc(&d_chan_b, sizeof(int) * nch));
-    HFGPU_CHECK(hipMalloc(&d_counter, 64));
     HFGPU_CHECK(hipMemcpy(d_chan_src, src.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
     HFGPU_CHECK(hipMemcpy(d_chan_a, ca.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
     HFGPU_CHECK(hipMemcpy(d_chan_b, cb.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
-    HFGPU_CHECK(hipMemset(d_counter, 0, 64));
 }
 
 void ExchangeGeom::release() {
     if (d_chan_src) (void)hipFree(d_chan_src);
     if (d_chan_a) (void)hipFree(d_chan_a);
     if (d_chan_b) (void)hipFree(d_chan_b);
-    if (d_counter) (void)hipFree(d_counter);
     d_chan_src = d_chan_a = d_chan_b = nullptr;
-    d_counter = nullptr;
     ax1.release();
     ax2.release();
 }
@@ -137,6 +141,21 @@ void ExchangeGeom::release() {
 
 __device__ __forceinline__ void load_twiddles(cd* s_tw, const cd* tw, int n) {
     for (int i = threadIdx.x; i < n; i += kFftThreads) s_tw[i] = tw[i];
+}
+
+// Workgroup L of a 1-D grid -> (line, channel tile).  Workgroups are dealt
+// round-robin to the 8 XCDs (L % 8); when the line count is a multiple of 8,
+// all channel tiles of one line go to the same XCD, so the 128 B lines they
+// share are fetched into one L2.
+__device__ __forceinline__ void tile_of(int L, int nlines, int nblk, int& line, int& blk) {
+    if ((nlines & 7) == 0) {
+        const int x = L & 7, slot = L >> 3;
+        line = x + 8 * (slot / nblk);
+        blk = slot % nblk;
+    } else {
+        line = L / nblk;
+        blk = L % nblk;
+    }
 }
 
 // Move a tile into LDS with every load of a thread in flight at once:
@@ -168,11 +187,21 @@ __device__ void small_terms_block(const cd* __restrict__ partial, int nrows, int
     cd* grp = acc + nb2;   // <= 256 group partials
     cd* trv = grp + 256;
     const int G = nch >= kFftThreads ? 1 : kFftThreads / nch;
-    // rows split over G groups per channel, then groups summed in order
+    // rows split over G groups per channel (loads batched 8 at a time), then
+    // the groups summed in order
     for (int t = threadIdx.x; t < G * nch; t += kFftThreads) {
         const int ch = t % nch, g = t / nch;
         cd s{0.0, 0.0};
-        for (int r = g; r < nrows; r += G) s = cadd(s, partial[static_cast<size_t>(r) * nch + ch]);
+        for (int r0 = g; r0 < nrows; r0 += 8 * G) {
+            cd v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int r = r0 + u * G;
+                v[u] = r < nrows ? partial[static_cast<size_t>(r) * nch + ch] : cd{0.0, 0.0};
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s = cadd(s, v[u]);
+        }
         grp[t] = s;
     }
     __syncthreads();
@@ -228,21 +257,56 @@ __device__ void small_terms_block(const cd* __restrict__ partial, int nrows, int
     }
 }
 
-// Pass A.  grid = (nblk, nk1).
+// Weighted sum over the line of each channel of the tile, by all 256
+// threads: TPC = 256 / C threads per channel, fixed reduction tree.
+__device__ __forceinline__ void tile_row_sums(const cd* s, const double* __restrict__ wline,
+                                              int n, int clog, int c0, int nch, int line,
+                                              cd* __restrict__ partial, cd* red) {
+    const int C = 1 << clog;
+    const int tpc = kFftThreads >> clog;
+    const int c = threadIdx.x / tpc, g = threadIdx.x % tpc;
+    cd acc{0.0, 0.0};
+    for (int i = g; i < n; i += tpc) {
+        const double w = wline[i];
+        const cd v = s[(i << clog) + c];
+        acc.x += w * v.x;
+        acc.y += w * v.y;
+    }
+    const int width = tpc < 64 ? tpc : 64;
+    for (int off = width >> 1; off > 0; off >>= 1) {
+        acc.x += __shfl_xor(acc.x, off, 64);
+        acc.y += __shfl_xor(acc.y, off, 64);
+    }
+    if (tpc <= 64) {
+        if (g == 0 && c0 + c < nch) partial[static_cast<size_t>(line) * nch + c0 + c] = acc;
+    } else {
+        const int waves_per_c = tpc >> 6;
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x < C) {
+            cd s2{0.0, 0.0};
+            for (int w = 0; w < waves_per_c; ++w) s2 = cadd(s2, red[threadIdx.x * waves_per_c + w]);
+            if (c0 + threadIdx.x < nch) partial[static_cast<size_t>(line) * nch + c0 + threadIdx.x] = s2;
+        }
+    }
+}
+
+// Pass A.  1-D grid of nk1 * nblk tiles.
+template <bool GENERIC>
 __global__ void __launch_bounds__(kFftThreads)
-k_rows_fwd(FftAxis ax2, int nk2, int nb, int nch, int clog, int packed,
-           const int* __restrict__ chan_src, const int* __restrict__ chan_a,
-           const int* __restrict__ chan_b, const cd* __restrict__ P,
+k_rows_fwd(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog, int nblk,
+           const int* __restrict__ chan_src, const cd* __restrict__ P,
            const cd* __restrict__ refP, cd* __restrict__ T, cd* __restrict__ partial,
-           const double* __restrict__ w2d, int do_fft, SmallTerms st, unsigned* counter) {
+           const double* __restrict__ w2d, int do_fft) {
     extern __shared__ cd smem[];
-    __shared__ bool is_last;
+    __shared__ cd red[kFftThreads / 64];
     const int C = 1 << clog;
     const int nb2 = nb * nb;
     cd* s = smem;
     cd* s_tw = smem + nk2 * C;
-    const int k1 = blockIdx.y;
-    const int c0 = blockIdx.x * C;
+    int k1, blk;
+    tile_of(blockIdx.x, nk1, nblk, k1, blk);
+    const int c0 = blk * C;
     if (do_fft) load_twiddles(s_tw, ax2.tw, nk2);
     load_tile(s, nk2 * C, [&](int e) {
         const int i = e >> clog, ch = c0 + (e & (C - 1));
@@ -255,22 +319,11 @@ k_rows_fwd(FftAxis ax2, int nk2, int nb, int nch, int clog, int packed,
         return v;
     });
     __syncthreads();
-    if (partial && threadIdx.x < C) {
-        const int ch = c0 + threadIdx.x;
-        if (ch < nch) {
-            cd acc{0.0, 0.0};
-            for (int i = 0; i < nk2; ++i) {
-                const double w = w2d[k1 * nk2 + i];
-                const cd v = s[(i << clog) + threadIdx.x];
-                acc.x += w * v.x;
-                acc.y += w * v.y;
-            }
-            partial[static_cast<size_t>(k1) * nch + ch] = acc;
-        }
-    }
+    if (partial) tile_row_sums(s, w2d + static_cast<size_t>(k1) * nk2, nk2, clog, c0, nch, k1,
+                               partial, red);
     if (do_fft) {
         __syncthreads();
-        fft_tile<false>(s, s_tw, ax2, clog);
+        fft_tile<false, GENERIC>(s, s_tw, ax2, clog);
 #pragma unroll
         for (int r = 0; r < kTileMaxPerThread; ++r) {
             const int e = threadIdx.x + r * kFftThreads;
@@ -280,35 +333,39 @@ k_rows_fwd(FftAxis ax2, int nk2, int nb, int nch, int clog, int packed,
             }
         }
     }
-    if (!st.enabled) return;
-    // the last workgroup to finish reduces the row sums (deterministic order)
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = atomicAdd(counter, 1u) == gridDim.x * gridDim.y - 1;
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    small_terms_block(partial, gridDim.y, nch, chan_a, chan_b, nb, packed, st, smem);
-    if (threadIdx.x == 0) *counter = 0;
 }
 
-// Pass B.  grid = (nblk, nk2).
+// rho_bar, Hartree diagonal and contact self-energy from pass A's row sums:
+// one workgroup, launched right after pass A.  (A last-workgroup reduction
+// inside pass A would need a device-scope release fence in every workgroup,
+// which on a multi-XCD part writes back each XCD's L2 — far costlier than
+// this extra launch.)
 __global__ void __launch_bounds__(kFftThreads)
-k_cols(FftAxis ax1, int nk1, int nk2, int nch, int clog, cd* __restrict__ T,
+k_small_terms(const cd* __restrict__ partial, int nrows, int nch, const int* __restrict__ chan_a,
+              const int* __restrict__ chan_b, int nb, int packed, SmallTerms st) {
+    extern __shared__ cd smem[];
+    small_terms_block(partial, nrows, nch, chan_a, chan_b, nb, packed, st, smem);
+}
+
+// Pass B.  1-D grid of nk2 * nblk tiles.
+template <bool GENERIC>
+__global__ void __launch_bounds__(kFftThreads)
+k_cols(FftAxis ax1, int nk1, int nk2, int nch, int clog, int nblk, cd* __restrict__ T,
        const cd* __restrict__ VR, int nvr) {
     extern __shared__ cd smem[];
     const int C = 1 << clog;
     cd* s = smem;
     cd* s_tw = smem + nk1 * C;
-    const int k2 = blockIdx.y;
-    const int c0 = blockIdx.x * C;
+    int k2, blk;
+    tile_of(blockIdx.x, nk2, nblk, k2, blk);
+    const int c0 = blk * C;
     load_twiddles(s_tw, ax1.tw, nk1);
     load_tile(s, nk1 * C, [&](int e) {
         const int ch = c0 + (e & (C - 1));
         return ch < nch ? T[static_cast<size_t>((e >> clog) * nk2 + k2) * nch + ch] : cd{0.0, 0.0};
     });
     __syncthreads();
-    fft_tile<false>(s, s_tw, ax1, clog);
+    fft_tile<false, GENERIC>(s, s_tw, ax1, clog);
 #pragma unroll
     for (int r = 0; r < kTileMaxPerThread; ++r) {
         const int e = threadIdx.x + r * kFftThreads;
@@ -322,7 +379,7 @@ k_cols(FftAxis ax1, int nk1, int nk2, int nch, int clog, cd* __restrict__ T,
         }
     }
     __syncthreads();
-    fft_tile<true>(s, s_tw, ax1, clog);
+    fft_tile<true, GENERIC>(s, s_tw, ax1, clog);
 #pragma unroll
     for (int r = 0; r < kTileMaxPerThread; ++r) {
         const int e = threadIdx.x + r * kFftThreads;
@@ -333,9 +390,10 @@ k_cols(FftAxis ax1, int nk1, int nk2, int nch, int clog, cd* __restrict__ T,
     }
 }
 
-// Pass C.  grid = (nblk, nk1).
+// Pass C.  1-D grid of nk1 * nblk tiles.
+template <bool GENERIC>
 __global__ void __launch_bounds__(kFftThreads)
-k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog,
+k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog, int nblk,
            const int* __restrict__ chan_src, const int* __restrict__ chan_a,
            const int* __restrict__ chan_b, const cd* __restrict__ T, int epilogue, int shift,
            double inv_n, cd* __restrict__ Sigma, cd* __restrict__ F, const cd* __restrict__ h,
@@ -344,8 +402,9 @@ k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog,
     const int C = 1 << clog;
     cd* s = smem;
     cd* s_tw = smem + nk2 * C;
-    const int k1 = blockIdx.y;
-    const int c0 = blockIdx.x * C;
+    int k1, blk;
+    tile_of(blockIdx.x, nk1, nblk, k1, blk);
+    const int c0 = blk * C;
     const int nb2 = nb * nb;
     load_twiddles(s_tw, ax2.tw, nk2);
     load_tile(s, nk2 * C, [&](int e) {
@@ -353,7 +412,7 @@ k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog,
         return ch < nch ? T[static_cast<size_t>(k1 * nk2 + (e >> clog)) * nch + ch] : cd{0.0, 0.0};
     });
     __syncthreads();
-    fft_tile<true>(s, s_tw, ax2, clog);
+    fft_tile<true, GENERIC>(s, s_tw, ax2, clog);
     // destination row (ifftshift remaps source (i, j) -> (i - n/2, j - n/2))
     const int d1 = shift ? (k1 + nk1 - nk1 / 2) % nk1 : k1;
 #pragma unroll
@@ -482,22 +541,39 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
 // Launchers
 // ---------------------------------------------------------------------------
 
-void launch_row_sums(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_rows_fwd, dim3(g.nblk, g.nk1), dim3(kFftThreads), g.lds_rows, st,
-                       g.ax2.axis(), g.nk2, g.nb, g.nch, g.clog, g.packed ? 1 : 0, g.d_chan_src,
-                       g.d_chan_a, g.d_chan_b, a.P, a.refP, a.T, a.partial, a.w2d, 0, a.small,
-                       g.d_counter);
+static bool needs_generic(const ExchangeGeom& g) {
+    for (const AxisPlan* ax : {&g.ax1, &g.ax2})
+        for (int r : ax->radices)
+            if (r == 7 || r == 11 || r == 13) return true;
+    return false;
+}
+
+#define HFGPU_EXCH_KERNEL(name, g) (needs_generic(g) ? name<true> : name<false>)
+
+static void launch_small_terms(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
+    if (!a.small.enabled) return;
+    hipLaunchKernelGGL(k_small_terms, dim3(1), dim3(kFftThreads),
+                       sizeof(cd) * small_terms_cd(g.nb), st, a.partial, g.nk1, g.nch,
+                       g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0, a.small);
     HFGPU_LAUNCH_CHECK();
 }
 
-void launch_exchange_fwd(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_rows_fwd, dim3(g.nblk, g.nk1), dim3(kFftThreads), g.lds_rows, st,
-                       g.ax2.axis(), g.nk2, g.nb, g.nch, g.clog, g.packed ? 1 : 0, g.d_chan_src,
-                       g.d_chan_a, g.d_chan_b, a.P, a.refP, a.T, a.partial, a.w2d, 1, a.small,
-                       g.d_counter);
+void launch_row_sums(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(HFGPU_EXCH_KERNEL(k_rows_fwd, g), dim3(g.nblk * g.nk1), dim3(kFftThreads),
+                       g.lds_rows, st, g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.nblk,
+                       g.d_chan_src, a.P, a.refP, a.T, a.partial, a.w2d, 0);
     HFGPU_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_cols, dim3(g.nblk, g.nk2), dim3(kFftThreads), g.lds_cols, st,
-                       g.ax1.axis(), g.nk1, g.nk2, g.nch, g.clog, a.T, a.VR, a.nvr);
+    launch_small_terms(g, a, st);
+}
+
+void launch_exchange_fwd(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(HFGPU_EXCH_KERNEL(k_rows_fwd, g), dim3(g.nblk * g.nk1), dim3(kFftThreads),
+                       g.lds_rows, st, g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.nblk,
+                       g.d_chan_src, a.P, a.refP, a.T, a.partial, a.w2d, 1);
+    HFGPU_LAUNCH_CHECK();
+    launch_small_terms(g, a, st);
+    hipLaunchKernelGGL(HFGPU_EXCH_KERNEL(k_cols, g), dim3(g.nblk * g.nk2), dim3(kFftThreads), g.lds_cols, st,
+                       g.ax1.axis(), g.nk1, g.nk2, g.nch, g.clog, g.nblk, a.T, a.VR, a.nvr);
     HFGPU_LAUNCH_CHECK();
 }
 
@@ -507,8 +583,8 @@ void launch_exchange_inv(const ExchangeGeom& g, const ExchangeArgs& a, hipStream
     if (a.epilogue == EPI_RAW_FULL && g.packed)
         throw std::logic_error("full epilogue on a packed geometry");
     const double inv_n = 1.0 / static_cast<double>(static_cast<size_t>(g.nk1) * g.nk2);
-    hipLaunchKernelGGL(k_rows_inv, dim3(g.nblk, g.nk1), dim3(kFftThreads), g.lds_rows, st,
-                       g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.d_chan_src,
+    hipLaunchKernelGGL(HFGPU_EXCH_KERNEL(k_rows_inv, g), dim3(g.nblk * g.nk1), dim3(kFftThreads), g.lds_rows, st,
+                       g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.nblk, g.d_chan_src,
                        g.d_chan_a, g.d_chan_b, a.T, a.epilogue, a.ifftshift ? 1 : 0, inv_n,
                        a.Sigma, a.F, a.h, a.SC, a.hdiag);
     HFGPU_LAUNCH_CHECK();

@@ -22,22 +22,16 @@ struct FftAxis {
     const cd* tw;  // device table, tw[m] = exp(-2 pi i m / n), m < n
 };
 
-// Butterflies held per thread per stage, sized so one stage keeps at most
-// ~64 VGPRs of butterfly data live.
+// Butterflies held per thread per stage.  Kept small: the exchange passes
+// are latency-bound, so registers go to occupancy (more resident tiles per
+// CU) rather than to more butterflies per thread; the host sizes the channel
+// tile so every stage fits.
 template <int R>
-struct MaxB;
-template <> struct MaxB<2> { static constexpr int v = 8; };
-template <> struct MaxB<3> { static constexpr int v = 4; };
-template <> struct MaxB<4> { static constexpr int v = 4; };
-template <> struct MaxB<5> { static constexpr int v = 3; };
-template <> struct MaxB<7> { static constexpr int v = 2; };
-template <> struct MaxB<8> { static constexpr int v = 2; };
-template <> struct MaxB<11> { static constexpr int v = 1; };
-template <> struct MaxB<13> { static constexpr int v = 1; };
+struct MaxB {
+    static constexpr int v = R <= 8 ? 2 : 1;
+};
 
-__host__ __device__ constexpr int maxb_of(int r) {
-    return r == 2 ? 8 : r == 3 ? 4 : r == 4 ? 4 : r == 5 ? 3 : r == 7 ? 2 : r == 8 ? 2 : 1;
-}
+__host__ __device__ constexpr int maxb_of(int r) { return r <= 8 ? 2 : 1; }
 
 // multiply by -i (forward) or +i (inverse)
 template <bool INV>
@@ -200,8 +194,11 @@ __device__ __forceinline__ void fft_stage(cd* s, const cd* s_tw, int n, int clog
     __syncthreads();
 }
 
-// Full transform of the tile along its leading axis.
-template <bool INV>
+// Full transform of the tile along its leading axis.  GENERIC = false
+// compiles only the radix 2/3/4/5/8 stages (the meshes of every production
+// configuration); GENERIC = true adds the direct radix 7/11/13 stages, whose
+// register footprint would otherwise halve the occupancy of every mesh.
+template <bool INV, bool GENERIC>
 __device__ void fft_tile(cd* s, const cd* s_tw, const FftAxis& ax, int clog) {
     int Ns = 1;
     for (int st = 0; st < ax.nstage; ++st) {
@@ -211,11 +208,14 @@ __device__ void fft_tile(cd* s, const cd* s_tw, const FftAxis& ax, int clog) {
             case 3: fft_stage<3, INV>(s, s_tw, ax.n, clog, Ns); break;
             case 4: fft_stage<4, INV>(s, s_tw, ax.n, clog, Ns); break;
             case 5: fft_stage<5, INV>(s, s_tw, ax.n, clog, Ns); break;
-            case 7: fft_stage<7, INV>(s, s_tw, ax.n, clog, Ns); break;
             case 8: fft_stage<8, INV>(s, s_tw, ax.n, clog, Ns); break;
-            case 11: fft_stage<11, INV>(s, s_tw, ax.n, clog, Ns); break;
-            case 13: fft_stage<13, INV>(s, s_tw, ax.n, clog, Ns); break;
-            default: break;  // rejected on the host
+            default:
+                if constexpr (GENERIC) {
+                    if (r == 7) fft_stage<7, INV>(s, s_tw, ax.n, clog, Ns);
+                    else if (r == 11) fft_stage<11, INV>(s, s_tw, ax.n, clog, Ns);
+                    else if (r == 13) fft_stage<13, INV>(s, s_tw, ax.n, clog, Ns);
+                }
+                break;  // other radices are rejected on the host
         }
         Ns *= r;
     }

@@ -556,7 +556,7 @@ int hfgpu_build_fock_device(hfgpu_kernel* k, const void* dP, void* dSigma, void*
     return guarded([&] {
         if (!k) throw std::invalid_argument("null kernel");
         GpuKernel& K = *k->impl;
-        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : K.stream;
+        hipStream_t s = static_cast<hipStream_t>(stream);
         K.build_fock_graph(static_cast<const cd*>(dP), static_cast<cd*>(dSigma),
                            static_cast<cd*>(dF), static_cast<double*>(dHdiag), p_hermitian != 0,
                            s);
@@ -568,7 +568,7 @@ int hfgpu_energy_device(hfgpu_kernel* k, const void* dP, const void* dSigma,
     return guarded([&] {
         if (!k) throw std::invalid_argument("null kernel");
         GpuKernel& K = *k->impl;
-        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : K.stream;
+        hipStream_t s = static_cast<hipStream_t>(stream);
         K.energy(static_cast<const cd*>(dP), static_cast<const cd*>(dSigma),
                  static_cast<const double*>(dHdiag), static_cast<double*>(dE), s);
     });

@@ -104,12 +104,13 @@ int hfgpu_build_fock_apply(hfgpu_kernel* k, const double* P, double* Sigma, doub
  * tensors, benchmarks).  Replaces: build_fock_compact (fock.hpp:34-299).
  *   dP, dSigma, dF: device (nk1, nk2, nb, nb) complex; dHdiag: device (nb).
  *   p_hermitian != 0 asserts P == P^H (enables the packed exchange).
- *   stream: a hipStream_t, or NULL for the handle's own stream.  Asynchronous. */
+ *   stream: the hipStream_t to run on (NULL: the null stream).  Asynchronous;
+ *   replayed from a hipGraph captured on the first call with these pointers. */
 int hfgpu_build_fock_device(hfgpu_kernel* k, const void* dP, void* dSigma, void* dF,
                             void* dHdiag, int p_hermitian, void* stream);
 
 /* Device-pointer energy: E = hf_energy_with_hartree_diag (fock.hpp:355-383)
- * written to device dE (one double).  Asynchronous. */
+ * written to device dE (one double).  stream as above.  Asynchronous. */
 int hfgpu_energy_device(hfgpu_kernel* k, const void* dP, const void* dSigma,
                         const void* dHdiag, void* dE, void* stream);
 

@@ -44,4 +44,9 @@ def test_graph_replays_match_eager(gpu, which):
             # full nb^2 channels versus the eager (packed) path: same values
             # to rounding
             assert np.max(np.abs(F_g - F_e)) <= 1e-13 * np.max(np.abs(F_e))
-        np.testing.assert_array_equal(dH.download(), np.real(np.diagonal(Hh_e[0, 0])))
+        H_e = np.real(np.diagonal(Hh_e[0, 0]))
+        if hermitian:
+            np.testing.assert_array_equal(dH.download(), H_e)
+        else:
+            # the full-channel row sums group the density differently
+            np.testing.assert_allclose(dH.download(), H_e, rtol=0, atol=1e-14 * np.max(np.abs(H_e)))
