@@ -25,36 +25,51 @@ struct AxisPlan {
 };
 
 // Channel layout of the exchange transform for one (nk1, nk2, nb) geometry.
-//   packed: channels are the upper triangle (a <= b) of the nb x nb block,
-//           valid when the transformed density is Hermitian and the
-//           interaction is symmetric in (a, b).
+//   packed: channels are the upper triangle (a <= b) of the nb x nb block in
+//           row-major order, valid when the transformed density is Hermitian
+//           and the interaction is symmetric in (a, b).
 //   full:   all nb*nb channels in row-major order.
+// Channels are processed in tiles of C = 2^clog; the last tile is padded
+// with zero channels.
+//
+// Workspace layout (tile-major): T[blk][k1][k2][c], so the row passes move
+// one contiguous nk2 * C block per workgroup and the column pass reads C
+// contiguous channels per k point.  The per-geometry constant operands use
+// the layout of the pass that reads them:
+//   VR tile-major by column: VRb[blk][k2][k1][c]   (pass B)
+//   herm(h) tile-major by row: hhb[blk][k1][k2][c] (pass C, packed Fock)
 struct ExchangeGeom {
     int nk1 = 0, nk2 = 0, nb = 0;
     int nch = 0;
     bool packed = false;
-    int clog = 0;   // log2 of channels per workgroup tile
-    int nblk = 0;   // channel tiles
+    int clog = 0;     // log2 of channels per workgroup tile
+    int nblk = 0;     // channel tiles
+    int slots_row = 0, slots_col = 0;  // tile elements per thread (power of 2)
+    bool generic = false;              // a radix 7, 11 or 13 stage is present
     size_t lds_rows = 0, lds_cols = 0;
-    int* d_chan_src = nullptr;  // nch: a * nb + b
-    int* d_chan_a = nullptr;    // nch
-    int* d_chan_b = nullptr;    // nch
-    AxisPlan ax1, ax2;          // along k1 (nk1) and along k2 (nk2)
+    int* d_chan_a = nullptr;  // nch
+    int* d_chan_b = nullptr;  // nch
+    AxisPlan ax1, ax2;        // along k1 (nk1) and along k2 (nk2)
 
     void init(int nk1_, int nk2_, int nb_, bool packed_);
     void release();
+    int tile() const { return 1 << clog; }
+    // complex elements of the workspace and of the tile-major operands
+    size_t tile_elems() const {
+        return static_cast<size_t>(nblk) * tile() * nk1 * nk2;
+    }
 };
 
 // Epilogue of the inverse row pass.
 enum ExchEpilogue : int {
     EPI_RAW_FULL = 0,    // Sigma_x for all channels
     EPI_RAW_PACKED = 1,  // Sigma_x upper triangle + conj mirror
-    EPI_FOCK_PACKED = 2  // Sigma = Sigma_x + SC, F = herm(h + Sigma + diag(H))
+    EPI_FOCK_PACKED = 2  // Sigma = Sigma_x + SC, F = herm(h) + herm(Sigma) + diag(H)
 };
 
 // k-independent terms derived from rho_bar = sum_k w_k (P_k - refP_k):
 // the Hartree diagonal HH n (fock.hpp:222-241) and the contact self-energy
-// (fock.hpp:244-279).  Computed by the last workgroup of pass A.
+// (fock.hpp:244-279).  Computed by one workgroup after pass A.
 struct SmallTerms {
     int enabled = 0;
     int include_hartree = 0;
@@ -69,31 +84,39 @@ struct SmallTerms {
 };
 
 struct ExchangeArgs {
-    const cd* P = nullptr;     // (nk, nb, nb)
-    const cd* refP = nullptr;  // nullable
-    const cd* VR = nullptr;    // (nk, nvr) in the geometry's channel order, nvr = 1 or nch
-    int nvr = 1;
-    cd* T = nullptr;           // workspace (nk, nch)
-    cd* partial = nullptr;     // (nk1, nch) weighted row sums of P - refP, nullable
+    const cd* P = nullptr;      // (nk, nb, nb)
+    const cd* refP = nullptr;   // (nk, nb, nb), nullable
+    const cd* refP0 = nullptr;  // (nb, nb) k-independent reference density, nullable
+    const double* wrow = nullptr;  // (nk1) row sums of w2d, needed with refP0
+    const cd* VR = nullptr;     // VRb (tile-major) or scalar (nk) when vr_scalar
+    bool vr_scalar = false;
+    cd* T = nullptr;            // workspace, tile_elems()
+    cd* partial = nullptr;      // (nk1, nch) weighted row sums of P - refP, nullable
     const double* w2d = nullptr;
     SmallTerms small;
     int epilogue = EPI_RAW_FULL;
     bool ifftshift = false;
-    cd* Sigma = nullptr;       // output
-    cd* F = nullptr;           // EPI_FOCK_PACKED
-    const cd* h = nullptr;     // EPI_FOCK_PACKED
-    const cd* SC = nullptr;    // (nb, nb) contact self-energy, nullable
+    cd* Sigma = nullptr;        // output
+    cd* F = nullptr;            // EPI_FOCK_PACKED
+    const cd* hh = nullptr;     // EPI_FOCK_PACKED: herm(h), tile-major by row
+    const cd* SC = nullptr;     // (nb, nb) contact self-energy, nullable
     const double* hdiag = nullptr;  // (nb) Hartree diagonal, nullable
 };
 
 // Pass A without the transform: weighted row sums (+ small terms) only, for
 // kernels with exchange off but Hartree/contact terms on.
 void launch_row_sums(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st);
-// Passes A (rows fwd, + row sums and small terms) and B (cols fwd, x VR,
-// cols inv).
+// Passes A (rows fwd, + row sums) and B (cols fwd, x VR, cols inv; + the
+// small terms).
 void launch_exchange_fwd(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st);
 // Pass C (rows inv + epilogue).
 void launch_exchange_inv(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st);
+
+// Tile-major operands of a geometry:
+//   VRb from a (nk, nb*nb) interaction (channel (a, b) <- VR[q][a * nb + b]);
+//   hhb = herm(h) over the geometry's channels.
+void launch_pack_vr(const ExchangeGeom& g, const cd* VR_full, cd* VRb, hipStream_t st);
+void launch_pack_herm_h(const ExchangeGeom& g, const cd* h, cd* hhb, hipStream_t st);
 
 // Full-path assembly: Sigma += SC, F = herm(h + Sigma + diag(hdiag)).
 void launch_fock_assemble(int nk, int nb, const cd* h, cd* Sigma, cd* F, const cd* SC,

@@ -2,18 +2,25 @@
 // Fock assembly that follows it (reference: selfenergy.hpp:91-171,
 // fock.hpp:34-299).
 //
-// Data layout in HBM is the reference's C order, (nk1, nk2, nb, nb) with the
-// band pair innermost.  The 2-D transform runs as three passes over a
-// channel-major workspace T(nk1, nk2, nch):
-//   A  rows:  load P - refP for a tile of C channels of one k1 row
-//             (C * 16 B contiguous per k2), FFT along k2 in LDS, store T;
-//             optionally the weighted row sums feeding Hartree/contact terms.
-//   B  cols:  load one k2 column of T, FFT along k1, multiply by VR, inverse
-//             FFT along k1, store T.  The V(q) product is fused between the
-//             forward and inverse transforms, so the k-space product never
-//             makes its own trip to HBM.
-//   C  rows:  inverse FFT along k2, scale by -1/(nk1 nk2), and write Sigma
-//             (plus F, for the fused Fock epilogue).
+// Inputs and outputs keep the reference's C order, (nk1, nk2, nb, nb) with
+// the band pair innermost.  The 2-D transform runs as three passes over a
+// tile-major workspace T[blk][k1][k2][c] (C channels per tile):
+//   A  rows:  gather P - refP for a tile of C channels of one k1 row (each
+//             thread owns one channel, so its (a, b) is fixed), FFT along k2
+//             in LDS, store the tile as one contiguous block; optionally the
+//             weighted row sums feeding the Hartree/contact terms.
+//   B  cols:  load one k2 column of T together with the matching tile-major
+//             VR block, FFT along k1, multiply by VR, inverse FFT along k1,
+//             store T.  The V(q) product is fused between the forward and
+//             inverse transforms, so the k-space product never makes its own
+//             trip to HBM.  One extra workgroup reduces the row sums to the
+//             Hartree diagonal and contact self-energy meanwhile.
+//   C  rows:  inverse FFT along k2 of a contiguous T block, scale by
+//             -1/(nk1 nk2), and write Sigma (plus F for the fused Fock
+//             epilogue, reading herm(h) in the same tile-major order).
+// Every operand a pass reads is requested before its transform starts, so a
+// workgroup pays one memory round trip per pass.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <stdexcept>
@@ -77,8 +84,14 @@ static bool tile_fits(const AxisPlan& ax, int C) {
     return true;
 }
 
-// complex slots the last workgroup of pass A needs for the small terms
+// complex LDS slots of the small-terms workgroup
 static size_t small_terms_cd(int nb) { return 3 * static_cast<size_t>(nb) * nb + 257; }
+
+static int pow2_slots(long elems) {
+    int s = 2;  // smallest instantiated slot count
+    while (static_cast<long>(s) * kFftThreads < elems) s <<= 1;
+    return s;
+}
 
 void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
     release();
@@ -87,21 +100,20 @@ void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
     nb = nb_;
     packed = packed_;
     if (nb > 32) throw std::invalid_argument("the GPU Fock build supports nb <= 32");
-    std::vector<int> src, ca, cb;
+    std::vector<int> ca, cb;
     for (int a = 0; a < nb; ++a)
         for (int b = packed ? a : 0; b < nb; ++b) {
-            src.push_back(a * nb + b);
             ca.push_back(a);
             cb.push_back(b);
         }
-    nch = static_cast<int>(src.size());
+    nch = static_cast<int>(ca.size());
     ax1.init(nk1);
     ax2.init(nk2);
-    // Channel tile: 4 channels (64 B per k point) by default, which gives
-    // several resident tiles per CU at production sizes; HFGPU_EXCH_TILE
-    // overrides it (1, 2, 4, 8 or 16).  Shrunk until every FFT stage fits the
-    // per-thread butterfly budget of a 256-thread workgroup.
-    int want = 4;
+    // Channel tile: 8 channels (128 B per k point, one cache line of the
+    // workspace) by default; HFGPU_EXCH_TILE overrides it (1, 2, 4, 8 or
+    // 16).  Shrunk until every FFT stage fits the per-thread butterfly budget
+    // of a 256-thread workgroup.
+    int want = 8;
     if (const char* e = std::getenv("HFGPU_EXCH_TILE")) {
         const int v = std::atoi(e);
         if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) want = v;
@@ -114,23 +126,26 @@ void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
     clog = 0;
     while ((1 << clog) < C) ++clog;
     nblk = (nch + C - 1) / C;
+    slots_row = pow2_slots(static_cast<long>(nk2) * C);
+    slots_col = pow2_slots(static_cast<long>(nk1) * C);
+    generic = false;
+    for (const AxisPlan* ax : {&ax1, &ax2})
+        for (int r : ax->radices)
+            if (r == 7 || r == 11 || r == 13) generic = true;
     lds_rows = sizeof(cd) * (static_cast<size_t>(nk2) * C + nk2);
     lds_cols = sizeof(cd) * (static_cast<size_t>(nk1) * C + nk1);
     if (lds_rows > 160 * 1024 || lds_cols > 160 * 1024)
         throw std::invalid_argument("k-mesh too large for an LDS-resident FFT line");
-    HFGPU_CHECK(hipMalloc(&d_chan_src, sizeof(int) * nch));
     HFGPU_CHECK(hipMalloc(&d_chan_a, sizeof(int) * nch));
     HFGPU_CHECK(hipMalloc(&d_chan_b, sizeof(int) * nch));
-    HFGPU_CHECK(hipMemcpy(d_chan_src, src.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
     HFGPU_CHECK(hipMemcpy(d_chan_a, ca.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
     HFGPU_CHECK(hipMemcpy(d_chan_b, cb.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
 }
 
 void ExchangeGeom::release() {
-    if (d_chan_src) (void)hipFree(d_chan_src);
     if (d_chan_a) (void)hipFree(d_chan_a);
     if (d_chan_b) (void)hipFree(d_chan_b);
-    d_chan_src = d_chan_a = d_chan_b = nullptr;
+    d_chan_a = d_chan_b = nullptr;
     ax1.release();
     ax2.release();
 }
@@ -139,8 +154,26 @@ void ExchangeGeom::release() {
 // Kernels
 // ---------------------------------------------------------------------------
 
+namespace {
+
 __device__ __forceinline__ void load_twiddles(cd* s_tw, const cd* tw, int n) {
     for (int i = threadIdx.x; i < n; i += kFftThreads) s_tw[i] = tw[i];
+}
+
+// Band pair of channel ch: row-major upper triangle (packed) or row-major.
+__device__ __forceinline__ void chan_ab(int ch, int nb, int packed, int& a, int& b) {
+    if (!packed) {
+        a = ch / nb;
+        b = ch - a * nb;
+        return;
+    }
+    int r = ch;
+    a = 0;
+    while (r >= nb - a) {
+        r -= nb - a;
+        ++a;
+    }
+    b = a + r;
 }
 
 // Workgroup L of a 1-D grid -> (line, channel tile).  Workgroups are dealt
@@ -158,25 +191,8 @@ __device__ __forceinline__ void tile_of(int L, int nlines, int nblk, int& line, 
     }
 }
 
-// Move a tile into LDS with every load of a thread in flight at once:
-// element e of the tile comes from src(e).  total <= 16 * 256.
-template <class Src>
-__device__ __forceinline__ void load_tile(cd* s, int total, Src src) {
-    cd v[kTileMaxPerThread];
-#pragma unroll
-    for (int r = 0; r < kTileMaxPerThread; ++r) {
-        const int e = threadIdx.x + r * kFftThreads;
-        if (e < total) v[r] = src(e);
-    }
-#pragma unroll
-    for (int r = 0; r < kTileMaxPerThread; ++r) {
-        const int e = threadIdx.x + r * kFftThreads;
-        if (e < total) s[e] = v[r];
-    }
-}
-
-// rho_bar from the row sums, then the Hartree diagonal and the contact
-// self-energy; run by one workgroup (the last one of pass A).
+// rho_bar from pass A's row sums, then the Hartree diagonal and the contact
+// self-energy; run by one workgroup.
 __device__ void small_terms_block(const cd* __restrict__ partial, int nrows, int nch,
                                   const int* __restrict__ chan_a, const int* __restrict__ chan_b,
                                   int nb, int packed, const SmallTerms& st, cd* smem) {
@@ -258,88 +274,130 @@ __device__ void small_terms_block(const cd* __restrict__ partial, int nrows, int
 }
 
 // Weighted sum over the line of each channel of the tile, by all 256
-// threads: TPC = 256 / C threads per channel, fixed reduction tree.
-__device__ __forceinline__ void tile_row_sums(const cd* s, const double* __restrict__ wline,
-                                              int n, int clog, int c0, int nch, int line,
-                                              cd* __restrict__ partial, cd* red) {
+// threads: TPC = 256 / C threads per channel, fixed reduction tree.  wv holds
+// the weights this thread needs (prefetched with the tile).  With a
+// k-independent reference density the row sum of P is corrected by
+// refP0_ab * sum_k2 w(k1, k2).
+template <int S>
+__device__ __forceinline__ void tile_row_sums(const cd* s, const double (&wv)[S], int n,
+                                              int clog, int c0, int nch, int line, int nb,
+                                              int packed, const cd* __restrict__ refP0,
+                                              double wrow, cd* __restrict__ partial, cd* red) {
     const int C = 1 << clog;
     const int tpc = kFftThreads >> clog;
     const int c = threadIdx.x / tpc, g = threadIdx.x % tpc;
     cd acc{0.0, 0.0};
-    for (int i = g; i < n; i += tpc) {
-        const double w = wline[i];
-        const cd v = s[(i << clog) + c];
-        acc.x += w * v.x;
-        acc.y += w * v.y;
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int i = g + r * tpc;
+        if (i < n) {
+            const cd v = s[(i << clog) + c];
+            acc.x += wv[r] * v.x;
+            acc.y += wv[r] * v.y;
+        }
     }
     const int width = tpc < 64 ? tpc : 64;
     for (int off = width >> 1; off > 0; off >>= 1) {
         acc.x += __shfl_xor(acc.x, off, 64);
         acc.y += __shfl_xor(acc.y, off, 64);
     }
+    int cw = -1;  // channel this thread writes, if any
     if (tpc <= 64) {
-        if (g == 0 && c0 + c < nch) partial[static_cast<size_t>(line) * nch + c0 + c] = acc;
+        if (g == 0) cw = c;
     } else {
         const int waves_per_c = tpc >> 6;
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
         __syncthreads();
         if (threadIdx.x < C) {
-            cd s2{0.0, 0.0};
-            for (int w = 0; w < waves_per_c; ++w) s2 = cadd(s2, red[threadIdx.x * waves_per_c + w]);
-            if (c0 + threadIdx.x < nch) partial[static_cast<size_t>(line) * nch + c0 + threadIdx.x] = s2;
+            acc = cd{0.0, 0.0};
+            for (int w = 0; w < waves_per_c; ++w) acc = cadd(acc, red[threadIdx.x * waves_per_c + w]);
+            cw = threadIdx.x;
         }
+    }
+    if (cw >= 0 && c0 + cw < nch) {
+        if (refP0) {
+            int a, b;
+            chan_ab(c0 + cw, nb, packed, a, b);
+            const cd r = refP0[a * nb + b];
+            acc.x -= r.x * wrow;
+            acc.y -= r.y * wrow;
+        }
+        partial[static_cast<size_t>(line) * nch + c0 + cw] = acc;
     }
 }
 
-// Pass A.  1-D grid of nk1 * nblk tiles.
-template <bool GENERIC>
+// Pass A.  1-D grid of nk1 * nblk tiles; S tile elements per thread.
+template <bool GENERIC, int S>
 __global__ void __launch_bounds__(kFftThreads)
-k_rows_fwd(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog, int nblk,
-           const int* __restrict__ chan_src, const cd* __restrict__ P,
-           const cd* __restrict__ refP, cd* __restrict__ T, cd* __restrict__ partial,
+k_rows_fwd(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog, int nblk, int packed,
+           const cd* __restrict__ P, const cd* __restrict__ refP, const cd* __restrict__ refP0,
+           const double* __restrict__ wrow, cd* __restrict__ T, cd* __restrict__ partial,
            const double* __restrict__ w2d, int do_fft) {
     extern __shared__ cd smem[];
     __shared__ cd red[kFftThreads / 64];
     const int C = 1 << clog;
-    const int nb2 = nb * nb;
+    const int tot = nk2 << clog;
     cd* s = smem;
-    cd* s_tw = smem + nk2 * C;
+    cd* s_tw = smem + tot;
     int k1, blk;
     tile_of(blockIdx.x, nk1, nblk, k1, blk);
-    const int c0 = blk * C;
+    // thread -> channel is fixed: element e = tid + 256 r has c = tid % C
+    const int c = threadIdx.x & (C - 1);
+    const int ch = blk * C + c;
+    const bool valid = ch < nch;
+    int a = 0, b = 0;
+    if (valid) chan_ab(ch, nb, packed, a, b);
+    const int nb2 = nb * nb;
+    const size_t base = static_cast<size_t>(k1) * nk2 * nb2 + a * nb + b;
     if (do_fft) load_twiddles(s_tw, ax2.tw, nk2);
-    load_tile(s, nk2 * C, [&](int e) {
-        const int i = e >> clog, ch = c0 + (e & (C - 1));
-        cd v{0.0, 0.0};
-        if (ch < nch) {
-            const size_t off = static_cast<size_t>(k1 * nk2 + i) * nb2 + chan_src[ch];
-            v = P[off];
-            if (refP) v = csub(v, refP[off]);
-        }
-        return v;
-    });
-    __syncthreads();
-    if (partial) tile_row_sums(s, w2d + static_cast<size_t>(k1) * nk2, nk2, clog, c0, nch, k1,
-                               partial, red);
-    if (do_fft) {
-        __syncthreads();
-        fft_tile<false, GENERIC>(s, s_tw, ax2, clog);
+    // weights of this thread's row-sum elements, requested with the tile
+    double wv[S];
+    if (partial) {
+        const int tpc = kFftThreads >> clog, g = threadIdx.x % tpc;
+        const double* wline = w2d + static_cast<size_t>(k1) * nk2;
 #pragma unroll
-        for (int r = 0; r < kTileMaxPerThread; ++r) {
-            const int e = threadIdx.x + r * kFftThreads;
-            if (e < nk2 * C) {
-                const int ch = c0 + (e & (C - 1));
-                if (ch < nch) T[static_cast<size_t>(k1 * nk2 + (e >> clog)) * nch + ch] = s[e];
-            }
+        for (int r = 0; r < S; ++r) {
+            const int i = g + r * tpc;
+            wv[r] = i < nk2 ? wline[i] : 0.0;
         }
+    }
+    cd v[S];
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        v[r] = cd{0.0, 0.0};
+        if (valid && e < tot) {
+            const size_t off = base + static_cast<size_t>(e >> clog) * nb2;
+            v[r] = P[off];
+            if (refP) v[r] = csub(v[r], refP[off]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot) s[e] = v[r];
+    }
+    __syncthreads();
+    if (partial)
+        tile_row_sums<S>(s, wv, nk2, clog, blk * C, nch, k1, nb, packed, refP0,
+                         refP0 ? wrow[k1] : 0.0, partial, red);
+    if (!do_fft) return;
+    __syncthreads();
+    fft_tile<false, GENERIC>(s, s_tw, ax2, clog);
+    // a k-independent reference density only reaches q2 = 0: nk2 * refP0
+    if (refP0 && valid && threadIdx.x < C) {
+        const cd r = refP0[a * nb + b];
+        s[c] = cd{s[c].x - nk2 * r.x, s[c].y - nk2 * r.y};
+    }
+    cd* Tt = T + (static_cast<size_t>(blk) * nk1 + k1) * tot;
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot) Tt[e] = s[e];
     }
 }
 
-// rho_bar, Hartree diagonal and contact self-energy from pass A's row sums:
-// one workgroup, launched right after pass A.  (A last-workgroup reduction
-// inside pass A would need a device-scope release fence in every workgroup,
-// which on a multi-XCD part writes back each XCD's L2 — far costlier than
-// this extra launch.)
+// Small terms on their own, for geometries whose pass B cannot host them.
 __global__ void __launch_bounds__(kFftThreads)
 k_small_terms(const cd* __restrict__ partial, int nrows, int nch, const int* __restrict__ chan_a,
               const int* __restrict__ chan_b, int nb, int packed, SmallTerms st) {
@@ -347,115 +405,195 @@ k_small_terms(const cd* __restrict__ partial, int nrows, int nch, const int* __r
     small_terms_block(partial, nrows, nch, chan_a, chan_b, nb, packed, st, smem);
 }
 
-// Pass B.  1-D grid of nk2 * nblk tiles.
-template <bool GENERIC>
+// Pass B.  1-D grid of nk2 * nblk tiles, plus workgroup 0 for the small
+// terms when has_small.
+template <bool GENERIC, int S>
 __global__ void __launch_bounds__(kFftThreads)
 k_cols(FftAxis ax1, int nk1, int nk2, int nch, int clog, int nblk, cd* __restrict__ T,
-       const cd* __restrict__ VR, int nvr) {
+       const cd* __restrict__ VR, int vr_scalar, int has_small, const cd* __restrict__ partial,
+       const int* __restrict__ chan_a, const int* __restrict__ chan_b, int nb, int packed,
+       SmallTerms st) {
     extern __shared__ cd smem[];
+    // the first workgroup reduces pass A's row sums to the Hartree diagonal
+    // and contact self-energy, off the transform's critical path
+    if (has_small && blockIdx.x == 0) {
+        small_terms_block(partial, nk1, nch, chan_a, chan_b, nb, packed, st, smem);
+        return;
+    }
     const int C = 1 << clog;
+    const int tot = nk1 << clog;
     cd* s = smem;
-    cd* s_tw = smem + nk1 * C;
+    cd* s_tw = smem + tot;
     int k2, blk;
-    tile_of(blockIdx.x, nk2, nblk, k2, blk);
-    const int c0 = blk * C;
+    tile_of(static_cast<int>(blockIdx.x) - has_small, nk2, nblk, k2, blk);
+    const int c = threadIdx.x & (C - 1);
+    cd* Tb = T + static_cast<size_t>(blk) * nk1 * nk2 * C;   // tile block of T
+    const cd* Vb = VR + (static_cast<size_t>(blk) * nk2 + k2) * tot;  // contiguous VR tile
     load_twiddles(s_tw, ax1.tw, nk1);
-    load_tile(s, nk1 * C, [&](int e) {
-        const int ch = c0 + (e & (C - 1));
-        return ch < nch ? T[static_cast<size_t>((e >> clog) * nk2 + k2) * nch + ch] : cd{0.0, 0.0};
-    });
+    cd v[S], w[S];
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot) {
+            const int i = e >> clog;
+            v[r] = Tb[(static_cast<size_t>(i) * nk2 + k2) * C + c];
+            w[r] = vr_scalar ? VR[static_cast<size_t>(i) * nk2 + k2] : Vb[e];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot) s[e] = v[r];
+    }
     __syncthreads();
     fft_tile<false, GENERIC>(s, s_tw, ax1, clog);
 #pragma unroll
-    for (int r = 0; r < kTileMaxPerThread; ++r) {
+    for (int r = 0; r < S; ++r) {
         const int e = threadIdx.x + r * kFftThreads;
-        if (e < nk1 * C) {
-            const int ch = c0 + (e & (C - 1));
-            if (ch < nch) {
-                const size_t q = static_cast<size_t>((e >> clog) * nk2 + k2);
-                const cd v = nvr == 1 ? VR[q] : VR[q * nvr + ch];
-                s[e] = cmul(s[e], v);
-            }
-        }
+        if (e < tot) s[e] = cmul(s[e], w[r]);
     }
     __syncthreads();
     fft_tile<true, GENERIC>(s, s_tw, ax1, clog);
 #pragma unroll
-    for (int r = 0; r < kTileMaxPerThread; ++r) {
+    for (int r = 0; r < S; ++r) {
         const int e = threadIdx.x + r * kFftThreads;
-        if (e < nk1 * C) {
-            const int ch = c0 + (e & (C - 1));
-            if (ch < nch) T[static_cast<size_t>((e >> clog) * nk2 + k2) * nch + ch] = s[e];
-        }
+        if (e < tot) Tb[(static_cast<size_t>(e >> clog) * nk2 + k2) * C + c] = s[e];
     }
 }
 
 // Pass C.  1-D grid of nk1 * nblk tiles.
-template <bool GENERIC>
+template <bool GENERIC, int S>
 __global__ void __launch_bounds__(kFftThreads)
-k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog, int nblk,
-           const int* __restrict__ chan_src, const int* __restrict__ chan_a,
-           const int* __restrict__ chan_b, const cd* __restrict__ T, int epilogue, int shift,
-           double inv_n, cd* __restrict__ Sigma, cd* __restrict__ F, const cd* __restrict__ h,
+k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog, int nblk, int packed,
+           const cd* __restrict__ T, int epilogue, int shift, double inv_n,
+           cd* __restrict__ Sigma, cd* __restrict__ F, const cd* __restrict__ hh,
            const cd* __restrict__ SC, const double* __restrict__ hdiag) {
     extern __shared__ cd smem[];
     const int C = 1 << clog;
+    const int tot = nk2 << clog;
     cd* s = smem;
-    cd* s_tw = smem + nk2 * C;
+    cd* s_tw = smem + tot;
     int k1, blk;
     tile_of(blockIdx.x, nk1, nblk, k1, blk);
-    const int c0 = blk * C;
-    const int nb2 = nb * nb;
+    const int c = threadIdx.x & (C - 1);
+    const int ch = blk * C + c;
+    const bool valid = ch < nch;
+    int a = 0, b = 0;
+    if (valid) chan_ab(ch, nb, packed, a, b);
+    const size_t toff = (static_cast<size_t>(blk) * nk1 + k1) * tot;
+    const bool fock = epilogue == EPI_FOCK_PACKED;
     load_twiddles(s_tw, ax2.tw, nk2);
-    load_tile(s, nk2 * C, [&](int e) {
-        const int ch = c0 + (e & (C - 1));
-        return ch < nch ? T[static_cast<size_t>(k1 * nk2 + (e >> clog)) * nch + ch] : cd{0.0, 0.0};
-    });
+    cd v[S], hv[S];
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot) {
+            v[r] = T[toff + e];
+            if (fock) hv[r] = hh[toff + e];
+        }
+    }
+    // per-thread constants of the Fock epilogue
+    cd sc_ab{0.0, 0.0}, sc_ba{0.0, 0.0};
+    double hd = 0.0;
+    if (fock && valid) {
+        if (SC) {
+            sc_ab = SC[a * nb + b];
+            sc_ba = SC[b * nb + a];
+        }
+        if (hdiag && a == b) hd = hdiag[a];
+    }
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot) s[e] = v[r];
+    }
     __syncthreads();
     fft_tile<true, GENERIC>(s, s_tw, ax2, clog);
+    if (!valid) return;
     // destination row (ifftshift remaps source (i, j) -> (i - n/2, j - n/2))
     const int d1 = shift ? (k1 + nk1 - nk1 / 2) % nk1 : k1;
+    const int nb2 = nb * nb;
 #pragma unroll
-    for (int r = 0; r < kTileMaxPerThread; ++r) {
+    for (int r = 0; r < S; ++r) {
         const int e = threadIdx.x + r * kFftThreads;
-        if (e >= nk2 * C) continue;
-        const int i = e >> clog, ch = c0 + (e & (C - 1));
-        if (ch >= nch) continue;
+        if (e >= tot) continue;
+        const int i = e >> clog;
         const cd y = s[e];
         // reference order: ifftn = backward * (1/N), then negate
         const cd sig{-(y.x * inv_n), -(y.y * inv_n)};
         const int d2 = shift ? (i + nk2 - nk2 / 2) % nk2 : i;
         const size_t kb = static_cast<size_t>(d1 * nk2 + d2) * nb2;
+        const size_t iab = kb + a * nb + b, iba = kb + b * nb + a;
         if (epilogue == EPI_RAW_FULL) {
-            Sigma[kb + chan_src[ch]] = sig;
+            Sigma[iab] = sig;
+        } else if (epilogue == EPI_RAW_PACKED) {
+            Sigma[iab] = sig;
+            if (a != b) Sigma[iba] = cconj(sig);
         } else {
-            const int a = chan_a[ch], b = chan_b[ch];
-            const size_t iab = kb + a * nb + b, iba = kb + b * nb + a;
-            if (epilogue == EPI_RAW_PACKED) {
-                Sigma[iab] = sig;
-                if (a != b) Sigma[iba] = cconj(sig);
-            } else {
-                // a == b is one element: its mirror is itself, not its conjugate
-                cd s_ab = sig, s_ba = a == b ? sig : cconj(sig);
-                if (SC) {
-                    s_ab = cadd(s_ab, SC[a * nb + b]);
-                    s_ba = cadd(s_ba, SC[b * nb + a]);
-                }
-                cd x_ab = cadd(h[iab], s_ab);
-                cd x_ba = cadd(h[iba], s_ba);
-                if (hdiag && a == b) {
-                    x_ab.x += hdiag[a];
-                    x_ba.x += hdiag[a];
-                }
-                const cd f{0.5 * (x_ab.x + x_ba.x), 0.5 * (x_ab.y - x_ba.y)};
-                Sigma[iab] = s_ab;
-                F[iab] = f;
-                if (a != b) {
-                    Sigma[iba] = s_ba;
-                    F[iba] = cconj(f);
-                }
+            // a == b is one element: its mirror is itself, not its conjugate
+            const cd s_ab = cadd(sig, sc_ab);
+            const cd s_ba = cadd(a == b ? sig : cconj(sig), sc_ba);
+            // F = herm(h) + herm(Sigma) + diag(H)
+            cd f{hv[r].x + 0.5 * (s_ab.x + s_ba.x), hv[r].y + 0.5 * (s_ab.y - s_ba.y)};
+            f.x += hd;
+            Sigma[iab] = s_ab;
+            F[iab] = f;
+            if (a != b) {
+                Sigma[iba] = s_ba;
+                F[iba] = cconj(f);
             }
         }
+    }
+}
+
+// Tile-major operand packing (run once per kernel geometry).
+__global__ void k_pack_vr(int nk1, int nk2, int nb, int nch, int clog, int nblk, int packed,
+                          const cd* __restrict__ VR, cd* __restrict__ VRb) {
+    const int C = 1 << clog;
+    const size_t total = static_cast<size_t>(nblk) * C * nk1 * nk2;
+    for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < total;
+         t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        // t = ((blk * nk2 + k2) * nk1 + k1) * C + c
+        const int c = static_cast<int>(t & (C - 1));
+        size_t r = t >> clog;
+        const int k1 = static_cast<int>(r % nk1);
+        r /= nk1;
+        const int k2 = static_cast<int>(r % nk2);
+        const int blk = static_cast<int>(r / nk2);
+        const int ch = blk * C + c;
+        cd v{0.0, 0.0};
+        if (ch < nch) {
+            int a, b;
+            chan_ab(ch, nb, packed, a, b);
+            v = VR[(static_cast<size_t>(k1) * nk2 + k2) * nb * nb + a * nb + b];
+        }
+        VRb[t] = v;
+    }
+}
+
+__global__ void k_pack_herm_h(int nk1, int nk2, int nb, int nch, int clog, int nblk,
+                              int packed, const cd* __restrict__ h, cd* __restrict__ hhb) {
+    const int C = 1 << clog;
+    const size_t total = static_cast<size_t>(nblk) * C * nk1 * nk2;
+    for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < total;
+         t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        // t = ((blk * nk1 + k1) * nk2 + k2) * C + c
+        const int c = static_cast<int>(t & (C - 1));
+        size_t r = t >> clog;
+        const int k2 = static_cast<int>(r % nk2);
+        r /= nk2;
+        const int k1 = static_cast<int>(r % nk1);
+        const int blk = static_cast<int>(r / nk1);
+        const int ch = blk * C + c;
+        cd v{0.0, 0.0};
+        if (ch < nch) {
+            int a, b;
+            chan_ab(ch, nb, packed, a, b);
+            const size_t kb = (static_cast<size_t>(k1) * nk2 + k2) * nb * nb;
+            const cd x = h[kb + a * nb + b], y = h[kb + b * nb + a];
+            v = cd{0.5 * (x.x + y.x), 0.5 * (x.y - y.y)};
+        }
+        hhb[t] = v;
     }
 }
 
@@ -537,44 +675,60 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
     if (threadIdx.x == 0) *out = s;
 }
 
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 
-static bool needs_generic(const ExchangeGeom& g) {
-    for (const AxisPlan* ax : {&g.ax1, &g.ax2})
-        for (int r : ax->radices)
-            if (r == 7 || r == 11 || r == 13) return true;
-    return false;
-}
-
-#define HFGPU_EXCH_KERNEL(name, g) (needs_generic(g) ? name<true> : name<false>)
+// Launch kern<generic, S> with S the geometry's slot count (2, 4, 8 or 16).
+#define HFGPU_EXCH_LAUNCH(kern, gen, slots, grid, lds, st, ...)                          \
+    do {                                                                                   \
+        const dim3 blk_(kFftThreads);                                                      \
+        if (gen) {                                                                         \
+            if ((slots) <= 2) hipLaunchKernelGGL((kern<true, 2>), grid, blk_, lds, st, __VA_ARGS__);       \
+            else if ((slots) <= 4) hipLaunchKernelGGL((kern<true, 4>), grid, blk_, lds, st, __VA_ARGS__);  \
+            else if ((slots) <= 8) hipLaunchKernelGGL((kern<true, 8>), grid, blk_, lds, st, __VA_ARGS__);  \
+            else hipLaunchKernelGGL((kern<true, 16>), grid, blk_, lds, st, __VA_ARGS__);                   \
+        } else {                                                                           \
+            if ((slots) <= 2) hipLaunchKernelGGL((kern<false, 2>), grid, blk_, lds, st, __VA_ARGS__);      \
+            else if ((slots) <= 4) hipLaunchKernelGGL((kern<false, 4>), grid, blk_, lds, st, __VA_ARGS__); \
+            else if ((slots) <= 8) hipLaunchKernelGGL((kern<false, 8>), grid, blk_, lds, st, __VA_ARGS__); \
+            else hipLaunchKernelGGL((kern<false, 16>), grid, blk_, lds, st, __VA_ARGS__);                  \
+        }                                                                                  \
+        HFGPU_LAUNCH_CHECK();                                                              \
+    } while (0)
 
 static void launch_small_terms(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
-    if (!a.small.enabled) return;
     hipLaunchKernelGGL(k_small_terms, dim3(1), dim3(kFftThreads),
                        sizeof(cd) * small_terms_cd(g.nb), st, a.partial, g.nk1, g.nch,
                        g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0, a.small);
     HFGPU_LAUNCH_CHECK();
 }
 
+static void launch_rows_fwd(const ExchangeGeom& g, const ExchangeArgs& a, int do_fft,
+                            hipStream_t st) {
+    if (a.refP0 && !a.wrow) throw std::logic_error("refP0 without row weights");
+    HFGPU_EXCH_LAUNCH(k_rows_fwd, g.generic, g.slots_row, dim3(g.nblk * g.nk1), g.lds_rows, st,
+                      g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.nblk, g.packed ? 1 : 0,
+                      a.P, a.refP, a.refP0, a.wrow, a.T, a.partial, a.w2d, do_fft);
+}
+
 void launch_row_sums(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(HFGPU_EXCH_KERNEL(k_rows_fwd, g), dim3(g.nblk * g.nk1), dim3(kFftThreads),
-                       g.lds_rows, st, g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.nblk,
-                       g.d_chan_src, a.P, a.refP, a.T, a.partial, a.w2d, 0);
-    HFGPU_LAUNCH_CHECK();
-    launch_small_terms(g, a, st);
+    launch_rows_fwd(g, a, 0, st);
+    if (a.small.enabled) launch_small_terms(g, a, st);
 }
 
 void launch_exchange_fwd(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(HFGPU_EXCH_KERNEL(k_rows_fwd, g), dim3(g.nblk * g.nk1), dim3(kFftThreads),
-                       g.lds_rows, st, g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.nblk,
-                       g.d_chan_src, a.P, a.refP, a.T, a.partial, a.w2d, 1);
-    HFGPU_LAUNCH_CHECK();
-    launch_small_terms(g, a, st);
-    hipLaunchKernelGGL(HFGPU_EXCH_KERNEL(k_cols, g), dim3(g.nblk * g.nk2), dim3(kFftThreads), g.lds_cols, st,
-                       g.ax1.axis(), g.nk1, g.nk2, g.nch, g.clog, g.nblk, a.T, a.VR, a.nvr);
-    HFGPU_LAUNCH_CHECK();
+    launch_rows_fwd(g, a, 1, st);
+    // the small terms ride in pass B as one extra workgroup when its LDS fits
+    const bool fuse_small = a.small.enabled && sizeof(cd) * small_terms_cd(g.nb) <= g.lds_cols;
+    if (a.small.enabled && !fuse_small) launch_small_terms(g, a, st);
+    const int hs = fuse_small ? 1 : 0;
+    HFGPU_EXCH_LAUNCH(k_cols, g.generic, g.slots_col, dim3(g.nblk * g.nk2 + hs), g.lds_cols, st,
+                      g.ax1.axis(), g.nk1, g.nk2, g.nch, g.clog, g.nblk, a.T, a.VR,
+                      a.vr_scalar ? 1 : 0, hs, a.partial, g.d_chan_a, g.d_chan_b, g.nb,
+                      g.packed ? 1 : 0, a.small);
 }
 
 void launch_exchange_inv(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
@@ -582,11 +736,28 @@ void launch_exchange_inv(const ExchangeGeom& g, const ExchangeArgs& a, hipStream
         throw std::logic_error("packed epilogue on a full-channel geometry");
     if (a.epilogue == EPI_RAW_FULL && g.packed)
         throw std::logic_error("full epilogue on a packed geometry");
+    if (a.epilogue == EPI_FOCK_PACKED && !a.hh)
+        throw std::logic_error("Fock epilogue without herm(h)");
     const double inv_n = 1.0 / static_cast<double>(static_cast<size_t>(g.nk1) * g.nk2);
-    hipLaunchKernelGGL(HFGPU_EXCH_KERNEL(k_rows_inv, g), dim3(g.nblk * g.nk1), dim3(kFftThreads), g.lds_rows, st,
-                       g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.nblk, g.d_chan_src,
-                       g.d_chan_a, g.d_chan_b, a.T, a.epilogue, a.ifftshift ? 1 : 0, inv_n,
-                       a.Sigma, a.F, a.h, a.SC, a.hdiag);
+    HFGPU_EXCH_LAUNCH(k_rows_inv, g.generic, g.slots_row, dim3(g.nblk * g.nk1), g.lds_rows, st,
+                      g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.nblk, g.packed ? 1 : 0,
+                      a.T, a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh, a.SC,
+                      a.hdiag);
+}
+
+static unsigned pack_blocks(size_t total) {
+    return static_cast<unsigned>(std::min<size_t>(cdiv(total, 256), 8192));
+}
+
+void launch_pack_vr(const ExchangeGeom& g, const cd* VR_full, cd* VRb, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack_vr, dim3(pack_blocks(g.tile_elems())), dim3(256), 0, st, g.nk1,
+                       g.nk2, g.nb, g.nch, g.clog, g.nblk, g.packed ? 1 : 0, VR_full, VRb);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_pack_herm_h(const ExchangeGeom& g, const cd* h, cd* hhb, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack_herm_h, dim3(pack_blocks(g.tile_elems())), dim3(256), 0, st,
+                       g.nk1, g.nk2, g.nb, g.nch, g.clog, g.nblk, g.packed ? 1 : 0, h, hhb);
     HFGPU_LAUNCH_CHECK();
 }
 

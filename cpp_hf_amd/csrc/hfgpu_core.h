@@ -73,9 +73,19 @@ struct GpuKernel {
     hipStream_t stream = nullptr;
 
     DevBuf h, refP, w2d, HH, cg, Oi, Oj;
-    DevBuf VR_full;    // (nk, dv*dv) as given
-    DevBuf VR_packed;  // (nk, nb(nb+1)/2) for dense V, when packable
-    std::unique_ptr<ExchangeGeom> geom_full, geom_packed;
+    DevBuf VR_full;  // (nk, dv*dv) as given
+    // k-independent reference density (refP_k identical for every k): its
+    // transform is a single q = 0 term, so pass A reads P alone
+    bool refP_const = false;
+    DevBuf refP0, wrow;  // (nb, nb) and (nk1) row sums of w2d
+
+    // An exchange geometry with its tile-major constant operands.
+    struct Geom {
+        std::unique_ptr<ExchangeGeom> g;
+        DevBuf VRb;  // VR in pass B order (dense V only)
+        DevBuf hhb;  // herm(h) in pass C order (packed geometry only)
+    };
+    Geom geo_full, geo_packed;
 
     // Fock workspaces
     DevBuf T_ws, partial, SC, hdiag, e_partial, e_out;
@@ -84,7 +94,7 @@ struct GpuKernel {
     ~GpuKernel();
 
     bool packed_for(bool p_hermitian) const { return hcp || (dense_packable && p_hermitian); }
-    ExchangeGeom& geom(bool packed);
+    Geom& geom(bool packed);
 
     // build_fock_compact on device: Sigma, F, hdiag (nb) from P.
     void build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, bool p_hermitian,

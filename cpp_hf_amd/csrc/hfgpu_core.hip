@@ -182,25 +182,24 @@ GpuKernel::GpuKernel(const hf_kernel_desc& d) {
     const size_t nvr = static_cast<size_t>(dv) * dv;
     VR_full.alloc(nk * nvr * cz);
     h2d(VR_full.get(), d.VR, nk * nvr * cz, stream);
-    if (exchange && dv > 1 && nb > 1) {
-        dense_packable = dense_vr_packable(d.VR, d.nk1, d.nk2, d.nb);
-        if (dense_packable) {
-            const size_t np = d.nb * (d.nb + 1) / 2;
-            std::vector<double> vp(2 * nk * np);
-            for (size_t q = 0; q < nk; ++q) {
-                size_t c = 0;
-                for (size_t a = 0; a < d.nb; ++a)
-                    for (size_t b = a; b < d.nb; ++b, ++c) {
-                        vp[2 * (q * np + c)] = d.VR[2 * (q * nb2 + a * d.nb + b)];
-                        vp[2 * (q * np + c) + 1] = d.VR[2 * (q * nb2 + a * d.nb + b) + 1];
-                    }
-            }
-            VR_packed.alloc(vp.size() * sizeof(double));
-            h2d(VR_packed.get(), vp.data(), vp.size() * sizeof(double), stream);
+    if (exchange && dv > 1 && nb > 1) dense_packable = dense_vr_packable(d.VR, d.nk1, d.nk2, d.nb);
+    if (has_refP) {
+        refP_const = true;
+        for (size_t k = 1; k < nk && refP_const; ++k)
+            refP_const = std::memcmp(d.refP, d.refP + 2 * k * nb2, nb2 * cz) == 0;
+        if (refP_const) {
+            refP0.alloc(nb2 * cz);
+            h2d(refP0.get(), d.refP, nb2 * cz, stream);
+            std::vector<double> wr(d.nk1, 0.0);
+            for (size_t i = 0; i < d.nk1; ++i)
+                for (size_t j = 0; j < d.nk2; ++j) wr[i] += d.w2d[i * d.nk2 + j];
+            wrow.alloc(d.nk1 * sizeof(double));
+            h2d(wrow.get(), wr.data(), d.nk1 * sizeof(double), stream);
             HFGPU_CHECK(hipStreamSynchronize(stream));
         }
     }
-    T_ws.alloc(nd * cz);
+    // workspace of either geometry: at most nb^2 + 15 padded channels
+    T_ws.alloc((nb2 + 16) * nk * cz);
     partial.alloc(static_cast<size_t>(nk1) * nb2 * cz);
     SC.alloc(nb2 * cz);
     hdiag.alloc(std::max<size_t>(nb, 1) * sizeof(double));
@@ -217,36 +216,56 @@ GpuKernel::~GpuKernel() {
         (void)hipStreamDestroy(stream);
     }
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
-    if (geom_full) geom_full->release();
-    if (geom_packed) geom_packed->release();
+    if (geo_full.g) geo_full.g->release();
+    if (geo_packed.g) geo_packed.g->release();
 }
 
-ExchangeGeom& GpuKernel::geom(bool packed) {
-    auto& g = packed ? geom_packed : geom_full;
-    if (!g) {
-        g = std::make_unique<ExchangeGeom>();
+GpuKernel::Geom& GpuKernel::geom(bool packed) {
+    Geom& G = packed ? geo_packed : geo_full;
+    if (!G.g) {
+        auto g = std::make_unique<ExchangeGeom>();
         g->init(nk1, nk2, nb, packed);
+        const size_t bytes = g->tile_elems() * sizeof(cd);
+        if (bytes > T_ws.size()) throw std::logic_error("exchange workspace too small");
+        if (exchange && dv > 1) {
+            G.VRb.alloc(bytes);
+            launch_pack_vr(*g, VR_full.as<cd>(), G.VRb.as<cd>(), stream);
+        }
+        if (packed) {
+            G.hhb.alloc(bytes);
+            launch_pack_herm_h(*g, h.as<cd>(), G.hhb.as<cd>(), stream);
+        }
+        HFGPU_CHECK(hipStreamSynchronize(stream));
+        G.g = std::move(g);
     }
-    return *g;
+    return G;
 }
 
 void GpuKernel::build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, bool p_hermitian,
                            hipStream_t st) {
     const bool need_partials = hartree || contact;
     const bool packed = exchange && packed_for(p_hermitian);
-    ExchangeGeom& g = geom(packed);
+    Geom& G = geom(packed);
+    const ExchangeGeom& g = *G.g;
     ExchangeArgs a;
     a.P = P;
-    a.refP = has_refP ? refP.as<cd>() : nullptr;
+    if (has_refP) {
+        if (refP_const) {
+            a.refP0 = refP0.as<cd>();
+            a.wrow = wrow.as<double>();
+        } else {
+            a.refP = refP.as<cd>();
+        }
+    }
     a.T = T_ws.as<cd>();
     a.partial = need_partials ? partial.as<cd>() : nullptr;
     a.w2d = w2d.as<double>();
-    if (packed && dv > 1) {
-        a.VR = VR_packed.as<cd>();
-        a.nvr = g.nch;
+    if (dv > 1) {
+        a.VR = G.VRb.as<cd>();
+        a.vr_scalar = false;
     } else {
         a.VR = VR_full.as<cd>();
-        a.nvr = dv * dv;
+        a.vr_scalar = true;
     }
     double* hd = hdiag_out ? hdiag_out : hdiag.as<double>();
     if (need_partials) {
@@ -275,7 +294,7 @@ void GpuKernel::build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, boo
         if (packed) {
             a.epilogue = EPI_FOCK_PACKED;
             a.F = F;
-            a.h = h.as<cd>();
+            a.hh = G.hhb.as<cd>();
             a.SC = sc;
             a.hdiag = hdp;
             launch_exchange_inv(g, a, st);
@@ -444,15 +463,17 @@ int hfgpu_selfenergy_fft_full(const double* VR, const double* P, double* sigma, 
         if (nd == 0) return;
         ExchangeGeom& g = geom_cache().get(static_cast<int>(nk1), static_cast<int>(nk2),
                                            static_cast<int>(nb), hcp != 0);
+        const bool dense = dv1 > 1;
         Stream st;
-        DevBuf dP(nd * sizeof(cd)), dS(nd * sizeof(cd)), dT(nk * g.nch * sizeof(cd)),
-            dV(nk * dv1 * dv2 * sizeof(cd));
+        DevBuf dP(nd * sizeof(cd)), dS(nd * sizeof(cd)), dT(g.tile_elems() * sizeof(cd)),
+            dV(nk * dv1 * dv2 * sizeof(cd)), dVb(dense ? g.tile_elems() * sizeof(cd) : 0);
         h2d(dP.get(), P, nd * sizeof(cd), st.s);
         h2d(dV.get(), VR, nk * dv1 * dv2 * sizeof(cd), st.s);
+        if (dense) launch_pack_vr(g, dV.as<cd>(), dVb.as<cd>(), st.s);
         ExchangeArgs a;
         a.P = dP.as<cd>();
-        a.VR = dV.as<cd>();
-        a.nvr = static_cast<int>(dv1 * dv2);
+        a.VR = dense ? dVb.as<cd>() : dV.as<cd>();
+        a.vr_scalar = !dense;
         a.T = dT.as<cd>();
         a.epilogue = hcp ? EPI_RAW_PACKED : EPI_RAW_FULL;
         a.ifftshift = apply_ifftshift != 0;

@@ -17,11 +17,9 @@ struct MuState {
     double emin, emax;
     int done;
     int pad;
-    unsigned counter;
-    unsigned pad2;
 };
 
-// Workspace for the two-stage reductions: >= kMuPartials * 32 doubles.
+// Workspace of the two-launch reductions: kMuBlocks workgroups x 32 doubles.
 constexpr int kMuBlocks = 512;
 constexpr int kMuPartialDoubles = kMuBlocks * 32;
 

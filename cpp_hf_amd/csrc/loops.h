@@ -31,7 +31,6 @@ enum ScSlot : int {
 
 struct LoopScratch {
     double* partial = nullptr;  // >= 1024 * 2 kMaxBt doubles
-    unsigned* counter = nullptr;
     double* sc = nullptr;       // SC_COUNT doubles
 };
 constexpr size_t kLoopPartialDoubles = 1024 * 2 * kMaxBt;

@@ -14,9 +14,10 @@
 //
 // One wavefront per k point with the k point's nb x nb blocks in LDS; W
 // wavefronts per workgroup, grid-stride over k.  Scalar results go through
-// per-block partials and a last-block reduction in block order into the
-// solver's device scalar array, so no host round trip is needed inside an
-// iteration and every sum is deterministic for a fixed launch shape.
+// per-workgroup partials and a one-workgroup finalising launch that sums them
+// in workgroup order into the solver's device scalar array, so no host round
+// trip is needed inside an iteration, every sum is deterministic for a fixed
+// launch shape, and no workgroup needs a device-scope fence.
 #include <cmath>
 
 #include "loops.h"
@@ -79,26 +80,47 @@ __device__ __forceinline__ void store_block_partials(double (&v)[NS], double* re
     }
 }
 
-// Last block to arrive sums the partials slot by slot (fixed order) into
-// out[0..NS).  Returns true in that block only.
-template <int NS, int W>
-__device__ bool reduce_last_block(double* partial, unsigned* counter, double* out, double* red,
-                                  bool* is_last) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) *is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!*is_last) return false;
-    __threadfence();
-    for (int s = 0; s < NS; ++s) {
+// Finalising launch: out[s] = sum over workgroups of partial[b * ns + s], in
+// workgroup order within each lane, then a fixed wave tree.  MODE 1 also
+// selects the line-search step from the trial free energies (the reference's
+// sequential acceptance test, solver_dm.hpp:561-589).
+template <int MODE>
+__global__ void __launch_bounds__(256)
+k_finalize(const double* __restrict__ partial, int nparts, int ns, double* __restrict__ out,
+           double* sc, int nbt, double shrink, double T_r, double max_step) {
+    __shared__ double s_sum[2 * kMaxBt];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int slot = wave; slot < ns; slot += 4) {
         double t = 0.0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) t += partial[b * NS + s];
-        t = block_sum<W * 64>(t, red);
-        if (threadIdx.x == 0) out[s] = t;
+        for (int b = lane; b < nparts; b += 64) t += partial[b * ns + slot];
+        t = wave_sum(t);
+        if (lane == 0) {
+            out[slot] = t;
+            if (MODE == 1) s_sum[slot] = t;
+        }
     }
-    if (threadIdx.x == 0) *counter = 0;
+    if (MODE != 1) return;
     __syncthreads();
-    return true;
+    if (threadIdx.x != 0) return;
+    const double dqn = sqrt(fmax(0.0, sc[SC_DQN2]));
+    const double tau0 = fmin(1.0, max_step / fmax(dqn, TINY_REAL));
+    const double Tv = fmax(T_r, 1.0e-14);
+    const double om0 = sc[SC_E0] - Tv * sc[SC_S0];
+    double t = tau0;
+    bool ok = false;
+    for (int j = 0; j < nbt; ++j) {
+        const double om = s_sum[2 * j] - Tv * s_sum[2 * j + 1];
+        if (om < om0) {
+            ok = true;
+            break;
+        }
+        t *= shrink;
+    }
+    if (!ok) t = tau0 * pow(shrink, static_cast<double>(nbt));
+    sc[SC_TAU0] = tau0;
+    sc[SC_OMEGA0] = om0;
+    sc[SC_TAU] = t;
+    sc[SC_ACCEPTED] = ok ? 1.0 : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -112,10 +134,9 @@ __global__ void __launch_bounds__(W * 64)
 k_orbital_grad(int nk, int nb, const cd* __restrict__ Q, cd* __restrict__ F,
                const double* __restrict__ p, const double* __restrict__ w, const MuState* mus,
                double T_r, int grad, double* __restrict__ eps, cd* __restrict__ G,
-               double* __restrict__ g_p, double* partial, unsigned* counter, double* sc) {
+               double* __restrict__ g_p, double* partial, double* sc) {
     extern __shared__ cd smem[];
     __shared__ double red[W * 2];
-    __shared__ bool is_last;
     const int nb2 = nb * nb;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     cd* sQ = smem + wave * 3 * nb2;
@@ -171,7 +192,6 @@ k_orbital_grad(int nk, int nb, const cd* __restrict__ Q, cd* __restrict__ F,
     }
     if (!grad) return;
     store_block_partials<2, W>(acc, red, partial);
-    reduce_last_block<2, W>(partial, counter, sc + SC_GN2, red, &is_last);
 }
 
 // d_Q = G / sqrt(gap^2 + lam^2), h_p = g_p / curv, plus the PR+ products.
@@ -182,9 +202,8 @@ k_precond(int nk, int nb, const cd* __restrict__ G, const cd* __restrict__ Gp,
           const double* __restrict__ eps, const double* __restrict__ p,
           const double* __restrict__ g_p, const double* __restrict__ g_pp,
           const double* __restrict__ w, double T_r, double denom_scale, cd* __restrict__ dQ,
-          double* __restrict__ h_p, double* partial, unsigned* counter, double* sc) {
+          double* __restrict__ h_p, double* partial, const double* sc) {
     __shared__ double red[W * 6];
-    __shared__ bool is_last;
     const int nb2 = nb * nb;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const size_t nkb = static_cast<size_t>(nk) * nb;
@@ -227,7 +246,6 @@ k_precond(int nk, int nb, const cd* __restrict__ G, const cd* __restrict__ Gp,
         acc[5] += wk * s6;
     }
     store_block_partials<6, W>(acc, red, partial);
-    reduce_last_block<6, W>(partial, counter, sc + SC_PR1, red, &is_last);
 }
 
 // beta from the PR+ products; d_Q += beta d_Q_prev, d_p = h_p + beta d_p_prev.
@@ -238,9 +256,8 @@ k_cg_combine(int nk, int nb, int restart, cd* __restrict__ dQ, const cd* __restr
              const double* __restrict__ h_p, const double* __restrict__ d_pp,
              double* __restrict__ d_p, const double* __restrict__ eps,
              const double* __restrict__ p, const double* __restrict__ w, double T_r,
-             double* partial, unsigned* counter, double* sc) {
+             double* partial, double* sc) {
     __shared__ double red[W * 3];
-    __shared__ bool is_last;
     const int nb2 = nb * nb;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const double num = ((sc[SC_PR1] - sc[SC_PR1 + 1]) + sc[SC_PR1 + 2]) - sc[SC_PR1 + 3];
@@ -272,9 +289,8 @@ k_cg_combine(int nk, int nb, int restart, cd* __restrict__ dQ, const cd* __restr
         acc[2] += s0;
     }
     store_block_partials<3, W>(acc, red, partial);
-    if (reduce_last_block<3, W>(partial, counter, sc + SC_DQN2, red, &is_last) &&
-        threadIdx.x == 0)
-        sc[SC_BETA] = beta;
+    // every workgroup computes the same beta; one of them records it
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[SC_BETA] = beta;
 }
 
 // Per-k: Fe = V^H Ft V (stored), then for each tau: diag(U^H Ft U) via the
@@ -306,11 +322,9 @@ __global__ void __launch_bounds__(W * 64)
 k_linesearch(int nk, int nb, const cd* __restrict__ V, const double* __restrict__ lam,
              const cd* __restrict__ Ft, cd* __restrict__ Fe, const double* __restrict__ p,
              const double* __restrict__ d_p, const double* __restrict__ w, double T_r,
-             double max_step, double shrink, int nbt, double* partial, unsigned* counter,
-             double* sc) {
+             double max_step, double shrink, int nbt, double* partial, const double* sc) {
     extern __shared__ cd smem[];
     __shared__ double red[W * 2 * kMaxBt];
-    __shared__ bool is_last;
     const int nb2 = nb * nb;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     cd* sV = smem + wave * ls_wave_cd(nb);
@@ -365,26 +379,6 @@ k_linesearch(int nk, int nb, const cd* __restrict__ V, const double* __restrict_
         }
     }
     store_block_partials<2 * kMaxBt, W>(acc, red, partial);
-    if (reduce_last_block<2 * kMaxBt, W>(partial, counter, sc + SC_TRIAL, red, &is_last) &&
-        threadIdx.x == 0) {
-        const double Tv = fmax(T_r, 1.0e-14);
-        const double om0 = sc[SC_E0] - Tv * sc[SC_S0];
-        double t = tau0;
-        bool ok = false;
-        for (int j = 0; j < nbt; ++j) {
-            const double om = sc[SC_TRIAL + 2 * j] - Tv * sc[SC_TRIAL + 2 * j + 1];
-            if (om < om0) {
-                ok = true;
-                break;
-            }
-            t *= shrink;
-        }
-        if (!ok) t = tau0 * pow(shrink, static_cast<double>(nbt));
-        sc[SC_TAU0] = tau0;
-        sc[SC_OMEGA0] = om0;
-        sc[SC_TAU] = t;
-        sc[SC_ACCEPTED] = ok ? 1.0 : 0.0;
-    }
 }
 
 // Q <- Q V diag(c(tau)) V^H and eps_work = diag(U^H Ft U) at the chosen tau.
@@ -470,10 +464,9 @@ template <int W>
 __global__ void __launch_bounds__(W * 64)
 k_scf_residuals(int nk, int nb, const cd* __restrict__ F, const cd* __restrict__ D,
                 const cd* __restrict__ Pn, cd* __restrict__ delta, cd* __restrict__ comm,
-                const double* __restrict__ w, double* partial, unsigned* counter, double* out) {
+                const double* __restrict__ w, double* partial) {
     extern __shared__ cd smem[];
     __shared__ double red[W * 2];
-    __shared__ bool is_last;
     const int nb2 = nb * nb;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     cd* sF = smem + wave * 2 * nb2;
@@ -513,15 +506,13 @@ k_scf_residuals(int nk, int nb, const cd* __restrict__ F, const cd* __restrict__
         __syncthreads();
     }
     store_block_partials<2, W>(acc, red, partial);
-    reduce_last_block<2, W>(partial, counter, out, red, &is_last);
 }
 
 // out = sum_k w_k sum_i Re(conj(X_ki) Y_ki)  (m = elements per k)
 __global__ void __launch_bounds__(256)
 k_wdot(int nk, int m, const cd* __restrict__ X, const cd* __restrict__ Y,
-       const double* __restrict__ w, double* partial, unsigned* counter, double* out) {
+       const double* __restrict__ w, double* partial) {
     __shared__ double red[4];
-    __shared__ bool is_last;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double acc[1] = {0.0};
     for (int k = blockIdx.x * 4 + wave; k < nk; k += gridDim.x * 4) {
@@ -534,7 +525,6 @@ k_wdot(int nk, int m, const cd* __restrict__ X, const cd* __restrict__ Y,
         acc[0] += w[k] * s;
     }
     store_block_partials<1, 4>(acc, red, partial);
-    reduce_last_block<1, 4>(partial, counter, out, red, &is_last);
 }
 
 unsigned loop_blocks(int nk, int W) {
@@ -556,6 +546,14 @@ static void check_nb(int nb) {
     if (nb < 1 || nb > 32) throw std::invalid_argument("solver kernels support 1 <= nb <= 32");
 }
 
+static int waves_for(int nb) { return nb <= 16 ? 4 : 1; }
+
+static void finalize(LoopScratch& ls, int nparts, int ns, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_finalize<0>, dim3(1), dim3(256), 0, st, ls.partial, nparts, ns, out,
+                       ls.sc, 0, 0.0, 0.0, 0.0);
+    HFGPU_LAUNCH_CHECK();
+}
+
 void launch_orbital_grad(int nk, int nb, const cd* Q, cd* F, const double* p, const double* w,
                          const MuState* mus, double T_r, bool grad, double* eps, cd* G,
                          double* g_p, LoopScratch& ls, hipStream_t st) {
@@ -564,11 +562,12 @@ void launch_orbital_grad(int nk, int nb, const cd* Q, cd* F, const double* p, co
     HFGPU_DISPATCH_W(nb,
         hipLaunchKernelGGL(k_orbital_grad<4>, dim3(loop_blocks(nk, 4)), dim3(256), 4 * per_wave, st,
                            nk, nb, Q, F, p, w, mus, T_r, grad ? 1 : 0, eps, G, g_p, ls.partial,
-                           ls.counter, ls.sc),
+                           ls.sc),
         hipLaunchKernelGGL(k_orbital_grad<1>, dim3(loop_blocks(nk, 1)), dim3(64), per_wave, st,
                            nk, nb, Q, F, p, w, mus, T_r, grad ? 1 : 0, eps, G, g_p, ls.partial,
-                           ls.counter, ls.sc));
+                           ls.sc));
     HFGPU_LAUNCH_CHECK();
+    if (grad) finalize(ls, loop_blocks(nk, waves_for(nb)), 2, ls.sc + SC_GN2, st);
 }
 
 void launch_precond(int nk, int nb, const cd* G, const cd* Gp, const double* eps,
@@ -578,12 +577,11 @@ void launch_precond(int nk, int nb, const cd* G, const cd* Gp, const double* eps
     check_nb(nb);
     HFGPU_DISPATCH_W(nb,
         hipLaunchKernelGGL(k_precond<4>, dim3(loop_blocks(nk, 4)), dim3(256), 0, st, nk, nb, G, Gp,
-                           eps, p, g_p, g_pp, w, T_r, denom_scale, dQ, h_p, ls.partial,
-                           ls.counter, ls.sc),
+                           eps, p, g_p, g_pp, w, T_r, denom_scale, dQ, h_p, ls.partial, ls.sc),
         hipLaunchKernelGGL(k_precond<1>, dim3(loop_blocks(nk, 1)), dim3(64), 0, st, nk, nb, G, Gp,
-                           eps, p, g_p, g_pp, w, T_r, denom_scale, dQ, h_p, ls.partial,
-                           ls.counter, ls.sc));
+                           eps, p, g_p, g_pp, w, T_r, denom_scale, dQ, h_p, ls.partial, ls.sc));
     HFGPU_LAUNCH_CHECK();
+    finalize(ls, loop_blocks(nk, waves_for(nb)), 6, ls.sc + SC_PR1, st);
 }
 
 void launch_cg_combine(int nk, int nb, bool restart, cd* dQ, const cd* dQp, const double* h_p,
@@ -593,11 +591,12 @@ void launch_cg_combine(int nk, int nb, bool restart, cd* dQ, const cd* dQp, cons
     HFGPU_DISPATCH_W(nb,
         hipLaunchKernelGGL(k_cg_combine<4>, dim3(loop_blocks(nk, 4)), dim3(256), 0, st, nk, nb,
                            restart ? 1 : 0, dQ, dQp, h_p, d_pp, d_p, eps, p, w, T_r, ls.partial,
-                           ls.counter, ls.sc),
+                           ls.sc),
         hipLaunchKernelGGL(k_cg_combine<1>, dim3(loop_blocks(nk, 1)), dim3(64), 0, st, nk, nb,
                            restart ? 1 : 0, dQ, dQp, h_p, d_pp, d_p, eps, p, w, T_r, ls.partial,
-                           ls.counter, ls.sc));
+                           ls.sc));
     HFGPU_LAUNCH_CHECK();
+    finalize(ls, loop_blocks(nk, waves_for(nb)), 3, ls.sc + SC_DQN2, st);
 }
 
 static size_t ls_wave_bytes(int nb) { return sizeof(cd) * static_cast<size_t>(ls_wave_cd(nb)); }
@@ -613,10 +612,14 @@ void launch_linesearch(int nk, int nb, const cd* V, const double* lam, const cd*
     HFGPU_DISPATCH_W(nb,
         hipLaunchKernelGGL(k_linesearch<4>, dim3(loop_blocks(nk, 4)), dim3(256), 4 * per_wave, st,
                            nk, nb, V, lam, Ft, Fe, p, d_p, w, T_r, max_step, shrink, nbt,
-                           ls.partial, ls.counter, ls.sc),
+                           ls.partial, ls.sc),
         hipLaunchKernelGGL(k_linesearch<1>, dim3(loop_blocks(nk, 1)), dim3(64), per_wave, st,
                            nk, nb, V, lam, Ft, Fe, p, d_p, w, T_r, max_step, shrink, nbt,
-                           ls.partial, ls.counter, ls.sc));
+                           ls.partial, ls.sc));
+    HFGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(256), 0, st, ls.partial,
+                       static_cast<int>(loop_blocks(nk, waves_for(nb))), 2 * kMaxBt,
+                       ls.sc + SC_TRIAL, ls.sc, nbt, shrink, T_r, max_step);
     HFGPU_LAUNCH_CHECK();
 }
 
@@ -639,17 +642,19 @@ void launch_scf_residuals(int nk, int nb, const cd* F, const cd* D, const cd* Pn
     const size_t per_wave = sizeof(cd) * 2 * nb * nb;
     HFGPU_DISPATCH_W(nb,
         hipLaunchKernelGGL(k_scf_residuals<4>, dim3(loop_blocks(nk, 4)), dim3(256), 4 * per_wave,
-                           st, nk, nb, F, D, Pn, delta, comm, w, ls.partial, ls.counter, out),
+                           st, nk, nb, F, D, Pn, delta, comm, w, ls.partial),
         hipLaunchKernelGGL(k_scf_residuals<1>, dim3(loop_blocks(nk, 1)), dim3(64), per_wave, st,
-                           nk, nb, F, D, Pn, delta, comm, w, ls.partial, ls.counter, out));
+                           nk, nb, F, D, Pn, delta, comm, w, ls.partial));
     HFGPU_LAUNCH_CHECK();
+    finalize(ls, loop_blocks(nk, waves_for(nb)), 2, out, st);
 }
 
 void launch_wdot(int nk, int m, const cd* X, const cd* Y, const double* w, LoopScratch& ls,
                  double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_wdot, dim3(loop_blocks(nk, 4)), dim3(256), 0, st, nk, m, X, Y, w,
-                       ls.partial, ls.counter, out);
+                       ls.partial);
     HFGPU_LAUNCH_CHECK();
+    finalize(ls, loop_blocks(nk, 4), 1, out, st);
 }
 
 }  // namespace hfgpu

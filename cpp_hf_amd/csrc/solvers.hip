@@ -71,19 +71,16 @@ void validate_block_sizes(const size_t* sizes, size_t n, size_t nb) {
 
 // Workspace shared by both solvers.
 struct SolveWs {
-    DevBuf mus, mu_partial, ls_partial, counter, sc;
+    DevBuf mus, mu_partial, ls_partial, sc;
     LoopScratch ls;
     explicit SolveWs(hipStream_t st) {
         mus.alloc(sizeof(MuState));
         mu_partial.alloc(kMuPartialDoubles * sizeof(double));
         ls_partial.alloc(kLoopPartialDoubles * sizeof(double));
-        counter.alloc(64);
         sc.alloc(SC_COUNT * sizeof(double));
         fill_zero(mus.get(), sizeof(MuState), st);
-        fill_zero(counter.get(), 64, st);
         fill_zero(sc.get(), SC_COUNT * sizeof(double), st);
         ls.partial = ls_partial.as<double>();
-        ls.counter = counter.as<unsigned>();
         ls.sc = sc.as<double>();
     }
     MuState* mu() { return mus.as<MuState>(); }
