@@ -30,6 +30,17 @@ __host__ __device__ __forceinline__ cd cmulc(cd a, cd b) {
 }
 __host__ __device__ __forceinline__ double cnorm(cd a) { return a.x * a.x + a.y * a.y; }
 
+// Streaming store of one complex value (one 16-byte non-temporal store): for
+// data the writing kernel never reads back, so it drains during the kernel
+// instead of sitting dirty in L2 until the kernel-end write-back.
+typedef double cd_vec __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_stream(cd* p, cd v) {
+    cd_vec t;
+    t.x = v.x;
+    t.y = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<cd_vec*>(p));
+}
+
 struct DeviceError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };

@@ -46,6 +46,9 @@ struct ExchangeGeom {
     int nblk = 0;     // channel tiles
     int slots_row = 0, slots_col = 0;  // tile elements per thread (power of 2)
     bool generic = false;              // a radix 7, 11 or 13 stage is present
+    // two-step register transforms (tile of 8 channels, N = N1 x N2 with
+    // both factors at most 16): rows along k2, cols along k1
+    bool fast_rows = false, fast_cols = false;
     size_t lds_rows = 0, lds_cols = 0;
     int* d_chan_a = nullptr;  // nch
     int* d_chan_b = nullptr;  // nch

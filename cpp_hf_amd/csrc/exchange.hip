@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "dft_reg.h"
 #include "exchange.h"
 
 namespace hfgpu {
@@ -87,6 +88,18 @@ static bool tile_fits(const AxisPlan& ax, int C) {
 // complex LDS slots of the small-terms workgroup
 static size_t small_terms_cd(int nb) { return 3 * static_cast<size_t>(nb) * nb + 257; }
 
+// Mesh sizes with a two-step register transform, as (N, N1, N2).
+#define HFGPU_FAST_SIZES(X)                                                               \
+    X(12, 4, 3) X(16, 4, 4) X(24, 4, 6) X(32, 4, 8) X(36, 6, 6) X(48, 8, 6) X(60, 6, 10) \
+    X(64, 8, 8) X(72, 8, 9) X(96, 8, 12) X(120, 8, 15) X(144, 12, 12) X(192, 16, 12)
+
+static bool fast_size(int n) {
+#define HFGPU_FAST_CASE(N, A, B) if (n == N) return true;
+    HFGPU_FAST_SIZES(HFGPU_FAST_CASE)
+#undef HFGPU_FAST_CASE
+    return false;
+}
+
 static int pow2_slots(long elems) {
     int s = 2;  // smallest instantiated slot count
     while (static_cast<long>(s) * kFftThreads < elems) s <<= 1;
@@ -132,6 +145,10 @@ void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
     for (const AxisPlan* ax : {&ax1, &ax2})
         for (int r : ax->radices)
             if (r == 7 || r == 11 || r == 13) generic = true;
+    const char* fe = std::getenv("HFGPU_EXCH_FAST");
+    const bool fast_ok = C == 8 && !(fe && fe[0] == '0');
+    fast_rows = fast_ok && fast_size(nk2);
+    fast_cols = fast_ok && fast_size(nk1);
     lds_rows = sizeof(cd) * (static_cast<size_t>(nk2) * C + nk2);
     lds_cols = sizeof(cd) * (static_cast<size_t>(nk1) * C + nk1);
     if (lds_rows > 160 * 1024 || lds_cols > 160 * 1024)
@@ -546,6 +563,360 @@ k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog, int nblk, i
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two-step line transforms (N = N1 x N2, both at most 16): one wavefront per
+// tile of kFastLines channels, four tiles per workgroup.
+//
+// Forward, x[i1 + N1 i2] -> X[k2 + N2 k1]:
+//   F1  task (l, i1): N2-point DFT over i2 in registers, times W_N^(i1 k2),
+//       into Y[i1][k2][l] (LDS);
+//   F2  task (l, k2): N1-point DFT over i1 -> X[k2 + N2 k1], k1 = 0..N1-1.
+// Inverse, Z[k2 + N2 k1] -> z[i1 + N1 i2]:
+//   I1  task (l, k2): N1-point inverse DFT over k1, times conj W_N^(i1 k2),
+//       into Y[i1][k2][l];
+//   I2  task (l, i1): N2-point inverse DFT over k2 -> z[i1 + N1 i2].
+// Each transform crosses LDS once; lanes are ordered channel-fastest, so
+// every LDS access of eight consecutive lanes covers 128 contiguous bytes and
+// every global access of a tile is 128 B per k point.  In pass B the forward
+// F2, the V(q) product and the inverse I1 run on the same registers.
+// ---------------------------------------------------------------------------
+
+constexpr int kFastLines = 8;  // channels per tile (the geometry's C)
+constexpr int kFastWaves = 4;  // tiles per workgroup of the column pass
+// The row passes put all nblk channel tiles of one k1 row in one workgroup
+// when nblk <= kFastRowWavesMax and the LDS fits, so every (k1, k2) block
+// of P, Sigma and F is read or written whole by one CU.
+constexpr int kFastRowWavesMax = 8;
+
+template <int N1, int N2>
+struct FastShape {
+    static constexpr int N = N1 * N2;
+    static constexpr int L = kFastLines;
+    static constexpr int R1 = (L * N1 + 63) / 64;  // F1 / I2 tasks per lane
+    static constexpr int R2 = (L * N2 + 63) / 64;  // F2 / I1 tasks per lane
+    static constexpr int Y = N * L;                // LDS complex slots per wave
+};
+
+// Pass A (rows, forward) on the two-step transform.
+template <int N1, int N2>
+__global__ void __launch_bounds__(64 * kFastRowWavesMax)
+k_rows_fwd_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, int packed,
+                const cd* __restrict__ P, const cd* __restrict__ refP,
+                const cd* __restrict__ refP0, const double* __restrict__ wrow,
+                cd* __restrict__ T, cd* __restrict__ partial, const double* __restrict__ w2d) {
+    using Sh = FastShape<N1, N2>;
+    constexpr int N = Sh::N, L = Sh::L;
+    extern __shared__ cd smem[];
+    cd* s_tw = smem;
+    const int nw = blockDim.x >> 6;  // tiles (wavefronts) per workgroup
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    cd* Y = smem + N + wave * Sh::Y;
+    cd* s_part = smem + N + nw * Sh::Y + wave * N1 * L;
+    const int tile = blockIdx.x * nw + wave;
+    const bool active = tile < nk1 * nblk;
+    const int k1 = active ? tile / nblk : 0, blk = active ? tile % nblk : 0;
+    const int nb2 = nb * nb;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
+    cd u[Sh::R1][N2];
+    double wv[Sh::R1][N2];
+#pragma unroll
+    for (int r = 0; r < Sh::R1; ++r) {
+        const int t = lane + 64 * r;
+        const int l = t & (L - 1), i1 = t >> 3;
+        const int ch = blk * L + l;
+        const bool ok = active && t < L * N1 && ch < nch;
+        int a = 0, b = 0;
+        if (ok) chan_ab(ch, nb, packed, a, b);
+        const size_t base = static_cast<size_t>(k1) * N * nb2 + a * nb + b;
+#pragma unroll
+        for (int i2 = 0; i2 < N2; ++i2) {
+            const int k2 = i1 + N1 * i2;
+            cd x{0.0, 0.0};
+            if (ok) {
+                x = P[base + static_cast<size_t>(k2) * nb2];
+                if (refP) x = csub(x, refP[base + static_cast<size_t>(k2) * nb2]);
+            }
+            u[r][i2] = x;
+            wv[r][i2] = (ok && partial) ? w2d[static_cast<size_t>(k1) * N + k2] : 0.0;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < Sh::R1; ++r) {
+        const int t = lane + 64 * r;
+        if (t < L * N1) {
+            const int l = t & (L - 1), i1 = t >> 3;
+            if (partial) {
+                cd acc{0.0, 0.0};
+#pragma unroll
+                for (int i2 = 0; i2 < N2; ++i2) {
+                    acc.x += wv[r][i2] * u[r][i2].x;
+                    acc.y += wv[r][i2] * u[r][i2].y;
+                }
+                s_part[i1 * L + l] = acc;
+            }
+            Dft<N2, false>::run(u[r]);
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                cd x = u[r][k2];
+                if (k2 > 0) x = cmul(x, s_tw[i1 * k2]);
+                Y[(i1 * N2 + k2) * L + l] = x;
+            }
+        }
+    }
+    __syncthreads();
+    if (partial && active && lane < L) {
+        const int ch = blk * L + lane;
+        if (ch < nch) {
+            cd acc{0.0, 0.0};
+            for (int i1 = 0; i1 < N1; ++i1) acc = cadd(acc, s_part[i1 * L + lane]);
+            if (refP0) {
+                int a, b;
+                chan_ab(ch, nb, packed, a, b);
+                const cd rv = refP0[a * nb + b];
+                const double wr = wrow[k1];
+                acc.x -= rv.x * wr;
+                acc.y -= rv.y * wr;
+            }
+            partial[static_cast<size_t>(k1) * nch + ch] = acc;
+        }
+    }
+    cd* Tt = T + (static_cast<size_t>(blk) * nk1 + k1) * N * L;
+#pragma unroll
+    for (int r = 0; r < Sh::R2; ++r) {
+        const int t = lane + 64 * r;
+        if (active && t < L * N2) {
+            const int l = t & (L - 1), k2 = t >> 3;
+            cd v[N1];
+#pragma unroll
+            for (int i1 = 0; i1 < N1; ++i1) v[i1] = Y[(i1 * N2 + k2) * L + l];
+            Dft<N1, false>::run(v);
+            // a k-independent reference density only reaches q2 = 0: N refP0
+            if (refP0 && k2 == 0 && blk * L + l < nch) {
+                int a, b;
+                chan_ab(blk * L + l, nb, packed, a, b);
+                const cd rv = refP0[a * nb + b];
+                v[0] = cd{v[0].x - N * rv.x, v[0].y - N * rv.y};
+            }
+#pragma unroll
+            for (int k1p = 0; k1p < N1; ++k1p) store_stream(Tt + (k2 + N2 * k1p) * L + l, v[k1p]);
+        }
+    }
+}
+
+// Pass B (columns: forward, x VR, inverse) on the two-step transform;
+// workgroup 0 hosts the small terms when has_small.
+template <int N1, int N2>
+__global__ void __launch_bounds__(64 * kFastWaves)
+k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, cd* __restrict__ T,
+            const cd* __restrict__ VR, int vr_scalar, int has_small,
+            const cd* __restrict__ partial, const int* __restrict__ chan_a,
+            const int* __restrict__ chan_b, int nb, int packed, SmallTerms st) {
+    using Sh = FastShape<N1, N2>;
+    constexpr int N = Sh::N, L = Sh::L;
+    extern __shared__ cd smem[];
+    if (has_small && blockIdx.x == 0) {
+        small_terms_block(partial, N, nch, chan_a, chan_b, nb, packed, st, smem);
+        return;
+    }
+    cd* s_tw = smem;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    cd* Y = smem + N + wave * Sh::Y;
+    const int tile = (static_cast<int>(blockIdx.x) - has_small) * kFastWaves + wave;
+    const bool active = tile < nk2 * nblk;
+    const int kc = active ? tile / nblk : 0, blk = active ? tile % nblk : 0;
+    cd* Tb = T + static_cast<size_t>(blk) * N * nk2 * L;                 // [k1][k2][c]
+    const cd* Vb = VR + (static_cast<size_t>(blk) * nk2 + kc) * N * L;   // [q1][c]
+    for (int i = threadIdx.x; i < N; i += 64 * kFastWaves) s_tw[i] = tw[i];
+    cd u[Sh::R1][N2];
+    cd vv[Sh::R2][N1];
+#pragma unroll
+    for (int r = 0; r < Sh::R1; ++r) {
+        const int t = lane + 64 * r;
+        const int l = t & (L - 1), i1 = t >> 3;
+        const bool ok = active && t < L * N1;
+#pragma unroll
+        for (int i2 = 0; i2 < N2; ++i2)
+            u[r][i2] = ok ? Tb[(static_cast<size_t>(i1 + N1 * i2) * nk2 + kc) * L + l]
+                          : cd{0.0, 0.0};
+    }
+#pragma unroll
+    for (int r = 0; r < Sh::R2; ++r) {
+        const int t = lane + 64 * r;
+        const int l = t & (L - 1), k2 = t >> 3;
+        const bool ok = active && t < L * N2;
+#pragma unroll
+        for (int k1p = 0; k1p < N1; ++k1p) {
+            const int q = k2 + N2 * k1p;
+            vv[r][k1p] = !ok ? cd{0.0, 0.0}
+                             : (vr_scalar ? VR[static_cast<size_t>(q) * nk2 + kc] : Vb[q * L + l]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < Sh::R1; ++r) {
+        const int t = lane + 64 * r;
+        if (t < L * N1) {
+            const int l = t & (L - 1), i1 = t >> 3;
+            Dft<N2, false>::run(u[r]);
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                cd x = u[r][k2];
+                if (k2 > 0) x = cmul(x, s_tw[i1 * k2]);
+                Y[(i1 * N2 + k2) * L + l] = x;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < Sh::R2; ++r) {
+        const int t = lane + 64 * r;
+        if (t < L * N2) {
+            const int l = t & (L - 1), k2 = t >> 3;
+            cd v[N1];
+#pragma unroll
+            for (int i1 = 0; i1 < N1; ++i1) v[i1] = Y[(i1 * N2 + k2) * L + l];
+            Dft<N1, false>::run(v);
+#pragma unroll
+            for (int k1p = 0; k1p < N1; ++k1p) v[k1p] = cmul(v[k1p], vv[r][k1p]);
+            Dft<N1, true>::run(v);
+            // this task reads and rewrites the same Y slots: no barrier needed
+#pragma unroll
+            for (int i1 = 0; i1 < N1; ++i1) {
+                cd x = v[i1];
+                if (i1 > 0 && k2 > 0) x = cmulc(x, s_tw[i1 * k2]);
+                Y[(i1 * N2 + k2) * L + l] = x;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < Sh::R1; ++r) {
+        const int t = lane + 64 * r;
+        if (active && t < L * N1) {
+            const int l = t & (L - 1), i1 = t >> 3;
+            cd z[N2];
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) z[k2] = Y[(i1 * N2 + k2) * L + l];
+            Dft<N2, true>::run(z);
+#pragma unroll
+            for (int i2 = 0; i2 < N2; ++i2)
+                store_stream(Tb + (static_cast<size_t>(i1 + N1 * i2) * nk2 + kc) * L + l, z[i2]);
+        }
+    }
+}
+
+// Pass C (rows, inverse + epilogue) on the two-step transform.
+template <int N1, int N2>
+__global__ void __launch_bounds__(64 * kFastRowWavesMax)
+k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, int packed,
+                const cd* __restrict__ T, int epilogue, int shift, double inv_n,
+                cd* __restrict__ Sigma, cd* __restrict__ F, const cd* __restrict__ hh,
+                const cd* __restrict__ SC, const double* __restrict__ hdiag) {
+    using Sh = FastShape<N1, N2>;
+    constexpr int N = Sh::N, L = Sh::L;
+    extern __shared__ cd smem[];
+    cd* s_tw = smem;
+    const int nw = blockDim.x >> 6;  // tiles (wavefronts) per workgroup
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    cd* Y = smem + N + wave * Sh::Y;
+    const int tile = blockIdx.x * nw + wave;
+    const bool active = tile < nk1 * nblk;
+    const int k1 = active ? tile / nblk : 0, blk = active ? tile % nblk : 0;
+    const size_t toff = (static_cast<size_t>(blk) * nk1 + k1) * N * L;
+    const bool fock = epilogue == EPI_FOCK_PACKED;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
+    cd v[Sh::R2][N1];
+#pragma unroll
+    for (int r = 0; r < Sh::R2; ++r) {
+        const int t = lane + 64 * r;
+        const int l = t & (L - 1), k2 = t >> 3;
+        const bool ok = active && t < L * N2;
+#pragma unroll
+        for (int k1p = 0; k1p < N1; ++k1p)
+            v[r][k1p] = ok ? T[toff + (k2 + N2 * k1p) * L + l] : cd{0.0, 0.0};
+    }
+    cd hv[Sh::R1][N2];
+    if (fock) {
+#pragma unroll
+        for (int r = 0; r < Sh::R1; ++r) {
+            const int t = lane + 64 * r;
+            const int l = t & (L - 1), i1 = t >> 3;
+            const bool ok = active && t < L * N1;
+#pragma unroll
+            for (int i2 = 0; i2 < N2; ++i2)
+                hv[r][i2] = ok ? hh[toff + (i1 + N1 * i2) * L + l] : cd{0.0, 0.0};
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < Sh::R2; ++r) {
+        const int t = lane + 64 * r;
+        if (t < L * N2) {
+            const int l = t & (L - 1), k2 = t >> 3;
+            Dft<N1, true>::run(v[r]);
+#pragma unroll
+            for (int i1 = 0; i1 < N1; ++i1) {
+                cd x = v[r][i1];
+                if (i1 > 0 && k2 > 0) x = cmulc(x, s_tw[i1 * k2]);
+                Y[(i1 * N2 + k2) * L + l] = x;
+            }
+        }
+    }
+    __syncthreads();
+    const int d1 = shift ? (k1 + nk1 - nk1 / 2) % nk1 : k1;
+    const int nb2 = nb * nb;
+#pragma unroll
+    for (int r = 0; r < Sh::R1; ++r) {
+        const int t = lane + 64 * r;
+        const int l = t & (L - 1), i1 = t >> 3;
+        const int ch = blk * L + l;
+        if (!(active && t < L * N1 && ch < nch)) continue;
+        int a, b;
+        chan_ab(ch, nb, packed, a, b);
+        cd sc_ab{0.0, 0.0}, sc_ba{0.0, 0.0};
+        double hd = 0.0;
+        if (fock) {
+            if (SC) {
+                sc_ab = SC[a * nb + b];
+                sc_ba = SC[b * nb + a];
+            }
+            if (hdiag && a == b) hd = hdiag[a];
+        }
+        cd z[N2];
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) z[k2] = Y[(i1 * N2 + k2) * L + l];
+        Dft<N2, true>::run(z);
+#pragma unroll
+        for (int i2 = 0; i2 < N2; ++i2) {
+            const int k2 = i1 + N1 * i2;
+            const cd y = z[i2];
+            // reference order: ifftn = backward * (1/N), then negate
+            const cd sig{-(y.x * inv_n), -(y.y * inv_n)};
+            const int d2 = shift ? (k2 + N - N / 2) % N : k2;
+            const size_t kb = static_cast<size_t>(d1 * N + d2) * nb2;
+            const size_t iab = kb + a * nb + b, iba = kb + b * nb + a;
+            if (epilogue == EPI_RAW_FULL) {
+                Sigma[iab] = sig;
+            } else if (epilogue == EPI_RAW_PACKED) {
+                Sigma[iab] = sig;
+                if (a != b) Sigma[iba] = cconj(sig);
+            } else {
+                const cd s_ab = cadd(sig, sc_ab);
+                const cd s_ba = cadd(a == b ? sig : cconj(sig), sc_ba);
+                cd f{hv[r][i2].x + 0.5 * (s_ab.x + s_ba.x), hv[r][i2].y + 0.5 * (s_ab.y - s_ba.y)};
+                f.x += hd;
+                Sigma[iab] = s_ab;
+                F[iab] = f;
+                if (a != b) {
+                    Sigma[iba] = s_ba;
+                    F[iba] = cconj(f);
+                }
+            }
+        }
+    }
+}
+
 // Tile-major operand packing (run once per kernel geometry).
 __global__ void k_pack_vr(int nk1, int nk2, int nb, int nch, int clog, int nblk, int packed,
                           const cd* __restrict__ VR, cd* __restrict__ VRb) {
@@ -706,9 +1077,82 @@ static void launch_small_terms(const ExchangeGeom& g, const ExchangeArgs& a, hip
     HFGPU_LAUNCH_CHECK();
 }
 
+// LDS bytes of the two-step passes for a line of n points, w wavefronts
+// (+ row-sum slots)
+static size_t fast_lds(int n, int n1, int w, bool row_sums) {
+    return sizeof(cd) * (static_cast<size_t>(n) * (1 + static_cast<size_t>(w) * kFastLines) +
+                         (row_sums ? static_cast<size_t>(w) * kFastLines * n1 : 0));
+}
+
+static unsigned fast_grid(int lines, int nblk, int w) {
+    return cdiv(static_cast<size_t>(lines) * nblk, w);
+}
+
+// wavefronts per workgroup of a row pass: one k1 row per workgroup if it fits
+static int row_waves(const ExchangeGeom& g, int n1, bool row_sums) {
+    if (g.nblk <= kFastRowWavesMax && fast_lds(g.nk2, n1, g.nblk, row_sums) <= 160 * 1024)
+        return g.nblk;
+    return kFastWaves;
+}
+
+static void launch_rows_fwd_fast(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
+#define HFGPU_FAST_CASE(N, A, B)                                                              \
+    if (g.nk2 == N) {                                                                         \
+        const int w = row_waves(g, A, true);                                                  \
+        hipLaunchKernelGGL((k_rows_fwd_fast<A, B>), dim3(fast_grid(g.nk1, g.nblk, w)),        \
+                           dim3(64 * w), fast_lds(N, A, w, true), st, g.ax2.d_tw, g.nk1,      \
+                           g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, a.refP, a.refP0,       \
+                           a.wrow, a.T, a.partial, a.w2d);                                    \
+        HFGPU_LAUNCH_CHECK();                                                                 \
+        return;                                                                               \
+    }
+    HFGPU_FAST_SIZES(HFGPU_FAST_CASE)
+#undef HFGPU_FAST_CASE
+    throw std::logic_error("no two-step transform for this row length");
+}
+
+static void launch_cols_fast(const ExchangeGeom& g, const ExchangeArgs& a, int hs,
+                             hipStream_t st) {
+    const dim3 grid(fast_grid(g.nk2, g.nblk, kFastWaves) + hs), block(64 * kFastWaves);
+#define HFGPU_FAST_CASE(N, A, B)                                                              \
+    if (g.nk1 == N) {                                                                         \
+        const size_t lds = std::max(fast_lds(N, A, kFastWaves, false),                                    \
+                                    hs ? sizeof(cd) * small_terms_cd(g.nb) : size_t(0));      \
+        hipLaunchKernelGGL((k_cols_fast<A, B>), grid, block, lds, st, g.ax1.d_tw, g.nk2,      \
+                           g.nch, g.nblk, a.T, a.VR, a.vr_scalar ? 1 : 0, hs, a.partial,      \
+                           g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0, a.small);          \
+        HFGPU_LAUNCH_CHECK();                                                                 \
+        return;                                                                               \
+    }
+    HFGPU_FAST_SIZES(HFGPU_FAST_CASE)
+#undef HFGPU_FAST_CASE
+    throw std::logic_error("no two-step transform for this column length");
+}
+
+static void launch_rows_inv_fast(const ExchangeGeom& g, const ExchangeArgs& a, double inv_n,
+                                 hipStream_t st) {
+#define HFGPU_FAST_CASE(N, A, B)                                                              \
+    if (g.nk2 == N) {                                                                         \
+        const int w = row_waves(g, A, false);                                                 \
+        hipLaunchKernelGGL((k_rows_inv_fast<A, B>), dim3(fast_grid(g.nk1, g.nblk, w)),        \
+                           dim3(64 * w), fast_lds(N, A, w, false), st, g.ax2.d_tw, g.nk1,     \
+                           g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.T, a.epilogue,            \
+                           a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh, a.SC, a.hdiag);    \
+        HFGPU_LAUNCH_CHECK();                                                                 \
+        return;                                                                               \
+    }
+    HFGPU_FAST_SIZES(HFGPU_FAST_CASE)
+#undef HFGPU_FAST_CASE
+    throw std::logic_error("no two-step transform for this row length");
+}
+
 static void launch_rows_fwd(const ExchangeGeom& g, const ExchangeArgs& a, int do_fft,
                             hipStream_t st) {
     if (a.refP0 && !a.wrow) throw std::logic_error("refP0 without row weights");
+    if (do_fft && g.fast_rows) {
+        launch_rows_fwd_fast(g, a, st);
+        return;
+    }
     HFGPU_EXCH_LAUNCH(k_rows_fwd, g.generic, g.slots_row, dim3(g.nblk * g.nk1), g.lds_rows, st,
                       g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.nblk, g.packed ? 1 : 0,
                       a.P, a.refP, a.refP0, a.wrow, a.T, a.partial, a.w2d, do_fft);
@@ -721,6 +1165,11 @@ void launch_row_sums(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t s
 
 void launch_exchange_fwd(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
     launch_rows_fwd(g, a, 1, st);
+    if (g.fast_cols) {
+        // workgroup 0 of the column pass hosts the small terms
+        launch_cols_fast(g, a, a.small.enabled ? 1 : 0, st);
+        return;
+    }
     // the small terms ride in pass B as one extra workgroup when its LDS fits
     const bool fuse_small = a.small.enabled && sizeof(cd) * small_terms_cd(g.nb) <= g.lds_cols;
     if (a.small.enabled && !fuse_small) launch_small_terms(g, a, st);
@@ -739,6 +1188,10 @@ void launch_exchange_inv(const ExchangeGeom& g, const ExchangeArgs& a, hipStream
     if (a.epilogue == EPI_FOCK_PACKED && !a.hh)
         throw std::logic_error("Fock epilogue without herm(h)");
     const double inv_n = 1.0 / static_cast<double>(static_cast<size_t>(g.nk1) * g.nk2);
+    if (g.fast_rows) {
+        launch_rows_inv_fast(g, a, inv_n, st);
+        return;
+    }
     HFGPU_EXCH_LAUNCH(k_rows_inv, g.generic, g.slots_row, dim3(g.nblk * g.nk1), g.lds_rows, st,
                       g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.nblk, g.packed ? 1 : 0,
                       a.T, a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh, a.SC,

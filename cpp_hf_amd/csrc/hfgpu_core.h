@@ -103,7 +103,8 @@ struct GpuKernel {
     void energy(const cd* P, const cd* Sigma, const double* hdiag_in, double* E, hipStream_t st);
 
     // build_fock replayed from a captured hipGraph (one per pointer set,
-    // small FIFO); falls back to eager launches when HFGPU_NO_GRAPH is set.
+    // small FIFO) when use_graph, else launched eagerly.
+    bool use_graph = false;
     void build_fock_graph(const cd* P, cd* Sigma, cd* F, double* hdiag_out, bool p_hermitian,
                           hipStream_t st);
     struct GraphEntry {

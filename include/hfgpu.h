@@ -104,8 +104,10 @@ int hfgpu_build_fock_apply(hfgpu_kernel* k, const double* P, double* Sigma, doub
  * tensors, benchmarks).  Replaces: build_fock_compact (fock.hpp:34-299).
  *   dP, dSigma, dF: device (nk1, nk2, nb, nb) complex; dHdiag: device (nb).
  *   p_hermitian != 0 asserts P == P^H (enables the packed exchange).
- *   stream: the hipStream_t to run on (NULL: the null stream).  Asynchronous;
- *   replayed from a hipGraph captured on the first call with these pointers. */
+ *   stream: the hipStream_t to run on (NULL: the null stream).  Asynchronous.
+ *   With HFGPU_GRAPH=1 in the environment when the kernel is created, the
+ *   launches are captured into a hipGraph on the first call with these
+ *   pointers and replayed afterwards. */
 int hfgpu_build_fock_device(hfgpu_kernel* k, const void* dP, void* dSigma, void* dF,
                             void* dHdiag, int p_hermitian, void* stream);
 

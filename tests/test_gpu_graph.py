@@ -1,6 +1,7 @@
-"""The device-pointer Fock build replays a captured hipGraph; every replay must
-equal the eager result, with eager work and synchronisations in between and
-new data written into the captured input buffer."""
+"""With HFGPU_GRAPH=1 the device-pointer Fock build replays a captured
+hipGraph; every replay must equal the eager result, with eager work and
+synchronisations in between and new data written into the captured input
+buffer."""
 
 from __future__ import annotations
 
@@ -13,7 +14,9 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("which", ["layered", "qwz", "full_channels"])
-def test_graph_replays_match_eager(gpu, which):
+def test_graph_replays_match_eager(gpu, which, monkeypatch):
+    # read when the kernel is created
+    monkeypatch.setenv("HFGPU_GRAPH", "1")
     from hipmem import DeviceArray, synchronize
 
     from cpp_hf_amd.workloads import layered_kernel, qwz_kernel
