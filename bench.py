@@ -10,7 +10,9 @@ Hermitian density held in HBM.
 The JSON line also carries:
   roofline      achieved algorithmic GB/s of the Fock build (6 A bytes per
                 build, A = 16 nk nb^2: P, refP, h and VR in, Sigma and F out)
-                over its HIP-event-timed duration, against 8 TB/s HBM.
+                over its HIP-event-timed duration, against 8 TB/s HBM;
+                traffic = HBM bytes per build from the committed
+                FETCH_SIZE/WRITE_SIZE summary of this workload.
   cpu_baseline  the CPU oracle's Fock build on this host, timed for a
                 bounded number of builds (a restatement of the reference's
                 FFTW/Eigen path, which cannot be built here).
@@ -33,6 +35,18 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = ROOT / "profiles" / "fock_build_96x96_nb8.md"
+
+
+def measured_traffic(nk1: int, nk2: int, nb: int):
+    """HBM bytes per Fock build from the committed FETCH_SIZE / WRITE_SIZE
+    summary (copy-calibrated), for the workload that summary was taken on."""
+    if (nk1, nk2, nb) != (96, 96, 8) or not TRAFFIC_PROFILE.exists():
+        return None
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_build:"):
+            return int(line.split(":", 1)[1])
+    return None
 
 
 def main() -> int:
@@ -156,12 +170,12 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "fock_build (k_rows_fwd + k_cols + k_small_terms + k_rows_inv)",
+            "kernel": "fock_build (k_rows_fwd_fast + k_cols_fast + k_rows_inv_fast)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(nk1, nk2, nb) if herm else None,
             "algorithmic_bytes_per_build": alg_bytes,
             "event_us_per_build": per_build_s * 1e6,
         },
