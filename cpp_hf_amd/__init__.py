@@ -17,6 +17,7 @@ from .solver import SolverConfig, SolveResult, solve, solve_direct_minimization
 from .reference_scf import SCFConfig, SCFResult, solve_scf
 from .fock import build_fock, hf_energy, free_energy, occupation_entropy
 from .utils import resample_kgrid
+from .sweep import solve_many
 
 __all__ = [
     "HartreeFockKernel",
@@ -32,4 +33,5 @@ __all__ = [
     "free_energy",
     "occupation_entropy",
     "resample_kgrid",
+    "solve_many",
 ]

@@ -45,6 +45,14 @@ struct DeviceError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
 
+// Device memory from the library's caching pool (power-of-two size classes
+// per device, never returned to the runtime until the process ends).  A
+// hipFree synchronises the whole device, so a solve that allocates and frees
+// its workspaces would serialise every stream; the pool keeps solves on
+// separate streams independent.  Thread-safe.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p, size_t bytes);
+
 inline void hip_check(hipError_t e, const char* what, const char* file, int line) {
     if (e != hipSuccess)
         throw DeviceError(std::string(what) + " failed: " + hipGetErrorString(e) + " (" +

@@ -2,11 +2,13 @@
 // matrix (reference: eigh_one / eigh_batched, linalg.hpp:39-66, and the
 // Cayley setup eigh of herm(i d), solver_dm.hpp:206-222).
 //
-// The nb x nb matrix and its eigenvector accumulator live in LDS.  Each round
-// of a sweep applies nb/2 disjoint complex Jacobi rotations at once (circle
-// ordering), so every lane updates its own elements A'_ij = sum conj(J_ki)
-// A_kl J_lj over the two rotation pairs that touch row i and column j.
-// Sweeps stop once the off-diagonal mass falls to (4 nb eps)^2 ||A||_F^2.
+// Each round of a sweep applies the nb/2 disjoint complex Jacobi rotations of
+// the circle ordering at once; sweeps stop once the off-diagonal mass falls to
+// (4 nb eps)^2 ||A||_F^2; the eigenvalues are returned in ascending, stable
+// order with the eigenvectors as columns.  nb <= 8 runs the register-row
+// kernel (k_eigh_rows), larger nb the LDS kernel (k_eigh_jacobi), where the
+// matrix and its eigenvector accumulator live in LDS and every lane updates
+// its own elements A'_ij = sum conj(J_ki) A_kl J_lj.
 #include <stdexcept>
 
 #include "kernels.h"
@@ -181,19 +183,236 @@ k_eigh_jacobi(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* _
     }
 }
 
+// ---------------------------------------------------------------------------
+// Register-row variant for nb <= 8: NP (nb rounded up to 2, 4 or 8) lanes
+// per matrix, lane i holding row i of A and of the eigenvector accumulator V
+// in registers, 64 / NP matrices per wavefront and no LDS.  A round of the
+// circle ordering pairs lane i with lane partner(i, r); both lanes of a pair
+// compute the same rotation, the column update A <- A J (and V <- V J) is
+// local to each lane, and the row update A <- J^H A swaps the partner's row
+// through cross-lane shuffles.  Rounds are unrolled, so every register index
+// is a compile-time constant.
+// ---------------------------------------------------------------------------
+
+__host__ __device__ constexpr int circle_partner(int i, int r, int np) {
+    return i == np - 1 ? r : (i == r ? np - 1 : ((2 * r - i) % (np - 1) + (np - 1)) % (np - 1));
+}
+
+// Per-wavefront LDS exchange area of the register-row kernel: each lane's
+// row (for the partner's row update), rotation parameters and diagonal.
+// Lanes of one wavefront execute LDS operations in order, so a lane may read
+// what another lane of its wavefront wrote earlier without a barrier; the
+// wave barrier only keeps the compiler from moving LDS accesses across it.
+template <int NP>
+struct RowLds {
+    cd row[64][NP];
+    double par[64][4];  // c, Re(s e), Im(s e), t |b|
+    double dg[64];
+};
+
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
+// One round R of the circle ordering.  Every register index below is a
+// compile-time constant: lane-dependent choices are made with selects over
+// the round's (compile-time) pairs, never by indexing the row with the lane's
+// own position, which would push the row out to scratch memory.  dg is this
+// row's diagonal element, kept alongside the row.
+template <int NP, int R>
+__device__ __forceinline__ void jacobi_round(cd (&A)[NP], cd (&V)[NP], double& dg, int i,
+                                             int lane, int gbase, int nb, RowLds<NP>& X) {
+    const int pi = circle_partner(i, R, NP);
+    const bool is_p = i < pi;
+    const int pmin = is_p ? i : pi, q = is_p ? pi : i;
+    // A_pq on lane p: lane k of pair (k, kp) reads its own column kp
+    cd my_off{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int kp = circle_partner(k, R, NP);
+        if (k < kp && i == k) my_off = A[kp];
+    }
+    X.dg[lane] = dg;
+    wave_sync();
+    const double other_d = X.dg[gbase + pi];
+    // rotation of the pair (p, q) as seen from row p; row q takes row p's
+    // values, so both rows of a pair apply the same J.  Rutishauser's form:
+    // tau = (d - a) / 2|b|, t = sgn(tau) / (|tau| + sqrt(1 + tau^2)),
+    // c = 1 / sqrt(1 + t^2), s = t c, e = b / |b|.  An unrotated pair keeps
+    // c = 1, s = 0, t|b| = 0: the identity.
+    double c = 1.0, tab = 0.0;
+    cd se{0.0, 0.0};  // s e
+    if (is_p && q < nb) {
+        const double a = dg, d = other_d;
+        const cd b = my_off;
+        const double b2 = cnorm(b);
+        if (b2 > 0.0) {
+            const double inv_ab = rsqrt(b2);
+            const double ab = b2 * inv_ab;
+            const double tau = (d - a) * (0.5 * inv_ab);
+            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = rsqrt(1.0 + t * t);
+            const double sv = t * c * inv_ab;
+            se = cd{sv * b.x, sv * b.y};
+            tab = t * ab;
+        }
+    }
+    X.par[lane][0] = c;
+    X.par[lane][1] = se.x;
+    X.par[lane][2] = se.y;
+    X.par[lane][3] = tab;
+    wave_sync();
+    // column update A <- A J, V <- V J for every pair (k, kp) of the round,
+    // with the rotation of lane k
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int kp = circle_partner(k, R, NP);
+        if (k < kp) {
+            const double* pk = X.par[gbase + k];
+            const double ck = pk[0];
+            const cd sek{pk[1], pk[2]};
+            const cd x = A[k], y = A[kp];
+            A[k] = csub(cscale(x, ck), cmul(cconj(sek), y));
+            A[kp] = cadd(cmul(sek, x), cscale(y, ck));
+            const cd vx = V[k], vy = V[kp];
+            V[k] = csub(cscale(vx, ck), cmul(cconj(sek), vy));
+            V[kp] = cadd(cmul(sek, vx), cscale(vy, ck));
+        }
+    }
+    const double* pm = X.par[gbase + pmin];
+    const double myc = pm[0], mytab = pm[3];
+    const cd myse{pm[1], pm[2]};
+    // row update A <- J^H A with the partner's (column-updated) row:
+    //   row p' = c row p - s e row q,  row q' = s conj(e) row p + c row q
+#pragma unroll
+    for (int j = 0; j < NP; ++j) X.row[lane][j] = A[j];
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const cd other = X.row[gbase + pi][j];
+        A[j] = is_p ? csub(cscale(A[j], myc), cmul(myse, other))
+                    : cadd(cmul(cconj(myse), other), cscale(A[j], myc));
+    }
+    wave_sync();
+    // exact values on the 2 x 2 block: a - t|b|, d + t|b|, 0
+    dg = is_p ? dg - mytab : dg + mytab;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int kp = circle_partner(k, R, NP);
+        if (k < kp) {
+            if (i == k) {
+                A[k] = cd{dg, 0.0};
+                A[kp] = cd{0.0, 0.0};
+            } else if (i == kp) {
+                A[kp] = cd{dg, 0.0};
+                A[k] = cd{0.0, 0.0};
+            }
+        }
+    }
+}
+
+template <int NP, int R>
+struct JacobiSweep {
+    static __device__ __forceinline__ void run(cd (&A)[NP], cd (&V)[NP], double& dg, int i,
+                                               int lane, int gbase, int nb, RowLds<NP>& X) {
+        if constexpr (R < NP - 1) {
+            jacobi_round<NP, R>(A, V, dg, i, lane, gbase, nb, X);
+            JacobiSweep<NP, R + 1>::run(A, V, dg, i, lane, gbase, nb, X);
+        }
+    }
+};
+
+template <int NP>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int off = NP / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int NP>
+__global__ void __launch_bounds__(256)
+k_eigh_rows(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __restrict__ w_out,
+            cd* __restrict__ V_out) {
+    constexpr int G = 64 / NP;  // matrices per wavefront
+    __shared__ RowLds<NP> lds[4];
+    const int lane = threadIdx.x & 63;
+    RowLds<NP>& X = lds[threadIdx.x >> 6];
+    const int i = lane % NP;
+    const int gbase = lane - i;
+    const int mat = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * G + lane / NP;
+    const bool live = mat < nmat;
+    const int nb2 = nb * nb;
+    const cd* Mk = M + static_cast<size_t>(live ? mat : 0) * nb2;
+    cd A[NP], V[NP];
+    double dg = 0.0, nrm = 0.0;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        cd h{0.0, 0.0};
+        if (live && i < nb && j < nb) {
+            cd xij = Mk[i * nb + j], xji = Mk[j * nb + i];
+            if (in_mode == 1) {
+                xij = cd{-xij.y, xij.x};
+                xji = cd{-xji.y, xji.x};
+            }
+            h = cd{0.5 * (xij.x + xji.x), 0.5 * (xij.y - xji.y)};
+        }
+        A[j] = h;
+        V[j] = cd{j == i ? 1.0 : 0.0, 0.0};
+        if (j == i) dg = h.x;
+        nrm += cnorm(h);
+    }
+    nrm = group_sum<NP>(nrm);
+    const double eps = 2.220446049250313e-16;
+    const double stop = (4.0 * nb * eps) * (4.0 * nb * eps) * nrm;
+    for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
+        double off = 0.0;
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+            if (j > i) off += cnorm(A[j]);
+        off = group_sum<NP>(off);
+        // every matrix of the wavefront converged (or is empty / 1 x 1)
+        if (__all(!(nrm > 0.0 && nb > 1) || off <= stop)) break;
+        JacobiSweep<NP, 0>::run(A, V, dg, i, lane, gbase, nb, X);
+    }
+    // ascending, stable order of the diagonal over the real rows
+    const double di = dg;
+    int rank = 0;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const double dj = __shfl(di, gbase + j, 64);
+        if (j < nb && j != i && (dj < di || (dj == di && j < i))) ++rank;
+    }
+    if (live && i < nb) w_out[static_cast<size_t>(mat) * nb + rank] = di;
+    cd* Vk = V_out + static_cast<size_t>(live ? mat : 0) * nb2;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const int rj = __shfl(rank, gbase + j, 64);
+        if (live && i < nb && j < nb) Vk[i * nb + rj] = V[j];
+    }
+}
+
 }  // namespace
 
 void launch_eigh(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V, hipStream_t st) {
     if (nb < 1 || nb > kMaxNb)
         throw std::invalid_argument("GPU eigh supports 1 <= nb <= 32");
     if (nmat == 0) return;
+    if (nb <= 8) {
+        const int np = nb <= 2 ? 2 : (nb <= 4 ? 4 : 8);
+        const int per_block = 4 * (64 / np);
+        const dim3 grid(cdiv(nmat, per_block));
+        if (np == 2)
+            hipLaunchKernelGGL(k_eigh_rows<2>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V);
+        else if (np == 4)
+            hipLaunchKernelGGL(k_eigh_rows<4>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V);
+        else
+            hipLaunchKernelGGL(k_eigh_rows<8>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V);
+        HFGPU_LAUNCH_CHECK();
+        return;
+    }
     const int nb2 = nb * nb;
     const size_t nbs = static_cast<size_t>(nb) + 1;
     const size_t lds = sizeof(cd) * (2 * static_cast<size_t>(nb2) + nbs) +
                        2 * sizeof(double) * nbs + 2 * sizeof(int) * nbs;
-    if (nb2 <= 64)
-        hipLaunchKernelGGL(k_eigh_jacobi<1>, dim3(nmat), dim3(64), lds, st, nmat, nb, in_mode, M, w, V);
-    else if (nb2 <= 256)
+    if (nb2 <= 256)
         hipLaunchKernelGGL(k_eigh_jacobi<4>, dim3(nmat), dim3(64), lds, st, nmat, nb, in_mode, M, w, V);
     else
         hipLaunchKernelGGL(k_eigh_jacobi<16>, dim3(nmat), dim3(64), lds, st, nmat, nb, in_mode, M, w, V);

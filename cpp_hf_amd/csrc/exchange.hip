@@ -66,12 +66,12 @@ void AxisPlan::init(int n_) {
                               static_cast<long double>(k) / static_cast<long double>(n);
         tw[k] = cd{static_cast<double>(std::cos(a)), static_cast<double>(std::sin(a))};
     }
-    HFGPU_CHECK(hipMalloc(&d_tw, sizeof(cd) * tw.size()));
+    d_tw = static_cast<cd*>(dev_alloc(sizeof(cd) * tw.size()));
     HFGPU_CHECK(hipMemcpy(d_tw, tw.data(), sizeof(cd) * tw.size(), hipMemcpyHostToDevice));
 }
 
 void AxisPlan::release() {
-    if (d_tw) (void)hipFree(d_tw);
+    if (d_tw) dev_free(d_tw, sizeof(cd) * static_cast<size_t>(n));
     d_tw = nullptr;
 }
 
@@ -153,15 +153,15 @@ void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
     lds_cols = sizeof(cd) * (static_cast<size_t>(nk1) * C + nk1);
     if (lds_rows > 160 * 1024 || lds_cols > 160 * 1024)
         throw std::invalid_argument("k-mesh too large for an LDS-resident FFT line");
-    HFGPU_CHECK(hipMalloc(&d_chan_a, sizeof(int) * nch));
-    HFGPU_CHECK(hipMalloc(&d_chan_b, sizeof(int) * nch));
+    d_chan_a = static_cast<int*>(dev_alloc(sizeof(int) * nch));
+    d_chan_b = static_cast<int*>(dev_alloc(sizeof(int) * nch));
     HFGPU_CHECK(hipMemcpy(d_chan_a, ca.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
     HFGPU_CHECK(hipMemcpy(d_chan_b, cb.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
 }
 
 void ExchangeGeom::release() {
-    if (d_chan_a) (void)hipFree(d_chan_a);
-    if (d_chan_b) (void)hipFree(d_chan_b);
+    if (d_chan_a) dev_free(d_chan_a, sizeof(int) * nch);
+    if (d_chan_b) dev_free(d_chan_b, sizeof(int) * nch);
     d_chan_a = d_chan_b = nullptr;
     ax1.release();
     ax2.release();

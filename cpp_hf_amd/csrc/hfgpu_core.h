@@ -28,12 +28,12 @@ public:
     }
     void alloc(size_t bytes) {
         release();
-        if (bytes) HFGPU_CHECK(hipMalloc(&p_, bytes));
+        if (bytes) p_ = dev_alloc(bytes);
         n_ = bytes;
     }
     void ensure(size_t bytes) { if (bytes > n_) alloc(bytes); }
     void release() {
-        if (p_) (void)hipFree(p_);
+        if (p_) dev_free(p_, n_);
         p_ = nullptr;
         n_ = 0;
     }
@@ -119,22 +119,27 @@ struct GpuKernel {
 extern thread_local std::string g_last_error;
 
 // Run fn, mapping exceptions to hf_status codes: std::invalid_argument ->
-// HF_EINVAL, DeviceError -> HF_EDEVICE, anything else -> HF_ERUNTIME.
+// HF_EINVAL, DeviceError -> HF_EDEVICE, anything else -> HF_ERUNTIME.  On a
+// failure the device is drained before returning, so workspaces that the
+// unwinding returned to the memory pool are idle before anyone reuses them.
 template <class Fn>
 int guarded(Fn&& fn) {
+    int rc = HF_OK;
     try {
         fn();
         return HF_OK;
     } catch (const std::invalid_argument& e) {
         g_last_error = e.what();
-        return HF_EINVAL;
+        rc = HF_EINVAL;
     } catch (const DeviceError& e) {
         g_last_error = e.what();
-        return HF_EDEVICE;
+        rc = HF_EDEVICE;
     } catch (const std::exception& e) {
         g_last_error = e.what();
-        return HF_ERUNTIME;
+        rc = HF_ERUNTIME;
     }
+    (void)hipDeviceSynchronize();
+    return rc;
 }
 
 // True when M (nk, nb, nb) equals its conjugate transpose bit for bit.

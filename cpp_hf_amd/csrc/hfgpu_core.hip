@@ -104,15 +104,69 @@ void host_hermitize(double* M, size_t nk, size_t nb) {
 }
 
 // ---------------------------------------------------------------------------
+// Device memory pool
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct DevPool {
+    std::mutex mu;
+    std::map<std::pair<int, size_t>, std::vector<void*>> free_lists;  // (device, class)
+};
+
+DevPool& dev_pool() {
+    static DevPool* p = new DevPool;  // outlives every static DevBuf
+    return *p;
+}
+
+size_t size_class(size_t bytes) {
+    size_t c = 256;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+
+}  // namespace
+
+void* dev_alloc(size_t bytes) {
+    if (bytes == 0) return nullptr;
+    int dev = 0;
+    HFGPU_CHECK(hipGetDevice(&dev));
+    const size_t cls = size_class(bytes);
+    DevPool& pool = dev_pool();
+    {
+        std::lock_guard<std::mutex> lock(pool.mu);
+        auto it = pool.free_lists.find({dev, cls});
+        if (it != pool.free_lists.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            return p;
+        }
+    }
+    void* p = nullptr;
+    HFGPU_CHECK(hipMalloc(&p, cls));
+    return p;
+}
+
+void dev_free(void* p, size_t bytes) {
+    if (!p) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    DevPool& pool = dev_pool();
+    std::lock_guard<std::mutex> lock(pool.mu);
+    pool.free_lists[{dev, size_class(bytes)}].push_back(p);
+}
+
+// ---------------------------------------------------------------------------
 // GpuKernel
 // ---------------------------------------------------------------------------
 
 static bool dense_vr_packable(const double* VR, size_t nk1, size_t nk2, size_t nb) {
     // Packed exchange for (nb, nb) V needs the index-space kernel W_ab =
     // IFFT(VR_ab) to be real and symmetric in (a, b):
-    //   VR_ab(q) == VR_ba(q)  and  VR_ab(-q) == conj(VR_ab(q)).
+    //   VR_ab(q) == VR_ba(q)  and  VR_ab(-q) == conj(VR_ab(q)),
+    // each to 64 eps of max |VR| (compared as squared magnitudes).
     const size_t nb2 = nb * nb;
-    double vmax = 0.0, err = 0.0;
+    double vmax2 = 0.0, err2 = 0.0;
     for (size_t q1 = 0; q1 < nk1; ++q1)
         for (size_t q2 = 0; q2 < nk2; ++q2) {
             const size_t q = q1 * nk2 + q2;
@@ -122,12 +176,14 @@ static bool dense_vr_packable(const double* VR, size_t nk1, size_t nk2, size_t n
                     const double* v = VR + 2 * (q * nb2 + a * nb + b);
                     const double* vt = VR + 2 * (q * nb2 + b * nb + a);
                     const double* vm = VR + 2 * (mq * nb2 + a * nb + b);
-                    vmax = std::max(vmax, std::hypot(v[0], v[1]));
-                    err = std::max(err, std::hypot(v[0] - vt[0], v[1] - vt[1]));
-                    err = std::max(err, std::hypot(vm[0] - v[0], vm[1] + v[1]));
+                    const double d1r = v[0] - vt[0], d1i = v[1] - vt[1];
+                    const double d2r = vm[0] - v[0], d2i = vm[1] + v[1];
+                    vmax2 = std::max(vmax2, v[0] * v[0] + v[1] * v[1]);
+                    err2 = std::max(err2, std::max(d1r * d1r + d1i * d1i, d2r * d2r + d2i * d2i));
                 }
         }
-    return err <= 64.0 * 2.220446049250313e-16 * vmax;
+    const double tol = 64.0 * 2.220446049250313e-16;
+    return err2 <= tol * tol * vmax2;
 }
 
 GpuKernel::GpuKernel(const hf_kernel_desc& d) {

@@ -19,19 +19,20 @@ struct MuState {
     int pad;
 };
 
-// Workspace of the two-launch reductions: kMuBlocks workgroups x 32 doubles.
-constexpr int kMuBlocks = 512;
-constexpr int kMuPartialDoubles = kMuBlocks * 32;
+// Workspace of the mu solvers (doubles): two partial-sum buffers of up to
+// kMuBlocks workgroups x 32 slots, then two ping-pong states.
+constexpr int kMuBlocks = 240;
+constexpr int kMuPartialDoubles = 16384;
 
-// find_mu_bisection (utils.hpp:49-75): result mu = (lo+hi)/2 left in st->mu.
+// find_mu_bisection (utils.hpp:49-75): result mu = (lo+hi)/2 left in
+// state->mu.  ws: kMuPartialDoubles of workspace.
 void launch_mu_bisection(const double* eps, const double* w, int nk, int nb, double n_e,
-                         double T, int maxiter, MuState* state, double* partial,
-                         hipStream_t st);
+                         double T, int maxiter, MuState* state, double* ws, hipStream_t st);
 // solve_mu_inloop (utils.hpp:79-116); warm start read from state->mu when
 // mu0_from_state, else mu0.
 void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
                       double n_target_norm, double mu0, bool mu0_from_state, double T,
-                      int maxiter, MuState* state, double* partial, hipStream_t st);
+                      int maxiter, MuState* state, double* ws, hipStream_t st);
 
 // occupations from state->mu.
 //   mode 0 (SCF): occ = expit(-(eps - mu) / (T + 1e-12))

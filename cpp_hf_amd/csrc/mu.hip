@@ -4,19 +4,22 @@
 // (utils.hpp:79-116), fermidirac/expit (utils.hpp:18-29).
 //
 // Each solve keeps its scalar state (bracket, mu, done flag) in device memory,
-// so a sequence of launches runs without host round trips.  Every reduction
-// is two launches: a wide kernel writes per-workgroup partials, then a
-// one-workgroup kernel sums them in workgroup order and updates the state, so
-// results are deterministic for a given launch shape and no workgroup ever
-// needs a device-scope fence (on a multi-XCD part each such fence writes back
-// the XCD's L2, which costs far more than the extra launch).
+// so a sequence of launches runs without host round trips.  Every launch
+// writes per-workgroup partial sums; the next launch starts by reducing them
+// in every workgroup, in the same fixed order, so all workgroups derive the
+// same updated state without any cross-workgroup synchronisation (no
+// device-scope fences: on a multi-XCD part each one writes back the XCD's
+// L2), and the results are deterministic for a given launch shape.  Partials
+// and states ping-pong between two buffers; a one-workgroup launch applies
+// the last update.
 //
 // Bisection runs as a multisection: one launch evaluates the electron count
 // at all 2^L - 1 midpoints of the next L bisection levels (the midpoint tree
 // is built with the same (lo + hi) / 2 arithmetic as the sequential loop),
-// then the finalising workgroup walks the tree with the reference's
-// `count > n_e` decisions.  30 bisection steps cost 6 launch pairs instead of
-// 30 dependent reductions, with bit-identical brackets.
+// and the next launch walks the tree with the reference's `count > n_e`
+// decisions.  30 bisection steps take 6 evaluation launches, with brackets
+// bit-identical to the sequential loop's.  Newton (solve_mu_inloop) takes one
+// launch per iteration and stops computing at an exact fixed point.
 #include <algorithm>
 #include <cmath>
 
@@ -30,11 +33,28 @@ constexpr int kBlock = 256;
 constexpr int kMaxLevels = 5;
 constexpr int kMaxNodes = (1 << kMaxLevels) - 1;  // 31
 constexpr int kNodeStride = 32;                   // partial doubles per workgroup
+// workspace layout (doubles): partial buffers 0 and 1, then two states
+constexpr int kPart0 = 0;
+constexpr int kPart1 = 8192;
+constexpr int kStates = 16368;
+constexpr int kStateDoubles = 8;
+static_assert(kPart1 + kMuBlocks * kNodeStride <= kStates, "mu workspace layout");
+static_assert(kStates + 2 * kStateDoubles <= kMuPartialDoubles, "mu workspace size");
+static_assert(sizeof(MuState) <= kStateDoubles * sizeof(double), "MuState slot");
 
+// expit as in utils.hpp:18-25 (1 / (1 + e^-x) for x >= 0, e^x / (1 + e^x)
+// otherwise), written without a branch: both forms share e = exp(-|x|), so
+// lanes of either sign run the same instructions and round identically.
 __device__ __forceinline__ double expit_d(double x) {
-    if (x >= 0.0) return 1.0 / (1.0 + exp(-x));
-    const double e = exp(x);
-    return e / (1.0 + e);
+    const double e = exp(-fabs(x));
+    return (x >= 0.0 ? 1.0 : e) / (1.0 + e);
+}
+
+__device__ __forceinline__ MuState* pp_state(double* ws, int k) {
+    return reinterpret_cast<MuState*>(ws + kStates + (k & 1) * kStateDoubles);
+}
+__device__ __forceinline__ double* pp_part(double* ws, int k) {
+    return ws + ((k & 1) ? kPart1 : kPart0);
 }
 
 // ---- bracket ---------------------------------------------------------------
@@ -56,35 +76,35 @@ k_minmax(const double* __restrict__ eps, size_t n, double* __restrict__ partial)
     }
 }
 
-// mode 0: bisection bracket; mode 1: Newton bracket + clamped warm start.
-__global__ void __launch_bounds__(kBlock)
-k_minmax_final(const double* __restrict__ partial, int nparts, int mode, double T, double mu0,
-               int mu0_from_state, MuState* state) {
-    __shared__ double red[kBlock / 64];
+// Bracket from the min/max partials (identical in every workgroup).
+//   mode 0: bisection bracket (utils.hpp:53-59);
+//   mode 1: Newton bracket + clamped warm start (utils.hpp:84-90).
+__device__ MuState bracket_from(const double* __restrict__ part, int nparts, int mode, double T,
+                                double mu0, double* red) {
     double lo = INFINITY, hi = -INFINITY;
     for (int b = threadIdx.x; b < nparts; b += kBlock) {
-        lo = fmin(lo, partial[2 * b]);
-        hi = fmax(hi, partial[2 * b + 1]);
+        lo = fmin(lo, part[2 * b]);
+        hi = fmax(hi, part[2 * b + 1]);
     }
     const double emin = block_min<kBlock>(lo, red);
     const double emax = block_max<kBlock>(hi, red);
-    if (threadIdx.x != 0) return;
     const double Tj = fmax(T, 1.0e-12);
-    state->emin = emin;
-    state->emax = emax;
+    MuState st{};
+    st.emin = emin;
+    st.emax = emax;
     if (mode == 0) {
         const double span = emax - emin + 10.0 * fmax(Tj, 1.0e-6);
-        state->lo = emin - span;
-        state->hi = emax + span;
-        state->mu = 0.5 * (state->lo + state->hi);
+        st.lo = emin - span;
+        st.hi = emax + span;
+        st.mu = 0.5 * (st.lo + st.hi);
     } else {
         const double l = emin - 50.0 * Tj, h = emax + 50.0 * Tj;
-        const double m0 = mu0_from_state ? state->mu : mu0;
-        state->lo = l;
-        state->hi = h;
-        state->mu = m0 < l ? l : (h < m0 ? h : m0);  // std::clamp
+        st.lo = l;
+        st.hi = h;
+        st.mu = mu0 < l ? l : (h < mu0 ? h : mu0);  // std::clamp
     }
-    state->done = 0;
+    st.done = 0;
+    return st;
 }
 
 // ---- bisection -------------------------------------------------------------
@@ -107,83 +127,163 @@ __device__ void build_tree(double lo, double hi, int nodes, double* s_mid, doubl
     }
 }
 
+// Apply `levels` bisection steps to st from the per-workgroup node counts of
+// the launch that evaluated st's midpoint tree.  Counts are summed in a fixed
+// order (deterministic, identical in every workgroup).
+__device__ MuState bisect_apply(MuState st, const double* __restrict__ part, int nparts,
+                                int levels, double n_e) {
+    constexpr int kGroups = kBlock / kNodeStride;  // 8 row groups of 32 node slots
+    __shared__ double s_mid[kMaxNodes], s_lo[kMaxNodes], s_hi[kMaxNodes];
+    __shared__ double s_grp[kGroups][kNodeStride];
+    __shared__ double s_cnt[kMaxNodes];
+    const int nodes = (1 << levels) - 1;
+    if (threadIdx.x == 0) build_tree(st.lo, st.hi, nodes, s_mid, s_lo, s_hi);
+    const int nd = threadIdx.x % kNodeStride, g = threadIdx.x / kNodeStride;
+    double s = 0.0;
+    if (nd < nodes) {
+        // eight loads in flight per step
+        for (int b0 = g; b0 < nparts; b0 += 8 * kGroups) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int b = b0 + u * kGroups;
+                v[u] = b < nparts ? part[b * kNodeStride + nd] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+    }
+    s_grp[g][nd] = s;
+    __syncthreads();
+    if (threadIdx.x < nodes) {
+        double t = 0.0;
+        for (int gg = 0; gg < kGroups; ++gg) t += s_grp[gg][threadIdx.x];
+        s_cnt[threadIdx.x] = t;
+    }
+    __syncthreads();
+    double l = s_lo[0], h = s_hi[0];
+    int k = 0;
+    for (int lev = 0; lev < levels; ++lev) {
+        const double m = s_mid[k];
+        if (s_cnt[k] > n_e) {
+            h = m;
+            k = 2 * k + 1;
+        } else {
+            l = m;
+            k = 2 * k + 2;
+        }
+    }
+    st.lo = l;
+    st.hi = h;
+    st.mu = 0.5 * (l + h);
+    __syncthreads();  // shared arrays are reused by the caller
+    return st;
+}
+
+// Step j of the bisection: bring the state up to date from the previous
+// launch's partials (the min/max partials for j = 0), record it, then
+// evaluate the electron count at the NODES midpoints of the next levels.
+template <int NODES>
 __global__ void __launch_bounds__(kBlock)
-k_bisect(const double* __restrict__ eps, const double* __restrict__ w, int nb, size_t n,
-         double Tj, int levels, const MuState* __restrict__ state, double* __restrict__ partial) {
+k_bisect_step(const double* __restrict__ eps, const double* __restrict__ w, int nb, size_t n,
+              double n_e, double T, int j, int prev_levels, int prev_parts, double* ws) {
+    __shared__ double red[kBlock / 64];
     __shared__ double s_mid[kMaxNodes], s_lo[kMaxNodes], s_hi[kMaxNodes];
     __shared__ double s_red[kBlock / 64][kNodeStride];
-    const int nodes = (1 << levels) - 1;
-    if (threadIdx.x == 0) build_tree(state->lo, state->hi, nodes, s_mid, s_lo, s_hi);
+    MuState st;
+    if (j == 0)
+        st = bracket_from(pp_part(ws, 1), prev_parts, 0, T, 0.0, red);
+    else
+        st = bisect_apply(*pp_state(ws, j - 1), pp_part(ws, j - 1), prev_parts, prev_levels, n_e);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *pp_state(ws, j) = st;
+    if (threadIdx.x == 0) build_tree(st.lo, st.hi, NODES, s_mid, s_lo, s_hi);
     __syncthreads();
-    double acc[kMaxNodes];
+    double mid[NODES], acc[NODES];
 #pragma unroll
-    for (int nd = 0; nd < kMaxNodes; ++nd) acc[nd] = 0.0;
-    const double denom = Tj + 1.0e-12;  // fermidirac_scalar(x, Tj)
+    for (int nd = 0; nd < NODES; ++nd) {
+        mid[nd] = s_mid[nd];
+        acc[nd] = 0.0;
+    }
+    const double denom = fmax(T, 1.0e-12) + 1.0e-12;  // fermidirac_scalar(x, Tj)
     for (size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * kBlock) {
         const double e = eps[i];
         const double wk = w[i / nb];
 #pragma unroll
-        for (int nd = 0; nd < kMaxNodes; ++nd)
-            if (nd < nodes) acc[nd] += wk * expit_d(-(e - s_mid[nd]) / denom);
+        for (int nd = 0; nd < NODES; ++nd) acc[nd] += wk * expit_d(-(e - mid[nd]) / denom);
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int nd = 0; nd < kMaxNodes; ++nd) {
-        if (nd < nodes) {
-            const double s = wave_sum(acc[nd]);
-            if (lane == 0) s_red[wave][nd] = s;
-        }
+    for (int nd = 0; nd < NODES; ++nd) {
+        const double s = wave_sum(acc[nd]);
+        if (lane == 0) s_red[wave][nd] = s;
     }
     __syncthreads();
-    if (threadIdx.x < nodes) {
+    if (threadIdx.x < NODES) {
         double s = 0.0;
         for (int wv = 0; wv < kBlock / 64; ++wv) s += s_red[wv][threadIdx.x];
-        partial[blockIdx.x * kNodeStride + threadIdx.x] = s;
+        pp_part(ws, j)[blockIdx.x * kNodeStride + threadIdx.x] = s;
     }
 }
 
+// Last bisection levels applied by one workgroup into the caller's state.
 __global__ void __launch_bounds__(kBlock)
-k_bisect_final(const double* __restrict__ partial, int nparts, double n_e, int levels,
-               MuState* state) {
-    __shared__ double s_mid[kMaxNodes], s_lo[kMaxNodes], s_hi[kMaxNodes], s_cnt[kMaxNodes];
-    const int nodes = (1 << levels) - 1;
-    if (threadIdx.x == 0) build_tree(state->lo, state->hi, nodes, s_mid, s_lo, s_hi);
-    // wave v sums nodes v, v + 4, ...; lanes stride over the workgroups
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int nd = wave; nd < nodes; nd += kBlock / 64) {
-        double s = 0.0;
-        for (int b = lane; b < nparts; b += 64) s += partial[b * kNodeStride + nd];
-        s = wave_sum(s);
-        if (lane == 0) s_cnt[nd] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    double l = s_lo[0], h = s_hi[0];
-    int nd = 0;
-    for (int lev = 0; lev < levels; ++lev) {
-        const double m = s_mid[nd];
-        if (s_cnt[nd] > n_e) {
-            h = m;
-            nd = 2 * nd + 1;
-        } else {
-            l = m;
-            nd = 2 * nd + 2;
-        }
-    }
-    state->lo = l;
-    state->hi = h;
-    state->mu = 0.5 * (l + h);
+k_bisect_last(double n_e, int j, int prev_levels, int prev_parts, double* ws, MuState* state) {
+    const MuState st =
+        bisect_apply(*pp_state(ws, j - 1), pp_part(ws, j - 1), prev_parts, prev_levels, n_e);
+    if (threadIdx.x == 0) *state = st;
 }
 
 // ---- Newton ----------------------------------------------------------------
 
+// One Newton update of st from summed N, Z (utils.hpp:105-115); marks an
+// exact fixed point (the remaining iterations would reproduce it) as done.
+__device__ MuState newton_apply(MuState st, const double* __restrict__ part, int nparts,
+                                double n_target, double* red) {
+    double N = 0.0, Z = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) {
+        N += part[2 * b];
+        Z += part[2 * b + 1];
+    }
+    const double Ns = block_sum<kBlock>(N, red);
+    const double Zs = block_sum<kBlock>(Z, red);
+    const double mu = st.mu;
+    double lo = st.lo, hi = st.hi;
+    const double lo0 = lo, hi0 = hi;
+    const double g = Ns - n_target;
+    if (g < 0.0) lo = mu;
+    if (g > 0.0) hi = mu;
+    const double Zsafe = fmax(Zs, 1.0e-18);
+    double mu_new = mu - g / Zsafe;
+    const double mu_bis = 0.5 * (lo + hi);
+    if (mu_new <= lo || mu_new >= hi) mu_new = mu_bis;
+    mu_new = mu_new < lo ? lo : (hi < mu_new ? hi : mu_new);
+    if (!isfinite(mu_new)) mu_new = mu_bis;
+    if (mu_new == mu && lo == lo0 && hi == hi0) st.done = 1;
+    st.lo = lo;
+    st.hi = hi;
+    st.mu = mu_new;
+    return st;
+}
+
+// Step j of solve_mu_inloop: apply iteration j - 1 (or set up the bracket for
+// j = 0), record the state, then sum N and Z at the new mu.
 __global__ void __launch_bounds__(kBlock)
-k_newton(const double* __restrict__ eps, const double* __restrict__ w_norm, int nb, size_t n,
-         double Tj, const MuState* __restrict__ state, double* __restrict__ partial) {
+k_newton_step(const double* __restrict__ eps, const double* __restrict__ w_norm, int nb,
+              size_t n, double n_target, double T, double mu0, int mu0_from_state,
+              const MuState* __restrict__ warm, int j, int prev_parts, double* ws) {
     __shared__ double red[kBlock / 64];
-    if (state->done) return;
-    const double mu = state->mu;
+    MuState st;
+    if (j == 0) {
+        st = bracket_from(pp_part(ws, 1), prev_parts, 1, T, mu0_from_state ? warm->mu : mu0, red);
+    } else {
+        st = *pp_state(ws, j - 1);
+        if (!st.done) st = newton_apply(st, pp_part(ws, j - 1), prev_parts, n_target, red);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *pp_state(ws, j) = st;
+    if (st.done) return;  // uniform over the grid
+    const double Tj = fmax(T, 1.0e-12);
+    const double mu = st.mu;
     double N = 0.0, Z = 0.0;
     for (size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * kBlock) {
@@ -195,42 +295,18 @@ k_newton(const double* __restrict__ eps, const double* __restrict__ w_norm, int 
     N = block_sum<kBlock>(N, red);
     Z = block_sum<kBlock>(Z, red);
     if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = N;
-        partial[2 * blockIdx.x + 1] = Z;
+        double* part = pp_part(ws, j);
+        part[2 * blockIdx.x] = N;
+        part[2 * blockIdx.x + 1] = Z;
     }
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_newton_final(const double* __restrict__ partial, int nparts, double n_target,
-               MuState* state) {
+k_newton_last(double n_target, int j, int prev_parts, int apply, double* ws, MuState* state) {
     __shared__ double red[kBlock / 64];
-    if (state->done) return;
-    double N = 0.0, Z = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += kBlock) {
-        N += partial[2 * b];
-        Z += partial[2 * b + 1];
-    }
-    const double Ns = block_sum<kBlock>(N, red);
-    const double Zs = block_sum<kBlock>(Z, red);
-    if (threadIdx.x != 0) return;
-    const double mu = state->mu;
-    double lo = state->lo, hi = state->hi;
-    const double lo0 = lo, hi0 = hi;
-    const double g = Ns - n_target;
-    if (g < 0.0) lo = mu;
-    if (g > 0.0) hi = mu;
-    const double Zsafe = fmax(Zs, 1.0e-18);
-    double mu_new = mu - g / Zsafe;
-    const double mu_bis = 0.5 * (lo + hi);
-    if (mu_new <= lo || mu_new >= hi) mu_new = mu_bis;
-    mu_new = mu_new < lo ? lo : (hi < mu_new ? hi : mu_new);
-    if (!isfinite(mu_new)) mu_new = mu_bis;
-    // an unchanged (mu, lo, hi) is a fixed point of the iteration: the
-    // remaining iterations would reproduce it exactly.
-    if (mu_new == mu && lo == lo0 && hi == hi0) state->done = 1;
-    state->lo = lo;
-    state->hi = hi;
-    state->mu = mu_new;
+    MuState st = *pp_state(ws, j - 1);
+    if (apply && !st.done) st = newton_apply(st, pp_part(ws, j - 1), prev_parts, n_target, red);
+    if (threadIdx.x == 0) *state = st;
 }
 
 // ---- occupations -----------------------------------------------------------
@@ -256,55 +332,70 @@ __global__ void k_level_shift(const double* __restrict__ eps, size_t n,
     }
 }
 
-unsigned reduce_blocks(size_t n) {
-    return std::max(1u, std::min<unsigned>(cdiv(n, kBlock), static_cast<unsigned>(kMuBlocks)));
-}
-
-void launch_bracket(const double* eps, size_t n, int mode, double T, double mu0,
-                    bool mu0_from_state, MuState* state, double* partial, hipStream_t st) {
-    const unsigned blocks = reduce_blocks(n);
-    hipLaunchKernelGGL(k_minmax, dim3(blocks), dim3(kBlock), 0, st, eps, n, partial);
-    HFGPU_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(kBlock), 0, st, partial,
-                       static_cast<int>(blocks), mode, T, mu0, mu0_from_state ? 1 : 0, state);
-    HFGPU_LAUNCH_CHECK();
+unsigned reduce_blocks(size_t n, unsigned per_thread) {
+    return std::max(1u, std::min<unsigned>(cdiv(n, kBlock * per_thread),
+                                           static_cast<unsigned>(kMuBlocks)));
 }
 
 }  // namespace
 
 void launch_mu_bisection(const double* eps, const double* w, int nk, int nb, double n_e,
-                         double T, int maxiter, MuState* state, double* partial,
-                         hipStream_t st) {
+                         double T, int maxiter, MuState* state, double* ws, hipStream_t st) {
     if (maxiter <= 0) maxiter = 30;
     const size_t n = static_cast<size_t>(nk) * nb;
-    const unsigned blocks = reduce_blocks(n);
-    const double Tj = std::max(T, 1.0e-12);
-    launch_bracket(eps, n, 0, T, 0.0, false, state, partial, st);
-    for (int done = 0; done < maxiter;) {
+    const unsigned blocks = reduce_blocks(n, 1);
+    hipLaunchKernelGGL(k_minmax, dim3(blocks), dim3(kBlock), 0, st, eps, n, ws + kPart1);
+    HFGPU_LAUNCH_CHECK();
+    int j = 0, prev_levels = 0;
+    for (int done = 0; done < maxiter; ++j) {
         const int lv = std::min(kMaxLevels, maxiter - done);
-        hipLaunchKernelGGL(k_bisect, dim3(blocks), dim3(kBlock), 0, st, eps, w, nb, n, Tj, lv,
-                           state, partial);
+        const int pp = static_cast<int>(blocks);
+        switch (lv) {
+            case 1: hipLaunchKernelGGL(k_bisect_step<1>, dim3(blocks), dim3(kBlock), 0, st, eps, w, nb, n, n_e, T, j, prev_levels, pp, ws); break;
+            case 2: hipLaunchKernelGGL(k_bisect_step<3>, dim3(blocks), dim3(kBlock), 0, st, eps, w, nb, n, n_e, T, j, prev_levels, pp, ws); break;
+            case 3: hipLaunchKernelGGL(k_bisect_step<7>, dim3(blocks), dim3(kBlock), 0, st, eps, w, nb, n, n_e, T, j, prev_levels, pp, ws); break;
+            case 4: hipLaunchKernelGGL(k_bisect_step<15>, dim3(blocks), dim3(kBlock), 0, st, eps, w, nb, n, n_e, T, j, prev_levels, pp, ws); break;
+            default: hipLaunchKernelGGL(k_bisect_step<31>, dim3(blocks), dim3(kBlock), 0, st, eps, w, nb, n, n_e, T, j, prev_levels, pp, ws); break;
+        }
         HFGPU_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_bisect_final, dim3(1), dim3(kBlock), 0, st, partial,
-                           static_cast<int>(blocks), n_e, lv, state);
-        HFGPU_LAUNCH_CHECK();
+        prev_levels = lv;
         done += lv;
     }
+    hipLaunchKernelGGL(k_bisect_last, dim3(1), dim3(kBlock), 0, st, n_e, j, prev_levels,
+                       static_cast<int>(blocks), ws, state);
+    HFGPU_LAUNCH_CHECK();
 }
 
 void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
                       double n_target_norm, double mu0, bool mu0_from_state, double T,
-                      int maxiter, MuState* state, double* partial, hipStream_t st) {
+                      int maxiter, MuState* state, double* ws, hipStream_t st) {
     const size_t n = static_cast<size_t>(nk) * nb;
-    const unsigned blocks = reduce_blocks(n);
-    const double Tj = std::max(T, 1.0e-12);
-    launch_bracket(eps, n, 1, T, mu0, mu0_from_state, state, partial, st);
-    for (int it = 0; it < maxiter; ++it) {
-        hipLaunchKernelGGL(k_newton, dim3(blocks), dim3(kBlock), 0, st, eps, w_norm, nb, n, Tj,
-                           state, partial);
+    // a few eigenvalues per thread: each step also re-reduces the previous
+    // step's partials in every workgroup
+    const unsigned blocks = reduce_blocks(n, 4);
+    const unsigned mm_blocks = reduce_blocks(n, 1);
+    hipLaunchKernelGGL(k_minmax, dim3(mm_blocks), dim3(kBlock), 0, st, eps, n, ws + kPart1);
+    HFGPU_LAUNCH_CHECK();
+    int prev_parts = static_cast<int>(mm_blocks);
+    for (int j = 0; j < maxiter; ++j) {
+        hipLaunchKernelGGL(k_newton_step, dim3(blocks), dim3(kBlock), 0, st, eps, w_norm, nb, n,
+                           n_target_norm, T, mu0, mu0_from_state ? 1 : 0, state, j, prev_parts,
+                           ws);
         HFGPU_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_newton_final, dim3(1), dim3(kBlock), 0, st, partial,
-                           static_cast<int>(blocks), n_target_norm, state);
+        prev_parts = static_cast<int>(blocks);
+    }
+    if (maxiter > 0) {
+        hipLaunchKernelGGL(k_newton_last, dim3(1), dim3(kBlock), 0, st, n_target_norm, maxiter,
+                           prev_parts, 1, ws, state);
+        HFGPU_LAUNCH_CHECK();
+    } else {
+        // no iterations: the clamped warm start is the answer (utils.hpp:90)
+        hipLaunchKernelGGL(k_newton_step, dim3(1), dim3(kBlock), 0, st, eps, w_norm, nb,
+                           size_t(0), n_target_norm, T, mu0, mu0_from_state ? 1 : 0, state, 0,
+                           prev_parts, ws);
+        HFGPU_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_newton_last, dim3(1), dim3(kBlock), 0, st, n_target_norm, 1,
+                           0, 0, ws, state);
         HFGPU_LAUNCH_CHECK();
     }
 }

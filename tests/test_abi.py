@@ -120,3 +120,14 @@ def test_electron_count_validation_message():
     with pytest.raises(ValueError, match="physically reachable range"):
         validate_electron_count(np.ones((1, 1)), 2, 3.0)
     validate_electron_count(np.ones((1, 1)), 2, 2.0 + 1e-9)
+
+
+def test_solve_many_validates_arguments():
+    from cpp_hf_amd import solve_many
+
+    with pytest.raises(ValueError, match="same length"):
+        solve_many([1, 2], [1], [1.0, 2.0])
+    with pytest.raises(ValueError, match="method"):
+        solve_many([], [], [], method="newton")
+    with pytest.raises(ValueError, match="concurrency"):
+        solve_many([], [], [], concurrency=0)
