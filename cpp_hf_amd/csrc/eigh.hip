@@ -9,6 +9,7 @@
 // kernel (k_eigh_rows), larger nb the LDS kernel (k_eigh_jacobi), where the
 // matrix and its eigenvector accumulator live in LDS and every lane updates
 // its own elements A'_ij = sum conj(J_ki) A_kl J_lj.
+#include <algorithm>
 #include <stdexcept>
 
 #include "kernels.h"
@@ -23,7 +24,7 @@ constexpr int kMaxSweeps = 40;
 template <int MAXE>
 __global__ void __launch_bounds__(64)
 k_eigh_jacobi(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __restrict__ w_out,
-              cd* __restrict__ V_out) {
+              cd* __restrict__ V_out, BlockMap bm, int sort) {
     extern __shared__ cd smem[];
     const int nb2 = nb * nb;
     const int nbs = nb + 1;
@@ -49,7 +50,8 @@ k_eigh_jacobi(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* _
             xij = cd{-xij.y, xij.x};
             xji = cd{-xji.y, xji.x};
         }
-        const cd h{0.5 * (xij.x + xji.x), 0.5 * (xij.y - xji.y)};
+        cd h{0.5 * (xij.x + xji.x), 0.5 * (xij.y - xji.y)};
+        if (bm.blk[i] != bm.blk[j]) h = cd{0.0, 0.0};  // off-block: ignored
         A[e] = h;
         V[e] = cd{i == j ? 1.0 : 0.0, 0.0};
         nrm += cnorm(h);
@@ -163,11 +165,12 @@ k_eigh_jacobi(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* _
         }
     }
 
-    // ascending, stable order of the diagonal
+    // ascending, stable order of the diagonal (within each block unless sort)
     for (int i = lane; i < nb; i += 64) {
         const double wi = A[i * nb + i].x;
-        int rank = 0;
+        int rank = sort ? 0 : bm.start[i];
         for (int j = 0; j < nb; ++j) {
+            if (!sort && bm.blk[j] != bm.blk[i]) continue;
             const double wj = A[j * nb + j].x;
             rank += (wj < wi) || (wj == wi && j < i);
         }
@@ -330,7 +333,7 @@ __device__ __forceinline__ double group_sum(double v) {
 template <int NP>
 __global__ void __launch_bounds__(256)
 k_eigh_rows(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __restrict__ w_out,
-            cd* __restrict__ V_out) {
+            cd* __restrict__ V_out, BlockMap bm, int sort) {
     constexpr int G = 64 / NP;  // matrices per wavefront
     __shared__ RowLds<NP> lds[4];
     const int lane = threadIdx.x & 63;
@@ -353,6 +356,7 @@ k_eigh_rows(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __r
                 xji = cd{-xji.y, xji.x};
             }
             h = cd{0.5 * (xij.x + xji.x), 0.5 * (xij.y - xji.y)};
+            if (bm.blk[i] != bm.blk[j]) h = cd{0.0, 0.0};  // off-block: ignored
         }
         A[j] = h;
         V[j] = cd{j == i ? 1.0 : 0.0, 0.0};
@@ -372,13 +376,16 @@ k_eigh_rows(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __r
         if (__all(!(nrm > 0.0 && nb > 1) || off <= stop)) break;
         JacobiSweep<NP, 0>::run(A, V, dg, i, lane, gbase, nb, X);
     }
-    // ascending, stable order of the diagonal over the real rows
+    // ascending, stable order of the diagonal over the real rows (within
+    // each block unless sort)
     const double di = dg;
-    int rank = 0;
+    const int bi = i < nb ? bm.blk[i] : -1;
+    int rank = (sort || i >= nb) ? 0 : bm.start[i];
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
         const double dj = __shfl(di, gbase + j, 64);
-        if (j < nb && j != i && (dj < di || (dj == di && j < i))) ++rank;
+        const bool same = sort || (j < nb && bm.blk[j] == bi);
+        if (j < nb && j != i && same && (dj < di || (dj == di && j < i))) ++rank;
     }
     if (live && i < nb) w_out[static_cast<size_t>(mat) * nb + rank] = di;
     cd* Vk = V_out + static_cast<size_t>(live ? mat : 0) * nb2;
@@ -391,20 +398,22 @@ k_eigh_rows(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __r
 
 }  // namespace
 
-void launch_eigh(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V, hipStream_t st) {
+void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V,
+                        const BlockMap& bm, bool sort, hipStream_t st) {
     if (nb < 1 || nb > kMaxNb)
         throw std::invalid_argument("GPU eigh supports 1 <= nb <= 32");
     if (nmat == 0) return;
+    const int so = sort ? 1 : 0;
     if (nb <= 8) {
         const int np = nb <= 2 ? 2 : (nb <= 4 ? 4 : 8);
         const int per_block = 4 * (64 / np);
         const dim3 grid(cdiv(nmat, per_block));
         if (np == 2)
-            hipLaunchKernelGGL(k_eigh_rows<2>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V);
+            hipLaunchKernelGGL(k_eigh_rows<2>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so);
         else if (np == 4)
-            hipLaunchKernelGGL(k_eigh_rows<4>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V);
+            hipLaunchKernelGGL(k_eigh_rows<4>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so);
         else
-            hipLaunchKernelGGL(k_eigh_rows<8>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V);
+            hipLaunchKernelGGL(k_eigh_rows<8>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so);
         HFGPU_LAUNCH_CHECK();
         return;
     }
@@ -413,9 +422,53 @@ void launch_eigh(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V, h
     const size_t lds = sizeof(cd) * (2 * static_cast<size_t>(nb2) + nbs) +
                        2 * sizeof(double) * nbs + 2 * sizeof(int) * nbs;
     if (nb2 <= 256)
-        hipLaunchKernelGGL(k_eigh_jacobi<4>, dim3(nmat), dim3(64), lds, st, nmat, nb, in_mode, M, w, V);
+        hipLaunchKernelGGL(k_eigh_jacobi<4>, dim3(nmat), dim3(64), lds, st, nmat, nb, in_mode, M, w, V, bm, so);
     else
-        hipLaunchKernelGGL(k_eigh_jacobi<16>, dim3(nmat), dim3(64), lds, st, nmat, nb, in_mode, M, w, V);
+        hipLaunchKernelGGL(k_eigh_jacobi<16>, dim3(nmat), dim3(64), lds, st, nmat, nb, in_mode, M, w, V, bm, so);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_eigh(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V, hipStream_t st) {
+    BlockMap bm{};  // one block
+    launch_eigh_blocks(nmat, nb, in_mode, M, w, V, bm, true, st);
+}
+
+namespace {
+
+__global__ void k_max_offblock(int nmat, int nb, const cd* __restrict__ M, BlockMap bm,
+                               double* partial) {
+    __shared__ double red[4];
+    const int nb2 = nb * nb;
+    const size_t n = static_cast<size_t>(nmat) * nb2;
+    double mx = 0.0;
+    for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
+         t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const int e = static_cast<int>(t % nb2);
+        const int i = e / nb, j = e - i * nb;
+        if (bm.blk[i] != bm.blk[j] || bm.blk[i] < 0) mx = fmax(mx, sqrt(cnorm(M[t])));
+    }
+    mx = block_max<256>(mx, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = mx;
+}
+
+__global__ void k_max_final(const double* __restrict__ partial, int n, double* out) {
+    __shared__ double red[4];
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) mx = fmax(mx, partial[i]);
+    mx = block_max<256>(mx, red);
+    if (threadIdx.x == 0) *out = mx;
+}
+
+}  // namespace
+
+void launch_max_offblock(int nmat, int nb, const cd* M, const BlockMap& bm, double* partial,
+                         double* out, hipStream_t st) {
+    const size_t n = static_cast<size_t>(nmat) * nb * nb;
+    const unsigned blocks = std::max(1u, std::min<unsigned>(cdiv(n, 256), 1024u));
+    hipLaunchKernelGGL(k_max_offblock, dim3(blocks), dim3(256), 0, st, nmat, nb, M, bm, partial);
+    HFGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, st, partial, static_cast<int>(blocks),
+                       out);
     HFGPU_LAUNCH_CHECK();
 }
 

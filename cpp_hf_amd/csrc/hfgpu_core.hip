@@ -559,6 +559,60 @@ int hfgpu_eigh_batched(const double* M, double* w, double* V, size_t nmat, size_
     });
 }
 
+// Block map of the given sizes.  exact: the sizes must sum to nb; else
+// indices past their sum get block -1 (off-block to everything, itself
+// included), which is the reference mask's reading of short size lists.
+static BlockMap block_map(size_t nb, const size_t* sizes, size_t nsizes, bool exact = true) {
+    if (nb > 32) throw std::invalid_argument("GPU eigh supports 1 <= nb <= 32");
+    size_t total = 0;
+    for (size_t b = 0; b < nsizes; ++b) total += sizes[b];
+    if (exact ? total != nb : total > nb)
+        throw std::invalid_argument("block sizes must sum to nb");
+    BlockMap bm{};
+    for (size_t i = total; i < nb; ++i) bm.blk[i] = -1;
+    size_t i = 0;
+    for (size_t b = 0; b < nsizes; ++b)
+        for (size_t t = 0; t < sizes[b]; ++t, ++i) {
+            bm.blk[i] = static_cast<signed char>(b);
+            bm.start[i] = static_cast<signed char>(i - t);
+        }
+    return bm;
+}
+
+int hfgpu_eigh_block_sizes(const double* M, double* w, double* V, size_t nmat, size_t nb,
+                           const size_t* sizes, size_t nsizes, int sort) {
+    return guarded([&] {
+        const BlockMap bm = block_map(nb, sizes, nsizes);
+        if (nmat == 0 || nb == 0) return;
+        Stream st;
+        const size_t n = nmat * nb * nb;
+        DevBuf dM(n * sizeof(cd)), dV(n * sizeof(cd)), dw(nmat * nb * sizeof(double));
+        h2d(dM.get(), M, n * sizeof(cd), st.s);
+        launch_eigh_blocks(static_cast<int>(nmat), static_cast<int>(nb), 0, dM.as<cd>(),
+                           dw.as<double>(), dV.as<cd>(), bm, sort != 0, st.s);
+        d2h(w, dw.get(), nmat * nb * sizeof(double), st.s);
+        d2h(V, dV.get(), n * sizeof(cd), st.s);
+        st.sync();
+    });
+}
+
+int hfgpu_max_offblock_sizes(const double* M, size_t nmat, size_t nb, const size_t* sizes,
+                             size_t nsizes, double* out) {
+    return guarded([&] {
+        const BlockMap bm = block_map(nb, sizes, nsizes, false);
+        *out = 0.0;
+        if (nmat == 0 || nb == 0) return;
+        Stream st;
+        const size_t n = nmat * nb * nb;
+        DevBuf dM(n * sizeof(cd)), dp(1024 * sizeof(double)), dout(sizeof(double));
+        h2d(dM.get(), M, n * sizeof(cd), st.s);
+        launch_max_offblock(static_cast<int>(nmat), static_cast<int>(nb), dM.as<cd>(), bm,
+                            dp.as<double>(), dout.as<double>(), st.s);
+        d2h(out, dout.get(), sizeof(double), st.s);
+        st.sync();
+    });
+}
+
 int hfgpu_find_mu_bisection(const double* bands, const double* weights, size_t nk, size_t nb,
                             double n_e, double T, int maxiter, double* mu) {
     return guarded([&] {

@@ -10,6 +10,23 @@ namespace hfgpu {
 // in_mode 0: eigh(herm(M)); in_mode 1: eigh(herm(i M)) (Cayley setup).
 void launch_eigh(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V, hipStream_t st);
 
+// Block-diagonal structure of an nb x nb matrix (nb <= 32): block of each
+// index and the first index of that block.  One block: the plain eigh.
+struct BlockMap {
+    signed char blk[32];
+    signed char start[32];
+};
+// Block-diagonal eigh (linalg.hpp:72-120): the off-block entries are ignored,
+// each block is diagonalised on its own; sort orders all eigenpairs
+// ascending, otherwise each block's ascending pairs stay at the block's
+// position.
+void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V,
+                        const BlockMap& bm, bool sort, hipStream_t st);
+// max |M_ij| over the off-block entries of all nmat matrices (linalg.hpp
+// max_offblock_sizes); partial needs 1024 doubles.
+void launch_max_offblock(int nmat, int nb, const cd* M, const BlockMap& bm, double* partial,
+                         double* out, hipStream_t st);
+
 // ---- chemical potential (mu.hip) -------------------------------------------
 // Device-side state shared by the mu kernels of one solve.
 struct MuState {

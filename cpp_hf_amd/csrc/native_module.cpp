@@ -225,6 +225,42 @@ PYBIND11_MODULE(_native, m) {
         },
         py::arg("M"));
 
+    // --- block-diagonal eigh (cpp_hf_native.cpp:314-358) ---
+    m.def(
+        "eigh_block_sizes",
+        [](CArr M, std::vector<size_t> sizes, bool sort) {
+            if (M.ndim() < 2) throw py::value_error("M needs at least 2 dims");
+            const size_t nb = M.shape(M.ndim() - 1);
+            size_t nmat = 1;
+            for (int i = 0; i < M.ndim() - 2; ++i) nmat *= M.shape(i);
+            auto w = new_r(lead_shape(M, 1));
+            auto V = new_c(lead_shape(M, 0));
+            py::array wa = w, va = V;
+            int rc;
+            {
+                py::gil_scoped_release rel;
+                rc = hfgpu_eigh_block_sizes(cdp(M), dp(wa), dp(va), nmat, nb, sizes.data(),
+                                            sizes.size(), sort ? 1 : 0);
+            }
+            check(rc);
+            return py::make_tuple(w, V);
+        },
+        py::arg("M"), py::arg("sizes"), py::arg("sort") = true);
+    m.def("max_offblock_sizes", [](CArr M, std::vector<size_t> sizes) {
+        if (M.ndim() < 2) throw py::value_error("M needs at least 2 dims");
+        const size_t nb = M.shape(M.ndim() - 1);
+        size_t nmat = 1;
+        for (int i = 0; i < M.ndim() - 2; ++i) nmat *= M.shape(i);
+        double mx = 0.0;
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_max_offblock_sizes(cdp(M), nmat, nb, sizes.data(), sizes.size(), &mx);
+        }
+        check(rc);
+        return mx;
+    });
+
     // --- mu solvers (cpp_hf_native.cpp:361-377) ---
     m.def(
         "find_mu_bisection",

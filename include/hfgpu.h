@@ -75,6 +75,20 @@ int hfgpu_selfenergy_fft_full(const double* VR, const double* P, double* sigma, 
  * ---------------------------------------------------------------------- */
 int hfgpu_eigh_batched(const double* M, double* w, double* V, size_t nmat, size_t nb);
 
+/* Block-diagonal eigh: each diagonal block of the given sizes diagonalised
+ * on its own (off-block entries ignored); sort != 0 orders all eigenpairs
+ * ascending, else each block's ascending pairs stay at the block's position.
+ * Replaces: _native.eigh_block_sizes (cpp/cpp_hf_native.cpp:314-343) ->
+ * eigh_block_sizes_batched (cpp/include/cpp_hf/linalg.hpp:72-113). */
+int hfgpu_eigh_block_sizes(const double* M, double* w, double* V, size_t nmat, size_t nb,
+                           const size_t* sizes, size_t nsizes, int sort);
+
+/* max |M_ij| over the off-block entries (blocks of the given sizes) of all
+ * nmat matrices.  Replaces: _native.max_offblock_sizes
+ * (cpp/cpp_hf_native.cpp:345-358) -> max_offblock_sizes (linalg.hpp:115-130). */
+int hfgpu_max_offblock_sizes(const double* M, size_t nmat, size_t nb, const size_t* sizes,
+                             size_t nsizes, double* out);
+
 /* ------------------------------------------------------------------------
  * Chemical potential by bisection (maxiter <= 0 means 30).
  * Replaces: _native.find_mu_bisection (cpp/cpp_hf_native.cpp:361-377) ->

@@ -372,6 +372,51 @@ void eigh_batched(const c64* M, double* w, c64* V, size_t nk, size_t nb) {
     parallel_for(nk, [&](size_t k) { eigh_one(M + k * nb2, nb, w + k * nb, V + k * nb2); });
 }
 
+// linalg.hpp:72-104: each diagonal block hermitised and diagonalised on its
+// own, placed at its position; sort: all pairs ascending (ties by position).
+void eigh_block_sizes_one(const c64* M, size_t nb, const std::vector<size_t>& sizes, bool sort,
+                          double* w, c64* V) {
+    for (size_t e = 0; e < nb * nb; ++e) V[e] = 0.0;
+    size_t start = 0;
+    for (size_t s : sizes) {
+        std::vector<c64> sub(s * s), Vs(s * s);
+        for (size_t i = 0; i < s; ++i)
+            for (size_t j = 0; j < s; ++j) sub[i * s + j] = M[(start + i) * nb + start + j];
+        eigh_one(sub.data(), s, w + start, Vs.data());
+        for (size_t i = 0; i < s; ++i)
+            for (size_t j = 0; j < s; ++j) V[(start + i) * nb + start + j] = Vs[i * s + j];
+        start += s;
+    }
+    if (!sort) return;
+    std::vector<size_t> order(nb);
+    for (size_t i = 0; i < nb; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return w[a] < w[b]; });
+    std::vector<double> wt(nb);
+    std::vector<c64> Vt(nb * nb);
+    for (size_t c = 0; c < nb; ++c) {
+        wt[c] = w[order[c]];
+        for (size_t r = 0; r < nb; ++r) Vt[r * nb + c] = V[r * nb + order[c]];
+    }
+    std::copy(wt.begin(), wt.end(), w);
+    std::copy(Vt.begin(), Vt.end(), V);
+}
+
+// linalg.hpp:115-130
+double max_offblock_sizes(const c64* M, size_t nk, size_t nb, const std::vector<size_t>& sizes) {
+    std::vector<char> mask(nb * nb, 1);
+    size_t start = 0;
+    for (size_t s : sizes) {
+        for (size_t i = start; i < start + s; ++i)
+            for (size_t j = start; j < start + s; ++j) mask[i * nb + j] = 0;
+        start += s;
+    }
+    double mx = 0.0;
+    for (size_t k = 0; k < nk; ++k)
+        for (size_t e = 0; e < nb * nb; ++e)
+            if (mask[e]) mx = std::max(mx, std::abs(M[k * nb * nb + e]));
+    return mx;
+}
+
 // ---------------------------------------------------------------------------
 // Physics utilities (utils.hpp)
 // ---------------------------------------------------------------------------
@@ -1365,6 +1410,31 @@ int hfcpu_selfenergy_fft_full(const double* VR, const double* P, double* sigma, 
 
 int hfcpu_eigh_batched(const double* M, double* w, double* V, size_t nk, size_t nb) {
     return guarded([&] { eigh_batched(C(M), w, C(V), nk, nb); });
+}
+
+int hfcpu_eigh_block_sizes(const double* M, double* w, double* V, size_t nk, size_t nb,
+                           const size_t* sizes, size_t nsizes, int sort) {
+    return guarded([&] {
+        const std::vector<size_t> sz(sizes, sizes + nsizes);
+        size_t total = 0;
+        for (size_t x : sz) total += x;
+        if (total != nb) throw std::invalid_argument("block sizes must sum to nb");
+        const size_t nb2 = nb * nb;
+        parallel_for(nk, [&](size_t k) {
+            eigh_block_sizes_one(C(M) + k * nb2, nb, sz, sort != 0, w + k * nb, C(V) + k * nb2);
+        });
+    });
+}
+
+int hfcpu_max_offblock_sizes(const double* M, size_t nk, size_t nb, const size_t* sizes,
+                             size_t nsizes, double* out) {
+    return guarded([&] {
+        const std::vector<size_t> sz(sizes, sizes + nsizes);
+        size_t total = 0;
+        for (size_t x : sz) total += x;
+        if (total > nb) throw std::invalid_argument("block sizes must sum to nb");
+        *out = max_offblock_sizes(C(M), nk, nb, sz);
+    });
 }
 
 int hfcpu_find_mu_bisection(const double* bands, const double* weights, size_t nk, size_t nb,

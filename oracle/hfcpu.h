@@ -17,6 +17,10 @@ int hfcpu_selfenergy_fft_full(const double* VR, const double* P, double* sigma, 
                               size_t nk2, size_t nb, size_t dv1, size_t dv2,
                               int apply_ifftshift, int hcp);
 int hfcpu_eigh_batched(const double* M, double* w, double* V, size_t nk, size_t nb);
+int hfcpu_eigh_block_sizes(const double* M, double* w, double* V, size_t nk, size_t nb,
+                           const size_t* sizes, size_t nsizes, int sort);
+int hfcpu_max_offblock_sizes(const double* M, size_t nk, size_t nb, const size_t* sizes,
+                             size_t nsizes, double* out);
 int hfcpu_find_mu_bisection(const double* bands, const double* weights, size_t nk, size_t nb,
                             double n_e, double T, int maxiter, double* mu);
 int hfcpu_solve_mu_inloop(const double* eps, const double* w_norm, size_t nk, size_t nb,

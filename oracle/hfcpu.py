@@ -157,6 +157,29 @@ def eigh_batched(M):
     return w, V
 
 
+def eigh_block_sizes(M, sizes, sort=True):
+    M = _c128(M)
+    nb = M.shape[-1]
+    nk = M.size // (nb * nb)
+    sz = (C.c_size_t * max(len(sizes), 1))(*[int(x) for x in sizes])
+    w = np.empty(M.shape[:-1], dtype=np.float64)
+    V = np.empty_like(M)
+    _check(lib().hfcpu_eigh_block_sizes(_p(M), _p(w), _p(V), _sz(nk), _sz(nb), sz,
+                                        _sz(len(sizes)), C.c_int(1 if sort else 0)))
+    return w, V
+
+
+def max_offblock_sizes(M, sizes):
+    M = _c128(M)
+    nb = M.shape[-1]
+    nk = M.size // (nb * nb)
+    sz = (C.c_size_t * max(len(sizes), 1))(*[int(x) for x in sizes])
+    out = C.c_double(0.0)
+    _check(lib().hfcpu_max_offblock_sizes(_p(M), _sz(nk), _sz(nb), sz, _sz(len(sizes)),
+                                          C.byref(out)))
+    return out.value
+
+
 def find_mu_bisection(bands, weights, n_e, T, maxiter=0):
     bands = _f64(bands)
     weights = _f64(weights)

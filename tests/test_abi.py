@@ -46,7 +46,8 @@ def test_native_module_exports_reference_names():
     for name in ["selfenergy_fft_full", "clear_fft_plan_cache", "eigh_batched",
                  "find_mu_bisection", "build_fock_apply", "solve_scf", "solve_dm",
                  "_cayley_spectral_setup", "_cayley_unitary_from_spectrum",
-                 "_diag_UFU_from_spectrum", "resample_kgrid_2d"]:
+                 "_diag_UFU_from_spectrum", "resample_kgrid_2d", "eigh_block_sizes",
+                 "max_offblock_sizes"]:
         assert hasattr(nat, name), name
 
 

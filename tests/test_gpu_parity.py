@@ -336,3 +336,33 @@ def test_solve_scf_meshgrid_fixture_matches_oracle(gpu, oracle):
     assert abs(g[2] - r[2]) < 1e-10 and abs(g[3] - r[3]) < 1e-10
     np.testing.assert_allclose(g[0], d["P_expected"], rtol=1e-6, atol=1e-6)
     np.testing.assert_allclose(g[1], d["F_expected"], rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("nb,sizes", [(8, (3, 2, 3)), (6, (2, 4)), (12, (4, 4, 4)), (4, (1, 3))])
+@pytest.mark.parametrize("sort", [True, False])
+def test_block_eigh_matches_oracle(gpu, oracle, nb, sizes, sort):
+    rng = np.random.default_rng(nb + len(sizes))
+    X = rng.standard_normal((50, nb, nb)) + 1j * rng.standard_normal((50, nb, nb))
+    M = X + np.conj(np.swapaxes(X, -1, -2))
+    w, V = gpu.eigh_block_sizes(M, list(sizes), sort)
+    wo, Vo = oracle.eigh_block_sizes(M, sizes, sort=sort)
+    np.testing.assert_allclose(w, wo, rtol=0, atol=1e-12 * np.max(np.abs(wo)))
+    starts = np.cumsum((0,) + tuple(sizes[:-1]))
+    B = np.zeros_like(M)
+    for s, n in zip(starts, sizes):
+        B[:, s:s + n, s:s + n] = 0.5 * (M[:, s:s + n, s:s + n]
+                                        + np.conj(np.swapaxes(M[:, s:s + n, s:s + n], -1, -2)))
+    R = V @ (w[..., None] * np.conj(np.swapaxes(V, -1, -2)))
+    np.testing.assert_allclose(R, B, rtol=0, atol=1e-12 * np.max(np.abs(B)))
+    if not sort:
+        for s, n in zip(starts, sizes):  # each block's vectors stay in its block
+            assert np.all(V[:, :s, s:s + n] == 0) and np.all(V[:, s + n:, s:s + n] == 0)
+
+
+def test_max_offblock_matches_oracle(gpu, oracle):
+    rng = np.random.default_rng(9)
+    M = rng.standard_normal((40, 8, 8)) + 1j * rng.standard_normal((40, 8, 8))
+    for sizes in [(8,), (4, 4), (2, 3, 3), (3, 3)]:
+        got = gpu.max_offblock_sizes(M, list(sizes))
+        ref = oracle.max_offblock_sizes(M, sizes)
+        assert abs(got - ref) <= 4e-16 * ref

@@ -143,6 +143,43 @@ def test_oracle_eigh_degenerate_and_diagonal(oracle):
     np.testing.assert_array_equal(w[2], [0.0] * 4)
 
 
+def test_oracle_block_eigh_matches_lapack_per_block(oracle):
+    rng = np.random.default_rng(5)
+    sizes = (3, 2, 3)
+    X = rng.standard_normal((4, 8, 8)) + 1j * rng.standard_normal((4, 8, 8))
+    M = X + np.conj(np.swapaxes(X, -1, -2))  # off-block entries present, ignored
+    starts = np.cumsum((0,) + sizes[:-1])
+    ref_blocks = [np.linalg.eigvalsh(0.5 * (M[:, s:s + n, s:s + n]
+                                            + np.conj(np.swapaxes(M[:, s:s + n, s:s + n], -1, -2))))
+                  for s, n in zip(starts, sizes)]
+    w, V = oracle.eigh_block_sizes(M, sizes, sort=False)
+    for s, n, ref in zip(starts, sizes, ref_blocks):
+        np.testing.assert_allclose(w[:, s:s + n], ref, rtol=0, atol=1e-12)
+        # eigenvectors live in their block only
+        assert np.all(V[:, :s, s:s + n] == 0) and np.all(V[:, s + n:, s:s + n] == 0)
+    ws, Vs = oracle.eigh_block_sizes(M, sizes, sort=True)
+    np.testing.assert_allclose(ws, np.sort(np.concatenate(ref_blocks, axis=1), axis=1),
+                               rtol=0, atol=1e-12)
+    # block-diagonal reconstruction
+    B = np.zeros_like(M)
+    for s, n in zip(starts, sizes):
+        B[:, s:s + n, s:s + n] = M[:, s:s + n, s:s + n]
+    R = Vs @ (ws[..., None] * np.conj(np.swapaxes(Vs, -1, -2)))
+    np.testing.assert_allclose(R, B, rtol=0, atol=1e-12)
+    with pytest.raises(ValueError, match="sum to nb"):
+        oracle.eigh_block_sizes(M, (3, 3), sort=True)
+
+
+def test_oracle_max_offblock_matches_numpy(oracle):
+    rng = np.random.default_rng(6)
+    M = rng.standard_normal((3, 6, 6)) + 1j * rng.standard_normal((3, 6, 6))
+    mask = np.ones((6, 6), dtype=bool)
+    mask[:2, :2] = False
+    mask[2:, 2:] = False
+    ref = np.max(np.abs(M[:, mask]))
+    assert abs(oracle.max_offblock_sizes(M, (2, 4)) - ref) <= 2.3e-16 * ref  # hypot to 1 ulp
+
+
 # ------------------------------------------------------------------- mu -----
 
 def test_oracle_mu_closed_forms(oracle):
