@@ -18,6 +18,8 @@ from .reference_scf import SCFConfig, SCFResult, solve_scf
 from .fock import build_fock, hf_energy, free_energy, occupation_entropy
 from .utils import resample_kgrid
 from .sweep import solve_many
+from .continuation import ContinuationResult, solve_continuation
+from .symmetry import make_project_fn
 
 __all__ = [
     "HartreeFockKernel",
@@ -34,4 +36,7 @@ __all__ = [
     "occupation_entropy",
     "resample_kgrid",
     "solve_many",
+    "ContinuationResult",
+    "solve_continuation",
+    "make_project_fn",
 ]

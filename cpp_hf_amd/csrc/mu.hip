@@ -255,6 +255,15 @@ __device__ MuState newton_apply(MuState st, const double* __restrict__ part, int
     if (g > 0.0) hi = mu;
     const double Zsafe = fmax(Zs, 1.0e-18);
     double mu_new = mu - g / Zsafe;
+    // A nonzero residual whose Newton step rounds away (mu - g/Z == mu) has
+    // moved a bracket edge onto mu, and the safeguard below would replace the
+    // step by a jump to the bracket middle.  The residual's last bits depend on
+    // the summation order (the sequential sum can be exactly zero here, which
+    // keeps mu), so mu is taken as converged instead of jumping.
+    if (g != 0.0 && mu_new == mu) {
+        st.done = 1;
+        return st;
+    }
     const double mu_bis = 0.5 * (lo + hi);
     if (mu_new <= lo || mu_new >= hi) mu_new = mu_bis;
     mu_new = mu_new < lo ? lo : (hi < mu_new ? hi : mu_new);

@@ -162,6 +162,24 @@ def test_mu_newton_matches_oracle(gpu, oracle):
         assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
 
 
+def test_mu_newton_stays_at_converged_mu(gpu, oracle):
+    """Flat bands at low filling (the two-band model of tests/test_continuation.py
+    on 8 x 8 with unit weights): once Newton has converged, the residual is
+    zero or a rounding-level number whose step rounds away; mu must stay put
+    for the remaining iterations instead of jumping to the bracket middle."""
+    eps = np.zeros((8, 8, 2))
+    eps[..., 0], eps[..., 1] = -0.5, 0.5
+    wn = np.full((8, 8), 1.0 / 64)
+    # From mu0 = 0 the sequential sum takes two bisection steps inside a
+    # 1e-12-wide bracket (iterations 6-7) before it lands on a zero residual,
+    # so iterations 6-8 are not compared; both end on the same mu.
+    for mu0, its in ((0.0, (2, 3, 5, 25)), (-1.33, (2, 3, 5, 8, 25))):
+        for it in its:
+            got = gpu.solve_mu_inloop(eps, wn, 1.0 / 64, mu0, 0.2, it)
+            ref = oracle.solve_mu_inloop(eps, wn, 1.0 / 64, mu0, 0.2, it)
+            assert abs(got - ref) <= 1e-14, (mu0, it, got, ref)
+
+
 # --------------------------------------------------------------- build_fock -
 
 def _fock_cases():
