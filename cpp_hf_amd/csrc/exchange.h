@@ -50,6 +50,9 @@ struct ExchangeGeom {
     // both factors at most 16): rows along k2, cols along k1
     bool fast_rows = false, fast_cols = false;
     size_t lds_rows = 0, lds_cols = 0;
+    // two-step launch shape: wavefronts (channel tiles) per row workgroup
+    // (0: the whole row), wavefronts per column workgroup
+    int row_waves = 1, col_waves = 2;
     int* d_chan_a = nullptr;  // nch
     int* d_chan_b = nullptr;  // nch
     AxisPlan ax1, ax2;        // along k1 (nk1) and along k2 (nk2)

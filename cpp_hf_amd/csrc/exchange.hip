@@ -149,6 +149,16 @@ void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
     const bool fast_ok = C == 8 && !(fe && fe[0] == '0');
     fast_rows = fast_ok && fast_size(nk2);
     fast_cols = fast_ok && fast_size(nk1);
+    // launch shape of the two-step passes (measured best at 96 x 96, nb = 8:
+    // one channel tile per row workgroup, two per column workgroup);
+    // HFGPU_ROW_WAVES (0: whole rows) and HFGPU_COL_WAVES override it
+    row_waves = 1;
+    col_waves = 2;
+    if (const char* e = std::getenv("HFGPU_ROW_WAVES")) row_waves = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("HFGPU_COL_WAVES")) {
+        const int v = std::atoi(e);
+        if (v == 1 || v == 2 || v == 4) col_waves = v;
+    }
     lds_rows = sizeof(cd) * (static_cast<size_t>(nk2) * C + nk2);
     lds_cols = sizeof(cd) * (static_cast<size_t>(nk1) * C + nk1);
     if (lds_rows > 160 * 1024 || lds_cols > 160 * 1024)
@@ -217,44 +227,53 @@ __device__ void small_terms_block(const cd* __restrict__ partial, int nrows, int
     cd* rho = smem;
     cd* tmp = rho + nb2;
     cd* acc = tmp + nb2;
-    cd* grp = acc + nb2;   // <= 256 group partials
+    cd* grp = acc + nb2;   // <= 256 group partials (G nch <= blockDim.x <= 256)
     cd* trv = grp + 256;
-    const int G = nch >= kFftThreads ? 1 : kFftThreads / nch;
-    // rows split over G groups per channel (loads batched 8 at a time), then
-    // the groups summed in order
-    for (int t = threadIdx.x; t < G * nch; t += kFftThreads) {
+    const int nt = static_cast<int>(blockDim.x);
+    const int G = nch >= nt ? 1 : nt / nch;
+    // rows split over G groups per channel (loads batched 16 at a time), then
+    // the groups summed in order; with one group a thread owns whole channels
+    for (int t = threadIdx.x; t < G * nch; t += nt) {
         const int ch = t % nch, g = t / nch;
         cd s{0.0, 0.0};
-        for (int r0 = g; r0 < nrows; r0 += 8 * G) {
-            cd v[8];
+        for (int r0 = g; r0 < nrows; r0 += 16 * G) {
+            cd v[16];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 16; ++u) {
                 const int r = r0 + u * G;
                 v[u] = r < nrows ? partial[static_cast<size_t>(r) * nch + ch] : cd{0.0, 0.0};
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) s = cadd(s, v[u]);
+            for (int u = 0; u < 16; ++u) s = cadd(s, v[u]);
         }
-        grp[t] = s;
+        if (G == 1) {
+            const int a = chan_a[ch], b = chan_b[ch];
+            rho[a * nb + b] = s;
+            if (packed && a != b) rho[b * nb + a] = cconj(s);
+        } else {
+            grp[t] = s;
+        }
     }
     __syncthreads();
-    for (int ch = threadIdx.x; ch < nch; ch += kFftThreads) {
-        cd s{0.0, 0.0};
-        for (int g = 0; g < G; ++g) s = cadd(s, grp[g * nch + ch]);
-        const int a = chan_a[ch], b = chan_b[ch];
-        rho[a * nb + b] = s;
-        if (packed && a != b) rho[b * nb + a] = cconj(s);
+    if (G > 1) {
+        for (int ch = threadIdx.x; ch < nch; ch += nt) {
+            cd s{0.0, 0.0};
+            for (int g = 0; g < G; ++g) s = cadd(s, grp[g * nch + ch]);
+            const int a = chan_a[ch], b = chan_b[ch];
+            rho[a * nb + b] = s;
+            if (packed && a != b) rho[b * nb + a] = cconj(s);
+        }
     }
-    for (int e = threadIdx.x; e < nb2; e += kFftThreads) acc[e] = cd{0.0, 0.0};
+    for (int e = threadIdx.x; e < nb2; e += nt) acc[e] = cd{0.0, 0.0};
     __syncthreads();
     if (st.include_hartree) {
-        for (int a = threadIdx.x; a < nb; a += kFftThreads) {
+        for (int a = threadIdx.x; a < nb; a += nt) {
             double s = 0.0;
             for (int b = 0; b < nb; ++b) s += st.HH[a * nb + b] * rho[b * nb + b].x;
             st.hdiag[a] = s;
         }
     } else {
-        for (int a = threadIdx.x; a < nb; a += kFftThreads) st.hdiag[a] = 0.0;
+        for (int a = threadIdx.x; a < nb; a += nt) st.hdiag[a] = 0.0;
     }
     if (st.has_contact) {
         for (int t = 0; t < st.n_contact; ++t) {
@@ -268,14 +287,14 @@ __device__ void small_terms_block(const cd* __restrict__ partial, int nrows, int
                     for (int j = 0; j < nb; ++j) tr = cadd(tr, cmul(oj[i * nb + j], rho[j * nb + i]));
                 trv[0] = tr;
             }
-            for (int e = threadIdx.x; e < nb2; e += kFftThreads) {
+            for (int e = threadIdx.x; e < nb2; e += nt) {
                 const int i = e / nb, j = e % nb;
                 cd s{0.0, 0.0};
                 for (int k = 0; k < nb; ++k) s = cadd(s, cmul(oi[i * nb + k], rho[k * nb + j]));
                 tmp[e] = s;
             }
             __syncthreads();
-            for (int e = threadIdx.x; e < nb2; e += kFftThreads) {
+            for (int e = threadIdx.x; e < nb2; e += nt) {
                 const int i = e / nb, j = e % nb;
                 cd s{0.0, 0.0};
                 for (int k = 0; k < nb; ++k) s = cadd(s, cmul(tmp[i * nb + k], oj[k * nb + j]));
@@ -286,7 +305,7 @@ __device__ void small_terms_block(const cd* __restrict__ partial, int nrows, int
             }
             __syncthreads();
         }
-        for (int e = threadIdx.x; e < nb2; e += kFftThreads) st.SC[e] = acc[e];
+        for (int e = threadIdx.x; e < nb2; e += nt) st.SC[e] = acc[e];
     }
 }
 
@@ -565,7 +584,7 @@ k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog, int nblk, i
 
 // ---------------------------------------------------------------------------
 // Two-step line transforms (N = N1 x N2, both at most 16): one wavefront per
-// tile of kFastLines channels, four tiles per workgroup.
+// tile of kFastLines channels.
 //
 // Forward, x[i1 + N1 i2] -> X[k2 + N2 k1]:
 //   F1  task (l, i1): N2-point DFT over i2 in registers, times W_N^(i1 k2),
@@ -581,11 +600,27 @@ k_rows_inv(FftAxis ax2, int nk1, int nk2, int nb, int nch, int clog, int nblk, i
 // F2, the V(q) product and the inverse I1 run on the same registers.
 // ---------------------------------------------------------------------------
 
+// Workgroup g of a row pass whose k1 rows are each covered by J workgroups
+// -> (row, part).  Workgroups are dealt round-robin to the 8 XCDs (g % 8); the
+// J parts of a row get ids congruent mod 8, so they run on one XCD: the 128 B
+// lines of P, Sigma and F that several parts share meet in one L2.
+__device__ __forceinline__ void split_of(int g, int J, int& row, int& part) {
+    if (J == 1) {
+        row = g;
+        part = 0;
+        return;
+    }
+    const int s = g >> 3;
+    part = s % J;
+    row = (s / J) * 8 + (g & 7);
+}
+
 constexpr int kFastLines = 8;  // channels per tile (the geometry's C)
-constexpr int kFastWaves = 4;  // tiles per workgroup of the column pass
-// The row passes put all nblk channel tiles of one k1 row in one workgroup
-// when nblk <= kFastRowWavesMax and the LDS fits, so every (k1, k2) block
-// of P, Sigma and F is read or written whole by one CU.
+constexpr int kFastWaves = 4;  // most tiles per workgroup of the column pass
+// Most tiles per workgroup of a row pass.  Fewer tiles per workgroup spread a
+// row over more CUs: the epilogues are FP64/integer VALU work at 4 cycles per
+// wavefront instruction, and five tiles on a CU's four SIMDs double the
+// critical path of one of them.  The parts of a row stay on one XCD.
 constexpr int kFastRowWavesMax = 8;
 
 template <int N1, int N2>
@@ -603,7 +638,8 @@ __global__ void __launch_bounds__(64 * kFastRowWavesMax)
 k_rows_fwd_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, int packed,
                 const cd* __restrict__ P, const cd* __restrict__ refP,
                 const cd* __restrict__ refP0, const double* __restrict__ wrow,
-                cd* __restrict__ T, cd* __restrict__ partial, const double* __restrict__ w2d) {
+                cd* __restrict__ T, cd* __restrict__ partial, const double* __restrict__ w2d,
+                int rpw) {
     using Sh = FastShape<N1, N2>;
     constexpr int N = Sh::N, L = Sh::L;
     extern __shared__ cd smem[];
@@ -612,9 +648,11 @@ k_rows_fwd_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     cd* Y = smem + N + wave * Sh::Y;
     cd* s_part = smem + N + nw * Sh::Y + wave * N1 * L;
-    const int tile = blockIdx.x * nw + wave;
-    const bool active = tile < nk1 * nblk;
-    const int k1 = active ? tile / nblk : 0, blk = active ? tile % nblk : 0;
+    // rpw workgroups per k1 row, nw channel tiles each
+    int row, part;
+    split_of(static_cast<int>(blockIdx.x), rpw, row, part);
+    const bool active = row < nk1 && part * nw + wave < nblk;
+    const int k1 = active ? row : 0, blk = active ? part * nw + wave : 0;
     const int nb2 = nb * nb;
     for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
     cd u[Sh::R1][N2];
@@ -720,14 +758,18 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, cd* __restric
         return;
     }
     cd* s_tw = smem;
+    const int nw = blockDim.x >> 6;  // tiles (wavefronts) per workgroup
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     cd* Y = smem + N + wave * Sh::Y;
-    const int tile = (static_cast<int>(blockIdx.x) - has_small) * kFastWaves + wave;
+    // the wavefronts of a workgroup take adjacent columns of one channel
+    // tile, so each k1 row of the workspace is read and written as nw * 128
+    // contiguous bytes
+    const int tile = (static_cast<int>(blockIdx.x) - has_small) * nw + wave;
     const bool active = tile < nk2 * nblk;
-    const int kc = active ? tile / nblk : 0, blk = active ? tile % nblk : 0;
+    const int kc = active ? tile % nk2 : 0, blk = active ? tile / nk2 : 0;
     cd* Tb = T + static_cast<size_t>(blk) * N * nk2 * L;                 // [k1][k2][c]
     const cd* Vb = VR + (static_cast<size_t>(blk) * nk2 + kc) * N * L;   // [q1][c]
-    for (int i = threadIdx.x; i < N; i += 64 * kFastWaves) s_tw[i] = tw[i];
+    for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
     cd u[Sh::R1][N2];
     cd vv[Sh::R2][N1];
 #pragma unroll
@@ -812,7 +854,7 @@ __global__ void __launch_bounds__(64 * kFastRowWavesMax)
 k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, int packed,
                 const cd* __restrict__ T, int epilogue, int shift, double inv_n,
                 cd* __restrict__ Sigma, cd* __restrict__ F, const cd* __restrict__ hh,
-                const cd* __restrict__ SC, const double* __restrict__ hdiag) {
+                const cd* __restrict__ SC, const double* __restrict__ hdiag, int rpw) {
     using Sh = FastShape<N1, N2>;
     constexpr int N = Sh::N, L = Sh::L;
     extern __shared__ cd smem[];
@@ -820,9 +862,11 @@ k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
     const int nw = blockDim.x >> 6;  // tiles (wavefronts) per workgroup
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     cd* Y = smem + N + wave * Sh::Y;
-    const int tile = blockIdx.x * nw + wave;
-    const bool active = tile < nk1 * nblk;
-    const int k1 = active ? tile / nblk : 0, blk = active ? tile % nblk : 0;
+    // rpw workgroups per k1 row, nw channel tiles each
+    int row, part;
+    split_of(static_cast<int>(blockIdx.x), rpw, row, part);
+    const bool active = row < nk1 && part * nw + wave < nblk;
+    const int k1 = active ? row : 0, blk = active ? part * nw + wave : 0;
     const size_t toff = (static_cast<size_t>(blk) * nk1 + k1) * N * L;
     const bool fock = epilogue == EPI_FOCK_PACKED;
     for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
@@ -906,6 +950,8 @@ k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
                 const cd s_ba = cadd(a == b ? sig : cconj(sig), sc_ba);
                 cd f{hv[r][i2].x + 0.5 * (s_ab.x + s_ba.x), hv[r][i2].y + 0.5 * (s_ab.y - s_ba.y)};
                 f.x += hd;
+                // regular stores: the mirrored entries are single 16 B
+                // pieces of 128 B lines, which streaming stores handle badly
                 Sigma[iab] = s_ab;
                 F[iab] = f;
                 if (a != b) {
@@ -1088,21 +1134,29 @@ static unsigned fast_grid(int lines, int nblk, int w) {
     return cdiv(static_cast<size_t>(lines) * nblk, w);
 }
 
-// wavefronts per workgroup of a row pass: one k1 row per workgroup if it fits
+// grid of a row pass: rpw workgroups for each of nk1 rows (split_of order)
+static unsigned row_grid(int nk1, int rpw) {
+    return rpw == 1 ? static_cast<unsigned>(nk1)
+                    : static_cast<unsigned>(cdiv(static_cast<size_t>(nk1), 8) * 8 * rpw);
+}
+
+// wavefronts (channel tiles) per workgroup of a row pass: the whole row if it
+// fits (or g.row_waves when set), at most kFastRowWavesMax and the LDS
 static int row_waves(const ExchangeGeom& g, int n1, bool row_sums) {
-    if (g.nblk <= kFastRowWavesMax && fast_lds(g.nk2, n1, g.nblk, row_sums) <= 160 * 1024)
-        return g.nblk;
-    return kFastWaves;
+    int w = g.row_waves > 0 ? g.row_waves : g.nblk;
+    w = std::min({w, g.nblk, kFastRowWavesMax});
+    while (w > 1 && fast_lds(g.nk2, n1, w, row_sums) > 160 * 1024) --w;
+    return w;
 }
 
 static void launch_rows_fwd_fast(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
 #define HFGPU_FAST_CASE(N, A, B)                                                              \
     if (g.nk2 == N) {                                                                         \
-        const int w = row_waves(g, A, true);                                                  \
-        hipLaunchKernelGGL((k_rows_fwd_fast<A, B>), dim3(fast_grid(g.nk1, g.nblk, w)),        \
-                           dim3(64 * w), fast_lds(N, A, w, true), st, g.ax2.d_tw, g.nk1,      \
-                           g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, a.refP, a.refP0,       \
-                           a.wrow, a.T, a.partial, a.w2d);                                    \
+        const int w = row_waves(g, A, true), rpw = (g.nblk + w - 1) / w;                      \
+        hipLaunchKernelGGL((k_rows_fwd_fast<A, B>), dim3(row_grid(g.nk1, rpw)), dim3(64 * w), \
+                           fast_lds(N, A, w, true), st, g.ax2.d_tw, g.nk1, g.nb, g.nch,       \
+                           g.nblk, g.packed ? 1 : 0, a.P, a.refP, a.refP0, a.wrow, a.T,       \
+                           a.partial, a.w2d, rpw);                                            \
         HFGPU_LAUNCH_CHECK();                                                                 \
         return;                                                                               \
     }
@@ -1113,10 +1167,11 @@ static void launch_rows_fwd_fast(const ExchangeGeom& g, const ExchangeArgs& a, h
 
 static void launch_cols_fast(const ExchangeGeom& g, const ExchangeArgs& a, int hs,
                              hipStream_t st) {
-    const dim3 grid(fast_grid(g.nk2, g.nblk, kFastWaves) + hs), block(64 * kFastWaves);
+    const int cw = g.col_waves;
+    const dim3 grid(fast_grid(g.nk2, g.nblk, cw) + hs), block(64 * cw);
 #define HFGPU_FAST_CASE(N, A, B)                                                              \
     if (g.nk1 == N) {                                                                         \
-        const size_t lds = std::max(fast_lds(N, A, kFastWaves, false),                                    \
+        const size_t lds = std::max(fast_lds(N, A, cw, false),                                \
                                     hs ? sizeof(cd) * small_terms_cd(g.nb) : size_t(0));      \
         hipLaunchKernelGGL((k_cols_fast<A, B>), grid, block, lds, st, g.ax1.d_tw, g.nk2,      \
                            g.nch, g.nblk, a.T, a.VR, a.vr_scalar ? 1 : 0, hs, a.partial,      \
@@ -1133,11 +1188,11 @@ static void launch_rows_inv_fast(const ExchangeGeom& g, const ExchangeArgs& a, d
                                  hipStream_t st) {
 #define HFGPU_FAST_CASE(N, A, B)                                                              \
     if (g.nk2 == N) {                                                                         \
-        const int w = row_waves(g, A, false);                                                 \
-        hipLaunchKernelGGL((k_rows_inv_fast<A, B>), dim3(fast_grid(g.nk1, g.nblk, w)),        \
-                           dim3(64 * w), fast_lds(N, A, w, false), st, g.ax2.d_tw, g.nk1,     \
-                           g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.T, a.epilogue,            \
-                           a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh, a.SC, a.hdiag);    \
+        const int w = row_waves(g, A, false), rpw = (g.nblk + w - 1) / w;                     \
+        hipLaunchKernelGGL((k_rows_inv_fast<A, B>), dim3(row_grid(g.nk1, rpw)), dim3(64 * w), \
+                           fast_lds(N, A, w, false), st, g.ax2.d_tw, g.nk1, g.nb, g.nch,      \
+                           g.nblk, g.packed ? 1 : 0, a.T, a.epilogue, a.ifftshift ? 1 : 0,    \
+                           inv_n, a.Sigma, a.F, a.hh, a.SC, a.hdiag, rpw);                    \
         HFGPU_LAUNCH_CHECK();                                                                 \
         return;                                                                               \
     }

@@ -1,0 +1,43 @@
+"""The two-step exchange passes under every launch shape they accept: row
+passes with 1 to 5 channel tiles per workgroup (the parts of one k1 row on
+one XCD) or the whole row, and 1, 2 or 4 wavefronts (adjacent columns of one
+tile) per column workgroup.  Each shape must give the oracle's Fock build."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import herm, kernel_args_dict, rand_c
+
+pytestmark = pytest.mark.gpu
+
+# (tiles per row workgroup, 0 = whole row; wavefronts per column workgroup)
+SHAPES = [("0", "4"), ("1", "1"), ("2", "2"), ("3", "4"), ("5", "2"), ("4", "1")]
+
+
+def _cases():
+    from cpp_hf_amd.workloads import layered_kernel, qwz_kernel
+
+    wl = layered_kernel(96, 4)
+    yield "layered96_nb8", wl.kernel, herm(wl.P0)
+    wl = qwz_kernel(24)
+    yield "qwz24", wl.kernel, herm(wl.P0)
+    wl = layered_kernel(24, 2)
+    yield "layered24_nonherm", wl.kernel, rand_c(np.random.default_rng(9), wl.kernel.h.shape)
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=["-".join(s) for s in SHAPES])
+def test_launch_shapes_match_oracle(gpu, oracle, shape, monkeypatch):
+    # read when a kernel's exchange geometry is built
+    monkeypatch.setenv("HFGPU_ROW_WAVES", shape[0])
+    monkeypatch.setenv("HFGPU_COL_WAVES", shape[1])
+    for name, k, P in _cases():
+        print(f"launch shape {shape} case {name}", flush=True)
+        ka = kernel_args_dict(k)
+        S, H, F, E = gpu.build_fock_apply(ka, P, None)
+        Sr, Hr, Fr, Er = oracle.build_fock_apply(ka, P)
+        scale = float(np.max(np.abs(Fr)))
+        assert float(np.max(np.abs(F - Fr))) / scale < 1e-12, name
+        assert float(np.max(np.abs(S - Sr))) / max(1e-300, float(np.max(np.abs(Sr)))) < 1e-12
+        assert abs(E - Er) <= 1e-12 * max(1.0, abs(Er))

@@ -199,10 +199,16 @@ def _fock_cases():
     wl = layered_kernel(24, 2)
     rng = np.random.default_rng(5)
     out.append(("layered24_nonherm", wl.kernel, rand_c(rng, wl.kernel.h.shape)))
+    # more channels than the small-terms reduction has threads: 300 packed
+    # (nb = 24) and 289 full (nb = 17, non-Hermitian density)
+    k, P = hartree_kernel(nk=4, nb=24, seed=7)
+    out.append(("hartree_nb24_packed", k, P))
+    k, _ = hartree_kernel(nk=4, nb=17, seed=8)
+    out.append(("hartree_nb17_full", k, rand_c(rng, k.h.shape)))
     return out
 
 
-@pytest.mark.parametrize("case", range(6))
+@pytest.mark.parametrize("case", range(8))
 def test_build_fock_matches_oracle(gpu, oracle, case):
     name, k, P = _fock_cases()[case]
     print(f"build_fock case {name}", flush=True)
