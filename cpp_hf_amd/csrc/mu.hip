@@ -17,11 +17,15 @@
 // at all 2^L - 1 midpoints of the next L bisection levels (the midpoint tree
 // is built with the same (lo + hi) / 2 arithmetic as the sequential loop),
 // and the next launch walks the tree with the reference's `count > n_e`
-// decisions.  30 bisection steps take 6 evaluation launches, with brackets
-// bit-identical to the sequential loop's.  Newton (solve_mu_inloop) takes one
+// decisions.  Each midpoint costs an expit per eigenvalue and each launch a
+// fixed latency; L = 3 (7 midpoints, 30 steps in 10 launches) measured fastest
+// at 96 x 96 x 8 (93 us of kernel time against 132 us for L = 5 and 167 us for
+// L = 1); HFGPU_BISECT_LEVELS overrides it.  Brackets are bit-identical to the
+// sequential loop's for every L.  Newton (solve_mu_inloop) takes one
 // launch per iteration and stops computing at an exact fixed point.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "kernels.h"
 
@@ -355,9 +359,14 @@ void launch_mu_bisection(const double* eps, const double* w, int nk, int nb, dou
     const unsigned blocks = reduce_blocks(n, 1);
     hipLaunchKernelGGL(k_minmax, dim3(blocks), dim3(kBlock), 0, st, eps, n, ws + kPart1);
     HFGPU_LAUNCH_CHECK();
+    static const int max_levels = [] {
+        const char* e = std::getenv("HFGPU_BISECT_LEVELS");
+        const int v = e ? std::atoi(e) : 3;
+        return v >= 1 && v <= kMaxLevels ? v : 3;
+    }();
     int j = 0, prev_levels = 0;
     for (int done = 0; done < maxiter; ++j) {
-        const int lv = std::min(kMaxLevels, maxiter - done);
+        const int lv = std::min(max_levels, maxiter - done);
         const int pp = static_cast<int>(blocks);
         switch (lv) {
             case 1: hipLaunchKernelGGL(k_bisect_step<1>, dim3(blocks), dim3(kBlock), 0, st, eps, w, nb, n, n_e, T, j, prev_levels, pp, ws); break;
