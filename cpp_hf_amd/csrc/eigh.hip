@@ -29,7 +29,11 @@ constexpr int kMaxSweeps = 40;
 template <int MAXE>
 __global__ void __launch_bounds__(64)
 k_eigh_jacobi(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __restrict__ w_out,
-              cd* __restrict__ V_out, BlockMap bm, int sort) {
+              cd* __restrict__ V_out, BlockMap bm, int sort, const double* __restrict__ cond) {
+    // the block structure applies unless cond = (off-block max, max) says the
+    // batch is not block-diagonal (solver_scf.hpp:111-120): then a dense eigh
+    const bool ub = !cond || cond[0] <= 1.0e-10 + 1.0e-10 * cond[1];
+    if (!ub) sort = 1;
     extern __shared__ cd smem[];
     const int nb2 = nb * nb;
     const int nbs = nb + 1;
@@ -56,7 +60,7 @@ k_eigh_jacobi(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* _
             xji = cd{-xji.y, xji.x};
         }
         cd h{0.5 * (xij.x + xji.x), 0.5 * (xij.y - xji.y)};
-        if (bm.blk[i] != bm.blk[j]) h = cd{0.0, 0.0};  // off-block: ignored
+        if (ub && bm.blk[i] != bm.blk[j]) h = cd{0.0, 0.0};  // off-block: ignored
         A[e] = h;
         V[e] = cd{i == j ? 1.0 : 0.0, 0.0};
         nrm += cnorm(h);
@@ -235,7 +239,11 @@ __device__ __forceinline__ cd group_bcast(cd v, int src) {
 template <int NP>
 __global__ void __launch_bounds__(256)
 k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __restrict__ w_out,
-           cd* __restrict__ V_out, BlockMap bm, int sort) {
+           cd* __restrict__ V_out, BlockMap bm, int sort, const double* __restrict__ cond) {
+    // the block structure applies unless cond = (off-block max, max) says the
+    // batch is not block-diagonal (solver_scf.hpp:111-120): then a dense eigh
+    const bool ub = !cond || cond[0] <= 1.0e-10 + 1.0e-10 * cond[1];
+    if (!ub) sort = 1;
     constexpr int G = 64 / NP;  // matrices per wavefront
     constexpr double kEps = 2.220446049250313e-16;
     __shared__ TriLds<NP> lds[4];
@@ -258,7 +266,7 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
                 xji = cd{-xji.y, xji.x};
             }
             h = cd{0.5 * (xij.x + xji.x), 0.5 * (xij.y - xji.y)};
-            if (bm.blk[i] != bm.blk[j]) h = cd{0.0, 0.0};  // off-block: ignored
+            if (ub && bm.blk[i] != bm.blk[j]) h = cd{0.0, 0.0};  // off-block: ignored
         }
         A[j] = h;
         Z[j] = cd{j == i ? 1.0 : 0.0, 0.0};
@@ -442,7 +450,7 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
 }  // namespace
 
 void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V,
-                        const BlockMap& bm, bool sort, hipStream_t st) {
+                        const BlockMap& bm, bool sort, hipStream_t st, const double* cond) {
     if (nb < 1 || nb > kMaxNb)
         throw std::invalid_argument("GPU eigh supports 1 <= nb <= 32");
     if (nmat == 0) return;
@@ -452,13 +460,13 @@ void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, c
         const int per_block = 4 * (64 / np);
         const dim3 grid(cdiv(nmat, per_block));
         if (np == 2)
-            hipLaunchKernelGGL(k_eigh_tri<2>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so);
+            hipLaunchKernelGGL(k_eigh_tri<2>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
         else if (np == 4)
-            hipLaunchKernelGGL(k_eigh_tri<4>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so);
+            hipLaunchKernelGGL(k_eigh_tri<4>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
         else if (np == 8)
-            hipLaunchKernelGGL(k_eigh_tri<8>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so);
+            hipLaunchKernelGGL(k_eigh_tri<8>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
         else
-            hipLaunchKernelGGL(k_eigh_tri<16>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so);
+            hipLaunchKernelGGL(k_eigh_tri<16>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
         HFGPU_LAUNCH_CHECK();
         return;
     }
@@ -467,9 +475,9 @@ void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, c
     const size_t lds = sizeof(cd) * (2 * static_cast<size_t>(nb2) + nbs) +
                        2 * sizeof(double) * nbs + 2 * sizeof(int) * nbs;
     if (nb2 <= 256)
-        hipLaunchKernelGGL(k_eigh_jacobi<4>, dim3(nmat), dim3(64), lds, st, nmat, nb, in_mode, M, w, V, bm, so);
+        hipLaunchKernelGGL(k_eigh_jacobi<4>, dim3(nmat), dim3(64), lds, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
     else
-        hipLaunchKernelGGL(k_eigh_jacobi<16>, dim3(nmat), dim3(64), lds, st, nmat, nb, in_mode, M, w, V, bm, so);
+        hipLaunchKernelGGL(k_eigh_jacobi<16>, dim3(nmat), dim3(64), lds, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
     HFGPU_LAUNCH_CHECK();
 }
 
@@ -480,28 +488,43 @@ void launch_eigh(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V, h
 
 namespace {
 
+// Per-workgroup maxima of |M| over the off-block entries and over all
+// entries (the scale of the block-structure test).
 __global__ void k_max_offblock(int nmat, int nb, const cd* __restrict__ M, BlockMap bm,
                                double* partial) {
     __shared__ double red[4];
     const int nb2 = nb * nb;
     const size_t n = static_cast<size_t>(nmat) * nb2;
-    double mx = 0.0;
+    double mx = 0.0, ma = 0.0;
     for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
          t += static_cast<size_t>(gridDim.x) * blockDim.x) {
         const int e = static_cast<int>(t % nb2);
         const int i = e / nb, j = e - i * nb;
-        if (bm.blk[i] != bm.blk[j] || bm.blk[i] < 0) mx = fmax(mx, sqrt(cnorm(M[t])));
+        const double a = sqrt(cnorm(M[t]));
+        if (bm.blk[i] != bm.blk[j] || bm.blk[i] < 0) mx = fmax(mx, a);
+        ma = fmax(ma, a);
     }
     mx = block_max<256>(mx, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = mx;
+    ma = block_max<256>(ma, red);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = mx;
+        partial[2 * blockIdx.x + 1] = ma;
+    }
 }
 
 __global__ void k_max_final(const double* __restrict__ partial, int n, double* out) {
     __shared__ double red[4];
-    double mx = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) mx = fmax(mx, partial[i]);
+    double mx = 0.0, ma = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        mx = fmax(mx, partial[2 * i]);
+        ma = fmax(ma, partial[2 * i + 1]);
+    }
     mx = block_max<256>(mx, red);
-    if (threadIdx.x == 0) *out = mx;
+    ma = block_max<256>(ma, red);
+    if (threadIdx.x == 0) {
+        out[0] = mx;
+        out[1] = ma;
+    }
 }
 
 }  // namespace
@@ -509,7 +532,8 @@ __global__ void k_max_final(const double* __restrict__ partial, int n, double* o
 void launch_max_offblock(int nmat, int nb, const cd* M, const BlockMap& bm, double* partial,
                          double* out, hipStream_t st) {
     const size_t n = static_cast<size_t>(nmat) * nb * nb;
-    const unsigned blocks = std::max(1u, std::min<unsigned>(cdiv(n, 256), 1024u));
+    const unsigned blocks =
+        std::max(1u, std::min<unsigned>(cdiv(n, 256), kMaxOffblockBlocks));
     hipLaunchKernelGGL(k_max_offblock, dim3(blocks), dim3(256), 0, st, nmat, nb, M, bm, partial);
     HFGPU_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, st, partial, static_cast<int>(blocks),

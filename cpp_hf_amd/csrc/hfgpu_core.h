@@ -69,7 +69,8 @@ struct GpuKernel {
     bool hartree = false, exchange = true, hcp = false, has_refP = false, contact = false;
     bool dense_packable = false;  // (nb, nb) V with a real, band-symmetric kernel
     double T = 0.0, weight_sum = 0.0;
-    std::vector<double> w_host;  // (nk) weights, host copy
+    std::vector<double> w_host;   // (nk) weights, host copy
+    std::vector<double> cg_host;  // (n_contact) contact strengths, host copy
     hipStream_t stream = nullptr;
 
     DevBuf h, refP, w2d, HH, cg, Oi, Oj;
@@ -114,6 +115,10 @@ struct GpuKernel {
     };
     std::vector<GraphEntry> graphs;
 };
+
+// Block map of block sizes summing to nb (exact) or at most nb (indices past
+// the last block belong to no block); throws std::invalid_argument otherwise.
+BlockMap block_map(size_t nb, const size_t* sizes, size_t nsizes, bool exact = true);
 
 // Last error message of the calling thread (hfgpu_last_error).
 extern thread_local std::string g_last_error;

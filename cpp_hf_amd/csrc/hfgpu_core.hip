@@ -234,6 +234,7 @@ GpuKernel::GpuKernel(const hf_kernel_desc& d) {
     if (n_contact > 0) {
         cg.alloc(n_contact * sizeof(double));
         h2d(cg.get(), d.contact_g, n_contact * sizeof(double), stream);
+        cg_host.assign(d.contact_g, d.contact_g + n_contact);
         Oi.alloc(n_contact * nb2 * cz);
         Oj.alloc(n_contact * nb2 * cz);
         h2d(Oi.get(), d.contact_Oi, n_contact * nb2 * cz, stream);
@@ -466,6 +467,26 @@ struct Stream {
 
 void clear_geometry_cache() { geom_cache().clear(); }
 
+// Block map of the given sizes.  exact: the sizes must sum to nb; else
+// indices past their sum get block -1 (off-block to everything, itself
+// included), which is the reference mask's reading of short size lists.
+BlockMap block_map(size_t nb, const size_t* sizes, size_t nsizes, bool exact) {
+    if (nb > 32) throw std::invalid_argument("GPU eigh supports 1 <= nb <= 32");
+    size_t total = 0;
+    for (size_t b = 0; b < nsizes; ++b) total += sizes[b];
+    if (exact ? total != nb : total > nb)
+        throw std::invalid_argument("block sizes must sum to nb");
+    BlockMap bm{};
+    for (size_t i = total; i < nb; ++i) bm.blk[i] = -1;
+    size_t i = 0;
+    for (size_t b = 0; b < nsizes; ++b)
+        for (size_t t = 0; t < sizes[b]; ++t, ++i) {
+            bm.blk[i] = static_cast<signed char>(b);
+            bm.start[i] = static_cast<signed char>(i - t);
+        }
+    return bm;
+}
+
 }  // namespace hfgpu
 
 using namespace hfgpu;
@@ -559,25 +580,6 @@ int hfgpu_eigh_batched(const double* M, double* w, double* V, size_t nmat, size_
     });
 }
 
-// Block map of the given sizes.  exact: the sizes must sum to nb; else
-// indices past their sum get block -1 (off-block to everything, itself
-// included), which is the reference mask's reading of short size lists.
-static BlockMap block_map(size_t nb, const size_t* sizes, size_t nsizes, bool exact = true) {
-    if (nb > 32) throw std::invalid_argument("GPU eigh supports 1 <= nb <= 32");
-    size_t total = 0;
-    for (size_t b = 0; b < nsizes; ++b) total += sizes[b];
-    if (exact ? total != nb : total > nb)
-        throw std::invalid_argument("block sizes must sum to nb");
-    BlockMap bm{};
-    for (size_t i = total; i < nb; ++i) bm.blk[i] = -1;
-    size_t i = 0;
-    for (size_t b = 0; b < nsizes; ++b)
-        for (size_t t = 0; t < sizes[b]; ++t, ++i) {
-            bm.blk[i] = static_cast<signed char>(b);
-            bm.start[i] = static_cast<signed char>(i - t);
-        }
-    return bm;
-}
 
 int hfgpu_eigh_block_sizes(const double* M, double* w, double* V, size_t nmat, size_t nb,
                            const size_t* sizes, size_t nsizes, int sort) {
@@ -604,7 +606,8 @@ int hfgpu_max_offblock_sizes(const double* M, size_t nmat, size_t nb, const size
         if (nmat == 0 || nb == 0) return;
         Stream st;
         const size_t n = nmat * nb * nb;
-        DevBuf dM(n * sizeof(cd)), dp(1024 * sizeof(double)), dout(sizeof(double));
+        DevBuf dM(n * sizeof(cd)), dp(2 * kMaxOffblockBlocks * sizeof(double)),
+            dout(2 * sizeof(double));
         h2d(dM.get(), M, n * sizeof(cd), st.s);
         launch_max_offblock(static_cast<int>(nmat), static_cast<int>(nb), dM.as<cd>(), bm,
                             dp.as<double>(), dout.as<double>(), st.s);

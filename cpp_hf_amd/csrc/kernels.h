@@ -19,11 +19,18 @@ struct BlockMap {
 // Block-diagonal eigh (linalg.hpp:72-120): the off-block entries are ignored,
 // each block is diagonalised on its own; sort orders all eigenpairs
 // ascending, otherwise each block's ascending pairs stay at the block's
-// position.
+// position.  With cond (two device doubles from launch_max_offblock: the
+// off-block maximum and the overall maximum of M), the blocks apply only when
+// off <= 1e-10 + 1e-10 max (solver_scf.hpp:111-120); otherwise the call is
+// the plain sorted eigh.  The test runs on the device: no host round trip.
 void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V,
-                        const BlockMap& bm, bool sort, hipStream_t st);
-// max |M_ij| over the off-block entries of all nmat matrices (linalg.hpp
-// max_offblock_sizes); partial needs 1024 doubles.
+                        const BlockMap& bm, bool sort, hipStream_t st,
+                        const double* cond = nullptr);
+// out[0] = max |M_ij| over the off-block entries of all nmat matrices
+// (linalg.hpp:123-140, max_offblock_sizes), out[1] = max |M_ij| over all
+// entries (linalg.hpp:142-146, max_abs); partial needs
+// 2 kMaxOffblockBlocks doubles.
+constexpr unsigned kMaxOffblockBlocks = 1024;
 void launch_max_offblock(int nmat, int nb, const cd* M, const BlockMap& bm, double* partial,
                          double* out, hipStream_t st);
 

@@ -57,8 +57,8 @@ void validate_electron_count(const std::vector<double>& w, size_t nb, double n_e
                                     std::to_string(max_e) + "].");
 }
 
-// block_sizes validation (solver_dm.hpp:80-90); the dense path is used for
-// every accepted block structure.
+// block_sizes validation of the direct-minimisation solver
+// (solver_dm.hpp:80-90).
 void validate_block_sizes(const size_t* sizes, size_t n, size_t nb) {
     if (n == 0) return;
     size_t total = 0;
@@ -67,6 +67,18 @@ void validate_block_sizes(const size_t* sizes, size_t n, size_t nb) {
         total += sizes[i];
     }
     if (total != nb) throw std::invalid_argument("block_sizes must sum to nb.");
+}
+
+// Block structure of a batch of nb x nb matrices (solver_dm.hpp:92-98): the
+// off-block maximum at most 1e-10 + 1e-10 max |M|.  Reduced on the device,
+// decided on the host (a one-time check at the start of a solve).
+bool block_ok_host(const cd* M, int nmat, int nb, const BlockMap& bm, hipStream_t st) {
+    DevBuf part(2 * kMaxOffblockBlocks * sizeof(double)), out(2 * sizeof(double));
+    launch_max_offblock(nmat, nb, M, bm, part.as<double>(), out.as<double>(), st);
+    double h[2];
+    d2h(h, out.get(), sizeof(h), st);
+    HFGPU_CHECK(hipStreamSynchronize(st));
+    return h[0] <= 1.0e-10 + 1.0e-10 * h[1];
 }
 
 // Workspace shared by both solvers.
@@ -148,21 +160,44 @@ struct ScfState {
     DevBuf eps, shifted, occ, V;
     Projector& proj;
     double n_e, level_shift;
+    // block_sizes summing to nb with more than one entry (solver_scf.hpp:
+    // 107-110; other values leave the dense eigh, without an error)
+    bool blocks = false;
+    BlockMap bm{};
+    DevBuf bpart, bcond;
 
-    ScfState(GpuKernel& K_, Projector& pr, double ne, double ls)
+    ScfState(GpuKernel& K_, Projector& pr, double ne, double ls, const size_t* sizes,
+             size_t nsizes)
         : K(K_), st(K_.stream), ws(K_.stream), proj(pr), n_e(ne), level_shift(ls) {
         const size_t nkb = K.nk * K.nb;
         eps.alloc(nkb * sizeof(double));
         shifted.alloc(nkb * sizeof(double));
         occ.alloc(nkb * sizeof(double));
         V.alloc(K.nd * sizeof(cd));
+        size_t total = 0;
+        for (size_t i = 0; i < nsizes; ++i) total += sizes[i];
+        if (nsizes > 1 && total == static_cast<size_t>(K.nb)) {
+            blocks = true;
+            bm = block_map(static_cast<size_t>(K.nb), sizes, nsizes, true);
+            bpart.alloc(2 * kMaxOffblockBlocks * sizeof(double));
+            bcond.alloc(2 * sizeof(double));
+        }
     }
 
-    // density_from_fock (solver_scf.hpp:96-154), dense eigh.
+    // density_from_fock (solver_scf.hpp:96-154).  With blocks, F's off-block
+    // test is reduced on the device and read there by the eigh, which treats
+    // the blocks separately (each block ascending at its position) only when
+    // F passes it.
     void density_from_fock(const cd* F, cd* P_out) {
         const int nk = static_cast<int>(K.nk), nb = K.nb;
         const size_t nkb = K.nk * K.nb;
-        launch_eigh(nk, nb, 0, F, eps.as<double>(), V.as<cd>(), st);
+        if (blocks) {
+            launch_max_offblock(nk, nb, F, bm, bpart.as<double>(), bcond.as<double>(), st);
+            launch_eigh_blocks(nk, nb, 0, F, eps.as<double>(), V.as<cd>(), bm, false, st,
+                               bcond.as<double>());
+        } else {
+            launch_eigh(nk, nb, 0, F, eps.as<double>(), V.as<cd>(), st);
+        }
         launch_mu_bisection(eps.as<double>(), K.w2d.as<double>(), nk, nb, n_e, K.T, 0,
                             ws.mu(), ws.mu_partial.as<double>(), st);
         const double* e_used = eps.as<double>();
@@ -215,12 +250,11 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
                    Projector& proj, hf_scf_result* out) {
     if (cfg.acceleration < 0 || cfg.acceleration > 2)
         throw std::invalid_argument("SCFConfig.acceleration must be 'linear', 'diis', or 'oda'");
-    validate_block_sizes(cfg.block_sizes, cfg.n_blocks, static_cast<size_t>(K.nb));
     const bool use_diis = cfg.acceleration == 1, use_oda = cfg.acceleration == 2;
     const size_t nd = K.nd, bytes = nd * sizeof(cd);
     const int nk = static_cast<int>(K.nk), nb = K.nb;
     hipStream_t st = K.stream;
-    ScfState S(K, proj, n_e, cfg.level_shift);
+    ScfState S(K, proj, n_e, cfg.level_shift, cfg.block_sizes, cfg.n_blocks);
     DevBuf density(bytes), Sigma(bytes), F(bytes), P_new(bytes), delta(bytes), hdiag(nb * 8);
     DevBuf comm(use_diis ? bytes : 0), Sigma_new(use_oda ? bytes : 0), F_new(use_oda ? bytes : 0),
         hdiag_new(use_oda ? nb * 8 : 0), pre(cfg.trust_radius > 0.0 ? bytes : 0),
@@ -420,11 +454,38 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
         launch_hermitize(nk, nb, P.as<cd>(), st);
     };
 
+    // ---- block structure (solver_dm.hpp:100-113, 403-405): more than one
+    // block, no projector, and h, refP and the active contact operators
+    // block-diagonal; then F at P0 must be too (446-447).  The blocks then
+    // stay separate in every eigh of the solve: each block's ascending pairs
+    // at the block's position.
+    BlockMap bm{};
+    bool use_block = false;
+    if (cfg.n_blocks > 1) {
+        bm = block_map(static_cast<size_t>(nb), cfg.block_sizes, cfg.n_blocks, true);
+        use_block = !proj && block_ok_host(K.h.as<cd>(), nk, nb, bm, st) &&
+                    (!K.has_refP || block_ok_host(K.refP.as<cd>(), nk, nb, bm, st));
+        for (int t = 0; use_block && t < K.n_contact; ++t) {
+            if (K.cg_host[t] == 0.0) continue;
+            use_block = block_ok_host(K.Oi.as<cd>() + static_cast<size_t>(t) * K.nb2, 1, nb, bm,
+                                      st) &&
+                        block_ok_host(K.Oj.as<cd>() + static_cast<size_t>(t) * K.nb2, 1, nb, bm,
+                                      st);
+        }
+    }
+    auto eigh = [&](int in_mode, const cd* M, double* w, cd* V) {
+        if (use_block)
+            launch_eigh_blocks(nk, nb, in_mode, M, w, V, bm, false, st);
+        else
+            launch_eigh(nk, nb, in_mode, M, w, V, st);
+    };
+
     // ---- init (solver_dm.hpp:436-459)
     h2d(P.get(), P0, bytes, st);
     launch_hermitize(nk, nb, P.as<cd>(), st);
     K.build_fock(P.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(), true, st);
-    launch_eigh(nk, nb, 0, F.as<cd>(), eps.as<double>(), Q.as<cd>(), st);
+    if (use_block && !block_ok_host(F.as<cd>(), nk, nb, bm, st)) use_block = false;
+    eigh(0, F.as<cd>(), eps.as<double>(), Q.as<cd>());
     mu_solve(eps.as<double>(), false, 0.0);
 
     std::vector<double> hist_E, hist_G;
@@ -454,7 +515,7 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
         launch_cg_combine(nk, nb, restart, dQ.as<cd>(), dQp.as<cd>(), h_p.as<double>(),
                           d_pp.as<double>(), d_p.as<double>(), eps.as<double>(), p.as<double>(),
                           wd, T_r, W.ls, st);
-        launch_eigh(nk, nb, 1, dQ.as<cd>(), lam.as<double>(), Vd.as<cd>(), st);
+        eigh(1, dQ.as<cd>(), lam.as<double>(), Vd.as<cd>());
         launch_linesearch(nk, nb, Vd.as<cd>(), lam.as<double>(), F.as<cd>(), Sigma.as<cd>(),
                           p.as<double>(), d_p.as<double>(), wd, T_r, cfg.max_step, cfg.bt_shrink,
                           static_cast<int>(cfg.bt_max), W.ls, st);
@@ -482,7 +543,7 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
     K.build_fock(P.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(), true, st);
     launch_orbital_grad(nk, nb, Q.as<cd>(), F.as<cd>(), nullptr, wd, W.mu(), T_r, false, nullptr,
                         nullptr, nullptr, W.ls, st);
-    launch_eigh(nk, nb, 0, F.as<cd>(), eps_w.as<double>(), Vd.as<cd>(), st);
+    eigh(0, F.as<cd>(), eps_w.as<double>(), Vd.as<cd>());
     launch_retract(nk, nb, Q.as<cd>(), Vd.as<cd>(), nullptr, nullptr, nullptr, false, W.ls, st);
     mu_solve(eps_w.as<double>(), true, 0.0);
     density_step();

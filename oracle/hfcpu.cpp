@@ -379,6 +379,7 @@ void eigh_block_sizes_one(const c64* M, size_t nb, const std::vector<size_t>& si
     for (size_t e = 0; e < nb * nb; ++e) V[e] = 0.0;
     size_t start = 0;
     for (size_t s : sizes) {
+        if (s == 0) continue;
         std::vector<c64> sub(s * s), Vs(s * s);
         for (size_t i = 0; i < s; ++i)
             for (size_t j = 0; j < s; ++j) sub[i * s + j] = M[(start + i) * nb + start + j];
@@ -938,7 +939,52 @@ double frozen_free_energy(const double* eps, const double* p, const double* d_p,
     return E - std::max(T, 1.0e-14) * S;
 }
 
-// solver_dm.hpp:392-653 (optimizer "cg"; block_sizes path not restated)
+// solver_dm.hpp:92-98: every matrix of the batch block-diagonal in `sizes` up
+// to 1e-10 + 1e-10 max|M|
+bool block_structure_ok(const c64* M, size_t nmat, size_t nb, const std::vector<size_t>& sizes) {
+    double scale = 0.0;
+    for (size_t i = 0; i < nmat * nb * nb; ++i) scale = std::max(scale, std::abs(M[i]));
+    return max_offblock_sizes(M, nmat, nb, sizes) <= 1.0e-10 + 1.0e-10 * scale;
+}
+
+// solver_dm.hpp:100-113: h, refP and the active contact operators keep the
+// block structure, and no projector may mix the blocks
+bool block_problem_invariant(const Kernel& K, const Projector* proj,
+                             const std::vector<size_t>& sizes) {
+    if (proj && *proj) return false;
+    const size_t nk = K.nk(), nb = K.nb, nb2 = nb * nb;
+    if (!block_structure_ok(K.h, nk, nb, sizes)) return false;
+    if (K.has_refP && !block_structure_ok(K.refP, nk, nb, sizes)) return false;
+    for (size_t t = 0; t < K.n_contact; ++t) {
+        if (K.contact_g[t] == 0.0) continue;
+        if (!block_structure_ok(K.contact_Oi + t * nb2, 1, nb, sizes)) return false;
+        if (!block_structure_ok(K.contact_Oj + t * nb2, 1, nb, sizes)) return false;
+    }
+    return true;
+}
+
+// linalg.hpp:110-121 with sort = false: each block's ascending pairs at the
+// block's position
+void eigh_blocks_batched(const c64* M, double* w, c64* V, size_t nk, size_t nb,
+                         const std::vector<size_t>& sizes) {
+    const size_t nb2 = nb * nb;
+    parallel_for(nk, [&](size_t k) {
+        eigh_block_sizes_one(M + k * nb2, nb, sizes, false, w + k * nb, V + k * nb2);
+    });
+}
+
+// solver_dm.hpp:224-249: the Cayley spectrum block by block
+void cayley_spectral_setup_blocks(const c64* d, c64* V, double* lam, size_t nk, size_t nb,
+                                  const std::vector<size_t>& sizes) {
+    const size_t nb2 = nb * nb;
+    parallel_for(nk, [&](size_t k) {
+        std::vector<c64> iA(nb2);
+        for (size_t i = 0; i < nb2; ++i) iA[i] = c64(0.0, 1.0) * d[k * nb2 + i];
+        eigh_block_sizes_one(iA.data(), nb, sizes, false, lam + k * nb, V + k * nb2);
+    });
+}
+
+// solver_dm.hpp:392-653 (optimizer "cg")
 void solve_dm(const Kernel& K, const c64* P0, double n_e, const hf_dm_config& cfg,
               const Projector* proj, hf_dm_result* out) {
     const size_t nk1 = K.nk1, nk2 = K.nk2, nb = K.nb, nk = nk1 * nk2, nb2 = nb * nb;
@@ -956,11 +1002,28 @@ void solve_dm(const Kernel& K, const c64* P0, double n_e, const hf_dm_config& cf
         g_p_prev(nkb, 0.0), d_p(nkb), d_p_prev(nkb, 0.0), lam_d(nkb), eps_work(nkb);
     std::vector<double> hist_E(cfg.max_iter, 0.0), hist_G(cfg.max_iter, 0.0);
 
+    // block structure (80-90, 403-405, 446-447)
+    const std::vector<size_t> sizes(cfg.block_sizes, cfg.block_sizes + cfg.n_blocks);
+    bool use_block = false;
+    if (!sizes.empty()) {
+        size_t total = 0;
+        for (size_t s : sizes) {
+            if (s == 0) throw std::invalid_argument("block_sizes entries must be positive.");
+            total += s;
+        }
+        if (total != nb) throw std::invalid_argument("block_sizes must sum to nb.");
+        use_block = sizes.size() > 1 && block_problem_invariant(K, proj, sizes);
+    }
+
     // init (436-459)
     std::memcpy(P_cur.data(), P0, n_tot * sizeof(c64));
     hermitize_inplace(P_cur.data(), nk, nb);
     build_fock_compact(K, P_cur.data(), Sigma.data(), F.data(), hdiag.data(), nullptr);
-    eigh_batched(F.data(), eps.data(), Q.data(), nk, nb);
+    if (use_block && !block_structure_ok(F.data(), nk, nb, sizes)) use_block = false;
+    if (use_block)
+        eigh_blocks_batched(F.data(), eps.data(), Q.data(), nk, nb, sizes);
+    else
+        eigh_batched(F.data(), eps.data(), Q.data(), nk, nb);
     const double mu0 = solve_mu_inloop(eps.data(), nkb, w_norm.data(), nk, nb, n_target_norm,
                                        0.0, T_r, cfg.mu_maxiter);
     for (size_t i = 0; i < nkb; ++i) p[i] = expit_scalar((mu0 - eps[i]) / T_r);
@@ -1029,7 +1092,10 @@ void solve_dm(const Kernel& K, const c64* P0, double n_e, const hf_dm_config& cf
         for (size_t i = 0; i < n_tot; ++i) d_Q[i] = d_Q[i] + beta * d_Q_prev[i];
         for (size_t i = 0; i < nkb; ++i) d_p[i] = h_p[i] + beta * d_p_prev[i];
 
-        cayley_spectral_setup(d_Q.data(), V_d.data(), lam_d.data(), nk, nb);
+        if (use_block)
+            cayley_spectral_setup_blocks(d_Q.data(), V_d.data(), lam_d.data(), nk, nb, sizes);
+        else
+            cayley_spectral_setup(d_Q.data(), V_d.data(), lam_d.data(), nk, nb);
         compute_Ft_eig(F.data(), V_d.data(), Sigma.data(), nk, nb);
         const double dQn = norm_matrix(d_Q.data(), w_norm.data(), nk, nb);
         const double tau0 = std::min(1.0, cfg.max_step / std::max(dQn, TINY_REAL));
@@ -1071,7 +1137,10 @@ void solve_dm(const Kernel& K, const c64* P0, double n_e, const hf_dm_config& cf
     hermitize_inplace(P_cur.data(), nk, nb);
     build_fock_compact(K, P_cur.data(), Sigma.data(), F.data(), hdiag.data(), nullptr);
     fock_in_orbital_basis(Q.data(), F.data(), F.data(), nk, nb);
-    eigh_batched(F.data(), eps_work.data(), V_d.data(), nk, nb);
+    if (use_block)
+        eigh_blocks_batched(F.data(), eps_work.data(), V_d.data(), nk, nb, sizes);
+    else
+        eigh_batched(F.data(), eps_work.data(), V_d.data(), nk, nb);
     Q_times_U(Q.data(), V_d.data(), Q.data(), nk, nb);
     mu = solve_mu_inloop(eps_work.data(), nkb, w_norm.data(), nk, nb, n_target_norm, mu, T_r,
                          cfg.mu_maxiter);
@@ -1110,13 +1179,24 @@ double weighted_matrix_norm(const c64* M, const double* w, size_t nk, size_t nb,
     return std::sqrt(total / std::max(ws, 1.0e-30));
 }
 
-// solver_scf.hpp:96-154 (dense eigh; block path not restated)
+// solver_scf.hpp:96-154; per-block eigh when block_sizes (summing to nb, more
+// than one entry) describes F's structure up to 1e-10 + 1e-10 max|F|
 double density_from_fock(const c64* F, size_t nk, size_t nb, const double* w, double n_e,
-                         double T, double level_shift, c64* P_new) {
+                         double T, double level_shift, c64* P_new,
+                         const std::vector<size_t>& sizes = {}) {
     const size_t nb2 = nb * nb, nkb = nk * nb;
     std::vector<double> eps(nkb), shifted(nkb), occ(nkb);
     std::vector<c64> V(nk * nb2);
-    eigh_batched(F, eps.data(), V.data(), nk, nb);
+    bool used_blocks = false;
+    if (!sizes.empty()) {
+        size_t total = 0;
+        for (size_t s : sizes) total += s;
+        if (total == nb && sizes.size() > 1 && block_structure_ok(F, nk, nb, sizes)) {
+            eigh_blocks_batched(F, eps.data(), V.data(), nk, nb, sizes);
+            used_blocks = true;
+        }
+    }
+    if (!used_blocks) eigh_batched(F, eps.data(), V.data(), nk, nb);
     const double mu_raw = find_mu_bisection(eps.data(), nkb, w, nk, nb, n_e, T, 0);
     if (level_shift != 0.0) {
         for (size_t i = 0; i < nkb; ++i)
@@ -1191,6 +1271,7 @@ void solve_scf(const Kernel& K, const c64* P0, double n_e, const hf_scf_config& 
         delta(n_tot), comm(n_tot);
     std::vector<double> hdiag(nb, 0.0);
     std::vector<double> hE(cfg.max_iter, 0.0), hD(cfg.max_iter, 0.0), hC(cfg.max_iter, 0.0);
+    const std::vector<size_t> sizes(cfg.block_sizes, cfg.block_sizes + cfg.n_blocks);
     double mu = 0.0, E = 0.0;
     bool converged = false;
     size_t k = 0;
@@ -1206,7 +1287,8 @@ void solve_scf(const Kernel& K, const c64* P0, double n_e, const hf_scf_config& 
         hermitize_inplace(d.data(), nk, nb);
         build_fock_compact(K, d.data(), Sigma.data(), F.data(), hdiag.data(), proj);
         E_out = hf_energy_with_hartree_diag(K, d.data(), Sigma.data(), hdiag.data());
-        mu_out = density_from_fock(F.data(), nk, nb, K.w2d, n_e, K.T, cfg.level_shift, P_out);
+        mu_out = density_from_fock(F.data(), nk, nb, K.w2d, n_e, K.T, cfg.level_shift, P_out,
+                                   sizes);
         project(P_out);
         hermitize_inplace(P_out, nk, nb);
     };
@@ -1285,7 +1367,8 @@ void solve_scf(const Kernel& K, const c64* P0, double n_e, const hf_scf_config& 
                             Fx[j] = c64(damp, 0.0) * Fx[j] + c64(1.0 - damp, 0.0) * F[j];
                     hermitize_inplace(Fx.data(), nk, nb);
                     std::vector<c64> Pd(n_tot);
-                    density_from_fock(Fx.data(), nk, nb, K.w2d, n_e, K.T, cfg.level_shift, Pd.data());
+                    density_from_fock(Fx.data(), nk, nb, K.w2d, n_e, K.T, cfg.level_shift,
+                                      Pd.data(), sizes);
                     project(Pd.data());
                     hermitize_inplace(Pd.data(), nk, nb);
                     density = std::move(Pd);

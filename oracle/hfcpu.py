@@ -296,6 +296,12 @@ def density_from_fock(F, weights, n_e, T, level_shift=0.0):
 _ACCEL = {"linear": 0, "diis": 1, "oda": 2}
 
 
+def _size_array(block_sizes):
+    # a ctypes size_t array (None when empty); the caller's frame keeps it alive
+    sizes = [int(v) for v in (block_sizes or ())]
+    return (_sz * len(sizes))(*sizes) if sizes else None
+
+
 def solve_scf(kernel_args, P0, n_e, max_iter, density_tol, comm_tol, mixing, level_shift,
               project_fn, block_sizes, acceleration, diis_size, diis_start, diis_damping,
               trust_radius, return_density, return_fock):
@@ -303,11 +309,12 @@ def solve_scf(kernel_args, P0, n_e, max_iter, density_tol, comm_tol, mixing, lev
     P0 = _c128(P0)
     if acceleration not in _ACCEL:
         raise ValueError("SCFConfig.acceleration must be 'linear', 'diis', or 'oda'")
+    sizes = _size_array(block_sizes)
     cfg = ScfConfig(max_iter=max_iter, density_tol=density_tol, comm_tol=comm_tol,
                     mixing=mixing, level_shift=level_shift,
                     acceleration=_ACCEL[acceleration], diis_size=diis_size,
                     diis_start=diis_start, diis_damping=diis_damping,
-                    trust_radius=trust_radius, block_sizes=None, n_blocks=0)
+                    trust_radius=trust_radius, block_sizes=sizes, n_blocks=len(sizes or ()))
     dens = np.empty(d.shape, np.complex128)
     fock = np.empty(d.shape, np.complex128)
     hE = np.zeros(max_iter)
@@ -332,10 +339,12 @@ def solve_dm(kernel_args, P0, n_e, max_iter, tol_E, tol_grad, max_step, bt_shrin
         raise ValueError("the oracle restates optimizer='cg' only")
     d = _Desc(kernel_args)
     P0 = _c128(P0)
+    sizes = _size_array(block_sizes)
     cfg = DmConfig(max_iter=max_iter, tol_E=tol_E, tol_grad=tol_grad,
                    plateau_window=plateau_window, denom_scale=denom_scale,
                    max_step=max_step, cg_restart=cg_restart, bt_shrink=bt_shrink,
-                   bt_max=bt_max, mu_maxiter=mu_maxiter, block_sizes=None, n_blocks=0)
+                   bt_max=bt_max, mu_maxiter=mu_maxiter, block_sizes=sizes,
+                   n_blocks=len(sizes or ()))
     Q = np.empty(d.shape, np.complex128)
     p = np.empty(d.shape[:-1], np.float64)
     dens = np.empty(d.shape, np.complex128)

@@ -95,3 +95,32 @@ def brute_force_exchange(VR, P):
                 for q2 in range(n2):
                     out[a, b] -= W[(a - q1) % n1, (b - q2) % n2] * P[q1, q2]
     return out
+
+
+def block_kernel(nk=6, T=0.1, couple=0.0):
+    """Two 2-band QWZ copies on one mesh (masses 0.6 and -0.9, the second
+    shifted by 0.2), scalar V, Hartree with refP = I/2: block-diagonal in the
+    band blocks (2, 2) unless `couple` adds an off-block hopping."""
+    from cpp_hf_amd import HartreeFockKernel
+
+    k = 2.0 * np.pi * np.arange(nk) / nk - np.pi
+    KX, KY = np.meshgrid(k, k, indexing="ij")
+    sx = np.array([[0, 1], [1, 0]], dtype=complex)
+    sy = np.array([[0, -1j], [1j, 0]])
+    sz = np.diag([1.0, -1.0]).astype(complex)
+
+    def qwz(m):
+        return (np.sin(KX)[..., None, None] * sx + np.sin(KY)[..., None, None] * sy
+                + (m - np.cos(KX) - np.cos(KY))[..., None, None] * sz)
+
+    h = np.zeros((nk, nk, 4, 4), dtype=complex)
+    h[..., :2, :2] = qwz(0.6)
+    h[..., 2:, 2:] = qwz(-0.9) + 0.2 * np.eye(2)
+    h[..., 0, 2] = couple
+    h[..., 2, 0] = couple
+    w = np.full((nk, nk), 1.0 / (nk * nk))
+    V = np.full((nk, nk, 1, 1), 0.3)
+    HH = 0.2 * np.ones((4, 4)) + 0.3 * np.eye(4)
+    refP = np.broadcast_to(0.5 * np.eye(4, dtype=complex), h.shape).copy()
+    return HartreeFockKernel(w, h, V, T=T, include_hartree=True, reference_density=refP,
+                             hartree_matrix=HH)

@@ -13,8 +13,8 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from helpers import (contact_kernel, hartree_kernel, herm, kernel_args_dict, rand_c, rand_herm,
-                     two_band_kernel)
+from helpers import (block_kernel, contact_kernel, hartree_kernel, herm, kernel_args_dict, rand_c,
+                     rand_herm, two_band_kernel)
 
 pytestmark = pytest.mark.gpu
 
@@ -390,3 +390,57 @@ def test_max_offblock_matches_oracle(gpu, oracle):
         got = gpu.max_offblock_sizes(M, list(sizes))
         ref = oracle.max_offblock_sizes(M, sizes)
         assert abs(got - ref) <= 4e-16 * ref
+
+
+# ------------------------------------------------ block-structured solvers ---
+
+def _block_dm_args(k, sizes):
+    return (kernel_args_dict(k), np.zeros_like(k.h), 2.0, 200, 1e-10, 0.0, 0.6, 0.5, 1e-3, 8, 10,
+            5, 25, 0, 0.5, 20, np.empty((0,) + k.h.shape), 0.0, 1.0, list(sizes), None, True,
+            True, True)
+
+
+@pytest.mark.parametrize("couple", [0.0, 0.05])
+def test_solve_dm_block_sizes_matches_oracle(gpu, oracle, couple):
+    """Per-block eigh in the init, Cayley and final steps (solver_dm.hpp:446-450,
+    562-564, 622-624) when the problem is block-diagonal; dense otherwise."""
+    k = block_kernel(couple=couple)
+    args = _block_dm_args(k, [2, 2])
+    print(f"solve_dm block_sizes couple={couple}", flush=True)
+    Q, p, P, F, mu, E, n, conv, hE, hG = gpu.solve_dm(*args)
+    r = oracle.solve_dm(*args)
+    assert n == r[6] and conv == r[7]
+    assert abs(E - r[5]) < 1e-10 and abs(mu - r[4]) < 1e-10
+    np.testing.assert_allclose(hE, r[8], rtol=0, atol=1e-10)
+    # occupations in the same (block-wise or global) order as the oracle's
+    np.testing.assert_allclose(p, r[1], atol=1e-9)
+    assert relerr(P, r[2]) < 1e-9
+    if couple == 0.0:
+        assert not np.any(Q[..., :2, 2:]) and not np.any(Q[..., 2:, :2])
+
+
+@pytest.mark.parametrize("couple,accel", [(0.0, "linear"), (0.0, "diis"), (0.05, "linear")])
+def test_solve_scf_block_sizes_matches_oracle(gpu, oracle, couple, accel):
+    """density_from_fock with block_sizes (solver_scf.hpp:96-123): the off-block
+    test of every F runs on the device."""
+    k = block_kernel(couple=couple)
+    args = (kernel_args_dict(k), np.zeros_like(k.h), 2.0, 400, 1e-9, 1e-8, 0.5, 0.0, None,
+            [2, 2], accel, 6, 2, 1.0, 0.0, True, True)
+    print(f"solve_scf block_sizes couple={couple} {accel}", flush=True)
+    g = gpu.solve_scf(*args)
+    r = oracle.solve_scf(*args)
+    assert g[4] == r[4] and g[5] == r[5]
+    assert abs(g[2] - r[2]) < 1e-10 and abs(g[3] - r[3]) < 1e-10
+    assert relerr(g[0], r[0]) < 1e-9
+
+
+def test_block_sizes_validation_follows_solver(gpu):
+    k = block_kernel()
+    with pytest.raises(ValueError, match="must sum to nb"):
+        gpu.solve_dm(*_block_dm_args(k, [3, 2]))
+    # SCF: sizes that do not describe nb mean "no blocks" (solver_scf.hpp:107-110)
+    args = (kernel_args_dict(k), np.zeros_like(k.h), 2.0, 20, 1e-9, 1e-8, 0.5, 0.0, None,
+            [3, 2], "linear", 6, 2, 1.0, 0.0, True, True)
+    dense = list(args)
+    dense[9] = []
+    np.testing.assert_array_equal(gpu.solve_scf(*args)[0], gpu.solve_scf(*dense)[0])

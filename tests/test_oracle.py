@@ -15,7 +15,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from helpers import (brute_force_exchange, contact_kernel, hartree_kernel, herm, kernel_args_dict,
+from helpers import (block_kernel, brute_force_exchange, contact_kernel, hartree_kernel, herm, kernel_args_dict,
                      rand_c, rand_herm, two_band_kernel)
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -341,3 +341,68 @@ def test_oracle_scf_meshgrid_fixture(oracle):
     e0 = float(np.sum(np.real(k.w2d * np.einsum("...ij,...ji->...", k.h, P0)
                               + 0.5 * k.w2d * np.einsum("...ij,...ji->...", S0 + H0, P0))))
     assert abs(e0 - float(d["hist_E_expected"][0])) < 1e-6
+
+
+# ------------------------------------------------- block-structured solvers --
+# solver_dm.hpp:80-113, 403-450, 562-564, 622-624 and solver_scf.hpp:96-123:
+# with block_sizes the solvers diagonalise block by block (each block's
+# ascending pairs at the block's position) while the problem stays
+# block-diagonal, and fall back to the dense eigh otherwise.
+
+def _dm_b(oracle, k, n_e, sizes, max_iter=200, tol_E=1e-10):
+    return oracle.solve_dm(kernel_args_dict(k), np.zeros_like(k.h), n_e, max_iter, tol_E, 0.0,
+                           0.6, 0.5, 1e-3, 8, 10, 5, 25, 0, 0.5, 20,
+                           np.empty((0,) + k.h.shape), 0.0, 1.0, list(sizes), None, True, True,
+                           True)
+
+
+def _scf_b(oracle, k, n_e, sizes, accel="linear"):
+    return oracle.solve_scf(kernel_args_dict(k), np.zeros_like(k.h), n_e, 400, 1e-9, 1e-8, 0.5,
+                            0.0, None, list(sizes), accel, 6, 2, 1.0, 0.0, True, True)
+
+
+def test_oracle_dm_block_path_same_fixed_point_block_ordered(oracle):
+    k = block_kernel()
+    dense = _dm_b(oracle, k, 2.0, [])
+    blk = _dm_b(oracle, k, 2.0, [2, 2])
+    assert dense[7] and blk[7]
+    assert abs(dense[5] - blk[5]) < 1e-9
+    assert np.max(np.abs(dense[2] - blk[2])) < 1e-6
+    Q, p = blk[0], blk[1]
+    # the orbitals never leave their block ...
+    assert not np.any(Q[..., :2, 2:]) and not np.any(Q[..., 2:, :2])
+    # ... and come ordered within each block (occupations fall with energy)
+    assert np.all(np.diff(p[..., :2], axis=-1) <= 1e-12)
+    assert np.all(np.diff(p[..., 2:], axis=-1) <= 1e-12)
+    # while the dense run orders all four globally
+    assert np.all(np.diff(dense[1], axis=-1) <= 1e-12)
+
+
+@pytest.mark.parametrize("accel", ["linear", "diis"])
+def test_oracle_scf_block_path_matches_dense(oracle, accel):
+    k = block_kernel()
+    dense = _scf_b(oracle, k, 2.0, [], accel)
+    blk = _scf_b(oracle, k, 2.0, [2, 2], accel)
+    assert dense[5] and blk[5] and dense[4] == blk[4]
+    assert abs(dense[2] - blk[2]) < 1e-10
+    assert np.max(np.abs(dense[0] - blk[0])) < 1e-9
+
+
+def test_oracle_block_sizes_fall_back_on_coupled_problems(oracle):
+    k = block_kernel(couple=0.05)
+    for a, b in zip(_dm_b(oracle, k, 2.0, []), _dm_b(oracle, k, 2.0, [2, 2])):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(_scf_b(oracle, k, 2.0, []), _scf_b(oracle, k, 2.0, [2, 2])):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_oracle_block_sizes_validation(oracle):
+    k = block_kernel()
+    with pytest.raises(ValueError, match="must sum to nb"):
+        _dm_b(oracle, k, 2.0, [3, 2], max_iter=2)
+    with pytest.raises(ValueError, match="must be positive"):
+        _dm_b(oracle, k, 2.0, [0, 4], max_iter=2)
+    # the SCF solver takes sizes that do not describe nb as "no blocks"
+    a = _scf_b(oracle, k, 2.0, [3, 2])
+    b = _scf_b(oracle, k, 2.0, [])
+    np.testing.assert_array_equal(a[0], b[0])
