@@ -129,12 +129,13 @@ k_finalize(const double* __restrict__ partial, int nparts, int ns, double* __res
 
 // Ft = Q^H F Q (written over F), eps = Re diag Ft, G, g_p.
 // Slots: [GN2, EPS_SQ].  grad=false: only the rotation (finalise step).
-template <int W>
+template <int W, int NB>
 __global__ void __launch_bounds__(W * 64)
-k_orbital_grad(int nk, int nb, const cd* __restrict__ Q, cd* __restrict__ F,
+k_orbital_grad(int nk, int nb_arg, const cd* __restrict__ Q, cd* __restrict__ F,
                const double* __restrict__ p, const double* __restrict__ w, const MuState* mus,
                double T_r, int grad, double* __restrict__ eps, cd* __restrict__ G,
                double* __restrict__ g_p, double* partial, double* sc) {
+    const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     extern __shared__ cd smem[];
     __shared__ double red[W * 2];
     const int nb2 = nb * nb;
@@ -196,13 +197,14 @@ k_orbital_grad(int nk, int nb, const cd* __restrict__ Q, cd* __restrict__ F,
 
 // d_Q = G / sqrt(gap^2 + lam^2), h_p = g_p / curv, plus the PR+ products.
 // Slots: [PR_S1 .. PR_S6].
-template <int W>
+template <int W, int NB>
 __global__ void __launch_bounds__(W * 64)
-k_precond(int nk, int nb, const cd* __restrict__ G, const cd* __restrict__ Gp,
+k_precond(int nk, int nb_arg, const cd* __restrict__ G, const cd* __restrict__ Gp,
           const double* __restrict__ eps, const double* __restrict__ p,
           const double* __restrict__ g_p, const double* __restrict__ g_pp,
           const double* __restrict__ w, double T_r, double denom_scale, cd* __restrict__ dQ,
           double* __restrict__ h_p, double* partial, const double* sc) {
+    const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     __shared__ double red[W * 6];
     const int nb2 = nb * nb;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -250,13 +252,14 @@ k_precond(int nk, int nb, const cd* __restrict__ G, const cd* __restrict__ Gp,
 
 // beta from the PR+ products; d_Q += beta d_Q_prev, d_p = h_p + beta d_p_prev.
 // Slots: [DQN2, E0, S0].
-template <int W>
+template <int W, int NB>
 __global__ void __launch_bounds__(W * 64)
-k_cg_combine(int nk, int nb, int restart, cd* __restrict__ dQ, const cd* __restrict__ dQp,
+k_cg_combine(int nk, int nb_arg, int restart, cd* __restrict__ dQ, const cd* __restrict__ dQp,
              const double* __restrict__ h_p, const double* __restrict__ d_pp,
              double* __restrict__ d_p, const double* __restrict__ eps,
              const double* __restrict__ p, const double* __restrict__ w, double T_r,
              double* partial, double* sc) {
+    const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     __shared__ double red[W * 3];
     const int nb2 = nb * nb;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -317,12 +320,13 @@ __device__ __forceinline__ void diag_at_tau(const cd* sV, const cd* sFe, const d
 }
 
 // Slots: [2 * nbt trial (E_j, S_j)]; the last block then selects tau.
-template <int W>
+template <int W, int NB>
 __global__ void __launch_bounds__(W * 64)
-k_linesearch(int nk, int nb, const cd* __restrict__ V, const double* __restrict__ lam,
+k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restrict__ lam,
              const cd* __restrict__ Ft, cd* __restrict__ Fe, const double* __restrict__ p,
              const double* __restrict__ d_p, const double* __restrict__ w, double T_r,
              double max_step, double shrink, int nbt, double* partial, const double* sc) {
+    const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     extern __shared__ cd smem[];
     __shared__ double red[W * 2 * kMaxBt];
     const int nb2 = nb * nb;
@@ -383,11 +387,12 @@ k_linesearch(int nk, int nb, const cd* __restrict__ V, const double* __restrict_
 
 // Q <- Q V diag(c(tau)) V^H and eps_work = diag(U^H Ft U) at the chosen tau.
 // use_sc_tau = 0 with U = V (tau ignored): the finalise rotation Q <- Q V.
-template <int W>
+template <int W, int NB>
 __global__ void __launch_bounds__(W * 64)
-k_retract(int nk, int nb, cd* __restrict__ Q, const cd* __restrict__ V,
+k_retract(int nk, int nb_arg, cd* __restrict__ Q, const cd* __restrict__ V,
           const double* __restrict__ lam, const cd* __restrict__ Fe,
           double* __restrict__ eps_work, int cayley, const double* sc) {
+    const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     extern __shared__ cd smem[];
     const int nb2 = nb * nb;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -460,11 +465,12 @@ k_retract(int nk, int nb, cd* __restrict__ Q, const cd* __restrict__ V,
 
 // delta = Pn - D; comm = F D - D F (stored when comm != nullptr).
 // Slots: [sum w |delta|^2, sum w |comm|^2].
-template <int W>
+template <int W, int NB>
 __global__ void __launch_bounds__(W * 64)
-k_scf_residuals(int nk, int nb, const cd* __restrict__ F, const cd* __restrict__ D,
+k_scf_residuals(int nk, int nb_arg, const cd* __restrict__ F, const cd* __restrict__ D,
                 const cd* __restrict__ Pn, cd* __restrict__ delta, cd* __restrict__ comm,
                 const double* __restrict__ w, double* partial) {
+    const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     extern __shared__ cd smem[];
     __shared__ double red[W * 2];
     const int nb2 = nb * nb;
@@ -537,9 +543,26 @@ unsigned loop_blocks(int nk, int W) {
 // launchers
 // ---------------------------------------------------------------------------
 
-#define HFGPU_DISPATCH_W(nb, CALL4, CALL1) \
-    do {                                   \
-        if ((nb) <= 16) { CALL4; } else { CALL1; } \
+// Launch a per-k loop kernel: four wavefronts (k points) per workgroup up to
+// nb = 16 with nb a compile-time constant for 2, 4, 8, 12 and 16 (the index
+// arithmetic and the small matrix products then unroll), one wavefront and a
+// run-time nb above.  LDS is lds_wave bytes per wavefront.
+#define HFGPU_LOOP_LAUNCH(KERNEL, nb, nk, lds_wave, st, ...)                                \
+    do {                                                                                   \
+        const size_t lw_ = (lds_wave);                                                     \
+        const dim3 g4_(loop_blocks(nk, 4)), b4_(256);                                      \
+        switch ((nb) > 16 ? -1 : (nb)) {                                                   \
+            case -1:                                                                       \
+                hipLaunchKernelGGL((KERNEL<1, 0>), dim3(loop_blocks(nk, 1)), dim3(64), lw_, \
+                                   st, __VA_ARGS__);                                       \
+                break;                                                                     \
+            case 2: hipLaunchKernelGGL((KERNEL<4, 2>), g4_, b4_, 4 * lw_, st, __VA_ARGS__); break; \
+            case 4: hipLaunchKernelGGL((KERNEL<4, 4>), g4_, b4_, 4 * lw_, st, __VA_ARGS__); break; \
+            case 8: hipLaunchKernelGGL((KERNEL<4, 8>), g4_, b4_, 4 * lw_, st, __VA_ARGS__); break; \
+            case 12: hipLaunchKernelGGL((KERNEL<4, 12>), g4_, b4_, 4 * lw_, st, __VA_ARGS__); break; \
+            case 16: hipLaunchKernelGGL((KERNEL<4, 16>), g4_, b4_, 4 * lw_, st, __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL((KERNEL<4, 0>), g4_, b4_, 4 * lw_, st, __VA_ARGS__); break; \
+        }                                                                                  \
     } while (0)
 
 static void check_nb(int nb) {
@@ -559,13 +582,8 @@ void launch_orbital_grad(int nk, int nb, const cd* Q, cd* F, const double* p, co
                          double* g_p, LoopScratch& ls, hipStream_t st) {
     check_nb(nb);
     const size_t per_wave = sizeof(cd) * 3 * nb * nb;
-    HFGPU_DISPATCH_W(nb,
-        hipLaunchKernelGGL(k_orbital_grad<4>, dim3(loop_blocks(nk, 4)), dim3(256), 4 * per_wave, st,
-                           nk, nb, Q, F, p, w, mus, T_r, grad ? 1 : 0, eps, G, g_p, ls.partial,
-                           ls.sc),
-        hipLaunchKernelGGL(k_orbital_grad<1>, dim3(loop_blocks(nk, 1)), dim3(64), per_wave, st,
-                           nk, nb, Q, F, p, w, mus, T_r, grad ? 1 : 0, eps, G, g_p, ls.partial,
-                           ls.sc));
+    HFGPU_LOOP_LAUNCH(k_orbital_grad, nb, nk, per_wave, st, nk, nb, Q, F, p, w, mus, T_r,
+                      grad ? 1 : 0, eps, G, g_p, ls.partial, ls.sc);
     HFGPU_LAUNCH_CHECK();
     if (grad) finalize(ls, loop_blocks(nk, waves_for(nb)), 2, ls.sc + SC_GN2, st);
 }
@@ -575,11 +593,8 @@ void launch_precond(int nk, int nb, const cd* G, const cd* Gp, const double* eps
                     double T_r, double denom_scale, cd* dQ, double* h_p, LoopScratch& ls,
                     hipStream_t st) {
     check_nb(nb);
-    HFGPU_DISPATCH_W(nb,
-        hipLaunchKernelGGL(k_precond<4>, dim3(loop_blocks(nk, 4)), dim3(256), 0, st, nk, nb, G, Gp,
-                           eps, p, g_p, g_pp, w, T_r, denom_scale, dQ, h_p, ls.partial, ls.sc),
-        hipLaunchKernelGGL(k_precond<1>, dim3(loop_blocks(nk, 1)), dim3(64), 0, st, nk, nb, G, Gp,
-                           eps, p, g_p, g_pp, w, T_r, denom_scale, dQ, h_p, ls.partial, ls.sc));
+    HFGPU_LOOP_LAUNCH(k_precond, nb, nk, 0, st, nk, nb, G, Gp, eps, p, g_p, g_pp, w, T_r,
+                      denom_scale, dQ, h_p, ls.partial, ls.sc);
     HFGPU_LAUNCH_CHECK();
     finalize(ls, loop_blocks(nk, waves_for(nb)), 6, ls.sc + SC_PR1, st);
 }
@@ -588,13 +603,8 @@ void launch_cg_combine(int nk, int nb, bool restart, cd* dQ, const cd* dQp, cons
                        const double* d_pp, double* d_p, const double* eps, const double* p,
                        const double* w, double T_r, LoopScratch& ls, hipStream_t st) {
     check_nb(nb);
-    HFGPU_DISPATCH_W(nb,
-        hipLaunchKernelGGL(k_cg_combine<4>, dim3(loop_blocks(nk, 4)), dim3(256), 0, st, nk, nb,
-                           restart ? 1 : 0, dQ, dQp, h_p, d_pp, d_p, eps, p, w, T_r, ls.partial,
-                           ls.sc),
-        hipLaunchKernelGGL(k_cg_combine<1>, dim3(loop_blocks(nk, 1)), dim3(64), 0, st, nk, nb,
-                           restart ? 1 : 0, dQ, dQp, h_p, d_pp, d_p, eps, p, w, T_r, ls.partial,
-                           ls.sc));
+    HFGPU_LOOP_LAUNCH(k_cg_combine, nb, nk, 0, st, nk, nb, restart ? 1 : 0, dQ, dQp, h_p, d_pp,
+                      d_p, eps, p, w, T_r, ls.partial, ls.sc);
     HFGPU_LAUNCH_CHECK();
     finalize(ls, loop_blocks(nk, waves_for(nb)), 3, ls.sc + SC_DQN2, st);
 }
@@ -609,13 +619,8 @@ void launch_linesearch(int nk, int nb, const cd* V, const double* lam, const cd*
     if (nbt < 0 || nbt > kMaxBt)
         throw std::invalid_argument("bt_max above the GPU line-search limit of 16");
     const size_t per_wave = ls_wave_bytes(nb);
-    HFGPU_DISPATCH_W(nb,
-        hipLaunchKernelGGL(k_linesearch<4>, dim3(loop_blocks(nk, 4)), dim3(256), 4 * per_wave, st,
-                           nk, nb, V, lam, Ft, Fe, p, d_p, w, T_r, max_step, shrink, nbt,
-                           ls.partial, ls.sc),
-        hipLaunchKernelGGL(k_linesearch<1>, dim3(loop_blocks(nk, 1)), dim3(64), per_wave, st,
-                           nk, nb, V, lam, Ft, Fe, p, d_p, w, T_r, max_step, shrink, nbt,
-                           ls.partial, ls.sc));
+    HFGPU_LOOP_LAUNCH(k_linesearch, nb, nk, per_wave, st, nk, nb, V, lam, Ft, Fe, p, d_p, w, T_r,
+                      max_step, shrink, nbt, ls.partial, ls.sc);
     HFGPU_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(256), 0, st, ls.partial,
                        static_cast<int>(loop_blocks(nk, waves_for(nb))), 2 * kMaxBt,
@@ -627,11 +632,8 @@ void launch_retract(int nk, int nb, cd* Q, const cd* V, const double* lam, const
                     double* eps_work, bool cayley, LoopScratch& ls, hipStream_t st) {
     check_nb(nb);
     const size_t per_wave = ls_wave_bytes(nb);
-    HFGPU_DISPATCH_W(nb,
-        hipLaunchKernelGGL(k_retract<4>, dim3(loop_blocks(nk, 4)), dim3(256), 4 * per_wave, st, nk,
-                           nb, Q, V, lam, Fe, eps_work, cayley ? 1 : 0, ls.sc),
-        hipLaunchKernelGGL(k_retract<1>, dim3(loop_blocks(nk, 1)), dim3(64), per_wave, st, nk, nb,
-                           Q, V, lam, Fe, eps_work, cayley ? 1 : 0, ls.sc));
+    HFGPU_LOOP_LAUNCH(k_retract, nb, nk, per_wave, st, nk, nb, Q, V, lam, Fe, eps_work,
+                      cayley ? 1 : 0, ls.sc);
     HFGPU_LAUNCH_CHECK();
 }
 
@@ -640,11 +642,8 @@ void launch_scf_residuals(int nk, int nb, const cd* F, const cd* D, const cd* Pn
                           hipStream_t st) {
     check_nb(nb);
     const size_t per_wave = sizeof(cd) * 2 * nb * nb;
-    HFGPU_DISPATCH_W(nb,
-        hipLaunchKernelGGL(k_scf_residuals<4>, dim3(loop_blocks(nk, 4)), dim3(256), 4 * per_wave,
-                           st, nk, nb, F, D, Pn, delta, comm, w, ls.partial),
-        hipLaunchKernelGGL(k_scf_residuals<1>, dim3(loop_blocks(nk, 1)), dim3(64), per_wave, st,
-                           nk, nb, F, D, Pn, delta, comm, w, ls.partial));
+    HFGPU_LOOP_LAUNCH(k_scf_residuals, nb, nk, per_wave, st, nk, nb, F, D, Pn, delta, comm, w,
+                      ls.partial);
     HFGPU_LAUNCH_CHECK();
     finalize(ls, loop_blocks(nk, waves_for(nb)), 2, out, st);
 }

@@ -306,10 +306,16 @@ def _dm_cases():
         ("contact", _dm_args(contact_kernel(0.3, nk=2), 4.0)),
         ("qwz12", _dm_args_p0(wl1.kernel, wl1.P0, 1.0)),
         ("layered12_nb4", _dm_args_p0(wl2.kernel, wl2.P0, 2.1, max_iter=60)),
+    ] + [
+        # the per-k solver kernels with nb fixed at compile time (8, 12, 16)
+        # and at run time (6)
+        (f"layered{n}_nb{2 * nl}", _dm_args_p0(w.kernel, w.P0, w.n_electrons, max_iter=40))
+        for n, nl in ((12, 4), (12, 6), (8, 8), (12, 3))
+        for w in [layered_kernel(n, nl, T=0.05, filling=nl + 0.1)]
     ]
 
 
-@pytest.mark.parametrize("case", range(4))
+@pytest.mark.parametrize("case", range(8))
 def test_solve_dm_matches_oracle(gpu, oracle, case):
     name, args = _dm_cases()[case]
     print(f"solve_dm case {name}", flush=True)
@@ -324,6 +330,30 @@ def test_solve_dm_matches_oracle(gpu, oracle, case):
     assert relerr(P, r[2]) < 1e-9
     assert relerr(F, r[3]) < 1e-9
     np.testing.assert_allclose(p, r[1], atol=1e-9)
+
+
+@pytest.mark.parametrize("n,nl", [(12, 4), (12, 6), (8, 8)])
+def test_solve_scf_layered_matches_oracle(gpu, oracle, n, nl):
+    """SCF residual kernels with nb fixed at compile time (8, 12, 16)."""
+    from cpp_hf_amd.workloads import layered_kernel
+
+    w = layered_kernel(n, nl, T=0.05, filling=nl + 0.1)
+    args = (kernel_args_dict(w.kernel), w.P0, w.n_electrons, 30, 1e-9, 1e-8, 0.5, 0.0, None, [],
+            "linear", 6, 2, 1.0, 0.0, True, True)
+    print(f"solve_scf layered n={n} nb={2 * nl}", flush=True)
+    g = gpu.solve_scf(*args)
+    r = oracle.solve_scf(*args)
+    assert g[4] == r[4] and g[5] == r[5]
+    # relative bound on E (|E| ~ 14 at nb 16)
+    assert abs(g[2] - r[2]) < 1e-10 * max(1.0, abs(r[2]))
+    # mu is the midpoint of 30 bisection halvings of a bracket ~60 wide at
+    # nb 16 (resolution ~6e-8); one rounding-level `count > n_e` decision at
+    # the last levels moves it by that much
+    dmu = abs(g[3] - r[3])
+    assert dmu < 1e-7
+    # occupations move by at most dmu f (1 - f) / T <= dmu / (4 T)
+    assert relerr(g[0], r[0]) < max(1e-9, 2.0 * dmu / (4 * 0.05))
+    np.testing.assert_allclose(g[6], r[6], rtol=1e-10, atol=1e-10)
 
 
 @pytest.mark.parametrize("accel", ["linear", "diis", "oda"])
