@@ -87,10 +87,17 @@ __device__ __forceinline__ void store_block_partials(double (&v)[NS], double* re
 template <int MODE>
 __global__ void __launch_bounds__(256)
 k_finalize(const double* __restrict__ partial, int nparts, int ns, double* __restrict__ out,
-           double* sc, int nbt, double shrink, double T_r, double max_step) {
+           double* sc, int nbt, double shrink, double T_r, double max_step, int j_lo, int j_hi,
+           int final_stage, int gate) {
+    // MODE 1 (line search): test trials [j_lo, j_hi) in order (solver_dm.hpp:
+    // 575-587); the last stage applies the no-acceptance rule; gate: a
+    // trial accepted by an earlier stage stands
+    if (MODE == 1 && gate && sc[SC_ACCEPTED] > 0.5) return;
     __shared__ double s_sum[2 * kMaxBt];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int slot = wave; slot < ns; slot += 4) {
+    // MODE 1 needs only the slots of the trials it tests
+    const int s_lo = MODE == 1 ? 2 * j_lo : 0, s_hi = MODE == 1 ? 2 * j_hi : ns;
+    for (int slot = s_lo + wave; slot < s_hi; slot += 4) {
         double t = 0.0;
         for (int b = lane; b < nparts; b += 64) t += partial[b * ns + slot];
         t = wave_sum(t);
@@ -107,8 +114,9 @@ k_finalize(const double* __restrict__ partial, int nparts, int ns, double* __res
     const double Tv = fmax(T_r, 1.0e-14);
     const double om0 = sc[SC_E0] - Tv * sc[SC_S0];
     double t = tau0;
+    for (int j = 0; j < j_lo; ++j) t *= shrink;
     bool ok = false;
-    for (int j = 0; j < nbt; ++j) {
+    for (int j = j_lo; j < j_hi; ++j) {
         const double om = s_sum[2 * j] - Tv * s_sum[2 * j + 1];
         if (om < om0) {
             ok = true;
@@ -116,10 +124,10 @@ k_finalize(const double* __restrict__ partial, int nparts, int ns, double* __res
         }
         t *= shrink;
     }
-    if (!ok) t = tau0 * pow(shrink, static_cast<double>(nbt));
+    if (!ok && final_stage) t = tau0 * pow(shrink, static_cast<double>(nbt));
     sc[SC_TAU0] = tau0;
     sc[SC_OMEGA0] = om0;
-    sc[SC_TAU] = t;
+    if (ok || final_stage) sc[SC_TAU] = t;
     sc[SC_ACCEPTED] = ok ? 1.0 : 0.0;
 }
 
@@ -319,16 +327,23 @@ __device__ __forceinline__ void diag_at_tau(const cd* sV, const cd* sFe, const d
     __syncthreads();
 }
 
-// Slots: [2 * nbt trial (E_j, S_j)]; the last block then selects tau.
+// Backtracking trials j in [j_lo, j_hi) of the spectral line search
+// (solver_dm.hpp:561-589).  Slots: [2 kMaxBt trial sums (E_j, S_j)], each
+// trial reduced over the wavefront as it is evaluated.  Stage 1 (stage2 = 0)
+// computes Fe = V^H Ft V, writes it, and evaluates tau0; stage 2 returns at
+// once when the finaliser has accepted tau0, else reloads Fe and evaluates
+// the remaining trials.
 template <int W, int NB>
 __global__ void __launch_bounds__(W * 64)
 k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restrict__ lam,
              const cd* __restrict__ Ft, cd* __restrict__ Fe, const double* __restrict__ p,
              const double* __restrict__ d_p, const double* __restrict__ w, double T_r,
-             double max_step, double shrink, int nbt, double* partial, const double* sc) {
+             double max_step, double shrink, int j_lo, int j_hi, int stage2,
+             double* partial, const double* sc) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
+    if (stage2 && sc[SC_ACCEPTED] > 0.5) return;  // uniform over the grid
     extern __shared__ cd smem[];
-    __shared__ double red[W * 2 * kMaxBt];
+    __shared__ double racc[W][2 * kMaxBt];
     const int nb2 = nb * nb;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     cd* sV = smem + wave * ls_wave_cd(nb);
@@ -338,11 +353,11 @@ k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restr
     cd* scf = sVC + nb2;
     double* srow = reinterpret_cast<double*>(scf + nb);
     double* slam = srow + nb2;
+    for (int i = lane; i < 2 * kMaxBt; i += 64) racc[wave][i] = 0.0;
     const double dqn = sqrt(fmax(0.0, sc[SC_DQN2]));
     const double tau0 = fmin(1.0, max_step / fmax(dqn, TINY_REAL));
-    double acc[2 * kMaxBt];
-#pragma unroll
-    for (int s = 0; s < 2 * kMaxBt; ++s) acc[s] = 0.0;
+    double tau_lo = tau0;  // tau0 shrink^j_lo by repeated products, as the reference steps
+    for (int j = 0; j < j_lo; ++j) tau_lo *= shrink;
     const int nsteps = (nk + gridDim.x * W - 1) / (gridDim.x * W);
     for (int step = 0; step < nsteps; ++step) {
         const int k = (step * gridDim.x + blockIdx.x) * W + wave;
@@ -351,22 +366,23 @@ k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restr
         if (act) {
             for (int e = lane; e < nb2; e += 64) {
                 sV[e] = V[kb + e];
-                sF[e] = Ft[kb + e];
+                sF[e] = stage2 ? Fe[kb + e] : Ft[kb + e];
             }
             for (int i = lane; i < nb; i += 64) slam[i] = lam[static_cast<size_t>(k) * nb + i];
         }
         __syncthreads();
-        if (act) wave_mm(sV, sF, sT, nb, true, false, lane);
-        __syncthreads();
-        if (act) {
-            wave_mm(sT, sV, sF, nb, false, false, lane);  // sF <- Fe
+        if (!stage2) {
+            if (act) wave_mm(sV, sF, sT, nb, true, false, lane);
+            __syncthreads();
+            if (act) wave_mm(sT, sV, sF, nb, false, false, lane);  // sF <- Fe
+            __syncthreads();
+            if (act)
+                for (int e = lane; e < nb2; e += 64) Fe[kb + e] = sF[e];
         }
-        __syncthreads();
-        if (act)
-            for (int e = lane; e < nb2; e += 64) Fe[kb + e] = sF[e];
-        double tau = tau0;
-        for (int j = 0; j < nbt; ++j) {
+        double tau = tau_lo;
+        for (int j = j_lo; j < j_hi; ++j) {
             diag_at_tau(sV, sF, slam, sVC, scf, srow, nb, tau, lane);
+            double ej = 0.0, sj = 0.0;
             if (act) {
                 const double wk = w[k];
                 for (int i = lane; i < nb; i += 64) {
@@ -374,15 +390,26 @@ k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restr
                     for (int jj = 0; jj < nb; ++jj) di += srow[i * nb + jj];
                     const size_t idx = static_cast<size_t>(k) * nb + i;
                     const double pv = clampd(p[idx] - tau * d_p[idx], OCC_CLIP, 1.0 - OCC_CLIP);
-                    acc[2 * j] += wk * pv * di;
-                    acc[2 * j + 1] -= wk * (pv * log(pv) + (1.0 - pv) * log1p(-pv));
+                    ej += wk * pv * di;
+                    sj -= wk * (pv * log(pv) + (1.0 - pv) * log1p(-pv));
                 }
+            }
+            ej = wave_sum(ej);
+            sj = wave_sum(sj);
+            if (lane == 0) {
+                racc[wave][2 * j] += ej;
+                racc[wave][2 * j + 1] += sj;
             }
             __syncthreads();
             tau *= shrink;
         }
     }
-    store_block_partials<2 * kMaxBt, W>(acc, red, partial);
+    __syncthreads();
+    if (threadIdx.x < 2 * kMaxBt) {
+        double t = 0.0;
+        for (int wv = 0; wv < W; ++wv) t += racc[wv][threadIdx.x];
+        partial[blockIdx.x * 2 * kMaxBt + threadIdx.x] = t;
+    }
 }
 
 // Q <- Q V diag(c(tau)) V^H and eps_work = diag(U^H Ft U) at the chosen tau.
@@ -573,7 +600,7 @@ static int waves_for(int nb) { return nb <= 16 ? 4 : 1; }
 
 static void finalize(LoopScratch& ls, int nparts, int ns, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_finalize<0>, dim3(1), dim3(256), 0, st, ls.partial, nparts, ns, out,
-                       ls.sc, 0, 0.0, 0.0, 0.0);
+                       ls.sc, 0, 0.0, 0.0, 0.0, 0, 0, 0, 0);
     HFGPU_LAUNCH_CHECK();
 }
 
@@ -619,13 +646,21 @@ void launch_linesearch(int nk, int nb, const cd* V, const double* lam, const cd*
     if (nbt < 0 || nbt > kMaxBt)
         throw std::invalid_argument("bt_max above the GPU line-search limit of 16");
     const size_t per_wave = ls_wave_bytes(nb);
-    HFGPU_LOOP_LAUNCH(k_linesearch, nb, nk, per_wave, st, nk, nb, V, lam, Ft, Fe, p, d_p, w, T_r,
-                      max_step, shrink, nbt, ls.partial, ls.sc);
-    HFGPU_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(256), 0, st, ls.partial,
-                       static_cast<int>(loop_blocks(nk, waves_for(nb))), 2 * kMaxBt,
-                       ls.sc + SC_TRIAL, ls.sc, nbt, shrink, T_r, max_step);
-    HFGPU_LAUNCH_CHECK();
+    const int nparts = static_cast<int>(loop_blocks(nk, waves_for(nb)));
+    // stage 1: Fe and tau0, which the reference's backtracking usually accepts;
+    // stage 2 (device-gated): the remaining trials only when tau0 was rejected
+    const int first = std::min(nbt, 1);
+    for (int stage = 0; stage < (nbt > 1 ? 2 : 1); ++stage) {
+        const int j_lo = stage == 0 ? 0 : 1, j_hi = stage == 0 ? first : nbt;
+        const int last = stage == 1 || nbt <= 1;
+        HFGPU_LOOP_LAUNCH(k_linesearch, nb, nk, per_wave, st, nk, nb, V, lam, Ft, Fe, p, d_p, w,
+                          T_r, max_step, shrink, j_lo, j_hi, stage, ls.partial, ls.sc);
+        HFGPU_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(256), 0, st, ls.partial, nparts,
+                           2 * kMaxBt, ls.sc + SC_TRIAL, ls.sc, nbt, shrink, T_r, max_step, j_lo,
+                           j_hi, last, stage);
+        HFGPU_LAUNCH_CHECK();
+    }
 }
 
 void launch_retract(int nk, int nb, cd* Q, const cd* V, const double* lam, const cd* Fe,
