@@ -236,7 +236,7 @@ __device__ __forceinline__ cd group_bcast(cd v, int src) {
     return cd{__shfl(v.x, src, 64), __shfl(v.y, src, 64)};
 }
 
-template <int NP>
+template <int NP, int NBC>
 __global__ void __launch_bounds__(256)
 k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __restrict__ w_out,
            cd* __restrict__ V_out, BlockMap bm, int sort, const double* __restrict__ cond) {
@@ -245,6 +245,9 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
     const bool ub = !cond || cond[0] <= 1.0e-10 + 1.0e-10 * cond[1];
     if (!ub) sort = 1;
     constexpr int G = 64 / NP;  // matrices per wavefront
+    // matrix dimension of the register rows: NBC when the launch fixes nb at
+    // compile time (lanes NBC..NP-1 of a group are then padding), else NP
+    constexpr int NL = NBC > 0 ? NBC : NP;
     constexpr double kEps = 2.220446049250313e-16;
     __shared__ TriLds<NP> lds[4];
     const int lane = threadIdx.x & 63;
@@ -255,9 +258,9 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
     const bool live = mat < nmat;
     const int nb2 = nb * nb;
     const cd* Mk = M + static_cast<size_t>(live ? mat : 0) * nb2;
-    cd A[NP], Z[NP];
+    cd A[NL], Z[NL];
 #pragma unroll
-    for (int j = 0; j < NP; ++j) {
+    for (int j = 0; j < NL; ++j) {
         cd h{0.0, 0.0};
         if (live && i < nb && j < nb) {
             cd xij = Mk[i * nb + j], xji = Mk[j * nb + i];
@@ -279,7 +282,7 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
     for (int k = 0; k < nb - 2; ++k) {
         cd ak{0.0, 0.0};
 #pragma unroll
-        for (int j = 0; j < NP; ++j)
+        for (int j = 0; j < NL; ++j)
             if (j == k) ak = A[j];
         const cd xi = (i >= k + 1 && i < nb) ? ak : cd{0.0, 0.0};
         double part = i >= k + 2 ? cnorm(xi) : 0.0;
@@ -300,7 +303,7 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
         wave_sync();
         cd p{0.0, 0.0}, sz{0.0, 0.0};
 #pragma unroll
-        for (int j = 1; j < NP; ++j) {
+        for (int j = 1; j < NL; ++j) {
             if (j <= k) continue;
             const cd uj = X.u[gbase + j];
             p = cadd(p, cmul(A[j], uj));
@@ -316,7 +319,7 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
         wave_sync();
         const cd bsz = cscale(sz, beta);
 #pragma unroll
-        for (int j = 0; j < NP; ++j) {
+        for (int j = 0; j < NL; ++j) {
             const cd uj = j > k ? X.u[gbase + j] : cd{0.0, 0.0};
             const cd wj = X.w[gbase + j];
             // A_ij -= u_i conj(w_j) + w_i conj(u_j)
@@ -329,7 +332,7 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
             // exact values of column and row k after the reflection
             const cd gamma{-ph.x * sigma, -ph.y * sigma};
 #pragma unroll
-            for (int j = 0; j < NP; ++j) {
+            for (int j = 0; j < NL; ++j) {
                 if (j == k) {
                     if (i == k + 1) A[j] = gamma;
                     else if (i >= k + 2) A[j] = cd{0.0, 0.0};
@@ -343,20 +346,20 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
     double dmine = 0.0;
     cd tmine{0.0, 0.0};  // T[i][i-1]
 #pragma unroll
-    for (int j = 0; j < NP; ++j) {
+    for (int j = 0; j < NL; ++j) {
         if (j == i) dmine = A[j].x;
         if (j + 1 == i) tmine = A[j];
     }
     X.d[lane] = dmine;
     X.t[lane] = tmine;
     wave_sync();
-    double d[NP], e[NP];
+    double d[NL], e[NL];
     cd ph{1.0, 0.0};
 #pragma unroll
-    for (int j = 0; j < NP; ++j) {
+    for (int j = 0; j < NL; ++j) {
         d[j] = X.d[gbase + j];
         Z[j] = cmul(Z[j], ph);
-        if (j + 1 < NP) {
+        if (j + 1 < NL) {
             const cd t = X.t[gbase + j + 1];
             const double a2 = cnorm(t);
             const double ia = a2 > 0.0 ? rsqrt(a2) : 0.0;
@@ -369,22 +372,22 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
     wave_sync();
 
     // ---- 3. implicit QR with Wilkinson shift
-    for (int it = 0; it < 30 * NP; ++it) {
+    for (int it = 0; it < 30 * NL; ++it) {
 #pragma unroll
-        for (int k = 0; k + 1 < NP; ++k)
+        for (int k = 0; k + 1 < NL; ++k)
             if (fabs(e[k]) <= kEps * (fabs(d[k]) + fabs(d[k + 1]))) e[k] = 0.0;
         int m = -1;
 #pragma unroll
-        for (int k = 0; k + 1 < NP; ++k)
+        for (int k = 0; k + 1 < NL; ++k)
             if (e[k] != 0.0) m = k + 1;
         if (__all(m < 0)) break;
         int l = m;
 #pragma unroll
-        for (int k = NP - 2; k >= 0; --k)
+        for (int k = NL - 2; k >= 0; --k)
             if (k == l - 1 && e[k] != 0.0) l = k;
         double am = 0.0, am1 = 0.0, bm1 = 0.0, x = 0.0, z = 0.0;
 #pragma unroll
-        for (int k = 0; k < NP; ++k) {
+        for (int k = 0; k < NL; ++k) {
             if (k == m) am = d[k];
             if (k + 1 == m) {
                 am1 = d[k];
@@ -400,7 +403,7 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
         const double mu = m >= 0 ? am - bm1 * (bm1 / den) : 0.0;
         x -= mu;
 #pragma unroll
-        for (int k = 0; k + 1 < NP; ++k) {
+        for (int k = 0; k + 1 < NL; ++k) {
             if (k < l || k >= m) continue;
             // r = |(x, z)|, c = x / r, s = z / r from one reciprocal square
             // root (a Givens rotation only needs c^2 + s^2 = 1 to rounding)
@@ -414,7 +417,7 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
             d[k] = cc * a0 + 2.0 * cs * b0 + ss * a1;
             d[k + 1] = ss * a0 - 2.0 * cs * b0 + cc * a1;
             e[k] = cs * (a1 - a0) + (cc - ss) * b0;
-            if (k + 2 < NP && k + 1 < m) {
+            if (k + 2 < NL && k + 1 < m) {
                 const double b1 = e[k + 1];
                 x = e[k];
                 z = s * b1;
@@ -429,19 +432,19 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
     // ---- ascending, stable order (within each block unless sort)
     double di = 0.0;
 #pragma unroll
-    for (int j = 0; j < NP; ++j)
+    for (int j = 0; j < NL; ++j)
         if (j == i) di = d[j];
     const int bi = i < nb ? bm.blk[i] : -1;
     int rank = (sort || i >= nb) ? 0 : bm.start[i];
 #pragma unroll
-    for (int j = 0; j < NP; ++j) {
+    for (int j = 0; j < NL; ++j) {
         const bool same = sort || (j < nb && bm.blk[j] == bi);
         if (j < nb && j != i && same && (d[j] < di || (d[j] == di && j < i))) ++rank;
     }
     if (live && i < nb) w_out[static_cast<size_t>(mat) * nb + rank] = di;
     cd* Vk = V_out + static_cast<size_t>(live ? mat : 0) * nb2;
 #pragma unroll
-    for (int j = 0; j < NP; ++j) {
+    for (int j = 0; j < NL; ++j) {
         const int rj = __shfl(rank, gbase + j, 64);
         if (live && i < nb && j < nb) Vk[i * nb + rj] = Z[j];
     }
@@ -460,13 +463,15 @@ void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, c
         const int per_block = 4 * (64 / np);
         const dim3 grid(cdiv(nmat, per_block));
         if (np == 2)
-            hipLaunchKernelGGL(k_eigh_tri<2>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
+            hipLaunchKernelGGL((k_eigh_tri<2, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
         else if (np == 4)
-            hipLaunchKernelGGL(k_eigh_tri<4>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
+            hipLaunchKernelGGL((k_eigh_tri<4, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
         else if (np == 8)
-            hipLaunchKernelGGL(k_eigh_tri<8>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
+            hipLaunchKernelGGL((k_eigh_tri<8, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
+        else if (nb == 12)  // the 12-band models: loops sized to 12, not 16
+            hipLaunchKernelGGL((k_eigh_tri<16, 12>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
         else
-            hipLaunchKernelGGL(k_eigh_tri<16>, grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
+            hipLaunchKernelGGL((k_eigh_tri<16, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
         HFGPU_LAUNCH_CHECK();
         return;
     }
