@@ -20,6 +20,7 @@ from .utils import resample_kgrid
 from .sweep import solve_many
 from .continuation import ContinuationResult, solve_continuation
 from .symmetry import make_project_fn
+from .deflation import DeflatedResult, solve_deflated
 
 __all__ = [
     "HartreeFockKernel",
@@ -39,4 +40,6 @@ __all__ = [
     "ContinuationResult",
     "solve_continuation",
     "make_project_fn",
+    "DeflatedResult",
+    "solve_deflated",
 ]

@@ -69,7 +69,12 @@ typedef struct hf_scf_config {
     size_t n_blocks;
 } hf_scf_config;
 
-/* Reference: SolverConfig, solver_dm.hpp:22-63 (optimizer "cg" only). */
+/* Reference: SolverConfig, solver_dm.hpp:22-63.  optimizer 0: Riemannian CG
+ * (solver_dm.hpp:392-653); 1: trust-region Newton (solver_rtr.hpp:162-459)
+ * with tr_delta0, tr_cg_max and the deflation bias around n_deflation
+ * densities (each nk1 x nk2 x nb x nb complex, interleaved) when
+ * deflation_sigma > 0.  Zero-initialised trailing fields select CG without
+ * deflation. */
 typedef struct hf_dm_config {
     size_t max_iter;
     double tol_E;
@@ -83,6 +88,13 @@ typedef struct hf_dm_config {
     int mu_maxiter;
     const size_t* block_sizes;
     size_t n_blocks;
+    int optimizer;
+    double tr_delta0;
+    size_t tr_cg_max;
+    const double* deflation_targets; /* may be NULL when n_deflation == 0 */
+    size_t n_deflation;
+    double deflation_sigma;
+    double deflation_length;
 } hf_dm_config;
 
 /*

@@ -1167,6 +1167,360 @@ void solve_dm(const Kernel& K, const c64* P0, double n_e, const hf_dm_config& cf
 // Roothaan SCF (solver_scf.hpp)
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// Trust-region Newton (solver_rtr.hpp)
+// ---------------------------------------------------------------------------
+
+// solver_rtr.hpp:32-39: R[dP] = F[dP] - h with the reference density switched
+// off (the self-energy, Hartree and contact terms are linear in the density)
+void fock_response(const Kernel& K, const c64* dP, c64* resp, c64* Sg, c64* Fb, double* hb) {
+    Kernel Kr = K;
+    Kr.has_refP = false;
+    Kr.refP = nullptr;
+    build_fock_compact(Kr, dP, Sg, Fb, hb, nullptr);
+    for (size_t i = 0; i < K.n_dense(); ++i) resp[i] = Fb[i] - K.h[i];
+}
+
+// solver_rtr.hpp:42-50: remove the w-weighted mean so sum_k w_k sum_b v = 0
+void project_occ(double* v, const double* w, size_t nk, size_t nb) {
+    double num = 0.0, den = 0.0;
+    for (size_t k = 0; k < nk; ++k) {
+        for (size_t b = 0; b < nb; ++b) num += w[k] * v[k * nb + b];
+        den += w[k] * static_cast<double>(nb);
+    }
+    const double shift = den > 0.0 ? num / den : 0.0;
+    for (size_t i = 0; i < nk * nb; ++i) v[i] -= shift;
+}
+
+// solver_rtr.hpp:55-112: joint (Q, p) Hessian-vector product, one Fock
+// response; pen adds the deflation curvature pen * dP to the response.
+void joint_hvp(const Kernel& K, const c64* X, const double* dp, const c64* Q, const double* p,
+               const c64* Ft, double T_r, c64* HX, double* Hp, c64* dPb, c64* rb, c64* Sb,
+               c64* Fb, double* hb, double pen) {
+    const size_t nk = K.nk(), nb = K.nb, nb2 = nb * nb;
+    parallel_for(nk, [&](size_t k) {
+        std::vector<c64> M(nb2), T1(nb2), D(nb2);
+        const double* pk = p + k * nb;
+        for (size_t i = 0; i < nb; ++i)
+            for (size_t j = 0; j < nb; ++j)
+                M[i * nb + j] = i == j ? c64(dp[k * nb + i], 0.0)
+                                       : X[k * nb2 + i * nb + j] * (pk[j] - pk[i]);
+        matmul(Q + k * nb2, M.data(), T1.data(), nb, 0);
+        matmul(T1.data(), Q + k * nb2, D.data(), nb, 2);
+        for (size_t i = 0; i < nb; ++i)
+            for (size_t j = 0; j < nb; ++j)
+                dPb[k * nb2 + i * nb + j] = 0.5 * (D[i * nb + j] + std::conj(D[j * nb + i]));
+    });
+    fock_response(K, dPb, rb, Sb, Fb, hb);
+    if (pen != 0.0)
+        for (size_t i = 0; i < K.n_dense(); ++i) rb[i] += c64(pen, 0.0) * dPb[i];
+    parallel_for(nk, [&](size_t k) {
+        std::vector<c64> T1(nb2), St(nb2), A(nb2), C(nb2);
+        const c64* Fk = Ft + k * nb2;
+        const c64* Xk = X + k * nb2;
+        matmul(Q + k * nb2, rb + k * nb2, T1.data(), nb, 1);
+        matmul(T1.data(), Q + k * nb2, St.data(), nb, 0);
+        for (size_t i = 0; i < nb; ++i)
+            for (size_t j = 0; j < nb; ++j) {
+                c64 fx(0.0, 0.0), xf(0.0, 0.0);
+                for (size_t m = 0; m < nb; ++m) {
+                    fx += Fk[i * nb + m] * Xk[m * nb + j];
+                    xf += Xk[i * nb + m] * Fk[m * nb + j];
+                }
+                A[i * nb + j] = fx - xf + St[i * nb + j];
+            }
+        const double* pk = p + k * nb;
+        const double* dk = dp + k * nb;
+        for (size_t i = 0; i < nb; ++i)
+            for (size_t j = 0; j < nb; ++j)
+                C[i * nb + j] = (dk[j] - dk[i]) * Fk[i * nb + j] + (pk[j] - pk[i]) * A[i * nb + j];
+        for (size_t i = 0; i < nb; ++i)
+            for (size_t j = 0; j < nb; ++j)
+                HX[k * nb2 + i * nb + j] =
+                    i == j ? c64(0.0, 0.0) : 0.5 * (C[i * nb + j] - std::conj(C[j * nb + i]));
+        for (size_t b = 0; b < nb; ++b) {
+            const double pc = std::clamp(pk[b], OCC_CLIP, 1.0 - OCC_CLIP);
+            Hp[k * nb + b] = A[b * nb + b].real() + T_r / (pc * (1.0 - pc)) * dk[b];
+        }
+    });
+    project_occ(Hp, K.w2d, nk, nb);
+}
+
+// solver_rtr.hpp:127-160: Gaussian repulsion around found densities;
+// returns Phi, writes S_pen (dPhi/dP_k / w_k) and pen = 2 sum_i phi'(d_i^2).
+double deflation_bias(const c64* P, const c64* targets, size_t n_defl, const double* w,
+                      size_t nk, size_t nb, double sigma, double length, c64* S_pen,
+                      double& pen) {
+    const size_t nb2 = nb * nb, n_tot = nk * nb2;
+    std::fill(S_pen, S_pen + n_tot, c64(0.0, 0.0));
+    pen = 0.0;
+    if (n_defl == 0 || sigma == 0.0 || !targets) return 0.0;
+    const double inv2L2 = 1.0 / (2.0 * length * length);
+    double Phi = 0.0, diag = 0.0;
+    for (size_t t = 0; t < n_defl; ++t) {
+        const c64* Tt = targets + t * n_tot;
+        double d2 = 0.0;
+        for (size_t k = 0; k < nk; ++k) {
+            double per = 0.0;
+            for (size_t a = 0; a < nb2; ++a) per += std::norm(P[k * nb2 + a] - Tt[k * nb2 + a]);
+            d2 += w[k] * per;
+        }
+        const double phi = sigma * std::exp(-d2 * inv2L2);
+        const double dphi = -phi * inv2L2;
+        Phi += phi;
+        diag += dphi;
+        for (size_t i = 0; i < n_tot; ++i) S_pen[i] += c64(2.0 * dphi, 0.0) * (P[i] - Tt[i]);
+    }
+    pen = 2.0 * diag;
+    return Phi;
+}
+
+// solver_rtr.hpp:162-459
+void solve_rtr(const Kernel& K, const c64* P0, double n_e, const hf_dm_config& cfg,
+               const Projector* proj, hf_dm_result* out) {
+    const size_t nk = K.nk(), nb = K.nb, nb2 = nb * nb, n_tot = nk * nb2, npb = nk * nb;
+    const double T_r = std::max(K.T, 1.0e-12);
+    const double tol_g = cfg.tol_grad > 0.0 ? cfg.tol_grad : 1.0e-6;
+    std::vector<double> w_norm(nk), ones(nk, 1.0);
+    const double ws = std::max(K.weight_sum, TINY_REAL);
+    for (size_t i = 0; i < nk; ++i) w_norm[i] = K.w2d[i] / ws;
+    const double n_target = n_e / ws;
+
+    std::vector<c64> Q(n_tot), P(n_tot), Sigma(n_tot), F(n_tot), Ft(n_tot), G(n_tot);
+    std::vector<c64> vX(n_tot), dPb(n_tot), rb(n_tot), Fb(n_tot), Sb(n_tot);
+    std::vector<c64> rX(n_tot), dX(n_tot), zX(n_tot), HvX(n_tot), HdX(n_tot);
+    std::vector<c64> Vd(n_tot), Qt(n_tot), U(n_tot), Gs(n_tot), Fts(n_tot);
+    std::vector<double> p(npb), eps(npb), hdiag(nb, 0.0), hb(nb, 0.0), vp(npb), rp(npb),
+        dpv(npb), zp(npb), Hvp(npb), Hdp(npb), lam(npb), eps_s(npb), eps_t(npb), p_t(npb);
+    std::vector<double> hist_E(cfg.max_iter, 0.0), hist_G(cfg.max_iter, 0.0);
+
+    const size_t n_defl = cfg.n_deflation;
+    const c64* targets = reinterpret_cast<const c64*>(cfg.deflation_targets);
+    const double d_sigma = cfg.deflation_sigma;
+    const double d_len = std::max(cfg.deflation_length, 1.0e-12);
+    const bool defl_on = n_defl > 0 && d_sigma > 0.0 && targets;
+    std::vector<c64> S_pen(defl_on ? n_tot : 0);
+    double pen = 0.0;
+
+    // joint metric: orbital part unweighted, occupation part w2d-weighted
+    auto jip = [&](const c64* aX, const double* ap, const c64* bX, const double* bp) {
+        return ip_matrix(aX, bX, ones.data(), nk, nb) + ip_vec(ap, bp, K.w2d, nk, nb);
+    };
+    auto jnorm = [&](const c64* aX, const double* ap) {
+        return std::sqrt(std::max(0.0, jip(aX, ap, aX, ap)));
+    };
+    // block preconditioner: orbital 1 / sqrt(gap^2 + lam^2), occupation p (1 - p) / T
+    auto precond = [&](const c64* aX, const double* ap, c64* oX, double* op, double lam_pc) {
+        for (size_t k = 0; k < nk; ++k)
+            for (size_t i = 0; i < nb; ++i)
+                for (size_t j = 0; j < nb; ++j) {
+                    const double gap = eps[k * nb + i] - eps[k * nb + j];
+                    oX[k * nb2 + i * nb + j] =
+                        aX[k * nb2 + i * nb + j] / std::sqrt(gap * gap + lam_pc * lam_pc);
+                }
+        for (size_t i = 0; i < npb; ++i) {
+            const double pc = std::clamp(p[i], OCC_CLIP, 1.0 - OCC_CLIP);
+            op[i] = ap[i] * pc * (1.0 - pc) / T_r;
+        }
+        project_occ(op, K.w2d, nk, nb);
+    };
+    auto project = [&](c64* M) {
+        if (proj && *proj) (*proj)(M, K);
+    };
+    // density, Fock (+ deflation), Ft, eps and G at (Qs, ps); returns E (+ Phi)
+    auto eval_state = [&](const c64* Qs, const double* ps) {
+        density_from_Qp(Qs, ps, P.data(), nk, nb);
+        project(P.data());
+        hermitize_inplace(P.data(), nk, nb);
+        build_fock_compact(K, P.data(), Sigma.data(), F.data(), hdiag.data(), nullptr);
+        double E = hf_energy_with_hartree_diag(K, P.data(), Sigma.data(), hdiag.data());
+        if (defl_on) {
+            E += deflation_bias(P.data(), targets, n_defl, K.w2d, nk, nb, d_sigma, d_len,
+                                S_pen.data(), pen);
+            for (size_t i = 0; i < n_tot; ++i) F[i] += S_pen[i];
+        }
+        fock_in_orbital_basis(Qs, F.data(), Ft.data(), nk, nb);
+        for (size_t k = 0; k < nk; ++k)
+            for (size_t b = 0; b < nb; ++b) eps[k * nb + b] = Ft[k * nb2 + b * nb + b].real();
+        compute_orbital_gradient(Ft.data(), ps, G.data(), nk, nb);
+        return E;
+    };
+
+    // init (solver_rtr.hpp:262-272)
+    std::memcpy(P.data(), P0, n_tot * sizeof(c64));
+    hermitize_inplace(P.data(), nk, nb);
+    build_fock_compact(K, P.data(), Sigma.data(), F.data(), hdiag.data(), nullptr);
+    eigh_batched(F.data(), eps.data(), Q.data(), nk, nb);
+    double mu = solve_mu_inloop(eps.data(), npb, w_norm.data(), nk, nb, n_target, 0.0, T_r,
+                                cfg.mu_maxiter);
+    for (size_t i = 0; i < npb; ++i) p[i] = expit_scalar((mu - eps[i]) / T_r);
+
+    double E = eval_state(Q.data(), p.data());
+    double Delta = cfg.tr_delta0;
+    size_t k = 0;
+    bool converged = false;
+    while (k < cfg.max_iter) {
+        const double gn = norm_matrix(G.data(), w_norm.data(), nk, nb);
+        hist_E[k] = E;
+        hist_G[k] = gn;
+        if (gn < tol_g) {
+            converged = true;
+            ++k;
+            break;
+        }
+        double eps_sq = 0.0;
+        for (size_t i = 0; i < npb; ++i) eps_sq += eps[i] * eps[i];
+        const double lam_pc = std::max(
+            T_r, cfg.denom_scale *
+                     std::sqrt(eps_sq / static_cast<double>(std::max<size_t>(npb, 1)) + TINY_REAL));
+
+        // Steihaug truncated CG on the joint quadratic model (288-347)
+        std::fill(vX.begin(), vX.end(), c64(0.0, 0.0));
+        std::fill(vp.begin(), vp.end(), 0.0);
+        std::fill(HvX.begin(), HvX.end(), c64(0.0, 0.0));
+        std::fill(Hvp.begin(), Hvp.end(), 0.0);
+        for (size_t i = 0; i < n_tot; ++i) rX[i] = -G[i];
+        std::fill(rp.begin(), rp.end(), 0.0);
+        precond(rX.data(), rp.data(), zX.data(), zp.data(), lam_pc);
+        dX = zX;
+        dpv = zp;
+        double rz = jip(rX.data(), rp.data(), zX.data(), zp.data());
+        for (size_t it = 0; it < cfg.tr_cg_max; ++it) {
+            joint_hvp(K, dX.data(), dpv.data(), Q.data(), p.data(), Ft.data(), T_r, HdX.data(),
+                      Hdp.data(), dPb.data(), rb.data(), Sb.data(), Fb.data(), hb.data(), pen);
+            const double dHd = jip(dX.data(), dpv.data(), HdX.data(), Hdp.data());
+            const double dd = jip(dX.data(), dpv.data(), dX.data(), dpv.data());
+            auto to_boundary = [&]() {
+                const double a = dd;
+                const double b = 2.0 * jip(vX.data(), vp.data(), dX.data(), dpv.data());
+                const double c = jnorm(vX.data(), vp.data());
+                const double cc = c * c - Delta * Delta;
+                const double t =
+                    (-b + std::sqrt(std::max(b * b - 4.0 * a * cc, 0.0))) / (2.0 * a + 1e-30);
+                for (size_t i = 0; i < n_tot; ++i) {
+                    vX[i] += t * dX[i];
+                    HvX[i] += t * HdX[i];
+                }
+                for (size_t i = 0; i < npb; ++i) {
+                    vp[i] += t * dpv[i];
+                    Hvp[i] += t * Hdp[i];
+                }
+            };
+            if (dHd <= 1e-14 * dd) {  // negative curvature
+                to_boundary();
+                break;
+            }
+            const double alpha = rz / dHd;
+            double s2 = 0.0;
+            for (size_t i = 0; i < n_tot; ++i) s2 += std::norm(vX[i] + alpha * dX[i]);
+            for (size_t kk = 0; kk < nk; ++kk) {
+                double per = 0.0;
+                for (size_t b = 0; b < nb; ++b) {
+                    const double e = vp[kk * nb + b] + alpha * dpv[kk * nb + b];
+                    per += e * e;
+                }
+                s2 += K.w2d[kk] * per;
+            }
+            if (std::sqrt(std::max(0.0, s2)) >= Delta) {
+                to_boundary();
+                break;
+            }
+            for (size_t i = 0; i < n_tot; ++i) {
+                vX[i] += alpha * dX[i];
+                HvX[i] += alpha * HdX[i];
+                rX[i] -= alpha * HdX[i];
+            }
+            for (size_t i = 0; i < npb; ++i) {
+                vp[i] += alpha * dpv[i];
+                Hvp[i] += alpha * Hdp[i];
+                rp[i] -= alpha * Hdp[i];
+            }
+            const double rn = jnorm(rX.data(), rp.data());
+            if (rn < 0.05 * gn) break;
+            precond(rX.data(), rp.data(), zX.data(), zp.data(), lam_pc);
+            const double rz_new = jip(rX.data(), rp.data(), zX.data(), zp.data());
+            const double beta = rz_new / rz;
+            for (size_t i = 0; i < n_tot; ++i) dX[i] = zX[i] + beta * dX[i];
+            for (size_t i = 0; i < npb; ++i) dpv[i] = zp[i] + beta * dpv[i];
+            rz = rz_new;
+        }
+
+        // predicted reduction in energy units (w2d-weighted, 349-361)
+        const double gv = ip_matrix(G.data(), vX.data(), K.w2d, nk, nb);
+        const double vHv = ip_matrix(vX.data(), HvX.data(), K.w2d, nk, nb) +
+                           ip_vec(vp.data(), Hvp.data(), K.w2d, nk, nb);
+        const double pred = -(gv + 0.5 * vHv);
+        if (Delta < 1e-10) break;
+
+        // retraction along +vX (Cayley at tau = -1), frozen-Ft trial energies,
+        // occupations at the re-solved mu (363-385)
+        cayley_spectral_setup(vX.data(), Vd.data(), lam.data(), nk, nb);
+        cayley_unitary_from_spectrum(Vd.data(), lam.data(), -1.0, U.data(), nk, nb);
+        Q_times_U(Q.data(), U.data(), Qt.data(), nk, nb);
+        parallel_for(nk, [&](size_t kk) {
+            std::vector<c64> T1(nb2), T2(nb2);
+            matmul(U.data() + kk * nb2, Ft.data() + kk * nb2, T1.data(), nb, 1);
+            matmul(T1.data(), U.data() + kk * nb2, T2.data(), nb, 0);
+            for (size_t b = 0; b < nb; ++b) eps_t[kk * nb + b] = T2[b * nb + b].real();
+        });
+        const double mu_t = solve_mu_inloop(eps_t.data(), npb, w_norm.data(), nk, nb, n_target,
+                                            mu, T_r, cfg.mu_maxiter);
+        for (size_t i = 0; i < npb; ++i) p_t[i] = expit_scalar((mu_t - eps_t[i]) / T_r);
+
+        Gs = G;
+        Fts = Ft;
+        eps_s = eps;
+        const double pen_s = pen;
+        const double E_t = eval_state(Qt.data(), p_t.data());
+        const double g_t = norm_matrix(G.data(), w_norm.data(), nk, nb);
+
+        // acceptance and trust radius (399-420)
+        const double noise = 1e-12 * std::max(std::fabs(E), 1.0);
+        bool accept;
+        if (pred <= noise) {
+            accept = g_t <= gn && E_t <= E + noise;
+            if (!accept) Delta *= 0.5;
+        } else {
+            const double ratio = (E - E_t) / pred;
+            if (ratio < 0.25)
+                Delta *= 0.25;
+            else if (ratio > 0.75 && jnorm(vX.data(), vp.data()) > 0.9 * Delta)
+                Delta = std::min(2.0 * Delta, 5.0);
+            accept = ratio > 0.1;
+        }
+        if (accept) {
+            Q = Qt;
+            p = p_t;
+            E = E_t;
+            mu = mu_t;
+        } else {
+            G = Gs;
+            Ft = Fts;
+            eps = eps_s;
+            pen = pen_s;
+        }
+        ++k;
+    }
+
+    // finalize (436-441)
+    density_from_Qp(Q.data(), p.data(), P.data(), nk, nb);
+    project(P.data());
+    hermitize_inplace(P.data(), nk, nb);
+    build_fock_compact(K, P.data(), Sigma.data(), F.data(), hdiag.data(), nullptr);
+    const double E_fin = hf_energy_with_hartree_diag(K, P.data(), Sigma.data(), hdiag.data());
+
+    if (out->Q) std::memcpy(out->Q, Q.data(), n_tot * sizeof(c64));
+    std::memcpy(out->p, p.data(), npb * sizeof(double));
+    if (out->density) std::memcpy(out->density, P.data(), n_tot * sizeof(c64));
+    if (out->fock) std::memcpy(out->fock, F.data(), n_tot * sizeof(c64));
+    out->mu = mu;
+    out->energy = E_fin;
+    out->n_iter = k;
+    out->converged = converged ? 1 : 0;
+    if (out->hist_E) std::memcpy(out->hist_E, hist_E.data(), k * sizeof(double));
+    if (out->hist_grad) std::memcpy(out->hist_grad, hist_G.data(), k * sizeof(double));
+}
+
 // solver_scf.hpp:64-75
 double weighted_matrix_norm(const c64* M, const double* w, size_t nk, size_t nb, double ws) {
     const size_t nb2 = nb * nb;
@@ -1581,7 +1935,32 @@ int hfcpu_solve_dm(const hf_kernel_desc* kd, const double* P0, double n_e,
         const Kernel K = make_kernel(kd);
         validate_electron_count(K.w2d, K.nk(), K.nb, n_e);
         Projector pr{pfn, puser};
-        solve_dm(K, C(P0), n_e, *cfg, pfn ? &pr : nullptr, out);
+        if (cfg->optimizer == 1)
+            solve_rtr(K, C(P0), n_e, *cfg, pfn ? &pr : nullptr, out);
+        else
+            solve_dm(K, C(P0), n_e, *cfg, pfn ? &pr : nullptr, out);
+    });
+}
+
+int hfcpu_rtr_joint_hvp(const hf_kernel_desc* kd, const double* X, const double* dp,
+                        const double* Q, const double* p, const double* Ft, double* HX,
+                        double* Hp) {
+    return guarded([&] {
+        const Kernel K = make_kernel(kd);
+        const size_t n_tot = K.n_dense();
+        std::vector<c64> dPb(n_tot), rb(n_tot), Sb(n_tot), Fb(n_tot);
+        std::vector<double> hb(K.nb, 0.0);
+        joint_hvp(K, C(X), dp, C(Q), p, C(Ft), std::max(K.T, 1.0e-12), C(HX), Hp, dPb.data(),
+                  rb.data(), Sb.data(), Fb.data(), hb.data(), 0.0);
+    });
+}
+
+int hfcpu_deflation_bias(const double* P, const double* targets, size_t n_defl,
+                         const double* w2d, size_t nk, size_t nb, double sigma, double length,
+                         double* S_pen, double* Phi, double* diag) {
+    return guarded([&] {
+        *Phi = deflation_bias(C(P), n_defl ? C(targets) : nullptr, n_defl, w2d, nk, nb, sigma,
+                              length, C(S_pen), *diag);
     });
 }
 

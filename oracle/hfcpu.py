@@ -54,6 +54,9 @@ class DmConfig(C.Structure):
         ("max_step", C.c_double), ("cg_restart", _sz), ("bt_shrink", C.c_double),
         ("bt_max", _sz), ("mu_maxiter", C.c_int),
         ("block_sizes", C.POINTER(_sz)), ("n_blocks", _sz),
+        ("optimizer", C.c_int), ("tr_delta0", C.c_double), ("tr_cg_max", _sz),
+        ("deflation_targets", _dp), ("n_deflation", _sz), ("deflation_sigma", C.c_double),
+        ("deflation_length", C.c_double),
     ]
 
 
@@ -335,16 +338,22 @@ def solve_dm(kernel_args, P0, n_e, max_iter, tol_E, tol_grad, max_step, bt_shrin
              denom_scale, bt_max, cg_restart, plateau_window, mu_maxiter, optimizer,
              tr_delta0, tr_cg_max, deflation_targets, deflation_sigma, deflation_length,
              block_sizes, project_fn, return_Q, return_density, return_fock):
-    if int(optimizer) != 0:
-        raise ValueError("the oracle restates optimizer='cg' only")
+    if int(optimizer) not in (0, 1):
+        raise ValueError("optimizer code must be 0 (cg) or 1 (newton)")
     d = _Desc(kernel_args)
     P0 = _c128(P0)
     sizes = _size_array(block_sizes)
+    defl = _c128(deflation_targets) if deflation_targets is not None else None
+    n_defl = int(defl.shape[0]) if defl is not None and defl.ndim == 5 else 0
     cfg = DmConfig(max_iter=max_iter, tol_E=tol_E, tol_grad=tol_grad,
                    plateau_window=plateau_window, denom_scale=denom_scale,
                    max_step=max_step, cg_restart=cg_restart, bt_shrink=bt_shrink,
                    bt_max=bt_max, mu_maxiter=mu_maxiter, block_sizes=sizes,
-                   n_blocks=len(sizes or ()))
+                   n_blocks=len(sizes or ()), optimizer=int(optimizer),
+                   tr_delta0=float(tr_delta0), tr_cg_max=int(tr_cg_max),
+                   deflation_targets=_p(defl) if n_defl else None, n_deflation=n_defl,
+                   deflation_sigma=float(deflation_sigma),
+                   deflation_length=float(deflation_length))
     Q = np.empty(d.shape, np.complex128)
     p = np.empty(d.shape[:-1], np.float64)
     dens = np.empty(d.shape, np.complex128)
@@ -360,6 +369,36 @@ def solve_dm(kernel_args, P0, n_e, max_iter, tol_E, tol_grad, max_step, bt_shrin
     return (Q if return_Q else None, p, dens if return_density else None,
             fock if return_fock else None, res.mu, res.energy, n, bool(res.converged),
             hE[:n].copy(), hG[:n].copy())
+
+
+def _rtr_joint_hvp(kernel_args, X, dp, Q, p, Ft):
+    """Joint (Q, p) Hessian-vector product of the trust-region solver
+    (solver_rtr.hpp:55-112), as the reference's _native._rtr_joint_hvp."""
+    d = _Desc(kernel_args)
+    X, Q, Ft = _c128(X), _c128(Q), _c128(Ft)
+    dp, p = _f64(dp), _f64(p)
+    HX = np.empty(d.shape, np.complex128)
+    Hp = np.empty(d.shape[:-1], np.float64)
+    _check(lib().hfcpu_rtr_joint_hvp(C.byref(d.desc), _p(X), _p(dp), _p(Q), _p(p), _p(Ft),
+                                     _p(HX), _p(Hp)))
+    return HX, Hp
+
+
+def _deflation_bias(P, targets, w2d, sigma, length):
+    """(Phi, S_pen, diag_coeff) of the deflation bias (solver_rtr.hpp:127-160)."""
+    P = _c128(P)
+    T = _c128(targets)
+    w = _f64(w2d)
+    nb = P.shape[-1]
+    nk = int(np.prod(P.shape[:-2]))
+    n_defl = int(T.shape[0]) if T.ndim == 5 else 0
+    S = np.empty_like(P)
+    Phi = C.c_double()
+    dg = C.c_double()
+    _check(lib().hfcpu_deflation_bias(_p(P), _p(T) if n_defl else None, _sz(n_defl), _p(w),
+                                      _sz(nk), _sz(nb), C.c_double(sigma), C.c_double(length),
+                                      _p(S), C.byref(Phi), C.byref(dg)))
+    return Phi.value, S, dg.value
 
 
 def _cayley_spectral_setup(d):
