@@ -97,9 +97,11 @@ struct GpuKernel {
     bool packed_for(bool p_hermitian) const { return hcp || (dense_packable && p_hermitian); }
     Geom& geom(bool packed);
 
-    // build_fock_compact on device: Sigma, F, hdiag (nb) from P.
+    // build_fock_compact on device: Sigma, F, hdiag (nb) from P.  use_refP =
+    // false drops the reference density (the linear response to a density
+    // variation, solver_rtr.hpp:32-39).
     void build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, bool p_hermitian,
-                    hipStream_t st);
+                    hipStream_t st, bool use_refP = true);
     // hf_energy_with_hartree_diag on device into E (one double).
     void energy(const cd* P, const cd* Sigma, const double* hdiag_in, double* E, hipStream_t st);
 

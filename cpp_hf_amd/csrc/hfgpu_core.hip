@@ -303,14 +303,14 @@ GpuKernel::Geom& GpuKernel::geom(bool packed) {
 }
 
 void GpuKernel::build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, bool p_hermitian,
-                           hipStream_t st) {
+                           hipStream_t st, bool use_refP) {
     const bool need_partials = hartree || contact;
     const bool packed = exchange && packed_for(p_hermitian);
     Geom& G = geom(packed);
     const ExchangeGeom& g = *G.g;
     ExchangeArgs a;
     a.P = P;
-    if (has_refP) {
+    if (has_refP && use_refP) {
         if (refP_const) {
             a.refP0 = refP0.as<cd>();
             a.wrow = wrow.as<double>();

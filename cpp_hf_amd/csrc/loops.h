@@ -35,6 +35,10 @@ struct LoopScratch {
 };
 constexpr size_t kLoopPartialDoubles = 1024 * 2 * kMaxBt;
 
+// out[s] = sum over nparts workgroups of partial[b * ns + s] (fixed order).
+void launch_finalize_sums(const double* partial, int nparts, int ns, double* out,
+                          hipStream_t st);
+
 void launch_orbital_grad(int nk, int nb, const cd* Q, cd* F, const double* p, const double* w,
                          const MuState* mus, double T_r, bool grad, double* eps, cd* G,
                          double* g_p, LoopScratch& ls, hipStream_t st);

@@ -388,7 +388,8 @@ PYBIND11_MODULE(_native, m) {
                               hC[py::slice(0, n, 1)].attr("copy")());
     });
 
-    // --- Direct minimisation (cpp_hf_native.cpp:488-582), optimizer "cg" ---
+    // --- Direct minimisation (cpp_hf_native.cpp:488-582): optimizer 1 is the
+    // trust-region Newton solver (with deflation), any other code the CG loop ---
     m.def("solve_dm", [](py::object kernel_args, CArr P0, double n_e, size_t max_iter,
                          double tol_E, double tol_grad, double max_step, double bt_shrink,
                          double denom_scale, size_t bt_max, size_t cg_restart,
@@ -397,16 +398,6 @@ PYBIND11_MODULE(_native, m) {
                          double deflation_length, std::vector<size_t> block_sizes,
                          py::object project_fn, bool return_Q, bool return_density,
                          bool return_fock) {
-        (void)tr_delta0;
-        (void)tr_cg_max;
-        (void)deflation_sigma;
-        (void)deflation_length;
-        if (optimizer != 0)
-            throw py::value_error(
-                "optimizer='newton' (trust-region Newton) is not available in cpp_hf_amd; "
-                "use optimizer='cg'");
-        if (deflation_targets.ndim() == 5 && deflation_targets.shape(0) > 0 && deflation_sigma > 0)
-            throw py::value_error("deflation is only supported with optimizer='newton'");
         KernelArgs ka(kernel_args);
         if (P0.ndim() != 4 || (size_t)P0.shape(0) != ka.d.nk1 ||
             (size_t)P0.shape(1) != ka.d.nk2 || (size_t)P0.shape(2) != ka.d.nb ||
@@ -425,6 +416,21 @@ PYBIND11_MODULE(_native, m) {
         cfg.mu_maxiter = mu_maxiter;
         cfg.block_sizes = block_sizes.empty() ? nullptr : block_sizes.data();
         cfg.n_blocks = block_sizes.size();
+        cfg.optimizer = optimizer;
+        cfg.tr_delta0 = tr_delta0;
+        cfg.tr_cg_max = tr_cg_max;
+        // (n_defl, nk1, nk2, nb, nb) or empty (cpp_hf_native.cpp:533-547)
+        if (deflation_targets.ndim() == 5 && deflation_targets.shape(0) > 0) {
+            cfg.n_deflation = (size_t)deflation_targets.shape(0);
+            cfg.deflation_targets = cdp(deflation_targets);
+            const size_t n_tot = ka.d.nk1 * ka.d.nk2 * ka.d.nb * ka.d.nb;
+            if (deflation_sigma > 0.0 &&
+                (size_t)deflation_targets.size() < cfg.n_deflation * n_tot)
+                throw py::value_error(
+                    "deflation_targets buffer smaller than n_deflation * nk * nb * nb.");
+        }
+        cfg.deflation_sigma = deflation_sigma;
+        cfg.deflation_length = deflation_length;
         Handle H(ka.d);
         std::vector<py::ssize_t> shape{(py::ssize_t)ka.d.nk1, (py::ssize_t)ka.d.nk2,
                                        (py::ssize_t)ka.d.nb, (py::ssize_t)ka.d.nb};

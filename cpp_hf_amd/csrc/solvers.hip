@@ -17,6 +17,7 @@
 #include "../../include/hfgpu.h"
 #include "hfgpu_core.h"
 #include "loops.h"
+#include "rtr.h"
 
 namespace hfgpu {
 
@@ -568,6 +569,263 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
     }
 }
 
+// ---------------------------------------------------------------------------
+// Trust-region Newton (solver_rtr.hpp:162-459)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// One point (Q, p) of the trust-region iteration and what eval_state derives
+// from it: Ft = Q^H F Q, eps = Re diag Ft, G, and mu.  The solver keeps the
+// accepted point and a trial point and swaps them on acceptance, so a
+// rejected step costs no copies (solver_rtr.hpp:383-428).
+struct RtrPoint {
+    DevBuf Q, p, Ft, G, eps, mus;
+    RtrPoint(size_t bytes, size_t rb)
+        : Q(bytes), p(rb), Ft(bytes), G(bytes), eps(rb), mus(sizeof(MuState)) {}
+};
+
+// Slots of the solver's own scalar array.
+enum RtrSlot : int { RS_UPD = 0, RS_HVP = 8, RS_DEFL = 16, RS_D2 = 18, RS_FIXED = 18 };
+
+}  // namespace
+
+void solve_rtr_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config& cfg,
+                   Projector& proj, hf_dm_result* out) {
+    const int nk = static_cast<int>(K.nk), nb = K.nb;
+    const size_t nd = K.nd, bytes = nd * sizeof(cd), nkb = K.nk * K.nb;
+    const size_t rb = nkb * sizeof(double);
+    hipStream_t st = K.stream;
+    SolveWs W(st);
+    double* sc = W.sc.as<double>();
+
+    const double wsum = std::max(K.weight_sum, TINY_REAL);
+    std::vector<double> wn(K.w_host.size());
+    for (size_t i = 0; i < wn.size(); ++i) wn[i] = K.w_host[i] / wsum;
+    double w_total = 0.0;  // project_occ's denominator / nb (solver_rtr.hpp:42-50)
+    for (double x : K.w_host) w_total += x;
+    const double occ_den = w_total * static_cast<double>(nb);
+    const double n_target = n_e / wsum;
+    const double T_r = std::max(K.T, 1.0e-12);
+    const double tol_g = cfg.tol_grad > 0.0 ? cfg.tol_grad : 1.0e-6;
+
+    const size_t n_defl = cfg.n_deflation;
+    const bool defl_on = n_defl > 0 && cfg.deflation_sigma > 0.0 && cfg.deflation_targets;
+    const double defl_len = std::max(cfg.deflation_length, 1.0e-12);
+
+    DevBuf w_norm(K.nk * sizeof(double));
+    h2d(w_norm.get(), wn.data(), K.nk * sizeof(double), st);
+    const double* wd = w_norm.as<double>();
+    const double* w2d = K.w2d.as<double>();
+    RtrPoint A(bytes, rb), B(bytes, rb);
+    RtrPoint* cur = &A;
+    RtrPoint* trial = &B;
+    DevBuf P(bytes), Sigma(bytes), hdiag(nb * sizeof(double)), gp(rb);
+    DevBuf vX(bytes), HvX(bytes), rX(bytes), dX(bytes), HdX(bytes);
+    DevBuf vp(rb), Hvp(rb), rp(rb), dpv(rb), Hdp(rb);
+    DevBuf dPb(bytes), Fb(bytes), Sb(bytes), hb(nb * sizeof(double));
+    DevBuf Vd(bytes), lam(rb), eps_t(rb);
+    DevBuf rsc((RS_FIXED + (defl_on ? n_defl : 0)) * sizeof(double));
+    DevBuf tgt(defl_on ? n_defl * bytes : 0);
+    if (defl_on) h2d(tgt.get(), cfg.deflation_targets, n_defl * bytes, st);
+    double* rs = rsc.as<double>();
+    fill_zero(rs, rsc.size(), st);
+
+    std::vector<double> hrs(rsc.size() / sizeof(double));
+    double hsc[SC_COUNT];
+    auto read_back = [&]() {
+        d2h(hsc, sc, sizeof(hsc), st);
+        d2h(hrs.data(), rs, rsc.size(), st);
+        HFGPU_CHECK(hipStreamSynchronize(st));
+    };
+
+    // density, Fock (+ deflation bias), Ft, eps and G at a point
+    // (solver_rtr.hpp:275-293); energy, |G|_w^2 and sum eps^2 land in sc
+    auto eval_state = [&](RtrPoint& s) {
+        launch_density(nk, nb, s.Q.as<cd>(), s.p.as<double>(), P.as<cd>(), false, st);
+        if (proj) proj.apply(P.as<cd>(), st);
+        launch_hermitize(nk, nb, P.as<cd>(), st);
+        K.build_fock(P.as<cd>(), Sigma.as<cd>(), s.Ft.as<cd>(), hdiag.as<double>(), true, st);
+        K.energy(P.as<cd>(), Sigma.as<cd>(), hdiag.as<double>(), sc + SC_E, st);
+        if (defl_on)
+            launch_deflation(nk, nb, static_cast<int>(n_defl), P.as<cd>(), tgt.as<cd>(), w2d,
+                             cfg.deflation_sigma, defl_len, rs + RS_D2, s.Ft.as<cd>(),
+                             rs + RS_DEFL, W.ls, st);
+        launch_orbital_grad(nk, nb, s.Q.as<cd>(), s.Ft.as<cd>(), s.p.as<double>(), wd,
+                            s.mus.as<MuState>(), T_r, true, s.eps.as<double>(), s.G.as<cd>(),
+                            gp.as<double>(), W.ls, st);
+    };
+    struct EvalOut {
+        double E, gn, eps_sq, pen;
+    };
+    auto eval_result = [&]() {
+        read_back();
+        EvalOut r;
+        r.E = hsc[SC_E] + (defl_on ? hrs[RS_DEFL] : 0.0);
+        r.gn = std::sqrt(std::max(0.0, hsc[SC_GN2]));
+        r.eps_sq = hsc[SC_EPS_SQ];
+        r.pen = defl_on ? hrs[RS_DEFL + 1] : 0.0;
+        return r;
+    };
+
+    // ---- init (solver_rtr.hpp:262-272)
+    h2d(P.get(), P0, bytes, st);
+    launch_hermitize(nk, nb, P.as<cd>(), st);
+    K.build_fock(P.as<cd>(), Sigma.as<cd>(), Fb.as<cd>(), hdiag.as<double>(), true, st);
+    launch_eigh(nk, nb, 0, Fb.as<cd>(), cur->eps.as<double>(), cur->Q.as<cd>(), st);
+    fill_zero(cur->mus.get(), sizeof(MuState), st);
+    launch_mu_newton(cur->eps.as<double>(), wd, nk, nb, n_target, 0.0, false, T_r,
+                     cfg.mu_maxiter, cur->mus.as<MuState>(), W.mu_partial.as<double>(), st);
+    launch_occupations(cur->eps.as<double>(), nkb, cur->mus.as<MuState>(), T_r, 1,
+                       cur->p.as<double>(), st);
+    eval_state(*cur);
+    EvalOut now = eval_result();
+
+    std::vector<double> hist_E, hist_G;
+    double Delta = cfg.tr_delta0;
+    size_t k = 0;
+    bool converged = false;
+    while (k < cfg.max_iter) {
+        hist_E.push_back(now.E);
+        hist_G.push_back(now.gn);
+        if (now.gn < tol_g) {
+            converged = true;
+            ++k;
+            break;
+        }
+        const double lam_pc = std::max(
+            T_r, cfg.denom_scale *
+                     std::sqrt(now.eps_sq / static_cast<double>(std::max<size_t>(nkb, 1)) +
+                               TINY_REAL));
+        const cd* Qc = cur->Q.as<cd>();
+        const double* pc = cur->p.as<double>();
+        const cd* Ftc = cur->Ft.as<cd>();
+        const double* epc = cur->eps.as<double>();
+        auto update = [&](int mode, double a, double s_H) {
+            launch_rtr_update(nk, nb, mode, a, s_H, lam_pc, T_r, epc, pc, cur->G.as<cd>(), w2d,
+                              dX.as<cd>(), dpv.as<double>(), HdX.as<cd>(), Hdp.as<double>(),
+                              vX.as<cd>(), vp.as<double>(), HvX.as<cd>(), Hvp.as<double>(),
+                              rX.as<cd>(), rp.as<double>(), W.ls, rs + RS_UPD, st);
+        };
+        auto hvp = [&](bool init, double beta, double s_z) {
+            launch_rtr_direction(nk, nb, init, beta, s_z, lam_pc, T_r, Qc, pc, epc, rX.as<cd>(),
+                                 rp.as<double>(), dX.as<cd>(), dpv.as<double>(), dPb.as<cd>(),
+                                 st);
+            K.build_fock(dPb.as<cd>(), Sb.as<cd>(), Fb.as<cd>(), hb.as<double>(), true, st,
+                         false);
+            launch_rtr_hvp(nk, nb, T_r, now.pen, Qc, pc, Ftc, dX.as<cd>(), dpv.as<double>(),
+                           Fb.as<cd>(), K.h.as<cd>(), dPb.as<cd>(), vX.as<cd>(),
+                           vp.as<double>(), w2d, HdX.as<cd>(), Hdp.as<double>(), W.ls,
+                           rs + RS_HVP, st);
+        };
+
+        // ---- Steihaug truncated CG (solver_rtr.hpp:283-346)
+        update(0, 0.0, 0.0);  // v = Hv = 0, r = -G
+        double rz = 0.0, gv = 0.0, vhv = 0.0, vv = 0.0;
+        double beta = 0.0, s_z = 0.0;
+        for (size_t it = 0; it < cfg.tr_cg_max; ++it) {
+            hvp(it == 0, beta, s_z);
+            read_back();
+            const double* H = hrs.data() + RS_HVP;
+            if (it == 0) rz = hrs[RS_UPD + RU_RZX];  // <r, z> with r = (-G, 0)
+            const double s_H = H[RH_HP] / occ_den;
+            const double dHd = H[RH_DHDX] + (H[RH_DHP] - s_H * H[RH_DP]);
+            const double dd = H[RH_DD], vd = H[RH_VD], vv0 = H[RH_VV];
+            bool boundary = !(dHd > 1e-14 * dd);
+            double alpha = 0.0;
+            if (!boundary) {
+                alpha = rz / dHd;
+                const double s2 = vv0 + 2.0 * alpha * vd + alpha * alpha * dd;
+                boundary = std::sqrt(std::max(0.0, s2)) >= Delta;
+            }
+            if (boundary) {
+                const double a = dd, b = 2.0 * vd, c2 = std::max(0.0, vv0);
+                const double cc = c2 - Delta * Delta;
+                const double t =
+                    (-b + std::sqrt(std::max(b * b - 4.0 * a * cc, 0.0))) / (2.0 * a + 1e-30);
+                update(2, t, s_H);
+                read_back();
+                gv = hrs[RS_UPD + RU_GV];
+                vhv = hrs[RS_UPD + RU_VHV];
+                vv = hrs[RS_UPD + RU_VV];
+                break;
+            }
+            update(1, alpha, s_H);
+            read_back();
+            const double* U = hrs.data() + RS_UPD;
+            gv = U[RU_GV];
+            vhv = U[RU_VHV];
+            vv = U[RU_VV];
+            if (std::sqrt(std::max(0.0, U[RU_RN2])) < 0.05 * now.gn) break;
+            s_z = U[RU_ZP] / occ_den;
+            const double rz_new = U[RU_RZX] + (U[RU_RZP] - s_z * U[RU_RP]);
+            beta = rz_new / rz;
+            rz = rz_new;
+        }
+
+        // predicted reduction in energy units (solver_rtr.hpp:348-361)
+        const double pred = -(gv + 0.5 * vhv);
+        if (Delta < 1e-10) break;
+
+        // ---- trial point: Q Cayley(tau = -1, v), frozen-Ft energies, mu
+        // (solver_rtr.hpp:363-385), then eval_state at it
+        launch_eigh(nk, nb, 1, vX.as<cd>(), lam.as<double>(), Vd.as<cd>(), st);
+        launch_rtr_retract(nk, nb, -1.0, Vd.as<cd>(), lam.as<double>(), Qc, Ftc,
+                           trial->Q.as<cd>(), eps_t.as<double>(), st);
+        d2d(trial->mus.get(), cur->mus.get(), sizeof(MuState), st);
+        launch_mu_newton(eps_t.as<double>(), wd, nk, nb, n_target, 0.0, true, T_r,
+                         cfg.mu_maxiter, trial->mus.as<MuState>(), W.mu_partial.as<double>(), st);
+        launch_occupations(eps_t.as<double>(), nkb, trial->mus.as<MuState>(), T_r, 1,
+                           trial->p.as<double>(), st);
+        eval_state(*trial);
+        const EvalOut tr = eval_result();
+
+        // ---- acceptance and trust radius (solver_rtr.hpp:398-420)
+        const double noise = 1e-12 * std::max(std::fabs(now.E), 1.0);
+        bool accept;
+        if (pred <= noise) {
+            accept = tr.gn <= now.gn && tr.E <= now.E + noise;
+            if (!accept) Delta *= 0.5;
+        } else {
+            const double ratio = (now.E - tr.E) / pred;
+            if (ratio < 0.25)
+                Delta *= 0.25;
+            else if (ratio > 0.75 && std::sqrt(std::max(0.0, vv)) > 0.9 * Delta)
+                Delta = std::min(2.0 * Delta, 5.0);
+            accept = ratio > 0.1;
+        }
+        if (accept) {
+            std::swap(cur, trial);
+            now = tr;
+        }
+        ++k;
+    }
+
+    // ---- finalise (solver_rtr.hpp:435-441)
+    launch_density(nk, nb, cur->Q.as<cd>(), cur->p.as<double>(), P.as<cd>(), false, st);
+    if (proj) proj.apply(P.as<cd>(), st);
+    launch_hermitize(nk, nb, P.as<cd>(), st);
+    K.build_fock(P.as<cd>(), Sigma.as<cd>(), Fb.as<cd>(), hdiag.as<double>(), true, st);
+    K.energy(P.as<cd>(), Sigma.as<cd>(), hdiag.as<double>(), sc + SC_E, st);
+
+    MuState hmu;
+    d2h(hsc, sc, sizeof(hsc), st);
+    d2h(&hmu, cur->mus.get(), sizeof(MuState), st);
+    if (out->Q) d2h(out->Q, cur->Q.get(), bytes, st);
+    if (out->p) d2h(out->p, cur->p.get(), rb, st);
+    if (out->density) d2h(out->density, P.get(), bytes, st);
+    if (out->fock) d2h(out->fock, Fb.get(), bytes, st);
+    HFGPU_CHECK(hipStreamSynchronize(st));
+    out->mu = hmu.mu;
+    out->energy = hsc[SC_E];
+    out->n_iter = k;
+    out->converged = converged ? 1 : 0;
+    for (size_t i = 0; i < k; ++i) {
+        if (out->hist_E) out->hist_E[i] = hist_E[i];
+        if (out->hist_grad) out->hist_grad[i] = hist_G[i];
+    }
+}
+
 }  // namespace hfgpu
 
 using namespace hfgpu;
@@ -603,7 +861,10 @@ int hfgpu_solve_dm(hfgpu_kernel* k, const double* P0, double n_e, const hf_dm_co
         pr.nk2 = K.nk2;
         pr.nb = K.nb;
         pr.nd = K.nd;
-        solve_dm_gpu(K, P0, n_e, *cfg, pr, out);
+        if (cfg->optimizer == 1)
+            solve_rtr_gpu(K, P0, n_e, *cfg, pr, out);
+        else
+            solve_dm_gpu(K, P0, n_e, *cfg, pr, out);
     });
 }
 

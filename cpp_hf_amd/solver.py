@@ -2,7 +2,8 @@
 
 Packs the kernel into the reference's ``kernel_args`` dict, validates the
 inputs and unpacks the tuple returned by ``_native.solve_dm``, which runs the
-preconditioned Riemannian CG loop on the GPU.
+preconditioned Riemannian CG loop (or, with ``optimizer="newton"``, the
+trust-region Newton loop) on the GPU.
 """
 
 from __future__ import annotations
@@ -20,9 +21,10 @@ from .utils import validate_electron_count
 class SolverConfig:
     """Direct-minimisation settings; fields and defaults of solver.py:19-64.
 
-    ``optimizer="newton"`` and deflation are not part of this GPU build and
-    raise ``ValueError``; ``block_sizes`` is validated and the dense path is
-    used (same fixed point, same iteration up to rounding).
+    ``optimizer="newton"`` selects the trust-region Newton solver
+    (solver_rtr.hpp), which alone honours ``deflation_*``; ``block_sizes``
+    applies to the CG loop (per-block eigh when the problem is
+    block-diagonal), as in the reference.
     """
 
     max_iter: int = 200

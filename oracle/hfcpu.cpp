@@ -1488,6 +1488,12 @@ void solve_rtr(const Kernel& K, const c64* P0, double n_e, const hf_dm_config& c
                 Delta = std::min(2.0 * Delta, 5.0);
             accept = ratio > 0.1;
         }
+        if (std::getenv("HFCPU_RTR_DEBUG"))
+            std::fprintf(stderr,
+                         "[rtr] k=%zu g=%.3e |v|=%.3e Delta=%.3e pred=%.3e dE=%+.3e gtr=%.3e "
+                         "accept=%d\n",
+                         k, gn, jnorm(vX.data(), vp.data()), Delta, pred, E_t - E, g_t,
+                         static_cast<int>(accept));
         if (accept) {
             Q = Qt;
             p = p_t;
