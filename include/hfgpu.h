@@ -159,9 +159,11 @@ int hfgpu_density_from_fock(const double* F, const double* weights, size_t nk, s
 /* ------------------------------------------------------------------------
  * Solvers.
  * Replaces: _native.solve_scf (cpp/cpp_hf_native.cpp:418-485) -> solve_scf
- * (cpp/include/cpp_hf/solver_scf.hpp:156-444), and _native.solve_dm with
- * optimizer "cg" (cpp/cpp_hf_native.cpp:488-582) -> solve_dm
- * (cpp/include/cpp_hf/solver_dm.hpp:392-653).
+ * (cpp/include/cpp_hf/solver_scf.hpp:156-444), and _native.solve_dm
+ * (cpp/cpp_hf_native.cpp:488-582) -> solve_dm
+ * (cpp/include/cpp_hf/solver_dm.hpp:392-653) for cfg->optimizer != 1, solve_rtr
+ * (cpp/include/cpp_hf/solver_rtr.hpp:162-459, trust-region Newton with the
+ * deflation bias) for cfg->optimizer == 1.
  * ---------------------------------------------------------------------- */
 int hfgpu_solve_scf(hfgpu_kernel* k, const double* P0, double n_e, const hf_scf_config* cfg,
                     hf_project_fn project, void* user, hf_scf_result* out);
