@@ -21,6 +21,7 @@ from .sweep import solve_many
 from .continuation import ContinuationResult, solve_continuation
 from .symmetry import make_project_fn
 from .deflation import DeflatedResult, solve_deflated
+from .adjoint import build_fock_vjp, energy_gradients, free_energy_derivative
 
 __all__ = [
     "HartreeFockKernel",
@@ -42,4 +43,7 @@ __all__ = [
     "make_project_fn",
     "DeflatedResult",
     "solve_deflated",
+    "build_fock_vjp",
+    "energy_gradients",
+    "free_energy_derivative",
 ]

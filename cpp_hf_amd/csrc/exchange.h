@@ -142,4 +142,23 @@ void launch_energy_fullH(int nk, int nb, const cd* h, const cd* P, const cd* ref
 void launch_entropy(const double* p, const double* w, int nk, int nb, double* partial,
                     double* S_out, hipStream_t st);
 
+
+// ---- adjoint building blocks (natural layout X[k1][k2][ch]) ----------------
+// Unnormalised 2-D transform of nch <= g.nch channels of a full-channel
+// geometry (forward exp(-2 pi i / n) or inverse); in == out allowed.
+void launch_fft2_plain(const ExchangeGeom& g, const cd* in, cd* out, int nch, bool inverse,
+                       hipStream_t st);
+// out = scale A conj(B) per element, or summed over channels per q.
+void launch_cross_spectrum(size_t nq, int nch, const cd* A, const cd* B, double scale,
+                           bool sum_channels, cd* out, hipStream_t st);
+// X *= scale op(V), V per (q, ch) or per q (v_scalar), op = conj with conj_v.
+void launch_mul_spectrum(size_t nq, int nch, cd* X, const cd* V, bool v_scalar, bool conj_v,
+                         double scale, hipStream_t st);
+// X *= conj(ifftshift phase of q) (the phase folded into VR, problem.py:95-104).
+void launch_unphase(int nk1, int nk2, int nch, cd* X, hipStream_t st);
+// out[e] = sum_k (w[k]) X[k][e], e < m; partial needs kKsumParts * m complex.
+constexpr int kKsumParts = 256;
+void launch_ksum(int nk, int m, const cd* X, const double* w, cd* partial, cd* out,
+                 hipStream_t st);
+
 }  // namespace hfgpu

@@ -1092,6 +1092,162 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
     if (threadIdx.x == 0) *out = s;
 }
 
+
+// ---------------------------------------------------------------------------
+// Plain batched 2-D transforms and spectral products (the adjoint of the
+// exchange).  Arrays in natural layout X[k1][k2][ch], nch channels
+// innermost, processed in tiles of C = 2^clog channels per workgroup;
+// transforms unnormalised (forward exp(-2 pi i / n)).
+// ---------------------------------------------------------------------------
+
+// Along k2 (rows: one k1 line and one channel tile per workgroup).
+template <bool GENERIC, int S>
+__global__ void __launch_bounds__(kFftThreads)
+k_plain_rows(FftAxis ax2, int nk1, int nk2, int nch, int clog, int nblk, int inv,
+             const cd* __restrict__ in, cd* __restrict__ out) {
+    extern __shared__ cd smem[];
+    const int C = 1 << clog;
+    const int tot = nk2 << clog;
+    cd* s = smem;
+    cd* s_tw = smem + tot;
+    int k1, blk;
+    tile_of(static_cast<int>(blockIdx.x), nk1, nblk, k1, blk);
+    const int ch = blk * C + (threadIdx.x & (C - 1));
+    const bool valid = ch < nch;
+    const size_t base = static_cast<size_t>(k1) * nk2 * nch + ch;
+    load_twiddles(s_tw, ax2.tw, nk2);
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot)
+            s[e] = valid ? in[base + static_cast<size_t>(e >> clog) * nch] : cd{0.0, 0.0};
+    }
+    __syncthreads();
+    if (inv)
+        fft_tile<true, GENERIC>(s, s_tw, ax2, clog);
+    else
+        fft_tile<false, GENERIC>(s, s_tw, ax2, clog);
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot && valid) out[base + static_cast<size_t>(e >> clog) * nch] = s[e];
+    }
+}
+
+// Along k1 (columns), in place.
+template <bool GENERIC, int S>
+__global__ void __launch_bounds__(kFftThreads)
+k_plain_cols(FftAxis ax1, int nk1, int nk2, int nch, int clog, int nblk, int inv,
+             cd* __restrict__ X) {
+    extern __shared__ cd smem[];
+    const int C = 1 << clog;
+    const int tot = nk1 << clog;
+    cd* s = smem;
+    cd* s_tw = smem + tot;
+    int k2, blk;
+    tile_of(static_cast<int>(blockIdx.x), nk2, nblk, k2, blk);
+    const int ch = blk * C + (threadIdx.x & (C - 1));
+    const bool valid = ch < nch;
+    const size_t base = static_cast<size_t>(k2) * nch + ch;
+    const size_t stride = static_cast<size_t>(nk2) * nch;
+    load_twiddles(s_tw, ax1.tw, nk1);
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot) s[e] = valid ? X[base + (e >> clog) * stride] : cd{0.0, 0.0};
+    }
+    __syncthreads();
+    if (inv)
+        fft_tile<true, GENERIC>(s, s_tw, ax1, clog);
+    else
+        fft_tile<false, GENERIC>(s, s_tw, ax1, clog);
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot && valid) X[base + (e >> clog) * stride] = s[e];
+    }
+}
+
+// out[q][ch] = scale A[q][ch] conj(B[q][ch]), or with sum_ch the sum over
+// channels into out[q] (channel order fixed: deterministic)
+__global__ void __launch_bounds__(256)
+k_cross_spectrum(size_t nq, int nch, const cd* __restrict__ A, const cd* __restrict__ B,
+                 double scale, int sum_ch, cd* __restrict__ out) {
+    const size_t n = sum_ch ? nq : nq * nch;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        if (sum_ch) {
+            cd s{0.0, 0.0};
+            for (int c = 0; c < nch; ++c) {
+                const size_t j = i * nch + c;
+                s = cadd(s, cmulc(A[j], B[j]));
+            }
+            out[i] = cscale(s, scale);
+        } else {
+            out[i] = cscale(cmulc(A[i], B[i]), scale);
+        }
+    }
+}
+
+// X[q][ch] *= scale op(V[q][ch]) (or V[q] when v_scalar); op = conj with conj_v
+__global__ void __launch_bounds__(256)
+k_mul_spectrum(size_t nq, int nch, cd* __restrict__ X, const cd* __restrict__ V, int v_scalar,
+               int conj_v, double scale) {
+    const size_t n = nq * nch;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        cd v = v_scalar ? V[i / nch] : V[i];
+        if (conj_v) v.y = -v.y;
+        X[i] = cscale(cmul(X[i], v), scale);
+    }
+}
+
+// X[q1][q2][ch] *= conj(exp(2 pi i (q1 s1 / n1 + q2 s2 / n2))), s = n / 2: the
+// adjoint of the ifftshift phase folded into VR
+__global__ void __launch_bounds__(256)
+k_unphase(int nk1, int nk2, int nch, cd* __restrict__ X) {
+    const size_t n = static_cast<size_t>(nk1) * nk2 * nch;
+    const int s1 = nk1 / 2, s2 = nk2 / 2;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t q = i / nch;
+        const int q1 = static_cast<int>(q / nk2), q2 = static_cast<int>(q % nk2);
+        double s_a, c_a, s_b, c_b;
+        sincospi(2.0 * static_cast<double>((static_cast<long>(q1) * s1) % nk1) / nk1, &s_a, &c_a);
+        sincospi(2.0 * static_cast<double>((static_cast<long>(q2) * s2) % nk2) / nk2, &s_b, &c_b);
+        const cd ph = cmul(cd{c_a, s_a}, cd{c_b, s_b});
+        X[i] = cmulc(X[i], ph);
+    }
+}
+
+// Deterministic sum over k of X[k][e] (m entries per k), weighted by w[k]
+// when w is given: per-workgroup partials over k chunks, then one
+// workgroup adds them in order.
+__global__ void __launch_bounds__(256)
+k_ksum_partial(int nk, int m, const cd* __restrict__ X, const double* __restrict__ w,
+               cd* __restrict__ partial) {
+    const int chunk = (nk + gridDim.x - 1) / gridDim.x;
+    const int k0 = blockIdx.x * chunk, k1 = min(nk, k0 + chunk);
+    for (int e = threadIdx.x; e < m; e += blockDim.x) {
+        cd s{0.0, 0.0};
+        for (int k = k0; k < k1; ++k) {
+            cd x = X[static_cast<size_t>(k) * m + e];
+            if (w) x = cscale(x, w[k]);
+            s = cadd(s, x);
+        }
+        partial[static_cast<size_t>(blockIdx.x) * m + e] = s;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_ksum_final(int nparts, int m, const cd* __restrict__ partial, cd* __restrict__ out) {
+    for (int e = threadIdx.x; e < m; e += blockDim.x) {
+        cd s{0.0, 0.0};
+        for (int b = 0; b < nparts; ++b) s = cadd(s, partial[static_cast<size_t>(b) * m + e]);
+        out[e] = s;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1327,6 +1483,55 @@ void launch_entropy(const double* p, const double* w, int nk, int nb, double* pa
     HFGPU_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partial,
                        static_cast<int>(blocks), S_out);
+    HFGPU_LAUNCH_CHECK();
+}
+
+
+// ---------------------------------------------------------------------------
+// adjoint building blocks
+// ---------------------------------------------------------------------------
+
+void launch_fft2_plain(const ExchangeGeom& g, const cd* in, cd* out, int nch, bool inverse,
+                       hipStream_t st) {
+    if (g.packed) throw std::logic_error("plain transforms need a full-channel geometry");
+    if (nch > g.nch) throw std::logic_error("more channels than the geometry holds");
+    const int nblk = (nch + g.tile() - 1) / g.tile();
+    HFGPU_EXCH_LAUNCH(k_plain_rows, g.generic, g.slots_row, dim3(nblk * g.nk1), g.lds_rows, st,
+                      g.ax2.axis(), g.nk1, g.nk2, nch, g.clog, nblk, inverse ? 1 : 0, in, out);
+    HFGPU_EXCH_LAUNCH(k_plain_cols, g.generic, g.slots_col, dim3(nblk * g.nk2), g.lds_cols, st,
+                      g.ax1.axis(), g.nk1, g.nk2, nch, g.clog, nblk, inverse ? 1 : 0, out);
+}
+
+static unsigned ew_blocks(size_t n) {
+    return std::max(1u, std::min<unsigned>(cdiv(n, 256), 8192u));
+}
+
+void launch_cross_spectrum(size_t nq, int nch, const cd* A, const cd* B, double scale,
+                           bool sum_channels, cd* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_cross_spectrum, dim3(ew_blocks(sum_channels ? nq : nq * nch)),
+                       dim3(256), 0, st, nq, nch, A, B, scale, sum_channels ? 1 : 0, out);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_mul_spectrum(size_t nq, int nch, cd* X, const cd* V, bool v_scalar, bool conj_v,
+                         double scale, hipStream_t st) {
+    hipLaunchKernelGGL(k_mul_spectrum, dim3(ew_blocks(nq * nch)), dim3(256), 0, st, nq, nch, X,
+                       V, v_scalar ? 1 : 0, conj_v ? 1 : 0, scale);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_unphase(int nk1, int nk2, int nch, cd* X, hipStream_t st) {
+    hipLaunchKernelGGL(k_unphase, dim3(ew_blocks(static_cast<size_t>(nk1) * nk2 * nch)),
+                       dim3(256), 0, st, nk1, nk2, nch, X);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_ksum(int nk, int m, const cd* X, const double* w, cd* partial, cd* out,
+                 hipStream_t st) {
+    const int nparts = std::max(1, std::min(kKsumParts, nk));
+    hipLaunchKernelGGL(k_ksum_partial, dim3(nparts), dim3(256), 0, st, nk, m, X, w, partial);
+    HFGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_ksum_final, dim3(1), dim3(256), 0, st, nparts, m, partial, out);
     HFGPU_LAUNCH_CHECK();
 }
 

@@ -91,6 +91,10 @@ struct GpuKernel {
     // Fock workspaces
     DevBuf T_ws, partial, SC, hdiag, e_partial, e_out;
 
+    // Full-channel geometries of the adjoint's plain transforms (nb^2
+    // channels, and one channel for a scalar interaction), made on first use.
+    std::unique_ptr<ExchangeGeom> adj_geo, adj_geo_v;
+
     explicit GpuKernel(const hf_kernel_desc& d);
     ~GpuKernel();
 
@@ -117,6 +121,15 @@ struct GpuKernel {
     };
     std::vector<GraphEntry> graphs;
 };
+
+// Vector-Jacobian product of build_fock (fock.hpp:34-321) at density P for
+// cotangents on its outputs Sigma, Hh and F (nullable = zero), in the
+// convention dL = Re sum conj(g) dX: gP (nd), gVq (nk * dv * dv, the
+// gradient with respect to the interaction coulomb_q before the VR
+// transform, complex; its real part for a real Vq), gh (nd), gHH (nb * nb,
+// real).  All device pointers on stream st; gVq, gh, gHH nullable.
+void fock_vjp(GpuKernel& K, const cd* P, const cd* Gs, const cd* Gh, const cd* GF, cd* gP,
+              cd* gVq, cd* gh, double* gHH, hipStream_t st);
 
 // Block map of block sizes summing to nb (exact) or at most nb (indices past
 // the last block belong to no block); throws std::invalid_argument otherwise.

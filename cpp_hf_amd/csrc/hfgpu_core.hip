@@ -279,6 +279,8 @@ GpuKernel::~GpuKernel() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (geo_full.g) geo_full.g->release();
     if (geo_packed.g) geo_packed.g->release();
+    if (adj_geo) adj_geo->release();
+    if (adj_geo_v) adj_geo_v->release();
 }
 
 GpuKernel::Geom& GpuKernel::geom(bool packed) {

@@ -325,6 +325,46 @@ PYBIND11_MODULE(_native, m) {
         return py::make_tuple(S, Hh, F, E);
     });
 
+    // --- adjoint of build_fock (no reference counterpart): (gP, gVq, gh, gHH) ---
+    m.def("build_fock_vjp", [](py::object kernel_args, CArr P, py::object cot_sigma,
+                               py::object cot_hartree, py::object cot_F) {
+        KernelArgs ka(kernel_args);
+        if (P.ndim() != 4 || (size_t)P.shape(0) != ka.d.nk1 || (size_t)P.shape(1) != ka.d.nk2 ||
+            (size_t)P.shape(2) != ka.d.nb || (size_t)P.shape(3) != ka.d.nb)
+            throw py::value_error("P must match h shape");
+        std::vector<py::ssize_t> shape{(py::ssize_t)ka.d.nk1, (py::ssize_t)ka.d.nk2,
+                                       (py::ssize_t)ka.d.nb, (py::ssize_t)ka.d.nb};
+        // cotangents: None -> NULL (zero); arrays kept alive for the call
+        std::vector<CArr> keep;
+        auto cot = [&](py::object o, const char* name) -> const double* {
+            if (o.is_none()) return nullptr;
+            CArr a = py::cast<CArr>(o);
+            if (a.ndim() != 4 || a.shape(0) != shape[0] || a.shape(1) != shape[1] ||
+                a.shape(2) != shape[2] || a.shape(3) != shape[3])
+                throw py::value_error(std::string(name) + " must match h shape");
+            keep.push_back(a);
+            return cdp(keep.back());
+        };
+        keep.reserve(3);
+        const double* Gs = cot(cot_sigma, "cot_sigma");
+        const double* Gh = cot(cot_hartree, "cot_hartree");
+        const double* GF = cot(cot_F, "cot_F");
+        Handle H(ka.d);
+        std::vector<py::ssize_t> vshape{(py::ssize_t)ka.d.nk1, (py::ssize_t)ka.d.nk2,
+                                        (py::ssize_t)ka.d.dv1, (py::ssize_t)ka.d.dv2};
+        auto gP = new_c(shape), gh = new_c(shape), gV = new_c(vshape);
+        auto gHH = new_r({(py::ssize_t)ka.d.nb, (py::ssize_t)ka.d.nb});
+        py::array pa = gP, ha = gh, va = gV, ra = gHH;
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_build_fock_vjp(H.k, cdp(P), Gs, Gh, GF, dp(pa), dp(va), dp(ha), dp(ra));
+        }
+        check(rc);
+        return py::make_tuple(gP, gV, gh, gHH);
+    }, py::arg("kernel_args"), py::arg("P"), py::arg("cot_sigma") = py::none(),
+       py::arg("cot_hartree") = py::none(), py::arg("cot_F") = py::none());
+
     // --- SCF solver (cpp_hf_native.cpp:418-485) ---
     m.def("solve_scf", [](py::object kernel_args, CArr P0, double n_e, size_t max_iter,
                           double density_tol, double comm_tol, double mixing, double level_shift,
@@ -649,5 +689,22 @@ PYBIND11_MODULE(_native, m) {
                 }
                 check(rc);
             },
-            py::arg("P"), py::arg("Sigma"), py::arg("hdiag"), py::arg("E"), py::arg("stream") = 0);
+            py::arg("P"), py::arg("Sigma"), py::arg("hdiag"), py::arg("E"), py::arg("stream") = 0)
+        .def(
+            "build_fock_vjp_device",
+            [](PyKernel& k, std::uintptr_t P, std::uintptr_t Gs, std::uintptr_t Gh,
+               std::uintptr_t GF, std::uintptr_t gP, std::uintptr_t gVq, std::uintptr_t gh,
+               std::uintptr_t gHH, std::uintptr_t stream) {
+                int rc;
+                {
+                    py::gil_scoped_release rel;
+                    rc = hfgpu_build_fock_vjp_device(k.k, (const void*)P, (const void*)Gs,
+                                                     (const void*)Gh, (const void*)GF, (void*)gP,
+                                                     (void*)gVq, (void*)gh, (void*)gHH,
+                                                     (void*)stream);
+                }
+                check(rc);
+            },
+            py::arg("P"), py::arg("cot_sigma"), py::arg("cot_hartree"), py::arg("cot_F"),
+            py::arg("gP"), py::arg("gVq"), py::arg("gh"), py::arg("gHH"), py::arg("stream") = 0);
 }

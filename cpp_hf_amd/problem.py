@@ -79,9 +79,11 @@ class HartreeFockKernel:
             self.exchange_hermitian_channel_packing = bool(
                 Vq.shape[-2:] == (1, 1) and not np.iscomplexobj(Vq)
             )
+            self.coulomb_q_is_real = not np.iscomplexobj(Vq)
             self._VR_shifted = _shifted_interaction(self.weights_b, Vq, h.dtype)
         else:
             self.exchange_hermitian_channel_packing = False
+            self.coulomb_q_is_real = True
             self._VR_shifted = np.zeros((n0, n1, 1, 1), dtype=h.dtype)
 
         # reference density (problem.py:109-121)

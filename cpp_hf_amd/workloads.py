@@ -112,6 +112,49 @@ def layered_kernel(n: int = 96, n_layers: int = 4, *, T: float = 0.05, mass: flo
     return Workload(kern, initial_density(n, nb, seed), float(n_e), f"layered_n{n}_nb{nb}")
 
 
+def layered_tangents(n: int = 96, n_layers: int = 4, *, mass: float = 0.5, u: float = 0.1,
+                     gamma1: float = 0.4, scale: float = 0.8, gate: float = 4.0,
+                     alpha: float = 0.3) -> dict:
+    """Derivatives (dh, dVq, dHH) of layered_kernel's inputs with respect to
+    u, scale, gate (the screening length) and gamma1, in closed form."""
+    nb = 2 * n_layers
+    k = kgrid(n)
+    KX, KY = np.meshgrid(k, k, indexing="ij")
+    layer = np.arange(nb) // 2
+    dl = np.abs(layer[:, None] - layer[None, :]).astype(np.float64)
+    absq = np.sqrt(KX ** 2 + KY ** 2)
+    nz = absq > 0
+    screen = np.exp(-alpha * absq[..., None, None] * dl[None, None])
+    zero_h = np.zeros((n, n, nb, nb), dtype=np.complex128)
+
+    dh_u = zero_h.copy()
+    for l in range(n_layers):
+        c = l - 0.5 * (n_layers - 1)
+        dh_u[..., 2 * l, 2 * l] = c
+        dh_u[..., 2 * l + 1, 2 * l + 1] = c
+    dh_g1 = zero_h.copy()
+    for l in range(n_layers - 1):
+        dh_g1[..., 2 * l + 1, 2 * (l + 1)] = 1.0
+        dh_g1[..., 2 * (l + 1), 2 * l + 1] = 1.0
+
+    # V = 2 pi s tanh(g q)/q exp(-a q dl), V(0) = 2 pi s g; HH = 2 pi s g
+    base_s = np.zeros_like(absq)
+    base_s[nz] = 2.0 * np.pi * np.tanh(gate * absq[nz]) / absq[nz]
+    dV_s = base_s[..., None, None] * screen
+    dV_s[~nz] = 2.0 * np.pi * gate
+    base_g = np.zeros_like(absq)
+    base_g[nz] = 2.0 * np.pi * scale / np.cosh(gate * absq[nz]) ** 2
+    dV_g = base_g[..., None, None] * screen
+    dV_g[~nz] = 2.0 * np.pi * scale
+    zero_v = np.zeros((n, n, nb, nb))
+    return {
+        "u": (dh_u, zero_v, np.zeros((nb, nb))),
+        "scale": (zero_h, dV_s, np.full((nb, nb), 2.0 * np.pi * gate)),
+        "gate": (zero_h, dV_g, np.full((nb, nb), 2.0 * np.pi * scale)),
+        "gamma1": (dh_g1, zero_v, np.zeros((nb, nb))),
+    }
+
+
 def config2(n: int = 96) -> Workload:
     """8 bands (4 layers), T = 0.05, 4.1 electrons per k, Hartree on."""
     return layered_kernel(n, 4, T=0.05, filling=4.1)

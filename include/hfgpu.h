@@ -125,6 +125,29 @@ int hfgpu_build_fock_apply(hfgpu_kernel* k, const double* P, double* Sigma, doub
 int hfgpu_build_fock_device(hfgpu_kernel* k, const void* dP, void* dSigma, void* dF,
                             void* dHdiag, int p_hermitian, void* stream);
 
+/* Vector-Jacobian product of the Fock build.  No counterpart in the
+ * reference, which has no derivatives; for callers that differentiate
+ * through build_fock (autograd, parameter gradients at a converged state).
+ * It is the adjoint of build_fock_compact + build_fock (fock.hpp:34-321)
+ * with the interaction transform VR = phase * fftn(w Vq) of
+ * HartreeFockKernel (src/cpp_hf/problem.py:84-107):
+ *   P: (nk1, nk2, nb, nb) complex density;
+ *   cot_sigma, cot_hartree, cot_F: cotangents of the outputs Sigma, Hh and F
+ *     (same shape; NULL = zero);
+ *   gP, gh: (nk1, nk2, nb, nb) complex gradients with respect to P and h;
+ *   gVq: (nk1, nk2, dv1, dv2) complex gradient with respect to coulomb_q
+ *     (its real part for a real interaction); gHH: (nb, nb) real.
+ * Convention dL = Re sum conj(g) dX.  gVq, gh, gHH may be NULL.  The
+ * transforms are recomputed, not stored. */
+int hfgpu_build_fock_vjp(hfgpu_kernel* k, const double* P, const double* cot_sigma,
+                         const double* cot_hartree, const double* cot_F, double* gP,
+                         double* gVq, double* gh, double* gHH);
+/* The same on device pointers and a hipStream_t (NULL: the null stream);
+ * returns after the stream has drained the call's work. */
+int hfgpu_build_fock_vjp_device(hfgpu_kernel* k, const void* dP, const void* dGs,
+                                const void* dGh, const void* dGF, void* dgP, void* dgVq,
+                                void* dgh, void* dgHH, void* stream);
+
 /* Device-pointer energy: E = hf_energy_with_hartree_diag (fock.hpp:355-383)
  * written to device dE (one double).  stream as above.  Asynchronous. */
 int hfgpu_energy_device(hfgpu_kernel* k, const void* dP, const void* dSigma,
