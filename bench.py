@@ -18,6 +18,11 @@ The JSON line also carries:
                 FFTW/Eigen path, which cannot be built here).
   solves        wall time of a full direct-minimisation solve and an SCF solve
                 (mixing 0.5) on the same model.
+  mixed_precision  the FP32-workspace Fock build against the FP64 one at
+                96 x 96 (nb 8) and 192 x 192 (nb 16), and SCF solves at
+                192 x 192 with and without the mixed-precision phase: times,
+                |dF| per k, max |dSigma|, |dmu|, gap error.
+  adjoint       time of one device VJP of the Fock build at 144 x 144, nb 8.
 """
 
 from __future__ import annotations
@@ -47,6 +52,118 @@ def measured_traffic(nk1: int, nk2: int, nb: int):
         if line.startswith("traffic_bytes_per_build:"):
             return int(line.split(":", 1)[1])
     return None
+
+
+def _gap_about(F: np.ndarray, mu: float) -> float:
+    """Smallest band energy above mu minus the largest below (over all k)."""
+    e = np.linalg.eigvalsh(F)
+    return float(np.min(e[e > mu]) - np.max(e[e <= mu]))
+
+
+def run_mixed_precision(torch, dev) -> dict:
+    """16 bands (8 layers) on 192 x 192: the FP32-workspace Fock build against
+    the FP64 one (time and error of one build), and SCF solves (DIIS) with and
+    without the mixed-precision phase (errors of the converged state)."""
+    from cpp_hf_amd import SCFConfig, solve_scf
+    from cpp_hf_amd._compat import _native
+    from cpp_hf_amd.solver import _kernel_args_for_native
+    from cpp_hf_amd.workloads import initial_density, layered_kernel
+
+    out = {}
+    for n, nl in ((96, 4), (192, 8)):
+        wl = layered_kernel(n, nl, T=0.05, filling=nl + 0.1)
+        gk = _native.GpuKernel(_kernel_args_for_native(wl.kernel))
+        nb = 2 * nl
+        P0 = initial_density(n, nb, 0)
+        P = torch.from_numpy(np.ascontiguousarray(0.5 * (P0 + np.conj(np.swapaxes(P0, -1, -2))))).to(dev)
+        S = [torch.empty_like(P), torch.empty_like(P)]
+        F = [torch.empty_like(P), torch.empty_like(P)]
+        hd = torch.empty(nb, dtype=torch.float64, device=dev)
+        stream = torch.cuda.Stream(device=dev)
+        calls = (gk.build_fock_device, gk.build_fock_mixed_device)
+        times = []
+        for i, call in enumerate(calls):
+            for _ in range(3):
+                call(P.data_ptr(), S[i].data_ptr(), F[i].data_ptr(), hd.data_ptr(), True,
+                     stream.cuda_stream)
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            ev0.record(stream)
+            for _ in range(20):
+                call(P.data_ptr(), S[i].data_ptr(), F[i].data_ptr(), hd.data_ptr(), True,
+                     stream.cuda_stream)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            times.append(ev0.elapsed_time(ev1) * 1e3 / 20)
+        dS = (S[1] - S[0]).abs().max().item()
+        dF = (F[1] - F[0]).abs().max().item()
+        out[f"build_{n}x{n}_nb{nb}"] = {
+            "fp64_us": times[0], "mixed_us": times[1], "speedup": times[0] / times[1],
+            "mixed_capable": bool(gk.mixed_capable(True)),
+            "max_abs_dSigma": dS, "max_rel_dSigma": dS / S[0].abs().max().item(),
+            "max_abs_dF": dF,
+        }
+        del S, F, P, gk
+        torch.cuda.empty_cache()
+
+    wl = layered_kernel(192, 8, T=0.05, filling=8.1)
+    base = dict(max_iter=400, mixing=0.5, acceleration="diis", density_tol=1e-7, comm_tol=1e-8)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        t = time.perf_counter()
+        r = solve_scf(wl.kernel, wl.P0, wl.n_electrons,
+                      config=SCFConfig(exchange_precision=prec, **base))
+        res[prec] = (r, time.perf_counter() - t)
+    r64, t64 = res["fp64"]
+    r32, t32 = res["mixed"]
+    F64 = np.asarray(r64.fock_matrix)
+    F32 = np.asarray(r32.fock_matrix)
+    mu64, mu32 = float(r64.chemical_potential), float(r32.chemical_potential)
+    out["scf_192x192_nb16"] = {
+        "config": "layered QWZ, 8 layers, 192x192 k, layer-resolved V, Hartree+refP, "
+                  "T=0.05, 8.1 electrons/k, SCF DIIS, density_tol 1e-7, comm_tol 1e-8",
+        "fp64_s": t64, "mixed_s": t32,
+        "fp64_iterations": int(r64.iterations), "mixed_iterations": int(r32.iterations),
+        "converged": [bool(r64.converged), bool(r32.converged)],
+        "max_k_dF": float(np.max(np.sqrt(np.sum(np.abs(F32 - F64) ** 2, axis=(-2, -1))))),
+        "dmu": abs(mu32 - mu64),
+        "dE": abs(float(r32.energy) - float(r64.energy)),
+        "gap_fp64": _gap_about(F64, mu64),
+        "gap_error": abs(_gap_about(F32, mu32) - _gap_about(F64, mu64)),
+    }
+    return out
+
+
+def run_adjoint(torch, dev) -> dict:
+    """Time of one device VJP of the Fock build at 144 x 144, 8 bands."""
+    from cpp_hf_amd._compat import _native
+    from cpp_hf_amd.solver import _kernel_args_for_native
+    from cpp_hf_amd.workloads import initial_density, layered_kernel
+
+    wl = layered_kernel(144, 4, T=0.05, filling=4.1)
+    gk = _native.GpuKernel(_kernel_args_for_native(wl.kernel))
+    P0 = initial_density(144, 8, 0)
+    P = torch.from_numpy(np.ascontiguousarray(P0)).to(dev)
+    G = torch.from_numpy(np.ascontiguousarray(initial_density(144, 8, 2))).to(dev)
+    gP = torch.empty_like(P)
+    gV = torch.empty((144, 144, 8, 8), dtype=torch.complex128, device=dev)
+    gh = torch.empty_like(P)
+    gHH = torch.empty((8, 8), dtype=torch.float64, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+
+    def vjp():
+        gk.build_fock_vjp_device(P.data_ptr(), 0, 0, G.data_ptr(), gP.data_ptr(), gV.data_ptr(),
+                                 gh.data_ptr(), gHH.data_ptr(), stream.cuda_stream)
+
+    vjp()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(10):
+        vjp()
+    torch.cuda.synchronize()
+    return {"workload": "layered QWZ 144x144 k, nb=8, cotangent on F",
+            "ms_per_vjp": (time.perf_counter() - t) * 1e2}
 
 
 def run_sweep(concurrency: int) -> dict:
@@ -95,6 +212,8 @@ def main() -> int:
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the 48-kernel sweep (sequential vs concurrent solves)")
     ap.add_argument("--sweep-concurrency", type=int, default=8)
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the 192 x 192 mixed-precision and the 144 x 144 adjoint sections")
     ap.add_argument("--full-channels", action="store_true",
                     help="time the unpacked nb^2-channel exchange instead")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -236,6 +355,10 @@ def main() -> int:
 
     if not args.no_sweep:
         out["sweep"] = run_sweep(args.sweep_concurrency)
+
+    if not args.no_mixed:
+        out["mixed_precision"] = run_mixed_precision(torch, dev)
+        out["adjoint"] = run_adjoint(torch, dev)
 
     if not args.no_cpu_baseline:
         from oracle import hfcpu

@@ -41,6 +41,34 @@ __device__ __forceinline__ void store_stream(cd* p, cd v) {
     __builtin_nontemporal_store(t, reinterpret_cast<cd_vec*>(p));
 }
 
+// Interleaved complex float: the FP32 storage of the exchange workspace and
+// interaction in the mixed-precision Fock build (arithmetic stays FP64).
+struct alignas(8) cf {  // 8-byte aligned: one dwordx2 access per element
+    float x, y;
+};
+
+// workspace loads and streaming stores for either storage precision
+__device__ __forceinline__ cd ld_ws(const cd* p) { return *p; }
+__device__ __forceinline__ cd ld_ws(const cf* p) {
+    const cf v = *p;
+    return cd{static_cast<double>(v.x), static_cast<double>(v.y)};
+}
+typedef float cf_vec __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_stream(cf* p, cd v) {
+    cf_vec t;
+    t.x = static_cast<float>(v.x);
+    t.y = static_cast<float>(v.y);
+    __builtin_nontemporal_store(t, reinterpret_cast<cf_vec*>(p));
+}
+
+// store of a workspace element where eight lanes cover half a 128 B line
+// (FP32 columns): a regular store, so the line's two halves from the two
+// wavefronts of a workgroup merge in L2; full-line FP64 stores stream
+__device__ __forceinline__ void store_col_ws(cd* p, cd v) { store_stream(p, v); }
+__device__ __forceinline__ void store_col_ws(cf* p, cd v) {
+    *p = cf{static_cast<float>(v.x), static_cast<float>(v.y)};
+}
+
 struct DeviceError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };

@@ -107,6 +107,11 @@ struct ExchangeArgs {
     const cd* hh = nullptr;     // EPI_FOCK_PACKED: herm(h), tile-major by row
     const cd* SC = nullptr;     // (nb, nb) contact self-energy, nullable
     const double* hdiag = nullptr;  // (nb) Hartree diagonal, nullable
+    // mixed precision: workspace and interaction stored as complex float
+    // (two-step meshes only); the transforms' arithmetic stays FP64
+    bool ws32 = false;
+    cf* T32 = nullptr;          // workspace, tile_elems()
+    const cf* VR32 = nullptr;   // VRb or scalar VR in FP32
 };
 
 // Pass A without the transform: weighted row sums (+ small terms) only, for
@@ -142,6 +147,9 @@ void launch_energy_fullH(int nk, int nb, const cd* h, const cd* P, const cd* ref
 void launch_entropy(const double* p, const double* w, int nk, int nb, double* partial,
                     double* S_out, hipStream_t st);
 
+
+// out = (complex float) in, n elements.
+void launch_to_cf(size_t n, const cd* in, cf* out, hipStream_t st);
 
 // ---- adjoint building blocks (natural layout X[k1][k2][ch]) ----------------
 // Unnormalised 2-D transform of nch <= g.nch channels of a full-channel

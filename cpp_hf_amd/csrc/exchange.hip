@@ -633,12 +633,13 @@ struct FastShape {
 };
 
 // Pass A (rows, forward) on the two-step transform.
-template <int N1, int N2>
+// TS: storage type of the workspace (cd, or cf for the mixed-precision build).
+template <int N1, int N2, class TS>
 __global__ void __launch_bounds__(64 * kFastRowWavesMax)
 k_rows_fwd_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, int packed,
                 const cd* __restrict__ P, const cd* __restrict__ refP,
                 const cd* __restrict__ refP0, const double* __restrict__ wrow,
-                cd* __restrict__ T, cd* __restrict__ partial, const double* __restrict__ w2d,
+                TS* __restrict__ T, cd* __restrict__ partial, const double* __restrict__ w2d,
                 int rpw) {
     using Sh = FastShape<N1, N2>;
     constexpr int N = Sh::N, L = Sh::L;
@@ -719,7 +720,7 @@ k_rows_fwd_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
             partial[static_cast<size_t>(k1) * nch + ch] = acc;
         }
     }
-    cd* Tt = T + (static_cast<size_t>(blk) * nk1 + k1) * N * L;
+    TS* Tt = T + (static_cast<size_t>(blk) * nk1 + k1) * N * L;
 #pragma unroll
     for (int r = 0; r < Sh::R2; ++r) {
         const int t = lane + 64 * r;
@@ -744,10 +745,10 @@ k_rows_fwd_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
 
 // Pass B (columns: forward, x VR, inverse) on the two-step transform;
 // workgroup 0 hosts the small terms when has_small.
-template <int N1, int N2>
+template <int N1, int N2, class TS>
 __global__ void __launch_bounds__(64 * kFastWaves)
-k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, cd* __restrict__ T,
-            const cd* __restrict__ VR, int vr_scalar, int has_small,
+k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restrict__ T,
+            const TS* __restrict__ VR, int vr_scalar, int has_small,
             const cd* __restrict__ partial, const int* __restrict__ chan_a,
             const int* __restrict__ chan_b, int nb, int packed, SmallTerms st) {
     using Sh = FastShape<N1, N2>;
@@ -767,8 +768,8 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, cd* __restric
     const int tile = (static_cast<int>(blockIdx.x) - has_small) * nw + wave;
     const bool active = tile < nk2 * nblk;
     const int kc = active ? tile % nk2 : 0, blk = active ? tile / nk2 : 0;
-    cd* Tb = T + static_cast<size_t>(blk) * N * nk2 * L;                 // [k1][k2][c]
-    const cd* Vb = VR + (static_cast<size_t>(blk) * nk2 + kc) * N * L;   // [q1][c]
+    TS* Tb = T + static_cast<size_t>(blk) * N * nk2 * L;                 // [k1][k2][c]
+    const TS* Vb = VR + (static_cast<size_t>(blk) * nk2 + kc) * N * L;   // [q1][c]
     for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
     cd u[Sh::R1][N2];
     cd vv[Sh::R2][N1];
@@ -779,7 +780,7 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, cd* __restric
         const bool ok = active && t < L * N1;
 #pragma unroll
         for (int i2 = 0; i2 < N2; ++i2)
-            u[r][i2] = ok ? Tb[(static_cast<size_t>(i1 + N1 * i2) * nk2 + kc) * L + l]
+            u[r][i2] = ok ? ld_ws(Tb + (static_cast<size_t>(i1 + N1 * i2) * nk2 + kc) * L + l)
                           : cd{0.0, 0.0};
     }
 #pragma unroll
@@ -791,7 +792,8 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, cd* __restric
         for (int k1p = 0; k1p < N1; ++k1p) {
             const int q = k2 + N2 * k1p;
             vv[r][k1p] = !ok ? cd{0.0, 0.0}
-                             : (vr_scalar ? VR[static_cast<size_t>(q) * nk2 + kc] : Vb[q * L + l]);
+                             : (vr_scalar ? ld_ws(VR + static_cast<size_t>(q) * nk2 + kc)
+                                          : ld_ws(Vb + q * L + l));
         }
     }
     __syncthreads();
@@ -843,16 +845,16 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, cd* __restric
             Dft<N2, true>::run(z);
 #pragma unroll
             for (int i2 = 0; i2 < N2; ++i2)
-                store_stream(Tb + (static_cast<size_t>(i1 + N1 * i2) * nk2 + kc) * L + l, z[i2]);
+                store_col_ws(Tb + (static_cast<size_t>(i1 + N1 * i2) * nk2 + kc) * L + l, z[i2]);
         }
     }
 }
 
 // Pass C (rows, inverse + epilogue) on the two-step transform.
-template <int N1, int N2>
+template <int N1, int N2, class TS>
 __global__ void __launch_bounds__(64 * kFastRowWavesMax)
 k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, int packed,
-                const cd* __restrict__ T, int epilogue, int shift, double inv_n,
+                const TS* __restrict__ T, int epilogue, int shift, double inv_n,
                 cd* __restrict__ Sigma, cd* __restrict__ F, const cd* __restrict__ hh,
                 const cd* __restrict__ SC, const double* __restrict__ hdiag, int rpw) {
     using Sh = FastShape<N1, N2>;
@@ -878,7 +880,7 @@ k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
         const bool ok = active && t < L * N2;
 #pragma unroll
         for (int k1p = 0; k1p < N1; ++k1p)
-            v[r][k1p] = ok ? T[toff + (k2 + N2 * k1p) * L + l] : cd{0.0, 0.0};
+            v[r][k1p] = ok ? ld_ws(T + toff + (k2 + N2 * k1p) * L + l) : cd{0.0, 0.0};
     }
     cd hv[Sh::R1][N2];
     if (fock) {
@@ -1309,10 +1311,16 @@ static void launch_rows_fwd_fast(const ExchangeGeom& g, const ExchangeArgs& a, h
 #define HFGPU_FAST_CASE(N, A, B)                                                              \
     if (g.nk2 == N) {                                                                         \
         const int w = row_waves(g, A, true), rpw = (g.nblk + w - 1) / w;                      \
-        hipLaunchKernelGGL((k_rows_fwd_fast<A, B>), dim3(row_grid(g.nk1, rpw)), dim3(64 * w), \
-                           fast_lds(N, A, w, true), st, g.ax2.d_tw, g.nk1, g.nb, g.nch,       \
-                           g.nblk, g.packed ? 1 : 0, a.P, a.refP, a.refP0, a.wrow, a.T,       \
-                           a.partial, a.w2d, rpw);                                            \
+        if (a.ws32)                                                                           \
+            hipLaunchKernelGGL((k_rows_fwd_fast<A, B, cf>), dim3(row_grid(g.nk1, rpw)),       \
+                               dim3(64 * w), fast_lds(N, A, w, true), st, g.ax2.d_tw, g.nk1,  \
+                               g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, a.refP, a.refP0,   \
+                               a.wrow, a.T32, a.partial, a.w2d, rpw);                         \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_rows_fwd_fast<A, B, cd>), dim3(row_grid(g.nk1, rpw)),       \
+                               dim3(64 * w), fast_lds(N, A, w, true), st, g.ax2.d_tw, g.nk1,  \
+                               g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, a.refP, a.refP0,   \
+                               a.wrow, a.T, a.partial, a.w2d, rpw);                           \
         HFGPU_LAUNCH_CHECK();                                                                 \
         return;                                                                               \
     }
@@ -1329,9 +1337,16 @@ static void launch_cols_fast(const ExchangeGeom& g, const ExchangeArgs& a, int h
     if (g.nk1 == N) {                                                                         \
         const size_t lds = std::max(fast_lds(N, A, cw, false),                                \
                                     hs ? sizeof(cd) * small_terms_cd(g.nb) : size_t(0));      \
-        hipLaunchKernelGGL((k_cols_fast<A, B>), grid, block, lds, st, g.ax1.d_tw, g.nk2,      \
-                           g.nch, g.nblk, a.T, a.VR, a.vr_scalar ? 1 : 0, hs, a.partial,      \
-                           g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0, a.small);          \
+        if (a.ws32)                                                                           \
+            hipLaunchKernelGGL((k_cols_fast<A, B, cf>), grid, block, lds, st, g.ax1.d_tw,     \
+                               g.nk2, g.nch, g.nblk, a.T32, a.VR32, a.vr_scalar ? 1 : 0, hs,  \
+                               a.partial, g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0,     \
+                               a.small);                                                      \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_cols_fast<A, B, cd>), grid, block, lds, st, g.ax1.d_tw,     \
+                               g.nk2, g.nch, g.nblk, a.T, a.VR, a.vr_scalar ? 1 : 0, hs,      \
+                               a.partial, g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0,     \
+                               a.small);                                                      \
         HFGPU_LAUNCH_CHECK();                                                                 \
         return;                                                                               \
     }
@@ -1345,10 +1360,18 @@ static void launch_rows_inv_fast(const ExchangeGeom& g, const ExchangeArgs& a, d
 #define HFGPU_FAST_CASE(N, A, B)                                                              \
     if (g.nk2 == N) {                                                                         \
         const int w = row_waves(g, A, false), rpw = (g.nblk + w - 1) / w;                     \
-        hipLaunchKernelGGL((k_rows_inv_fast<A, B>), dim3(row_grid(g.nk1, rpw)), dim3(64 * w), \
-                           fast_lds(N, A, w, false), st, g.ax2.d_tw, g.nk1, g.nb, g.nch,      \
-                           g.nblk, g.packed ? 1 : 0, a.T, a.epilogue, a.ifftshift ? 1 : 0,    \
-                           inv_n, a.Sigma, a.F, a.hh, a.SC, a.hdiag, rpw);                    \
+        if (a.ws32)                                                                           \
+            hipLaunchKernelGGL((k_rows_inv_fast<A, B, cf>), dim3(row_grid(g.nk1, rpw)),       \
+                               dim3(64 * w), fast_lds(N, A, w, false), st, g.ax2.d_tw, g.nk1, \
+                               g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.T32, a.epilogue,      \
+                               a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh, a.SC, a.hdiag, \
+                               rpw);                                                          \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_rows_inv_fast<A, B, cd>), dim3(row_grid(g.nk1, rpw)),       \
+                               dim3(64 * w), fast_lds(N, A, w, false), st, g.ax2.d_tw, g.nk1, \
+                               g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.T, a.epilogue,        \
+                               a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh, a.SC, a.hdiag, \
+                               rpw);                                                          \
         HFGPU_LAUNCH_CHECK();                                                                 \
         return;                                                                               \
     }
@@ -1375,6 +1398,8 @@ void launch_row_sums(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t s
 }
 
 void launch_exchange_fwd(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
+    if (a.ws32 && !(g.fast_rows && g.fast_cols))
+        throw std::logic_error("the FP32 workspace needs the two-step transforms");
     launch_rows_fwd(g, a, 1, st);
     if (g.fast_cols) {
         // workgroup 0 of the column pass hosts the small terms
@@ -1403,6 +1428,7 @@ void launch_exchange_inv(const ExchangeGeom& g, const ExchangeArgs& a, hipStream
         launch_rows_inv_fast(g, a, inv_n, st);
         return;
     }
+    if (a.ws32) throw std::logic_error("the FP32 workspace needs the two-step transforms");
     HFGPU_EXCH_LAUNCH(k_rows_inv, g.generic, g.slots_row, dim3(g.nblk * g.nk1), g.lds_rows, st,
                       g.ax2.axis(), g.nk1, g.nk2, g.nb, g.nch, g.clog, g.nblk, g.packed ? 1 : 0,
                       a.T, a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh, a.SC,
@@ -1490,6 +1516,20 @@ void launch_entropy(const double* p, const double* w, int nk, int nb, double* pa
 // ---------------------------------------------------------------------------
 // adjoint building blocks
 // ---------------------------------------------------------------------------
+
+namespace {
+__global__ void __launch_bounds__(256) k_to_cf(size_t n, const cd* __restrict__ in, cf* __restrict__ out) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        out[i] = cf{static_cast<float>(in[i].x), static_cast<float>(in[i].y)};
+}
+}  // namespace
+
+void launch_to_cf(size_t n, const cd* in, cf* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_cf, dim3(std::max(1u, std::min<unsigned>(cdiv(n, 256), 8192u))),
+                       dim3(256), 0, st, n, in, out);
+    HFGPU_LAUNCH_CHECK();
+}
 
 void launch_fft2_plain(const ExchangeGeom& g, const cd* in, cd* out, int nch, bool inverse,
                        hipStream_t st) {

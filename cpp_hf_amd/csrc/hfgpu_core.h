@@ -85,11 +85,12 @@ struct GpuKernel {
         std::unique_ptr<ExchangeGeom> g;
         DevBuf VRb;  // VR in pass B order (dense V only)
         DevBuf hhb;  // herm(h) in pass C order (packed geometry only)
+        DevBuf VR32;  // VRb (dense V) or VR (scalar V) as complex float, on first use
     };
     Geom geo_full, geo_packed;
 
-    // Fock workspaces
-    DevBuf T_ws, partial, SC, hdiag, e_partial, e_out;
+    // Fock workspaces (T32_ws: the mixed-precision build's, on first use)
+    DevBuf T_ws, T32_ws, partial, SC, hdiag, e_partial, e_out;
 
     // Full-channel geometries of the adjoint's plain transforms (nb^2
     // channels, and one channel for a scalar interaction), made on first use.
@@ -103,9 +104,13 @@ struct GpuKernel {
 
     // build_fock_compact on device: Sigma, F, hdiag (nb) from P.  use_refP =
     // false drops the reference density (the linear response to a density
-    // variation, solver_rtr.hpp:32-39).
+    // variation, solver_rtr.hpp:32-39).  fp32_ws: the exchange workspace and
+    // interaction held in complex float (the mixed-precision build) on
+    // meshes of the two-step transforms; FP64 elsewhere.
     void build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, bool p_hermitian,
-                    hipStream_t st, bool use_refP = true);
+                    hipStream_t st, bool use_refP = true, bool fp32_ws = false);
+    // true when build_fock(..., fp32_ws = true) runs the FP32 workspace
+    bool fp32_capable(bool p_hermitian);
     // hf_energy_with_hartree_diag on device into E (one double).
     void energy(const cd* P, const cd* Sigma, const double* hdiag_in, double* E, hipStream_t st);
 

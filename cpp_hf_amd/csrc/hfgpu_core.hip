@@ -304,13 +304,32 @@ GpuKernel::Geom& GpuKernel::geom(bool packed) {
     return G;
 }
 
+bool GpuKernel::fp32_capable(bool p_hermitian) {
+    if (!exchange) return false;
+    const ExchangeGeom& g = *geom(packed_for(p_hermitian)).g;
+    return g.fast_rows && g.fast_cols;
+}
+
 void GpuKernel::build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, bool p_hermitian,
-                           hipStream_t st, bool use_refP) {
+                           hipStream_t st, bool use_refP, bool fp32_ws) {
     const bool need_partials = hartree || contact;
     const bool packed = exchange && packed_for(p_hermitian);
     Geom& G = geom(packed);
     const ExchangeGeom& g = *G.g;
     ExchangeArgs a;
+    if (fp32_ws && exchange && g.fast_rows && g.fast_cols) {
+        // FP32 copies of the workspace and the interaction, made once
+        if (T32_ws.size() < g.tile_elems() * sizeof(cf)) T32_ws.alloc(g.tile_elems() * sizeof(cf));
+        if (!G.VR32.get()) {
+            const size_t n = dv > 1 ? g.tile_elems() : nk;
+            G.VR32.alloc(n * sizeof(cf));
+            launch_to_cf(n, dv > 1 ? G.VRb.as<cd>() : VR_full.as<cd>(), G.VR32.as<cf>(), stream);
+            HFGPU_CHECK(hipStreamSynchronize(stream));
+        }
+        a.ws32 = true;
+        a.T32 = T32_ws.as<cf>();
+        a.VR32 = G.VR32.as<cf>();
+    }
     a.P = P;
     if (has_refP && use_refP) {
         if (refP_const) {
@@ -658,8 +677,9 @@ int hfgpu_solve_mu_inloop(const double* eps, const double* w_norm, size_t nk, si
     });
 }
 
-int hfgpu_build_fock_apply(hfgpu_kernel* k, const double* P, double* Sigma, double* Hh,
-                           double* F, double* E, hf_project_fn project, void* user) {
+static int build_fock_apply_impl(hfgpu_kernel* k, const double* P, double* Sigma, double* Hh,
+                                 double* F, double* E, hf_project_fn project, void* user,
+                                 bool fp32_ws) {
     return guarded([&] {
         if (!k) throw std::invalid_argument("null kernel");
         GpuKernel& K = *k->impl;
@@ -668,7 +688,8 @@ int hfgpu_build_fock_apply(hfgpu_kernel* k, const double* P, double* Sigma, doub
         DevBuf dP(bytes), dS(bytes), dF(bytes), dH(K.nb * sizeof(double)), dE(sizeof(double));
         h2d(dP.get(), P, bytes, s);
         const bool herm = K.dense_packable ? host_is_hermitian(P, K.nk, K.nb) : false;
-        K.build_fock(dP.as<cd>(), dS.as<cd>(), dF.as<cd>(), dH.as<double>(), herm, s);
+        K.build_fock(dP.as<cd>(), dS.as<cd>(), dF.as<cd>(), dH.as<double>(), herm, s, true,
+                     fp32_ws);
         // hf_energy (fock.hpp:324-351) with the dense Hartree array equals the
         // compact-diagonal form exactly: 0.5 * (s + h) == 0.5 * s + 0.5 * h.
         K.energy(dP.as<cd>(), dS.as<cd>(), dH.as<double>(), dE.as<double>(), s);
@@ -688,6 +709,34 @@ int hfgpu_build_fock_apply(hfgpu_kernel* k, const double* P, double* Sigma, doub
             host_hermitize(F, K.nk, K.nb);
         }
     });
+}
+
+int hfgpu_build_fock_apply(hfgpu_kernel* k, const double* P, double* Sigma, double* Hh,
+                           double* F, double* E, hf_project_fn project, void* user) {
+    return build_fock_apply_impl(k, P, Sigma, Hh, F, E, project, user, false);
+}
+
+int hfgpu_build_fock_mixed(hfgpu_kernel* k, const double* P, double* Sigma, double* Hh,
+                           double* F, double* E, hf_project_fn project, void* user) {
+    return build_fock_apply_impl(k, P, Sigma, Hh, F, E, project, user, true);
+}
+
+int hfgpu_build_fock_mixed_device(hfgpu_kernel* k, const void* dP, void* dSigma, void* dF,
+                                  void* dHdiag, int p_hermitian, void* stream) {
+    return guarded([&] {
+        if (!k) throw std::invalid_argument("null kernel");
+        GpuKernel& K = *k->impl;
+        K.build_fock(static_cast<const cd*>(dP), static_cast<cd*>(dSigma), static_cast<cd*>(dF),
+                     static_cast<double*>(dHdiag), p_hermitian != 0,
+                     static_cast<hipStream_t>(stream), true, true);
+    });
+}
+
+int hfgpu_kernel_mixed_capable(hfgpu_kernel* k, int p_hermitian) {
+    if (!k) return 0;
+    int r = 0;
+    const int rc = guarded([&] { r = k->impl->fp32_capable(p_hermitian != 0) ? 1 : 0; });
+    return rc == HF_OK ? r : 0;
 }
 
 int hfgpu_build_fock_device(hfgpu_kernel* k, const void* dP, void* dSigma, void* dF,

@@ -301,7 +301,8 @@ PYBIND11_MODULE(_native, m) {
         py::arg("T"), py::arg("maxiter") = 25);
 
     // --- build_fock (cpp_hf_native.cpp:380-415) ---
-    m.def("build_fock_apply", [](py::object kernel_args, CArr P, py::object project_fn) {
+    m.def("build_fock_apply", [](py::object kernel_args, CArr P, py::object project_fn,
+                                 bool exchange_fp32) {
         KernelArgs ka(kernel_args);
         if (P.ndim() != 4 || (size_t)P.shape(0) != ka.d.nk1 || (size_t)P.shape(1) != ka.d.nk2 ||
             (size_t)P.shape(2) != ka.d.nb || (size_t)P.shape(3) != ka.d.nb)
@@ -318,12 +319,16 @@ PYBIND11_MODULE(_native, m) {
         int rc;
         {
             py::gil_scoped_release rel;
-            rc = hfgpu_build_fock_apply(H.k, cdp(P), dp(sa), dp(ha), dp(fa), &E,
-                                        has_proj ? project_trampoline : nullptr, &ctx);
+            rc = (exchange_fp32 ? hfgpu_build_fock_mixed : hfgpu_build_fock_apply)(
+                H.k, cdp(P), dp(sa), dp(ha), dp(fa), &E, has_proj ? project_trampoline : nullptr,
+                &ctx);
         }
         check_project(rc, &ctx);
         return py::make_tuple(S, Hh, F, E);
-    });
+    },
+          // exchange_fp32: the mixed-precision build (no reference counterpart)
+          py::arg("kernel_args"), py::arg("P"), py::arg("project_fn"),
+          py::arg("exchange_fp32") = false);
 
     // --- adjoint of build_fock (no reference counterpart): (gP, gVq, gh, gHH) ---
     m.def("build_fock_vjp", [](py::object kernel_args, CArr P, py::object cot_sigma,
@@ -371,7 +376,7 @@ PYBIND11_MODULE(_native, m) {
                           py::object project_fn, std::vector<size_t> block_sizes,
                           std::string acceleration, size_t diis_size, size_t diis_start,
                           double diis_damping, double trust_radius, bool return_density,
-                          bool return_fock) {
+                          bool return_fock, bool exchange_fp32, double fp32_switch) {
         KernelArgs ka(kernel_args);
         if (P0.ndim() != 4 || (size_t)P0.shape(0) != ka.d.nk1 ||
             (size_t)P0.shape(1) != ka.d.nk2 || (size_t)P0.shape(2) != ka.d.nb ||
@@ -395,6 +400,8 @@ PYBIND11_MODULE(_native, m) {
         cfg.trust_radius = trust_radius;
         cfg.block_sizes = block_sizes.empty() ? nullptr : block_sizes.data();
         cfg.n_blocks = block_sizes.size();
+        cfg.exchange_fp32 = exchange_fp32 ? 1 : 0;
+        cfg.fp32_switch = fp32_switch;
         Handle H(ka.d);
         std::vector<py::ssize_t> shape{(py::ssize_t)ka.d.nk1, (py::ssize_t)ka.d.nk2,
                                        (py::ssize_t)ka.d.nb, (py::ssize_t)ka.d.nb};
@@ -426,7 +433,15 @@ PYBIND11_MODULE(_native, m) {
                               (bool)res.converged, hE[py::slice(0, n, 1)].attr("copy")(),
                               hD[py::slice(0, n, 1)].attr("copy")(),
                               hC[py::slice(0, n, 1)].attr("copy")());
-    });
+    },
+          // the reference's positional arguments, then the mixed-precision
+          // switch (no counterpart in the reference; off by default)
+          py::arg("kernel_args"), py::arg("P0"), py::arg("n_e"), py::arg("max_iter"),
+          py::arg("density_tol"), py::arg("comm_tol"), py::arg("mixing"), py::arg("level_shift"),
+          py::arg("project_fn"), py::arg("block_sizes"), py::arg("acceleration"),
+          py::arg("diis_size"), py::arg("diis_start"), py::arg("diis_damping"),
+          py::arg("trust_radius"), py::arg("return_density"), py::arg("return_fock"),
+          py::arg("exchange_fp32") = false, py::arg("fp32_switch") = 1e-6);
 
     // --- Direct minimisation (cpp_hf_native.cpp:488-582): optimizer 1 is the
     // trust-region Newton solver (with deflation), any other code the CG loop ---
@@ -437,7 +452,7 @@ PYBIND11_MODULE(_native, m) {
                          size_t tr_cg_max, CArr deflation_targets, double deflation_sigma,
                          double deflation_length, std::vector<size_t> block_sizes,
                          py::object project_fn, bool return_Q, bool return_density,
-                         bool return_fock) {
+                         bool return_fock, bool exchange_fp32, double fp32_switch) {
         KernelArgs ka(kernel_args);
         if (P0.ndim() != 4 || (size_t)P0.shape(0) != ka.d.nk1 ||
             (size_t)P0.shape(1) != ka.d.nk2 || (size_t)P0.shape(2) != ka.d.nb ||
@@ -471,6 +486,8 @@ PYBIND11_MODULE(_native, m) {
         }
         cfg.deflation_sigma = deflation_sigma;
         cfg.deflation_length = deflation_length;
+        cfg.exchange_fp32 = exchange_fp32 ? 1 : 0;
+        cfg.fp32_switch = fp32_switch;
         Handle H(ka.d);
         std::vector<py::ssize_t> shape{(py::ssize_t)ka.d.nk1, (py::ssize_t)ka.d.nk2,
                                        (py::ssize_t)ka.d.nb, (py::ssize_t)ka.d.nb};
@@ -506,7 +523,16 @@ PYBIND11_MODULE(_native, m) {
         return py::make_tuple(q_obj, p, d_obj, f_obj, res.mu, res.energy, res.n_iter,
                               (bool)res.converged, hE[py::slice(0, n, 1)].attr("copy")(),
                               hG[py::slice(0, n, 1)].attr("copy")());
-    });
+    },
+          py::arg("kernel_args"), py::arg("P0"), py::arg("n_e"), py::arg("max_iter"),
+          py::arg("tol_E"), py::arg("tol_grad"), py::arg("max_step"), py::arg("bt_shrink"),
+          py::arg("denom_scale"), py::arg("bt_max"), py::arg("cg_restart"),
+          py::arg("plateau_window"), py::arg("mu_maxiter"), py::arg("optimizer"),
+          py::arg("tr_delta0"), py::arg("tr_cg_max"), py::arg("deflation_targets"),
+          py::arg("deflation_sigma"), py::arg("deflation_length"), py::arg("block_sizes"),
+          py::arg("project_fn"), py::arg("return_Q"), py::arg("return_density"),
+          py::arg("return_fock"), py::arg("exchange_fp32") = false,
+          py::arg("fp32_switch") = 1e-6);
 
     // --- building blocks of the Python layer ---
     m.def("hf_energy", [](CArr P, CArr h, CArr Sigma, CArr H, RArr w, py::object refP) {
@@ -677,6 +703,26 @@ PYBIND11_MODULE(_native, m) {
             },
             py::arg("P"), py::arg("Sigma"), py::arg("F"), py::arg("hdiag"),
             py::arg("p_hermitian") = true, py::arg("stream") = 0)
+        .def(
+            "build_fock_mixed_device",
+            [](PyKernel& k, std::uintptr_t P, std::uintptr_t Sigma, std::uintptr_t F,
+               std::uintptr_t hdiag, bool p_hermitian, std::uintptr_t stream) {
+                int rc;
+                {
+                    py::gil_scoped_release rel;
+                    rc = hfgpu_build_fock_mixed_device(k.k, (const void*)P, (void*)Sigma,
+                                                       (void*)F, (void*)hdiag,
+                                                       p_hermitian ? 1 : 0, (void*)stream);
+                }
+                check(rc);
+            },
+            py::arg("P"), py::arg("Sigma"), py::arg("F"), py::arg("hdiag"),
+            py::arg("p_hermitian") = true, py::arg("stream") = 0)
+        .def("mixed_capable",
+             [](PyKernel& k, bool p_hermitian) {
+                 return hfgpu_kernel_mixed_capable(k.k, p_hermitian ? 1 : 0) != 0;
+             },
+             py::arg("p_hermitian") = true)
         .def(
             "energy_device",
             [](PyKernel& k, std::uintptr_t P, std::uintptr_t Sigma, std::uintptr_t hdiag,

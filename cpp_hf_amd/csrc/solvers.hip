@@ -161,6 +161,9 @@ struct ScfState {
     DevBuf eps, shifted, occ, V;
     Projector& proj;
     double n_e, level_shift;
+    // Fock builds with the FP32 exchange workspace (mixed precision) until
+    // the solver switches to the FP64 correction phase
+    bool mixed = false;
     // block_sizes summing to nb with more than one entry (solver_scf.hpp:
     // 107-110; other values leave the dense eigh, without an error)
     bool blocks = false;
@@ -219,7 +222,7 @@ struct ScfState {
     void build_and_occupy(cd* d, cd* Sigma, cd* F, double* hdiag, cd* P_out, double* E_slot) {
         if (proj) proj.apply(d, st);
         launch_hermitize(static_cast<int>(K.nk), K.nb, d, st);
-        K.build_fock(d, Sigma, F, hdiag, true, st);
+        K.build_fock(d, Sigma, F, hdiag, true, st, true, mixed);
         if (proj) {
             proj.apply(F, st);
             launch_hermitize(static_cast<int>(K.nk), K.nb, F, st);
@@ -261,6 +264,8 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
         hdiag_new(use_oda ? nb * 8 : 0), pre(cfg.trust_radius > 0.0 ? bytes : 0),
         diff(cfg.trust_radius > 0.0 ? bytes : 0);
     std::vector<DevBuf> diis_F, diis_R;
+    S.mixed = cfg.exchange_fp32 != 0;
+    const double fp32_switch = cfg.fp32_switch > 0.0 ? cfg.fp32_switch : 1e-6;
     h2d(density.get(), P0, bytes, st);
     double* sc = S.ws.sc.as<double>();
 
@@ -283,6 +288,12 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
         const double d_res = std::sqrt(hsc[SC_RES] / ws);
         const double c_res = std::sqrt(hsc[SC_RES + 1] / ws);
         converged = (d_res <= cfg.density_tol) && (c_res <= cfg.comm_tol);
+        // mixed precision: the FP64 correction phase starts below the switch
+        // residual, and convergence counts only on FP64 builds
+        if (S.mixed && (converged || d_res <= fp32_switch)) {
+            S.mixed = false;
+            converged = false;
+        }
         const bool clip = cfg.trust_radius > 0.0 && !converged;
         if (clip) d2d(pre.get(), density.get(), bytes, st);
 
@@ -348,7 +359,7 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
             }
         } else if (use_oda) {
             K.build_fock(P_new.as<cd>(), Sigma_new.as<cd>(), F_new.as<cd>(),
-                         hdiag_new.as<double>(), true, st);
+                         hdiag_new.as<double>(), true, st, true, S.mixed);
             if (proj) {
                 proj.apply(F_new.as<cd>(), st);
                 launch_hermitize(nk, nb, F_new.as<cd>(), st);
@@ -392,6 +403,7 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
         (void)hmu;
         ++k;
     }
+    S.mixed = false;
     S.build_and_occupy(density.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(),
                        P_new.as<cd>(), sc + SC_E);
     d2h(hsc, sc, sizeof(hsc), st);
@@ -481,10 +493,16 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
             launch_eigh(nk, nb, in_mode, M, w, V, st);
     };
 
+    // mixed precision: FP32 exchange workspace until the gradient norm
+    // reaches the switch, FP64 builds afterwards and in the finalisation
+    bool mixed = cfg.exchange_fp32 != 0;
+    const double fp32_switch = cfg.fp32_switch > 0.0 ? cfg.fp32_switch : 1e-6;
+
     // ---- init (solver_dm.hpp:436-459)
     h2d(P.get(), P0, bytes, st);
     launch_hermitize(nk, nb, P.as<cd>(), st);
-    K.build_fock(P.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(), true, st);
+    K.build_fock(P.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(), true, st, true,
+                 mixed);
     if (use_block && !block_ok_host(F.as<cd>(), nk, nb, bm, st)) use_block = false;
     eigh(0, F.as<cd>(), eps.as<double>(), Q.as<cd>());
     mu_solve(eps.as<double>(), false, 0.0);
@@ -502,10 +520,14 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
         else
             e_nc = dE > cfg.tol_E;
         const bool g_nc = (cfg.tol_grad > 0.0) && (grad_norm > cfg.tol_grad);
-        if (!(e_nc || g_nc)) break;
+        if (!(e_nc || g_nc)) {
+            if (!mixed) break;
+            mixed = false;  // stop only on FP64 builds
+        }
 
         density_step();
-        K.build_fock(P.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(), true, st);
+        K.build_fock(P.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(), true, st, true,
+                     mixed);
         K.energy(P.as<cd>(), Sigma.as<cd>(), hdiag.as<double>(), sc + SC_E, st);
         launch_orbital_grad(nk, nb, Q.as<cd>(), F.as<cd>(), p.as<double>(), wd, W.mu(), T_r, true,
                             eps.as<double>(), G.as<cd>(), g_p.as<double>(), W.ls, st);
@@ -530,6 +552,7 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
         grad_norm = std::sqrt(std::max(0.0, hsc[SC_GN2]));
         hist_E.push_back(E);
         hist_G.push_back(grad_norm);
+        if (mixed && grad_norm <= fp32_switch) mixed = false;
         dE = std::fabs(E - E_prev);
         std::swap(dQ, dQp);
         std::swap(d_p, d_pp);

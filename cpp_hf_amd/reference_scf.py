@@ -31,6 +31,12 @@ class SCFConfig:
     trust_radius: float = 0.0
     return_density: bool = True
     return_fock: bool = True
+    # Mixed precision (no counterpart in the reference; "fp64" keeps its
+    # behaviour): "mixed" builds the Fock matrix with the FP32 exchange
+    # workspace while the density residual is above mixed_switch, then runs
+    # FP64 builds to convergence.
+    exchange_precision: str = "fp64"
+    mixed_switch: float = 1e-6
 
     def __post_init__(self) -> None:
         if self.max_iter <= 0:
@@ -49,6 +55,13 @@ class SCFResult:
     history: dict[str, Any] = field(default_factory=dict)
 
 
+def _mixed_flag(precision: str) -> bool:
+    p = str(precision).lower()
+    if p not in ("fp64", "mixed"):
+        raise ValueError(f"exchange_precision must be 'fp64' or 'mixed', got {precision!r}")
+    return p == "mixed"
+
+
 def solve_scf(kernel, P0, n_electrons: float, *, config: SCFConfig | None = None) -> SCFResult:
     """Reference SCF solve on the GPU (reference_scf.py:114-165)."""
     native_required()
@@ -62,6 +75,7 @@ def solve_scf(kernel, P0, n_electrons: float, *, config: SCFConfig | None = None
         [int(s) for s in (config.block_sizes or ())], str(config.acceleration),
         int(config.diis_size), int(config.diis_start), float(config.diis_damping),
         float(config.trust_radius), bool(config.return_density), bool(config.return_fock),
+        exchange_fp32=_mixed_flag(config.exchange_precision), fp32_switch=float(config.mixed_switch),
     )
     n_iter = int(iterations)
     is_conv = bool(converged)

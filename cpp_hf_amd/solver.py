@@ -48,6 +48,11 @@ class SolverConfig:
     return_Q: bool = True
     return_density: bool = True
     return_fock: bool = True
+    # Mixed precision for the CG loop (no counterpart in the reference;
+    # "fp64" keeps its behaviour): "mixed" uses the FP32 exchange workspace
+    # until the gradient norm reaches mixed_switch, then FP64 builds.
+    exchange_precision: str = "fp64"
+    mixed_switch: float = 1e-6
 
 
 class SolveResult(NamedTuple):
@@ -95,6 +100,13 @@ def _optimizer_code(name: str) -> int:
         ) from None
 
 
+def _mixed_flag(precision: str) -> bool:
+    p = str(precision).lower()
+    if p not in ("fp64", "mixed"):
+        raise ValueError(f"exchange_precision must be 'fp64' or 'mixed', got {precision!r}")
+    return p == "mixed"
+
+
 def solve_direct_minimization(kernel, P0, n_electrons: float, *,
                               config: SolverConfig | None = None) -> SolveResult:
     """Direct-minimisation Hartree-Fock solve on the GPU (solver.py:150-220)."""
@@ -131,7 +143,8 @@ def solve_direct_minimization(kernel, P0, n_electrons: float, *,
         int(config.mu_maxiter), int(code), float(config.tr_delta0), int(config.tr_cg_max),
         defl, float(config.deflation_sigma), float(config.deflation_length), block_sizes,
         config.project_fn, bool(config.return_Q), bool(config.return_density),
-        bool(config.return_fock),
+        bool(config.return_fock), exchange_fp32=_mixed_flag(config.exchange_precision),
+        fp32_switch=float(config.mixed_switch),
     )
     return SolveResult(
         Q=Q, p=p, mu=np.asarray(mu), density=density, fock=fock, energy=np.asarray(energy),

@@ -67,6 +67,12 @@ typedef struct hf_scf_config {
     double trust_radius;
     const size_t* block_sizes; /* may be NULL when n_blocks == 0 */
     size_t n_blocks;
+    /* Mixed precision (no counterpart in the reference): exchange_fp32 != 0
+     * runs the Fock builds with the FP32 exchange workspace while the
+     * density residual is above fp32_switch (1e-6 when <= 0), then FP64
+     * builds to convergence (the FP64 correction phase).  0: FP64 only. */
+    int exchange_fp32;
+    double fp32_switch;
 } hf_scf_config;
 
 /* Reference: SolverConfig, solver_dm.hpp:22-63.  optimizer 0: Riemannian CG
@@ -95,6 +101,12 @@ typedef struct hf_dm_config {
     size_t n_deflation;
     double deflation_sigma;
     double deflation_length;
+    /* Mixed precision for the CG loop (no counterpart in the reference): as
+     * in hf_scf_config, switching to FP64 builds once the gradient norm is
+     * at most fp32_switch (1e-6 when <= 0).  The trust-region solver always
+     * builds in FP64. */
+    int exchange_fp32;
+    double fp32_switch;
 } hf_dm_config;
 
 /*

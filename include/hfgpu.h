@@ -125,6 +125,22 @@ int hfgpu_build_fock_apply(hfgpu_kernel* k, const double* P, double* Sigma, doub
 int hfgpu_build_fock_device(hfgpu_kernel* k, const void* dP, void* dSigma, void* dF,
                             void* dHdiag, int p_hermitian, void* stream);
 
+/* Mixed-precision Fock build: the same outputs as hfgpu_build_fock_apply /
+ * hfgpu_build_fock_device with the exchange workspace and the interaction
+ * held in complex float between the transform passes (the transforms'
+ * arithmetic, P, Sigma, F, the Hartree and contact terms stay FP64), so
+ * Sigma carries a relative error of order 1e-7.  No counterpart in the
+ * reference (FP64 only); for the early iterations of a solve, whose FP64
+ * correction phase finishes it (hf_scf_config / hf_dm_config
+ * exchange_fp32).  Meshes without the two-step transforms (both axes in
+ * 12, 16, 24, 32, 36, 48, 60, 64, 72, 96, 120, 144, 192) run the FP64
+ * workspace; hfgpu_kernel_mixed_capable says which applies. */
+int hfgpu_build_fock_mixed(hfgpu_kernel* k, const double* P, double* Sigma, double* Hh,
+                           double* F, double* E, hf_project_fn project, void* user);
+int hfgpu_build_fock_mixed_device(hfgpu_kernel* k, const void* dP, void* dSigma, void* dF,
+                                  void* dHdiag, int p_hermitian, void* stream);
+int hfgpu_kernel_mixed_capable(hfgpu_kernel* k, int p_hermitian);
+
 /* Vector-Jacobian product of the Fock build.  No counterpart in the
  * reference, which has no derivatives; for callers that differentiate
  * through build_fock (autograd, parameter gradients at a converged state).

@@ -44,6 +44,7 @@ class ScfConfig(C.Structure):
         ("acceleration", C.c_int), ("diis_size", _sz), ("diis_start", _sz),
         ("diis_damping", C.c_double), ("trust_radius", C.c_double),
         ("block_sizes", C.POINTER(_sz)), ("n_blocks", _sz),
+        ("exchange_fp32", C.c_int), ("fp32_switch", C.c_double),
     ]
 
 
@@ -56,7 +57,8 @@ class DmConfig(C.Structure):
         ("block_sizes", C.POINTER(_sz)), ("n_blocks", _sz),
         ("optimizer", C.c_int), ("tr_delta0", C.c_double), ("tr_cg_max", _sz),
         ("deflation_targets", _dp), ("n_deflation", _sz), ("deflation_sigma", C.c_double),
-        ("deflation_length", C.c_double),
+        ("deflation_length", C.c_double), ("exchange_fp32", C.c_int),
+        ("fp32_switch", C.c_double),
     ]
 
 
