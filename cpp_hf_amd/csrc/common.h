@@ -61,6 +61,11 @@ __device__ __forceinline__ void store_stream(cf* p, cd v) {
     __builtin_nontemporal_store(t, reinterpret_cast<cf_vec*>(p));
 }
 
+__device__ __forceinline__ cd to_cd(cd v) { return v; }
+__device__ __forceinline__ cd to_cd(cf v) {
+    return cd{static_cast<double>(v.x), static_cast<double>(v.y)};
+}
+
 // store of a workspace element where eight lanes cover half a 128 B line
 // (FP32 columns): a regular store, so the line's two halves from the two
 // wavefronts of a workgroup merge in L2; full-line FP64 stores stream

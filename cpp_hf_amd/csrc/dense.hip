@@ -5,8 +5,8 @@
 // hermitize_inplace (linalg.hpp:20-33).
 //
 // One wavefront per k point, four per workgroup; the k point's V block is
-// staged in LDS with coalesced 16 B loads, each lane then forms its own
-// output elements.  At nb = 8 the density build moves 2A + a bytes
+// staged in LDS (transposed, padded) with coalesced 16 B loads, each lane
+// then forms its own output elements.  At nb = 8 the density build moves 2A + a bytes
 // (A = 16 nk nb^2) for 8 nb^3 nk flops: HBM-bound.
 #include "kernels.h"
 
@@ -14,31 +14,42 @@ namespace hfgpu {
 
 namespace {
 
+// V is staged transposed with a padded stride, sVt[n * (nb + 1) + j] = V_jn:
+// the lanes of an output row read consecutive j (no LDS bank conflicts; the
+// row-major block read with stride nb put every lane of a row on one bank
+// group at nb = 16), and the row index i is a broadcast.  NB > 0 fixes nb at
+// compile time.
+template <int NB>
 __global__ void __launch_bounds__(256)
-k_density(int nk, int nb, const cd* __restrict__ V, const double* __restrict__ f,
+k_density(int nk, int nb_arg, const cd* __restrict__ V, const double* __restrict__ f,
           cd* __restrict__ P, int hermitize) {
+    const int nb = NB > 0 ? NB : nb_arg;
     extern __shared__ cd smem[];
-    const int nb2 = nb * nb;
+    const int nb2 = nb * nb, ld = nb + 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = blockIdx.x * 4 + wave;
-    cd* sv = smem + wave * (nb2 + nb);
-    double* sf = reinterpret_cast<double*>(sv + nb2);
+    cd* sv = smem + wave * (nb * ld + nb);
+    double* sf = reinterpret_cast<double*>(sv + nb * ld);
     if (k < nk) {
         const size_t kb = static_cast<size_t>(k) * nb2;
-        for (int e = lane; e < nb2; e += 64) sv[e] = V[kb + e];
+        for (int e = lane; e < nb2; e += 64) {
+            const int j = e / nb, n = e - j * nb;
+            sv[n * ld + j] = V[kb + e];
+        }
         for (int e = lane; e < nb; e += 64) sf[e] = f[static_cast<size_t>(k) * nb + e];
     }
     __syncthreads();
     if (k >= nk) return;
     const size_t kb = static_cast<size_t>(k) * nb2;
     for (int e = lane; e < nb2; e += 64) {
-        const int i = e / nb, j = e % nb;
+        const int i = e / nb, j = e - i * nb;
         cd pij{0.0, 0.0};
-        for (int n = 0; n < nb; ++n) pij = cadd(pij, cmulc(cscale(sv[i * nb + n], sf[n]), sv[j * nb + n]));
+        for (int n = 0; n < nb; ++n)
+            pij = cadd(pij, cmulc(cscale(sv[n * ld + i], sf[n]), sv[n * ld + j]));
         if (hermitize) {
             cd pji{0.0, 0.0};
             for (int n = 0; n < nb; ++n)
-                pji = cadd(pji, cmulc(cscale(sv[j * nb + n], sf[n]), sv[i * nb + n]));
+                pji = cadd(pji, cmulc(cscale(sv[n * ld + j], sf[n]), sv[n * ld + i]));
             pij = cd{0.5 * (pij.x + pji.x), 0.5 * (pij.y - pji.y)};
         }
         P[kb + e] = pij;
@@ -68,9 +79,17 @@ __global__ void k_hermitize(int nk, int nb, cd* __restrict__ M) {
 
 void launch_density(int nk, int nb, const cd* V, const double* f, cd* P, bool hermitize,
                     hipStream_t st) {
-    const size_t lds = 4 * (sizeof(cd) * (nb * nb + nb));
-    hipLaunchKernelGGL(k_density, dim3(cdiv(nk, 4)), dim3(256), lds, st, nk, nb, V, f, P,
-                       hermitize ? 1 : 0);
+    const size_t lds = 4 * (sizeof(cd) * (nb * (nb + 1) + nb));
+    const dim3 grid(cdiv(nk, 4)), block(256);
+    const int h = hermitize ? 1 : 0;
+    switch (nb) {
+        case 2: hipLaunchKernelGGL(k_density<2>, grid, block, lds, st, nk, nb, V, f, P, h); break;
+        case 4: hipLaunchKernelGGL(k_density<4>, grid, block, lds, st, nk, nb, V, f, P, h); break;
+        case 8: hipLaunchKernelGGL(k_density<8>, grid, block, lds, st, nk, nb, V, f, P, h); break;
+        case 12: hipLaunchKernelGGL(k_density<12>, grid, block, lds, st, nk, nb, V, f, P, h); break;
+        case 16: hipLaunchKernelGGL(k_density<16>, grid, block, lds, st, nk, nb, V, f, P, h); break;
+        default: hipLaunchKernelGGL(k_density<0>, grid, block, lds, st, nk, nb, V, f, P, h); break;
+    }
     HFGPU_LAUNCH_CHECK();
 }
 

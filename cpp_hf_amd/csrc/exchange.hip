@@ -771,8 +771,10 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restric
     TS* Tb = T + static_cast<size_t>(blk) * N * nk2 * L;                 // [k1][k2][c]
     const TS* Vb = VR + (static_cast<size_t>(blk) * nk2 + kc) * N * L;   // [q1][c]
     for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
-    cd u[Sh::R1][N2];
-    cd vv[Sh::R2][N1];
+    // loads land in registers of the storage type and are widened after the
+    // barrier, so every load of the wave is in flight before the first use
+    TS u[Sh::R1][N2];
+    TS vv[Sh::R2][N1];
 #pragma unroll
     for (int r = 0; r < Sh::R1; ++r) {
         const int t = lane + 64 * r;
@@ -780,8 +782,7 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restric
         const bool ok = active && t < L * N1;
 #pragma unroll
         for (int i2 = 0; i2 < N2; ++i2)
-            u[r][i2] = ok ? ld_ws(Tb + (static_cast<size_t>(i1 + N1 * i2) * nk2 + kc) * L + l)
-                          : cd{0.0, 0.0};
+            u[r][i2] = ok ? Tb[(static_cast<size_t>(i1 + N1 * i2) * nk2 + kc) * L + l] : TS{};
     }
 #pragma unroll
     for (int r = 0; r < Sh::R2; ++r) {
@@ -791,9 +792,8 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restric
 #pragma unroll
         for (int k1p = 0; k1p < N1; ++k1p) {
             const int q = k2 + N2 * k1p;
-            vv[r][k1p] = !ok ? cd{0.0, 0.0}
-                             : (vr_scalar ? ld_ws(VR + static_cast<size_t>(q) * nk2 + kc)
-                                          : ld_ws(Vb + q * L + l));
+            vv[r][k1p] = !ok ? TS{}
+                             : (vr_scalar ? VR[static_cast<size_t>(q) * nk2 + kc] : Vb[q * L + l]);
         }
     }
     __syncthreads();
@@ -802,10 +802,13 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restric
         const int t = lane + 64 * r;
         if (t < L * N1) {
             const int l = t & (L - 1), i1 = t >> 3;
-            Dft<N2, false>::run(u[r]);
+            cd w[N2];
+#pragma unroll
+            for (int i2 = 0; i2 < N2; ++i2) w[i2] = to_cd(u[r][i2]);
+            Dft<N2, false>::run(w);
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2) {
-                cd x = u[r][k2];
+                cd x = w[k2];
                 if (k2 > 0) x = cmul(x, s_tw[i1 * k2]);
                 Y[(i1 * N2 + k2) * L + l] = x;
             }
@@ -822,7 +825,7 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restric
             for (int i1 = 0; i1 < N1; ++i1) v[i1] = Y[(i1 * N2 + k2) * L + l];
             Dft<N1, false>::run(v);
 #pragma unroll
-            for (int k1p = 0; k1p < N1; ++k1p) v[k1p] = cmul(v[k1p], vv[r][k1p]);
+            for (int k1p = 0; k1p < N1; ++k1p) v[k1p] = cmul(v[k1p], to_cd(vv[r][k1p]));
             Dft<N1, true>::run(v);
             // this task reads and rewrites the same Y slots: no barrier needed
 #pragma unroll
@@ -872,7 +875,7 @@ k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
     const size_t toff = (static_cast<size_t>(blk) * nk1 + k1) * N * L;
     const bool fock = epilogue == EPI_FOCK_PACKED;
     for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
-    cd v[Sh::R2][N1];
+    TS v[Sh::R2][N1];  // storage type, widened after the barrier
 #pragma unroll
     for (int r = 0; r < Sh::R2; ++r) {
         const int t = lane + 64 * r;
@@ -880,7 +883,7 @@ k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
         const bool ok = active && t < L * N2;
 #pragma unroll
         for (int k1p = 0; k1p < N1; ++k1p)
-            v[r][k1p] = ok ? ld_ws(T + toff + (k2 + N2 * k1p) * L + l) : cd{0.0, 0.0};
+            v[r][k1p] = ok ? T[toff + (k2 + N2 * k1p) * L + l] : TS{};
     }
     cd hv[Sh::R1][N2];
     if (fock) {
@@ -900,10 +903,13 @@ k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
         const int t = lane + 64 * r;
         if (t < L * N2) {
             const int l = t & (L - 1), k2 = t >> 3;
-            Dft<N1, true>::run(v[r]);
+            cd w[N1];
+#pragma unroll
+            for (int k1p = 0; k1p < N1; ++k1p) w[k1p] = to_cd(v[r][k1p]);
+            Dft<N1, true>::run(w);
 #pragma unroll
             for (int i1 = 0; i1 < N1; ++i1) {
-                cd x = v[r][i1];
+                cd x = w[i1];
                 if (i1 > 0 && k2 > 0) x = cmulc(x, s_tw[i1 * k2]);
                 Y[(i1 * N2 + k2) * L + l] = x;
             }

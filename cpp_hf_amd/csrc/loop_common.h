@@ -31,17 +31,22 @@ __device__ __forceinline__ cd cayley_c(double lam, double tau) {
     return cd{(1.0 - a * a) / den, (a + a) / den};
 }
 
-// C = op(A) op(B) on nb x nb LDS blocks by one wavefront.
+// C = op(A) op(B) on nb x nb LDS blocks by one wavefront.  With tb the
+// lanes of a row start the sum at n = j (the terms in rotated order), so
+// their reads of B's rows B[j][n] fall on distinct LDS banks instead of a
+// stride of nb complex (all one bank group at nb = 16).
 __device__ __forceinline__ void wave_mm(const cd* A, const cd* B, cd* C, int nb, bool ta,
                                         bool tb, int lane) {
     const int nb2 = nb * nb;
     for (int e = lane; e < nb2; e += 64) {
         const int i = e / nb, j = e % nb;
         cd s{0.0, 0.0};
-        for (int n = 0; n < nb; ++n) {
+        int n = tb ? j : 0;
+        for (int t = 0; t < nb; ++t) {
             const cd a = ta ? cconj(A[n * nb + i]) : A[i * nb + n];
             const cd b = tb ? cconj(B[j * nb + n]) : B[n * nb + j];
             s = cadd(s, cmul(a, b));
+            n = n + 1 == nb ? 0 : n + 1;
         }
         C[e] = s;
     }
