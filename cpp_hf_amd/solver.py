@@ -68,9 +68,10 @@ class SolveResult(NamedTuple):
 
 
 def _kernel_args_for_native(kernel) -> dict:
-    """The ``kernel_args`` dict ABI of solver.py:79-103."""
+    """The ``kernel_args`` dict ABI of solver.py:79-134 (with the superlattice
+    entries of a SuperlatticeHartreeFockKernel)."""
     refP = kernel._refP if kernel._refP is not None else kernel._empty_refP
-    return dict(
+    args = dict(
         h=np.ascontiguousarray(kernel.h, dtype=np.complex128),
         VR=np.ascontiguousarray(kernel._VR_shifted, dtype=np.complex128),
         refP=np.ascontiguousarray(refP, dtype=np.complex128),
@@ -86,6 +87,12 @@ def _kernel_args_for_native(kernel) -> dict:
         include_exchange=bool(kernel.include_exchange),
         exchange_hcp=bool(kernel.exchange_hermitian_channel_packing),
     )
+    if getattr(kernel, "superlattice_fock_active", False) \
+            or getattr(kernel, "superlattice_hartree_active", False):
+        from .superlattice import superlattice_kernel_args
+
+        args.update(superlattice_kernel_args(kernel))
+    return args
 
 
 _OPTIMIZER_CODES = {"cg": 0, "newton": 1}

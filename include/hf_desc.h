@@ -12,6 +12,7 @@
 #define HF_DESC_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -24,6 +25,37 @@ enum hf_status {
     HF_ERUNTIME = 2, /* runtime failure: maps to Python RuntimeError */
     HF_EDEVICE = 3   /* HIP error: maps to Python RuntimeError       */
 };
+
+/*
+ * Superlattice mode of a kernel (reference: the superlattice fields of
+ * HFKernel, kernel.hpp:36-69, filled by inject_superlattice_into_kernel,
+ * cpp_hf_native.cpp:104-165).  With fock_active the exchange is the
+ * extended-grid convolution of selfenergy_superlattice_streamed
+ * (superlattice.hpp:53-246); with hartree_active the Hartree term is the
+ * k-independent G-basis shift of fock.hpp:66-173, added to Sigma.
+ *   V_lag_fft         : (N_ext_x, N_ext_y) complex
+ *   V_lag_fft_orbital : (N_ext_x, N_ext_y, dim_orb, dim_orb) complex or NULL
+ *   g_a_off           : (n_G, 2); pair_i, pair_j: (n_pairs); pair_start:
+ *                       (n_delta + 1); pair_to_delta: (n_G, n_G)
+ *   HH_GG             : (n_G, n_G) real; HH_GG_orbital: (n_G, n_G, dim_orb,
+ *                       dim_orb) real or NULL
+ * n_G * dim_orb must equal the kernel's nb.
+ */
+typedef struct hf_superlattice_desc {
+    int fock_active;
+    int hartree_active;
+    double hartree_degeneracy;
+    size_t n_G, dim_orb, n_delta, N_ext_x, N_ext_y, n_pairs;
+    const double* V_lag_fft;
+    const double* V_lag_fft_orbital;
+    const int64_t* g_a_off;
+    const int64_t* pair_i;
+    const int64_t* pair_j;
+    const int64_t* pair_start;
+    const int64_t* pair_to_delta;
+    const double* HH_GG;
+    const double* HH_GG_orbital;
+} hf_superlattice_desc;
 
 /*
  * Hartree-Fock kernel description (reference: HFKernel, kernel.hpp:10-34,
@@ -51,6 +83,7 @@ typedef struct hf_kernel_desc {
     const double* contact_g;
     const double* contact_Oi;
     const double* contact_Oj;
+    const hf_superlattice_desc* superlattice; /* NULL: the mesh exchange */
 } hf_kernel_desc;
 
 /* Reference: SCFConfig, solver_scf.hpp:20-50. acceleration: 0 linear, 1 diis, 2 oda. */

@@ -20,6 +20,7 @@
 #include "hfcpu.h"
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -551,6 +552,7 @@ struct Kernel {
     const c64 *h, *VR, *refP;
     const double *w2d, *HH, *contact_g;
     const c64 *contact_Oi, *contact_Oj;
+    const hf_superlattice_desc* sl = nullptr;  // superlattice mode (kernel.hpp:36-69)
     size_t nk() const { return nk1 * nk2; }
     size_t nb2() const { return nb * nb; }
     size_t n_dense() const { return nk() * nb2(); }
@@ -577,7 +579,137 @@ Kernel make_kernel(const hf_kernel_desc* d) {
     K.w2d = d->w2d; K.HH = d->HH; K.contact_g = d->contact_g;
     K.contact_Oi = reinterpret_cast<const c64*>(d->contact_Oi);
     K.contact_Oj = reinterpret_cast<const c64*>(d->contact_Oj);
+    const hf_superlattice_desc* sl = d->superlattice;
+    if (sl && (sl->fock_active || sl->hartree_active)) {
+        // inject_superlattice_into_kernel (cpp_hf_native.cpp:109-165)
+        if (sl->n_G * sl->dim_orb != d->nb)
+            throw std::invalid_argument("superlattice mode requires n_G × dim_orb == nb");
+        if (!sl->V_lag_fft || !sl->g_a_off || !sl->pair_i || !sl->pair_j || !sl->pair_start ||
+            !sl->pair_to_delta || !sl->HH_GG)
+            throw std::invalid_argument("superlattice description has a null array");
+        K.sl = sl;
+    }
     return K;
+}
+
+// superlattice.hpp:53-246: the (alpha, beta, dG) channels of the
+// superlattice exchange, each scattered onto the extended grid, convolved
+// with the lag kernel and gathered back; with hermitian, one channel of
+// each (+dG, -dG) partner pair is computed and the other filled by
+// Sigma[k, j b, i a] = conj(Sigma[k, i a, j b]).
+void selfenergy_superlattice(const c64* rho, c64* sigma, const c64* V, const c64* V_orb,
+                             const int64_t* g_a_off, const int64_t* pair_i,
+                             const int64_t* pair_j, const int64_t* pair_start, size_t n_delta,
+                             size_t Nx, size_t Ny, size_t nkx, size_t nky, size_t n_G,
+                             size_t orb, bool hermitian) {
+    const size_t orb2 = orb * orb, chan = Nx * Ny * orb2;
+    const size_t D = n_G * orb, row_rho = nky * D * D, row_dg = Ny * orb2;
+    std::fill(sigma, sigma + nkx * nky * D * D, c64(0.0, 0.0));
+    std::vector<int64_t> partner(n_delta, -1);
+    if (hermitian) {
+        // channel whose (i, j) pair set is this channel's reversed
+        std::map<std::vector<int64_t>, size_t> by_key;
+        std::vector<std::vector<int64_t>> reversed(n_delta);
+        for (size_t d = 0; d < n_delta; ++d) {
+            std::vector<int64_t> key, rkey;
+            for (int64_t p = pair_start[d]; p < pair_start[d + 1]; ++p) {
+                key.push_back(pair_i[p] * static_cast<int64_t>(n_G) + pair_j[p]);
+                rkey.push_back(pair_j[p] * static_cast<int64_t>(n_G) + pair_i[p]);
+            }
+            std::sort(key.begin(), key.end());
+            std::sort(rkey.begin(), rkey.end());
+            by_key.emplace(std::move(key), d);
+            reversed[d] = std::move(rkey);
+        }
+        for (size_t d = 0; d < n_delta; ++d) {
+            auto it = by_key.find(reversed[d]);
+            if (it != by_key.end()) partner[d] = static_cast<int64_t>(it->second);
+        }
+    }
+    for (size_t d = 0; d < n_delta; ++d) {
+        const int64_t pr = partner[d];
+        const bool fill = pr >= 0 && static_cast<size_t>(pr) != d;
+        if (fill && static_cast<size_t>(pr) < d) continue;  // filled by its partner
+        const int64_t a = pair_start[d], b = pair_start[d + 1];
+        if (a == b) continue;
+        std::vector<c64> g(chan, c64(0.0, 0.0));
+        for (int64_t p = a; p < b; ++p) {
+            const size_t i = pair_i[p], j = pair_j[p];
+            const size_t ox = nkx * g_a_off[2 * i], oy = nky * g_a_off[2 * i + 1];
+            for (size_t kx = 0; kx < nkx; ++kx)
+                for (size_t ky = 0; ky < nky; ++ky)
+                    for (size_t al = 0; al < orb; ++al)
+                        for (size_t be = 0; be < orb; ++be)
+                            g[(ox + kx) * row_dg + (oy + ky) * orb2 + al * orb + be] =
+                                rho[kx * row_rho + (ky * D + i * orb + al) * D + j * orb + be];
+        }
+        bool nonzero = false;
+        for (size_t t = 0; t < chan && !nonzero; ++t) nonzero = g[t] != c64(0.0, 0.0);
+        if (!nonzero) continue;  // zero in, zero out
+        if (V_orb)
+            selfenergy_fft_full_inplace(g.data(), V_orb, Nx, Ny, orb, orb, orb);
+        else
+            selfenergy_fft_full_inplace(g.data(), V, Nx, Ny, orb, 1, 1);
+        for (int64_t p = a; p < b; ++p) {
+            const size_t i = pair_i[p], j = pair_j[p];
+            const size_t ox = nkx * g_a_off[2 * i], oy = nky * g_a_off[2 * i + 1];
+            for (size_t kx = 0; kx < nkx; ++kx)
+                for (size_t ky = 0; ky < nky; ++ky)
+                    for (size_t al = 0; al < orb; ++al)
+                        for (size_t be = 0; be < orb; ++be) {
+                            const c64 v = g[(ox + kx) * row_dg + (oy + ky) * orb2 + al * orb + be];
+                            sigma[kx * row_rho + (ky * D + i * orb + al) * D + j * orb + be] = v;
+                            if (fill)
+                                sigma[kx * row_rho + (ky * D + j * orb + be) * D + i * orb + al] =
+                                    std::conj(v);
+                        }
+        }
+    }
+}
+
+// fock.hpp:66-173: the k-independent superlattice Hartree shift from
+// rho_bar = sum_k w_k (P - refP)_k
+std::vector<c64> superlattice_hartree(const Kernel& K, const c64* dP) {
+    const hf_superlattice_desc& L = *K.sl;
+    const size_t nb = K.nb, nb2 = nb * nb, nG = L.n_G, orb = L.dim_orb;
+    std::vector<c64> rho_bar(nb2, c64(0.0, 0.0));
+    for (size_t k = 0; k < K.nk(); ++k)
+        for (size_t i = 0; i < nb2; ++i) rho_bar[i] += c64(K.w2d[k]) * dP[k * nb2 + i];
+    std::vector<c64> out(nb2, c64(0.0, 0.0));
+    const double deg = L.hartree_degeneracy;
+    if (!L.HH_GG_orbital) {
+        std::vector<c64> per_delta(L.n_delta, c64(0.0, 0.0));
+        for (size_t A = 0; A < nG; ++A)
+            for (size_t B = 0; B < nG; ++B) {
+                c64 s(0.0, 0.0);
+                for (size_t x = 0; x < orb; ++x) s += rho_bar[(A * orb + x) * nb + B * orb + x];
+                per_delta[L.pair_to_delta[A * nG + B]] += s;
+            }
+        for (size_t A = 0; A < nG; ++A)
+            for (size_t B = 0; B < nG; ++B) {
+                const c64 s = c64(deg * L.HH_GG[A * nG + B]) * per_delta[L.pair_to_delta[A * nG + B]];
+                for (size_t x = 0; x < orb; ++x) out[(A * orb + x) * nb + B * orb + x] = s;
+            }
+    } else {
+        std::vector<c64> per_delta(L.n_delta * orb, c64(0.0, 0.0));
+        for (size_t A = 0; A < nG; ++A)
+            for (size_t B = 0; B < nG; ++B)
+                for (size_t g = 0; g < orb; ++g)
+                    per_delta[L.pair_to_delta[A * nG + B] * orb + g] +=
+                        rho_bar[(A * orb + g) * nb + B * orb + g];
+        for (size_t A = 0; A < nG; ++A)
+            for (size_t B = 0; B < nG; ++B) {
+                const size_t d = L.pair_to_delta[A * nG + B];
+                for (size_t a = 0; a < orb; ++a) {
+                    c64 s(0.0, 0.0);
+                    for (size_t g = 0; g < orb; ++g)
+                        s += c64(deg * L.HH_GG_orbital[((A * nG + B) * orb + a) * orb + g]) *
+                             per_delta[d * orb + g];
+                    out[(A * orb + a) * nb + B * orb + a] = s;
+                }
+            }
+    }
+    return out;
 }
 
 struct Projector {
@@ -597,11 +729,14 @@ bool has_contact_terms(const Kernel& K) {
     return K.n_contact > 1;
 }
 
-// fock.hpp:34-299 (superlattice branches omitted: out of scope)
+// fock.hpp:34-299
 void build_fock_compact(const Kernel& K, const c64* P, c64* Sigma, c64* F, double* hdiag,
                         const Projector* proj) {
     const size_t nk = K.nk(), nb = K.nb, nb2 = K.nb2(), n_tot = K.n_dense();
-    if (K.include_exchange) {
+    const bool sl_fock = K.sl && K.sl->fock_active;
+    const bool sl_hartree = K.sl && K.sl->hartree_active;
+    std::vector<c64> sl_shift;
+    if (K.include_exchange || sl_hartree) {
         if (K.has_refP) {
             parallel_for(nk, [&](size_t k) {
                 for (size_t i = 0; i < nb2; ++i)
@@ -610,16 +745,31 @@ void build_fock_compact(const Kernel& K, const c64* P, c64* Sigma, c64* F, doubl
         } else {
             std::memcpy(Sigma, P, n_tot * sizeof(c64));
         }
-        if (K.exchange_hcp && K.dv1 == 1 && K.dv2 == 1)
+        if (K.include_hartree && sl_hartree) sl_shift = superlattice_hartree(K, Sigma);
+    }
+    if (K.include_exchange) {
+        if (sl_fock) {
+            const hf_superlattice_desc& L = *K.sl;
+            std::vector<c64> out(n_tot);
+            selfenergy_superlattice(Sigma, out.data(), reinterpret_cast<const c64*>(L.V_lag_fft),
+                                    reinterpret_cast<const c64*>(L.V_lag_fft_orbital), L.g_a_off,
+                                    L.pair_i, L.pair_j, L.pair_start, L.n_delta, L.N_ext_x,
+                                    L.N_ext_y, K.nk1, K.nk2, L.n_G, L.dim_orb, true);
+            std::memcpy(Sigma, out.data(), n_tot * sizeof(c64));
+        } else if (K.exchange_hcp && K.dv1 == 1 && K.dv2 == 1)
             selfenergy_fft_full_hcp(Sigma, Sigma, K.VR, K.nk1, K.nk2, nb);
         else
             selfenergy_fft_full_inplace(Sigma, K.VR, K.nk1, K.nk2, nb, K.dv1, K.dv2);
     } else {
         std::fill(Sigma, Sigma + n_tot, c64(0.0, 0.0));
     }
+    if (!sl_shift.empty())
+        parallel_for(nk, [&](size_t k) {
+            for (size_t i = 0; i < nb2; ++i) Sigma[k * nb2 + i] += sl_shift[i];
+        });
 
     std::fill(hdiag, hdiag + nb, 0.0);
-    if (K.include_hartree) {
+    if (K.include_hartree && !sl_hartree) {
         std::vector<double> n_vec(nb, 0.0);
         for (size_t k = 0; k < nk; ++k) {
             const c64* pk = P + k * nb2;
@@ -1945,6 +2095,18 @@ int hfcpu_solve_dm(const hf_kernel_desc* kd, const double* P0, double n_e,
             solve_rtr(K, C(P0), n_e, *cfg, pfn ? &pr : nullptr, out);
         else
             solve_dm(K, C(P0), n_e, *cfg, pfn ? &pr : nullptr, out);
+    });
+}
+
+int hfcpu_selfenergy_superlattice(const double* rho, double* sigma, const double* V,
+                                  const double* V_orb, const int64_t* g_a_off,
+                                  const int64_t* pair_i, const int64_t* pair_j,
+                                  const int64_t* pair_start, size_t n_delta, size_t Nx, size_t Ny,
+                                  size_t nkx, size_t nky, size_t n_G, size_t orb, int hermitian) {
+    return guarded([&] {
+        selfenergy_superlattice(C(rho), C(sigma), C(V), V_orb ? C(V_orb) : nullptr, g_a_off,
+                                pair_i, pair_j, pair_start, n_delta, Nx, Ny, nkx, nky, n_G, orb,
+                                hermitian != 0);
     });
 }
 

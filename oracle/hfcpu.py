@@ -25,6 +25,21 @@ _dp = C.POINTER(C.c_double)
 _sz = C.c_size_t
 
 
+_i64p = C.POINTER(C.c_int64)
+
+
+class SlDesc(C.Structure):
+    _fields_ = [
+        ("fock_active", C.c_int), ("hartree_active", C.c_int),
+        ("hartree_degeneracy", C.c_double),
+        ("n_G", _sz), ("dim_orb", _sz), ("n_delta", _sz), ("N_ext_x", _sz), ("N_ext_y", _sz),
+        ("n_pairs", _sz),
+        ("V_lag_fft", _dp), ("V_lag_fft_orbital", _dp), ("g_a_off", _i64p),
+        ("pair_i", _i64p), ("pair_j", _i64p), ("pair_start", _i64p), ("pair_to_delta", _i64p),
+        ("HH_GG", _dp), ("HH_GG_orbital", _dp),
+    ]
+
+
 class KernelDesc(C.Structure):
     _fields_ = [
         ("nk1", _sz), ("nk2", _sz), ("nb", _sz), ("dv1", _sz), ("dv2", _sz),
@@ -34,6 +49,7 @@ class KernelDesc(C.Structure):
         ("T", C.c_double), ("weight_sum", C.c_double),
         ("h", _dp), ("VR", _dp), ("refP", _dp), ("w2d", _dp), ("HH", _dp),
         ("contact_g", _dp), ("contact_Oi", _dp), ("contact_Oj", _dp),
+        ("superlattice", C.POINTER(SlDesc)),
     ]
 
 
@@ -251,6 +267,30 @@ class _Desc:
             contact_Oi=_p(self.Oi), contact_Oj=_p(self.Oj),
         )
         self.shape = h.shape
+        if ka.get("superlattice_fock_active") or ka.get("superlattice_hartree_active"):
+            # the superlattice entries of the kernel_args dict (solver.py:104-134)
+            i64 = lambda a: np.ascontiguousarray(a, dtype=np.int64)
+            self.sl_arrays = dict(
+                V=_c128(ka["V_lag_fft"]),
+                Vo=_c128(ka["V_lag_fft_orbital"]) if "V_lag_fft_orbital" in ka else None,
+                off=i64(ka["g_a_off"]), pi=i64(ka["pair_i"]), pj=i64(ka["pair_j"]),
+                ps=i64(ka["pair_start"]), p2d=i64(ka["pair_to_delta"]),
+                HH=_f64(ka["HH_GG"]),
+                HHo=_f64(ka["HH_GG_orbital"]) if "HH_GG_orbital" in ka else None)
+            a = self.sl_arrays
+            ip = lambda x: x.ctypes.data_as(_i64p)
+            self.sl = SlDesc(
+                fock_active=int(bool(ka["superlattice_fock_active"])),
+                hartree_active=int(bool(ka["superlattice_hartree_active"])),
+                hartree_degeneracy=float(ka.get("hartree_degeneracy", 1.0)),
+                n_G=int(ka["n_G"]), dim_orb=int(ka["dim_orb"]), n_delta=int(ka["n_delta"]),
+                N_ext_x=int(ka["N_ext_x"]), N_ext_y=int(ka["N_ext_y"]),
+                n_pairs=int(a["pi"].shape[0]),
+                V_lag_fft=_p(a["V"]), V_lag_fft_orbital=_p(a["Vo"]) if a["Vo"] is not None else None,
+                g_a_off=ip(a["off"]), pair_i=ip(a["pi"]), pair_j=ip(a["pj"]),
+                pair_start=ip(a["ps"]), pair_to_delta=ip(a["p2d"]), HH_GG=_p(a["HH"]),
+                HH_GG_orbital=_p(a["HHo"]) if a["HHo"] is not None else None)
+            self.desc.superlattice = C.pointer(self.sl)
 
 
 def _wrap_project(project_fn):
@@ -371,6 +411,25 @@ def solve_dm(kernel_args, P0, n_e, max_iter, tol_E, tol_grad, max_step, bt_shrin
     return (Q if return_Q else None, p, dens if return_density else None,
             fock if return_fock else None, res.mu, res.energy, n, bool(res.converged),
             hE[:n].copy(), hG[:n].copy())
+
+
+def selfenergy_superlattice_streamed(rho, VR_fft, g_a_off, pair_i, pair_j, pair_start, nkx,
+                                     nky, n_G, dim_orb, N_ext_x, N_ext_y, VR_fft_orbital=None,
+                                     hermitian_rho=False):
+    """The superlattice exchange primitive (superlattice.hpp:53-246) with the
+    reference module's keyword signature (cpp_hf_native.cpp:718-790)."""
+    rho = _c128(rho)
+    V = _c128(VR_fft)
+    Vo = _c128(VR_fft_orbital) if VR_fft_orbital is not None else None
+    i64 = lambda a: np.ascontiguousarray(a, dtype=np.int64)
+    off, pi, pj, ps = i64(g_a_off), i64(pair_i), i64(pair_j), i64(pair_start)
+    out = np.empty_like(rho)
+    ip = lambda x: x.ctypes.data_as(_i64p)
+    _check(lib().hfcpu_selfenergy_superlattice(
+        _p(rho), _p(out), _p(V), _p(Vo) if Vo is not None else None, ip(off), ip(pi), ip(pj),
+        ip(ps), _sz(ps.shape[0] - 1), _sz(N_ext_x), _sz(N_ext_y), _sz(nkx), _sz(nky), _sz(n_G),
+        _sz(dim_orb), C.c_int(int(bool(hermitian_rho)))))
+    return out
 
 
 def _rtr_joint_hvp(kernel_args, X, dp, Q, p, Ft):
