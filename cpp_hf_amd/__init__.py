@@ -22,6 +22,8 @@ from .continuation import ContinuationResult, solve_continuation
 from .symmetry import make_project_fn
 from .deflation import DeflatedResult, solve_deflated
 from .adjoint import build_fock_vjp, energy_gradients, free_energy_derivative
+from .superlattice import (ExtendedGridLayout, SuperlatticeHartreeFockKernel,
+                           build_extended_layout, superlattice_fock)
 
 __all__ = [
     "HartreeFockKernel",
@@ -46,4 +48,8 @@ __all__ = [
     "build_fock_vjp",
     "energy_gradients",
     "free_energy_derivative",
+    "ExtendedGridLayout",
+    "build_extended_layout",
+    "superlattice_fock",
+    "SuperlatticeHartreeFockKernel",
 ]

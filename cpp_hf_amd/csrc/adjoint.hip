@@ -85,6 +85,8 @@ ExchangeGeom& adjoint_geometry(std::unique_ptr<ExchangeGeom>& slot, int nk1, int
 
 void fock_vjp(GpuKernel& K, const cd* P, const cd* Gs, const cd* Gh, const cd* GF, cd* gP,
               cd* gVq, cd* gh, double* gHH, hipStream_t st) {
+    if (K.superlattice())
+        throw std::invalid_argument("the adjoint of the superlattice Fock build is not provided");
     const int nk = static_cast<int>(K.nk), nb = K.nb, nb2 = static_cast<int>(K.nb2);
     const size_t nd = K.nd, bytes = nd * sizeof(cd);
     const double* w = K.w2d.as<double>();

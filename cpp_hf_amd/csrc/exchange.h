@@ -87,6 +87,7 @@ struct SmallTerms {
     const cd* Oj = nullptr;
     double* hdiag = nullptr;  // out (nb)
     cd* SC = nullptr;         // out (nb, nb)
+    cd* rho_out = nullptr;    // out (nb, nb): rho_bar itself, nullable
 };
 
 struct ExchangeArgs {

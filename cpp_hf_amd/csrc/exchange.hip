@@ -266,6 +266,8 @@ __device__ void small_terms_block(const cd* __restrict__ partial, int nrows, int
     }
     for (int e = threadIdx.x; e < nb2; e += nt) acc[e] = cd{0.0, 0.0};
     __syncthreads();
+    if (st.rho_out)
+        for (int e = threadIdx.x; e < nb2; e += nt) st.rho_out[e] = rho[e];
     if (st.include_hartree) {
         for (int a = threadIdx.x; a < nb; a += nt) {
             double s = 0.0;

@@ -12,6 +12,11 @@
 #include "kernels.h"
 
 namespace hfgpu {
+struct SlPlan;
+struct SlHartree;
+}  // namespace hfgpu
+
+namespace hfgpu {
 
 // Owning device allocation.
 class DevBuf {
@@ -95,6 +100,15 @@ struct GpuKernel {
     // Full-channel geometries of the adjoint's plain transforms (nb^2
     // channels, and one channel for a scalar interaction), made on first use.
     std::unique_ptr<ExchangeGeom> adj_geo, adj_geo_v;
+
+    // Superlattice mode (kernel.hpp:36-69): the exchange on the extended
+    // momentum grid in place of the mesh exchange, and the superlattice
+    // Hartree shift in place of the mesh Hartree diagonal (hartree then
+    // stands for the mesh Hartree only).
+    std::unique_ptr<SlPlan> sl_plan;       // superlattice exchange, nullable
+    std::unique_ptr<SlHartree> sl_hartree;  // superlattice Hartree shift, nullable
+    DevBuf rho_bar;                          // (nb, nb) sum_k w_k (P - refP)_k
+    bool superlattice() const { return sl_plan || sl_hartree; }
 
     explicit GpuKernel(const hf_kernel_desc& d);
     ~GpuKernel();

@@ -14,6 +14,7 @@
 #include "../../include/hfgpu.h"
 #include "hfgpu_core.h"
 #include "solver_kernels.h"
+#include "superlattice.h"
 
 namespace hfgpu {
 
@@ -269,6 +270,58 @@ GpuKernel::GpuKernel(const hf_kernel_desc& d) {
     fill_zero(SC.get(), nb2 * cz, stream);
     fill_zero(hdiag.get(), nb * sizeof(double), stream);
     HFGPU_CHECK(hipStreamSynchronize(stream));
+
+    const hf_superlattice_desc* L = d.superlattice;
+    if (L && (L->fock_active || L->hartree_active)) {
+        // inject_superlattice_into_kernel's checks (cpp_hf_native.cpp:109-165)
+        if (L->n_G * L->dim_orb != d.nb)
+            throw std::invalid_argument("superlattice mode requires n_G × dim_orb == nb");
+        if (!L->V_lag_fft || !L->g_a_off || !L->pair_i || !L->pair_j || !L->pair_start ||
+            !L->pair_to_delta || !L->HH_GG)
+            throw std::invalid_argument("superlattice description has a null array");
+        // fock.hpp:176-199: with the exchange on, superlattice mode replaces
+        // the mesh exchange; the mesh exchange beside a superlattice Hartree
+        // shift is not a combination the superlattice kernel makes
+        if (exchange && !L->fock_active)
+            throw std::invalid_argument(
+                "the GPU superlattice kernel needs superlattice_fock_active when the exchange is on");
+        try {
+            if (exchange) {
+                SlInput in;
+                in.V = reinterpret_cast<const cd*>(L->V_lag_fft);
+                in.V_orb = reinterpret_cast<const cd*>(L->V_lag_fft_orbital);
+                in.g_a_off = L->g_a_off;
+                in.pair_i = L->pair_i;
+                in.pair_j = L->pair_j;
+                in.pair_start = L->pair_start;
+                in.n_pairs = L->n_pairs;
+                in.n_delta = L->n_delta;
+                in.Nx = L->N_ext_x;
+                in.Ny = L->N_ext_y;
+                in.nkx = d.nk1;
+                in.nky = d.nk2;
+                in.n_G = L->n_G;
+                in.orb = L->dim_orb;
+                // fock.hpp:180-188: P - refP is Hermitian, one channel of
+                // each (+dG, -dG) pair is transformed
+                in.hermitian = true;
+                sl_plan = std::make_unique<SlPlan>();
+                sl_plan->init(in, stream);
+            }
+            if (L->hartree_active && hartree) {
+                sl_hartree = std::make_unique<SlHartree>();
+                sl_hartree->init(*L, stream);
+                rho_bar.alloc(nb2 * cz);
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+            stream = nullptr;
+            throw;
+        }
+        // fock.hpp:222: the mesh Hartree diagonal only without the shift
+        if (L->hartree_active) hartree = false;
+    }
 }
 
 GpuKernel::~GpuKernel() {
@@ -305,19 +358,21 @@ GpuKernel::Geom& GpuKernel::geom(bool packed) {
 }
 
 bool GpuKernel::fp32_capable(bool p_hermitian) {
-    if (!exchange) return false;
+    if (!exchange || sl_plan) return false;
     const ExchangeGeom& g = *geom(packed_for(p_hermitian)).g;
     return g.fast_rows && g.fast_cols;
 }
 
 void GpuKernel::build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, bool p_hermitian,
                            hipStream_t st, bool use_refP, bool fp32_ws) {
-    const bool need_partials = hartree || contact;
-    const bool packed = exchange && packed_for(p_hermitian);
+    const bool sl_x = exchange && sl_plan;  // superlattice exchange
+    const bool need_partials = hartree || contact || sl_hartree;
+    const bool mesh_x = exchange && !sl_x;
+    const bool packed = mesh_x && packed_for(p_hermitian);
     Geom& G = geom(packed);
     const ExchangeGeom& g = *G.g;
     ExchangeArgs a;
-    if (fp32_ws && exchange && g.fast_rows && g.fast_cols) {
+    if (fp32_ws && mesh_x && g.fast_rows && g.fast_cols) {
         // FP32 copies of the workspace and the interaction, made once
         if (T32_ws.size() < g.tile_elems() * sizeof(cf)) T32_ws.alloc(g.tile_elems() * sizeof(cf));
         if (!G.VR32.get()) {
@@ -362,15 +417,24 @@ void GpuKernel::build_fock(const cd* P, cd* Sigma, cd* F, double* hdiag_out, boo
         s.Oj = Oj.as<cd>();
         s.hdiag = hd;
         s.SC = SC.as<cd>();
+        s.rho_out = sl_hartree ? rho_bar.as<cd>() : nullptr;
     }
-    if (exchange)
+    if (mesh_x)
         launch_exchange_fwd(g, a, st);
     else if (need_partials)
         launch_row_sums(g, a, st);
     if (!need_partials) fill_zero(hd, nb * sizeof(double), st);
-    const cd* sc = contact ? SC.as<cd>() : nullptr;
+    // fock.hpp:66-175, 200-212: the superlattice Hartree shift joins the
+    // contact self-energy added to Sigma
+    if (sl_hartree) sl_hartree->launch(rho_bar.as<cd>(), SC.as<cd>(), contact, st);
+    const cd* sc = contact || sl_hartree ? SC.as<cd>() : nullptr;
     const double* hdp = hartree ? hd : nullptr;
-    if (exchange) {
+    if (sl_x) {
+        const bool ref = has_refP && use_refP;
+        sl_plan->apply(P, ref && !refP_const ? refP.as<cd>() : nullptr,
+                       ref && refP_const ? refP0.as<cd>() : nullptr, Sigma, st);
+        launch_fock_assemble(static_cast<int>(nk), nb, h.as<cd>(), Sigma, F, sc, hdp, st);
+    } else if (exchange) {
         a.Sigma = Sigma;
         a.ifftshift = false;
         if (packed) {

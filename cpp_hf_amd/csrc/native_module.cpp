@@ -21,6 +21,7 @@ namespace py = pybind11;
 using c128 = std::complex<double>;
 using CArr = py::array_t<c128, py::array::c_style | py::array::forcecast>;
 using RArr = py::array_t<double, py::array::c_style | py::array::forcecast>;
+using IArr = py::array_t<std::int64_t, py::array::c_style | py::array::forcecast>;
 
 namespace {
 
@@ -45,6 +46,79 @@ struct KernelArgs {
     CArr h, VR, refP, Oi, Oj;
     RArr w2d, HH, cg;
     hf_kernel_desc d{};
+    // superlattice entries (solver.py:104-134), read when a superlattice
+    // flag is set (inject_superlattice_into_kernel, cpp_hf_native.cpp:105-165)
+    CArr sl_V, sl_Vo;
+    IArr sl_off, sl_pi, sl_pj, sl_ps, sl_p2d;
+    RArr sl_HH, sl_HHo;
+    hf_superlattice_desc sl{};
+
+    void read_superlattice(py::object ka) {
+        auto flag = [&](const char* key) {
+            return ka.contains(key) ? py::cast<bool>(ka[key]) : false;
+        };
+        const bool fock = flag("superlattice_fock_active");
+        const bool hart = flag("superlattice_hartree_active");
+        if (!fock && !hart) return;
+        sl.fock_active = fock;
+        sl.hartree_active = hart;
+        sl.hartree_degeneracy =
+            ka.contains("hartree_degeneracy") ? py::cast<double>(ka["hartree_degeneracy"]) : 1.0;
+        sl.n_G = py::cast<size_t>(ka["n_G"]);
+        sl.dim_orb = py::cast<size_t>(ka["dim_orb"]);
+        sl.n_delta = py::cast<size_t>(ka["n_delta"]);
+        sl.N_ext_x = py::cast<size_t>(ka["N_ext_x"]);
+        sl.N_ext_y = py::cast<size_t>(ka["N_ext_y"]);
+        if (sl.n_G * sl.dim_orb != d.nb)
+            throw py::value_error("superlattice mode requires n_G × dim_orb == nb");
+        sl_V = py::cast<CArr>(ka["V_lag_fft"]);
+        sl_off = py::cast<IArr>(ka["g_a_off"]);
+        sl_pi = py::cast<IArr>(ka["pair_i"]);
+        sl_pj = py::cast<IArr>(ka["pair_j"]);
+        sl_ps = py::cast<IArr>(ka["pair_start"]);
+        sl_p2d = py::cast<IArr>(ka["pair_to_delta"]);
+        sl_HH = py::cast<RArr>(ka["HH_GG"]);
+        if (sl_V.ndim() != 2 || (size_t)sl_V.shape(0) != sl.N_ext_x ||
+            (size_t)sl_V.shape(1) != sl.N_ext_y)
+            throw py::value_error("V_lag_fft must have shape (N_ext_x, N_ext_y)");
+        if (sl_off.ndim() != 2 || (size_t)sl_off.shape(0) != sl.n_G || sl_off.shape(1) != 2)
+            throw py::value_error("g_a_off must be (n_G, 2)");
+        if (sl_ps.ndim() != 1 || (size_t)sl_ps.shape(0) != sl.n_delta + 1)
+            throw py::value_error("pair_start must have length n_delta + 1");
+        if (sl_pi.ndim() != 1 || sl_pj.ndim() != 1 || sl_pi.shape(0) != sl_pj.shape(0))
+            throw py::value_error("pair_i and pair_j must have the same length");
+        if (sl_p2d.ndim() != 2 || (size_t)sl_p2d.shape(0) != sl.n_G ||
+            (size_t)sl_p2d.shape(1) != sl.n_G)
+            throw py::value_error("pair_to_delta must be (n_G, n_G)");
+        if (sl_HH.ndim() != 2 || (size_t)sl_HH.shape(0) != sl.n_G || (size_t)sl_HH.shape(1) != sl.n_G)
+            throw py::value_error("HH_GG must be (n_G, n_G)");
+        sl.n_pairs = sl_pi.shape(0);
+        sl.V_lag_fft = cdp(sl_V);
+        sl.g_a_off = sl_off.data();
+        sl.pair_i = sl_pi.data();
+        sl.pair_j = sl_pj.data();
+        sl.pair_start = sl_ps.data();
+        sl.pair_to_delta = sl_p2d.data();
+        sl.HH_GG = cdp(sl_HH);
+        if (ka.contains("HH_GG_orbital")) {
+            sl_HHo = py::cast<RArr>(ka["HH_GG_orbital"]);
+            if (sl_HHo.ndim() != 4 || (size_t)sl_HHo.shape(0) != sl.n_G ||
+                (size_t)sl_HHo.shape(1) != sl.n_G || (size_t)sl_HHo.shape(2) != sl.dim_orb ||
+                (size_t)sl_HHo.shape(3) != sl.dim_orb)
+                throw py::value_error("HH_GG_orbital must be (n_G, n_G, dim_orb, dim_orb)");
+            sl.HH_GG_orbital = cdp(sl_HHo);
+        }
+        if (ka.contains("V_lag_fft_orbital")) {
+            sl_Vo = py::cast<CArr>(ka["V_lag_fft_orbital"]);
+            if (sl_Vo.ndim() != 4 || (size_t)sl_Vo.shape(0) != sl.N_ext_x ||
+                (size_t)sl_Vo.shape(1) != sl.N_ext_y || (size_t)sl_Vo.shape(2) != sl.dim_orb ||
+                (size_t)sl_Vo.shape(3) != sl.dim_orb)
+                throw py::value_error(
+                    "V_lag_fft_orbital must be (N_ext_x, N_ext_y, dim_orb, dim_orb)");
+            sl.V_lag_fft_orbital = cdp(sl_Vo);
+        }
+        d.superlattice = &sl;
+    }
 
     explicit KernelArgs(py::object ka) {
         h = py::cast<CArr>(ka["h"]);
@@ -90,7 +164,10 @@ struct KernelArgs {
         d.contact_g = cdp(cg);
         d.contact_Oi = cdp(Oi);
         d.contact_Oj = cdp(Oj);
+        read_superlattice(ka);
     }
+    KernelArgs(const KernelArgs&) = delete;
+    KernelArgs& operator=(const KernelArgs&) = delete;
 };
 
 // RAII kernel handle.
@@ -195,6 +272,59 @@ PYBIND11_MODULE(_native, m) {
         },
         py::arg("VR"), py::arg("P"), py::arg("apply_ifftshift") = true,
         py::arg("hermitian_channel_packing") = false);
+
+    // --- superlattice exchange (cpp_hf_native.cpp:718-790) ---
+    m.def(
+        "selfenergy_superlattice_streamed",
+        [](CArr rho, CArr VR_fft, IArr g_a_off, IArr pair_i, IArr pair_j, IArr pair_start,
+           size_t nkx, size_t nky, size_t n_G, size_t dim_orb, size_t N_ext_x, size_t N_ext_y,
+           py::object VR_fft_orbital, bool hermitian_rho) {
+            if (rho.ndim() != 6)
+                throw py::value_error("rho must be (nkx, nky, n_G, dim_orb, n_G, dim_orb)");
+            if (VR_fft.ndim() != 2) throw py::value_error("VR_fft must be (N_ext_x, N_ext_y)");
+            if ((size_t)VR_fft.shape(0) != N_ext_x || (size_t)VR_fft.shape(1) != N_ext_y)
+                throw py::value_error("VR_fft shape does not match (N_ext_x, N_ext_y)");
+            if (g_a_off.ndim() != 2 || (size_t)g_a_off.shape(0) != n_G || g_a_off.shape(1) != 2)
+                throw py::value_error("g_a_off must be (n_G, 2)");
+            if (pair_start.ndim() != 1 || pair_start.shape(0) < 1)
+                throw py::value_error("pair_start must be 1-D, len >= 1");
+            if (pair_i.ndim() != 1 || pair_j.ndim() != 1 || pair_i.shape(0) != pair_j.shape(0))
+                throw py::value_error("pair_i and pair_j must be 1-D with the same length");
+            const py::ssize_t want[6] = {(py::ssize_t)nkx, (py::ssize_t)nky, (py::ssize_t)n_G,
+                                         (py::ssize_t)dim_orb, (py::ssize_t)n_G,
+                                         (py::ssize_t)dim_orb};
+            for (int a = 0; a < 6; ++a)
+                if (rho.shape(a) != want[a])
+                    throw py::value_error(
+                        "rho shape does not match (nkx, nky, n_G, dim_orb, n_G, dim_orb)");
+            CArr Vo;
+            const double* vo = nullptr;
+            if (!VR_fft_orbital.is_none()) {
+                Vo = py::cast<CArr>(VR_fft_orbital);
+                if (Vo.ndim() != 4 || (size_t)Vo.shape(0) != N_ext_x ||
+                    (size_t)Vo.shape(1) != N_ext_y || (size_t)Vo.shape(2) != dim_orb ||
+                    (size_t)Vo.shape(3) != dim_orb)
+                    throw py::value_error("VR_fft_orbital must be (N_ext_x, N_ext_y, dim_orb, dim_orb)");
+                vo = cdp(Vo);
+            }
+            auto sigma = new_c(std::vector<py::ssize_t>(want, want + 6));
+            py::array sg = sigma;
+            const size_t n_delta = pair_start.shape(0) - 1;
+            int rc;
+            {
+                py::gil_scoped_release rel;
+                rc = hfgpu_selfenergy_superlattice_streamed(
+                    cdp(rho), dp(sg), cdp(VR_fft), vo, g_a_off.data(), pair_i.data(),
+                    pair_j.data(), pair_start.data(), pair_i.shape(0), n_delta, N_ext_x, N_ext_y,
+                    nkx, nky, n_G, dim_orb, hermitian_rho ? 1 : 0);
+            }
+            check(rc);
+            return sigma;
+        },
+        py::arg("rho"), py::arg("VR_fft"), py::arg("g_a_off"), py::arg("pair_i"),
+        py::arg("pair_j"), py::arg("pair_start"), py::arg("nkx"), py::arg("nky"), py::arg("n_G"),
+        py::arg("dim_orb"), py::arg("N_ext_x"), py::arg("N_ext_y"),
+        py::arg("VR_fft_orbital") = py::none(), py::arg("hermitian_rho") = false);
 
     m.def("clear_fft_plan_cache", []() {
         py::gil_scoped_release rel;

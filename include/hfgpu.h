@@ -68,6 +68,27 @@ int hfgpu_selfenergy_fft_full(const double* VR, const double* P, double* sigma, 
                               int apply_ifftshift, int hermitian_channel_packing);
 
 /* ------------------------------------------------------------------------
+ * Superlattice exchange on the extended momentum grid.
+ * Replaces: _native.selfenergy_superlattice_streamed
+ * (cpp/cpp_hf_native.cpp:719-790) -> selfenergy_superlattice_streamed
+ * (cpp/include/cpp_hf/superlattice.hpp:50-246).
+ *   rho, sigma     : (nkx, nky, n_G, dim_orb, n_G, dim_orb) complex
+ *   VR_fft         : (N_ext_x, N_ext_y) complex lag-kernel transform
+ *   VR_fft_orbital : (N_ext_x, N_ext_y, dim_orb, dim_orb) complex or NULL
+ *   g_a_off        : (n_G, 2) tile offsets in units of (nkx, nky)
+ *   pair_i, pair_j : (n_pairs) pairs grouped by channel, pair_start
+ *                    (n_delta + 1) the CSR row pointer
+ * n_pairs is an addition of this interface: the length of pair_i / pair_j,
+ * against which pair_start is checked.  Every index and every tile is
+ * checked against the grid (HF_EINVAL otherwise).
+ * ---------------------------------------------------------------------- */
+int hfgpu_selfenergy_superlattice_streamed(
+    const double* rho, double* sigma, const double* VR_fft, const double* VR_fft_orbital,
+    const int64_t* g_a_off, const int64_t* pair_i, const int64_t* pair_j,
+    const int64_t* pair_start, size_t n_pairs, size_t n_delta, size_t N_ext_x, size_t N_ext_y,
+    size_t nkx, size_t nky, size_t n_G, size_t dim_orb, int hermitian_rho);
+
+/* ------------------------------------------------------------------------
  * Batched Hermitian eigendecomposition (hermitises its input first).
  * Replaces: _native.eigh_batched (cpp/cpp_hf_native.cpp:287-312) ->
  * eigh_batched / eigh_one (cpp/include/cpp_hf/linalg.hpp:39-66).
