@@ -23,6 +23,11 @@ The JSON line also carries:
                 192 x 192 with and without the mixed-precision phase: times,
                 |dF| per k, max |dSigma|, |dmu|, gap error.
   adjoint       time of one device VJP of the Fock build at 144 x 144, nb 8.
+  superlattice  device Fock builds of the moire continuum model in its
+                plane-wave basis (exchange on the extended grid + superlattice
+                Hartree shift) at 24 x 24 k (13 G x 2 orbitals) and 36 x 36 k
+                (19 G x 1 orbital), HIP-event timed, with the grid bytes one
+                build moves, the oracle's build on this host and an SCF solve.
 """
 
 from __future__ import annotations
@@ -166,6 +171,74 @@ def run_adjoint(torch, dev) -> dict:
             "ms_per_vjp": (time.perf_counter() - t) * 1e2}
 
 
+def run_superlattice(torch, dev, cpu: bool) -> dict:
+    """Device superlattice Fock builds (and one SCF solve) of the moire model."""
+    from cpp_hf_amd import SCFConfig, solve_scf
+    from cpp_hf_amd._compat import _native
+    from cpp_hf_amd.solver import _kernel_args_for_native
+    from cpp_hf_amd.workloads import moire_kernel
+
+    out = {}
+    for n, shells, orb in ((24, 2, 2), (36, 3, 1)):
+        kern, P0, n_e = moire_kernel(n, shells, orb)
+        L = kern.layout
+        D = kern.h.shape[-1]
+        ka = _kernel_args_for_native(kern)
+        gk = _native.GpuKernel(ka)
+        P = torch.from_numpy(np.ascontiguousarray(P0)).to(dev)
+        S = torch.empty_like(P)
+        F = torch.empty_like(P)
+        hd = torch.empty(D, dtype=torch.float64, device=dev)
+        stream = torch.cuda.Stream(device=dev)
+
+        def build():
+            gk.build_fock_device(P.data_ptr(), S.data_ptr(), F.data_ptr(), hd.data_ptr(), True,
+                                 stream.cuda_stream)
+
+        for _ in range(3):
+            build()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        steps = 20
+        e0.record(stream)
+        for _ in range(steps):
+            build()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        # one channel of each (+dG, -dG) pair is transformed (the basis is
+        # inversion symmetric, dG = 0 is its own partner)
+        channels = (L.n_delta + 1) // 2 * orb * orb
+        grid = L.N_ext_x * L.N_ext_y * channels * 16
+        # scatter write, 2 forward + 2 inverse transform passes (read +
+        # write), multiply (read + write): 11 grid sweeps; the gather reads
+        # the support only
+        sweeps = 11
+        rec = {
+            "workload": f"moire continuum {n}x{n} k, {kern.n_G} G x {orb} orbitals (D={D}), "
+                        f"extended grid {L.N_ext_x}x{L.N_ext_y}, {channels} channels",
+            "ms_per_build": ms,
+            "grid_bytes": grid,
+            "grid_sweep_GBs": sweeps * grid / (ms * 1e-3) / 1e9,
+        }
+        if cpu:
+            from oracle import hfcpu
+
+            t = time.perf_counter()
+            Sr, Hr, Fr, Er = hfcpu.build_fock_apply(ka, P0)
+            rec["cpu_oracle_ms_per_build"] = (time.perf_counter() - t) * 1e3
+            rec["cpu_threads"] = hfcpu.num_threads()
+            rec["parity_rel_err_F"] = float(np.max(np.abs(F.cpu().numpy() - Fr)) /
+                                            np.max(np.abs(Fr)))
+        t = time.perf_counter()
+        r = solve_scf(kern, P0, n_e, config=SCFConfig(acceleration="diis", max_iter=100))
+        rec["scf_diis"] = {"seconds": time.perf_counter() - t, "iterations": r.iterations,
+                           "converged": bool(r.converged), "energy": float(r.energy)}
+        out[f"moire_{n}x{n}_D{D}"] = rec
+    return out
+
+
 def run_sweep(concurrency: int) -> dict:
     """48-kernel parameter sweep (meshes 36..120, nb 2..12, seed-1 draws of
     u, interaction scale, screening length and filling): the DM solves one by
@@ -214,6 +287,8 @@ def main() -> int:
     ap.add_argument("--sweep-concurrency", type=int, default=8)
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the 192 x 192 mixed-precision and the 144 x 144 adjoint sections")
+    ap.add_argument("--no-superlattice", action="store_true",
+                    help="skip the moire superlattice Fock builds")
     ap.add_argument("--full-channels", action="store_true",
                     help="time the unpacked nb^2-channel exchange instead")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -359,6 +434,9 @@ def main() -> int:
     if not args.no_mixed:
         out["mixed_precision"] = run_mixed_precision(torch, dev)
         out["adjoint"] = run_adjoint(torch, dev)
+
+    if not args.no_superlattice:
+        out["superlattice"] = run_superlattice(torch, dev, not args.no_cpu_baseline)
 
     if not args.no_cpu_baseline:
         from oracle import hfcpu

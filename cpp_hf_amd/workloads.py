@@ -12,6 +12,12 @@ workloads:
   tanh(d_g |q|)/|q| with V(0) = 2 pi s d_g, Hartree on with reference
   density I/2 and Hartree matrix V_ll'(0).
 
+* ``moire_kernel``    continuum moire model in a plane-wave basis (G,
+  orbital) for the superlattice exchange: kinetic c |k - G|^2 per orbital,
+  a moire potential between first-shell neighbours G - G', an on-site
+  orbital hybridisation; screened V(q) = 2 pi / sqrt(q^2 + kappa^2);
+  superlattice Hartree from V(|G - G'|), reference density I/2.
+
 Meshes are k_i = 2 pi i / n - pi (index n // 2 holds k = 0).
 """
 
@@ -181,3 +187,62 @@ def sweep_workload(p: dict, *, T: float = 0.05) -> Workload:
     nl = p["nb"] // 2
     return layered_kernel(p["n"], nl, T=T, u=p["u"], scale=0.8 * p["scale"], gate=p["gate"],
                           filling=0.5 * p["nb"] + p["offset"])
+
+
+def hexagonal_g_basis(n_shells: int):
+    """Gamma plus the n_shells nearest shells of the hexagonal reciprocal
+    lattice: integer coordinates (n_G, 2) and the basis B (columns b1, b2,
+    |b| = 4 pi / sqrt(3))."""
+    b0 = 4.0 * np.pi / np.sqrt(3.0)
+    B = np.stack([np.array([np.sqrt(3.0) / 2.0, -0.5]) * b0, np.array([0.0, 1.0]) * b0], axis=1)
+    r = n_shells + 1
+    cand = np.array([(i, j) for i in range(-r, r + 1) for j in range(-r, r + 1)], dtype=np.int64)
+    norm = np.linalg.norm(cand @ B.T, axis=1)
+    shells = np.unique(np.round(norm, 6))
+    keep = norm <= shells[min(n_shells, len(shells) - 1)] + 1e-3
+    return cand[keep], B
+
+
+def moire_kernel(n: int = 24, n_shells: int = 2, dim_orb: int = 2, *, T: float = 0.05,
+                 kinetic: float = 0.02, potential: float = 0.15, hybrid: float = 0.1,
+                 kappa: float = 0.5, filling: float | None = None):
+    """Continuum moire model on an n x n mesh of the moire Brillouin zone.
+
+    Returns (SuperlatticeHartreeFockKernel, P0, n_electrons): P0 is the
+    reference density I/2 plus a small deterministic Hermitian perturbation;
+    the default filling is half of D = n_G dim_orb bands per k.
+    """
+    from .superlattice import SuperlatticeHartreeFockKernel, build_extended_layout
+
+    g, B = hexagonal_g_basis(n_shells)
+    n_G = g.shape[0]
+    D = n_G * dim_orb
+
+    def V(q):
+        return 2.0 * np.pi / np.sqrt(np.asarray(q) ** 2 + kappa ** 2)
+
+    layout = build_extended_layout(g, B, n, n, V, 1.0 / (n * n))
+    kf = np.stack(np.meshgrid(np.arange(n) / n, np.arange(n) / n, indexing="ij"), -1)
+    h = np.zeros((n, n, D, D), dtype=np.complex128)
+    for i in range(n_G):
+        kin = kinetic * np.sum(np.einsum("xy,...y->...x", B, kf - g[i]) ** 2, axis=-1)
+        for a in range(dim_orb):
+            h[:, :, i * dim_orb + a, i * dim_orb + a] = kin + 0.2 * a
+            if a + 1 < dim_orb:
+                h[:, :, i * dim_orb + a, i * dim_orb + a + 1] = hybrid
+                h[:, :, i * dim_orb + a + 1, i * dim_orb + a] = hybrid
+    b0 = np.linalg.norm(B[:, 0])
+    for i in range(n_G):
+        for j in range(n_G):
+            if abs(np.linalg.norm(B @ (g[i] - g[j])) - b0) < 1e-6:
+                for a in range(dim_orb):
+                    h[:, :, i * dim_orb + a, j * dim_orb + a] = potential
+    ref = np.broadcast_to(0.5 * np.eye(D, dtype=np.complex128), h.shape).copy()
+    kern = SuperlatticeHartreeFockKernel(np.full((n, n), 1.0 / (n * n)), h, layout,
+                                         coulomb_V=V, g_cart=(B @ g.T).T, T=T,
+                                         reference_density=ref, hartree_degeneracy=2.0)
+    x = np.cos(np.arange(D)[:, None] * 0.7 + np.arange(D)[None, :] * 1.3)
+    pert = 0.02 * (x + x.T) * np.cos(2.0 * np.pi * kf[..., 0])[..., None, None]
+    P0 = ref + pert
+    n_e = 0.5 * D if filling is None else float(filling)
+    return kern, P0, n_e
