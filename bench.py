@@ -28,6 +28,9 @@ The JSON line also carries:
                 Hartree shift) at 24 x 24 k (13 G x 2 orbitals) and 36 x 36 k
                 (19 G x 1 orbital), HIP-event timed, with the grid bytes one
                 build moves, the oracle's build on this host and an SCF solve.
+  symmetry      SCF solves at 96 x 96, nb 8 with a time-reversal projector
+                applied on the device (SymmetryProjector) and through the
+                host callback (a plain function): seconds and iterations.
 """
 
 from __future__ import annotations
@@ -239,6 +242,27 @@ def run_superlattice(torch, dev, cpu: bool) -> dict:
     return out
 
 
+def run_symmetry() -> dict:
+    """SCF at C2 with a projector: device averaging vs the host callback."""
+    from cpp_hf_amd import SCFConfig, make_project_fn, solve_scf
+    from cpp_hf_amd.workloads import config2
+
+    wl = config2(96)
+    nb = wl.kernel.h.shape[-1]
+    proj = make_project_fn(time_reversal_U=np.eye(nb))
+    out = {"workload": "layered QWZ 96x96 k, nb=8, SCF (DIIS, 40 iterations at most), "
+                       "time reversal U = I, mod convention"}
+    for name, pf in (("device", proj), ("host_callback", lambda A: proj(A))):
+        cfg = SCFConfig(acceleration="diis", max_iter=40, project_fn=pf)
+        solve_scf(wl.kernel, wl.P0, wl.n_electrons, config=SCFConfig(max_iter=2, project_fn=pf))
+        t = time.perf_counter()
+        r = solve_scf(wl.kernel, wl.P0, wl.n_electrons, config=cfg)
+        out[name] = {"seconds": time.perf_counter() - t, "iterations": int(r.iterations),
+                     "converged": bool(r.converged), "energy": float(r.energy)}
+    out["speedup"] = out["host_callback"]["seconds"] / out["device"]["seconds"]
+    return out
+
+
 def run_sweep(concurrency: int) -> dict:
     """48-kernel parameter sweep (meshes 36..120, nb 2..12, seed-1 draws of
     u, interaction scale, screening length and filling): the DM solves one by
@@ -289,6 +313,8 @@ def main() -> int:
                     help="skip the 192 x 192 mixed-precision and the 144 x 144 adjoint sections")
     ap.add_argument("--no-superlattice", action="store_true",
                     help="skip the moire superlattice Fock builds")
+    ap.add_argument("--no-symmetry", action="store_true",
+                    help="skip the SCF solves with a symmetry projector")
     ap.add_argument("--full-channels", action="store_true",
                     help="time the unpacked nb^2-channel exchange instead")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -437,6 +463,9 @@ def main() -> int:
 
     if not args.no_superlattice:
         out["superlattice"] = run_superlattice(torch, dev, not args.no_cpu_baseline)
+
+    if not args.no_symmetry:
+        out["symmetry"] = run_symmetry()
 
     if not args.no_cpu_baseline:
         from oracle import hfcpu

@@ -170,6 +170,46 @@ struct KernelArgs {
     KernelArgs& operator=(const KernelArgs&) = delete;
 };
 
+// Device symmetry projector from a project_fn that carries one
+// (cpp_hf_amd.symmetry.SymmetryProjector): its _hfgpu_symmetry() gives
+// (unitary | None, spatial | None, spatial_mod, spatial_axes, tr_U | None,
+// tr_mod, tr_axes), or None when the averaging has no device form.
+struct SymSpec {
+    CArr U, S, T;
+    hf_symmetry_desc d{};
+    bool active = false;
+
+    SymSpec(py::object project_fn, size_t nb) {
+        if (project_fn.is_none() || !py::hasattr(project_fn, "_hfgpu_symmetry") || nb > 32)
+            return;
+        py::object spec = project_fn.attr("_hfgpu_symmetry")();
+        if (spec.is_none()) return;
+        py::tuple t = spec;
+        d.nb = nb;
+        auto group = [&](py::handle h, CArr& keep, size_t& n, const double*& ptr) {
+            if (h.is_none()) return;
+            keep = py::cast<CArr>(h);
+            if (keep.ndim() != 3 || (size_t)keep.shape(1) != nb || (size_t)keep.shape(2) != nb)
+                throw py::value_error("symmetry group matrices must be (n, nb, nb)");
+            n = keep.shape(0);
+            ptr = cdp(keep);
+        };
+        group(t[0], U, d.n_unitary, d.unitary);
+        group(t[1], S, d.n_spatial, d.spatial);
+        d.spatial_mod = py::cast<bool>(t[2]) ? 1 : 0;
+        d.spatial_axes = py::cast<int>(t[3]);
+        if (!t[4].is_none()) {
+            T = py::cast<CArr>(t[4]);
+            if (T.ndim() != 2 || (size_t)T.shape(0) != nb || (size_t)T.shape(1) != nb)
+                throw py::value_error("time_reversal_U must be (nb, nb)");
+            d.tr_U = cdp(T);
+        }
+        d.tr_mod = py::cast<bool>(t[5]) ? 1 : 0;
+        d.tr_axes = py::cast<int>(t[6]);
+        active = d.n_unitary > 0 || d.n_spatial > 0 || d.tr_U != nullptr;
+    }
+};
+
 // RAII kernel handle.
 struct Handle {
     hfgpu_kernel* k = nullptr;
@@ -272,6 +312,52 @@ PYBIND11_MODULE(_native, m) {
         },
         py::arg("VR"), py::arg("P"), py::arg("apply_ifftshift") = true,
         py::arg("hermitian_channel_packing") = false);
+
+    // --- device symmetry projector (no counterpart in the reference) ---
+    m.def(
+        "project_symmetry",
+        [](CArr M, py::object unitary_group, py::object spatial_group, bool spatial_mod,
+           int spatial_axes, py::object time_reversal_U, bool tr_mod, int tr_axes) {
+            if (M.ndim() != 4 || M.shape(2) != M.shape(3))
+                throw py::value_error("M must be (nk1, nk2, nb, nb)");
+            const size_t nb = M.shape(2);
+            CArr U, S, T;
+            hf_symmetry_desc d{};
+            d.nb = nb;
+            auto group = [&](py::object h, CArr& keep, size_t& n, const double*& ptr) {
+                if (h.is_none()) return;
+                keep = py::cast<CArr>(h);
+                if (keep.ndim() != 3 || (size_t)keep.shape(1) != nb || (size_t)keep.shape(2) != nb)
+                    throw py::value_error("symmetry group matrices must be (n, nb, nb)");
+                n = keep.shape(0);
+                ptr = cdp(keep);
+            };
+            group(unitary_group, U, d.n_unitary, d.unitary);
+            group(spatial_group, S, d.n_spatial, d.spatial);
+            d.spatial_mod = spatial_mod ? 1 : 0;
+            d.spatial_axes = spatial_axes;
+            if (!time_reversal_U.is_none()) {
+                T = py::cast<CArr>(time_reversal_U);
+                if (T.ndim() != 2 || (size_t)T.shape(0) != nb || (size_t)T.shape(1) != nb)
+                    throw py::value_error("time_reversal_U must be (nb, nb)");
+                d.tr_U = cdp(T);
+            }
+            d.tr_mod = tr_mod ? 1 : 0;
+            d.tr_axes = tr_axes;
+            auto out = new_c(lead_shape(M, 0));
+            py::array oa = out;
+            std::memcpy(dp(oa), cdp(M), (size_t)M.size() * sizeof(c128));
+            int rc;
+            {
+                py::gil_scoped_release rel;
+                rc = hfgpu_project_symmetry(&d, dp(oa), M.shape(0), M.shape(1), nb);
+            }
+            check(rc);
+            return out;
+        },
+        py::arg("M"), py::arg("unitary_group") = py::none(), py::arg("spatial_group") = py::none(),
+        py::arg("spatial_mod") = true, py::arg("spatial_axes") = 3,
+        py::arg("time_reversal_U") = py::none(), py::arg("tr_mod") = true, py::arg("tr_axes") = 3);
 
     // --- superlattice exchange (cpp_hf_native.cpp:718-790) ---
     m.def(
@@ -545,9 +631,11 @@ PYBIND11_MODULE(_native, m) {
         res.hist_E = dp(ea);
         res.hist_density = dp(ra);
         res.hist_comm = dp(ca);
+        SymSpec sym(project_fn, ka.d.nb);
+        if (sym.active) cfg.symmetry = &sym.d;
         ProjectCtx ctx;
         ctx.fn = project_fn;
-        const bool has_proj = !project_fn.is_none();
+        const bool has_proj = !project_fn.is_none() && !sym.active;
         int rc;
         {
             py::gil_scoped_release rel;
@@ -635,9 +723,11 @@ PYBIND11_MODULE(_native, m) {
         res.fock = dp(fa);
         res.hist_E = dp(ea);
         res.hist_grad = dp(ga);
+        SymSpec sym(project_fn, ka.d.nb);
+        if (sym.active) cfg.symmetry = &sym.d;
         ProjectCtx ctx;
         ctx.fn = project_fn;
-        const bool has_proj = !project_fn.is_none();
+        const bool has_proj = !project_fn.is_none() && !sym.active;
         int rc;
         {
             py::gil_scoped_release rel;

@@ -18,6 +18,7 @@
 #include "hfgpu_core.h"
 #include "loops.h"
 #include "rtr.h"
+#include "symmetry.h"
 
 namespace hfgpu {
 
@@ -28,13 +29,19 @@ constexpr double TINY_REAL = 1.0e-30;
 struct Projector {
     hf_project_fn fn = nullptr;
     void* user = nullptr;
+    SymPlan* sym = nullptr;  // device symmetry projector in place of fn
     int nk1 = 0, nk2 = 0, nb = 0;
     size_t nd = 0;
     std::vector<double> host;
 
-    explicit operator bool() const { return fn != nullptr; }
-    // Host round trip: the callback sees and rewrites a host copy.
+    explicit operator bool() const { return fn != nullptr || sym != nullptr; }
+    // Host round trip: the callback sees and rewrites a host copy; the
+    // device projector runs on the stream.
     void apply(cd* dM, hipStream_t st) {
+        if (sym) {
+            sym->apply(dM, st);
+            return;
+        }
         host.resize(2 * nd);
         d2h(host.data(), dM, nd * sizeof(cd), st);
         HFGPU_CHECK(hipStreamSynchronize(st));
@@ -44,6 +51,18 @@ struct Projector {
         h2d(dM, host.data(), nd * sizeof(cd), st);
     }
 };
+
+// A device symmetry projector from the solver configuration, if any.
+void attach_symmetry(Projector& pr, SymPlan& sp, const hf_symmetry_desc* sym, GpuKernel& K) {
+    if (!sym) return;
+    if (pr.fn)
+        throw std::invalid_argument(
+            "pass either project_fn or a device symmetry projector, not both");
+    if (sym->nb != static_cast<size_t>(K.nb))
+        throw std::invalid_argument("symmetry matrices must be (nb, nb) with the kernel's nb");
+    sp.init(*sym, K.nk1, K.nk2, K.stream);
+    pr.sym = &sp;
+}
 
 // electron-count check (utils.hpp:31-46)
 void validate_electron_count(const std::vector<double>& w, size_t nb, double n_e) {
@@ -867,6 +886,8 @@ int hfgpu_solve_scf(hfgpu_kernel* k, const double* P0, double n_e, const hf_scf_
         pr.nk2 = K.nk2;
         pr.nb = K.nb;
         pr.nd = K.nd;
+        SymPlan sp;
+        attach_symmetry(pr, sp, cfg->symmetry, K);
         solve_scf_gpu(K, P0, n_e, *cfg, pr, out);
     });
 }
@@ -884,6 +905,8 @@ int hfgpu_solve_dm(hfgpu_kernel* k, const double* P0, double n_e, const hf_dm_co
         pr.nk2 = K.nk2;
         pr.nb = K.nb;
         pr.nd = K.nd;
+        SymPlan sp;
+        attach_symmetry(pr, sp, cfg->symmetry, K);
         if (cfg->optimizer == 1)
             solve_rtr_gpu(K, P0, n_e, *cfg, pr, out);
         else

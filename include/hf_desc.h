@@ -86,6 +86,33 @@ typedef struct hf_kernel_desc {
     const hf_superlattice_desc* superlattice; /* NULL: the mesh exchange */
 } hf_kernel_desc;
 
+/*
+ * Symmetry projector on the device (no counterpart in the reference, whose
+ * ProjectFn is a host callback; the averages are those of make_project_fn,
+ * symmetry.py:81-117).  On a (nk1, nk2, nb, nb) complex array A:
+ *   with n_spatial > 0: A_k <- (sum_g g A_k g^H + sum_s s A_-k s^H) /
+ *       (n_unitary' + n_spatial), where the unitary set is the identity
+ *       alone (n_unitary' = 1) when n_unitary == 0;
+ *   else with n_unitary > 0: A_k <- sum_g g A_k g^H / n_unitary;
+ *   then with tr_U: A_k <- (A_k + U conj(A_-k) U^H) / 2.
+ * -k maps index i to (-i) mod n (mod != 0) or n - 1 - i (flip) along the
+ * axes whose bit is set in axes (bit 0: k1, bit 1: k2).
+ *   unitary : (n_unitary, nb, nb) complex;  spatial : (n_spatial, nb, nb)
+ *   tr_U    : (nb, nb) complex or NULL
+ */
+typedef struct hf_symmetry_desc {
+    size_t nb;
+    size_t n_unitary;
+    const double* unitary;
+    size_t n_spatial;
+    const double* spatial;
+    int spatial_mod;
+    int spatial_axes;
+    const double* tr_U;
+    int tr_mod;
+    int tr_axes;
+} hf_symmetry_desc;
+
 /* Reference: SCFConfig, solver_scf.hpp:20-50. acceleration: 0 linear, 1 diis, 2 oda. */
 typedef struct hf_scf_config {
     size_t max_iter;
@@ -106,6 +133,9 @@ typedef struct hf_scf_config {
      * builds to convergence (the FP64 correction phase).  0: FP64 only. */
     int exchange_fp32;
     double fp32_switch;
+    /* Device symmetry projector in place of the project_fn callback (no
+     * counterpart in the reference); NULL: none.  Passing both is an error. */
+    const hf_symmetry_desc* symmetry;
 } hf_scf_config;
 
 /* Reference: SolverConfig, solver_dm.hpp:22-63.  optimizer 0: Riemannian CG
@@ -140,6 +170,8 @@ typedef struct hf_dm_config {
      * builds in FP64. */
     int exchange_fp32;
     double fp32_switch;
+    /* as in hf_scf_config */
+    const hf_symmetry_desc* symmetry;
 } hf_dm_config;
 
 /*

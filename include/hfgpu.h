@@ -68,6 +68,17 @@ int hfgpu_selfenergy_fft_full(const double* VR, const double* P, double* sigma, 
                               int apply_ifftshift, int hermitian_channel_packing);
 
 /* ------------------------------------------------------------------------
+ * Symmetry projector (no counterpart in the reference, whose ProjectFn is a
+ * host callback; the averages are make_project_fn's, symmetry.py:81-117).
+ * For callers who want the device averaging outside a solve; the solvers
+ * take the same description through hf_scf_config / hf_dm_config.symmetry,
+ * which keeps the projections on the device inside their loops.
+ *   M: (nk1, nk2, nb, nb) complex host array, rewritten in place.
+ * ---------------------------------------------------------------------- */
+int hfgpu_project_symmetry(const hf_symmetry_desc* sym, double* M, size_t nk1, size_t nk2,
+                           size_t nb);
+
+/* ------------------------------------------------------------------------
  * Superlattice exchange on the extended momentum grid.
  * Replaces: _native.selfenergy_superlattice_streamed
  * (cpp/cpp_hf_native.cpp:719-790) -> selfenergy_superlattice_streamed

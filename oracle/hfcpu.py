@@ -61,6 +61,7 @@ class ScfConfig(C.Structure):
         ("diis_damping", C.c_double), ("trust_radius", C.c_double),
         ("block_sizes", C.POINTER(_sz)), ("n_blocks", _sz),
         ("exchange_fp32", C.c_int), ("fp32_switch", C.c_double),
+        ("symmetry", C.c_void_p),  # the device projector's slot, always NULL here
     ]
 
 
@@ -74,7 +75,7 @@ class DmConfig(C.Structure):
         ("optimizer", C.c_int), ("tr_delta0", C.c_double), ("tr_cg_max", _sz),
         ("deflation_targets", _dp), ("n_deflation", _sz), ("deflation_sigma", C.c_double),
         ("deflation_length", C.c_double), ("exchange_fp32", C.c_int),
-        ("fp32_switch", C.c_double),
+        ("fp32_switch", C.c_double), ("symmetry", C.c_void_p),
     ]
 
 
