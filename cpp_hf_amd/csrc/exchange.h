@@ -53,6 +53,9 @@ struct ExchangeGeom {
     // two-step launch shape: wavefronts (channel tiles) per row workgroup
     // (0: the whole row), wavefronts per column workgroup
     int row_waves = 1, col_waves = 2;
+    // wavefronts sharing one channel tile in the two-step passes (1 or 2):
+    // rows forward, columns, rows inverse
+    int split_fwd = 2, split_col = 2, split_inv = 1;
     int* d_chan_a = nullptr;  // nch
     int* d_chan_b = nullptr;  // nch
     AxisPlan ax1, ax2;        // along k1 (nk1) and along k2 (nk2)

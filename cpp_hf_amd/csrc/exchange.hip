@@ -154,6 +154,15 @@ void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
     // HFGPU_ROW_WAVES (0: whole rows) and HFGPU_COL_WAVES override it
     row_waves = 1;
     col_waves = 2;
+    // wavefronts per channel tile of the two-step passes (measured at 96 x 96
+    // and 192 x 192: two help the forward row and the column passes, whose
+    // lanes then hold half the registers; one for the inverse rows, whose
+    // epilogue stores dominate); HFGPU_SPLIT = 1 or 2 sets all three
+    split_fwd = 2;
+    split_col = 2;
+    split_inv = 1;
+    if (const char* e = std::getenv("HFGPU_SPLIT"))
+        split_fwd = split_col = split_inv = std::atoi(e) == 2 ? 2 : 1;
     if (const char* e = std::getenv("HFGPU_ROW_WAVES")) row_waves = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("HFGPU_COL_WAVES")) {
         const int v = std::atoi(e);
@@ -625,44 +634,50 @@ constexpr int kFastWaves = 4;  // most tiles per workgroup of the column pass
 // critical path of one of them.  The parts of a row stay on one XCD.
 constexpr int kFastRowWavesMax = 8;
 
-template <int N1, int N2>
+// A tile's tasks are dealt to S wavefronts (S = 1 or 2): task t of the tile
+// runs on wavefront (t / 64) % S, lane t % 64; two wavefronts per tile halve
+// the registers of a lane and double the wavefronts of a pass.
+template <int N1, int N2, int S = 1>
 struct FastShape {
     static constexpr int N = N1 * N2;
     static constexpr int L = kFastLines;
-    static constexpr int R1 = (L * N1 + 63) / 64;  // F1 / I2 tasks per lane
-    static constexpr int R2 = (L * N2 + 63) / 64;  // F2 / I1 tasks per lane
-    static constexpr int Y = N * L;                // LDS complex slots per wave
+    static constexpr int R1 = (L * N1 + 63) / 64;  // F1 / I2 task rounds per tile
+    static constexpr int R2 = (L * N2 + 63) / 64;  // F2 / I1 task rounds per tile
+    static constexpr int R1S = (R1 + S - 1) / S;   // ... per wavefront
+    static constexpr int R2S = (R2 + S - 1) / S;
+    static constexpr int Y = N * L;                // LDS complex slots per tile
 };
 
 // Pass A (rows, forward) on the two-step transform.
 // TS: storage type of the workspace (cd, or cf for the mixed-precision build).
-template <int N1, int N2, class TS>
+template <int N1, int N2, class TS, int S>
 __global__ void __launch_bounds__(64 * kFastRowWavesMax)
 k_rows_fwd_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, int packed,
                 const cd* __restrict__ P, const cd* __restrict__ refP,
                 const cd* __restrict__ refP0, const double* __restrict__ wrow,
                 TS* __restrict__ T, cd* __restrict__ partial, const double* __restrict__ w2d,
                 int rpw) {
-    using Sh = FastShape<N1, N2>;
+    using Sh = FastShape<N1, N2, S>;
     constexpr int N = Sh::N, L = Sh::L;
     extern __shared__ cd smem[];
     cd* s_tw = smem;
-    const int nw = blockDim.x >> 6;  // tiles (wavefronts) per workgroup
+    const int nw = (blockDim.x >> 6) / S;  // tiles per workgroup
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    cd* Y = smem + N + wave * Sh::Y;
-    cd* s_part = smem + N + nw * Sh::Y + wave * N1 * L;
+    const int tix = wave / S, sub = wave % S;
+    cd* Y = smem + N + tix * Sh::Y;
+    cd* s_part = smem + N + nw * Sh::Y + tix * N1 * L;
     // rpw workgroups per k1 row, nw channel tiles each
     int row, part;
     split_of(static_cast<int>(blockIdx.x), rpw, row, part);
-    const bool active = row < nk1 && part * nw + wave < nblk;
-    const int k1 = active ? row : 0, blk = active ? part * nw + wave : 0;
+    const bool active = row < nk1 && part * nw + tix < nblk;
+    const int k1 = active ? row : 0, blk = active ? part * nw + tix : 0;
     const int nb2 = nb * nb;
     for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
-    cd u[Sh::R1][N2];
-    double wv[Sh::R1][N2];
+    cd u[Sh::R1S][N2];
+    double wv[Sh::R1S][N2];
 #pragma unroll
-    for (int r = 0; r < Sh::R1; ++r) {
-        const int t = lane + 64 * r;
+    for (int r = 0; r < Sh::R1S; ++r) {
+        const int t = lane + 64 * (r * S + sub);
         const int l = t & (L - 1), i1 = t >> 3;
         const int ch = blk * L + l;
         const bool ok = active && t < L * N1 && ch < nch;
@@ -683,8 +698,8 @@ k_rows_fwd_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < Sh::R1; ++r) {
-        const int t = lane + 64 * r;
+    for (int r = 0; r < Sh::R1S; ++r) {
+        const int t = lane + 64 * (r * S + sub);
         if (t < L * N1) {
             const int l = t & (L - 1), i1 = t >> 3;
             if (partial) {
@@ -706,7 +721,7 @@ k_rows_fwd_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
         }
     }
     __syncthreads();
-    if (partial && active && lane < L) {
+    if (partial && active && sub == 0 && lane < L) {
         const int ch = blk * L + lane;
         if (ch < nch) {
             cd acc{0.0, 0.0};
@@ -724,8 +739,8 @@ k_rows_fwd_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
     }
     TS* Tt = T + (static_cast<size_t>(blk) * nk1 + k1) * N * L;
 #pragma unroll
-    for (int r = 0; r < Sh::R2; ++r) {
-        const int t = lane + 64 * r;
+    for (int r = 0; r < Sh::R2S; ++r) {
+        const int t = lane + 64 * (r * S + sub);
         if (active && t < L * N2) {
             const int l = t & (L - 1), k2 = t >> 3;
             cd v[N1];
@@ -747,13 +762,13 @@ k_rows_fwd_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
 
 // Pass B (columns: forward, x VR, inverse) on the two-step transform;
 // workgroup 0 hosts the small terms when has_small.
-template <int N1, int N2, class TS>
+template <int N1, int N2, class TS, int S>
 __global__ void __launch_bounds__(64 * kFastWaves)
 k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restrict__ T,
             const TS* __restrict__ VR, int vr_scalar, int has_small,
             const cd* __restrict__ partial, const int* __restrict__ chan_a,
             const int* __restrict__ chan_b, int nb, int packed, SmallTerms st) {
-    using Sh = FastShape<N1, N2>;
+    using Sh = FastShape<N1, N2, S>;
     constexpr int N = Sh::N, L = Sh::L;
     extern __shared__ cd smem[];
     if (has_small && blockIdx.x == 0) {
@@ -761,13 +776,14 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restric
         return;
     }
     cd* s_tw = smem;
-    const int nw = blockDim.x >> 6;  // tiles (wavefronts) per workgroup
+    const int nw = (blockDim.x >> 6) / S;  // tiles per workgroup
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    cd* Y = smem + N + wave * Sh::Y;
-    // the wavefronts of a workgroup take adjacent columns of one channel
-    // tile, so each k1 row of the workspace is read and written as nw * 128
+    const int tix = wave / S, sub = wave % S;
+    cd* Y = smem + N + tix * Sh::Y;
+    // the tiles of a workgroup are adjacent columns of one channel tile, so
+    // each k1 row of the workspace is read and written as nw * 128
     // contiguous bytes
-    const int tile = (static_cast<int>(blockIdx.x) - has_small) * nw + wave;
+    const int tile = (static_cast<int>(blockIdx.x) - has_small) * nw + tix;
     const bool active = tile < nk2 * nblk;
     const int kc = active ? tile % nk2 : 0, blk = active ? tile / nk2 : 0;
     TS* Tb = T + static_cast<size_t>(blk) * N * nk2 * L;                 // [k1][k2][c]
@@ -775,11 +791,11 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restric
     for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
     // loads land in registers of the storage type and are widened after the
     // barrier, so every load of the wave is in flight before the first use
-    TS u[Sh::R1][N2];
-    TS vv[Sh::R2][N1];
+    TS u[Sh::R1S][N2];
+    TS vv[Sh::R2S][N1];
 #pragma unroll
-    for (int r = 0; r < Sh::R1; ++r) {
-        const int t = lane + 64 * r;
+    for (int r = 0; r < Sh::R1S; ++r) {
+        const int t = lane + 64 * (r * S + sub);
         const int l = t & (L - 1), i1 = t >> 3;
         const bool ok = active && t < L * N1;
 #pragma unroll
@@ -787,8 +803,8 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restric
             u[r][i2] = ok ? Tb[(static_cast<size_t>(i1 + N1 * i2) * nk2 + kc) * L + l] : TS{};
     }
 #pragma unroll
-    for (int r = 0; r < Sh::R2; ++r) {
-        const int t = lane + 64 * r;
+    for (int r = 0; r < Sh::R2S; ++r) {
+        const int t = lane + 64 * (r * S + sub);
         const int l = t & (L - 1), k2 = t >> 3;
         const bool ok = active && t < L * N2;
 #pragma unroll
@@ -800,8 +816,8 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restric
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < Sh::R1; ++r) {
-        const int t = lane + 64 * r;
+    for (int r = 0; r < Sh::R1S; ++r) {
+        const int t = lane + 64 * (r * S + sub);
         if (t < L * N1) {
             const int l = t & (L - 1), i1 = t >> 3;
             cd w[N2];
@@ -818,8 +834,8 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restric
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < Sh::R2; ++r) {
-        const int t = lane + 64 * r;
+    for (int r = 0; r < Sh::R2S; ++r) {
+        const int t = lane + 64 * (r * S + sub);
         if (t < L * N2) {
             const int l = t & (L - 1), k2 = t >> 3;
             cd v[N1];
@@ -840,8 +856,8 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restric
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < Sh::R1; ++r) {
-        const int t = lane + 64 * r;
+    for (int r = 0; r < Sh::R1S; ++r) {
+        const int t = lane + 64 * (r * S + sub);
         if (active && t < L * N1) {
             const int l = t & (L - 1), i1 = t >> 3;
             cd z[N2];
@@ -856,42 +872,43 @@ k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restric
 }
 
 // Pass C (rows, inverse + epilogue) on the two-step transform.
-template <int N1, int N2, class TS>
+template <int N1, int N2, class TS, int S>
 __global__ void __launch_bounds__(64 * kFastRowWavesMax)
 k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, int packed,
                 const TS* __restrict__ T, int epilogue, int shift, double inv_n,
                 cd* __restrict__ Sigma, cd* __restrict__ F, const cd* __restrict__ hh,
                 const cd* __restrict__ SC, const double* __restrict__ hdiag, int rpw) {
-    using Sh = FastShape<N1, N2>;
+    using Sh = FastShape<N1, N2, S>;
     constexpr int N = Sh::N, L = Sh::L;
     extern __shared__ cd smem[];
     cd* s_tw = smem;
-    const int nw = blockDim.x >> 6;  // tiles (wavefronts) per workgroup
+    const int nw = (blockDim.x >> 6) / S;  // tiles per workgroup
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    cd* Y = smem + N + wave * Sh::Y;
+    const int tix = wave / S, sub = wave % S;
+    cd* Y = smem + N + tix * Sh::Y;
     // rpw workgroups per k1 row, nw channel tiles each
     int row, part;
     split_of(static_cast<int>(blockIdx.x), rpw, row, part);
-    const bool active = row < nk1 && part * nw + wave < nblk;
-    const int k1 = active ? row : 0, blk = active ? part * nw + wave : 0;
+    const bool active = row < nk1 && part * nw + tix < nblk;
+    const int k1 = active ? row : 0, blk = active ? part * nw + tix : 0;
     const size_t toff = (static_cast<size_t>(blk) * nk1 + k1) * N * L;
     const bool fock = epilogue == EPI_FOCK_PACKED;
     for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = tw[i];
-    TS v[Sh::R2][N1];  // storage type, widened after the barrier
+    TS v[Sh::R2S][N1];  // storage type, widened after the barrier
 #pragma unroll
-    for (int r = 0; r < Sh::R2; ++r) {
-        const int t = lane + 64 * r;
+    for (int r = 0; r < Sh::R2S; ++r) {
+        const int t = lane + 64 * (r * S + sub);
         const int l = t & (L - 1), k2 = t >> 3;
         const bool ok = active && t < L * N2;
 #pragma unroll
         for (int k1p = 0; k1p < N1; ++k1p)
             v[r][k1p] = ok ? T[toff + (k2 + N2 * k1p) * L + l] : TS{};
     }
-    cd hv[Sh::R1][N2];
+    cd hv[Sh::R1S][N2];
     if (fock) {
 #pragma unroll
-        for (int r = 0; r < Sh::R1; ++r) {
-            const int t = lane + 64 * r;
+        for (int r = 0; r < Sh::R1S; ++r) {
+            const int t = lane + 64 * (r * S + sub);
             const int l = t & (L - 1), i1 = t >> 3;
             const bool ok = active && t < L * N1;
 #pragma unroll
@@ -901,8 +918,8 @@ k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < Sh::R2; ++r) {
-        const int t = lane + 64 * r;
+    for (int r = 0; r < Sh::R2S; ++r) {
+        const int t = lane + 64 * (r * S + sub);
         if (t < L * N2) {
             const int l = t & (L - 1), k2 = t >> 3;
             cd w[N1];
@@ -921,8 +938,8 @@ k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
     const int d1 = shift ? (k1 + nk1 - nk1 / 2) % nk1 : k1;
     const int nb2 = nb * nb;
 #pragma unroll
-    for (int r = 0; r < Sh::R1; ++r) {
-        const int t = lane + 64 * r;
+    for (int r = 0; r < Sh::R1S; ++r) {
+        const int t = lane + 64 * (r * S + sub);
         const int l = t & (L - 1), i1 = t >> 3;
         const int ch = blk * L + l;
         if (!(active && t < L * N1 && ch < nch)) continue;
@@ -1308,9 +1325,9 @@ static unsigned row_grid(int nk1, int rpw) {
 
 // wavefronts (channel tiles) per workgroup of a row pass: the whole row if it
 // fits (or g.row_waves when set), at most kFastRowWavesMax and the LDS
-static int row_waves(const ExchangeGeom& g, int n1, bool row_sums) {
+static int row_waves(const ExchangeGeom& g, int n1, bool row_sums, int split) {
     int w = g.row_waves > 0 ? g.row_waves : g.nblk;
-    w = std::min({w, g.nblk, kFastRowWavesMax});
+    w = std::min({w, g.nblk, kFastRowWavesMax / split});
     while (w > 1 && fast_lds(g.nk2, n1, w, row_sums) > 160 * 1024) --w;
     return w;
 }
@@ -1318,17 +1335,25 @@ static int row_waves(const ExchangeGeom& g, int n1, bool row_sums) {
 static void launch_rows_fwd_fast(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
 #define HFGPU_FAST_CASE(N, A, B)                                                              \
     if (g.nk2 == N) {                                                                         \
-        const int w = row_waves(g, A, true), rpw = (g.nblk + w - 1) / w;                      \
-        if (a.ws32)                                                                           \
-            hipLaunchKernelGGL((k_rows_fwd_fast<A, B, cf>), dim3(row_grid(g.nk1, rpw)),       \
-                               dim3(64 * w), fast_lds(N, A, w, true), st, g.ax2.d_tw, g.nk1,  \
-                               g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, a.refP, a.refP0,   \
-                               a.wrow, a.T32, a.partial, a.w2d, rpw);                         \
+        const int w = row_waves(g, A, true, g.split_fwd), rpw = (g.nblk + w - 1) / w;         \
+        const dim3 grid(row_grid(g.nk1, rpw)), block(64 * w * g.split_fwd);                   \
+        const size_t lds = fast_lds(N, A, w, true);                                           \
+        if (a.ws32 && g.split_fwd == 2)                                                       \
+            hipLaunchKernelGGL((k_rows_fwd_fast<A, B, cf, 2>), grid, block, lds, st,          \
+                               g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, \
+                               a.refP, a.refP0, a.wrow, a.T32, a.partial, a.w2d, rpw);        \
+        else if (a.ws32)                                                                      \
+            hipLaunchKernelGGL((k_rows_fwd_fast<A, B, cf, 1>), grid, block, lds, st,          \
+                               g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, \
+                               a.refP, a.refP0, a.wrow, a.T32, a.partial, a.w2d, rpw);        \
+        else if (g.split_fwd == 2)                                                            \
+            hipLaunchKernelGGL((k_rows_fwd_fast<A, B, cd, 2>), grid, block, lds, st,          \
+                               g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, \
+                               a.refP, a.refP0, a.wrow, a.T, a.partial, a.w2d, rpw);          \
         else                                                                                  \
-            hipLaunchKernelGGL((k_rows_fwd_fast<A, B, cd>), dim3(row_grid(g.nk1, rpw)),       \
-                               dim3(64 * w), fast_lds(N, A, w, true), st, g.ax2.d_tw, g.nk1,  \
-                               g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, a.refP, a.refP0,   \
-                               a.wrow, a.T, a.partial, a.w2d, rpw);                           \
+            hipLaunchKernelGGL((k_rows_fwd_fast<A, B, cd, 1>), grid, block, lds, st,          \
+                               g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, \
+                               a.refP, a.refP0, a.wrow, a.T, a.partial, a.w2d, rpw);          \
         HFGPU_LAUNCH_CHECK();                                                                 \
         return;                                                                               \
     }
@@ -1339,19 +1364,29 @@ static void launch_rows_fwd_fast(const ExchangeGeom& g, const ExchangeArgs& a, h
 
 static void launch_cols_fast(const ExchangeGeom& g, const ExchangeArgs& a, int hs,
                              hipStream_t st) {
-    const int cw = g.col_waves;
-    const dim3 grid(fast_grid(g.nk2, g.nblk, cw) + hs), block(64 * cw);
+    const int cw = std::max(1, std::min(g.col_waves, kFastWaves / g.split_col));
+    const dim3 grid(fast_grid(g.nk2, g.nblk, cw) + hs), block(64 * cw * g.split_col);
 #define HFGPU_FAST_CASE(N, A, B)                                                              \
     if (g.nk1 == N) {                                                                         \
         const size_t lds = std::max(fast_lds(N, A, cw, false),                                \
                                     hs ? sizeof(cd) * small_terms_cd(g.nb) : size_t(0));      \
-        if (a.ws32)                                                                           \
-            hipLaunchKernelGGL((k_cols_fast<A, B, cf>), grid, block, lds, st, g.ax1.d_tw,     \
+        if (a.ws32 && g.split_col == 2)                                                       \
+            hipLaunchKernelGGL((k_cols_fast<A, B, cf, 2>), grid, block, lds, st, g.ax1.d_tw,  \
                                g.nk2, g.nch, g.nblk, a.T32, a.VR32, a.vr_scalar ? 1 : 0, hs,  \
                                a.partial, g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0,     \
                                a.small);                                                      \
+        else if (a.ws32)                                                                      \
+            hipLaunchKernelGGL((k_cols_fast<A, B, cf, 1>), grid, block, lds, st, g.ax1.d_tw,  \
+                               g.nk2, g.nch, g.nblk, a.T32, a.VR32, a.vr_scalar ? 1 : 0, hs,  \
+                               a.partial, g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0,     \
+                               a.small);                                                      \
+        else if (g.split_col == 2)                                                            \
+            hipLaunchKernelGGL((k_cols_fast<A, B, cd, 2>), grid, block, lds, st, g.ax1.d_tw,  \
+                               g.nk2, g.nch, g.nblk, a.T, a.VR, a.vr_scalar ? 1 : 0, hs,      \
+                               a.partial, g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0,     \
+                               a.small);                                                      \
         else                                                                                  \
-            hipLaunchKernelGGL((k_cols_fast<A, B, cd>), grid, block, lds, st, g.ax1.d_tw,     \
+            hipLaunchKernelGGL((k_cols_fast<A, B, cd, 1>), grid, block, lds, st, g.ax1.d_tw,  \
                                g.nk2, g.nch, g.nblk, a.T, a.VR, a.vr_scalar ? 1 : 0, hs,      \
                                a.partial, g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0,     \
                                a.small);                                                      \
@@ -1367,19 +1402,29 @@ static void launch_rows_inv_fast(const ExchangeGeom& g, const ExchangeArgs& a, d
                                  hipStream_t st) {
 #define HFGPU_FAST_CASE(N, A, B)                                                              \
     if (g.nk2 == N) {                                                                         \
-        const int w = row_waves(g, A, false), rpw = (g.nblk + w - 1) / w;                     \
-        if (a.ws32)                                                                           \
-            hipLaunchKernelGGL((k_rows_inv_fast<A, B, cf>), dim3(row_grid(g.nk1, rpw)),       \
-                               dim3(64 * w), fast_lds(N, A, w, false), st, g.ax2.d_tw, g.nk1, \
-                               g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.T32, a.epilogue,      \
-                               a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh, a.SC, a.hdiag, \
-                               rpw);                                                          \
+        const int w = row_waves(g, A, false, g.split_inv), rpw = (g.nblk + w - 1) / w;        \
+        const dim3 grid(row_grid(g.nk1, rpw)), block(64 * w * g.split_inv);                   \
+        const size_t lds = fast_lds(N, A, w, false);                                          \
+        if (a.ws32 && g.split_inv == 2)                                                       \
+            hipLaunchKernelGGL((k_rows_inv_fast<A, B, cf, 2>), grid, block, lds, st,          \
+                               g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0,      \
+                               a.T32, a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F,   \
+                               a.hh, a.SC, a.hdiag, rpw);                                     \
+        else if (a.ws32)                                                                      \
+            hipLaunchKernelGGL((k_rows_inv_fast<A, B, cf, 1>), grid, block, lds, st,          \
+                               g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0,      \
+                               a.T32, a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F,   \
+                               a.hh, a.SC, a.hdiag, rpw);                                     \
+        else if (g.split_inv == 2)                                                            \
+            hipLaunchKernelGGL((k_rows_inv_fast<A, B, cd, 2>), grid, block, lds, st,          \
+                               g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.T, \
+                               a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh,    \
+                               a.SC, a.hdiag, rpw);                                           \
         else                                                                                  \
-            hipLaunchKernelGGL((k_rows_inv_fast<A, B, cd>), dim3(row_grid(g.nk1, rpw)),       \
-                               dim3(64 * w), fast_lds(N, A, w, false), st, g.ax2.d_tw, g.nk1, \
-                               g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.T, a.epilogue,        \
-                               a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh, a.SC, a.hdiag, \
-                               rpw);                                                          \
+            hipLaunchKernelGGL((k_rows_inv_fast<A, B, cd, 1>), grid, block, lds, st,          \
+                               g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.T, \
+                               a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh,    \
+                               a.SC, a.hdiag, rpw);                                           \
         HFGPU_LAUNCH_CHECK();                                                                 \
         return;                                                                               \
     }

@@ -108,3 +108,22 @@ def test_unknown_k_convention_raises():
     proj = make_project_fn(time_reversal_U=np.eye(2), time_reversal_k_convention="mirror")
     with pytest.raises(ValueError, match="k_convention"):
         proj(np.zeros((2, 2, 2, 2), dtype=np.complex128))
+
+
+def test_device_description_of_the_projector():
+    # what the GPU solvers receive: groups as complex arrays, conventions as
+    # flags, k axes as bits (k1 = 1, k2 = 2); axes outside (0, 1) have no
+    # device form, an unknown convention raises as a call would
+    G = np.stack([np.eye(2), SZ]).real
+    p = make_project_fn(unitary_group=G, spatial_group=SX[None], spatial_k_convention="flip",
+                        spatial_k_flip_axes=(1,), time_reversal_U=SY)
+    U, S, smod, sax, T, tmod, tax = p._hfgpu_symmetry()
+    assert U.dtype == np.complex128 and U.shape == (2, 2, 2)
+    np.testing.assert_array_equal(U, G)
+    np.testing.assert_array_equal(S, SX[None])
+    assert (smod, sax, tmod, tax) == (False, 2, True, 3)
+    np.testing.assert_array_equal(T, SY)
+    assert make_project_fn(unitary_group=G, spatial_group=SX[None],
+                           spatial_k_flip_axes=(0, 2))._hfgpu_symmetry() is None
+    with pytest.raises(ValueError):
+        make_project_fn(time_reversal_U=SY, time_reversal_k_convention="mirror")._hfgpu_symmetry()
