@@ -28,6 +28,7 @@ The JSON line also carries:
                 Hartree shift) at 24 x 24 k (13 G x 2 orbitals) and 36 x 36 k
                 (19 G x 1 orbital), HIP-event timed, with the grid bytes one
                 build moves, the oracle's build on this host and an SCF solve.
+  autotune      the launch-shape autotuner's pick and time for the same build.
   symmetry      SCF solves at 96 x 96, nb 8 with a time-reversal projector
                 applied on the device (SymmetryProjector) and through the
                 host callback (a plain function): seconds and iterations.
@@ -438,6 +439,11 @@ def main() -> int:
         },
         "parity_rel_err_F": parity,
     }
+
+    # the launch-shape autotuner on this build (the line above uses the
+    # default shape); results of the build do not depend on the shape
+    tuned = gk.autotune(herm)
+    out["autotune"] = {"default_us_per_build": per_build_s * 1e6, **tuned}
 
     if not args.no_solves:
         solves = {}

@@ -220,6 +220,7 @@ GpuKernel::GpuKernel(const hf_kernel_desc& d) {
     // kernels on one stream measured faster than the replay of a captured
     // hipGraph of the same launches.  HFGPU_GRAPH=1 selects the graph replay.
     if (const char* e = std::getenv("HFGPU_GRAPH")) use_graph = e[0] == '1';
+    if (const char* e = std::getenv("HFGPU_AUTOTUNE")) autotune_env = e[0] == '1';
     HFGPU_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     const size_t cz = sizeof(cd);
     h.alloc(nd * cz);
@@ -336,6 +337,32 @@ GpuKernel::~GpuKernel() {
     if (adj_geo_v) adj_geo_v->release();
 }
 
+// launch shapes of the two-step exchange passes (autotuner below)
+namespace {
+
+struct LaunchShape {
+    int row_waves, col_waves, split_fwd, split_col, split_inv;
+};
+
+LaunchShape shape_of(const ExchangeGeom& g) {
+    return {g.row_waves, g.col_waves, g.split_fwd, g.split_col, g.split_inv};
+}
+
+void set_shape(ExchangeGeom& g, const LaunchShape& s) {
+    g.row_waves = s.row_waves;
+    g.col_waves = s.col_waves;
+    g.split_fwd = s.split_fwd;
+    g.split_col = s.split_col;
+    g.split_inv = s.split_inv;
+}
+
+// (nk1, nk2, nb, packed, dense V, small terms)
+using TuneKey = std::tuple<int, int, int, bool, bool, bool>;
+std::mutex g_tune_mu;
+std::map<TuneKey, LaunchShape> g_tuned;
+
+}  // namespace
+
 GpuKernel::Geom& GpuKernel::geom(bool packed) {
     Geom& G = packed ? geo_packed : geo_full;
     if (!G.g) {
@@ -353,8 +380,121 @@ GpuKernel::Geom& GpuKernel::geom(bool packed) {
         }
         HFGPU_CHECK(hipStreamSynchronize(stream));
         G.g = std::move(g);
+        if (autotune_env && exchange && !superlattice() && packed == packed_for(packed)) {
+            const TuneKey key{nk1, nk2, nb, packed, dv > 1, hartree || contact};
+            bool cached = false;
+            {
+                std::lock_guard<std::mutex> lock(g_tune_mu);
+                const auto it = g_tuned.find(key);
+                if (it != g_tuned.end()) {
+                    set_shape(*G.g, it->second);
+                    cached = true;
+                }
+            }
+            if (!cached) {
+                int shape[5];
+                (void)autotune(packed, shape);
+            }
+        }
     }
     return G;
+}
+
+// ---------------------------------------------------------------------------
+// launch-shape autotuner
+// ---------------------------------------------------------------------------
+
+
+double GpuKernel::autotune(bool p_hermitian, int shape_out[5]) {
+    const bool packed = exchange && packed_for(p_hermitian);
+    Geom& G = geom(packed);
+    ExchangeGeom& g = *G.g;
+    auto report = [&](const LaunchShape& s) {
+        shape_out[0] = s.row_waves;
+        shape_out[1] = s.col_waves;
+        shape_out[2] = s.split_fwd;
+        shape_out[3] = s.split_col;
+        shape_out[4] = s.split_inv;
+    };
+    report(shape_of(g));
+    if (!exchange || superlattice() || !(g.fast_rows && g.fast_cols)) return 0.0;
+    const TuneKey key{nk1, nk2, nb, packed, dv > 1, hartree || contact};
+    DevBuf P(nd * sizeof(cd)), S(nd * sizeof(cd)), F(nd * sizeof(cd)),
+        hd(std::max<size_t>(nb, 1) * sizeof(double));
+    fill_zero(P.get(), nd * sizeof(cd), stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HFGPU_CHECK(hipEventCreate(&e0));
+    HFGPU_CHECK(hipEventCreate(&e1));
+    struct EventGuard {
+        hipEvent_t a, b;
+        ~EventGuard() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{e0, e1};
+    int reps = 5;
+    auto time_shape = [&](const LaunchShape& s) {
+        set_shape(g, s);
+        for (int i = 0; i < 2; ++i)
+            build_fock(P.as<cd>(), S.as<cd>(), F.as<cd>(), hd.as<double>(), p_hermitian, stream);
+        float best = 1e30f;
+        for (int t = 0; t < 2; ++t) {
+            HFGPU_CHECK(hipEventRecord(e0, stream));
+            for (int i = 0; i < reps; ++i)
+                build_fock(P.as<cd>(), S.as<cd>(), F.as<cd>(), hd.as<double>(), p_hermitian,
+                           stream);
+            HFGPU_CHECK(hipEventRecord(e1, stream));
+            HFGPU_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HFGPU_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, ms);
+        }
+        return 1e3 * static_cast<double>(best) / reps;
+    };
+    LaunchShape best = shape_of(g);
+    double best_us = time_shape(best);
+    // about 2 ms of timed builds per candidate
+    reps = std::max(5, std::min(50, static_cast<int>(2000.0 / std::max(best_us, 1.0))));
+    best_us = time_shape(best);
+    auto consider = [&](LaunchShape s) {
+        const double us = time_shape(s);
+        if (us < best_us) {
+            best_us = us;
+            best = s;
+        }
+    };
+    for (int sf : {1, 2}) {
+        LaunchShape s = best;
+        s.split_fwd = sf;
+        if (sf != best.split_fwd) consider(s);
+    }
+    for (int sc : {1, 2})
+        for (int cw : {1, 2, 4}) {
+            if (sc * cw > 4) continue;
+            LaunchShape s = best;
+            s.split_col = sc;
+            s.col_waves = cw;
+            if (sc != best.split_col || cw != best.col_waves) consider(s);
+        }
+    for (int si : {1, 2}) {
+        LaunchShape s = best;
+        s.split_inv = si;
+        if (si != best.split_inv) consider(s);
+    }
+    for (int rw : {1, 2, 0}) {
+        if (rw == 2 && g.nblk < 3) continue;
+        LaunchShape s = best;
+        s.row_waves = rw;
+        if (rw != best.row_waves) consider(s);
+    }
+    set_shape(g, best);
+    HFGPU_CHECK(hipStreamSynchronize(stream));
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tuned[key] = best;
+    }
+    report(best);
+    return best_us;
 }
 
 bool GpuKernel::fp32_capable(bool p_hermitian) {
@@ -613,6 +753,13 @@ int hfgpu_kernel_destroy(hfgpu_kernel* k) {
 int hfgpu_kernel_packable(const hfgpu_kernel* k) {
     if (!k) return 0;
     return (k->impl->hcp || k->impl->dense_packable) ? 1 : 0;
+}
+
+int hfgpu_kernel_autotune(hfgpu_kernel* k, int p_hermitian, int* shape, double* best_us) {
+    return guarded([&] {
+        if (!k || !shape || !best_us) throw std::invalid_argument("null argument");
+        *best_us = k->impl->autotune(p_hermitian != 0, shape);
+    });
 }
 
 int hfgpu_selfenergy_fft_full(const double* VR, const double* P, double* sigma, size_t nk1,

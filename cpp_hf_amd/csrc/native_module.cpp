@@ -938,6 +938,26 @@ PYBIND11_MODULE(_native, m) {
             },
             py::arg("P"), py::arg("Sigma"), py::arg("F"), py::arg("hdiag"),
             py::arg("p_hermitian") = true, py::arg("stream") = 0)
+        .def("autotune",
+             [](PyKernel& k, bool p_hermitian) {
+                 int shape[5] = {0, 0, 0, 0, 0};
+                 double us = 0.0;
+                 int rc;
+                 {
+                     py::gil_scoped_release rel;
+                     rc = hfgpu_kernel_autotune(k.k, p_hermitian ? 1 : 0, shape, &us);
+                 }
+                 check(rc);
+                 py::dict d;
+                 d["us_per_build"] = us;
+                 d["row_waves"] = shape[0];
+                 d["col_waves"] = shape[1];
+                 d["split_fwd"] = shape[2];
+                 d["split_col"] = shape[3];
+                 d["split_inv"] = shape[4];
+                 return d;
+             },
+             py::arg("p_hermitian") = true)
         .def("mixed_capable",
              [](PyKernel& k, bool p_hermitian) {
                  return hfgpu_kernel_mixed_capable(k.k, p_hermitian ? 1 : 0) != 0;

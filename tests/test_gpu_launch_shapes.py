@@ -41,3 +41,44 @@ def test_launch_shapes_match_oracle(gpu, oracle, shape, monkeypatch):
         assert float(np.max(np.abs(F - Fr))) / scale < 1e-12, name
         assert float(np.max(np.abs(S - Sr))) / max(1e-300, float(np.max(np.abs(Sr)))) < 1e-12
         assert abs(E - Er) <= 1e-12 * max(1.0, abs(Er))
+
+
+@pytest.mark.parametrize("split", ["1", "2"])
+def test_wavefronts_per_tile_match_oracle(gpu, oracle, split, monkeypatch):
+    # one or two wavefronts per channel tile in all three passes, with the
+    # row and column workgroup shapes they combine with
+    monkeypatch.setenv("HFGPU_SPLIT", split)
+    for rw, cw in (("1", "2"), ("0", "4"), ("2", "1")):
+        monkeypatch.setenv("HFGPU_ROW_WAVES", rw)
+        monkeypatch.setenv("HFGPU_COL_WAVES", cw)
+        for name, k, P in _cases():
+            print(f"split {split} shape {rw}/{cw} case {name}", flush=True)
+            ka = kernel_args_dict(k)
+            S, H, F, E = gpu.build_fock_apply(ka, P, None)
+            Sr, Hr, Fr, Er = oracle.build_fock_apply(ka, P)
+            assert float(np.max(np.abs(F - Fr))) / float(np.max(np.abs(Fr))) < 1e-12, name
+            assert abs(E - Er) <= 1e-12 * max(1.0, abs(Er))
+
+
+def test_autotune_keeps_results(gpu, oracle, monkeypatch):
+    from cpp_hf_amd.workloads import layered_kernel
+
+    wl = layered_kernel(48, 4)
+    ka = kernel_args_dict(wl.kernel)
+    P = herm(wl.P0)
+    before = gpu.build_fock_apply(ka, P, None)
+    gk = gpu.GpuKernel(ka)
+    t = gk.autotune(True)
+    print(f"autotune 48x48 nb8: {t}", flush=True)
+    assert t["us_per_build"] > 0.0
+    assert t["split_fwd"] in (1, 2) and t["split_col"] in (1, 2) and t["split_inv"] in (1, 2)
+    assert t["col_waves"] * t["split_col"] <= 4 and t["row_waves"] >= 0
+    # HFGPU_AUTOTUNE=1: new geometries take the tuned (cached) shape; launch
+    # shapes move tasks between lanes only, so every output is bit-identical
+    monkeypatch.setenv("HFGPU_AUTOTUNE", "1")
+    after = gpu.build_fock_apply(ka, P, None)
+    for a, b in zip(before[:3], after[:3]):
+        np.testing.assert_array_equal(a, b)
+    assert before[3] == after[3]
+    Fr = oracle.build_fock_apply(ka, P)[2]
+    assert float(np.max(np.abs(after[2] - Fr))) / float(np.max(np.abs(Fr))) < 1e-12
