@@ -16,8 +16,13 @@ The JSON line also carries:
   cpu_baseline  the CPU oracle's Fock build on this host, timed for a
                 bounded number of builds (a restatement of the reference's
                 FFTW/Eigen path, which cannot be built here).
-  solves        wall time of a full direct-minimisation solve and an SCF solve
-                (mixing 0.5) on the same model.
+  solves        wall time of full direct-minimisation and SCF (mixing 0.5)
+                solves at C1 (QWZ 24 x 24) and C2, on the GPU and in the
+                oracle on the host, with each solve's roofline time: the
+                reference dataflow's bytes per iteration (DM (30 + 2 trials) A
+                with the line-search trials counted, SCF 21 A) over 8 TB/s;
+                plus the oracle's sequential rate on the first kernels of the
+                48-kernel sweep (a bounded sample).
   mixed_precision  the FP32-workspace Fock build against the FP64 one at
                 96 x 96 (nb 8) and 192 x 192 (nb 16), and SCF solves at
                 192 x 192 with and without the mixed-precision phase: times,
@@ -264,6 +269,124 @@ def run_symmetry() -> dict:
     return out
 
 
+def _dm_args(kern, P0, n_e):
+    """solve()'s positional arguments of _native.solve_dm (default SolverConfig)."""
+    from cpp_hf_amd import SolverConfig
+    from cpp_hf_amd.solver import _kernel_args_for_native
+
+    c = SolverConfig()
+    return (_kernel_args_for_native(kern), np.ascontiguousarray(P0, dtype=np.complex128),
+            float(n_e), int(c.max_iter), float(c.tol_E), float(c.tol_grad), float(c.max_step),
+            float(c.bt_shrink), float(c.denom_scale), int(c.bt_max), int(c.cg_restart),
+            int(c.plateau_window), int(c.mu_maxiter), 0, float(c.tr_delta0), int(c.tr_cg_max),
+            np.empty((0,) + kern.h.shape, dtype=np.complex128), 0.0, 1.0, [], None, False,
+            False, False)
+
+
+def _scf_args(kern, P0, n_e, mixing=0.5):
+    """solve_scf()'s positional arguments of _native.solve_scf (SCFConfig(mixing))."""
+    from cpp_hf_amd import SCFConfig
+    from cpp_hf_amd.solver import _kernel_args_for_native
+
+    c = SCFConfig(mixing=mixing)
+    return (_kernel_args_for_native(kern), np.ascontiguousarray(P0, dtype=np.complex128),
+            float(n_e), int(c.max_iter), float(c.density_tol), float(c.comm_tol),
+            float(c.mixing), float(c.level_shift), None, [], c.acceleration, int(c.diis_size),
+            int(c.diis_start), float(c.diis_damping), float(c.trust_radius), False, False)
+
+
+def run_solves(cpu: bool) -> dict:
+    """DM and SCF solves at C1 and C2 on the GPU (and in the oracle), with
+    their roofline times."""
+    from cpp_hf_amd._compat import _native
+    from cpp_hf_amd.workloads import config2, qwz_kernel
+
+    out = {}
+    for name, wl in (("c1_qwz_24x24_nb2", qwz_kernel(24)), ("c2_layered_96x96_nb8", config2(96))):
+        kern = wl.kernel
+        nk = kern.h.shape[0] * kern.h.shape[1]
+        nb = kern.h.shape[-1]
+        A = 16 * nk * nb * nb
+        dm_args = _dm_args(kern, wl.P0, wl.n_electrons)
+        scf_args = _scf_args(kern, wl.P0, wl.n_electrons)
+        _native.solve_dm(*dm_args)  # pages in this mesh's code objects
+        t = time.perf_counter()
+        r = _native.solve_dm(*dm_args, return_stats=True)
+        t_dm = time.perf_counter() - t
+        n_it, trials = int(r[6]), int(r[10]["n_ls_trials"])
+        dm_bytes = (30 * n_it + 2 * trials) * A
+        t = time.perf_counter()
+        s_ = _native.solve_scf(*scf_args)
+        t_scf = time.perf_counter() - t
+        scf_bytes = 21 * int(s_[4]) * A
+        rec = {
+            "dm": {"seconds": t_dm, "n_iter": n_it, "ls_trials": trials,
+                   "converged": bool(r[7]), "energy": float(r[5]),
+                   "roofline_seconds": dm_bytes / (HBM_PEAK_GBS * 1e9),
+                   "roofline_frac": dm_bytes / (HBM_PEAK_GBS * 1e9) / t_dm},
+            "scf": {"seconds": t_scf, "iterations": int(s_[4]), "converged": bool(s_[5]),
+                    "energy": float(s_[2]),
+                    "roofline_seconds": scf_bytes / (HBM_PEAK_GBS * 1e9),
+                    "roofline_frac": scf_bytes / (HBM_PEAK_GBS * 1e9) / t_scf},
+        }
+        if cpu:
+            from oracle import hfcpu
+
+            t = time.perf_counter()
+            rc = hfcpu.solve_dm(*dm_args)
+            rec["dm"]["cpu_oracle_seconds"] = time.perf_counter() - t
+            rec["dm"]["cpu_oracle_n_iter"] = int(rc[6])
+            t = time.perf_counter()
+            sc = hfcpu.solve_scf(*scf_args)
+            rec["scf"]["cpu_oracle_seconds"] = time.perf_counter() - t
+            rec["scf"]["cpu_oracle_iterations"] = int(sc[4])
+        out[name] = rec
+    if cpu:
+        from oracle import hfcpu
+
+        from cpp_hf_amd.workloads import sweep_params, sweep_workload
+
+        ws = [sweep_workload(p) for p in sweep_params(48, seed=1)[:4]]
+        t = time.perf_counter()
+        for w in ws:
+            hfcpu.solve_dm(*_dm_args(w.kernel, w.P0, w.n_electrons))
+        dt = time.perf_counter() - t
+        out["c3_cpu_oracle_sample"] = {
+            "solves_per_s": len(ws) / dt,
+            "sample": f"the first {len(ws)} of the 48 sweep kernels "
+                      f"({', '.join(str(w.kernel.h.shape[0]) + 'x' + str(w.kernel.h.shape[0]) + ' nb' + str(w.kernel.h.shape[-1]) for w in ws)}), "
+                      "sequential DM solves",
+            "threads": hfcpu.num_threads(),
+        }
+    return out
+
+
+def cpu_one_thread_builds(seconds: float) -> dict:
+    """The oracle's C2 Fock builds on one host thread (a child process with
+    OMP_NUM_THREADS=1: the thread count is fixed when OpenMP starts)."""
+    import subprocess
+
+    code = (
+        "import sys, time, json, numpy as np; sys.path.insert(0, %r)\n"
+        "from cpp_hf_amd.workloads import config2\n"
+        "from cpp_hf_amd.solver import _kernel_args_for_native\n"
+        "from oracle import hfcpu\n"
+        "wl = config2(96); ka = _kernel_args_for_native(wl.kernel)\n"
+        "P = 0.5 * (wl.P0 + np.conj(np.swapaxes(wl.P0, -1, -2)))\n"
+        "hfcpu.build_fock_apply(ka, P); n = 0; t = time.perf_counter()\n"
+        "while time.perf_counter() - t < %r:\n"
+        "    hfcpu.build_fock_apply(ka, P); n += 1\n"
+        "dt = time.perf_counter() - t\n"
+        "print(json.dumps({'value': n / dt, 'builds': n, 'seconds': dt, "
+        "'threads': hfcpu.num_threads()}))\n") % (str(ROOT), float(seconds))
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=120)
+    if r.returncode != 0:
+        return {"error": r.stderr[-500:]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def run_sweep(concurrency: int) -> dict:
     """48-kernel parameter sweep (meshes 36..120, nb 2..12, seed-1 draws of
     u, interaction scale, screening length and filling): the DM solves one by
@@ -342,7 +465,7 @@ def main() -> int:
 
     import torch
 
-    from cpp_hf_amd import SCFConfig, SolverConfig, solve, solve_scf
+
     from cpp_hf_amd._compat import _native, native_required
     from cpp_hf_amd.solver import _kernel_args_for_native
     from cpp_hf_amd.workloads import config2
@@ -446,19 +569,7 @@ def main() -> int:
     out["autotune"] = {"default_us_per_build": per_build_s * 1e6, **tuned}
 
     if not args.no_solves:
-        solves = {}
-        try:
-            t = time.perf_counter()
-            r = solve(kern, wl.P0, wl.n_electrons, config=SolverConfig())
-            solves["dm"] = {"seconds": time.perf_counter() - t, "n_iter": int(r.n_iter),
-                            "energy": float(r.energy), "converged": bool(r.converged)}
-            t = time.perf_counter()
-            s = solve_scf(kern, wl.P0, wl.n_electrons, config=SCFConfig(mixing=0.5))
-            solves["scf"] = {"seconds": time.perf_counter() - t, "iterations": s.iterations,
-                             "energy": float(s.energy), "converged": bool(s.converged)}
-        except Exception as exc:
-            solves["error"] = repr(exc)
-        out["solves"] = solves
+        out["solves"] = run_solves(not args.no_cpu_baseline)
 
     if not args.no_sweep:
         out["sweep"] = run_sweep(args.sweep_concurrency)
@@ -491,6 +602,7 @@ def main() -> int:
             "kind": "port",
             "sample": f"{n_done} Fock builds (+ energy) of the same workload in {dt:.1f} s, "
                       "OpenMP over k / channels, serial reductions",
+            "one_thread": cpu_one_thread_builds(min(args.cpu_seconds, 5.0)),
         }
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
 

@@ -670,7 +670,8 @@ PYBIND11_MODULE(_native, m) {
                          size_t tr_cg_max, CArr deflation_targets, double deflation_sigma,
                          double deflation_length, std::vector<size_t> block_sizes,
                          py::object project_fn, bool return_Q, bool return_density,
-                         bool return_fock, bool exchange_fp32, double fp32_switch) {
+                         bool return_fock, bool exchange_fp32, double fp32_switch,
+                         bool return_stats) {
         KernelArgs ka(kernel_args);
         if (P0.ndim() != 4 || (size_t)P0.shape(0) != ka.d.nk1 ||
             (size_t)P0.shape(1) != ka.d.nk2 || (size_t)P0.shape(2) != ka.d.nb ||
@@ -740,9 +741,15 @@ PYBIND11_MODULE(_native, m) {
         if (return_Q) q_obj = Q;
         if (return_density) d_obj = dens;
         if (return_fock) f_obj = fock;
-        return py::make_tuple(q_obj, p, d_obj, f_obj, res.mu, res.energy, res.n_iter,
-                              (bool)res.converged, hE[py::slice(0, n, 1)].attr("copy")(),
-                              hG[py::slice(0, n, 1)].attr("copy")());
+        py::tuple out = py::make_tuple(q_obj, p, d_obj, f_obj, res.mu, res.energy, res.n_iter,
+                                       (bool)res.converged,
+                                       hE[py::slice(0, n, 1)].attr("copy")(),
+                                       hG[py::slice(0, n, 1)].attr("copy")());
+        if (!return_stats) return out;
+        // an addition of this module: the reference returns the ten entries
+        py::dict stats;
+        stats["n_ls_trials"] = res.n_ls_trials;
+        return py::tuple(out + py::make_tuple(stats));
     },
           py::arg("kernel_args"), py::arg("P0"), py::arg("n_e"), py::arg("max_iter"),
           py::arg("tol_E"), py::arg("tol_grad"), py::arg("max_step"), py::arg("bt_shrink"),
@@ -752,7 +759,7 @@ PYBIND11_MODULE(_native, m) {
           py::arg("deflation_sigma"), py::arg("deflation_length"), py::arg("block_sizes"),
           py::arg("project_fn"), py::arg("return_Q"), py::arg("return_density"),
           py::arg("return_fock"), py::arg("exchange_fp32") = false,
-          py::arg("fp32_switch") = 1e-6);
+          py::arg("fp32_switch") = 1e-6, py::arg("return_stats") = false);
 
     // --- building blocks of the Python layer ---
     m.def("hf_energy", [](CArr P, CArr h, CArr Sigma, CArr H, RArr w, py::object refP) {

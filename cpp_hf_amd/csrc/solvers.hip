@@ -527,6 +527,7 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
     mu_solve(eps.as<double>(), false, 0.0);
 
     std::vector<double> hist_E, hist_G;
+    size_t ls_trials = 0;
     double grad_norm = std::numeric_limits<double>::infinity();
     double E_prev = std::numeric_limits<double>::infinity();
     double dE = std::numeric_limits<double>::infinity();
@@ -571,6 +572,15 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
         grad_norm = std::sqrt(std::max(0.0, hsc[SC_GN2]));
         hist_E.push_back(E);
         hist_G.push_back(grad_norm);
+        // trials the reference's sequential search evaluates: up to and
+        // including the accepted tau0 shrink^j, all bt_max when none is
+        if (hsc[SC_ACCEPTED] > 0.5 && hsc[SC_TAU0] > 0.0 && cfg.bt_shrink > 0.0 &&
+            cfg.bt_shrink < 1.0) {
+            const double j = std::log(hsc[SC_TAU] / hsc[SC_TAU0]) / std::log(cfg.bt_shrink);
+            ls_trials += static_cast<size_t>(std::llround(std::max(0.0, j))) + 1;
+        } else {
+            ls_trials += cfg.bt_max;
+        }
         if (mixed && grad_norm <= fp32_switch) mixed = false;
         dE = std::fabs(E - E_prev);
         std::swap(dQ, dQp);
@@ -605,6 +615,7 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
     out->energy = hsc[SC_E];
     out->n_iter = k;
     out->converged = k < cfg.max_iter ? 1 : 0;
+    out->n_ls_trials = ls_trials;
     for (size_t i = 0; i < k; ++i) {
         if (out->hist_E) out->hist_E[i] = hist_E[i];
         if (out->hist_grad) out->hist_grad[i] = hist_G[i];
@@ -862,6 +873,7 @@ void solve_rtr_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_confi
     out->energy = hsc[SC_E];
     out->n_iter = k;
     out->converged = converged ? 1 : 0;
+    out->n_ls_trials = 0;
     for (size_t i = 0; i < k; ++i) {
         if (out->hist_E) out->hist_E[i] = hist_E[i];
         if (out->hist_grad) out->hist_grad[i] = hist_G[i];

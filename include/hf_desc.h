@@ -210,6 +210,10 @@ typedef struct hf_dm_result {
     int converged;
     double* hist_E;
     double* hist_grad;
+    /* line-search trials evaluated over the CG solve (frozen free-energy
+     * evaluations of solver_dm.hpp:561-589); 0 for the Newton solver.  No
+     * counterpart in the reference's result. */
+    size_t n_ls_trials;
 } hf_dm_result;
 
 #ifdef __cplusplus

@@ -1137,6 +1137,7 @@ void cayley_spectral_setup_blocks(const c64* d, c64* V, double* lam, size_t nk, 
 // solver_dm.hpp:392-653 (optimizer "cg")
 void solve_dm(const Kernel& K, const c64* P0, double n_e, const hf_dm_config& cfg,
               const Projector* proj, hf_dm_result* out) {
+    size_t ls_trials = 0;  // frozen free-energy evaluations of the line search
     const size_t nk1 = K.nk1, nk2 = K.nk2, nb = K.nb, nk = nk1 * nk2, nb2 = nb * nb;
     const size_t n_tot = nk * nb2, nkb = nk * nb;
     (void)nk2;
@@ -1254,6 +1255,7 @@ void solve_dm(const Kernel& K, const c64* P0, double n_e, const hf_dm_config& cf
         double tau = tau0;
         bool ok = false;
         for (size_t bt = 0; bt < cfg.bt_max; ++bt) {
+            ++ls_trials;
             diag_UFU_from_spectrum(V_d.data(), Sigma.data(), lam_d.data(), tau,
                                    eps_work.data(), nk, nb);
             const double Om = frozen_free_energy(eps_work.data(), p.data(), d_p.data(), tau,
@@ -1309,6 +1311,7 @@ void solve_dm(const Kernel& K, const c64* P0, double n_e, const hf_dm_config& cf
     out->energy = E_fin;
     out->n_iter = k;
     out->converged = k < cfg.max_iter ? 1 : 0;
+    out->n_ls_trials = ls_trials;
     if (out->hist_E) std::memcpy(out->hist_E, hist_E.data(), k * sizeof(double));
     if (out->hist_grad) std::memcpy(out->hist_grad, hist_G.data(), k * sizeof(double));
 }

@@ -92,6 +92,7 @@ class DmResult(C.Structure):
         ("Q", _dp), ("p", _dp), ("density", _dp), ("fock", _dp),
         ("mu", C.c_double), ("energy", C.c_double), ("n_iter", _sz),
         ("converged", C.c_int), ("hist_E", _dp), ("hist_grad", _dp),
+        ("n_ls_trials", _sz),
     ]
 
 
@@ -380,7 +381,8 @@ def solve_scf(kernel_args, P0, n_e, max_iter, density_tol, comm_tol, mixing, lev
 def solve_dm(kernel_args, P0, n_e, max_iter, tol_E, tol_grad, max_step, bt_shrink,
              denom_scale, bt_max, cg_restart, plateau_window, mu_maxiter, optimizer,
              tr_delta0, tr_cg_max, deflation_targets, deflation_sigma, deflation_length,
-             block_sizes, project_fn, return_Q, return_density, return_fock):
+             block_sizes, project_fn, return_Q, return_density, return_fock,
+             return_stats=False):
     if int(optimizer) not in (0, 1):
         raise ValueError("optimizer code must be 0 (cg) or 1 (newton)")
     d = _Desc(kernel_args)
@@ -409,9 +411,10 @@ def solve_dm(kernel_args, P0, n_e, max_iter, tol_E, tol_grad, max_step, bt_shrin
     _check(lib().hfcpu_solve_dm(C.byref(d.desc), _p(P0), C.c_double(n_e), C.byref(cfg),
                                 cb, None, C.byref(res)))
     n = int(res.n_iter)
-    return (Q if return_Q else None, p, dens if return_density else None,
-            fock if return_fock else None, res.mu, res.energy, n, bool(res.converged),
-            hE[:n].copy(), hG[:n].copy())
+    out = (Q if return_Q else None, p, dens if return_density else None,
+           fock if return_fock else None, res.mu, res.energy, n, bool(res.converged),
+           hE[:n].copy(), hG[:n].copy())
+    return out + ({"n_ls_trials": int(res.n_ls_trials)},) if return_stats else out
 
 
 def selfenergy_superlattice_streamed(rho, VR_fft, g_a_off, pair_i, pair_j, pair_start, nkx,

@@ -406,3 +406,19 @@ def test_oracle_block_sizes_validation(oracle):
     a = _scf_b(oracle, k, 2.0, [3, 2])
     b = _scf_b(oracle, k, 2.0, [])
     np.testing.assert_array_equal(a[0], b[0])
+
+
+def test_oracle_dm_counts_line_search_trials(oracle):
+    # one frozen free-energy evaluation per trial of the backtracking loop
+    # (solver_dm.hpp:561-589): at least one per CG iteration, at most bt_max
+    from helpers import kernel_args_dict, two_band_kernel
+
+    k = two_band_kernel(nk=2, T=0.1)
+    args = (kernel_args_dict(k), np.zeros_like(k.h), 4.0, 50, 1e-9, 0.0, 0.6, 0.5, 1e-3, 8, 10,
+            5, 25, 0, 0.5, 20, np.empty((0,) + k.h.shape), 0.0, 1.0, [], None, True, True,
+            True)
+    r = oracle.solve_dm(*args, return_stats=True)
+    assert len(r) == 11
+    n = r[6]
+    assert n <= r[10]["n_ls_trials"] <= 8 * n
+    assert len(oracle.solve_dm(*args)) == 10
