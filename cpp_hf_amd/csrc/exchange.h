@@ -56,8 +56,6 @@ struct ExchangeGeom {
     // wavefronts sharing one channel tile in the two-step passes (1 or 2):
     // rows forward, columns, rows inverse
     int split_fwd = 2, split_col = 2, split_inv = 1;
-    // streaming stores for the direct entries of the Fock epilogue (0 or 1)
-    int epi_stream = 0;
     int* d_chan_a = nullptr;  // nch
     int* d_chan_b = nullptr;  // nch
     AxisPlan ax1, ax2;        // along k1 (nk1) and along k2 (nk2)

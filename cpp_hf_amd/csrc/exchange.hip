@@ -163,9 +163,6 @@ void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
     split_inv = 1;
     if (const char* e = std::getenv("HFGPU_SPLIT"))
         split_fwd = split_col = split_inv = std::atoi(e) == 2 ? 2 : 1;
-    // streaming stores of the direct Sigma / F entries in the Fock epilogue
-    epi_stream = 0;
-    if (const char* e = std::getenv("HFGPU_EPI_STREAM")) epi_stream = e[0] == '1' ? 1 : 0;
     if (const char* e = std::getenv("HFGPU_ROW_WAVES")) row_waves = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("HFGPU_COL_WAVES")) {
         const int v = std::atoi(e);
@@ -880,8 +877,7 @@ __global__ void __launch_bounds__(64 * kFastRowWavesMax)
 k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, int packed,
                 const TS* __restrict__ T, int epilogue, int shift, double inv_n,
                 cd* __restrict__ Sigma, cd* __restrict__ F, const cd* __restrict__ hh,
-                const cd* __restrict__ SC, const double* __restrict__ hdiag, int rpw,
-                int stream_direct) {
+                const cd* __restrict__ SC, const double* __restrict__ hdiag, int rpw) {
     using Sh = FastShape<N1, N2, S>;
     constexpr int N = Sh::N, L = Sh::L;
     extern __shared__ cd smem[];
@@ -981,18 +977,10 @@ k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
                 const cd s_ba = cadd(a == b ? sig : cconj(sig), sc_ba);
                 cd f{hv[r][i2].x + 0.5 * (s_ab.x + s_ba.x), hv[r][i2].y + 0.5 * (s_ab.y - s_ba.y)};
                 f.x += hd;
-                // direct entries fill runs of a row: streaming stores when
-                // the geometry asks for them (faster for large meshes,
-                // slower where tiles straddle 128 B lines at small ones);
-                // the mirrored entries are single 16 B pieces of 128 B
-                // lines, which streaming stores handle badly
-                if (stream_direct) {
-                    store_stream(Sigma + iab, s_ab);
-                    store_stream(F + iab, f);
-                } else {
-                    Sigma[iab] = s_ab;
-                    F[iab] = f;
-                }
+                // regular stores: the mirrored entries are single 16 B
+                // pieces of 128 B lines, which streaming stores handle badly
+                Sigma[iab] = s_ab;
+                F[iab] = f;
                 if (a != b) {
                     Sigma[iba] = s_ba;
                     F[iba] = cconj(f);
@@ -1421,22 +1409,22 @@ static void launch_rows_inv_fast(const ExchangeGeom& g, const ExchangeArgs& a, d
             hipLaunchKernelGGL((k_rows_inv_fast<A, B, cf, 2>), grid, block, lds, st,          \
                                g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0,      \
                                a.T32, a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F,   \
-                               a.hh, a.SC, a.hdiag, rpw, g.epi_stream);                       \
+                               a.hh, a.SC, a.hdiag, rpw);                                     \
         else if (a.ws32)                                                                      \
             hipLaunchKernelGGL((k_rows_inv_fast<A, B, cf, 1>), grid, block, lds, st,          \
                                g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0,      \
                                a.T32, a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F,   \
-                               a.hh, a.SC, a.hdiag, rpw, g.epi_stream);                       \
+                               a.hh, a.SC, a.hdiag, rpw);                                     \
         else if (g.split_inv == 2)                                                            \
             hipLaunchKernelGGL((k_rows_inv_fast<A, B, cd, 2>), grid, block, lds, st,          \
                                g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.T, \
                                a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh,    \
-                               a.SC, a.hdiag, rpw, g.epi_stream);                             \
+                               a.SC, a.hdiag, rpw);                                           \
         else                                                                                  \
             hipLaunchKernelGGL((k_rows_inv_fast<A, B, cd, 1>), grid, block, lds, st,          \
                                g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.T, \
                                a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh,    \
-                               a.SC, a.hdiag, rpw, g.epi_stream);                             \
+                               a.SC, a.hdiag, rpw);                                           \
         HFGPU_LAUNCH_CHECK();                                                                 \
         return;                                                                               \
     }

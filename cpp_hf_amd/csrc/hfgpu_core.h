@@ -114,13 +114,12 @@ struct GpuKernel {
     ~GpuKernel();
 
     // Launch-shape autotuner of the two-step exchange passes: coordinate
-    // descent over wavefronts per tile (split), tiles per workgroup and the
-    // epilogue's store kind,
+    // descent over wavefronts per tile (split) and tiles per workgroup,
     // timing whole Fock builds with HIP events; the best shape is kept in
     // the geometry and in a per-process cache keyed by the build's shape.
     // Launch shapes only move tasks between lanes: results are unchanged.
     // Returns the best microseconds per build (0 when nothing is tunable).
-    double autotune(bool p_hermitian, int shape_out[6]);
+    double autotune(bool p_hermitian, int shape_out[5]);
     bool autotune_env = false;  // HFGPU_AUTOTUNE=1: tune each new geometry
 
     bool packed_for(bool p_hermitian) const { return hcp || (dense_packable && p_hermitian); }

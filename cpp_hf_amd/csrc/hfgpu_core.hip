@@ -341,11 +341,11 @@ GpuKernel::~GpuKernel() {
 namespace {
 
 struct LaunchShape {
-    int row_waves, col_waves, split_fwd, split_col, split_inv, epi_stream;
+    int row_waves, col_waves, split_fwd, split_col, split_inv;
 };
 
 LaunchShape shape_of(const ExchangeGeom& g) {
-    return {g.row_waves, g.col_waves, g.split_fwd, g.split_col, g.split_inv, g.epi_stream};
+    return {g.row_waves, g.col_waves, g.split_fwd, g.split_col, g.split_inv};
 }
 
 void set_shape(ExchangeGeom& g, const LaunchShape& s) {
@@ -354,7 +354,6 @@ void set_shape(ExchangeGeom& g, const LaunchShape& s) {
     g.split_fwd = s.split_fwd;
     g.split_col = s.split_col;
     g.split_inv = s.split_inv;
-    g.epi_stream = s.epi_stream;
 }
 
 // (nk1, nk2, nb, packed, dense V, small terms)
@@ -393,7 +392,7 @@ GpuKernel::Geom& GpuKernel::geom(bool packed) {
                 }
             }
             if (!cached) {
-                int shape[6];
+                int shape[5];
                 (void)autotune(packed, shape);
             }
         }
@@ -406,7 +405,7 @@ GpuKernel::Geom& GpuKernel::geom(bool packed) {
 // ---------------------------------------------------------------------------
 
 
-double GpuKernel::autotune(bool p_hermitian, int shape_out[6]) {
+double GpuKernel::autotune(bool p_hermitian, int shape_out[5]) {
     const bool packed = exchange && packed_for(p_hermitian);
     Geom& G = geom(packed);
     ExchangeGeom& g = *G.g;
@@ -416,7 +415,6 @@ double GpuKernel::autotune(bool p_hermitian, int shape_out[6]) {
         shape_out[2] = s.split_fwd;
         shape_out[3] = s.split_col;
         shape_out[4] = s.split_inv;
-        shape_out[5] = s.epi_stream;
     };
     report(shape_of(g));
     if (!exchange || superlattice() || !(g.fast_rows && g.fast_cols)) return 0.0;
@@ -482,11 +480,6 @@ double GpuKernel::autotune(bool p_hermitian, int shape_out[6]) {
         LaunchShape s = best;
         s.split_inv = si;
         if (si != best.split_inv) consider(s);
-    }
-    {
-        LaunchShape s = best;
-        s.epi_stream = 1 - best.epi_stream;
-        consider(s);
     }
     for (int rw : {1, 2, 0}) {
         if (rw == 2 && g.nblk < 3) continue;

@@ -947,7 +947,7 @@ PYBIND11_MODULE(_native, m) {
             py::arg("p_hermitian") = true, py::arg("stream") = 0)
         .def("autotune",
              [](PyKernel& k, bool p_hermitian) {
-                 int shape[6] = {0, 0, 0, 0, 0, 0};
+                 int shape[5] = {0, 0, 0, 0, 0};
                  double us = 0.0;
                  int rc;
                  {
@@ -962,7 +962,6 @@ PYBIND11_MODULE(_native, m) {
                  d["split_fwd"] = shape[2];
                  d["split_col"] = shape[3];
                  d["split_inv"] = shape[4];
-                 d["epi_stream"] = shape[5];
                  return d;
              },
              py::arg("p_hermitian") = true)

@@ -59,11 +59,10 @@ int hfgpu_kernel_packable(const hfgpu_kernel* k);
  * build of this kernel (Hermitian or general density) over the two-step
  * exchange's launch shapes and keeps the fastest for this kernel and for
  * later kernels of the same shape in the process (HFGPU_AUTOTUNE=1 does this
- * on every new geometry).  shape: 6 ints (tiles per row workgroup, 0 = whole
+ * on every new geometry).  shape: 5 ints (tiles per row workgroup, 0 = whole
  * row; tiles per column workgroup; wavefronts per tile in the forward-row,
- * column and inverse-row passes; 1 for streaming stores of the epilogue's
- * direct entries); best_us: microseconds per build (0 when the mesh has no
- * two-step transform).  Results of builds do not change. */
+ * column and inverse-row passes); best_us: microseconds per build (0 when
+ * the mesh has no two-step transform).  Results of builds do not change. */
 int hfgpu_kernel_autotune(hfgpu_kernel* k, int p_hermitian, int* shape, double* best_us);
 
 /* ------------------------------------------------------------------------
