@@ -8,11 +8,15 @@
 // 9216 matrices of 8 x 8 it takes 72 us against 168 us for a register-row
 // parallel Jacobi: the QR phase needs ~17 shifted steps of at most 7
 // rotations, cyclic Jacobi ~56 serial rounds of full-row updates.
-// nb 17..32: parallel cyclic Jacobi in LDS (k_eigh_jacobi), one wavefront per
-// matrix: each round applies the nb/2 disjoint complex rotations of the
-// circle ordering at once, every lane updating its own elements
-// A'_ij = sum conj(J_ki) A_kl J_lj; sweeps stop once the off-diagonal mass
-// falls to (4 nb eps)^2 ||A||_F^2.
+// nb 17..32: the same kernel with NP = 32 and the eigenvector rows in LDS
+// (registers for both A and Z would spill): at 576 matrices of 26 x 26 it
+// takes 542 us against 1281 us for the parallel cyclic Jacobi below, at
+// 9216 of 32 x 32 4.3 ms against 19.1 ms.
+// HFGPU_EIGH_JACOBI=1 selects the Jacobi kernel (k_eigh_jacobi), one
+// wavefront per matrix: each round applies the nb/2 disjoint complex
+// rotations of the circle ordering at once, every lane updating its own
+// elements A'_ij = sum conj(J_ki) A_kl J_lj; sweeps stop once the
+// off-diagonal mass falls to (4 nb eps)^2 ||A||_F^2.
 #include <algorithm>
 #include <cstdlib>
 #include <stdexcept>
@@ -231,6 +235,20 @@ struct TriLds {
     double d[64];
 };
 
+// Row i of the eigenvector accumulator Z: registers, or (NP = 32, where
+// registers for both A and Z would spill) a row of a padded LDS array that
+// only its own lane touches.
+template <int NL, bool IN_LDS>
+struct ZRow {
+    cd v[NL];
+    __device__ __forceinline__ cd& operator[](int j) { return v[j]; }
+};
+template <int NL>
+struct ZRow<NL, true> {
+    cd* p;
+    __device__ __forceinline__ cd& operator[](int j) { return p[j]; }
+};
+
 template <int NP>
 __device__ __forceinline__ cd group_bcast(cd v, int src) {
     return cd{__shfl(v.x, src, 64), __shfl(v.y, src, 64)};
@@ -258,7 +276,14 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
     const bool live = mat < nmat;
     const int nb2 = nb * nb;
     const cd* Mk = M + static_cast<size_t>(live ? mat : 0) * nb2;
-    cd A[NL], Z[NL];
+    constexpr bool kZLds = NP == 32;
+    cd A[NL];
+    ZRow<NL, kZLds> Z;
+    if constexpr (kZLds) {
+        // dynamic LDS: G matrices per wavefront, NP rows of NP + 1 each
+        extern __shared__ cd zlds[];
+        Z.p = zlds + ((threadIdx.x >> 6) * 64 + lane) * (NP + 1);
+    }
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
         cd h{0.0, 0.0};
@@ -472,6 +497,20 @@ void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, c
             hipLaunchKernelGGL((k_eigh_tri<16, 12>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
         else
             hipLaunchKernelGGL((k_eigh_tri<16, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
+        HFGPU_LAUNCH_CHECK();
+        return;
+    }
+    static const bool use_jacobi = [] {
+        const char* e = std::getenv("HFGPU_EIGH_JACOBI");
+        return e && e[0] == '1';
+    }();
+    if (!use_jacobi) {
+        // 32 lanes per matrix, two matrices per wavefront, one wavefront per
+        // workgroup: Z rows in LDS (64 rows of 33 complex)
+        const dim3 grid(cdiv(nmat, 2));
+        const size_t zl = sizeof(cd) * 64 * 33;
+        hipLaunchKernelGGL((k_eigh_tri<32, 0>), grid, dim3(64), zl, st, nmat, nb, in_mode, M, w,
+                           V, bm, so, cond);
         HFGPU_LAUNCH_CHECK();
         return;
     }
