@@ -160,6 +160,15 @@ void launch_to_cf(size_t n, const cd* in, cf* out, hipStream_t st);
 // geometry (forward exp(-2 pi i / n) or inverse); in == out allowed.
 void launch_fft2_plain(const ExchangeGeom& g, const cd* in, cd* out, int nch, bool inverse,
                        hipStream_t st);
+// In-place 2-D convolution of nch <= g.nch natural-layout channels: rows
+// forward, columns forward x scale V x inverse (V[q * period + ch % period];
+// channels whose flags[ch / per_slot] is 0 give exactly zero), rows inverse.
+// rowmask (nullable, (channel tiles, nk1) ints): row lines marked 0 are
+// skipped by both row passes: they must be zero on input (forward) and are
+// not read afterwards (inverse).
+void launch_fft2_conv_plain(const ExchangeGeom& g, cd* X, int nch, const cd* V, int period,
+                            const int* flags, int per_slot, double scale, const int* rowmask,
+                            hipStream_t st);
 // out = scale A conj(B) per element, or summed over channels per q.
 void launch_cross_spectrum(size_t nq, int nch, const cd* A, const cd* B, double scale,
                            bool sum_channels, cd* out, hipStream_t st);

@@ -1127,11 +1127,12 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
 // transforms unnormalised (forward exp(-2 pi i / n)).
 // ---------------------------------------------------------------------------
 
-// Along k2 (rows: one k1 line and one channel tile per workgroup).
+// Along k2 (rows: one k1 line and one channel tile per workgroup).  With a
+// row mask, lines whose rowmask[blk * nk1 + k1] is 0 are left untouched.
 template <bool GENERIC, int S>
 __global__ void __launch_bounds__(kFftThreads)
 k_plain_rows(FftAxis ax2, int nk1, int nk2, int nch, int clog, int nblk, int inv,
-             const cd* __restrict__ in, cd* __restrict__ out) {
+             const cd* __restrict__ in, cd* __restrict__ out, const int* __restrict__ rowmask) {
     extern __shared__ cd smem[];
     const int C = 1 << clog;
     const int tot = nk2 << clog;
@@ -1139,6 +1140,7 @@ k_plain_rows(FftAxis ax2, int nk1, int nk2, int nch, int clog, int nblk, int inv
     cd* s_tw = smem + tot;
     int k1, blk;
     tile_of(static_cast<int>(blockIdx.x), nk1, nblk, k1, blk);
+    if (rowmask && rowmask[static_cast<size_t>(blk) * nk1 + k1] == 0) return;  // uniform
     const int ch = blk * C + (threadIdx.x & (C - 1));
     const bool valid = ch < nch;
     const size_t base = static_cast<size_t>(k1) * nk2 * nch + ch;
@@ -1188,6 +1190,54 @@ k_plain_cols(FftAxis ax1, int nk1, int nk2, int nch, int clog, int nblk, int inv
         fft_tile<true, GENERIC>(s, s_tw, ax1, clog);
     else
         fft_tile<false, GENERIC>(s, s_tw, ax1, clog);
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot && valid) X[base + (e >> clog) * stride] = s[e];
+    }
+}
+
+// Along k1 (columns), in place: forward transform, times scale V, inverse
+// transform (the convolution's column half; one HBM read and write).  V is
+// V[q * period + ch % period] at q = q1 nk2 + k2; channels whose flag
+// (flags[ch / per_slot]) is 0 come out exactly zero.
+template <bool GENERIC, int S>
+__global__ void __launch_bounds__(kFftThreads)
+k_plain_cols_conv(FftAxis ax1, int nk1, int nk2, int nch, int clog, int nblk,
+                  cd* __restrict__ X, const cd* __restrict__ V, int period,
+                  const int* __restrict__ flags, int per_slot, double scale) {
+    extern __shared__ cd smem[];
+    const int C = 1 << clog;
+    const int tot = nk1 << clog;
+    cd* s = smem;
+    cd* s_tw = smem + tot;
+    int k2, blk;
+    tile_of(static_cast<int>(blockIdx.x), nk2, nblk, k2, blk);
+    const int ch = blk * C + (threadIdx.x & (C - 1));
+    const bool valid = ch < nch;
+    const bool live = valid && flags[valid ? ch / per_slot : 0] != 0;
+    const size_t base = static_cast<size_t>(k2) * nch + ch;
+    const size_t stride = static_cast<size_t>(nk2) * nch;
+    load_twiddles(s_tw, ax1.tw, nk1);
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot) s[e] = valid ? X[base + (e >> clog) * stride] : cd{0.0, 0.0};
+    }
+    __syncthreads();
+    fft_tile<false, GENERIC>(s, s_tw, ax1, clog);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot) {
+            const size_t q = static_cast<size_t>(e >> clog) * nk2 + k2;
+            s[e] = live ? cscale(cmul(s[e], V[q * period + (valid ? ch % period : 0)]), scale)
+                        : cd{0.0, 0.0};
+        }
+    }
+    __syncthreads();
+    fft_tile<true, GENERIC>(s, s_tw, ax1, clog);
 #pragma unroll
     for (int r = 0; r < S; ++r) {
         const int e = threadIdx.x + r * kFftThreads;
@@ -1590,9 +1640,25 @@ void launch_fft2_plain(const ExchangeGeom& g, const cd* in, cd* out, int nch, bo
     if (nch > g.nch) throw std::logic_error("more channels than the geometry holds");
     const int nblk = (nch + g.tile() - 1) / g.tile();
     HFGPU_EXCH_LAUNCH(k_plain_rows, g.generic, g.slots_row, dim3(nblk * g.nk1), g.lds_rows, st,
-                      g.ax2.axis(), g.nk1, g.nk2, nch, g.clog, nblk, inverse ? 1 : 0, in, out);
+                      g.ax2.axis(), g.nk1, g.nk2, nch, g.clog, nblk, inverse ? 1 : 0, in, out,
+                      static_cast<const int*>(nullptr));
     HFGPU_EXCH_LAUNCH(k_plain_cols, g.generic, g.slots_col, dim3(nblk * g.nk2), g.lds_cols, st,
                       g.ax1.axis(), g.nk1, g.nk2, nch, g.clog, nblk, inverse ? 1 : 0, out);
+}
+
+void launch_fft2_conv_plain(const ExchangeGeom& g, cd* X, int nch, const cd* V, int period,
+                            const int* flags, int per_slot, double scale, const int* rowmask,
+                            hipStream_t st) {
+    if (g.packed) throw std::logic_error("plain transforms need a full-channel geometry");
+    if (nch > g.nch) throw std::logic_error("more channels than the geometry holds");
+    const int nblk = (nch + g.tile() - 1) / g.tile();
+    HFGPU_EXCH_LAUNCH(k_plain_rows, g.generic, g.slots_row, dim3(nblk * g.nk1), g.lds_rows, st,
+                      g.ax2.axis(), g.nk1, g.nk2, nch, g.clog, nblk, 0, X, X, rowmask);
+    HFGPU_EXCH_LAUNCH(k_plain_cols_conv, g.generic, g.slots_col, dim3(nblk * g.nk2), g.lds_cols,
+                      st, g.ax1.axis(), g.nk1, g.nk2, nch, g.clog, nblk, X, V, period, flags,
+                      per_slot, scale);
+    HFGPU_EXCH_LAUNCH(k_plain_rows, g.generic, g.slots_row, dim3(nblk * g.nk1), g.lds_rows, st,
+                      g.ax2.axis(), g.nk1, g.nk2, nch, g.clog, nblk, 1, X, X, rowmask);
 }
 
 static unsigned ew_blocks(size_t n) {

@@ -10,9 +10,10 @@
 // on the extended grid, X[px][py][slot][a][b] (slot = transformed dG in
 // order, channel index fastest), and transformed together:
 //   scatter  X = (P - refP) placed at the tile of G_a (zero elsewhere)
-//   forward  2-D FFT of every channel (unnormalised)
-//   multiply X *= -V(q) / (N_ext_x N_ext_y)  (V per orbital pair optional)
-//   inverse  2-D FFT
+//   rows     forward transform along py
+//   columns  forward transform along px, X *= -V(q) / (N_ext_x N_ext_y) (V
+//            per orbital pair optional), inverse transform, in one pass
+//   rows     inverse transform along py
 //   gather   Sigma[k][(G_a, a)][(G_b, b)] = X at the tile of G_a, or, for a
 //            channel filled from its (-dG) partner, conj of the partner's
 //            value at the tile of G_b with (a, b) swapped.
@@ -53,6 +54,8 @@ struct SlPlan {
     DevBuf goff;       // (n_G, 2) int
     DevBuf X;          // grid of one batch
     DevBuf flags;      // (slots_per_batch) int: channel holds a nonzero value
+    DevBuf rowmask;    // (batches, channel tiles, Nx) int: grid row px of a tile holds data
+    int tiles_per_batch = 0;
     std::unique_ptr<ExchangeGeom> geo;  // Nx x Ny plain transform
 
     SlPlan() = default;

@@ -59,23 +59,6 @@ k_sl_scatter(size_t n, int Ny, int nch, int orb, int orb2, int nkx, int nky, int
     }
 }
 
-// X[q][c] *= scale V[q][c mod period]; channels whose input was all zero
-// stay exactly zero (the reference skips them)
-__global__ void __launch_bounds__(256)
-k_sl_mul(size_t n, int nch, int orb2, int period, const cd* __restrict__ V,
-         const int* __restrict__ flags, double scale, cd* __restrict__ X) {
-    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < n;
-         e += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const int c = static_cast<int>(e % nch);
-        const size_t q = e / nch;
-        if (!flags[c / orb2]) {
-            X[e] = cd{0.0, 0.0};
-            continue;
-        }
-        X[e] = cscale(cmul(X[e], V[q * period + c % period]), scale);
-    }
-}
-
 // Sigma entries whose channel lies in slots [s0, s0 + nslots); with first,
 // also the zero entries of pairs that no channel writes
 __global__ void __launch_bounds__(256)
@@ -268,6 +251,30 @@ void SlPlan::init(const SlInput& in, hipStream_t st) {
         while (nbg * nbg < nch) ++nbg;
         geo = std::make_unique<ExchangeGeom>();
         geo->init(Nx, Ny, nbg, false);
+        // rows px of each batch's channel tiles that hold data: the rows of
+        // the tiles that the scatter fills, which are also the only rows
+        // the gather reads (both follow the pairs' G_a tiles)
+        const int C = geo->tile();
+        tiles_per_batch = (nch + C - 1) / C;
+        const int n_batches = (n_slots + slots_per_batch - 1) / slots_per_batch;
+        std::vector<int> mask(static_cast<size_t>(n_batches) * tiles_per_batch * Nx, 0);
+        for (int b = 0; b < n_batches; ++b)
+            for (int t = 0; t < tiles_per_batch; ++t) {
+                int* m = mask.data() + (static_cast<size_t>(b) * tiles_per_batch + t) * Nx;
+                const int c0 = t * C, c1 = std::min(nch, c0 + C);
+                for (int c = c0; c < c1; ++c) {
+                    const int s = b * slots_per_batch + c / orb2;
+                    if (s >= n_slots) break;
+                    const int* tt = tile_tab.data() + static_cast<size_t>(s) * tiles_per;
+                    for (int tx = 0; tx < ntx; ++tx)
+                        for (int ty = 0; ty < nty; ++ty)
+                            if (tt[static_cast<size_t>(tx) * nty + ty] >= 0)
+                                for (int px = tx * nkx; px < std::min(Nx, (tx + 1) * nkx); ++px)
+                                    m[px] = 1;
+                }
+            }
+        rowmask.alloc(mask.size() * sizeof(int));
+        h2d(rowmask.get(), mask.data(), mask.size() * sizeof(int), st);
         tiles.alloc(tile_tab.size() * sizeof(int));
         h2d(tiles.get(), tile_tab.data(), tile_tab.size() * sizeof(int), st);
         X.alloc(nq * nch * sizeof(cd));
@@ -294,11 +301,10 @@ void SlPlan::apply(const cd* P, const cd* refP, const cd* refP0, cd* Sigma, hipS
                                tiles.as<int>() + static_cast<size_t>(s0) * tiles_per, P, refP,
                                refP0, X.as<cd>(), flags.as<int>());
             HFGPU_LAUNCH_CHECK();
-            launch_fft2_plain(*geo, X.as<cd>(), X.as<cd>(), nch, false, st);
-            hipLaunchKernelGGL(k_sl_mul, dim3(grid_for(n)), dim3(256), 0, st, n, nch, orb2, period,
-                               V.as<cd>(), flags.as<int>(), scale, X.as<cd>());
-            HFGPU_LAUNCH_CHECK();
-            launch_fft2_plain(*geo, X.as<cd>(), X.as<cd>(), nch, true, st);
+            launch_fft2_conv_plain(
+                *geo, X.as<cd>(), nch, V.as<cd>(), period, flags.as<int>(), orb2, scale,
+                rowmask.as<int>() + static_cast<size_t>(s0 / slots_per_batch) * tiles_per_batch * Nx,
+                st);
         }
         hipLaunchKernelGGL(k_sl_gather, dim3(grid_for(n_out)), dim3(256), 0, st, n_out, D, orb,
                            orb2, n_G, nkx, nky, Ny, std::max(nch, 1), src.as<SlSrc>(),
