@@ -432,7 +432,7 @@ def main() -> int:
     ap.add_argument("--no-solves", action="store_true")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the 48-kernel sweep (sequential vs concurrent solves)")
-    ap.add_argument("--sweep-concurrency", type=int, default=8)
+    ap.add_argument("--sweep-concurrency", type=int, default=16)
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the 192 x 192 mixed-precision and the 144 x 144 adjoint sections")
     ap.add_argument("--no-superlattice", action="store_true",
