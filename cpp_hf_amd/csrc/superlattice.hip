@@ -391,6 +391,7 @@ extern "C" int hfgpu_selfenergy_superlattice_streamed(
         in.n_G = n_G;
         in.orb = dim_orb;
         in.hermitian = hermitian_rho != 0;
+        check_layout(in);  // before any device call: bad layouts fail on any host
         hipStream_t st = nullptr;
         HFGPU_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         struct StreamGuard {
