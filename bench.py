@@ -16,8 +16,9 @@ The JSON line also carries:
   cpu_baseline  the CPU oracle's Fock build on this host, timed for a
                 bounded number of builds (a restatement of the reference's
                 FFTW/Eigen path, which cannot be built here).
-  solves        wall time of full direct-minimisation and SCF (mixing 0.5)
-                solves at C1 (QWZ 24 x 24) and C2, on the GPU and in the
+  solves        wall time of full direct-minimisation (CG and trust-region
+                Newton) and SCF (mixing 0.5) solves at C1 (QWZ 24 x 24) and
+                C2, on the GPU (Newton: GPU only) and in the
                 oracle on the host, with each solve's roofline time: the
                 reference dataflow's bytes per iteration (DM (30 + 2 trials) A
                 with the line-search trials counted, SCF 21 A) over 8 TB/s;
@@ -319,7 +320,15 @@ def run_solves(cpu: bool) -> dict:
         s_ = _native.solve_scf(*scf_args)
         t_scf = time.perf_counter() - t
         scf_bytes = 21 * int(s_[4]) * A
+        # trust-region Newton (optimizer code 1) from the same start
+        nt_args = dm_args[:13] + (1,) + dm_args[14:]
+        _native.solve_dm(*nt_args)
+        t = time.perf_counter()
+        rn = _native.solve_dm(*nt_args)
+        t_nt = time.perf_counter() - t
         rec = {
+            "newton": {"seconds": t_nt, "n_iter": int(rn[6]), "converged": bool(rn[7]),
+                       "energy": float(rn[5])},
             "dm": {"seconds": t_dm, "n_iter": n_it, "ls_trials": trials,
                    "converged": bool(r[7]), "energy": float(r[5]),
                    "roofline_seconds": dm_bytes / (HBM_PEAK_GBS * 1e9),

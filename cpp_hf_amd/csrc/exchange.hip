@@ -1307,20 +1307,57 @@ k_ksum_partial(int nk, int m, const cd* __restrict__ X, const double* __restrict
     const int k0 = blockIdx.x * chunk, k1 = min(nk, k0 + chunk);
     for (int e = threadIdx.x; e < m; e += blockDim.x) {
         cd s{0.0, 0.0};
-        for (int k = k0; k < k1; ++k) {
-            cd x = X[static_cast<size_t>(k) * m + e];
-            if (w) x = cscale(x, w[k]);
-            s = cadd(s, x);
+        // loads issued eight at a time, summed in k order
+        for (int kb = k0; kb < k1; kb += 8) {
+            cd v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = kb + u;
+                v[u] = cd{0.0, 0.0};
+                if (k < k1) {
+                    v[u] = X[static_cast<size_t>(k) * m + e];
+                    if (w) v[u] = cscale(v[u], w[k]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (kb + u < k1) s = cadd(s, v[u]);
         }
         partial[static_cast<size_t>(blockIdx.x) * m + e] = s;
     }
 }
 
+// out[e] = sum_b partial[b][e]: the partials are split into G = 256 / m
+// contiguous ranges (one per thread group, loads eight at a time), and the
+// group sums are added in group order: deterministic for given nparts, m.
 __global__ void __launch_bounds__(256)
 k_ksum_final(int nparts, int m, const cd* __restrict__ partial, cd* __restrict__ out) {
+    __shared__ cd red[256];
+    const int G = m >= 256 ? 1 : 256 / m;
+    const int span = (nparts + G - 1) / G;
+    for (int t = threadIdx.x; t < G * m; t += blockDim.x) {
+        const int e = t % m, g = t / m;
+        const int b0 = g * span, b1 = min(nparts, b0 + span);
+        cd s{0.0, 0.0};
+        for (int bb = b0; bb < b1; bb += 8) {
+            cd v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                v[u] = bb + u < b1 ? partial[static_cast<size_t>(bb + u) * m + e] : cd{0.0, 0.0};
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (bb + u < b1) s = cadd(s, v[u]);
+        }
+        if (G == 1)
+            out[e] = s;
+        else
+            red[t] = s;
+    }
+    if (G == 1) return;  // uniform over the block
+    __syncthreads();
     for (int e = threadIdx.x; e < m; e += blockDim.x) {
         cd s{0.0, 0.0};
-        for (int b = 0; b < nparts; ++b) s = cadd(s, partial[static_cast<size_t>(b) * m + e]);
+        for (int g = 0; g < G; ++g) s = cadd(s, red[g * m + e]);
         out[e] = s;
     }
 }
