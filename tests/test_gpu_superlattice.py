@@ -151,3 +151,37 @@ def test_solve_dm_superlattice_matches_oracle(gpu, oracle, orbital):
     assert abs(mu - r[4]) < 1e-8
     assert relerr(P, r[2]) < 1e-8
     assert relerr(F, r[3]) < 1e-8
+
+
+def test_build_fock_superlattice_k_dependent_reference(gpu, oracle):
+    # refP that varies with k: the scatter subtracts refP_k entry by entry
+    # (the constant I/2 case goes through the one-matrix path)
+    from cpp_hf_amd.superlattice import SuperlatticeHartreeFockKernel
+
+    k0 = sl_kernel(nk=4, g_cut=1, orb=2, refP=False)
+    rng = np.random.default_rng(21)
+    ref = rand_herm(rng, k0.h.shape) * 0.1 + 0.5 * np.eye(k0.h.shape[-1])
+    k = SuperlatticeHartreeFockKernel(k0.w2d, k0.h, k0.layout, HH_GG=k0.HH_GG, T=0.1,
+                                      reference_density=ref, hartree_degeneracy=2.0)
+    ka = kernel_args_dict(k)
+    P = rand_herm(rng, k.h.shape)
+    S, Hh, F, E = gpu.build_fock_apply(ka, P, None)
+    Sr, Hr, Fr, Er = oracle.build_fock_apply(ka, P)
+    assert relerr(S, Sr) < 1e-12
+    assert relerr(F, Fr) < 1e-12
+    assert abs(E - Er) <= 1e-12 * max(1.0, abs(Er))
+
+
+def test_newton_solve_superlattice_matches_oracle(gpu, oracle):
+    # the Newton solver's Hessian-vector products build the Fock matrix of a
+    # density variation without the reference density
+    k = sl_kernel(nk=4, g_cut=1, orb=2, T=0.05)
+    args = (kernel_args_dict(k), k._refP.copy(), 7.0, 40, 1e-10, 0.0, 0.6, 0.5, 1e-3, 8, 10,
+            5, 25, 1, 0.5, 20, np.empty((0,) + k.h.shape), 0.0, 1.0, [], None, True, True, True)
+    print("superlattice newton solve", flush=True)
+    g = gpu.solve_dm(*args)
+    r = oracle.solve_dm(*args)
+    assert g[6] == r[6] and g[7] == r[7]
+    np.testing.assert_allclose(g[8], r[8], rtol=0, atol=1e-9)
+    assert abs(g[5] - r[5]) < 1e-9
+    assert relerr(g[2], r[2]) < 1e-7
