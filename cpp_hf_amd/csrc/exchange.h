@@ -160,15 +160,29 @@ void launch_to_cf(size_t n, const cd* in, cf* out, hipStream_t st);
 // geometry (forward exp(-2 pi i / n) or inverse); in == out allowed.
 void launch_fft2_plain(const ExchangeGeom& g, const cd* in, cd* out, int nch, bool inverse,
                        hipStream_t st);
-// In-place 2-D convolution of nch <= g.nch natural-layout channels: rows
-// forward, columns forward x scale V x inverse (V[q * period + ch % period];
-// channels whose flags[ch / per_slot] is 0 give exactly zero), rows inverse.
-// rowmask (nullable, (channel tiles, nk1) ints): row lines marked 0 are
-// skipped by both row passes: they must be zero on input (forward) and are
-// not read afterwards (inverse).
-void launch_fft2_conv_plain(const ExchangeGeom& g, cd* X, int nch, const cd* V, int period,
-                            const int* flags, int per_slot, double scale, const int* rowmask,
-                            hipStream_t st);
+// Source of the superlattice convolution's forward rows: grid row px of
+// slot ls holds, at py, the (a, b) entry of the pair i n_G + j that
+// tiles[(ls ntx + px / nkx) nty + py / nky] names (-1: zero) at k =
+// (px mod nkx, py mod nky), minus the reference density; rows load it
+// straight from P (the scatter folded into the row pass) and set
+// flags[ls] when a value is nonzero.
+struct SlRowsSrc {
+    const cd* P = nullptr;
+    const cd* refP = nullptr;   // (nk, D, D), nullable
+    const cd* refP0 = nullptr;  // (D, D) for every k, nullable
+    const int* tiles = nullptr;  // the batch's first slot
+    int* flags = nullptr;
+    int nkx = 0, nky = 0, ntx = 0, nty = 0, D = 0, n_G = 0, orb = 0, orb2 = 0;
+};
+// 2-D convolution of the nch <= g.nch channels of one superlattice batch
+// in natural layout X[px][py][ch]: forward rows from the density (only the
+// rows that rowmask, (channel tiles, nk1) ints, marks as holding data), the
+// columns forward x scale V x inverse (V[q * period + ch % period]; rows not
+// marked read as zero; channels whose flags[ch / per_slot] is 0 give exactly
+// zero), inverse rows of the marked rows (the only ones read afterwards).
+void launch_sl_convolve(const ExchangeGeom& g, const SlRowsSrc& src, cd* X, int nch,
+                        const cd* V, int period, const int* flags, int per_slot, double scale,
+                        const int* rowmask, hipStream_t st);
 // out = scale A conj(B) per element, or summed over channels per q.
 void launch_cross_spectrum(size_t nq, int nch, const cd* A, const cd* B, double scale,
                            bool sum_channels, cd* out, hipStream_t st);

@@ -1197,6 +1197,62 @@ k_plain_cols(FftAxis ax1, int nk1, int nk2, int nch, int clog, int nblk, int inv
     }
 }
 
+// Superlattice forward rows with the scatter folded in (SlRowsSrc): row
+// k1 = px of channel tile blk, only when rowmask marks it.
+template <bool GENERIC, int S>
+__global__ void __launch_bounds__(kFftThreads)
+k_sl_rows_fwd(FftAxis ax2, int nk1, int nk2, int nch, int clog, int nblk, SlRowsSrc src,
+              const int* __restrict__ rowmask, cd* __restrict__ X) {
+    extern __shared__ cd smem[];
+    const int C = 1 << clog;
+    const int tot = nk2 << clog;
+    cd* s = smem;
+    cd* s_tw = smem + tot;
+    int k1, blk;
+    tile_of(static_cast<int>(blockIdx.x), nk1, nblk, k1, blk);
+    if (rowmask[static_cast<size_t>(blk) * nk1 + k1] == 0) return;  // uniform
+    const int ch = blk * C + (threadIdx.x & (C - 1));
+    const bool valid = ch < nch;
+    const int ls = valid ? ch / src.orb2 : 0, ab = valid ? ch % src.orb2 : 0;
+    const int a = ab / src.orb, b = ab % src.orb;
+    const int tx = k1 / src.nkx, kx = k1 - tx * src.nkx;
+    const size_t base = static_cast<size_t>(k1) * nk2 * nch + ch;
+    const size_t DD = static_cast<size_t>(src.D) * src.D;
+    load_twiddles(s_tw, ax2.tw, nk2);
+    bool nz = false;
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e >= tot) continue;
+        cd v{0.0, 0.0};
+        if (valid) {
+            const int py = e >> clog;
+            const int ty = py / src.nky, ky = py - ty * src.nky;
+            const int pair = src.tiles[(static_cast<size_t>(ls) * src.ntx + tx) * src.nty + ty];
+            if (pair >= 0) {
+                const int i = pair / src.n_G, j = pair % src.n_G;
+                const size_t rc = static_cast<size_t>(i * src.orb + a) * src.D + j * src.orb + b;
+                const size_t idx = (static_cast<size_t>(kx) * src.nky + ky) * DD + rc;
+                v = src.P[idx];
+                if (src.refP)
+                    v = csub(v, src.refP[idx]);
+                else if (src.refP0)
+                    v = csub(v, src.refP0[rc]);
+                nz = nz || v.x != 0.0 || v.y != 0.0;
+            }
+        }
+        s[e] = v;
+    }
+    if (nz) src.flags[ls] = 1;  // every writer stores the same value
+    __syncthreads();
+    fft_tile<false, GENERIC>(s, s_tw, ax2, clog);
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot && valid) X[base + static_cast<size_t>(e >> clog) * nch] = s[e];
+    }
+}
+
 // Along k1 (columns), in place: forward transform, times scale V, inverse
 // transform (the convolution's column half; one HBM read and write).  V is
 // V[q * period + ch % period] at q = q1 nk2 + k2; channels whose flag
@@ -1205,7 +1261,8 @@ template <bool GENERIC, int S>
 __global__ void __launch_bounds__(kFftThreads)
 k_plain_cols_conv(FftAxis ax1, int nk1, int nk2, int nch, int clog, int nblk,
                   cd* __restrict__ X, const cd* __restrict__ V, int period,
-                  const int* __restrict__ flags, int per_slot, double scale) {
+                  const int* __restrict__ flags, int per_slot, double scale,
+                  const int* __restrict__ rowmask) {
     extern __shared__ cd smem[];
     const int C = 1 << clog;
     const int tot = nk1 << clog;
@@ -1222,7 +1279,11 @@ k_plain_cols_conv(FftAxis ax1, int nk1, int nk2, int nch, int clog, int nblk,
 #pragma unroll
     for (int r = 0; r < S; ++r) {
         const int e = threadIdx.x + r * kFftThreads;
-        if (e < tot) s[e] = valid ? X[base + (e >> clog) * stride] : cd{0.0, 0.0};
+        if (e < tot) {
+            const int q1 = e >> clog;
+            const bool has = valid && (!rowmask || rowmask[static_cast<size_t>(blk) * nk1 + q1]);
+            s[e] = has ? X[base + q1 * stride] : cd{0.0, 0.0};
+        }
     }
     __syncthreads();
     fft_tile<false, GENERIC>(s, s_tw, ax1, clog);
@@ -1683,17 +1744,18 @@ void launch_fft2_plain(const ExchangeGeom& g, const cd* in, cd* out, int nch, bo
                       g.ax1.axis(), g.nk1, g.nk2, nch, g.clog, nblk, inverse ? 1 : 0, out);
 }
 
-void launch_fft2_conv_plain(const ExchangeGeom& g, cd* X, int nch, const cd* V, int period,
-                            const int* flags, int per_slot, double scale, const int* rowmask,
-                            hipStream_t st) {
+void launch_sl_convolve(const ExchangeGeom& g, const SlRowsSrc& src, cd* X, int nch,
+                        const cd* V, int period, const int* flags, int per_slot, double scale,
+                        const int* rowmask, hipStream_t st) {
     if (g.packed) throw std::logic_error("plain transforms need a full-channel geometry");
     if (nch > g.nch) throw std::logic_error("more channels than the geometry holds");
+    if (!rowmask) throw std::logic_error("the superlattice rows need a row mask");
     const int nblk = (nch + g.tile() - 1) / g.tile();
-    HFGPU_EXCH_LAUNCH(k_plain_rows, g.generic, g.slots_row, dim3(nblk * g.nk1), g.lds_rows, st,
-                      g.ax2.axis(), g.nk1, g.nk2, nch, g.clog, nblk, 0, X, X, rowmask);
+    HFGPU_EXCH_LAUNCH(k_sl_rows_fwd, g.generic, g.slots_row, dim3(nblk * g.nk1), g.lds_rows, st,
+                      g.ax2.axis(), g.nk1, g.nk2, nch, g.clog, nblk, src, rowmask, X);
     HFGPU_EXCH_LAUNCH(k_plain_cols_conv, g.generic, g.slots_col, dim3(nblk * g.nk2), g.lds_cols,
                       st, g.ax1.axis(), g.nk1, g.nk2, nch, g.clog, nblk, X, V, period, flags,
-                      per_slot, scale);
+                      per_slot, scale, rowmask);
     HFGPU_EXCH_LAUNCH(k_plain_rows, g.generic, g.slots_row, dim3(nblk * g.nk1), g.lds_rows, st,
                       g.ax2.axis(), g.nk1, g.nk2, nch, g.clog, nblk, 1, X, X, rowmask);
 }

@@ -9,11 +9,12 @@
 // kernel V.  The channels of all transformed dG are laid out side by side
 // on the extended grid, X[px][py][slot][a][b] (slot = transformed dG in
 // order, channel index fastest), and transformed together:
-//   scatter  X = (P - refP) placed at the tile of G_a (zero elsewhere)
-//   rows     forward transform along py
-//   columns  forward transform along px, X *= -V(q) / (N_ext_x N_ext_y) (V
-//            per orbital pair optional), inverse transform, in one pass
-//   rows     inverse transform along py
+//   rows     forward transform along py of the grid rows that hold data,
+//            loading P - refP straight into the tile of G_a (the scatter)
+//   columns  forward transform along px (rows without data read as zero),
+//            X *= -V(q) / (N_ext_x N_ext_y) (V per orbital pair optional),
+//            inverse transform, in one pass
+//   rows     inverse transform along py of the rows with data
 //   gather   Sigma[k][(G_a, a)][(G_b, b)] = X at the tile of G_a, or, for a
 //            channel filled from its (-dG) partner, conj of the partner's
 //            value at the tile of G_b with (a, b) swapped.

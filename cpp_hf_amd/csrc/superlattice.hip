@@ -26,39 +26,6 @@ struct SlSrc {
 constexpr size_t kSlGridBytes = size_t(2) << 30;  // grid of one batch, at most
 constexpr int kSlMaxChannels = 1024;              // plain transform channel limit
 
-// X[px][py][c] = (P - refP) of the pair whose G_a tile covers (px, py), else
-// 0.  tiles points at the batch's first slot: (slot, ntx, nty).
-__global__ void __launch_bounds__(256)
-k_sl_scatter(size_t n, int Ny, int nch, int orb, int orb2, int nkx, int nky, int ntx, int nty,
-             int D, int n_G, const int* __restrict__ tiles, const cd* __restrict__ P,
-             const cd* __restrict__ refP, const cd* __restrict__ refP0, cd* __restrict__ X,
-             int* __restrict__ flags) {
-    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < n;
-         e += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const int c = static_cast<int>(e % nch);
-        const size_t q = e / nch;
-        const int px = static_cast<int>(q / Ny), py = static_cast<int>(q % Ny);
-        const int ls = c / orb2, ab = c % orb2;
-        const int tx = px / nkx, ty = py / nky;
-        const int pair = tiles[(static_cast<size_t>(ls) * ntx + tx) * nty + ty];
-        cd v{0.0, 0.0};
-        if (pair >= 0) {
-            const int i = pair / n_G, j = pair % n_G;
-            const int r = i * orb + ab / orb, col = j * orb + ab % orb;
-            const int kx = px - tx * nkx, ky = py - ty * nky;
-            const size_t rc = static_cast<size_t>(r) * D + col;
-            const size_t idx = (static_cast<size_t>(kx) * nky + ky) * D * D + rc;
-            v = P[idx];
-            if (refP)
-                v = csub(v, refP[idx]);
-            else if (refP0)
-                v = csub(v, refP0[rc]);
-            if (v.x != 0.0 || v.y != 0.0) flags[ls] = 1;
-        }
-        X[e] = v;
-    }
-}
-
 // Sigma entries whose channel lies in slots [s0, s0 + nslots); with first,
 // also the zero entries of pairs that no channel writes
 __global__ void __launch_bounds__(256)
@@ -294,15 +261,23 @@ void SlPlan::apply(const cd* P, const cd* refP, const cd* refP0, cd* Sigma, hipS
         const int nslots = std::min(slots_per_batch, n_slots - s0);
         const int nch = nslots * orb2;
         if (nslots > 0) {
-            const size_t n = nq * nch;
             fill_zero(flags.get(), flags.size(), st);
-            hipLaunchKernelGGL(k_sl_scatter, dim3(grid_for(n)), dim3(256), 0, st, n, Ny, nch, orb,
-                               orb2, nkx, nky, ntx, nty, D, n_G,
-                               tiles.as<int>() + static_cast<size_t>(s0) * tiles_per, P, refP,
-                               refP0, X.as<cd>(), flags.as<int>());
-            HFGPU_LAUNCH_CHECK();
-            launch_fft2_conv_plain(
-                *geo, X.as<cd>(), nch, V.as<cd>(), period, flags.as<int>(), orb2, scale,
+            SlRowsSrc src;
+            src.P = P;
+            src.refP = refP;
+            src.refP0 = refP0;
+            src.tiles = tiles.as<int>() + static_cast<size_t>(s0) * tiles_per;
+            src.flags = flags.as<int>();
+            src.nkx = nkx;
+            src.nky = nky;
+            src.ntx = ntx;
+            src.nty = nty;
+            src.D = D;
+            src.n_G = n_G;
+            src.orb = orb;
+            src.orb2 = orb2;
+            launch_sl_convolve(
+                *geo, src, X.as<cd>(), nch, V.as<cd>(), period, flags.as<int>(), orb2, scale,
                 rowmask.as<int>() + static_cast<size_t>(s0 / slots_per_batch) * tiles_per_batch * Nx,
                 st);
         }
