@@ -173,13 +173,22 @@ struct SlRowsSrc {
     const int* tiles = nullptr;  // the batch's first slot
     int* flags = nullptr;
     int nkx = 0, nky = 0, ntx = 0, nty = 0, D = 0, n_G = 0, orb = 0, orb2 = 0;
+    // inverse rows storing Sigma directly (nullable Sigma: store X and let
+    // the caller gather): srct is the (n_G, n_G) table of 4 ints {slot, G
+    // whose tile holds the value, conj, 0} naming each Sigma block's source,
+    // s0 the batch's first slot
+    const int* srct = nullptr;
+    cd* Sigma = nullptr;
+    int s0 = 0;
 };
 // 2-D convolution of the nch <= g.nch channels of one superlattice batch
 // in natural layout X[px][py][ch]: forward rows from the density (only the
 // rows that rowmask, (channel tiles, nk1) ints, marks as holding data), the
 // columns forward x scale V x inverse (V[q * period + ch % period]; rows not
 // marked read as zero; channels whose flags[ch / per_slot] is 0 give exactly
-// zero), inverse rows of the marked rows (the only ones read afterwards).
+// zero), inverse rows of the marked rows (the only ones read afterwards),
+// stored into X, or with src.Sigma straight into the Sigma blocks whose
+// source (src.srct) is this batch's value.
 void launch_sl_convolve(const ExchangeGeom& g, const SlRowsSrc& src, cd* X, int nch,
                         const cd* V, int period, const int* flags, int per_slot, double scale,
                         const int* rowmask, hipStream_t st);

@@ -1253,6 +1253,59 @@ k_sl_rows_fwd(FftAxis ax2, int nk1, int nk2, int nch, int clog, int nblk, SlRows
     }
 }
 
+// Superlattice inverse rows with the gather folded in: after the transform,
+// the value of pair (i, j) at (k, a, b) goes to Sigma[k][(i a)][(j b)] and
+// its conjugate to Sigma[k][(j b)][(i a)] where the source table names this
+// slot and G_i's tile for that block (direct and partner fill).
+template <bool GENERIC, int S>
+__global__ void __launch_bounds__(kFftThreads)
+k_sl_rows_inv(FftAxis ax2, int nk1, int nk2, int nch, int clog, int nblk, SlRowsSrc src,
+              const int* __restrict__ rowmask, const cd* __restrict__ X) {
+    extern __shared__ cd smem[];
+    const int C = 1 << clog;
+    const int tot = nk2 << clog;
+    cd* s = smem;
+    cd* s_tw = smem + tot;
+    int k1, blk;
+    tile_of(static_cast<int>(blockIdx.x), nk1, nblk, k1, blk);
+    if (rowmask[static_cast<size_t>(blk) * nk1 + k1] == 0) return;  // uniform
+    const int ch = blk * C + (threadIdx.x & (C - 1));
+    const bool valid = ch < nch;
+    const int ls = valid ? ch / src.orb2 : 0, ab = valid ? ch % src.orb2 : 0;
+    const int a = ab / src.orb, b = ab % src.orb;
+    const int tx = k1 / src.nkx, kx = k1 - tx * src.nkx;
+    const size_t base = static_cast<size_t>(k1) * nk2 * nch + ch;
+    const size_t DD = static_cast<size_t>(src.D) * src.D;
+    const int slot = src.s0 + ls;
+    load_twiddles(s_tw, ax2.tw, nk2);
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e < tot) s[e] = valid ? X[base + static_cast<size_t>(e >> clog) * nch] : cd{0.0, 0.0};
+    }
+    __syncthreads();
+    fft_tile<true, GENERIC>(s, s_tw, ax2, clog);
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        const int e = threadIdx.x + r * kFftThreads;
+        if (e >= tot || !valid) continue;
+        const int py = e >> clog;
+        const int ty = py / src.nky, ky = py - ty * src.nky;
+        const int pair = src.tiles[(static_cast<size_t>(ls) * src.ntx + tx) * src.nty + ty];
+        if (pair < 0) continue;
+        const int i = pair / src.n_G, j = pair % src.n_G;
+        const cd v = s[e];
+        const size_t kb = (static_cast<size_t>(kx) * src.nky + ky) * DD;
+        const int* d = src.srct + 4 * (static_cast<size_t>(i) * src.n_G + j);
+        if (d[0] == slot && d[1] == i && d[2] == 0)
+            src.Sigma[kb + static_cast<size_t>(i * src.orb + a) * src.D + j * src.orb + b] = v;
+        const int* c = src.srct + 4 * (static_cast<size_t>(j) * src.n_G + i);
+        if (c[0] == slot && c[1] == i && c[2] == 1)
+            src.Sigma[kb + static_cast<size_t>(j * src.orb + b) * src.D + i * src.orb + a] =
+                cconj(v);
+    }
+}
+
 // Along k1 (columns), in place: forward transform, times scale V, inverse
 // transform (the convolution's column half; one HBM read and write).  V is
 // V[q * period + ch % period] at q = q1 nk2 + k2; channels whose flag
@@ -1756,8 +1809,13 @@ void launch_sl_convolve(const ExchangeGeom& g, const SlRowsSrc& src, cd* X, int 
     HFGPU_EXCH_LAUNCH(k_plain_cols_conv, g.generic, g.slots_col, dim3(nblk * g.nk2), g.lds_cols,
                       st, g.ax1.axis(), g.nk1, g.nk2, nch, g.clog, nblk, X, V, period, flags,
                       per_slot, scale, rowmask);
-    HFGPU_EXCH_LAUNCH(k_plain_rows, g.generic, g.slots_row, dim3(nblk * g.nk1), g.lds_rows, st,
-                      g.ax2.axis(), g.nk1, g.nk2, nch, g.clog, nblk, 1, X, X, rowmask);
+    if (src.Sigma)
+        HFGPU_EXCH_LAUNCH(k_sl_rows_inv, g.generic, g.slots_row, dim3(nblk * g.nk1), g.lds_rows,
+                          st, g.ax2.axis(), g.nk1, g.nk2, nch, g.clog, nblk, src, rowmask,
+                          static_cast<const cd*>(X));
+    else
+        HFGPU_EXCH_LAUNCH(k_plain_rows, g.generic, g.slots_row, dim3(nblk * g.nk1), g.lds_rows,
+                          st, g.ax2.axis(), g.nk1, g.nk2, nch, g.clog, nblk, 1, X, X, rowmask);
 }
 
 static unsigned ew_blocks(size_t n) {

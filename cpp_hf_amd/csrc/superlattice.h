@@ -57,6 +57,9 @@ struct SlPlan {
     DevBuf flags;      // (slots_per_batch) int: channel holds a nonzero value
     DevBuf rowmask;    // (batches, channel tiles, Nx) int: grid row px of a tile holds data
     int tiles_per_batch = 0;
+    // every Sigma block has a source and no two pairs of a slot share a
+    // tile: the inverse rows store Sigma themselves (no gather pass)
+    bool direct_store = false;
     std::unique_ptr<ExchangeGeom> geo;  // Nx x Ny plain transform
 
     SlPlan() = default;

@@ -177,6 +177,7 @@ void SlPlan::init(const SlInput& in, hipStream_t st) {
     std::vector<SlSrc> src_tab(static_cast<size_t>(n_G) * n_G, SlSrc{-1, 0, 0, 0});
     const size_t tiles_per = static_cast<size_t>(ntx) * nty;
     n_slots = 0;
+    bool collision = false;
     for (size_t d = 0; d < nd; ++d) {
         const bool fill = partner[d] >= 0 && static_cast<size_t>(partner[d]) != d;
         if (fill && static_cast<size_t>(partner[d]) < d) continue;
@@ -188,12 +189,17 @@ void SlPlan::init(const SlInput& in, hipStream_t st) {
             const int i = static_cast<int>(in.pair_i[p]), j = static_cast<int>(in.pair_j[p]);
             const int tx = static_cast<int>(in.g_a_off[2 * i]);
             const int ty = static_cast<int>(in.g_a_off[2 * i + 1]);
-            tt[static_cast<size_t>(tx) * nty + ty] = i * n_G + j;
+            int& cell = tt[static_cast<size_t>(tx) * nty + ty];
+            if (cell >= 0) collision = true;
+            cell = i * n_G + j;
             src_tab[static_cast<size_t>(i) * n_G + j] = SlSrc{s, i, 0, 0};
             if (fill) src_tab[static_cast<size_t>(j) * n_G + i] = SlSrc{s, i, 1, 0};
         }
     }
 
+    bool all_sourced = true;
+    for (const SlSrc& e : src_tab) all_sourced = all_sourced && e.slot >= 0;
+    direct_store = !collision && all_sourced;
     const size_t nq = static_cast<size_t>(Nx) * Ny;
     const size_t period = orbital ? static_cast<size_t>(orb2) : 1;
     V.alloc(nq * period * sizeof(cd));
@@ -276,15 +282,22 @@ void SlPlan::apply(const cd* P, const cd* refP, const cd* refP0, cd* Sigma, hipS
             src.n_G = n_G;
             src.orb = orb;
             src.orb2 = orb2;
+            if (direct_store) {
+                src.srct = this->src.as<int>();
+                src.Sigma = Sigma;
+                src.s0 = s0;
+            }
             launch_sl_convolve(
                 *geo, src, X.as<cd>(), nch, V.as<cd>(), period, flags.as<int>(), orb2, scale,
                 rowmask.as<int>() + static_cast<size_t>(s0 / slots_per_batch) * tiles_per_batch * Nx,
                 st);
         }
-        hipLaunchKernelGGL(k_sl_gather, dim3(grid_for(n_out)), dim3(256), 0, st, n_out, D, orb,
-                           orb2, n_G, nkx, nky, Ny, std::max(nch, 1), src.as<SlSrc>(),
-                           goff.as<int>(), s0, nslots, s0 == 0 ? 1 : 0, X.as<cd>(), Sigma);
-        HFGPU_LAUNCH_CHECK();
+        if (!direct_store || nslots == 0) {
+            hipLaunchKernelGGL(k_sl_gather, dim3(grid_for(n_out)), dim3(256), 0, st, n_out, D,
+                               orb, orb2, n_G, nkx, nky, Ny, std::max(nch, 1), src.as<SlSrc>(),
+                               goff.as<int>(), s0, nslots, s0 == 0 ? 1 : 0, X.as<cd>(), Sigma);
+            HFGPU_LAUNCH_CHECK();
+        }
         s0 += std::max(nslots, 1);
     } while (s0 < n_slots);
 }

@@ -185,3 +185,34 @@ def test_newton_solve_superlattice_matches_oracle(gpu, oracle):
     np.testing.assert_allclose(g[8], r[8], rtol=0, atol=1e-9)
     assert abs(g[5] - r[5]) < 1e-9
     assert relerr(g[2], r[2]) < 1e-7
+
+
+@pytest.mark.parametrize("herm_fill", [False, True])
+def test_streamed_incomplete_channel_table(gpu, oracle, herm_fill):
+    # a channel table without one dG: its blocks (and, with the partner fill,
+    # nothing else) have no source and stay zero, as in the reference's
+    # channel loop; the GPU then stores through the separate gather pass
+    nk, orb = 3, 2
+    L, g, _ = layout(nk=nk, g_cut=1)
+    n_G = g.shape[0]
+    D = n_G * orb
+    drop = 3
+    ps = L.delta_pair_start
+    keep = np.r_[0:ps[drop], ps[drop + 1]:ps[-1]]
+    pi, pj = L.delta_pair_i[keep], L.delta_pair_j[keep]
+    cnt = np.diff(ps).copy()
+    cnt = np.delete(cnt, drop)
+    start = np.zeros(len(cnt) + 1, dtype=np.int64)
+    start[1:] = np.cumsum(cnt)
+    rng = np.random.default_rng(4)
+    rho = rand_herm(rng, (nk, nk, D, D)).reshape(nk, nk, n_G, orb, n_G, orb)
+    out = []
+    for mod in (gpu, oracle):
+        out.append(mod.selfenergy_superlattice_streamed(
+            rho, L.V_lag_fft, L.g_a_off, pi, pj, start, nk, nk, n_G, orb, L.N_ext_x, L.N_ext_y,
+            VR_fft_orbital=None, hermitian_rho=herm_fill))
+    got, ref = out
+    assert relerr(got, ref) < 1e-12
+    dropped = [(int(L.delta_pair_i[p]), int(L.delta_pair_j[p])) for p in range(ps[drop], ps[drop + 1])]
+    G = got.reshape(nk, nk, n_G, orb, n_G, orb)
+    assert all(not np.any(G[:, :, i, :, j, :]) for i, j in dropped)
