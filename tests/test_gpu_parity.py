@@ -74,7 +74,7 @@ def test_selfenergy_unsupported_prime_mesh_raises(gpu):
 
 # -------------------------------------------------------------------- eigh --
 
-@pytest.mark.parametrize("nb", [1, 2, 3, 5, 8, 12, 16, 32])
+@pytest.mark.parametrize("nb", [1, 2, 3, 5, 8, 12, 16, 17, 24, 32])
 def test_eigh_matches_oracle(gpu, oracle, nb):
     print(f"eigh nb={nb}", flush=True)
     rng = np.random.default_rng(100 + nb)
@@ -392,7 +392,8 @@ def test_solve_scf_meshgrid_fixture_matches_oracle(gpu, oracle):
     np.testing.assert_allclose(g[1], d["F_expected"], rtol=1e-6, atol=1e-6)
 
 
-@pytest.mark.parametrize("nb,sizes", [(8, (3, 2, 3)), (6, (2, 4)), (12, (4, 4, 4)), (4, (1, 3))])
+@pytest.mark.parametrize("nb,sizes", [(8, (3, 2, 3)), (6, (2, 4)), (12, (4, 4, 4)), (4, (1, 3)),
+                                      (24, (8, 10, 6)), (20, (5, 15))])
 @pytest.mark.parametrize("sort", [True, False])
 def test_block_eigh_matches_oracle(gpu, oracle, nb, sizes, sort):
     rng = np.random.default_rng(nb + len(sizes))
