@@ -314,6 +314,44 @@ k_newton_step(const double* __restrict__ eps, const double* __restrict__ w_norm,
     }
 }
 
+// The whole in-loop Newton solve in one workgroup, for few eigenvalues
+// (n <= kNewtonSoloMax): the iterations of k_newton_step / k_newton_last
+// without a launch each, ending at the exact fixed point (the multi-launch
+// path runs every launch and lets the converged ones return at once).
+constexpr size_t kNewtonSoloMax = 16384;
+
+__global__ void __launch_bounds__(kBlock)
+k_newton_solo(const double* __restrict__ eps, const double* __restrict__ w_norm, int nb,
+              size_t n, double n_target, double T, double mu0, int mu0_from_state,
+              const MuState* warm, int maxiter, int mm_parts, double* ws,
+              MuState* state) {  // warm may be state: read before the first barrier
+    __shared__ double red[kBlock / 64];
+    __shared__ double part[2];
+    MuState st = bracket_from(pp_part(ws, 1), mm_parts, 1, T,
+                              mu0_from_state ? warm->mu : mu0, red);
+    const double Tj = fmax(T, 1.0e-12);
+    for (int j = 0; j < maxiter && !st.done; ++j) {  // st is uniform over the block
+        const double mu = st.mu;
+        double N = 0.0, Z = 0.0;
+        for (size_t i = threadIdx.x; i < n; i += kBlock) {
+            const double wk = w_norm[i / nb];
+            const double p = expit_d((mu - eps[i]) / Tj);
+            N += wk * p;
+            Z += wk * p * (1.0 - p) / Tj;
+        }
+        N = block_sum<kBlock>(N, red);
+        Z = block_sum<kBlock>(Z, red);
+        if (threadIdx.x == 0) {
+            part[0] = N;
+            part[1] = Z;
+        }
+        __syncthreads();
+        st = newton_apply(st, part, 1, n_target, red);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *state = st;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_newton_last(double n_target, int j, int prev_parts, int apply, double* ws, MuState* state) {
     __shared__ double red[kBlock / 64];
@@ -394,6 +432,13 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
     const unsigned mm_blocks = reduce_blocks(n, 1);
     hipLaunchKernelGGL(k_minmax, dim3(mm_blocks), dim3(kBlock), 0, st, eps, n, ws + kPart1);
     HFGPU_LAUNCH_CHECK();
+    if (maxiter > 0 && n <= kNewtonSoloMax) {
+        hipLaunchKernelGGL(k_newton_solo, dim3(1), dim3(kBlock), 0, st, eps, w_norm, nb, n,
+                           n_target_norm, T, mu0, mu0_from_state ? 1 : 0, state, maxiter,
+                           static_cast<int>(mm_blocks), ws, state);
+        HFGPU_LAUNCH_CHECK();
+        return;
+    }
     int prev_parts = static_cast<int>(mm_blocks);
     for (int j = 0; j < maxiter; ++j) {
         hipLaunchKernelGGL(k_newton_step, dim3(blocks), dim3(kBlock), 0, st, eps, w_norm, nb, n,
