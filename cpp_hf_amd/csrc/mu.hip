@@ -22,7 +22,8 @@
 // at 96 x 96 x 8 (93 us of kernel time against 132 us for L = 5 and 167 us for
 // L = 1); HFGPU_BISECT_LEVELS overrides it.  Brackets are bit-identical to the
 // sequential loop's for every L.  Newton (solve_mu_inloop) takes one
-// launch per iteration and stops computing at an exact fixed point.
+// launch per iteration and stops computing at an exact fixed point; for at
+// most 16384 eigenvalues one workgroup runs all iterations in one launch.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
