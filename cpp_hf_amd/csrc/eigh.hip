@@ -267,7 +267,10 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
     // compile time (lanes NBC..NP-1 of a group are then padding), else NP
     constexpr int NL = NBC > 0 ? NBC : NP;
     constexpr double kEps = 2.220446049250313e-16;
-    __shared__ TriLds<NP> lds[4];
+    // one TriLds per wavefront of the workgroup: 4 (256 threads), or 1 for
+    // NP = 32, whose launches are one wavefront wide (its Z rows take the
+    // dynamic LDS)
+    __shared__ TriLds<NP> lds[NP == 32 ? 1 : 4];
     const int lane = threadIdx.x & 63;
     TriLds<NP>& X = lds[threadIdx.x >> 6];
     const int i = lane % NP;

@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <exception>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -152,6 +153,11 @@ void dev_free(void* p, size_t bytes) {
     if (!p) return;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return;
+    // A buffer freed while an exception unwinds may still be read or written
+    // by work queued on a stream that has not drained (the stream objects of
+    // the entry points may be destroyed after their buffers): drain the
+    // device before the memory can reach another thread's allocation.
+    if (std::uncaught_exceptions() > 0) (void)hipDeviceSynchronize();
     DevPool& pool = dev_pool();
     std::lock_guard<std::mutex> lock(pool.mu);
     pool.free_lists[{dev, size_class(bytes)}].push_back(p);
@@ -647,27 +653,34 @@ void GpuKernel::energy(const cd* P, const cd* Sigma, const double* hdiag_in, dou
 namespace {
 
 struct GeomCache {
+    using Ptr = std::shared_ptr<ExchangeGeom>;
     std::mutex mu;
-    std::vector<std::pair<std::tuple<int, int, int, bool>, std::unique_ptr<ExchangeGeom>>> entries;
+    std::vector<std::pair<std::tuple<int, int, int, bool>, Ptr>> entries;
 
-    ExchangeGeom& get(int nk1, int nk2, int nb, bool packed) {
+    // Shared ownership: a caller keeps its geometry alive (device tables
+    // included) until it drops the pointer, after its stream has drained,
+    // even when another thread evicts the entry or clears the cache meanwhile.
+    Ptr get(int nk1, int nk2, int nb, bool packed) {
         std::lock_guard<std::mutex> lock(mu);
         const auto key = std::make_tuple(nk1, nk2, nb, packed);
         for (auto& e : entries)
-            if (e.first == key) return *e.second;
-        while (entries.size() >= 4) {
-            entries.front().second->release();
-            entries.erase(entries.begin());
-        }
-        auto g = std::make_unique<ExchangeGeom>();
+            if (e.first == key) return e.second;
+        while (entries.size() >= 4) entries.erase(entries.begin());
+        Ptr g(new ExchangeGeom(), [](ExchangeGeom* x) {
+            x->release();
+            delete x;
+        });
         g->init(nk1, nk2, nb, packed);
-        entries.emplace_back(key, std::move(g));
-        return *entries.back().second;
+        entries.emplace_back(key, g);
+        return g;
     }
     void clear() {
         std::lock_guard<std::mutex> lock(mu);
-        for (auto& e : entries) e.second->release();
         entries.clear();
+    }
+    size_t size() {
+        std::lock_guard<std::mutex> lock(mu);
+        return entries.size();
     }
 };
 
@@ -773,8 +786,10 @@ int hfgpu_selfenergy_fft_full(const double* VR, const double* P, double* sigma, 
             throw std::invalid_argument("VR shape must be (..., 1, 1) or (..., nb, nb).");
         const size_t nk = nk1 * nk2, nd = nk * nb * nb;
         if (nd == 0) return;
-        ExchangeGeom& g = geom_cache().get(static_cast<int>(nk1), static_cast<int>(nk2),
-                                           static_cast<int>(nb), hcp != 0);
+        // held past the stream's drain (st is destroyed first)
+        const auto gp = geom_cache().get(static_cast<int>(nk1), static_cast<int>(nk2),
+                                         static_cast<int>(nb), hcp != 0);
+        const ExchangeGeom& g = *gp;
         const bool dense = dv1 > 1;
         Stream st;
         DevBuf dP(nd * sizeof(cd)), dS(nd * sizeof(cd)), dT(g.tile_elems() * sizeof(cd)),
@@ -865,6 +880,21 @@ int hfgpu_find_mu_bisection(const double* bands, const double* weights, size_t n
         d2h(&hs, ds.get(), sizeof(MuState), st.s);
         st.sync();
         *mu = hs.mu;
+    });
+}
+
+int hfgpu_find_mu_newton(const double* bands, const double* weights, size_t nk, size_t nb,
+                         double n_e, double T, int maxiter, double* mu) {
+    return guarded([&] {
+        if (nk * nb == 0) throw std::invalid_argument("bands must not be empty");
+        Stream st;
+        DevBuf de(nk * nb * sizeof(double)), dw(nk * sizeof(double)), dm(sizeof(double));
+        h2d(de.get(), bands, nk * nb * sizeof(double), st.s);
+        h2d(dw.get(), weights, nk * sizeof(double), st.s);
+        launch_mu_newton_py(de.as<double>(), dw.as<double>(), static_cast<int>(nk),
+                            static_cast<int>(nb), n_e, T, maxiter, dm.as<double>(), st.s);
+        d2h(mu, dm.get(), sizeof(double), st.s);
+        st.sync();
     });
 }
 

@@ -58,6 +58,11 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
                       double n_target_norm, double mu0, bool mu0_from_state, double T,
                       int maxiter, MuState* state, double* ws, hipStream_t st);
 
+// The Python surface's Newton (src/cpp_hf/utils.py:311-360) on raw weights:
+// one workgroup, result in *mu_out.
+void launch_mu_newton_py(const double* eps, const double* w, int nk, int nb, double n_e,
+                         double T, int maxiter, double* mu_out, hipStream_t st);
+
 // occupations from state->mu.
 //   mode 0 (SCF): occ = expit(-(eps - mu) / (T + 1e-12))
 //   mode 1 (DM):  p   = expit((mu - eps) / T)

@@ -361,6 +361,52 @@ k_newton_last(double n_target, int j, int prev_parts, int apply, double* ws, MuS
     if (threadIdx.x == 0) *state = st;
 }
 
+// ---- the Python surface's Newton (utils.py:311-360) -------------------------
+// One workgroup runs every step: the bracket from the extremes, maxiter
+// Newton steps with the bisection safeguard on raw weights, then the final
+// count check that falls back to the bracket midpoint.  Every thread derives
+// the same scalar update from the block sums.
+__global__ void __launch_bounds__(1024)
+k_mu_newton_py(const double* __restrict__ eps, const double* __restrict__ w, int nb, size_t n,
+               double n_e, double T, int maxiter, double* __restrict__ mu_out) {
+    __shared__ double red[16];
+    double lo = INFINITY, hi = -INFINITY;
+    for (size_t i = threadIdx.x; i < n; i += 1024) {
+        lo = fmin(lo, eps[i]);
+        hi = fmax(hi, eps[i]);
+    }
+    const double emin = block_min<1024>(lo, red), emax = block_max<1024>(hi, red);
+    const double Tj = fmax(T, 1.0e-12);
+    const double span = emax - emin + 10.0 * fmax(Tj, 1.0e-6);
+    lo = emin - span;
+    hi = emax + span;
+    double mu = 0.5 * (lo + hi);
+    for (int it = 0; it <= maxiter; ++it) {
+        double N = 0.0, Z = 0.0;
+        for (size_t i = threadIdx.x; i < n; i += 1024) {
+            const double wk = w[i / nb];
+            const double p = expit_d((mu - eps[i]) / Tj);
+            N += wk * p;
+            Z += wk * p * (1.0 - p) / Tj;
+        }
+        N = block_sum<1024>(N, red);
+        if (it == maxiter) {  // final check (utils.py:357-359)
+            if (fabs(N - n_e) > 1e-12) mu = 0.5 * (lo + hi);
+            break;
+        }
+        Z = block_sum<1024>(Z, red);
+        const double g = N - n_e;
+        if (g < 0.0) lo = mu;
+        if (g > 0.0) hi = mu;
+        const double Z_safe = fmax(Z, 1.0e-18);
+        double mu_new = mu - g / Z_safe;
+        const double mu_bis = 0.5 * (lo + hi);
+        if (mu_new <= lo || mu_new >= hi || !isfinite(mu_new)) mu_new = mu_bis;
+        mu = fmax(fmin(mu_new, hi), lo);
+    }
+    if (threadIdx.x == 0) *mu_out = mu;
+}
+
 // ---- occupations -----------------------------------------------------------
 
 __global__ void k_occupations(const double* __restrict__ eps, size_t n,
@@ -462,6 +508,14 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
                            0, 0, ws, state);
         HFGPU_LAUNCH_CHECK();
     }
+}
+
+void launch_mu_newton_py(const double* eps, const double* w, int nk, int nb, double n_e,
+                         double T, int maxiter, double* mu_out, hipStream_t st) {
+    const size_t n = static_cast<size_t>(nk) * nb;
+    hipLaunchKernelGGL(k_mu_newton_py, dim3(1), dim3(1024), 0, st, eps, w, nb, n, n_e, T,
+                       std::max(0, maxiter), mu_out);
+    HFGPU_LAUNCH_CHECK();
 }
 
 void launch_occupations(const double* eps, size_t n, const MuState* state, double T, int mode,

@@ -498,6 +498,26 @@ PYBIND11_MODULE(_native, m) {
         py::arg("bands"), py::arg("weights"), py::arg("n_e"), py::arg("T"), py::arg("maxiter") = 0);
 
     m.def(
+        "find_mu_newton",
+        [](RArr bands, RArr weights, double n_e, double T, int maxiter) {
+            if (bands.ndim() < 1) throw py::value_error("bands must be at least 1-D");
+            size_t nk = 1;
+            for (int i = 0; i < bands.ndim() - 1; ++i) nk *= bands.shape(i);
+            const size_t nb = bands.shape(bands.ndim() - 1);
+            if ((size_t)weights.size() < nk) throw py::value_error("weights too short for bands");
+            double mu = 0.0;
+            int rc;
+            {
+                py::gil_scoped_release rel;
+                rc = hfgpu_find_mu_newton(cdp(bands), cdp(weights), nk, nb, n_e, T, maxiter, &mu);
+            }
+            check(rc);
+            return mu;
+        },
+        py::arg("bands"), py::arg("weights"), py::arg("n_e"), py::arg("T"),
+        py::arg("maxiter") = 15);
+
+    m.def(
         "solve_mu_inloop",
         [](RArr eps, RArr w_norm, double n_target_norm, double mu0, double T, int maxiter) {
             size_t nk = 1;

@@ -149,8 +149,8 @@ def find_chemical_potential(
     """mu with sum_k w_k sum_j f(eps_kj - mu) = n_electrons (utils.py:254-274).
 
     ``bisection`` runs the GPU multisection kernel (30 steps for float64
-    bands, 54 for float32, as the reference); ``newton`` runs the GPU
-    Newton/bisection hybrid from the reference's bracket.
+    bands, 54 for float32, as the reference); ``newton`` runs the
+    reference's Newton/bisection hybrid in one GPU workgroup.
     """
     validate_electron_count(weights, np.asarray(bands).shape[-1], n_electrons,
                             context="n_electrons")
@@ -173,26 +173,23 @@ def find_chemical_potential(
 
 
 def _find_mu_newton(bands, weights, n_electrons, T, *, maxiter=None) -> np.ndarray:
-    """Newton with bisection safeguard on the GPU.
+    """Newton with bisection safeguard on the GPU (utils.py:311-360).
 
-    Runs the in-loop Newton kernel (utils.hpp:79-116) from the reference
-    Python Newton's starting point, the midpoint of [emin - span,
-    emax + span] (utils.py:311-360).  Its safeguard bracket is
-    [emin - 50 T, emax + 50 T] rather than the Python version's span
-    bracket, so the iterates can differ while the root is the same; the
-    result is checked by the electron count, not step by step.
+    The reference's algorithm step for step: raw weights, the bracket
+    [emin - span, emax + span], its midpoint as the start, ``maxiter`` steps
+    (15 for float64 bands, 25 otherwise) and the bracket midpoint when the
+    final count misses the target by more than 1e-12.
     """
     native_required()
     bands = np.asarray(bands)
     real_dtype = bands.real.dtype if np.iscomplexobj(bands) else bands.dtype
     if maxiter is None:
         maxiter = 15 if real_dtype == np.float64 else 25
-    b = np.ascontiguousarray(bands, dtype=np.float64)
-    w = np.ascontiguousarray(weights, dtype=np.float64)
-    Tj = max(float(T), 1e-12)
-    span = float(np.max(b)) - float(np.min(b)) + 10.0 * max(Tj, 1e-6)
-    mu0 = 0.5 * ((float(np.min(b)) - span) + (float(np.max(b)) + span))
-    mu = _native.solve_mu_inloop(b, w, float(n_electrons), mu0, Tj, int(maxiter))
+    mu = _native.find_mu_newton(
+        np.ascontiguousarray(bands, dtype=np.float64),
+        np.ascontiguousarray(weights, dtype=np.float64),
+        float(n_electrons), float(T), int(maxiter),
+    )
     return np.asarray(mu, dtype=real_dtype)
 
 

@@ -138,6 +138,14 @@ int hfgpu_max_offblock_sizes(const double* M, size_t nmat, size_t nb, const size
 int hfgpu_find_mu_bisection(const double* bands, const double* weights, size_t nk, size_t nb,
                             double n_e, double T, int maxiter, double* mu);
 
+/* Chemical potential by Newton with bisection safeguard on raw weights,
+ * started at the middle of [emin - span, emax + span], with the bracket
+ * midpoint returned when the final count misses n_e by more than 1e-12.
+ * Replaces: cpp_hf.utils._find_mu_newton (src/cpp_hf/utils.py:311-360),
+ * reached through find_chemical_potential(method="newton") (utils.py:254-274). */
+int hfgpu_find_mu_newton(const double* bands, const double* weights, size_t nk, size_t nb,
+                         double n_e, double T, int maxiter, double* mu);
+
 /* In-loop Newton/bisection chemical potential on normalised weights.
  * Replaces: solve_mu_inloop (cpp/include/cpp_hf/utils.hpp:79-116). */
 int hfgpu_solve_mu_inloop(const double* eps, const double* w_norm, size_t nk, size_t nb,

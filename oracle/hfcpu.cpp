@@ -505,6 +505,52 @@ double solve_mu_inloop(const double* eps, size_t n_eps, const double* w_norm, si
     return mu;
 }
 
+// src/cpp_hf/utils.py:311-360 (_find_mu_newton, the Python surface's
+// method="newton"): raw weights and target, bracket [emin - span,
+// emax + span] with span = emax - emin + 10 max(Tj, 1e-6), start at its
+// midpoint, maxiter Newton steps with the bisection safeguard, and the
+// bracket midpoint when the final count misses the target by more than 1e-12.
+double find_mu_newton_py(const double* eps, const double* weights, size_t nk, size_t nb,
+                         double n_e, double T, int maxiter) {
+    const double Tj = std::max(T, 1.0e-12);
+    const size_t n_eps = nk * nb;
+    double emin = eps[0], emax = eps[0];
+    for (size_t i = 1; i < n_eps; ++i) {
+        emin = std::min(emin, eps[i]);
+        emax = std::max(emax, eps[i]);
+    }
+    const double span = emax - emin + 10.0 * std::max(Tj, 1.0e-6);
+    double lo = emin - span, hi = emax + span;
+    double mu = 0.5 * (lo + hi);
+    auto count_and_slope = [&](double m, double& N, double& Z) {
+        N = 0.0;
+        Z = 0.0;
+        for (size_t k = 0; k < nk; ++k)
+            for (size_t b = 0; b < nb; ++b) {
+                const double p = expit_scalar((m - eps[k * nb + b]) / Tj);
+                N += weights[k] * p;
+                Z += weights[k] * p * (1.0 - p) / Tj;
+            }
+    };
+    for (int it = 0; it < maxiter; ++it) {
+        double N, Z;
+        count_and_slope(mu, N, Z);
+        const double g = N - n_e;
+        if (g < 0.0) lo = mu;
+        if (g > 0.0) hi = mu;
+        const double Z_safe = std::max(Z, 1.0e-18);
+        double mu_new = mu - g / Z_safe;
+        const double mu_bis = 0.5 * (lo + hi);
+        if (mu_new <= lo || mu_new >= hi || !std::isfinite(mu_new)) mu_new = mu_bis;
+        mu_new = std::max(std::min(mu_new, hi), lo);
+        mu = mu_new;
+    }
+    double N_fin, Z_fin;
+    count_and_slope(mu, N_fin, Z_fin);
+    if (std::fabs(N_fin - n_e) > 1e-12) mu = 0.5 * (lo + hi);
+    return mu;
+}
+
 // utils.hpp:120-146 (no FMA contraction: the build uses -ffp-contract=off)
 void resample_axis(const c64* x, c64* y, size_t n_old, size_t n_new, size_t inner) {
     if (n_old == n_new) {
@@ -2036,6 +2082,11 @@ int hfcpu_max_offblock_sizes(const double* M, size_t nk, size_t nb, const size_t
 int hfcpu_find_mu_bisection(const double* bands, const double* weights, size_t nk, size_t nb,
                             double n_e, double T, int maxiter, double* mu) {
     return guarded([&] { *mu = find_mu_bisection(bands, nk * nb, weights, nk, nb, n_e, T, maxiter); });
+}
+
+int hfcpu_find_mu_newton(const double* bands, const double* weights, size_t nk, size_t nb,
+                         double n_e, double T, int maxiter, double* mu) {
+    return guarded([&] { *mu = find_mu_newton_py(bands, weights, nk, nb, n_e, T, maxiter); });
 }
 
 int hfcpu_solve_mu_inloop(const double* eps, const double* w_norm, size_t nk, size_t nb,

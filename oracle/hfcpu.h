@@ -23,6 +23,8 @@ int hfcpu_max_offblock_sizes(const double* M, size_t nk, size_t nb, const size_t
                              size_t nsizes, double* out);
 int hfcpu_find_mu_bisection(const double* bands, const double* weights, size_t nk, size_t nb,
                             double n_e, double T, int maxiter, double* mu);
+int hfcpu_find_mu_newton(const double* bands, const double* weights, size_t nk, size_t nb,
+                         double n_e, double T, int maxiter, double* mu);
 int hfcpu_solve_mu_inloop(const double* eps, const double* w_norm, size_t nk, size_t nb,
                           double n_target_norm, double mu0, double T, int maxiter, double* mu);
 int hfcpu_validate_electron_count(const double* weights, size_t nk, size_t nb, double n_e);

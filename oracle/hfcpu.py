@@ -215,6 +215,18 @@ def find_mu_bisection(bands, weights, n_e, T, maxiter=0):
     return mu.value
 
 
+def find_mu_newton(bands, weights, n_e, T, maxiter=15):
+    """The Python surface's Newton (utils.py:311-360)."""
+    bands = _f64(bands)
+    weights = _f64(weights)
+    nb = bands.shape[-1]
+    nk = int(np.prod(bands.shape[:-1]))
+    mu = C.c_double()
+    _check(lib().hfcpu_find_mu_newton(_p(bands), _p(weights), _sz(nk), _sz(nb), C.c_double(n_e),
+                                      C.c_double(T), C.c_int(maxiter), C.byref(mu)))
+    return mu.value
+
+
 def solve_mu_inloop(eps, w_norm, n_target_norm, mu0, T, maxiter=25):
     eps = _f64(eps)
     w_norm = _f64(w_norm)

@@ -180,6 +180,40 @@ def test_mu_newton_stays_at_converged_mu(gpu, oracle):
             assert abs(got - ref) <= 1e-14, (mu0, it, got, ref)
 
 
+def test_mu_newton_multi_launch_stays_at_converged_mu(gpu, oracle):
+    """The same flat-band model above the one-workgroup size (96 x 96 x 2 =
+    18432 eigenvalues: one launch per iteration and a last-step launch): the
+    converged mu stays put.  Here the oracle's sequential sum of 9216 terms
+    leaves a residual of a few 1e-13 / Z, so its mu wanders by that much
+    once converged, while a jump to the bracket middle would move mu by
+    ~0.5: the bound 1e-12 separates the two."""
+    eps = np.zeros((96, 96, 2))
+    eps[..., 0], eps[..., 1] = -0.5, 0.5
+    wn = np.full((96, 96), 1.0 / 9216)
+    for n_t in (1.0 / 64, 1.0):
+        for mu0 in (0.0, -1.33):
+            for it in (3, 5, 12, 25):
+                got = gpu.solve_mu_inloop(eps, wn, n_t, mu0, 0.2, it)
+                ref = oracle.solve_mu_inloop(eps, wn, n_t, mu0, 0.2, it)
+                assert abs(got - ref) <= 1e-12, (n_t, mu0, it, got, ref)
+
+
+def test_mu_newton_solo_and_multi_launch_agree(gpu):
+    """16384 eigenvalues run in one workgroup, 16385 in one launch per
+    iteration; the extra eigenvalue sits far above the Fermi level, where its
+    occupation and slope are exactly zero, so both solve the same problem."""
+    rng = np.random.RandomState(12)
+    eps = rng.randn(128, 128, 1)
+    wn = np.full((128, 128), 1.0 / 16384)
+    eps2 = np.concatenate([eps.reshape(-1), [1.0e6]]).reshape(1, 16385, 1)
+    wn2 = np.concatenate([wn.reshape(-1), [1.0 / 16384]]).reshape(1, 16385)
+    for n_t, mu0 in ((0.3, 0.0), (0.77, 1.5)):
+        for it in (1, 3, 25):
+            a = gpu.solve_mu_inloop(eps, wn, n_t, mu0, 0.05, it)
+            b = gpu.solve_mu_inloop(eps2, wn2, n_t, mu0, 0.05, it)
+            assert abs(a - b) <= 1e-13 * max(1.0, abs(a)), (n_t, it, a, b)
+
+
 # --------------------------------------------------------------- build_fock -
 
 def _fock_cases():

@@ -209,6 +209,61 @@ def test_oracle_mu_hits_target_multiband(oracle):
     assert abs(np.sum(w[..., None] * occn) - 3.0) < 1e-12
 
 
+def test_oracle_mu_newton_py_closed_forms(oracle):
+    """The Python surface's Newton (src/cpp_hf/utils.py:311-360)."""
+    w = np.ones((1, 1))
+    # symmetric two-level system at half filling: the bracket midpoint is
+    # the root and Newton stays on it
+    assert abs(oracle.find_mu_newton(np.array([[[0.0, 1.0]]]), w, 1.0, 0.1) - 0.5) < 1e-12
+    # no steps: the final check fails and the initial bracket's midpoint comes
+    # back, (emin + emax) / 2 (its span cancels)
+    rng = np.random.RandomState(42)
+    bands = rng.randn(4, 4, 6)
+    wk = np.ones((4, 4)) / 16
+    emin, emax = float(bands.min()), float(bands.max())
+    assert oracle.find_mu_newton(bands, wk, 3.7, 0.05, 0) == pytest.approx(
+        0.5 * (emin + emax), abs=1e-14)
+    # one step from the midpoint of a low filling: the count there is too
+    # high, so the upper bracket edge moves onto it, and the unconverged
+    # result is the midpoint of [emin - span, mid0]
+    span = emax - emin + 10 * 0.05
+    lo, hi = emin - span, emax + span
+    mid0 = 0.5 * (lo + hi)
+    assert oracle.find_mu_newton(bands, wk, 0.5, 0.05, 1) == pytest.approx(
+        0.5 * (lo + mid0), abs=1e-14)
+    # and at a high filling the lower edge moves
+    assert oracle.find_mu_newton(bands, wk, 5.5, 0.05, 1) == pytest.approx(
+        0.5 * (mid0 + hi), abs=1e-14)
+    # converged (tests/test_main.py:197-209): the count hits the target
+    for n_t in (3.0, 0.7, 5.2):
+        mu = oracle.find_mu_newton(bands, wk, n_t, 0.05, 15)
+        occ = 1.0 / (1.0 + np.exp((bands - mu) / 0.05))
+        assert abs(np.sum(wk[..., None] * occ) - n_t) < 1e-12
+
+
+# --------------------------------------------------------------- entropy ----
+
+def test_oracle_occupation_entropy_closed_forms(oracle):
+    """S = -sum w [p ln p + (1 - p) ln(1 - p)], p clipped to [1e-14, 1 - 1e-14]
+    (fock.hpp:385-400)."""
+    from scipy.special import entr
+
+    rng = np.random.default_rng(3)
+    w = rng.uniform(0.1, 1.0, (5, 7))
+    # p = 1/2: ln 2 per band
+    half = np.full((5, 7, 4), 0.5)
+    assert oracle.occupation_entropy(half, w) == pytest.approx(np.log(2) * 4 * w.sum(), rel=1e-14)
+    # p in {0, 1}: both clip to the same value -(c ln c + (1 - c) log1p(-c)), c = 1e-14
+    c = 1e-14
+    s_clip = -(c * np.log(c) + (1 - c) * np.log1p(-c))
+    ends = rng.integers(0, 2, (5, 7, 3)).astype(float)
+    assert oracle.occupation_entropy(ends, w) == pytest.approx(s_clip * 3 * w.sum(), rel=1e-12)
+    # interior p: scipy's entr (x ln x) on both sides of each occupation
+    p = rng.uniform(0.0, 1.0, (5, 7, 6))
+    ref = float(np.sum(w[..., None] * (entr(p) + entr(1.0 - p))))
+    assert oracle.occupation_entropy(p, w) == pytest.approx(ref, rel=1e-13)
+
+
 def test_oracle_rejects_unreachable_density(oracle):
     with pytest.raises(ValueError, match="physically reachable range"):
         k = two_band_kernel()
@@ -327,6 +382,15 @@ def test_oracle_scf_meshgrid_fixture(oracle):
         float(d["comm_tol"]), float(d["comm_tol"]), 0.5, 0.0, None, [], "linear", 6, 2, 1.0,
         0.0, True, True)
     assert conv
+    # Iteration count: the file's k_expected = 15 is the count of the run
+    # that wrote it, whose stored settings (diis_size 6, precond_mode "auto")
+    # and commutator history (sudden drops at steps 8 and 13) are those of a
+    # DIIS-accelerated solver.  The reference's own check solves the case
+    # with linear mixing 0.5 and only prints a note when the count differs
+    # (examples/contimod_example_meshgrid_offline.py:85-89); linear mixing
+    # takes 16 iterations.
+    assert int(d["k_expected"]) == 15 and n == 16
+    assert int(np.count_nonzero(d["hist_E_expected"])) == 15
     np.testing.assert_allclose(P, d["P_expected"], rtol=1e-6, atol=1e-6)
     np.testing.assert_allclose(F, d["F_expected"], rtol=1e-6, atol=1e-6)
     assert abs(mu - float(d["mu_expected"])) < 1e-6
