@@ -13,17 +13,17 @@ The JSON line also carries:
                 over its HIP-event-timed duration, against 8 TB/s HBM;
                 traffic = HBM bytes per build from the committed
                 FETCH_SIZE/WRITE_SIZE summary of this workload.
-  cpu_baseline  the CPU oracle's Fock build on this host, timed for a
-                bounded number of builds (a restatement of the reference's
-                FFTW/Eigen path, which cannot be built here).
+  cpu_baseline  the CPU oracle's Fock build on this host: 50-build warm-up,
+                median and interquartile range over 5 repeats (a
+                restatement of the reference's FFTW/Eigen path, which cannot
+                be built here), its one-thread rate, and the 48 sweep
+                kernels solved one after another by the oracle.
   solves        wall time of full direct-minimisation (CG and trust-region
                 Newton) and SCF (mixing 0.5) solves at C1 (QWZ 24 x 24) and
                 C2, on the GPU (Newton: GPU only) and in the
                 oracle on the host, with each solve's roofline time: the
                 reference dataflow's bytes per iteration (DM (30 + 2 trials) A
-                with the line-search trials counted, SCF 21 A) over 8 TB/s;
-                plus the oracle's sequential rate on the first kernels of the
-                48-kernel sweep (a bounded sample).
+                with the line-search trials counted, SCF 21 A) over 8 TB/s.
   mixed_precision  the FP32-workspace Fock build against the FP64 one at
                 96 x 96 (nb 8) and 192 x 192 (nb 16), and SCF solves at
                 192 x 192 with and without the mixed-precision phase: times,
@@ -34,7 +34,8 @@ The JSON line also carries:
                 Hartree shift) at 24 x 24 k (13 G x 2 orbitals) and 36 x 36 k
                 (19 G x 1 orbital), HIP-event timed, with the grid bytes one
                 build moves, the oracle's build on this host and an SCF solve.
-  autotune      the launch-shape autotuner's pick and time for the same build.
+  autotune      the launch-shape autotuner's pick (made at set-up: the timed
+                builds use it) and the default shape's time.
   symmetry      SCF solves at 96 x 96, nb 8 with a time-reversal projector
                 applied on the device (SymmetryProjector) and through the
                 host callback (a plain function): seconds and iterations.
@@ -350,23 +351,6 @@ def run_solves(cpu: bool) -> dict:
             rec["scf"]["cpu_oracle_seconds"] = time.perf_counter() - t
             rec["scf"]["cpu_oracle_iterations"] = int(sc[4])
         out[name] = rec
-    if cpu:
-        from oracle import hfcpu
-
-        from cpp_hf_amd.workloads import sweep_params, sweep_workload
-
-        ws = [sweep_workload(p) for p in sweep_params(48, seed=1)[:4]]
-        t = time.perf_counter()
-        for w in ws:
-            hfcpu.solve_dm(*_dm_args(w.kernel, w.P0, w.n_electrons))
-        dt = time.perf_counter() - t
-        out["c3_cpu_oracle_sample"] = {
-            "solves_per_s": len(ws) / dt,
-            "sample": f"the first {len(ws)} of the 48 sweep kernels "
-                      f"({', '.join(str(w.kernel.h.shape[0]) + 'x' + str(w.kernel.h.shape[0]) + ' nb' + str(w.kernel.h.shape[-1]) for w in ws)}), "
-                      "sequential DM solves",
-            "threads": hfcpu.num_threads(),
-        }
     return out
 
 
@@ -394,6 +378,57 @@ def cpu_one_thread_builds(seconds: float) -> dict:
     if r.returncode != 0:
         return {"error": r.stderr[-500:]}
     return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+CPU_LABEL = ("cpp_hf-semantics CPU restatement (own FFT/Jacobi, C++17 + OpenMP), "
+             "not the FFTW/Eigen binary")
+
+
+def cpu_baseline(ka, P_host, per_repeat: int) -> dict:
+    """The oracle's Fock builds on the host: 50-call warm-up, then 5 timed
+    repeats of `per_repeat` builds; median and interquartile range of the
+    repeats' rates."""
+    from oracle import hfcpu
+
+    for _ in range(50):
+        hfcpu.build_fock_apply(ka, P_host)
+    rates = []
+    for _ in range(5):
+        t = time.perf_counter()
+        for _ in range(per_repeat):
+            hfcpu.build_fock_apply(ka, P_host)
+        rates.append(per_repeat / (time.perf_counter() - t))
+    q1, med, q3 = (float(x) for x in np.percentile(rates, [25, 50, 75]))
+    return {
+        "value": med,
+        "unit": "Fock builds/s",
+        "cores": hfcpu.num_threads(),
+        "kind": "port",
+        "label": CPU_LABEL,
+        "sample": f"median of 5 repeats of {per_repeat} Fock builds (+ energy) of the same "
+                  "workload after a 50-build warm-up; OpenMP over k / channels, serial "
+                  "reductions",
+        "iqr": [q1, q3],
+        "repeats": rates,
+        "one_thread": cpu_one_thread_builds(5.0),
+    }
+
+
+def cpu_sweep_sequential() -> dict:
+    """The 48 sweep kernels solved one after another by the oracle (default
+    SolverConfig, DM solver), as the reference runs a sweep without a pool."""
+    from oracle import hfcpu
+
+    from cpp_hf_amd.workloads import sweep_params, sweep_workload
+
+    ws = [sweep_workload(p) for p in sweep_params(48, seed=1)]
+    t = time.perf_counter()
+    iters = 0
+    for w in ws:
+        iters += int(hfcpu.solve_dm(*_dm_args(w.kernel, w.P0, w.n_electrons))[6])
+    dt = time.perf_counter() - t
+    return {"solves_per_s": len(ws) / dt, "seconds": dt, "kernels": len(ws),
+            "total_iterations": iters, "threads": hfcpu.num_threads(), "label": CPU_LABEL}
 
 
 def run_sweep(concurrency: int) -> dict:
@@ -436,7 +471,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=96)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-repeat-builds", type=int, default=200,
+                    help="oracle Fock builds per timed repeat of the CPU baseline (5 repeats)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solves", action="store_true")
     ap.add_argument("--no-sweep", action="store_true",
@@ -511,6 +547,24 @@ def main() -> int:
         step()
     torch.cuda.synchronize()
 
+    def event_us(n):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e3 / n
+
+    # the launch-shape autotuner runs at set-up, outside the timed region, so
+    # the timed builds use the tuned shape (results do not depend on it)
+    default_us = event_us(200)
+    tuned = gk.autotune(herm)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
     # parity of this exact build against the oracle (outside the timed region)
     parity = None
     try:
@@ -572,10 +626,7 @@ def main() -> int:
         "parity_rel_err_F": parity,
     }
 
-    # the launch-shape autotuner on this build (the line above uses the
-    # default shape); results of the build do not depend on the shape
-    tuned = gk.autotune(herm)
-    out["autotune"] = {"default_us_per_build": per_build_s * 1e6, **tuned}
+    out["autotune"] = {"default_us_per_build": default_us, **tuned}
 
     if not args.no_solves:
         out["solves"] = run_solves(not args.no_cpu_baseline)
@@ -594,26 +645,10 @@ def main() -> int:
         out["symmetry"] = run_symmetry()
 
     if not args.no_cpu_baseline:
-        from oracle import hfcpu
-
-        n_done = 0
-        t = time.perf_counter()
-        while True:
-            hfcpu.build_fock_apply(ka, P_host)
-            n_done += 1
-            if time.perf_counter() - t >= args.cpu_seconds:
-                break
-        dt = time.perf_counter() - t
-        out["cpu_baseline"] = {
-            "value": n_done / dt,
-            "unit": "Fock builds/s",
-            "cores": hfcpu.num_threads(),
-            "kind": "port",
-            "sample": f"{n_done} Fock builds (+ energy) of the same workload in {dt:.1f} s, "
-                      "OpenMP over k / channels, serial reductions",
-            "one_thread": cpu_one_thread_builds(min(args.cpu_seconds, 5.0)),
-        }
+        out["cpu_baseline"] = cpu_baseline(ka, P_host, args.cpu_repeat_builds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        if not args.no_sweep:
+            out["cpu_baseline"]["sweep_48_sequential"] = cpu_sweep_sequential()
 
     print(json.dumps(out), flush=True)
     return 0

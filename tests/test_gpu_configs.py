@@ -65,11 +65,17 @@ def _check_dm(g, r):
     assert relerr(F, r[3]) < 1e-9
 
 
-def _check_scf(g, r, dmu_tol=1e-9):
+def _check_scf(g, r, T):
+    """The SCF's mu is the midpoint of 30 bisection halvings of a bracket
+    about 20-30 wide (resolution ~2e-8); a rounding-level difference of the
+    density moves the root, and when the last midpoints fall between the two
+    roots one `count > n_e` decision differs and mu moves by the resolution.
+    The occupations then move by at most dmu f (1 - f) / T <= dmu / (4 T)."""
     assert g[4] == r[4] and g[5] == r[5], (g[4], r[4], g[5], r[5])
     assert abs(g[2] - r[2]) < 1e-10 * max(1.0, abs(r[2])), (g[2], r[2])
-    assert abs(g[3] - r[3]) < dmu_tol
-    assert relerr(g[0], r[0]) < 1e-9
+    dmu = abs(g[3] - r[3])
+    assert dmu < 1e-7
+    assert relerr(g[0], r[0]) < max(1e-9, 2.0 * dmu / (4 * T))
     np.testing.assert_allclose(g[6], r[6], rtol=1e-10, atol=1e-10)
 
 
@@ -98,7 +104,7 @@ def test_qwz24_scf_matches_oracle(gpu, oracle):
     wl = qwz_kernel(24)
     args = _scf_args(wl.kernel, wl.P0, wl.n_electrons)
     print("qwz24 scf", flush=True)
-    _check_scf(gpu.solve_scf(*args), oracle.solve_scf(*args))
+    _check_scf(gpu.solve_scf(*args), oracle.solve_scf(*args), 0.1)
 
 
 # ------------------------------------------- 96 x 96, 8 layered bands ------
@@ -119,15 +125,12 @@ def test_layered96_dm_solve_matches_oracle(gpu, oracle, c2):
 
 
 def test_layered96_scf_solve_matches_oracle(gpu, oracle, c2):
-    """solve_scf(mixing=0.5): 200 iterations, unconverged on both sides.  mu
-    comes from 30 bisection halvings of a bracket ~30 wide (resolution
-    ~3e-8); one rounding-level `count > n_e` decision moves it by that much,
-    and the iterate follows to first order."""
+    """solve_scf(mixing=0.5): 200 iterations, unconverged on both sides."""
     args = _scf_args(c2.kernel, c2.P0, c2.n_electrons)
     print("layered96 scf solve", flush=True)
     g = gpu.solve_scf(*args)
     r = oracle.solve_scf(*args)
-    _check_scf(g, r, dmu_tol=1e-7)
+    _check_scf(g, r, 0.05)
 
 
 # --------------------------------------------------- the parameter sweep ----
@@ -187,7 +190,10 @@ def test_layered192_nb16_fock_build_matches_oracle(gpu, oracle):
     Sr, Hr, Fr, Er = oracle.build_fock_apply(ka, P)
     assert relerr(S, Sr) < 1e-12
     assert relerr(F, Fr) < 1e-12
-    assert relerr(Hh, Hr) < 1e-12
+    # the Hartree term HH n with n = sum_k w_k Re(P - refP)_ii: 36864 terms
+    # of ~1e-6 with cancellation, so its summation order sets ~4e-13 of a
+    # 1.9e-3 value; bounded on the scale of F, as the matrix it enters
+    assert float(np.max(np.abs(Hh - Hr))) < 1e-12 * float(np.max(np.abs(Fr)))
     assert abs(E - Er) <= 1e-12 * max(1.0, abs(Er))
     np.testing.assert_array_equal(F, np.conj(np.swapaxes(F, -1, -2)))
     S32, H32, F32, E32 = gpu.build_fock_apply(ka, P, None, exchange_fp32=True)
