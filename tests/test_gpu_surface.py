@@ -436,7 +436,8 @@ def test_solve_scf_trust_radius_matches_oracle(gpu, oracle, accel, radius):
     else:
         assert abs(g[4] - r[4]) <= 5
         assert abs(g[2] - r[2]) < 1e-9 * max(1.0, abs(r[2]))
-        assert relerr(g[0], r[0]) < 1e-6
+        # both densities satisfy comm_tol = 1e-6 and density_tol = 1e-7
+        assert relerr(g[0], r[0]) < 1e-5
         np.testing.assert_allclose(g[7][:10], r[7][:10], rtol=1e-8, atol=1e-12)
     # the clip is active: each step moves the density by the radius, where
     # the unclipped run's second step halves the residual
