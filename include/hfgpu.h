@@ -258,6 +258,21 @@ int hfgpu_solve_scf(hfgpu_kernel* k, const double* P0, double n_e, const hf_scf_
 int hfgpu_solve_dm(hfgpu_kernel* k, const double* P0, double n_e, const hf_dm_config* cfg,
                    hf_project_fn project, void* user, hf_dm_result* out);
 
+/* density_from_fock on device pointers that allocates nothing: F (nk, nb,
+ * nb) complex and weights (nk) in, P (nk, nb, nb) complex and mu (one
+ * double) out, all device memory, on `stream` (a hipStream_t; NULL: the null
+ * stream), asynchronous.  The eigenvectors, eigenvalues, occupations and the
+ * bisection workspace live in a fixed 16 MiB device array of the library
+ * (nk nb^2 up to about 10^6, e.g. 96 x 96 k with 8 bands); larger inputs are
+ * rejected with HF_EINVAL.  The array is shared by every call on a GPU:
+ * calls that overlap on one GPU race.  Results equal hfgpu_density_from_fock
+ * bit for bit.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). */
+int hfgpu_density_from_fock_noalloc(const double* dF, const double* dweights, size_t nk,
+                                    size_t nb, double n_e, double T, int maxiter, double* dP,
+                                    double* dmu, void* stream);
+
 /* ------------------------------------------------------------------------
  * Cayley spectral helpers (test hooks of the reference).
  * Replaces: _native._cayley_spectral_setup / _cayley_unitary_from_spectrum /
