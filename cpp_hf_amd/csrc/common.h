@@ -138,6 +138,14 @@ __device__ __forceinline__ double block_min(double v, double* red) {
     return s;
 }
 
+// expit as in utils.hpp:18-25 (1 / (1 + e^-x) for x >= 0, e^x / (1 + e^x)
+// otherwise), written without a branch: both forms share e = exp(-|x|), so
+// lanes of either sign run the same instructions and round identically.
+__device__ __forceinline__ double expit_fd(double x) {
+    const double e = exp(-fabs(x));
+    return (x >= 0.0 ? 1.0 : e) / (1.0 + e);
+}
+
 // Wave-wide sum (64 lanes), result valid in every lane.
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -19,24 +19,43 @@ namespace {
 // row-major block read with stride nb put every lane of a row on one bank
 // group at nb = 16), and the row index i is a broadcast.  NB > 0 fixes nb at
 // compile time.
+//
+// Occupations come from f, or (f null) from the eigenvalues and the chemical
+// potential as launch_occupations forms them: mode 0 (SCF)
+// expit(-(eps - mu) / (T + 1e-12)), mode 1 (DM) expit((mu - eps) / T).
+//
+// hermitize: P = 1/2 (X + X^H) with X = V diag(f) V^H.  Each lane forms its
+// own X_ij once; the mirrored X_ji comes from its lane through LDS (the same
+// sum, so the result equals the hermitisation of X bit for bit, and P comes
+// out exactly Hermitian).
 template <int NB>
 __global__ void __launch_bounds__(256)
 k_density(int nk, int nb_arg, const cd* __restrict__ V, const double* __restrict__ f,
-          cd* __restrict__ P, int hermitize) {
+          const double* __restrict__ eps, const MuState* __restrict__ mus, double T,
+          int occ_mode, cd* __restrict__ P, int hermitize) {
     const int nb = NB > 0 ? NB : nb_arg;
     extern __shared__ cd smem[];
     const int nb2 = nb * nb, ld = nb + 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = blockIdx.x * 4 + wave;
-    cd* sv = smem + wave * (nb * ld + nb);
+    cd* sv = smem + wave * (nb * ld + nb + nb2);
     double* sf = reinterpret_cast<double*>(sv + nb * ld);
+    cd* sx = sv + nb * ld + nb;
     if (k < nk) {
         const size_t kb = static_cast<size_t>(k) * nb2;
         for (int e = lane; e < nb2; e += 64) {
             const int j = e / nb, n = e - j * nb;
             sv[n * ld + j] = V[kb + e];
         }
-        for (int e = lane; e < nb; e += 64) sf[e] = f[static_cast<size_t>(k) * nb + e];
+        if (f) {
+            for (int e = lane; e < nb; e += 64) sf[e] = f[static_cast<size_t>(k) * nb + e];
+        } else {
+            const double mu = mus->mu;
+            for (int e = lane; e < nb; e += 64) {
+                const double x = eps[static_cast<size_t>(k) * nb + e];
+                sf[e] = occ_mode == 0 ? expit_fd(-(x - mu) / (T + 1.0e-12)) : expit_fd((mu - x) / T);
+            }
+        }
     }
     __syncthreads();
     if (k >= nk) return;
@@ -46,13 +65,17 @@ k_density(int nk, int nb_arg, const cd* __restrict__ V, const double* __restrict
         cd pij{0.0, 0.0};
         for (int n = 0; n < nb; ++n)
             pij = cadd(pij, cmulc(cscale(sv[n * ld + i], sf[n]), sv[n * ld + j]));
-        if (hermitize) {
-            cd pji{0.0, 0.0};
-            for (int n = 0; n < nb; ++n)
-                pji = cadd(pji, cmulc(cscale(sv[n * ld + j], sf[n]), sv[n * ld + i]));
-            pij = cd{0.5 * (pij.x + pji.x), 0.5 * (pij.y - pji.y)};
-        }
-        P[kb + e] = pij;
+        if (hermitize)
+            sx[e] = pij;
+        else
+            P[kb + e] = pij;
+    }
+    if (!hermitize) return;
+    __syncthreads();
+    for (int e = lane; e < nb2; e += 64) {
+        const int i = e / nb, j = e - i * nb;
+        const cd pij = sx[e], pji = sx[j * nb + i];
+        P[kb + e] = cd{0.5 * (pij.x + pji.x), 0.5 * (pij.y - pji.y)};
     }
 }
 
@@ -77,20 +100,31 @@ __global__ void k_hermitize(int nk, int nb, cd* __restrict__ M) {
 
 }  // namespace
 
-void launch_density(int nk, int nb, const cd* V, const double* f, cd* P, bool hermitize,
-                    hipStream_t st) {
-    const size_t lds = 4 * (sizeof(cd) * (nb * (nb + 1) + nb));
+static void density_launch(int nk, int nb, const cd* V, const double* f, const double* eps,
+                           const MuState* mus, double T, int mode, cd* P, bool hermitize,
+                           hipStream_t st) {
+    const size_t lds = 4 * (sizeof(cd) * (nb * (nb + 1) + nb + nb * nb));
     const dim3 grid(cdiv(nk, 4)), block(256);
     const int h = hermitize ? 1 : 0;
     switch (nb) {
-        case 2: hipLaunchKernelGGL(k_density<2>, grid, block, lds, st, nk, nb, V, f, P, h); break;
-        case 4: hipLaunchKernelGGL(k_density<4>, grid, block, lds, st, nk, nb, V, f, P, h); break;
-        case 8: hipLaunchKernelGGL(k_density<8>, grid, block, lds, st, nk, nb, V, f, P, h); break;
-        case 12: hipLaunchKernelGGL(k_density<12>, grid, block, lds, st, nk, nb, V, f, P, h); break;
-        case 16: hipLaunchKernelGGL(k_density<16>, grid, block, lds, st, nk, nb, V, f, P, h); break;
-        default: hipLaunchKernelGGL(k_density<0>, grid, block, lds, st, nk, nb, V, f, P, h); break;
+        case 2: hipLaunchKernelGGL(k_density<2>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
+        case 4: hipLaunchKernelGGL(k_density<4>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
+        case 8: hipLaunchKernelGGL(k_density<8>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
+        case 12: hipLaunchKernelGGL(k_density<12>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
+        case 16: hipLaunchKernelGGL(k_density<16>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
+        default: hipLaunchKernelGGL(k_density<0>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
     }
     HFGPU_LAUNCH_CHECK();
+}
+
+void launch_density(int nk, int nb, const cd* V, const double* f, cd* P, bool hermitize,
+                    hipStream_t st) {
+    density_launch(nk, nb, V, f, nullptr, nullptr, 0.0, 0, P, hermitize, st);
+}
+
+void launch_density_occ(int nk, int nb, const cd* V, const double* eps, const MuState* mus,
+                        double T, int mode, cd* P, bool hermitize, hipStream_t st) {
+    density_launch(nk, nb, V, nullptr, eps, mus, T, mode, P, hermitize, st);
 }
 
 void launch_hermitize(int nk, int nb, cd* M, hipStream_t st) {

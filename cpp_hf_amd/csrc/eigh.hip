@@ -257,12 +257,13 @@ __device__ __forceinline__ cd group_bcast(cd v, int src) {
 template <int NP, int NBC>
 __global__ void __launch_bounds__(256)
 k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __restrict__ w_out,
-           cd* __restrict__ V_out, BlockMap bm, int sort, const double* __restrict__ cond) {
+           cd* __restrict__ V_out, BlockMap bm, int sort, const double* __restrict__ cond,
+           int mpw) {
     // the block structure applies unless cond = (off-block max, max) says the
     // batch is not block-diagonal (solver_scf.hpp:111-120): then a dense eigh
     const bool ub = !cond || cond[0] <= 1.0e-10 + 1.0e-10 * cond[1];
     if (!ub) sort = 1;
-    constexpr int G = 64 / NP;  // matrices per wavefront
+    // lane groups of NP lanes: 64 / NP matrices per full wavefront (mpw used)
     // matrix dimension of the register rows: NBC when the launch fixes nb at
     // compile time (lanes NBC..NP-1 of a group are then padding), else NP
     constexpr int NL = NBC > 0 ? NBC : NP;
@@ -275,8 +276,11 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
     TriLds<NP>& X = lds[threadIdx.x >> 6];
     const int i = lane % NP;
     const int gbase = lane - i;
-    const int mat = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * G + lane / NP;
-    const bool live = mat < nmat;
+    // mpw <= G matrices per wavefront: fewer than G leaves lane groups idle
+    // but puts more wavefronts on each SIMD to hide the latency of the
+    // serial QR recurrence
+    const int mat = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * mpw + lane / NP;
+    const bool live = lane / NP < mpw && mat < nmat;
     const int nb2 = nb * nb;
     const cd* Mk = M + static_cast<size_t>(live ? mat : 0) * nb2;
     constexpr bool kZLds = NP == 32;
@@ -480,6 +484,23 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
 
 }  // namespace
 
+// Matrices per wavefront of the register kernels: a full wavefront (g).
+// Fewer leave lane groups idle to put more wavefronts on each SIMD; that
+// measured slower (96 x 96 x 8: 72 us at 8 per wavefront, 99 us at 4, 151 us
+// at 2): the kernel is bound by VALU issue, not by the latency of its serial
+// recurrences.  HFGPU_EIGH_MPW overrides for experiments.
+static int eigh_mpw(int nmat, int g) {
+    (void)nmat;
+    static const int env = [] {
+        const char* e = std::getenv("HFGPU_EIGH_MPW");
+        return e ? std::atoi(e) : 0;
+    }();
+    if (env <= 0) return g;
+    int p = 1;
+    while (p < env && p < g) p <<= 1;
+    return std::min(p, g);
+}
+
 void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V,
                         const BlockMap& bm, bool sort, hipStream_t st, const double* cond) {
     if (nb < 1 || nb > kMaxNb)
@@ -488,18 +509,19 @@ void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, c
     const int so = sort ? 1 : 0;
     if (nb <= 16) {
         const int np = nb <= 2 ? 2 : (nb <= 4 ? 4 : (nb <= 8 ? 8 : 16));
-        const int per_block = 4 * (64 / np);
+        const int mpw = eigh_mpw(nmat, 64 / np);
+        const int per_block = 4 * mpw;
         const dim3 grid(cdiv(nmat, per_block));
         if (np == 2)
-            hipLaunchKernelGGL((k_eigh_tri<2, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
+            hipLaunchKernelGGL((k_eigh_tri<2, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw);
         else if (np == 4)
-            hipLaunchKernelGGL((k_eigh_tri<4, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
+            hipLaunchKernelGGL((k_eigh_tri<4, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw);
         else if (np == 8)
-            hipLaunchKernelGGL((k_eigh_tri<8, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
+            hipLaunchKernelGGL((k_eigh_tri<8, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw);
         else if (nb == 12)  // the 12-band models: loops sized to 12, not 16
-            hipLaunchKernelGGL((k_eigh_tri<16, 12>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
+            hipLaunchKernelGGL((k_eigh_tri<16, 12>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw);
         else
-            hipLaunchKernelGGL((k_eigh_tri<16, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond);
+            hipLaunchKernelGGL((k_eigh_tri<16, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw);
         HFGPU_LAUNCH_CHECK();
         return;
     }
@@ -513,7 +535,7 @@ void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, c
         const dim3 grid(cdiv(nmat, 2));
         const size_t zl = sizeof(cd) * 64 * 33;
         hipLaunchKernelGGL((k_eigh_tri<32, 0>), grid, dim3(64), zl, st, nmat, nb, in_mode, M, w,
-                           V, bm, so, cond);
+                           V, bm, so, cond, 2);
         HFGPU_LAUNCH_CHECK();
         return;
     }

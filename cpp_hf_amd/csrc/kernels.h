@@ -44,9 +44,10 @@ struct MuState {
 };
 
 // Workspace of the mu solvers (doubles): two partial-sum buffers of up to
-// kMuBlocks workgroups x 32 slots, then two ping-pong states.
+// kMuBlocks workgroups x 32 slots and two ping-pong states (the first 16384),
+// then the exchange slots of the one-launch solves (rounds x 256 x 8).
 constexpr int kMuBlocks = 240;
-constexpr int kMuPartialDoubles = 16384;
+constexpr int kMuPartialDoubles = 16384 + 28 * 256 * 8;
 
 // find_mu_bisection (utils.hpp:49-75): result mu = (lo+hi)/2 left in
 // state->mu.  ws: kMuPartialDoubles of workspace.
@@ -77,6 +78,10 @@ void launch_level_shift(const double* eps, size_t n, const MuState* state, doubl
 // P_k = herm(V_k diag(f_k) V_k^H) when hermitize, else V diag(f) V^H.
 void launch_density(int nk, int nb, const cd* V, const double* f, cd* P, bool hermitize,
                     hipStream_t st);
+// The same with the occupations formed from eigenvalues and mu as
+// launch_occupations does (mode 0 SCF, mode 1 DM), without storing them.
+void launch_density_occ(int nk, int nb, const cd* V, const double* eps, const MuState* mus,
+                        double T, int mode, cd* P, bool hermitize, hipStream_t st);
 // in-place hermitisation of (nk, nb, nb)
 void launch_hermitize(int nk, int nb, cd* M, hipStream_t st);
 

@@ -47,13 +47,7 @@ static_assert(kPart1 + kMuBlocks * kNodeStride <= kStates, "mu workspace layout"
 static_assert(kStates + 2 * kStateDoubles <= kMuPartialDoubles, "mu workspace size");
 static_assert(sizeof(MuState) <= kStateDoubles * sizeof(double), "MuState slot");
 
-// expit as in utils.hpp:18-25 (1 / (1 + e^-x) for x >= 0, e^x / (1 + e^x)
-// otherwise), written without a branch: both forms share e = exp(-|x|), so
-// lanes of either sign run the same instructions and round identically.
-__device__ __forceinline__ double expit_d(double x) {
-    const double e = exp(-fabs(x));
-    return (x >= 0.0 ? 1.0 : e) / (1.0 + e);
-}
+__device__ __forceinline__ double expit_d(double x) { return expit_fd(x); }
 
 __device__ __forceinline__ MuState* pp_state(double* ws, int k) {
     return reinterpret_cast<MuState*>(ws + kStates + (k & 1) * kStateDoubles);
@@ -134,30 +128,35 @@ __device__ void build_tree(double lo, double hi, int nodes, double* s_mid, doubl
 
 // Apply `levels` bisection steps to st from the per-workgroup node counts of
 // the launch that evaluated st's midpoint tree.  Counts are summed in a fixed
-// order (deterministic, identical in every workgroup).
-__device__ MuState bisect_apply(MuState st, const double* __restrict__ part, int nparts,
-                                int levels, double n_e) {
-    constexpr int kGroups = kBlock / kNodeStride;  // 8 row groups of 32 node slots
+// order (deterministic, identical in every workgroup).  Split in two so a
+// caller can issue the partial loads (bisect_load) together with its other
+// loads and apply them (bisect_reduce) once the state has arrived.
+constexpr int kGroups = kBlock / kNodeStride;  // 8 row groups of 32 node slots
+constexpr int kPartPer = kMuBlocks / kGroups;  // partials per thread
+static_assert(kPartPer * kGroups == kMuBlocks, "partials per row group");
+
+__device__ __forceinline__ void bisect_load(const double* __restrict__ part, int nparts,
+                                            int nodes, double (&v)[kPartPer]) {
+    const int nd = threadIdx.x % kNodeStride, g = threadIdx.x / kNodeStride;
+#pragma unroll
+    for (int u = 0; u < kPartPer; ++u) {
+        const int b = g + u * kGroups;
+        v[u] = (nd < nodes && b < nparts) ? part[b * kNodeStride + nd] : 0.0;
+    }
+}
+
+__device__ MuState bisect_reduce(MuState st, const double (&v)[kPartPer], int levels,
+                                 double n_e) {
     __shared__ double s_mid[kMaxNodes], s_lo[kMaxNodes], s_hi[kMaxNodes];
     __shared__ double s_grp[kGroups][kNodeStride];
     __shared__ double s_cnt[kMaxNodes];
     const int nodes = (1 << levels) - 1;
     if (threadIdx.x == 0) build_tree(st.lo, st.hi, nodes, s_mid, s_lo, s_hi);
     const int nd = threadIdx.x % kNodeStride, g = threadIdx.x / kNodeStride;
+    // the sum in workgroup order b = g, g + 8, ...
     double s = 0.0;
-    if (nd < nodes) {
-        // eight loads in flight per step
-        for (int b0 = g; b0 < nparts; b0 += 8 * kGroups) {
-            double v[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int b = b0 + u * kGroups;
-                v[u] = b < nparts ? part[b * kNodeStride + nd] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) s += v[u];
-        }
-    }
+    for (int u = 0; u < kPartPer; ++u) s += v[u];
     s_grp[g][nd] = s;
     __syncthreads();
     if (threadIdx.x < nodes) {
@@ -185,6 +184,13 @@ __device__ MuState bisect_apply(MuState st, const double* __restrict__ part, int
     return st;
 }
 
+__device__ MuState bisect_apply(MuState st, const double* __restrict__ part, int nparts,
+                                int levels, double n_e) {
+    double v[kPartPer];
+    bisect_load(part, nparts, (1 << levels) - 1, v);
+    return bisect_reduce(st, v, levels, n_e);
+}
+
 // Step j of the bisection: bring the state up to date from the previous
 // launch's partials (the min/max partials for j = 0), record it, then
 // evaluate the electron count at the NODES midpoints of the next levels.
@@ -195,11 +201,26 @@ k_bisect_step(const double* __restrict__ eps, const double* __restrict__ w, int 
     __shared__ double red[kBlock / 64];
     __shared__ double s_mid[kMaxNodes], s_lo[kMaxNodes], s_hi[kMaxNodes];
     __shared__ double s_red[kBlock / 64][kNodeStride];
+    // this thread's first two eigenvalues (all of them at the usual sizes):
+    // loaded before the state update so both memory round trips overlap
+    const size_t i0 = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x;
+    const size_t stride = static_cast<size_t>(gridDim.x) * kBlock;
+    double pe[2], pw[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const size_t i = i0 + q * stride;
+        pe[q] = i < n ? eps[i] : 0.0;
+        pw[q] = i < n ? w[i / nb] : 0.0;
+    }
     MuState st;
-    if (j == 0)
+    if (j == 0) {
         st = bracket_from(pp_part(ws, 1), prev_parts, 0, T, 0.0, red);
-    else
-        st = bisect_apply(*pp_state(ws, j - 1), pp_part(ws, j - 1), prev_parts, prev_levels, n_e);
+    } else {
+        // the previous launch's partials and state, loads in flight together
+        double v[kPartPer];
+        bisect_load(pp_part(ws, j - 1), prev_parts, (1 << prev_levels) - 1, v);
+        st = bisect_reduce(*pp_state(ws, j - 1), v, prev_levels, n_e);
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) *pp_state(ws, j) = st;
     if (threadIdx.x == 0) build_tree(st.lo, st.hi, NODES, s_mid, s_lo, s_hi);
     __syncthreads();
@@ -210,8 +231,13 @@ k_bisect_step(const double* __restrict__ eps, const double* __restrict__ w, int 
         acc[nd] = 0.0;
     }
     const double denom = fmax(T, 1.0e-12) + 1.0e-12;  // fermidirac_scalar(x, Tj)
-    for (size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; i < n;
-         i += static_cast<size_t>(gridDim.x) * kBlock) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        if (i0 + q * stride >= n) break;
+#pragma unroll
+        for (int nd = 0; nd < NODES; ++nd) acc[nd] += pw[q] * expit_d(-(pe[q] - mid[nd]) / denom);
+    }
+    for (size_t i = i0 + 2 * stride; i < n; i += stride) {
         const double e = eps[i];
         const double wk = w[i / nb];
 #pragma unroll
@@ -361,6 +387,299 @@ k_newton_last(double n_target, int j, int prev_parts, int apply, double* ws, MuS
     if (threadIdx.x == 0) *state = st;
 }
 
+// ---- persistent multi-workgroup solves ------------------------------------
+// One launch runs a whole bisection or Newton solve on G workgroups.  Each
+// round ends with a grid-wide exchange of a few partial sums through
+// "slots" in the workspace: a workgroup publishes its values with atomic
+// stores at agent scope, and every workgroup gathers all G x K values with
+// atomic loads of the same scope, polling each slot until it no longer holds
+// the empty pattern (a NaN payload no arithmetic produces) that a fill
+// kernel wrote before the launch.  The data is its own flag, so no fence or
+// cache write-back is needed; atomics on one location are coherent across
+// the XCDs.  Every workgroup then sums the G values in the same fixed order
+// and takes the same decision, so all of them follow one trajectory and
+// leave the loop together.  Polls are bounded: a timeout (workgroups that
+// could not all be resident, which the small grids here make unlikely)
+// leaves NaN in the state instead of hanging.
+
+constexpr int kPersistSlotBase = 16384;  // after the multi-launch layout
+constexpr int kPersistK = 8;             // slot doubles per workgroup per round
+constexpr int kPersistMaxG = 256;
+constexpr unsigned long long kEmptySlot = 0x7FF4DEADBEEF0001ull;
+constexpr long long kSpinMax = 1ll << 24;
+
+__global__ void k_fill_slots(unsigned long long* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = kEmptySlot;
+}
+
+__device__ __forceinline__ void slot_put(double* s, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(s),
+                       static_cast<unsigned long long>(__double_as_longlong(v)),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long slot_get(const double* s) {
+    return __hip_atomic_load(reinterpret_cast<const unsigned long long*>(s), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Publish mine[0..K) (LDS, thread k < K holds the value) for round r and
+// gather every workgroup's into all[g * K + k] (LDS).  Returns false after a
+// poll timeout (uniform over the workgroup).
+template <int K>
+__device__ bool slot_exchange(double* slots, int r, int G, const double* mine, double* all,
+                              int* s_flag) {
+    double* base = slots + static_cast<size_t>(r) * G * kPersistK;
+    if (threadIdx.x < K) slot_put(base + blockIdx.x * kPersistK + threadIdx.x, mine[threadIdx.x]);
+    if (threadIdx.x == 0) *s_flag = 0;
+    constexpr int kPer = (kPersistMaxG * K + kBlock - 1) / kBlock;
+    unsigned long long u[kPer];
+    // every load of the first poll in flight at once
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int i = threadIdx.x + q * kBlock;
+        u[q] = i < G * K ? slot_get(base + (i / K) * kPersistK + i % K) : 0ull;
+    }
+    bool ok = true;
+    for (long long it = 0;; ++it) {
+        bool empty = false;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = threadIdx.x + q * kBlock;
+            if (i < G * K && u[q] == kEmptySlot) {
+                u[q] = slot_get(base + (i / K) * kPersistK + i % K);
+                empty = empty || u[q] == kEmptySlot;
+            }
+        }
+        if (!empty) break;
+        if (it > kSpinMax) {
+            ok = false;
+            break;
+        }
+        if (it < 8)
+            __builtin_amdgcn_s_sleep(1);
+        else
+            __builtin_amdgcn_s_sleep(8);  // back off: polls are loads through the fabric
+    }
+    __syncthreads();  // s_flag reset before any thread reports
+    if (!ok) *s_flag = 1;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int i = threadIdx.x + q * kBlock;
+        if (i < G * K) all[i] = __longlong_as_double(static_cast<long long>(u[q]));
+    }
+    __syncthreads();
+    return *s_flag == 0;
+}
+
+// Fixed-order reductions over the G gathered values of each k (identical in
+// every workgroup): thread k < K walks g = 0 .. G - 1.
+template <int K>
+__device__ void slot_sum(const double* all, int G, double* out) {
+    if (threadIdx.x < K) {
+        double s = 0.0;
+        for (int g = 0; g < G; ++g) s += all[g * K + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+// Extremes of eps over the grid (round 0 of both solves).
+__device__ bool persist_minmax(const double* __restrict__ eps, size_t n, double* slots, int G,
+                               double* s_mine, double* s_all, double* red, int* s_flag,
+                               double& emin, double& emax) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; i < n;
+         i += static_cast<size_t>(G) * kBlock) {
+        lo = fmin(lo, eps[i]);
+        hi = fmax(hi, eps[i]);
+    }
+    lo = block_min<kBlock>(lo, red);
+    hi = block_max<kBlock>(hi, red);
+    if (threadIdx.x == 0) {
+        s_mine[0] = lo;
+        s_mine[1] = hi;
+    }
+    __syncthreads();
+    if (!slot_exchange<2>(slots, 0, G, s_mine, s_all, s_flag)) return false;
+    emin = INFINITY;
+    emax = -INFINITY;
+    for (int g = 0; g < G; ++g) {
+        emin = fmin(emin, s_all[2 * g]);
+        emax = fmax(emax, s_all[2 * g + 1]);
+    }
+    __syncthreads();
+    return true;
+}
+
+// find_mu_bisection (utils.hpp:49-75) as a multisection: each round
+// evaluates the count at the 2^L - 1 midpoints of the next L levels (tree
+// built with the sequential loop's (lo + hi) / 2) and walks them with the
+// reference's `count > n_e` decisions.
+template <int NODES>
+__global__ void __launch_bounds__(kBlock)
+k_bisect_persist(const double* __restrict__ eps, const double* __restrict__ w, int nb, size_t n,
+                 double n_e, double T, int maxiter, int levels, double* slots,
+                 MuState* __restrict__ state) {
+    __shared__ double red[kBlock / 64];
+    __shared__ double s_mid[kMaxNodes], s_lo[kMaxNodes], s_hi[kMaxNodes];
+    __shared__ double s_red[kBlock / 64][kNodeStride];
+    __shared__ double s_mine[kPersistK], s_cnt[kPersistK];
+    __shared__ double s_all[kPersistMaxG * kPersistK];
+    __shared__ int s_flag;
+    const int G = gridDim.x;
+    double emin, emax;
+    MuState st{};
+    if (!persist_minmax(eps, n, slots, G, s_mine, s_all, red, &s_flag, emin, emax)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st.mu = NAN;
+            st.done = -1;
+            *state = st;
+        }
+        return;
+    }
+    const double Tj = fmax(T, 1.0e-12);
+    const double span = emax - emin + 10.0 * fmax(Tj, 1.0e-6);
+    double lo = emin - span, hi = emax + span;
+    const double denom = Tj + 1.0e-12;  // fermidirac_scalar(x, Tj)
+    int r = 1;
+    for (int done = 0; done < maxiter; ++r) {
+        const int lv = min(levels, maxiter - done);
+        const int nodes = (1 << lv) - 1;
+        if (threadIdx.x == 0) build_tree(lo, hi, nodes, s_mid, s_lo, s_hi);
+        __syncthreads();
+        double mid[NODES], acc[NODES];
+#pragma unroll
+        for (int nd = 0; nd < NODES; ++nd) {
+            mid[nd] = nd < nodes ? s_mid[nd] : 0.0;
+            acc[nd] = 0.0;
+        }
+        for (size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; i < n;
+             i += static_cast<size_t>(G) * kBlock) {
+            const double e = eps[i];
+            const double wk = w[i / nb];
+#pragma unroll
+            for (int nd = 0; nd < NODES; ++nd)
+                if (nd < nodes) acc[nd] += wk * expit_d(-(e - mid[nd]) / denom);
+        }
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+        for (int nd = 0; nd < NODES; ++nd) {
+            const double sv = wave_sum(acc[nd]);
+            if (lane == 0) s_red[wave][nd] = sv;
+        }
+        __syncthreads();
+        if (threadIdx.x < NODES) {
+            double sv = 0.0;
+            for (int wv = 0; wv < kBlock / 64; ++wv) sv += s_red[wv][threadIdx.x];
+            s_mine[threadIdx.x] = sv;
+        }
+        __syncthreads();
+        if (!slot_exchange<NODES>(slots, r, G, s_mine, s_all, &s_flag)) {
+            lo = hi = NAN;
+            break;
+        }
+        slot_sum<NODES>(s_all, G, s_cnt);
+        int k = 0;
+        for (int l = 0; l < lv; ++l) {
+            const double m = s_mid[k];
+            if (s_cnt[k] > n_e) {
+                hi = m;
+                k = 2 * k + 1;
+            } else {
+                lo = m;
+                k = 2 * k + 2;
+            }
+        }
+        done += lv;
+        __syncthreads();  // s_mid / s_cnt are rewritten by the next round
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st.emin = emin;
+        st.emax = emax;
+        st.lo = lo;
+        st.hi = hi;
+        st.mu = 0.5 * (lo + hi);
+        st.done = isnan(st.mu) ? -1 : 0;
+        *state = st;
+    }
+}
+
+// solve_mu_inloop (utils.hpp:79-116): Newton with the bisection safeguard,
+// stopping at an exact fixed point as the multi-launch path does.
+__global__ void __launch_bounds__(kBlock)
+k_newton_persist(const double* __restrict__ eps, const double* __restrict__ w_norm, int nb,
+                 size_t n, double n_target, double T, double mu0, int mu0_from_state,
+                 const MuState* warm, int maxiter, double* slots, MuState* state) {
+    __shared__ double red[kBlock / 64];
+    __shared__ double s_mine[kPersistK], s_sum[kPersistK];
+    __shared__ double s_all[kPersistMaxG * 2];
+    __shared__ int s_flag;
+    const int G = gridDim.x;
+    // warm may be state: every workgroup reads it before block 0 writes it
+    const double m0 = mu0_from_state ? warm->mu : mu0;
+    double emin, emax;
+    MuState st{};
+    if (!persist_minmax(eps, n, slots, G, s_mine, s_all, red, &s_flag, emin, emax)) {
+        st.mu = NAN;
+        st.done = -1;
+    } else {
+        const double Tj = fmax(T, 1.0e-12);
+        const double l = emin - 50.0 * Tj, h = emax + 50.0 * Tj;
+        st.emin = emin;
+        st.emax = emax;
+        st.lo = l;
+        st.hi = h;
+        st.mu = m0 < l ? l : (h < m0 ? h : m0);  // std::clamp
+        for (int j = 0; j < maxiter && !st.done; ++j) {
+            const double mu = st.mu;
+            double N = 0.0, Z = 0.0;
+            for (size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; i < n;
+                 i += static_cast<size_t>(G) * kBlock) {
+                const double wk = w_norm[i / nb];
+                const double p = expit_d((mu - eps[i]) / Tj);
+                N += wk * p;
+                Z += wk * p * (1.0 - p) / Tj;
+            }
+            N = block_sum<kBlock>(N, red);
+            Z = block_sum<kBlock>(Z, red);
+            if (threadIdx.x == 0) {
+                s_mine[0] = N;
+                s_mine[1] = Z;
+            }
+            __syncthreads();
+            if (!slot_exchange<2>(slots, j + 1, G, s_mine, s_all, &s_flag)) {
+                st.mu = NAN;
+                st.done = -1;
+                break;
+            }
+            slot_sum<2>(s_all, G, s_sum);
+            const double Ns = s_sum[0], Zs = s_sum[1];
+            __syncthreads();
+            // the update of newton_apply, from the exchanged sums
+            double lo = st.lo, hi = st.hi;
+            const double lo0 = lo, hi0 = hi;
+            const double g = Ns - n_target;
+            if (g < 0.0) lo = mu;
+            if (g > 0.0) hi = mu;
+            double mu_new = mu - g / fmax(Zs, 1.0e-18);
+            if (g != 0.0 && mu_new == mu) {
+                st.done = 1;
+                break;
+            }
+            const double mu_bis = 0.5 * (lo + hi);
+            if (mu_new <= lo || mu_new >= hi) mu_new = mu_bis;
+            mu_new = mu_new < lo ? lo : (hi < mu_new ? hi : mu_new);
+            if (!isfinite(mu_new)) mu_new = mu_bis;
+            if (mu_new == mu && lo == lo0 && hi == hi0) st.done = 1;
+            st.lo = lo;
+            st.hi = hi;
+            st.mu = mu_new;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *state = st;
+}
+
 // ---- the Python surface's Newton (utils.py:311-360) -------------------------
 // One workgroup runs every step: the bracket from the extremes, maxiter
 // Newton steps with the bisection safeguard on raw weights, then the final
@@ -437,10 +756,56 @@ unsigned reduce_blocks(size_t n, unsigned per_thread) {
 
 }  // namespace
 
+// Persistent (one-launch) solves: on unless HFGPU_MU_PERSIST=0.  Grid of G
+// workgroups with about one eigenvalue per thread, at most 256 (one per CU).
+static bool persist_on() {
+    static const bool on = [] {
+        const char* e = std::getenv("HFGPU_MU_PERSIST");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+// The bisection's one-launch form is kept behind HFGPU_MU_PERSIST_BISECT=1:
+// at 96 x 96 x 8 its 11 exchange rounds took 87 us against about 40 us for
+// the multi-launch multisection, whose launch boundaries cost less than a
+// grid-wide exchange through atomics.
+static bool persist_bisect_on() {
+    static const bool on = [] {
+        const char* e = std::getenv("HFGPU_MU_PERSIST_BISECT");
+        return e && e[0] == '1';
+    }();
+    return on && persist_on();
+}
+static int persist_grid(size_t n) {
+    return static_cast<int>(std::max<size_t>(1, std::min<size_t>(cdiv(n, kBlock), kPersistMaxG)));
+}
+static bool persist_fits(int rounds, int G) {
+    return static_cast<size_t>(rounds) * G * kPersistK <=
+           static_cast<size_t>(kMuPartialDoubles - kPersistSlotBase);
+}
+static void fill_slots(double* ws, int rounds, int G, hipStream_t st) {
+    const int n = rounds * G * kPersistK;
+    hipLaunchKernelGGL(k_fill_slots, dim3(std::max(1, std::min(64, (n + 255) / 256))), dim3(256), 0,
+                       st, reinterpret_cast<unsigned long long*>(ws + kPersistSlotBase), n);
+    HFGPU_LAUNCH_CHECK();
+}
+
 void launch_mu_bisection(const double* eps, const double* w, int nk, int nb, double n_e,
                          double T, int maxiter, MuState* state, double* ws, hipStream_t st) {
     if (maxiter <= 0) maxiter = 30;
     const size_t n = static_cast<size_t>(nk) * nb;
+    if (persist_bisect_on()) {
+        constexpr int kLv = 3;  // 7 midpoints per round (kPersistK slots)
+        const int G = persist_grid(n);
+        const int rounds = 1 + (maxiter + kLv - 1) / kLv;
+        if (persist_fits(rounds, G)) {
+            fill_slots(ws, rounds, G, st);
+            hipLaunchKernelGGL(k_bisect_persist<7>, dim3(G), dim3(kBlock), 0, st, eps, w, nb, n,
+                               n_e, T, maxiter, kLv, ws + kPersistSlotBase, state);
+            HFGPU_LAUNCH_CHECK();
+            return;
+        }
+    }
     const unsigned blocks = reduce_blocks(n, 1);
     hipLaunchKernelGGL(k_minmax, dim3(blocks), dim3(kBlock), 0, st, eps, n, ws + kPart1);
     HFGPU_LAUNCH_CHECK();
@@ -477,6 +842,17 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
     // step's partials in every workgroup
     const unsigned blocks = reduce_blocks(n, 4);
     const unsigned mm_blocks = reduce_blocks(n, 1);
+    if (maxiter > 0 && n > kNewtonSoloMax && persist_on()) {
+        const int G = persist_grid(n);
+        if (persist_fits(1 + maxiter, G)) {
+            fill_slots(ws, 1 + maxiter, G, st);
+            hipLaunchKernelGGL(k_newton_persist, dim3(G), dim3(kBlock), 0, st, eps, w_norm, nb, n,
+                               n_target_norm, T, mu0, mu0_from_state ? 1 : 0, state, maxiter,
+                               ws + kPersistSlotBase, state);
+            HFGPU_LAUNCH_CHECK();
+            return;
+        }
+    }
     hipLaunchKernelGGL(k_minmax, dim3(mm_blocks), dim3(kBlock), 0, st, eps, n, ws + kPart1);
     HFGPU_LAUNCH_CHECK();
     if (maxiter > 0 && n <= kNewtonSoloMax) {

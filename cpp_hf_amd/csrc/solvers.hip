@@ -177,7 +177,7 @@ struct ScfState {
     GpuKernel& K;
     hipStream_t st;
     SolveWs ws;
-    DevBuf eps, shifted, occ, V;
+    DevBuf eps, shifted, V;
     Projector& proj;
     double n_e, level_shift;
     // Fock builds with the FP32 exchange workspace (mixed precision) until
@@ -195,7 +195,6 @@ struct ScfState {
         const size_t nkb = K.nk * K.nb;
         eps.alloc(nkb * sizeof(double));
         shifted.alloc(nkb * sizeof(double));
-        occ.alloc(nkb * sizeof(double));
         V.alloc(K.nd * sizeof(cd));
         size_t total = 0;
         for (size_t i = 0; i < nsizes; ++i) total += sizes[i];
@@ -233,14 +232,23 @@ struct ScfState {
         }
         // (with level_shift == 0 the reference's second bisection repeats the
         // first on identical input, so its result is the same mu)
-        launch_occupations(e_used, nkb, ws.mu(), K.T, 0, occ.as<double>(), st);
-        launch_density(nk, nb, V.as<cd>(), occ.as<double>(), P_out, true, st);
+        // occupations formed inside the density kernel (solver_scf.hpp:138-152)
+        launch_density_occ(nk, nb, V.as<cd>(), e_used, ws.mu(), K.T, 0, P_out, true, st);
     }
 
     // build_and_occupy (solver_scf.hpp:179-197); energy -> sc[E].
-    void build_and_occupy(cd* d, cd* Sigma, cd* F, double* hdiag, cd* P_out, double* E_slot) {
+    //
+    // Hermitisation is idempotent bit for bit (herm(herm(X)) = herm(X)), and
+    // real linear combinations of exactly Hermitian matrices are exactly
+    // Hermitian; without a projector the launches below skip the
+    // hermitisations whose input is already exactly Hermitian: the density
+    // after the first iteration (every update ends in a hermitisation or a
+    // real combination of Hermitian matrices) and the density the eigenvectors
+    // give (the density kernel hermitises).  d_herm: d is exactly Hermitian.
+    void build_and_occupy(cd* d, cd* Sigma, cd* F, double* hdiag, cd* P_out, double* E_slot,
+                          bool d_herm = false) {
         if (proj) proj.apply(d, st);
-        launch_hermitize(static_cast<int>(K.nk), K.nb, d, st);
+        if (proj || !d_herm) launch_hermitize(static_cast<int>(K.nk), K.nb, d, st);
         K.build_fock(d, Sigma, F, hdiag, true, st, true, mixed);
         if (proj) {
             proj.apply(F, st);
@@ -248,8 +256,10 @@ struct ScfState {
         }
         K.energy(d, Sigma, hdiag, E_slot, st);
         density_from_fock(F, P_out);
-        if (proj) proj.apply(P_out, st);
-        launch_hermitize(static_cast<int>(K.nk), K.nb, P_out, st);
+        if (proj) {
+            proj.apply(P_out, st);
+            launch_hermitize(static_cast<int>(K.nk), K.nb, P_out, st);
+        }
     }
 };
 
@@ -296,7 +306,7 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
     MuState hmu;
     while (k < cfg.max_iter && !converged) {
         S.build_and_occupy(density.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(),
-                           P_new.as<cd>(), sc + SC_E);
+                           P_new.as<cd>(), sc + SC_E, k > 0);
         launch_scf_residuals(nk, nb, F.as<cd>(), density.as<cd>(), P_new.as<cd>(),
                              delta.as<cd>(), use_diis ? comm.as<cd>() : nullptr,
                              K.w2d.as<double>(), S.ws.ls, sc + SC_RES, st);
@@ -316,10 +326,14 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
         const bool clip = cfg.trust_radius > 0.0 && !converged;
         if (clip) d2d(pre.get(), density.get(), bytes, st);
 
+        // density + alpha delta of two exactly Hermitian matrices is exactly
+        // Hermitian (negation and the real scaling commute with rounding)
         auto linear_mix = [&](double alpha) {
             launch_axpy_c(nd, alpha, delta.as<cd>(), density.as<cd>(), st);
-            if (proj) proj.apply(density.as<cd>(), st);
-            launch_hermitize(nk, nb, density.as<cd>(), st);
+            if (proj) {
+                proj.apply(density.as<cd>(), st);
+                launch_hermitize(nk, nb, density.as<cd>(), st);
+            }
         };
 
         if (converged) {
@@ -424,7 +438,7 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
     }
     S.mixed = false;
     S.build_and_occupy(density.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(),
-                       P_new.as<cd>(), sc + SC_E);
+                       P_new.as<cd>(), sc + SC_E, k > 0);
     d2h(hsc, sc, sizeof(hsc), st);
     d2h(&hmu, S.ws.mu(), sizeof(MuState), st);
     if (out->density) d2h(out->density, density.get(), bytes, st);
@@ -481,9 +495,13 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
         launch_occupations(e, nkb, W.mu(), T_r, 1, p.as<double>(), st);
     };
     auto density_step = [&]() {
-        launch_density(nk, nb, Q.as<cd>(), p.as<double>(), P.as<cd>(), false, st);
-        if (proj) proj.apply(P.as<cd>(), st);
-        launch_hermitize(nk, nb, P.as<cd>(), st);
+        // the hermitising density kernel equals X followed by a hermitisation
+        // bit for bit; a projector acts on X itself (solver_dm.hpp:117-128)
+        launch_density(nk, nb, Q.as<cd>(), p.as<double>(), P.as<cd>(), !proj, st);
+        if (proj) {
+            proj.apply(P.as<cd>(), st);
+            launch_hermitize(nk, nb, P.as<cd>(), st);
+        }
     };
 
     // ---- block structure (solver_dm.hpp:100-113, 403-405): more than one
@@ -695,9 +713,13 @@ void solve_rtr_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_confi
     // density, Fock (+ deflation bias), Ft, eps and G at a point
     // (solver_rtr.hpp:275-293); energy, |G|_w^2 and sum eps^2 land in sc
     auto eval_state = [&](RtrPoint& s) {
-        launch_density(nk, nb, s.Q.as<cd>(), s.p.as<double>(), P.as<cd>(), false, st);
-        if (proj) proj.apply(P.as<cd>(), st);
-        launch_hermitize(nk, nb, P.as<cd>(), st);
+        // the hermitising density kernel equals X followed by a hermitisation
+        // bit for bit; a projector acts on X itself (solver_dm.hpp:117-128)
+        launch_density(nk, nb, s.Q.as<cd>(), s.p.as<double>(), P.as<cd>(), !proj, st);
+        if (proj) {
+            proj.apply(P.as<cd>(), st);
+            launch_hermitize(nk, nb, P.as<cd>(), st);
+        }
         K.build_fock(P.as<cd>(), Sigma.as<cd>(), s.Ft.as<cd>(), hdiag.as<double>(), true, st);
         K.energy(P.as<cd>(), Sigma.as<cd>(), hdiag.as<double>(), sc + SC_E, st);
         if (defl_on)
@@ -855,9 +877,13 @@ void solve_rtr_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_confi
     }
 
     // ---- finalise (solver_rtr.hpp:435-441)
-    launch_density(nk, nb, cur->Q.as<cd>(), cur->p.as<double>(), P.as<cd>(), false, st);
-    if (proj) proj.apply(P.as<cd>(), st);
-    launch_hermitize(nk, nb, P.as<cd>(), st);
+    // the hermitising density kernel equals X followed by a hermitisation
+    // bit for bit; a projector acts on X itself (solver_dm.hpp:117-128)
+    launch_density(nk, nb, cur->Q.as<cd>(), cur->p.as<double>(), P.as<cd>(), !proj, st);
+    if (proj) {
+        proj.apply(P.as<cd>(), st);
+        launch_hermitize(nk, nb, P.as<cd>(), st);
+    }
     K.build_fock(P.as<cd>(), Sigma.as<cd>(), Fb.as<cd>(), hdiag.as<double>(), true, st);
     K.energy(P.as<cd>(), Sigma.as<cd>(), hdiag.as<double>(), sc + SC_E, st);
 
