@@ -45,6 +45,18 @@ FftAxis AxisPlan::axis() const {
     return a;
 }
 
+// Synchronous upload on a private non-blocking stream, never the null
+// stream: a null-stream copy orders against the process's other streams,
+// which breaks hipGraph captures running in other host threads.
+static void upload_now(void* dst, const void* src, size_t bytes) {
+    hipStream_t s = nullptr;
+    HFGPU_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    HFGPU_CHECK(e);
+}
+
 void AxisPlan::init(int n_) {
     release();
     n = n_;
@@ -67,7 +79,7 @@ void AxisPlan::init(int n_) {
         tw[k] = cd{static_cast<double>(std::cos(a)), static_cast<double>(std::sin(a))};
     }
     d_tw = static_cast<cd*>(dev_alloc(sizeof(cd) * tw.size()));
-    HFGPU_CHECK(hipMemcpy(d_tw, tw.data(), sizeof(cd) * tw.size(), hipMemcpyHostToDevice));
+    upload_now(d_tw, tw.data(), sizeof(cd) * tw.size());
 }
 
 void AxisPlan::release() {
@@ -174,8 +186,8 @@ void ExchangeGeom::init(int nk1_, int nk2_, int nb_, bool packed_) {
         throw std::invalid_argument("k-mesh too large for an LDS-resident FFT line");
     d_chan_a = static_cast<int*>(dev_alloc(sizeof(int) * nch));
     d_chan_b = static_cast<int*>(dev_alloc(sizeof(int) * nch));
-    HFGPU_CHECK(hipMemcpy(d_chan_a, ca.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
-    HFGPU_CHECK(hipMemcpy(d_chan_b, cb.data(), sizeof(int) * nch, hipMemcpyHostToDevice));
+    upload_now(d_chan_a, ca.data(), sizeof(int) * nch);
+    upload_now(d_chan_b, cb.data(), sizeof(int) * nch);
 }
 
 void ExchangeGeom::release() {

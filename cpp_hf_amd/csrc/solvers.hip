@@ -8,6 +8,7 @@
 // consumed on the device.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <stdexcept>
@@ -49,6 +50,56 @@ struct Projector {
                static_cast<size_t>(nb)) != 0)
             throw std::runtime_error("project_fn failed");
         h2d(dM, host.data(), nd * sizeof(cd), st);
+    }
+};
+
+// Per-iteration launch sequences of the solver loops replayed from captured
+// hipGraphs (HFGPU_SOLVER_GRAPH=0 launches them eagerly): one host call per
+// iteration instead of one per kernel, which matters most when many solves
+// run concurrently from host threads and their launches contend in the
+// runtime.  The captured work allocates nothing and clears memory only with
+// fill kernels (no memset nodes).
+bool solver_graphs_on() {
+    const char* e = std::getenv("HFGPU_SOLVER_GRAPH");  // read per solve
+    return !(e && e[0] == '0');
+}
+
+struct IterGraphs {
+    hipStream_t st;
+    bool on;
+    std::vector<std::pair<int, hipGraphExec_t>> execs;
+    IterGraphs(hipStream_t s, bool enable) : st(s), on(enable) {}
+    ~IterGraphs() {
+        for (auto& e : execs) (void)hipGraphExecDestroy(e.second);
+    }
+    IterGraphs(const IterGraphs&) = delete;
+    IterGraphs& operator=(const IterGraphs&) = delete;
+
+    template <class Fn>
+    void run(int key, Fn&& body) {
+        for (auto& e : execs)
+            if (e.first == key) {
+                HFGPU_CHECK(hipGraphLaunch(e.second, st));
+                return;
+            }
+        hipGraph_t graph = nullptr;
+        // relaxed: other solves' threads may allocate or copy synchronously
+        // (new geometries, pool misses) while this one captures
+        HFGPU_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        HFGPU_CHECK(hipStreamEndCapture(st, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HFGPU_CHECK(ie);
+        execs.emplace_back(key, exec);
+        HFGPU_CHECK(hipGraphLaunch(exec, st));
     }
 };
 
@@ -304,12 +355,22 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
     const double ws = std::max(K.weight_sum, 1.0e-30);
     double hsc[SC_COUNT];
     MuState hmu;
+    IterGraphs graphs(st, !proj && solver_graphs_on());
     while (k < cfg.max_iter && !converged) {
-        S.build_and_occupy(density.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(),
-                           P_new.as<cd>(), sc + SC_E, k > 0);
-        launch_scf_residuals(nk, nb, F.as<cd>(), density.as<cd>(), P_new.as<cd>(),
-                             delta.as<cd>(), use_diis ? comm.as<cd>() : nullptr,
-                             K.w2d.as<double>(), S.ws.ls, sc + SC_RES, st);
+        auto body = [&]() {
+            S.build_and_occupy(density.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(),
+                               P_new.as<cd>(), sc + SC_E, k > 0);
+            launch_scf_residuals(nk, nb, F.as<cd>(), density.as<cd>(), P_new.as<cd>(),
+                                 delta.as<cd>(), use_diis ? comm.as<cd>() : nullptr,
+                                 K.w2d.as<double>(), S.ws.ls, sc + SC_RES, st);
+        };
+        // the build, occupation and residual launches of every iteration after
+        // the first are one captured graph per precision (the host decisions
+        // that follow read the residuals back)
+        if (graphs.on && k > 0)
+            graphs.run(S.mixed ? 1 : 0, body);
+        else
+            body();
         d2h(hsc, sc, sizeof(hsc), st);
         d2h(&hmu, S.ws.mu(), sizeof(MuState), st);
         HFGPU_CHECK(hipStreamSynchronize(st));
@@ -544,6 +605,7 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
     eigh(0, F.as<cd>(), eps.as<double>(), Q.as<cd>());
     mu_solve(eps.as<double>(), false, 0.0);
 
+    IterGraphs graphs(st, !proj && solver_graphs_on());
     std::vector<double> hist_E, hist_G;
     size_t ls_trials = 0;
     double grad_norm = std::numeric_limits<double>::infinity();
@@ -563,26 +625,36 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
             mixed = false;  // stop only on FP64 builds
         }
 
-        density_step();
-        K.build_fock(P.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(), true, st, true,
-                     mixed);
-        K.energy(P.as<cd>(), Sigma.as<cd>(), hdiag.as<double>(), sc + SC_E, st);
-        launch_orbital_grad(nk, nb, Q.as<cd>(), F.as<cd>(), p.as<double>(), wd, W.mu(), T_r, true,
-                            eps.as<double>(), G.as<cd>(), g_p.as<double>(), W.ls, st);
-        launch_precond(nk, nb, G.as<cd>(), Gp.as<cd>(), eps.as<double>(), p.as<double>(),
-                       g_p.as<double>(), g_pp.as<double>(), wd, T_r, cfg.denom_scale, dQ.as<cd>(),
-                       h_p.as<double>(), W.ls, st);
         const bool restart = (k == 0) || (cfg.cg_restart > 0 && k % cfg.cg_restart == 0);
-        launch_cg_combine(nk, nb, restart, dQ.as<cd>(), dQp.as<cd>(), h_p.as<double>(),
-                          d_pp.as<double>(), d_p.as<double>(), eps.as<double>(), p.as<double>(),
-                          wd, T_r, W.ls, st);
-        eigh(1, dQ.as<cd>(), lam.as<double>(), Vd.as<cd>());
-        launch_linesearch(nk, nb, Vd.as<cd>(), lam.as<double>(), F.as<cd>(), Sigma.as<cd>(),
-                          p.as<double>(), d_p.as<double>(), wd, T_r, cfg.max_step, cfg.bt_shrink,
-                          static_cast<int>(cfg.bt_max), W.ls, st);
-        launch_retract(nk, nb, Q.as<cd>(), Vd.as<cd>(), lam.as<double>(), Sigma.as<cd>(),
-                       eps_w.as<double>(), true, W.ls, st);
-        mu_solve(eps_w.as<double>(), true, 0.0);
+        auto body = [&]() {
+            density_step();
+            K.build_fock(P.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(), true, st,
+                         true, mixed);
+            K.energy(P.as<cd>(), Sigma.as<cd>(), hdiag.as<double>(), sc + SC_E, st);
+            launch_orbital_grad(nk, nb, Q.as<cd>(), F.as<cd>(), p.as<double>(), wd, W.mu(), T_r,
+                                true, eps.as<double>(), G.as<cd>(), g_p.as<double>(), W.ls, st);
+            launch_precond(nk, nb, G.as<cd>(), Gp.as<cd>(), eps.as<double>(), p.as<double>(),
+                           g_p.as<double>(), g_pp.as<double>(), wd, T_r, cfg.denom_scale,
+                           dQ.as<cd>(), h_p.as<double>(), W.ls, st);
+            launch_cg_combine(nk, nb, restart, dQ.as<cd>(), dQp.as<cd>(), h_p.as<double>(),
+                              d_pp.as<double>(), d_p.as<double>(), eps.as<double>(),
+                              p.as<double>(), wd, T_r, W.ls, st);
+            eigh(1, dQ.as<cd>(), lam.as<double>(), Vd.as<cd>());
+            launch_linesearch(nk, nb, Vd.as<cd>(), lam.as<double>(), F.as<cd>(), Sigma.as<cd>(),
+                              p.as<double>(), d_p.as<double>(), wd, T_r, cfg.max_step,
+                              cfg.bt_shrink, static_cast<int>(cfg.bt_max), W.ls, st);
+            launch_retract(nk, nb, Q.as<cd>(), Vd.as<cd>(), lam.as<double>(), Sigma.as<cd>(),
+                           eps_w.as<double>(), true, W.ls, st);
+            mu_solve(eps_w.as<double>(), true, 0.0);
+        };
+        // After the first iteration (which makes every lazy allocation) the
+        // body is the same launch sequence each time up to the CG restart
+        // flag, the FP32/FP64 build and the parity of the swapped buffer
+        // pairs below: one captured graph per variant, replayed.
+        if (graphs.on && k > 0)
+            graphs.run(((k & 1) ? 1 : 0) | (restart ? 2 : 0) | (mixed ? 4 : 0), body);
+        else
+            body();
 
         d2h(hsc, sc, sizeof(hsc), st);
         HFGPU_CHECK(hipStreamSynchronize(st));

@@ -1,0 +1,71 @@
+"""Solver iterations replayed from captured hipGraphs against the same
+iterations launched one kernel at a time.
+
+After its first iteration each solver captures its per-iteration launch
+sequence once per variant (buffer parity, CG restart, precision) and replays
+it; the replays run the same kernels on the same buffers in the same order,
+so every iterate, and with it the final state, must equal the eager run's bit
+for bit.  Each solve replays its graphs many times, with host read-backs and
+eager launches in between.
+"""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from helpers import kernel_args_dict
+
+pytestmark = pytest.mark.gpu
+
+
+def _dm_args(k, P0, n_e, max_iter):
+    # tol_E = -1: the plateau test never stops the run before max_iter
+    return (kernel_args_dict(k), P0, n_e, max_iter, -1.0, 0.0, 0.6, 0.5, 1e-3, 8, 10, 5, 25, 0,
+            0.5, 20, np.empty((0,) + k.h.shape), 0.0, 1.0, [], None, True, True, True)
+
+
+def _with_graphs(on, fn):
+    old = os.environ.get("HFGPU_SOLVER_GRAPH")
+    os.environ["HFGPU_SOLVER_GRAPH"] = "1" if on else "0"
+    try:
+        return fn()
+    finally:
+        if old is None:
+            del os.environ["HFGPU_SOLVER_GRAPH"]
+        else:
+            os.environ["HFGPU_SOLVER_GRAPH"] = old
+
+
+@pytest.mark.parametrize("n,nl", [(96, 4), (24, 6), (36, 1)])
+def test_dm_graph_replay_equals_eager(gpu, n, nl):
+    from cpp_hf_amd.workloads import layered_kernel
+
+    w = layered_kernel(n, nl, T=0.05, filling=nl + 0.1)
+    args = _dm_args(w.kernel, w.P0, w.n_electrons, 40)
+    print(f"dm graph replay {n}x{n} nb {2 * nl}", flush=True)
+    g = _with_graphs(True, lambda: gpu.solve_dm(*args))
+    e = _with_graphs(False, lambda: gpu.solve_dm(*args))
+    assert g[6] == e[6] == 40  # both buffer parities and CG restarts replayed
+    for a, b in zip(g[:6], e[:6]):
+        np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+    np.testing.assert_array_equal(g[8], e[8])
+    np.testing.assert_array_equal(g[9], e[9])
+
+
+@pytest.mark.parametrize("accel", ["linear", "diis"])
+def test_scf_graph_replay_equals_eager(gpu, accel):
+    from cpp_hf_amd.workloads import layered_kernel
+
+    w = layered_kernel(48, 4, T=0.05, filling=4.1)
+    args = (kernel_args_dict(w.kernel), w.P0, w.n_electrons, 30, 1e-9, 1e-8, 0.5, 0.0, None, [],
+            accel, 6, 2, 1.0, 0.0, True, True)
+    print(f"scf graph replay {accel}", flush=True)
+    g = _with_graphs(True, lambda: gpu.solve_scf(*args))
+    e = _with_graphs(False, lambda: gpu.solve_scf(*args))
+    assert g[4] == e[4] and g[4] > 5
+    for a, b in zip(g, e):
+        if a is not None:
+            np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
