@@ -488,7 +488,25 @@ def main() -> int:
                     help="time the unpacked nb^2-channel exchange instead")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 64 MiB x 10 (scales the PMC byte counters)")
+    ap.add_argument("--kernel-sample", action="store_true",
+                    help="only run the solver kernels a few times each for a profiler: eigh "
+                         "and density_from_fock at 96 x 96 x 8 and 192 x 192 x 16")
     args = ap.parse_args()
+
+    if args.kernel_sample:
+        from cpp_hf_amd._compat import _native
+
+        rng = np.random.default_rng(0)
+        for n, nb, n_e in ((96, 8, 4.1), (192, 16, 8.1)):
+            X = rng.standard_normal((n, n, nb, nb)) + 1j * rng.standard_normal((n, n, nb, nb))
+            F = 0.5 * (X + np.conj(np.swapaxes(X, -1, -2)))
+            w = np.full((n, n), 1.0 / (n * n))
+            for _ in range(5):
+                _native.eigh_batched(F)
+                _native.density_from_fock(F, w, n_e, 0.05, 0)
+        print(json.dumps({"kernel_sample": "eigh and density_from_fock at 96x96x8 and "
+                                           "192x192x16, 5 calls each"}), flush=True)
+        return 0
 
     if args.copy_calibration:
         import torch

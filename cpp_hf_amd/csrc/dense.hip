@@ -8,6 +8,8 @@
 // staged in LDS (transposed, padded) with coalesced 16 B loads, each lane
 // then forms its own output elements.  At nb = 8 the density build moves 2A + a bytes
 // (A = 16 nk nb^2) for 8 nb^3 nk flops: HBM-bound.
+#include <cstdlib>
+
 #include "kernels.h"
 
 namespace hfgpu {
@@ -79,6 +81,79 @@ k_density(int nk, int nb_arg, const cd* __restrict__ V, const double* __restrict
     }
 }
 
+// 9 <= nb <= 16: the same product on the FP64 matrix cores.  One wavefront
+// per k point; X = V diag(f) V^H is four real 16 x 16 x 16 products (padded
+// with zeros past nb),
+//   Re X = (f Vr) Vr^T + (f Vi) Vi^T,   Im X = (f Vi) Vr^T - (f Vr) Vi^T,
+// each four v_mfma_f64_16x16x4f64 steps over n.  Operand lanes (gfx950): A
+// lane l holds A[l % 16][l / 16], B lane l holds B[l / 16][l % 16], and
+// result value r of lane l is D[4 r + l / 16][l % 16].  With B = V^T both
+// operands of a step are lane l's V[l % 16][4 c + l / 16], so every lane
+// loads four elements of V and nothing is staged.  The hermitisation reads
+// the transposed element through LDS, as the VALU kernel does.
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+template <int NB>
+__global__ void __launch_bounds__(256)
+k_density_mfma(int nk, const cd* __restrict__ V, const double* __restrict__ f,
+               const double* __restrict__ eps, const MuState* __restrict__ mus, double T,
+               int occ_mode, cd* __restrict__ P, int hermitize) {
+    static_assert(NB > 8 && NB <= 16, "matrix-core density: 9..16 bands");
+    __shared__ double sx[4][2][16][17];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + wave;
+    if (k >= nk) return;  // no workgroup barrier below: the LDS block is the wavefront's
+    const int i = lane & 15, q = lane >> 4;
+    const size_t kb = static_cast<size_t>(k) * NB * NB;
+    cd v[4];
+    double fn[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int n = 4 * c + q;
+        v[c] = (i < NB && n < NB) ? V[kb + i * NB + n] : cd{0.0, 0.0};
+        if (n >= NB) {
+            fn[c] = 0.0;
+        } else if (f) {
+            fn[c] = f[static_cast<size_t>(k) * NB + n];
+        } else {
+            const double mu = mus->mu, x = eps[static_cast<size_t>(k) * NB + n];
+            fn[c] = occ_mode == 0 ? expit_fd(-(x - mu) / (T + 1.0e-12)) : expit_fd((mu - x) / T);
+        }
+    }
+    d4_t cre = {0.0, 0.0, 0.0, 0.0}, cim = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double ar = fn[c] * v[c].x, ai = fn[c] * v[c].y;
+        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, v[c].x, cre, 0, 0, 0);
+        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, v[c].y, cre, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, v[c].x, cim, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar, v[c].y, cim, 0, 0, 0);
+    }
+    // lane holds X[a][i] for a = 4 r + q
+    if (!hermitize) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int a = 4 * r + q;
+            if (a < NB && i < NB) P[kb + a * NB + i] = cd{cre[r], cim[r]};
+        }
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        sx[wave][0][4 * r + q][i] = cre[r];
+        sx[wave][1][4 * r + q][i] = cim[r];
+    }
+    __builtin_amdgcn_wave_barrier();  // one wavefront's LDS operations complete in order
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int a = 4 * r + q;
+        if (a < NB && i < NB) {
+            const double tr = sx[wave][0][i][a], ti = sx[wave][1][i][a];
+            P[kb + a * NB + i] = cd{0.5 * (cre[r] + tr), 0.5 * (cim[r] - ti)};
+        }
+    }
+}
+
 __global__ void k_hermitize(int nk, int nb, cd* __restrict__ M) {
     const int npair = nb * (nb + 1) / 2;
     const size_t total = static_cast<size_t>(nk) * npair;
@@ -100,12 +175,35 @@ __global__ void k_hermitize(int nk, int nb, cd* __restrict__ M) {
 
 }  // namespace
 
+// The matrix-core product for 9..16 bands unless HFGPU_DENSITY_MFMA=0.
+static bool density_mfma_on() {
+    static const bool on = [] {
+        const char* e = std::getenv("HFGPU_DENSITY_MFMA");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 static void density_launch(int nk, int nb, const cd* V, const double* f, const double* eps,
                            const MuState* mus, double T, int mode, cd* P, bool hermitize,
                            hipStream_t st) {
     const size_t lds = 4 * (sizeof(cd) * (nb * (nb + 1) + nb + nb * nb));
     const dim3 grid(cdiv(nk, 4)), block(256);
     const int h = hermitize ? 1 : 0;
+    if (nb > 8 && nb <= 16 && density_mfma_on()) {
+        switch (nb) {
+            case 9: hipLaunchKernelGGL(k_density_mfma<9>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
+            case 10: hipLaunchKernelGGL(k_density_mfma<10>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
+            case 11: hipLaunchKernelGGL(k_density_mfma<11>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
+            case 12: hipLaunchKernelGGL(k_density_mfma<12>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
+            case 13: hipLaunchKernelGGL(k_density_mfma<13>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
+            case 14: hipLaunchKernelGGL(k_density_mfma<14>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
+            case 15: hipLaunchKernelGGL(k_density_mfma<15>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
+            default: hipLaunchKernelGGL(k_density_mfma<16>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
+        }
+        HFGPU_LAUNCH_CHECK();
+        return;
+    }
     switch (nb) {
         case 2: hipLaunchKernelGGL(k_density<2>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
         case 4: hipLaunchKernelGGL(k_density<4>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;

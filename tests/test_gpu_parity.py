@@ -273,14 +273,18 @@ def test_build_fock_exchange_off_and_project(gpu, oracle):
     assert not np.any(F)
 
 
-def test_density_from_fock_matches_oracle(gpu, oracle):
-    rng = np.random.default_rng(21)
-    F = rand_herm(rng, (12, 12, 4, 4))
+@pytest.mark.parametrize("nb", [4, 9, 12, 16])
+def test_density_from_fock_matches_oracle(gpu, oracle, nb):
+    """nb 9..16 run the density product on the FP64 matrix cores."""
+    rng = np.random.default_rng(21 + nb)
+    F = rand_herm(rng, (12, 12, nb, nb))
     w = np.full((12, 12), 1 / 144)
-    P, mu = gpu.density_from_fock(F, w, 2.3, 0.05, 0)
-    Pr, mur = oracle.density_from_fock(F, w, 2.3, 0.05)
+    n_e = 0.55 * nb
+    P, mu = gpu.density_from_fock(F, w, n_e, 0.05, 0)
+    Pr, mur = oracle.density_from_fock(F, w, n_e, 0.05)
     assert abs(mu - mur) < 1e-14
     assert relerr(P, Pr) < 1e-12
+    np.testing.assert_array_equal(P, np.conj(np.swapaxes(P, -1, -2)))
 
 
 # --------------------------------------------------------------- resample ---
