@@ -148,6 +148,42 @@ def run_mixed_precision(torch, dev) -> dict:
         "gap_fp64": _gap_about(F64, mu64),
         "gap_error": abs(_gap_about(F32, mu32) - _gap_about(F64, mu64)),
     }
+
+    def max_k_dF(A, B):
+        return float(np.max(np.sqrt(np.sum(np.abs(np.asarray(A) - np.asarray(B)) ** 2,
+                                           axis=(-2, -1)))))
+
+    # the SCF's own reproducibility: two FP64 runs from P0 and from P0 moved
+    # by 1e-3 (mu from 30 bisection halvings limits how close they end)
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal(wl.P0.shape) + 1j * rng.standard_normal(wl.P0.shape)
+    P1 = wl.P0 + 5e-4 * (X + np.conj(np.swapaxes(X, -1, -2)))
+    tight = dict(base, density_tol=1e-9, comm_tol=1e-9, max_iter=300)
+    ra = solve_scf(wl.kernel, wl.P0, wl.n_electrons, config=SCFConfig(**tight))
+    rb = solve_scf(wl.kernel, P1, wl.n_electrons, config=SCFConfig(**tight))
+    out["scf_192x192_nb16"]["fp64_two_starts_max_k_dF"] = max_k_dF(ra.fock_matrix, rb.fock_matrix)
+
+    # DM with the in-loop Newton mu (resolved to rounding): FP32-workspace
+    # builds until |G| <= 1e-4, FP64 afterwards, against FP64 throughout
+    from cpp_hf_amd import SolverConfig, solve
+
+    dm = {}
+    for prec in ("fp64", "mixed"):
+        t = time.perf_counter()
+        r = solve(wl.kernel, wl.P0, wl.n_electrons,
+                  config=SolverConfig(max_iter=300, tol_E=0.0, tol_grad=0.0, plateau_window=0,
+                                      exchange_precision=prec, mixed_switch=1e-4))
+        dm[prec] = (r, time.perf_counter() - t)
+    (d64, td64), (d32, td32) = dm["fp64"], dm["mixed"]
+    out["dm_192x192_nb16"] = {
+        "config": "same model, direct minimisation (CG), 300 iterations, Newton mu; mixed: "
+                  "FP32 workspace until |G| <= 1e-4",
+        "fp64_s": td64, "mixed_s": td32, "speedup": td64 / td32,
+        "grad_norm": [float(d64.history["grad_norm"][-1]), float(d32.history["grad_norm"][-1])],
+        "max_k_dF": max_k_dF(d64.fock, d32.fock),
+        "dmu": abs(float(d64.mu) - float(d32.mu)),
+        "dE": abs(float(d64.energy) - float(d32.energy)),
+    }
     return out
 
 

@@ -108,3 +108,25 @@ def test_mixed_dm_matches_fp64(gpu):
     assert dE < 1e-6
     with pytest.raises(ValueError):
         solve(wl.kernel, wl.P0, wl.n_electrons, config=SolverConfig(exchange_precision="fp16"))
+
+
+def test_mixed_dm_192x192_nb16_fock_within_1e8_of_fp64(gpu):
+    """16 bands on 192 x 192: the mixed-precision DM solve (FP32 workspace
+    until |G| <= 1e-4, then FP64) ends within 1e-8 per k point of the FP64
+    solve's Fock matrix (the DM's Newton mu is resolved to rounding; the
+    SCF's 30-step bisection mu is not, which leaves two FP64 SCF runs from
+    different starts ~3e-7 apart)."""
+    from cpp_hf_amd import SolverConfig, solve
+    from cpp_hf_amd.workloads import layered_kernel
+
+    wl = layered_kernel(192, 8, T=0.05, filling=8.1)
+    base = dict(max_iter=300, tol_E=0.0, tol_grad=0.0, plateau_window=0, mixed_switch=1e-4)
+    print("mixed dm 192x192 nb16", flush=True)
+    r64 = solve(wl.kernel, wl.P0, wl.n_electrons, config=SolverConfig(**base))
+    r32 = solve(wl.kernel, wl.P0, wl.n_electrons,
+                config=SolverConfig(exchange_precision="mixed", **base))
+    assert r32.history["E"][1] != r64.history["E"][1]  # the FP32 phase ran
+    dF = np.sqrt(np.sum(np.abs(np.asarray(r32.fock) - np.asarray(r64.fock)) ** 2, axis=(-2, -1)))
+    print(f"  max_k |dF| {np.max(dF):.2e}", flush=True)
+    assert float(np.max(dF)) <= 1e-8
+    assert abs(float(r32.mu) - float(r64.mu)) <= 1e-9
