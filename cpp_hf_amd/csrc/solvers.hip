@@ -103,6 +103,40 @@ struct IterGraphs {
     }
 };
 
+// The SCF loop with linear mixing runs one iteration ahead of the host's
+// convergence test (HFGPU_SCF_PIPELINE=0: the test before every launch).
+bool scf_pipeline_on() {
+    const char* e = std::getenv("HFGPU_SCF_PIPELINE");  // read per solve
+    return !(e && e[0] == '0');
+}
+
+// Pinned host memory for scalars read back while later work is in flight: an
+// asynchronous copy into it completes without the host waiting.  One page
+// per host thread, kept for the thread's lifetime (freeing pinned memory
+// synchronises the device, which would stall concurrent solves).
+double* pinned_scalars() {
+    thread_local double* p = nullptr;
+    if (!p) {
+        void* q = nullptr;
+        HFGPU_CHECK(hipHostMalloc(&q, 4096, hipHostMallocDefault));
+        p = static_cast<double*>(q);
+    }
+    return p;
+}
+
+struct StreamEvent {
+    hipEvent_t e = nullptr;
+    StreamEvent() = default;
+    void create() {
+        if (!e) HFGPU_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    ~StreamEvent() {
+        if (e) (void)hipEventDestroy(e);
+    }
+    StreamEvent(const StreamEvent&) = delete;
+    StreamEvent& operator=(const StreamEvent&) = delete;
+};
+
 // A device symmetry projector from the solver configuration, if any.
 void attach_symmetry(Projector& pr, SymPlan& sp, const hf_symmetry_desc* sym, GpuKernel& K) {
     if (!sym) return;
@@ -356,7 +390,77 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
     double hsc[SC_COUNT];
     MuState hmu;
     IterGraphs graphs(st, !proj && solver_graphs_on());
-    while (k < cfg.max_iter && !converged) {
+
+    // Linear mixing with no trust radius, projector or FP32 phase: iteration
+    // j + 1 depends on the host only through the convergence test of
+    // iteration j (converged: the density becomes P_new; otherwise it takes
+    // the mixing step), so iteration j + 1 is enqueued, mixing step first,
+    // before the residuals of iteration j are read back.  Its density,
+    // energy and residuals go to the other of two slots; when iteration j
+    // converges the speculative one is discarded, so every result equals the
+    // synchronous loop's bit for bit.  The device then runs the iterations
+    // back to back instead of idling through each host round trip.
+    const bool pipelined = !use_diis && !use_oda && cfg.trust_radius <= 0.0 && !proj &&
+                           !S.mixed && cfg.max_iter > 0 && scf_pipeline_on();
+    // the second slot and the events live until the solve's final stream
+    // synchronisation: a discarded speculative iteration may still be running
+    // when the loop ends, and a buffer back in the pool could reach another
+    // thread's solve before this stream drains
+    DevBuf P_alt, sc_alt;
+    StreamEvent ev[2];
+    if (pipelined) {
+        P_alt.alloc(bytes);
+        sc_alt.alloc(SC_COUNT * sizeof(double));
+        ev[0].create();
+        ev[1].create();
+        fill_zero(sc_alt.get(), SC_COUNT * sizeof(double), st);
+        cd* const Pn[2] = {P_new.as<cd>(), P_alt.as<cd>()};
+        double* const scs[2] = {sc, sc_alt.as<double>()};
+        constexpr size_t kSlot = 32;  // doubles per host slot (>= SC_RES + 2)
+        static_assert(SC_RES + 2 <= kSlot && SC_E < SC_RES, "host slot layout");
+        double* const hs = pinned_scalars();
+        auto iteration = [&](size_t j) {
+            const int s = static_cast<int>(j & 1);
+            if (j > 0) launch_axpy_c(nd, cfg.mixing, delta.as<cd>(), density.as<cd>(), st);
+            S.build_and_occupy(density.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(),
+                               Pn[s], scs[s] + SC_E, j > 0);
+            launch_scf_residuals(nk, nb, F.as<cd>(), density.as<cd>(), Pn[s], delta.as<cd>(),
+                                 nullptr, K.w2d.as<double>(), S.ws.ls, scs[s] + SC_RES, st);
+        };
+        auto enqueue = [&](size_t j) {
+            const int s = static_cast<int>(j & 1);
+            if (graphs.on && j > 0)
+                graphs.run(2 | s, [&] { iteration(j); });
+            else
+                iteration(j);
+            d2h(hs + s * kSlot, scs[s], (SC_RES + 2) * sizeof(double), st);
+            HFGPU_CHECK(hipEventRecord(ev[s].e, st));
+        };
+        enqueue(0);
+        for (size_t j = 0;; ++j) {
+            const bool ahead = j + 1 < cfg.max_iter;
+            if (ahead) enqueue(j + 1);
+            HFGPU_CHECK(hipEventSynchronize(ev[j & 1].e));
+            const double* h = hs + (j & 1) * kSlot;
+            const double d_res = std::sqrt(h[SC_RES] / ws);
+            const double c_res = std::sqrt(h[SC_RES + 1] / ws);
+            converged = (d_res <= cfg.density_tol) && (c_res <= cfg.comm_tol);
+            hE.push_back(h[SC_E]);
+            hD.push_back(d_res);
+            hC.push_back(c_res);
+            k = j + 1;
+            if (converged) {
+                d2d(density.get(), Pn[j & 1], bytes, st);
+                break;
+            }
+            if (!ahead) {
+                launch_axpy_c(nd, cfg.mixing, delta.as<cd>(), density.as<cd>(), st);
+                break;
+            }
+        }
+    }
+
+    while (!pipelined && k < cfg.max_iter && !converged) {
         auto body = [&]() {
             S.build_and_occupy(density.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(),
                                P_new.as<cd>(), sc + SC_E, k > 0);

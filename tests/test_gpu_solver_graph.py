@@ -1,5 +1,7 @@
 """Solver iterations replayed from captured hipGraphs against the same
-iterations launched one kernel at a time.
+iterations launched one kernel at a time, and the SCF loop that enqueues
+each iteration before the previous one's convergence test against the loop
+that tests first.
 
 After its first iteration each solver captures its per-iteration launch
 sequence once per variant (buffer parity, CG restart, precision) and replays
@@ -27,16 +29,20 @@ def _dm_args(k, P0, n_e, max_iter):
             0.5, 20, np.empty((0,) + k.h.shape), 0.0, 1.0, [], None, True, True, True)
 
 
-def _with_graphs(on, fn):
-    old = os.environ.get("HFGPU_SOLVER_GRAPH")
-    os.environ["HFGPU_SOLVER_GRAPH"] = "1" if on else "0"
+def _with_env(name, on, fn):
+    old = os.environ.get(name)
+    os.environ[name] = "1" if on else "0"
     try:
         return fn()
     finally:
         if old is None:
-            del os.environ["HFGPU_SOLVER_GRAPH"]
+            del os.environ[name]
         else:
-            os.environ["HFGPU_SOLVER_GRAPH"] = old
+            os.environ[name] = old
+
+
+def _with_graphs(on, fn):
+    return _with_env("HFGPU_SOLVER_GRAPH", on, fn)
 
 
 @pytest.mark.parametrize("n,nl", [(96, 4), (24, 6), (36, 1)])
@@ -67,5 +73,37 @@ def test_scf_graph_replay_equals_eager(gpu, accel):
     e = _with_graphs(False, lambda: gpu.solve_scf(*args))
     assert g[4] == e[4] and g[4] > 5
     for a, b in zip(g, e):
+        if a is not None:
+            np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+@pytest.mark.parametrize("case", ["converges", "max_iter", "one_iteration"])
+def test_scf_pipelined_equals_synchronous(gpu, case, graphs):
+    """Linear mixing runs iteration j + 1 ahead of iteration j's test; a
+    converged test discards it.  Converging before max_iter (a speculative
+    iteration is thrown away), stopping at max_iter (none is enqueued past
+    it) and max_iter = 1 must all equal the synchronous loop bit for bit."""
+    from cpp_hf_amd.workloads import layered_kernel, qwz_kernel
+
+    if case == "converges":
+        w = qwz_kernel(24)
+        max_iter, tol = 200, (1e-7, 1e-6)
+    else:
+        w = layered_kernel(48, 4, T=0.05, filling=4.1)
+        max_iter, tol = (30 if case == "max_iter" else 1), (1e-12, 1e-12)
+    args = (kernel_args_dict(w.kernel), w.P0, w.n_electrons, max_iter, tol[0], tol[1], 0.5, 0.0,
+            None, [], "linear", 6, 2, 1.0, 0.0, True, True)
+    print(f"scf pipelined {case} graphs={graphs}", flush=True)
+    p = _with_graphs(graphs, lambda: _with_env("HFGPU_SCF_PIPELINE", True,
+                                                lambda: gpu.solve_scf(*args)))
+    s = _with_graphs(graphs, lambda: _with_env("HFGPU_SCF_PIPELINE", False,
+                                                lambda: gpu.solve_scf(*args)))
+    assert p[4] == s[4] and p[5] == s[5]
+    if case == "converges":
+        assert p[5] and 1 < p[4] < max_iter
+    else:
+        assert not p[5] and p[4] == max_iter
+    for a, b in zip(p, s):
         if a is not None:
             np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
