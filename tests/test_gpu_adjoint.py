@@ -254,3 +254,24 @@ def test_fock_vjp_144_seed2_cotangent(gpu):
     fd = (loss(kern_with(Vq + step * dV), P) - loss(kern_with(Vq - step * dV), P)) / (2 * step)
     an = float(np.sum(g.coulomb_q * dV))
     assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+@pytest.mark.parametrize("name", ["u", "scale", "gate", "gamma1"])
+def test_gpu_adjoint_matches_oracle_resolves(gpu, oracle, name):
+    """The GPU adjoint's dOmega/dtheta at the oracle's converged 12 x 12,
+    4-band state against central differences of the oracle's own SCF
+    re-solves (the differences and the oracle adjoint are pinned against
+    each other in test_oracle_fd_resolve.py) and against the oracle adjoint
+    at the same state."""
+    import test_oracle_fd_resolve as fdr
+    from cpp_hf_amd import free_energy_derivative
+    from cpp_hf_amd.workloads import layered_tangents
+
+    k0, P0, _ = fdr.oracle_converged(oracle)
+    dh, dVq, dHH = layered_tangents(fdr.N_MESH, fdr.N_LAYERS, **fdr.BASE)[name]
+    print(f"gpu adjoint vs oracle re-solves: {name}", flush=True)
+    g = free_energy_derivative(k0, P0, dh=dh, dVq=dVq, dHH=dHH)
+    o = fdr.oracle_derivative(k0, P0, dh, dVq, dHH)
+    fd = fdr.central_differences(oracle, name)
+    assert abs(g - o) <= 1e-11 * max(abs(o), 1e-3), (name, g, o)
+    assert abs(g - fd) <= 1e-6 * max(abs(fd), 1e-3), (name, g, fd)
