@@ -61,4 +61,24 @@ void launch_scf_residuals(int nk, int nb, const cd* F, const cd* D, const cd* Pn
 void launch_wdot(int nk, int m, const cd* X, const cd* Y, const double* w, LoopScratch& ls,
                  double* out, hipStream_t st);
 
+// Up to kMaxDots arrays (or coefficients) passed by value to one launch.
+constexpr int kMaxDots = 16;
+struct CdPtrs {
+    const cd* p[kMaxDots];
+};
+struct DotCoefs {
+    double c[kMaxDots];
+};
+
+// out[d] = launch_wdot(X_d, Y) for d < nd <= kMaxDots, bit for bit, in one
+// pass over Y; out holds kMaxDots slots.
+void launch_wdot_multi(int nk, int m, const CdPtrs& X, int nd, const cd* Y, const double* w,
+                       LoopScratch& ls, double* out, hipStream_t st);
+
+// out = sum_{i < nx} c_i X_i (starting from zero, or from out with
+// accumulate), then, with Fcur, out = a out + b Fcur: the arithmetic of a
+// zero fill, one launch_axpy_c per term and an axpby, in one pass.
+void launch_lincomb(size_t n, const CdPtrs& X, const DotCoefs& c, int nx, bool accumulate,
+                    double a, double b, const cd* Fcur, cd* out, hipStream_t st);
+
 }  // namespace hfgpu

@@ -508,6 +508,58 @@ k_wdot(int nk, int m, const cd* __restrict__ X, const cd* __restrict__ Y,
     store_block_partials<1, 4>(acc, red, partial);
 }
 
+// k_wdot for nd arrays X_d against one Y: the same per-thread terms, k
+// order and reduction tree per dot, Y read once.
+__global__ void __launch_bounds__(256)
+k_wdot_multi(int nk, int m, CdPtrs X, int nd, const cd* __restrict__ Y,
+             const double* __restrict__ w, double* partial) {
+    __shared__ double red[4 * kMaxDots];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double acc[kMaxDots];
+#pragma unroll
+    for (int d = 0; d < kMaxDots; ++d) acc[d] = 0.0;
+    for (int k = blockIdx.x * 4 + wave; k < nk; k += gridDim.x * 4) {
+        const size_t kb = static_cast<size_t>(k) * m;
+        double s[kMaxDots];
+#pragma unroll
+        for (int d = 0; d < kMaxDots; ++d) s[d] = 0.0;
+        for (int e = lane; e < m; e += 64) {
+            const cd y = Y[kb + e];
+#pragma unroll
+            for (int d = 0; d < kMaxDots; ++d) {
+                if (d < nd) {
+                    const cd x = X.p[d][kb + e];
+                    s[d] += x.x * y.x + x.y * y.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < kMaxDots; ++d)
+            if (d < nd) acc[d] += w[k] * s[d];
+    }
+    store_block_partials<kMaxDots, 4>(acc, red, partial);
+}
+
+__global__ void __launch_bounds__(256)
+k_lincomb(size_t n, CdPtrs X, DotCoefs c, int nx, int accumulate, double a, double b,
+          const cd* __restrict__ Fcur, cd* __restrict__ out) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        cd y = accumulate ? out[i] : cd{0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d < kMaxDots; ++d) {
+            if (d < nx) {
+                const cd v = X.p[d][i];
+                y = cd{y.x + c.c[d] * v.x, y.y + c.c[d] * v.y};
+            }
+        }
+        if (Fcur) {
+            const cd v = Fcur[i];
+            y = cd{a * y.x + b * v.x, a * y.y + b * v.y};
+        }
+        out[i] = y;
+    }
+}
 
 }  // namespace
 
@@ -611,6 +663,25 @@ void launch_wdot(int nk, int m, const cd* X, const cd* Y, const double* w, LoopS
                        ls.partial);
     HFGPU_LAUNCH_CHECK();
     finalize(ls, loop_blocks(nk, 4), 1, out, st);
+}
+
+void launch_wdot_multi(int nk, int m, const CdPtrs& X, int nd, const cd* Y, const double* w,
+                       LoopScratch& ls, double* out, hipStream_t st) {
+    if (nd < 1 || nd > kMaxDots) throw std::invalid_argument("launch_wdot_multi: 1 <= nd <= 16");
+    hipLaunchKernelGGL(k_wdot_multi, dim3(loop_blocks(nk, 4)), dim3(256), 0, st, nk, m, X, nd,
+                       Y, w, ls.partial);
+    HFGPU_LAUNCH_CHECK();
+    finalize(ls, loop_blocks(nk, 4), kMaxDots, out, st);
+}
+
+void launch_lincomb(size_t n, const CdPtrs& X, const DotCoefs& c, int nx, bool accumulate,
+                    double a, double b, const cd* Fcur, cd* out, hipStream_t st) {
+    if (nx < 0 || nx > kMaxDots) throw std::invalid_argument("launch_lincomb: 0 <= nx <= 16");
+    if (!n) return;
+    const unsigned grid = std::max(1u, std::min<unsigned>(cdiv(n, 256), 8192u));
+    hipLaunchKernelGGL(k_lincomb, dim3(grid), dim3(256), 0, st, n, X, c, nx, accumulate ? 1 : 0,
+                       a, b, Fcur, out);
+    HFGPU_LAUNCH_CHECK();
 }
 
 }  // namespace hfgpu

@@ -348,20 +348,6 @@ struct ScfState {
     }
 };
 
-__global__ void k_axpby(size_t n, double a, cd* __restrict__ y, double b, const cd* __restrict__ x) {
-    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
-         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const cd u = y[i], v = x[i];
-        y[i] = cd{a * u.x + b * v.x, a * u.y + b * v.y};
-    }
-}
-
-void launch_axpby(size_t n, double a, cd* y, double b, const cd* x, hipStream_t st) {
-    hipLaunchKernelGGL(k_axpby, dim3(std::max(1u, std::min<unsigned>(cdiv(n, 256), 8192u))),
-                       dim3(256), 0, st, n, a, y, b, x);
-    HFGPU_LAUNCH_CHECK();
-}
-
 }  // namespace
 
 void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_config& cfg,
@@ -378,6 +364,12 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
         hdiag_new(use_oda ? nb * 8 : 0), pre(cfg.trust_radius > 0.0 ? bytes : 0),
         diff(cfg.trust_radius > 0.0 ? bytes : 0);
     std::vector<DevBuf> diis_F, diis_R;
+    // DIIS: B entries of the kept residual pairs (row-major, history order),
+    // the new residual's products, and the extrapolated Fock matrix
+    std::vector<double> Bc, hdots;
+    const size_t dot_slots = use_diis ? kMaxDots * (cfg.diis_size / kMaxDots + 1) : 0;
+    DevBuf dots(dot_slots * sizeof(double)), Fx(use_diis ? bytes : 0);
+    hdots.resize(dot_slots);
     S.mixed = cfg.exchange_fp32 != 0;
     const double fp32_switch = cfg.fp32_switch > 0.0 ? cfg.fp32_switch : 1e-6;
     h2d(density.get(), P0, bytes, st);
@@ -475,6 +467,25 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
             graphs.run(S.mixed ? 1 : 0, body);
         else
             body();
+        // DIIS: the products of the new commutator residual with the
+        // residuals the history keeps (its oldest leaves when it is full) and
+        // with itself, in the read-back of the convergence test.  The
+        // reference recomputes every B entry each iteration
+        // (solver_scf.hpp:263-290); an entry depends only on its two stored
+        // residuals, so the older ones are kept on the host instead.
+        const size_t drop = (use_diis && !diis_R.empty() && diis_R.size() >= cfg.diis_size) ? 1 : 0;
+        const size_t ndots = use_diis ? diis_R.size() - drop + 1 : 0;
+        for (size_t c0 = 0; c0 < ndots; c0 += kMaxDots) {
+            CdPtrs X{};
+            const size_t cnt = std::min<size_t>(kMaxDots, ndots - c0);
+            for (size_t d = 0; d < cnt; ++d) {
+                const size_t i = c0 + d + drop;
+                X.p[d] = i < diis_R.size() ? diis_R[i].as<cd>() : comm.as<cd>();
+            }
+            launch_wdot_multi(nk, static_cast<int>(K.nb2), X, static_cast<int>(cnt), comm.as<cd>(),
+                              K.w2d.as<double>(), S.ws.ls, dots.as<double>() + c0, st);
+        }
+        if (ndots) d2h(hdots.data(), dots.get(), dots.size(), st);
         d2h(hsc, sc, sizeof(hsc), st);
         d2h(&hmu, S.ws.mu(), sizeof(MuState), st);
         HFGPU_CHECK(hipStreamSynchronize(st));
@@ -504,32 +515,35 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
         if (converged) {
             d2d(density.get(), P_new.get(), bytes, st);
         } else if (use_diis) {
-            if (diis_F.size() >= cfg.diis_size && !diis_F.empty()) {
+            if (drop) {
                 diis_F.erase(diis_F.begin());
                 diis_R.erase(diis_R.begin());
+                const size_t mo = diis_R.size() + 1;
+                std::vector<double> Bn((mo - 1) * (mo - 1));
+                for (size_t i = 1; i < mo; ++i)
+                    for (size_t j = 1; j < mo; ++j) Bn[(i - 1) * (mo - 1) + j - 1] = Bc[i * mo + j];
+                Bc.swap(Bn);
             }
             diis_F.emplace_back(bytes);
             diis_R.emplace_back(bytes);
             d2d(diis_F.back().get(), F.get(), bytes, st);
             d2d(diis_R.back().get(), comm.get(), bytes, st);
             const size_t m = diis_F.size();
+            {
+                std::vector<double> Bn(m * m);
+                for (size_t i = 0; i + 1 < m; ++i)
+                    for (size_t j = 0; j + 1 < m; ++j) Bn[i * m + j] = Bc[i * (m - 1) + j];
+                for (size_t i = 0; i < m; ++i) {
+                    Bn[i * m + (m - 1)] = hdots[i];
+                    Bn[(m - 1) * m + i] = hdots[i];
+                }
+                Bc.swap(Bn);
+            }
             if (m >= std::max<size_t>(2, cfg.diis_start)) {
                 const size_t n = m + 1;
-                DevBuf dB(m * m * sizeof(double));
-                for (size_t i = 0; i < m; ++i)
-                    for (size_t j = i; j < m; ++j)
-                        launch_wdot(nk, static_cast<int>(K.nb2), diis_R[i].as<cd>(),
-                                    diis_R[j].as<cd>(), K.w2d.as<double>(), S.ws.ls,
-                                    dB.as<double>() + i * m + j, st);
-                std::vector<double> hb(m * m);
-                d2h(hb.data(), dB.get(), m * m * sizeof(double), st);
-                HFGPU_CHECK(hipStreamSynchronize(st));
                 std::vector<double> B(n * n, 0.0), rhs(n, 0.0), sol;
                 for (size_t i = 0; i < m; ++i) {
-                    for (size_t j = i; j < m; ++j) {
-                        B[i * n + j] = hb[i * m + j];
-                        B[j * n + i] = hb[i * m + j];
-                    }
+                    for (size_t j = 0; j < m; ++j) B[i * n + j] = Bc[i * m + j];
                     B[i * n + m] = -1.0;
                     B[m * n + i] = -1.0;
                 }
@@ -537,18 +551,31 @@ void solve_scf_gpu(GpuKernel& K, const double* P0, double n_e, const hf_scf_conf
                 bool ok = fullpiv_solve(B, rhs, n, sol);
                 for (size_t i = 0; ok && i < m; ++i) ok = std::isfinite(sol[i]);
                 if (ok) {
-                    DevBuf Fx(bytes), Pd(bytes);
-                    fill_zero(Fx.get(), bytes, st);
-                    for (size_t i = 0; i < m; ++i)
-                        launch_axpy_c(nd, sol[i], diis_F[i].as<cd>(), Fx.as<cd>(), st);
+                    // F_extrap = sum_i c_i F_i, then the damping blend, in one
+                    // pass per 16 terms (solver_scf.hpp:305-324).  Real
+                    // combinations of exactly Hermitian Fock matrices are
+                    // exactly Hermitian and the density kernel hermitises,
+                    // so without a projector the reference's two
+                    // hermitisations are no-ops and are skipped.
                     const double damp = std::clamp(cfg.diis_damping, 0.0, 1.0);
-                    if (damp < 1.0) launch_axpby(nd, damp, Fx.as<cd>(), 1.0 - damp, F.as<cd>(), st);
-                    launch_hermitize(nk, nb, Fx.as<cd>(), st);
-                    S.density_from_fock(Fx.as<cd>(), Pd.as<cd>());
-                    if (proj) proj.apply(Pd.as<cd>(), st);
-                    launch_hermitize(nk, nb, Pd.as<cd>(), st);
-                    d2d(density.get(), Pd.get(), bytes, st);
-                    HFGPU_CHECK(hipStreamSynchronize(st));
+                    for (size_t c0 = 0; c0 < m; c0 += kMaxDots) {
+                        CdPtrs X{};
+                        DotCoefs cf{};
+                        const size_t cnt = std::min<size_t>(kMaxDots, m - c0);
+                        for (size_t d = 0; d < cnt; ++d) {
+                            X.p[d] = diis_F[c0 + d].as<cd>();
+                            cf.c[d] = sol[c0 + d];
+                        }
+                        const bool last = c0 + cnt == m;
+                        launch_lincomb(nd, X, cf, static_cast<int>(cnt), c0 > 0, damp, 1.0 - damp,
+                                       last && damp < 1.0 ? F.as<cd>() : nullptr, Fx.as<cd>(), st);
+                    }
+                    if (proj) launch_hermitize(nk, nb, Fx.as<cd>(), st);
+                    S.density_from_fock(Fx.as<cd>(), density.as<cd>());
+                    if (proj) {
+                        proj.apply(density.as<cd>(), st);
+                        launch_hermitize(nk, nb, density.as<cd>(), st);
+                    }
                 } else {
                     linear_mix(cfg.mixing);
                 }
