@@ -53,11 +53,12 @@ constexpr int kMuPartialDoubles = 16384 + 28 * 256 * 8;
 // state->mu.  ws: kMuPartialDoubles of workspace.
 void launch_mu_bisection(const double* eps, const double* w, int nk, int nb, double n_e,
                          double T, int maxiter, MuState* state, double* ws, hipStream_t st);
-// solve_mu_inloop (utils.hpp:79-116); warm start read from state->mu when
-// mu0_from_state, else mu0.
+// solve_mu_inloop (utils.hpp:79-116); warm start read from warm->mu (warm
+// defaults to state) when mu0_from_state, else mu0; the result goes to state.
 void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
                       double n_target_norm, double mu0, bool mu0_from_state, double T,
-                      int maxiter, MuState* state, double* ws, hipStream_t st);
+                      int maxiter, MuState* state, double* ws, hipStream_t st,
+                      const MuState* warm = nullptr);
 
 // The Python surface's Newton (src/cpp_hf/utils.py:311-360) on raw weights:
 // one workgroup, result in *mu_out.

@@ -53,8 +53,10 @@ void launch_linesearch(int nk, int nb, const cd* V, const double* lam, const cd*
                        const double* p, const double* d_p, const double* w, double T_r,
                        double max_step, double shrink, int nbt, LoopScratch& ls,
                        hipStream_t st);
+// Q_out (default Q itself) receives the rotated orbitals.
 void launch_retract(int nk, int nb, cd* Q, const cd* V, const double* lam, const cd* Fe,
-                    double* eps_work, bool cayley, LoopScratch& ls, hipStream_t st);
+                    double* eps_work, bool cayley, LoopScratch& ls, hipStream_t st,
+                    cd* Q_out = nullptr);
 void launch_scf_residuals(int nk, int nb, const cd* F, const cd* D, const cd* Pn, cd* delta,
                           cd* comm, const double* w, LoopScratch& ls, double* out,
                           hipStream_t st);

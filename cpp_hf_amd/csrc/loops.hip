@@ -360,13 +360,14 @@ k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restr
     }
 }
 
-// Q <- Q V diag(c(tau)) V^H and eps_work = diag(U^H Ft U) at the chosen tau.
+// Q_out <- Q V diag(c(tau)) V^H and eps_work = diag(U^H Ft U) at the chosen
+// tau (Q_out may be Q: each k point's block is read before it is written).
 // use_sc_tau = 0 with U = V (tau ignored): the finalise rotation Q <- Q V.
 template <int W, int NB>
 __global__ void __launch_bounds__(W * 64)
-k_retract(int nk, int nb_arg, cd* __restrict__ Q, const cd* __restrict__ V,
+k_retract(int nk, int nb_arg, const cd* Q, const cd* __restrict__ V,
           const double* __restrict__ lam, const cd* __restrict__ Fe,
-          double* __restrict__ eps_work, int cayley, const double* sc) {
+          double* __restrict__ eps_work, int cayley, const double* sc, cd* Q_out) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     extern __shared__ cd smem[];
     const int nb2 = nb * nb;
@@ -414,7 +415,7 @@ k_retract(int nk, int nb_arg, cd* __restrict__ Q, const cd* __restrict__ V,
                 cd s{0.0, 0.0};
                 const cd* R = cayley ? sU : sV;
                 for (int n = 0; n < nb; ++n) s = cadd(s, cmul(sQ[i * nb + n], R[n * nb + j]));
-                Q[kb + e] = s;
+                Q_out[kb + e] = s;
             }
         }
         __syncthreads();
@@ -638,11 +639,11 @@ void launch_linesearch(int nk, int nb, const cd* V, const double* lam, const cd*
 }
 
 void launch_retract(int nk, int nb, cd* Q, const cd* V, const double* lam, const cd* Fe,
-                    double* eps_work, bool cayley, LoopScratch& ls, hipStream_t st) {
+                    double* eps_work, bool cayley, LoopScratch& ls, hipStream_t st, cd* Q_out) {
     check_nb(nb);
     const size_t per_wave = ls_wave_bytes(nb);
     HFGPU_LOOP_LAUNCH(k_retract, nb, nk, per_wave, st, nk, nb, Q, V, lam, Fe, eps_work,
-                      cayley ? 1 : 0, ls.sc);
+                      cayley ? 1 : 0, ls.sc, Q_out ? Q_out : Q);
     HFGPU_LAUNCH_CHECK();
 }
 

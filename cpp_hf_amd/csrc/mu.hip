@@ -836,7 +836,9 @@ void launch_mu_bisection(const double* eps, const double* w, int nk, int nb, dou
 
 void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
                       double n_target_norm, double mu0, bool mu0_from_state, double T,
-                      int maxiter, MuState* state, double* ws, hipStream_t st) {
+                      int maxiter, MuState* state, double* ws, hipStream_t st,
+                      const MuState* warm) {
+    if (!warm) warm = state;
     const size_t n = static_cast<size_t>(nk) * nb;
     // a few eigenvalues per thread: each step also re-reduces the previous
     // step's partials in every workgroup
@@ -847,7 +849,7 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
         if (persist_fits(1 + maxiter, G)) {
             fill_slots(ws, 1 + maxiter, G, st);
             hipLaunchKernelGGL(k_newton_persist, dim3(G), dim3(kBlock), 0, st, eps, w_norm, nb, n,
-                               n_target_norm, T, mu0, mu0_from_state ? 1 : 0, state, maxiter,
+                               n_target_norm, T, mu0, mu0_from_state ? 1 : 0, warm, maxiter,
                                ws + kPersistSlotBase, state);
             HFGPU_LAUNCH_CHECK();
             return;
@@ -857,7 +859,7 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
     HFGPU_LAUNCH_CHECK();
     if (maxiter > 0 && n <= kNewtonSoloMax) {
         hipLaunchKernelGGL(k_newton_solo, dim3(1), dim3(kBlock), 0, st, eps, w_norm, nb, n,
-                           n_target_norm, T, mu0, mu0_from_state ? 1 : 0, state, maxiter,
+                           n_target_norm, T, mu0, mu0_from_state ? 1 : 0, warm, maxiter,
                            static_cast<int>(mm_blocks), ws, state);
         HFGPU_LAUNCH_CHECK();
         return;
@@ -865,7 +867,7 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
     int prev_parts = static_cast<int>(mm_blocks);
     for (int j = 0; j < maxiter; ++j) {
         hipLaunchKernelGGL(k_newton_step, dim3(blocks), dim3(kBlock), 0, st, eps, w_norm, nb, n,
-                           n_target_norm, T, mu0, mu0_from_state ? 1 : 0, state, j, prev_parts,
+                           n_target_norm, T, mu0, mu0_from_state ? 1 : 0, warm, j, prev_parts,
                            ws);
         HFGPU_LAUNCH_CHECK();
         prev_parts = static_cast<int>(blocks);
@@ -877,7 +879,7 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
     } else {
         // no iterations: the clamped warm start is the answer (utils.hpp:90)
         hipLaunchKernelGGL(k_newton_step, dim3(1), dim3(kBlock), 0, st, eps, w_norm, nb,
-                           size_t(0), n_target_norm, T, mu0, mu0_from_state ? 1 : 0, state, 0,
+                           size_t(0), n_target_norm, T, mu0, mu0_from_state ? 1 : 0, warm, 0,
                            prev_parts, ws);
         HFGPU_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_newton_last, dim3(1), dim3(kBlock), 0, st, n_target_norm, 1,

@@ -54,14 +54,16 @@ struct Projector {
 };
 
 // Per-iteration launch sequences of the solver loops replayed from captured
-// hipGraphs (HFGPU_SOLVER_GRAPH=0 launches them eagerly): one host call per
-// iteration instead of one per kernel, which matters most when many solves
-// run concurrently from host threads and their launches contend in the
-// runtime.  The captured work allocates nothing and clears memory only with
-// fill kernels (no memset nodes).
+// hipGraphs with HFGPU_SOLVER_GRAPH=1: one host call per iteration instead of
+// one per kernel.  Off by default: with the next iteration enqueued ahead of
+// the host's tests the launches are off the critical path, and a replay
+// measured slower than the same kernels launched one by one (C1 direct
+// minimisation 3.0 against 2.4 ms, C2 14.5 against 13.8 ms; the 48-solve
+// sweep unchanged).  The captured work allocates nothing and clears memory
+// only with fill kernels (no memset nodes).
 bool solver_graphs_on() {
     const char* e = std::getenv("HFGPU_SOLVER_GRAPH");  // read per solve
-    return !(e && e[0] == '0');
+    return e && e[0] == '1';
 }
 
 struct IterGraphs {
@@ -107,6 +109,13 @@ struct IterGraphs {
 // convergence test (HFGPU_SCF_PIPELINE=0: the test before every launch).
 bool scf_pipeline_on() {
     const char* e = std::getenv("HFGPU_SCF_PIPELINE");  // read per solve
+    return !(e && e[0] == '0');
+}
+
+// The direct-minimisation loop enqueues each iteration before the previous
+// one's stop test (HFGPU_DM_PIPELINE=0: the test before every launch).
+bool dm_pipeline_on() {
+    const char* e = std::getenv("HFGPU_DM_PIPELINE");  // read per solve
     return !(e && e[0] == '0');
 }
 
@@ -744,14 +753,127 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
     double dE = std::numeric_limits<double>::infinity();
     size_t k = 0;
     double hsc[SC_COUNT];
-    while (k < cfg.max_iter) {
+    // the reference's test at the top of iteration k (solver_dm.hpp:476-490)
+    auto keep_going = [&]() {
         bool e_nc;
         if (cfg.plateau_window > 0 && k > cfg.plateau_window)
             e_nc = (hist_E[k - 1 - cfg.plateau_window] - hist_E[k - 1]) > cfg.tol_E;
         else
             e_nc = dE > cfg.tol_E;
         const bool g_nc = (cfg.tol_grad > 0.0) && (grad_norm > cfg.tol_grad);
-        if (!(e_nc || g_nc)) {
+        return e_nc || g_nc;
+    };
+    // the host's record of an iteration from its scalar slots
+    auto record = [&](const double* h) {
+        const double E = h[SC_E];
+        grad_norm = std::sqrt(std::max(0.0, h[SC_GN2]));
+        hist_E.push_back(E);
+        hist_G.push_back(grad_norm);
+        // trials the reference's sequential search evaluates: up to and
+        // including the accepted tau0 shrink^j, all bt_max when none is
+        if (h[SC_ACCEPTED] > 0.5 && h[SC_TAU0] > 0.0 && cfg.bt_shrink > 0.0 &&
+            cfg.bt_shrink < 1.0) {
+            const double j = std::log(h[SC_TAU] / h[SC_TAU0]) / std::log(cfg.bt_shrink);
+            ls_trials += static_cast<size_t>(std::llround(std::max(0.0, j))) + 1;
+        } else {
+            ls_trials += cfg.bt_max;
+        }
+        dE = std::fabs(E - E_prev);
+        E_prev = E;
+    };
+
+    // Without a projector or FP32 builds, iteration k + 1 depends on the host
+    // only through the stop test at its top, which reads iteration k's energy
+    // and gradient norm: it is enqueued before those are read back.  Its new
+    // orbitals, occupations and chemical potential go to the other of two
+    // buffers (the retraction and the Newton solve write out of place) and
+    // its scalars to the other of two slot arrays, so when the test stops the
+    // loop the speculative iteration is discarded and the state is that of
+    // the test-first loop bit for bit.  HFGPU_DM_PIPELINE=0 tests first.
+    const bool spec = !proj && !mixed && cfg.max_iter > 0 && dm_pipeline_on();
+    // the second buffers and the events live until the solve's final stream
+    // synchronisation (a discarded iteration may still be running)
+    DevBuf Q_alt, p_alt, mu_alt, sc_alt;
+    StreamEvent ev[2];
+    if (spec) {
+        Q_alt.alloc(bytes);
+        p_alt.alloc(rb);
+        mu_alt.alloc(sizeof(MuState));
+        sc_alt.alloc(SC_COUNT * sizeof(double));
+        fill_zero(mu_alt.get(), sizeof(MuState), st);
+        fill_zero(sc_alt.get(), SC_COUNT * sizeof(double), st);
+        ev[0].create();
+        ev[1].create();
+        cd* const Qs[2] = {Q.as<cd>(), Q_alt.as<cd>()};
+        double* const ps[2] = {p.as<double>(), p_alt.as<double>()};
+        MuState* const ms[2] = {W.mu(), mu_alt.as<MuState>()};
+        LoopScratch lss[2] = {W.ls, W.ls};
+        lss[1].sc = sc_alt.as<double>();
+        constexpr size_t kSlot = 32;  // doubles per host slot
+        static_assert(SC_ACCEPTED < kSlot, "host slot layout");
+        double* const hs = pinned_scalars();
+        // iteration j reads state j & 1 and writes state (j + 1) & 1
+        auto iteration = [&](size_t j, bool restart) {
+            const int s0 = static_cast<int>(j & 1), s1 = s0 ^ 1;
+            LoopScratch& L = lss[s0];
+            launch_density(nk, nb, Qs[s0], ps[s0], P.as<cd>(), true, st);
+            K.build_fock(P.as<cd>(), Sigma.as<cd>(), F.as<cd>(), hdiag.as<double>(), true, st,
+                         true, false);
+            K.energy(P.as<cd>(), Sigma.as<cd>(), hdiag.as<double>(), L.sc + SC_E, st);
+            launch_orbital_grad(nk, nb, Qs[s0], F.as<cd>(), ps[s0], wd, ms[s0], T_r, true,
+                                eps.as<double>(), G.as<cd>(), g_p.as<double>(), L, st);
+            launch_precond(nk, nb, G.as<cd>(), Gp.as<cd>(), eps.as<double>(), ps[s0],
+                           g_p.as<double>(), g_pp.as<double>(), wd, T_r, cfg.denom_scale,
+                           dQ.as<cd>(), h_p.as<double>(), L, st);
+            launch_cg_combine(nk, nb, restart, dQ.as<cd>(), dQp.as<cd>(), h_p.as<double>(),
+                              d_pp.as<double>(), d_p.as<double>(), eps.as<double>(), ps[s0], wd,
+                              T_r, L, st);
+            eigh(1, dQ.as<cd>(), lam.as<double>(), Vd.as<cd>());
+            launch_linesearch(nk, nb, Vd.as<cd>(), lam.as<double>(), F.as<cd>(), Sigma.as<cd>(),
+                              ps[s0], d_p.as<double>(), wd, T_r, cfg.max_step, cfg.bt_shrink,
+                              static_cast<int>(cfg.bt_max), L, st);
+            launch_retract(nk, nb, Qs[s0], Vd.as<cd>(), lam.as<double>(), Sigma.as<cd>(),
+                           eps_w.as<double>(), true, L, st, Qs[s1]);
+            launch_mu_newton(eps_w.as<double>(), wd, nk, nb, n_target, 0.0, true, T_r,
+                             cfg.mu_maxiter, ms[s1], W.mu_partial.as<double>(), st, ms[s0]);
+            launch_occupations(eps_w.as<double>(), nkb, ms[s1], T_r, 1, ps[s1], st);
+        };
+        auto enqueue = [&](size_t j) {
+            const bool restart = (j == 0) || (cfg.cg_restart > 0 && j % cfg.cg_restart == 0);
+            const int s0 = static_cast<int>(j & 1);
+            if (graphs.on && j > 0)
+                graphs.run(8 | s0 | (restart ? 2 : 0), [&] { iteration(j, restart); });
+            else
+                iteration(j, restart);
+            d2h(hs + s0 * kSlot, lss[s0].sc, (SC_ACCEPTED + 1) * sizeof(double), st);
+            HFGPU_CHECK(hipEventRecord(ev[s0].e, st));
+            // the buffer pairs the next iteration reads as its history
+            std::swap(dQ, dQp);
+            std::swap(d_p, d_pp);
+            std::swap(G, Gp);
+            std::swap(g_p, g_pp);
+        };
+        if (keep_going()) {
+            enqueue(0);
+            for (;;) {
+                const bool ahead = k + 1 < cfg.max_iter;
+                if (ahead) enqueue(k + 1);
+                HFGPU_CHECK(hipEventSynchronize(ev[k & 1].e));
+                record(hs + (k & 1) * kSlot);
+                ++k;
+                if (!ahead || !keep_going()) break;  // an enqueued iteration k is discarded
+            }
+        }
+        // the state after k iterations is state k & 1
+        if (k & 1) {
+            std::swap(Q, Q_alt);
+            std::swap(p, p_alt);
+            std::swap(W.mus, mu_alt);
+        }
+    }
+
+    while (!spec && k < cfg.max_iter) {
+        if (!keep_going()) {
             if (!mixed) break;
             mixed = false;  // stop only on FP64 builds
         }
@@ -789,26 +911,12 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
 
         d2h(hsc, sc, sizeof(hsc), st);
         HFGPU_CHECK(hipStreamSynchronize(st));
-        const double E = hsc[SC_E];
-        grad_norm = std::sqrt(std::max(0.0, hsc[SC_GN2]));
-        hist_E.push_back(E);
-        hist_G.push_back(grad_norm);
-        // trials the reference's sequential search evaluates: up to and
-        // including the accepted tau0 shrink^j, all bt_max when none is
-        if (hsc[SC_ACCEPTED] > 0.5 && hsc[SC_TAU0] > 0.0 && cfg.bt_shrink > 0.0 &&
-            cfg.bt_shrink < 1.0) {
-            const double j = std::log(hsc[SC_TAU] / hsc[SC_TAU0]) / std::log(cfg.bt_shrink);
-            ls_trials += static_cast<size_t>(std::llround(std::max(0.0, j))) + 1;
-        } else {
-            ls_trials += cfg.bt_max;
-        }
+        record(hsc);
         if (mixed && grad_norm <= fp32_switch) mixed = false;
-        dE = std::fabs(E - E_prev);
         std::swap(dQ, dQp);
         std::swap(d_p, d_pp);
         std::swap(G, Gp);
         std::swap(g_p, g_pp);
-        E_prev = E;
         ++k;
     }
 

@@ -107,3 +107,43 @@ def test_scf_pipelined_equals_synchronous(gpu, case, graphs):
     for a, b in zip(p, s):
         if a is not None:
             np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+@pytest.mark.parametrize("case", ["converges", "max_iter", "one_iteration", "layered_converges"])
+def test_dm_pipelined_equals_synchronous(gpu, case, graphs):
+    """Direct minimisation enqueues iteration k + 1 before iteration k's stop
+    test; a stop discards it.  Stopping on the energy test before max_iter,
+    at max_iter and after one iteration must all equal the test-first loop
+    bit for bit (orbitals, occupations, density, Fock matrix, histories)."""
+    from cpp_hf_amd import SolverConfig
+    from cpp_hf_amd.workloads import layered_kernel, qwz_kernel
+
+    if case == "converges":
+        w = qwz_kernel(24)
+        c = SolverConfig()
+        args = (kernel_args_dict(w.kernel), w.P0, w.n_electrons, int(c.max_iter), float(c.tol_E),
+                float(c.tol_grad), float(c.max_step), float(c.bt_shrink), float(c.denom_scale),
+                int(c.bt_max), int(c.cg_restart), int(c.plateau_window), int(c.mu_maxiter), 0,
+                float(c.tr_delta0), int(c.tr_cg_max), np.empty((0,) + w.kernel.h.shape), 0.0,
+                1.0, [], None, True, True, True)
+    elif case == "layered_converges":
+        w = layered_kernel(48, 4, T=0.05, filling=4.1)
+        args = _dm_args(w.kernel, w.P0, w.n_electrons, 300)
+        args = args[:4] + (1e-9,) + args[5:]
+    else:
+        w = layered_kernel(36, 2, T=0.05, filling=2.1)
+        args = _dm_args(w.kernel, w.P0, w.n_electrons, 40 if case == "max_iter" else 1)
+    print(f"dm pipelined {case} graphs={graphs}", flush=True)
+    p = _with_graphs(graphs, lambda: _with_env("HFGPU_DM_PIPELINE", True,
+                                                lambda: gpu.solve_dm(*args)))
+    s = _with_graphs(graphs, lambda: _with_env("HFGPU_DM_PIPELINE", False,
+                                                lambda: gpu.solve_dm(*args)))
+    assert p[6] == s[6] and p[7] == s[7]
+    if case in ("converges", "layered_converges"):
+        assert p[7] and 1 < p[6] < args[3]
+    else:
+        assert not p[7] and p[6] == args[3]
+    for a, b in zip(p, s):
+        if a is not None:
+            np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
