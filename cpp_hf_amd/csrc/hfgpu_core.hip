@@ -167,30 +167,113 @@ void dev_free(void* p, size_t bytes) {
 // GpuKernel
 // ---------------------------------------------------------------------------
 
-static bool dense_vr_packable(const double* VR, size_t nk1, size_t nk2, size_t nb) {
-    // Packed exchange for (nb, nb) V needs the index-space kernel W_ab =
-    // IFFT(VR_ab) to be real and symmetric in (a, b):
-    //   VR_ab(q) == VR_ba(q)  and  VR_ab(-q) == conj(VR_ab(q)),
-    // each to 64 eps of max |VR| (compared as squared magnitudes).
-    const size_t nb2 = nb * nb;
+namespace {
+
+// std::max's selection (NaN operands ignored as the host loop ignored them)
+__device__ __forceinline__ double hmax(double a, double b) { return a < b ? b : a; }
+
+constexpr int kCheckBlocks = 256;
+
+// Packed exchange for (nb, nb) V needs the index-space kernel W_ab =
+// IFFT(VR_ab) to be real and symmetric in (a, b):
+//   VR_ab(q) == VR_ba(q)  and  VR_ab(-q) == conj(VR_ab(q)),
+// each to 64 eps of max |VR| (compared as squared magnitudes).  Per
+// workgroup: max |VR|^2 and the larger squared defect, without contraction
+// (maxima are exact, so the decision does not depend on the order).
+__global__ void __launch_bounds__(256)
+k_vr_check(const double* __restrict__ VR, int nk1, int nk2, int nb, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ double red[2][4];
+    const size_t nb2 = static_cast<size_t>(nb) * nb;
+    const size_t n = static_cast<size_t>(nk1) * nk2 * nb2;
     double vmax2 = 0.0, err2 = 0.0;
-    for (size_t q1 = 0; q1 < nk1; ++q1)
-        for (size_t q2 = 0; q2 < nk2; ++q2) {
-            const size_t q = q1 * nk2 + q2;
-            const size_t mq = ((nk1 - q1) % nk1) * nk2 + (nk2 - q2) % nk2;
-            for (size_t a = 0; a < nb; ++a)
-                for (size_t b = 0; b < nb; ++b) {
-                    const double* v = VR + 2 * (q * nb2 + a * nb + b);
-                    const double* vt = VR + 2 * (q * nb2 + b * nb + a);
-                    const double* vm = VR + 2 * (mq * nb2 + a * nb + b);
-                    const double d1r = v[0] - vt[0], d1i = v[1] - vt[1];
-                    const double d2r = vm[0] - v[0], d2i = vm[1] + v[1];
-                    vmax2 = std::max(vmax2, v[0] * v[0] + v[1] * v[1]);
-                    err2 = std::max(err2, std::max(d1r * d1r + d1i * d1i, d2r * d2r + d2i * d2i));
-                }
+    for (size_t idx = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; idx < n;
+         idx += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t q = idx / nb2, ab = idx % nb2;
+        const size_t a = ab / nb, b = ab % nb;
+        const size_t q1 = q / nk2, q2 = q % nk2;
+        const size_t mq = ((nk1 - q1) % nk1) * nk2 + (nk2 - q2) % nk2;
+        const double* v = VR + 2 * (q * nb2 + a * nb + b);
+        const double* vt = VR + 2 * (q * nb2 + b * nb + a);
+        const double* vm = VR + 2 * (mq * nb2 + a * nb + b);
+        const double d1r = v[0] - vt[0], d1i = v[1] - vt[1];
+        const double d2r = vm[0] - v[0], d2i = vm[1] + v[1];
+        vmax2 = hmax(vmax2, v[0] * v[0] + v[1] * v[1]);
+        err2 = hmax(err2, hmax(d1r * d1r + d1i * d1i, d2r * d2r + d2i * d2i));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        vmax2 = hmax(vmax2, __shfl_xor(vmax2, off, 64));
+        err2 = hmax(err2, __shfl_xor(err2, off, 64));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][wave] = vmax2;
+        red[1][wave] = err2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double m = red[threadIdx.x][0];
+        for (int w = 1; w < 4; ++w) m = hmax(m, red[threadIdx.x][w]);
+        out[2 * blockIdx.x + threadIdx.x] = m;
+    }
+}
+
+// 1 in out[block] when some k point's reference block differs bit for bit
+// from the first (the memcmp test the host ran before).
+__global__ void __launch_bounds__(256)
+k_refp_const(const unsigned long long* __restrict__ R, size_t n_words, size_t blk_words,
+             int* __restrict__ out) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    int diff = 0;
+    for (size_t i = blk_words + blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+         i < n_words; i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        diff |= R[i] != R[i % blk_words] ? 1 : 0;
+    if (diff) any = 1;  // every writer writes 1
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = any;
+}
+
+}  // namespace
+
+// The two set-up tests on the uploaded arrays, on the device: a host loop
+// over VR's transposed and mirrored entries took milliseconds at 120 x 120
+// k with 12 bands.
+static void device_setup_checks(const cd* VR, bool vr_test, const cd* refP, bool refp_test,
+                                int nk1, int nk2, int nb, hipStream_t st, bool* packable,
+                                bool* refp_const) {
+    const size_t nb2 = static_cast<size_t>(nb) * nb, nd = static_cast<size_t>(nk1) * nk2 * nb2;
+    const unsigned blocks = std::max(1u, std::min<unsigned>(cdiv(nd, 256), kCheckBlocks));
+    DevBuf vr_part(2 * kCheckBlocks * sizeof(double)), rp_part(kCheckBlocks * sizeof(int));
+    if (vr_test)
+        hipLaunchKernelGGL(k_vr_check, dim3(blocks), dim3(256), 0, st,
+                           reinterpret_cast<const double*>(VR), nk1, nk2, nb,
+                           vr_part.as<double>());
+    if (refp_test)
+        hipLaunchKernelGGL(k_refp_const, dim3(blocks), dim3(256), 0, st,
+                           reinterpret_cast<const unsigned long long*>(refP), 2 * nd, 2 * nb2,
+                           rp_part.as<int>());
+    HFGPU_LAUNCH_CHECK();
+    std::vector<double> hv(2 * blocks);
+    std::vector<int> hr(blocks);
+    if (vr_test) d2h(hv.data(), vr_part.get(), hv.size() * sizeof(double), st);
+    if (refp_test) d2h(hr.data(), rp_part.get(), hr.size() * sizeof(int), st);
+    HFGPU_CHECK(hipStreamSynchronize(st));
+    if (vr_test) {
+        double vmax2 = 0.0, err2 = 0.0;
+        for (unsigned b = 0; b < blocks; ++b) {
+            vmax2 = std::max(vmax2, hv[2 * b]);
+            err2 = std::max(err2, hv[2 * b + 1]);
         }
-    const double tol = 64.0 * 2.220446049250313e-16;
-    return err2 <= tol * tol * vmax2;
+        const double tol = 64.0 * 2.220446049250313e-16;
+        *packable = err2 <= tol * tol * vmax2;
+    }
+    if (refp_test) {
+        bool c = true;
+        for (unsigned b = 0; b < blocks; ++b) c = c && hr[b] == 0;
+        *refp_const = c;
+    }
 }
 
 GpuKernel::GpuKernel(const hf_kernel_desc& d) {
@@ -251,11 +334,11 @@ GpuKernel::GpuKernel(const hf_kernel_desc& d) {
     const size_t nvr = static_cast<size_t>(dv) * dv;
     VR_full.alloc(nk * nvr * cz);
     h2d(VR_full.get(), d.VR, nk * nvr * cz, stream);
-    if (exchange && dv > 1 && nb > 1) dense_packable = dense_vr_packable(d.VR, d.nk1, d.nk2, d.nb);
+    const bool vr_test = exchange && dv > 1 && nb > 1;
+    if (vr_test || has_refP)
+        device_setup_checks(VR_full.as<cd>(), vr_test, has_refP ? refP.as<cd>() : nullptr,
+                            has_refP, nk1, nk2, nb, stream, &dense_packable, &refP_const);
     if (has_refP) {
-        refP_const = true;
-        for (size_t k = 1; k < nk && refP_const; ++k)
-            refP_const = std::memcmp(d.refP, d.refP + 2 * k * nb2, nb2 * cz) == 0;
         if (refP_const) {
             refP0.alloc(nb2 * cz);
             h2d(refP0.get(), d.refP, nb2 * cz, stream);
