@@ -70,6 +70,11 @@ void launch_mu_newton_py(const double* eps, const double* w, int nk, int nb, dou
 //   mode 1 (DM):  p   = expit((mu - eps) / T)
 void launch_occupations(const double* eps, size_t n, const MuState* state, double T, int mode,
                         double* out, hipStream_t st);
+// launch_occupations for S solves of n values each stored one after another,
+// each with its own state (solves whose bit in active is clear are skipped).
+void launch_occupations_batch(const double* eps, size_t n, int S, unsigned active,
+                              const MuState* states, double T, int mode, double* out,
+                              hipStream_t st);
 
 // level shift: out = eps + shift * (1 - expit(-(eps - mu)/(T + 1e-12)))
 void launch_level_shift(const double* eps, size_t n, const MuState* state, double T,

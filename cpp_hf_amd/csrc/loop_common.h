@@ -70,6 +70,11 @@ __device__ __forceinline__ void store_block_partials(double (&v)[NS], double* re
     }
 }
 
+// Is solve blockIdx.y of the batch active?  (uniform over the workgroup)
+__device__ __forceinline__ bool solve_active(unsigned active, int s) {
+    return ((active >> s) & 1u) != 0u;
+}
+
 inline unsigned loop_blocks(int nk, int W) {
     return std::max(1u, std::min<unsigned>(cdiv(nk, W), 1024u));
 }
@@ -86,15 +91,16 @@ inline int waves_for(int nb) { return nb <= 16 ? 4 : 1; }
 // Launch a per-k loop kernel: four wavefronts (k points) per workgroup up to
 // nb = 16 with nb a compile-time constant for 2, 4, 8, 12 and 16 (the index
 // arithmetic and the small matrix products then unroll), one wavefront and a
-// run-time nb above.  LDS is lds_wave bytes per wavefront.
-#define HFGPU_LOOP_LAUNCH(KERNEL, nb, nk, lds_wave, st, ...)                                  \
+// run-time nb above.  LDS is lds_wave bytes per wavefront; S rows of
+// workgroups, one per solve of a batch.
+#define HFGPU_LOOP_LAUNCH(KERNEL, nb, nk, lds_wave, st, S, ...)                               \
     do {                                                                                     \
         const size_t lw_ = (lds_wave);                                                       \
-        const dim3 g4_(loopk::loop_blocks(nk, 4)), b4_(256);                                 \
+        const dim3 g4_(loopk::loop_blocks(nk, 4), (S)), b4_(256);                            \
         switch ((nb) > 16 ? -1 : (nb)) {                                                     \
             case -1:                                                                         \
-                hipLaunchKernelGGL((KERNEL<1, 0>), dim3(loopk::loop_blocks(nk, 1)), dim3(64), \
-                                   lw_, st, __VA_ARGS__);                                    \
+                hipLaunchKernelGGL((KERNEL<1, 0>), dim3(loopk::loop_blocks(nk, 1), (S)),       \
+                                   dim3(64), lw_, st, __VA_ARGS__);                          \
                 break;                                                                       \
             case 2: hipLaunchKernelGGL((KERNEL<4, 2>), g4_, b4_, 4 * lw_, st, __VA_ARGS__); break;   \
             case 4: hipLaunchKernelGGL((KERNEL<4, 4>), g4_, b4_, 4 * lw_, st, __VA_ARGS__); break;   \

@@ -30,10 +30,18 @@ enum ScSlot : int {
 };
 
 struct LoopScratch {
-    double* partial = nullptr;  // >= 1024 * 2 kMaxBt doubles
-    double* sc = nullptr;       // SC_COUNT doubles
+    double* partial = nullptr;  // kLoopPartialDoubles per solve
+    double* sc = nullptr;       // SC_COUNT doubles per solve
+    // A batch of S solves of one shape whose arrays are stored solve after
+    // solve: workgroup row s (blockIdx.y; blockIdx.x of a finalising launch)
+    // works on solve s, whose arrays, scalar slots and partials start s
+    // per-solve strides in.  Solves whose bit in active is clear are
+    // skipped.  S = 1 is a single solve.
+    int S = 1;
+    unsigned active = ~0u;
 };
 constexpr size_t kLoopPartialDoubles = 1024 * 2 * kMaxBt;
+constexpr int kMaxBatch = 32;  // bits of LoopScratch::active
 
 // out[s] = sum over nparts workgroups of partial[b * ns + s] (fixed order).
 void launch_finalize_sums(const double* partial, int nparts, int ns, double* out,

@@ -36,7 +36,14 @@ template <int MODE>
 __global__ void __launch_bounds__(256)
 k_finalize(const double* __restrict__ partial, int nparts, int ns, double* __restrict__ out,
            double* sc, int nbt, double shrink, double T_r, double max_step, int j_lo, int j_hi,
-           int final_stage, int gate) {
+           int final_stage, int gate, unsigned active) {
+    // one workgroup per solve of a batch (its partials, slots and outputs
+    // one per-solve stride apart)
+    const int sv = blockIdx.x;
+    if (!solve_active(active, sv)) return;
+    partial += sv * kLoopPartialDoubles;
+    out += sv * SC_COUNT;
+    if (sc) sc += sv * SC_COUNT;
     // MODE 1 (line search): test trials [j_lo, j_hi) in order (solver_dm.hpp:
     // 575-587); the last stage applies the no-acceptance rule; gate: a
     // trial accepted by an earlier stage stands
@@ -90,11 +97,25 @@ __global__ void __launch_bounds__(W * 64)
 k_orbital_grad(int nk, int nb_arg, const cd* __restrict__ Q, cd* __restrict__ F,
                const double* __restrict__ p, const double* __restrict__ w, const MuState* mus,
                double T_r, int grad, double* __restrict__ eps, cd* __restrict__ G,
-               double* __restrict__ g_p, double* partial, double* sc) {
+               double* __restrict__ g_p, double* partial, double* sc, unsigned active) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     extern __shared__ cd smem[];
     __shared__ double red[W * 2];
     const int nb2 = nb * nb;
+    {
+        const int sv = blockIdx.y;
+        if (!solve_active(active, sv)) return;
+        const size_t od = static_cast<size_t>(sv) * nk * nb2, ob = static_cast<size_t>(sv) * nk * nb;
+        Q += od;
+        F += od;
+        if (p) p += ob;
+        w += static_cast<size_t>(sv) * nk;
+        mus += sv;
+        if (eps) eps += ob;
+        if (G) G += od;
+        if (g_p) g_p += ob;
+        partial += sv * kLoopPartialDoubles;
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     cd* sQ = smem + wave * 3 * nb2;
     cd* sF = sQ + nb2;
@@ -159,10 +180,26 @@ k_precond(int nk, int nb_arg, const cd* __restrict__ G, const cd* __restrict__ G
           const double* __restrict__ eps, const double* __restrict__ p,
           const double* __restrict__ g_p, const double* __restrict__ g_pp,
           const double* __restrict__ w, double T_r, double denom_scale, cd* __restrict__ dQ,
-          double* __restrict__ h_p, double* partial, const double* sc) {
+          double* __restrict__ h_p, double* partial, const double* sc, unsigned active) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     __shared__ double red[W * 6];
     const int nb2 = nb * nb;
+    {
+        const int sv = blockIdx.y;
+        if (!solve_active(active, sv)) return;
+        const size_t od = static_cast<size_t>(sv) * nk * nb2, ob = static_cast<size_t>(sv) * nk * nb;
+        G += od;
+        Gp += od;
+        eps += ob;
+        p += ob;
+        g_p += ob;
+        g_pp += ob;
+        w += static_cast<size_t>(sv) * nk;
+        dQ += od;
+        h_p += ob;
+        partial += sv * kLoopPartialDoubles;
+        sc += sv * SC_COUNT;
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const size_t nkb = static_cast<size_t>(nk) * nb;
     const double eps_scale = sqrt(sc[SC_EPS_SQ] / static_cast<double>(nkb > 0 ? nkb : 1) + TINY_REAL);
@@ -214,10 +251,25 @@ k_cg_combine(int nk, int nb_arg, int restart, cd* __restrict__ dQ, const cd* __r
              const double* __restrict__ h_p, const double* __restrict__ d_pp,
              double* __restrict__ d_p, const double* __restrict__ eps,
              const double* __restrict__ p, const double* __restrict__ w, double T_r,
-             double* partial, double* sc) {
+             double* partial, double* sc, unsigned active) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     __shared__ double red[W * 3];
     const int nb2 = nb * nb;
+    {
+        const int sv = blockIdx.y;
+        if (!solve_active(active, sv)) return;
+        const size_t od = static_cast<size_t>(sv) * nk * nb2, ob = static_cast<size_t>(sv) * nk * nb;
+        dQ += od;
+        dQp += od;
+        h_p += ob;
+        d_pp += ob;
+        d_p += ob;
+        eps += ob;
+        p += ob;
+        w += static_cast<size_t>(sv) * nk;
+        partial += sv * kLoopPartialDoubles;
+        sc += sv * SC_COUNT;
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const double num = ((sc[SC_PR1] - sc[SC_PR1 + 1]) + sc[SC_PR1 + 2]) - sc[SC_PR1 + 3];
     const double den = sc[SC_PR1 + 4] + sc[SC_PR1 + 5];
@@ -287,9 +339,24 @@ k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restr
              const cd* __restrict__ Ft, cd* __restrict__ Fe, const double* __restrict__ p,
              const double* __restrict__ d_p, const double* __restrict__ w, double T_r,
              double max_step, double shrink, int j_lo, int j_hi, int stage2,
-             double* partial, const double* sc) {
+             double* partial, const double* sc, unsigned active) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
-    if (stage2 && sc[SC_ACCEPTED] > 0.5) return;  // uniform over the grid
+    {
+        const int sv = blockIdx.y;
+        if (!solve_active(active, sv)) return;
+        const size_t od = static_cast<size_t>(sv) * nk * nb * nb;
+        const size_t ob = static_cast<size_t>(sv) * nk * nb;
+        V += od;
+        lam += ob;
+        Ft += od;
+        Fe += od;
+        p += ob;
+        d_p += ob;
+        w += static_cast<size_t>(sv) * nk;
+        partial += sv * kLoopPartialDoubles;
+        sc += sv * SC_COUNT;
+    }
+    if (stage2 && sc[SC_ACCEPTED] > 0.5) return;  // uniform over the solve's workgroups
     extern __shared__ cd smem[];
     __shared__ double racc[W][2 * kMaxBt];
     const int nb2 = nb * nb;
@@ -367,8 +434,22 @@ template <int W, int NB>
 __global__ void __launch_bounds__(W * 64)
 k_retract(int nk, int nb_arg, const cd* Q, const cd* __restrict__ V,
           const double* __restrict__ lam, const cd* __restrict__ Fe,
-          double* __restrict__ eps_work, int cayley, const double* sc, cd* Q_out) {
+          double* __restrict__ eps_work, int cayley, const double* sc, cd* Q_out,
+          unsigned active) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
+    {
+        const int sv = blockIdx.y;
+        if (!solve_active(active, sv)) return;
+        const size_t od = static_cast<size_t>(sv) * nk * nb * nb;
+        const size_t ob = static_cast<size_t>(sv) * nk * nb;
+        Q += od;
+        Q_out += od;
+        V += od;
+        if (lam) lam += ob;
+        if (Fe) Fe += od;
+        if (eps_work) eps_work += ob;
+        sc += sv * SC_COUNT;
+    }
     extern __shared__ cd smem[];
     const int nb2 = nb * nb;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -445,8 +526,20 @@ template <int W, int NB>
 __global__ void __launch_bounds__(W * 64)
 k_scf_residuals(int nk, int nb_arg, const cd* __restrict__ F, const cd* __restrict__ D,
                 const cd* __restrict__ Pn, cd* __restrict__ delta, cd* __restrict__ comm,
-                const double* __restrict__ w, double* partial) {
+                const double* __restrict__ w, double* partial, unsigned active) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
+    {
+        const int sv = blockIdx.y;
+        if (!solve_active(active, sv)) return;
+        const size_t od = static_cast<size_t>(sv) * nk * nb * nb;
+        F += od;
+        D += od;
+        Pn += od;
+        delta += od;
+        if (comm) comm += od;
+        w += static_cast<size_t>(sv) * nk;
+        partial += sv * kLoopPartialDoubles;
+    }
     extern __shared__ cd smem[];
     __shared__ double red[W * 2];
     const int nb2 = nb * nb;
@@ -571,12 +664,15 @@ k_lincomb(size_t n, CdPtrs X, DotCoefs c, int nx, int accumulate, double a, doub
 void launch_finalize_sums(const double* partial, int nparts, int ns, double* out,
                           hipStream_t st) {
     hipLaunchKernelGGL(k_finalize<0>, dim3(1), dim3(256), 0, st, partial, nparts, ns, out,
-                       nullptr, 0, 0.0, 0.0, 0.0, 0, 0, 0, 0);
+                       nullptr, 0, 0.0, 0.0, 0.0, 0, 0, 0, 0, ~0u);
     HFGPU_LAUNCH_CHECK();
 }
 
+// per solve of the scratch's batch: out is solve 0's slot
 static void finalize(LoopScratch& ls, int nparts, int ns, double* out, hipStream_t st) {
-    launch_finalize_sums(ls.partial, nparts, ns, out, st);
+    hipLaunchKernelGGL(k_finalize<0>, dim3(ls.S), dim3(256), 0, st, ls.partial, nparts, ns, out,
+                       nullptr, 0, 0.0, 0.0, 0.0, 0, 0, 0, 0, ls.active);
+    HFGPU_LAUNCH_CHECK();
 }
 
 void launch_orbital_grad(int nk, int nb, const cd* Q, cd* F, const double* p, const double* w,
@@ -584,8 +680,8 @@ void launch_orbital_grad(int nk, int nb, const cd* Q, cd* F, const double* p, co
                          double* g_p, LoopScratch& ls, hipStream_t st) {
     check_nb(nb);
     const size_t per_wave = sizeof(cd) * 3 * nb * nb;
-    HFGPU_LOOP_LAUNCH(k_orbital_grad, nb, nk, per_wave, st, nk, nb, Q, F, p, w, mus, T_r,
-                      grad ? 1 : 0, eps, G, g_p, ls.partial, ls.sc);
+    HFGPU_LOOP_LAUNCH(k_orbital_grad, nb, nk, per_wave, st, ls.S, nk, nb, Q, F, p, w, mus, T_r,
+                      grad ? 1 : 0, eps, G, g_p, ls.partial, ls.sc, ls.active);
     HFGPU_LAUNCH_CHECK();
     if (grad) finalize(ls, loop_blocks(nk, waves_for(nb)), 2, ls.sc + SC_GN2, st);
 }
@@ -595,8 +691,8 @@ void launch_precond(int nk, int nb, const cd* G, const cd* Gp, const double* eps
                     double T_r, double denom_scale, cd* dQ, double* h_p, LoopScratch& ls,
                     hipStream_t st) {
     check_nb(nb);
-    HFGPU_LOOP_LAUNCH(k_precond, nb, nk, 0, st, nk, nb, G, Gp, eps, p, g_p, g_pp, w, T_r,
-                      denom_scale, dQ, h_p, ls.partial, ls.sc);
+    HFGPU_LOOP_LAUNCH(k_precond, nb, nk, 0, st, ls.S, nk, nb, G, Gp, eps, p, g_p, g_pp, w, T_r,
+                      denom_scale, dQ, h_p, ls.partial, ls.sc, ls.active);
     HFGPU_LAUNCH_CHECK();
     finalize(ls, loop_blocks(nk, waves_for(nb)), 6, ls.sc + SC_PR1, st);
 }
@@ -605,8 +701,8 @@ void launch_cg_combine(int nk, int nb, bool restart, cd* dQ, const cd* dQp, cons
                        const double* d_pp, double* d_p, const double* eps, const double* p,
                        const double* w, double T_r, LoopScratch& ls, hipStream_t st) {
     check_nb(nb);
-    HFGPU_LOOP_LAUNCH(k_cg_combine, nb, nk, 0, st, nk, nb, restart ? 1 : 0, dQ, dQp, h_p, d_pp,
-                      d_p, eps, p, w, T_r, ls.partial, ls.sc);
+    HFGPU_LOOP_LAUNCH(k_cg_combine, nb, nk, 0, st, ls.S, nk, nb, restart ? 1 : 0, dQ, dQp, h_p,
+                      d_pp, d_p, eps, p, w, T_r, ls.partial, ls.sc, ls.active);
     HFGPU_LAUNCH_CHECK();
     finalize(ls, loop_blocks(nk, waves_for(nb)), 3, ls.sc + SC_DQN2, st);
 }
@@ -628,12 +724,13 @@ void launch_linesearch(int nk, int nb, const cd* V, const double* lam, const cd*
     for (int stage = 0; stage < (nbt > 1 ? 2 : 1); ++stage) {
         const int j_lo = stage == 0 ? 0 : 1, j_hi = stage == 0 ? first : nbt;
         const int last = stage == 1 || nbt <= 1;
-        HFGPU_LOOP_LAUNCH(k_linesearch, nb, nk, per_wave, st, nk, nb, V, lam, Ft, Fe, p, d_p, w,
-                          T_r, max_step, shrink, j_lo, j_hi, stage, ls.partial, ls.sc);
+        HFGPU_LOOP_LAUNCH(k_linesearch, nb, nk, per_wave, st, ls.S, nk, nb, V, lam, Ft, Fe, p,
+                          d_p, w, T_r, max_step, shrink, j_lo, j_hi, stage, ls.partial, ls.sc,
+                          ls.active);
         HFGPU_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(256), 0, st, ls.partial, nparts,
+        hipLaunchKernelGGL(k_finalize<1>, dim3(ls.S), dim3(256), 0, st, ls.partial, nparts,
                            2 * kMaxBt, ls.sc + SC_TRIAL, ls.sc, nbt, shrink, T_r, max_step, j_lo,
-                           j_hi, last, stage);
+                           j_hi, last, stage, ls.active);
         HFGPU_LAUNCH_CHECK();
     }
 }
@@ -642,8 +739,8 @@ void launch_retract(int nk, int nb, cd* Q, const cd* V, const double* lam, const
                     double* eps_work, bool cayley, LoopScratch& ls, hipStream_t st, cd* Q_out) {
     check_nb(nb);
     const size_t per_wave = ls_wave_bytes(nb);
-    HFGPU_LOOP_LAUNCH(k_retract, nb, nk, per_wave, st, nk, nb, Q, V, lam, Fe, eps_work,
-                      cayley ? 1 : 0, ls.sc, Q_out ? Q_out : Q);
+    HFGPU_LOOP_LAUNCH(k_retract, nb, nk, per_wave, st, ls.S, nk, nb, Q, V, lam, Fe, eps_work,
+                      cayley ? 1 : 0, ls.sc, Q_out ? Q_out : Q, ls.active);
     HFGPU_LAUNCH_CHECK();
 }
 
@@ -652,8 +749,8 @@ void launch_scf_residuals(int nk, int nb, const cd* F, const cd* D, const cd* Pn
                           hipStream_t st) {
     check_nb(nb);
     const size_t per_wave = sizeof(cd) * 2 * nb * nb;
-    HFGPU_LOOP_LAUNCH(k_scf_residuals, nb, nk, per_wave, st, nk, nb, F, D, Pn, delta, comm, w,
-                      ls.partial);
+    HFGPU_LOOP_LAUNCH(k_scf_residuals, nb, nk, per_wave, st, ls.S, nk, nb, F, D, Pn, delta,
+                      comm, w, ls.partial, ls.active);
     HFGPU_LAUNCH_CHECK();
     finalize(ls, loop_blocks(nk, waves_for(nb)), 2, out, st);
 }

@@ -728,10 +728,16 @@ k_mu_newton_py(const double* __restrict__ eps, const double* __restrict__ w, int
 
 // ---- occupations -----------------------------------------------------------
 
+// Row s = blockIdx.y of the grid: solve s of a batch (n values each, its
+// own state), skipped when its bit in active is clear.
 __global__ void k_occupations(const double* __restrict__ eps, size_t n,
                               const MuState* __restrict__ state, double T, int mode,
-                              double* __restrict__ out) {
-    const double mu = state->mu;
+                              double* __restrict__ out, unsigned active) {
+    const int sv = blockIdx.y;
+    if (!((active >> sv) & 1u)) return;
+    eps += sv * n;
+    out += sv * n;
+    const double mu = state[sv].mu;
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * blockDim.x) {
         out[i] = mode == 0 ? expit_d(-(eps[i] - mu) / (T + 1.0e-12)) : expit_d((mu - eps[i]) / T);
@@ -899,7 +905,16 @@ void launch_mu_newton_py(const double* eps, const double* w, int nk, int nb, dou
 void launch_occupations(const double* eps, size_t n, const MuState* state, double T, int mode,
                         double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_occupations, dim3(std::min<unsigned>(cdiv(n, 256), 2048u)), dim3(256),
-                       0, st, eps, n, state, T, mode, out);
+                       0, st, eps, n, state, T, mode, out, ~0u);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_occupations_batch(const double* eps, size_t n, int S, unsigned active,
+                              const MuState* states, double T, int mode, double* out,
+                              hipStream_t st) {
+    hipLaunchKernelGGL(k_occupations,
+                       dim3(std::min<unsigned>(cdiv(n, 256), 2048u), static_cast<unsigned>(S)),
+                       dim3(256), 0, st, eps, n, states, T, mode, out, active);
     HFGPU_LAUNCH_CHECK();
 }
 

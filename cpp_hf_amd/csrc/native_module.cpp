@@ -681,6 +681,97 @@ PYBIND11_MODULE(_native, m) {
           py::arg("trust_radius"), py::arg("return_density"), py::arg("return_fock"),
           py::arg("exchange_fp32") = false, py::arg("fp32_switch") = 1e-6);
 
+    // --- Several direct-minimisation solves of one shape at once (no
+    // counterpart in the reference): lists of kernel_args, P0 and n_e, one
+    // configuration; a list of solve_dm's ten-entry tuples ---
+    m.def("solve_dm_batch", [](py::list kernel_args, py::list P0s, std::vector<double> n_es,
+                               size_t max_iter, double tol_E, double tol_grad, double max_step,
+                               double bt_shrink, double denom_scale, size_t bt_max,
+                               size_t cg_restart, size_t plateau_window, int mu_maxiter,
+                               bool return_Q, bool return_density, bool return_fock) {
+        const size_t n = kernel_args.size();
+        if (n == 0 || P0s.size() != n || n_es.size() != n)
+            throw py::value_error("kernel_args, P0s and n_es must have one equal, nonzero length");
+        std::vector<std::unique_ptr<KernelArgs>> kas;
+        std::vector<std::unique_ptr<Handle>> hs;
+        std::vector<CArr> p0;
+        for (size_t i = 0; i < n; ++i) {
+            kas.push_back(std::make_unique<KernelArgs>(kernel_args[i]));
+            const auto& d = kas.back()->d;
+            CArr a = py::cast<CArr>(P0s[i]);
+            if (a.ndim() != 4 || (size_t)a.shape(0) != d.nk1 || (size_t)a.shape(1) != d.nk2 ||
+                (size_t)a.shape(2) != d.nb || (size_t)a.shape(3) != d.nb)
+                throw py::value_error("P0 must match h shape");
+            p0.push_back(a);
+        }
+        for (size_t i = 0; i < n; ++i) hs.push_back(std::make_unique<Handle>(kas[i]->d));
+        hf_dm_config cfg{};
+        cfg.max_iter = max_iter;
+        cfg.tol_E = tol_E;
+        cfg.tol_grad = tol_grad;
+        cfg.plateau_window = plateau_window;
+        cfg.denom_scale = denom_scale;
+        cfg.max_step = max_step;
+        cfg.cg_restart = cg_restart;
+        cfg.bt_shrink = bt_shrink;
+        cfg.bt_max = bt_max;
+        cfg.mu_maxiter = mu_maxiter;
+        const py::ssize_t mh = (py::ssize_t)std::max<size_t>(max_iter, 1);
+        std::vector<py::array_t<std::complex<double>>> Qs, Ds, Fs;
+        std::vector<py::array_t<double>> ps, hEs, hGs;
+        std::vector<hf_dm_result> res(n);
+        std::vector<hfgpu_kernel*> ks(n);
+        std::vector<const double*> P0p(n);
+        for (size_t i = 0; i < n; ++i) {
+            const auto& d = kas[i]->d;
+            std::vector<py::ssize_t> shape{(py::ssize_t)d.nk1, (py::ssize_t)d.nk2,
+                                           (py::ssize_t)d.nb, (py::ssize_t)d.nb};
+            std::vector<py::ssize_t> pshape{(py::ssize_t)d.nk1, (py::ssize_t)d.nk2,
+                                            (py::ssize_t)d.nb};
+            Qs.push_back(new_c(shape));
+            Ds.push_back(new_c(shape));
+            Fs.push_back(new_c(shape));
+            ps.push_back(new_r(pshape));
+            hEs.push_back(new_r({mh}));
+            hGs.push_back(new_r({mh}));
+            py::array qa = Qs.back(), da = Ds.back(), fa = Fs.back(), pa = ps.back(),
+                      ea = hEs.back(), ga = hGs.back();
+            res[i] = hf_dm_result{};
+            res[i].Q = return_Q ? dp(qa) : nullptr;
+            res[i].p = dp(pa);
+            res[i].density = return_density ? dp(da) : nullptr;
+            res[i].fock = return_fock ? dp(fa) : nullptr;
+            res[i].hist_E = dp(ea);
+            res[i].hist_grad = dp(ga);
+            ks[i] = hs[i]->k;
+            P0p[i] = cdp(p0[i]);
+        }
+        int rc;
+        {
+            py::gil_scoped_release rel;
+            rc = hfgpu_solve_dm_batch(ks.data(), n, P0p.data(), n_es.data(), &cfg, res.data());
+        }
+        check(rc);
+        py::list out;
+        for (size_t i = 0; i < n; ++i) {
+            const py::ssize_t it = (py::ssize_t)res[i].n_iter;
+            py::object q_obj = py::none(), d_obj = py::none(), f_obj = py::none();
+            if (return_Q) q_obj = Qs[i];
+            if (return_density) d_obj = Ds[i];
+            if (return_fock) f_obj = Fs[i];
+            out.append(py::make_tuple(q_obj, ps[i], d_obj, f_obj, res[i].mu, res[i].energy,
+                                      res[i].n_iter, (bool)res[i].converged,
+                                      hEs[i][py::slice(0, it, 1)].attr("copy")(),
+                                      hGs[i][py::slice(0, it, 1)].attr("copy")()));
+        }
+        return out;
+    },
+          py::arg("kernel_args"), py::arg("P0s"), py::arg("n_es"), py::arg("max_iter"),
+          py::arg("tol_E"), py::arg("tol_grad"), py::arg("max_step"), py::arg("bt_shrink"),
+          py::arg("denom_scale"), py::arg("bt_max"), py::arg("cg_restart"),
+          py::arg("plateau_window"), py::arg("mu_maxiter"), py::arg("return_Q"),
+          py::arg("return_density"), py::arg("return_fock"));
+
     // --- Direct minimisation (cpp_hf_native.cpp:488-582): optimizer 1 is the
     // trust-region Newton solver (with deflation), any other code the CG loop ---
     m.def("solve_dm", [](py::object kernel_args, CArr P0, double n_e, size_t max_iter,

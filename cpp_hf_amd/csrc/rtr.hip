@@ -376,7 +376,7 @@ void launch_rtr_direction(int nk, int nb, bool init, double beta, double s_z, do
                           hipStream_t st) {
     check_nb(nb);
     const size_t per_wave = sizeof(cd) * 3 * nb * nb;
-    HFGPU_LOOP_LAUNCH(k_rtr_direction, nb, nk, per_wave, st, nk, nb, init ? 1 : 0, beta, s_z,
+    HFGPU_LOOP_LAUNCH(k_rtr_direction, nb, nk, per_wave, st, 1, nk, nb, init ? 1 : 0, beta, s_z,
                       lam_pc, T_r, Q, p, eps, rX, rp, dX, dp, dP);
     HFGPU_LAUNCH_CHECK();
 }
@@ -387,7 +387,7 @@ void launch_rtr_hvp(int nk, int nb, double T_r, double pen, const cd* Q, const d
                     double* Hp_raw, LoopScratch& ls, double* out, hipStream_t st) {
     check_nb(nb);
     const size_t per_wave = sizeof(cd) * 5 * nb * nb;
-    HFGPU_LOOP_LAUNCH(k_rtr_hvp, nb, nk, per_wave, st, nk, nb, T_r, pen, Q, p, Ft, dX, dp, Fr,
+    HFGPU_LOOP_LAUNCH(k_rtr_hvp, nb, nk, per_wave, st, 1, nk, nb, T_r, pen, Q, p, Ft, dX, dp, Fr,
                       h, dP, vX, vp, w, HdX, Hp_raw, ls.partial);
     HFGPU_LAUNCH_CHECK();
     launch_finalize_sums(ls.partial, loop_blocks(nk, waves_for(nb)), RH_COUNT, out, st);
@@ -400,7 +400,7 @@ void launch_rtr_update(int nk, int nb, int mode, double a, double s_H, double la
                        double* rp, LoopScratch& ls, double* out, hipStream_t st) {
     check_nb(nb);
     if (mode < 0 || mode > 2) throw std::invalid_argument("rtr update mode must be 0, 1 or 2");
-    HFGPU_LOOP_LAUNCH(k_rtr_update, nb, nk, 0, st, nk, nb, mode, a, s_H, lam_pc, T_r, eps, p, G,
+    HFGPU_LOOP_LAUNCH(k_rtr_update, nb, nk, 0, st, 1, nk, nb, mode, a, s_H, lam_pc, T_r, eps, p, G,
                       w, dX, dp, HdX, Hp_raw, vX, vp, HvX, Hvp, rX, rp, ls.partial);
     HFGPU_LAUNCH_CHECK();
     launch_finalize_sums(ls.partial, loop_blocks(nk, waves_for(nb)), RU_COUNT, out, st);
@@ -410,7 +410,7 @@ void launch_rtr_retract(int nk, int nb, double tau, const cd* V, const double* l
                         const cd* Q, const cd* Ft, cd* Qt, double* eps_t, hipStream_t st) {
     check_nb(nb);
     const size_t per_wave = sizeof(cd) * (4 * nb * nb + nb);
-    HFGPU_LOOP_LAUNCH(k_rtr_retract, nb, nk, per_wave, st, nk, nb, tau, V, lam, Q, Ft, Qt,
+    HFGPU_LOOP_LAUNCH(k_rtr_retract, nb, nk, per_wave, st, 1, nk, nb, tau, V, lam, Q, Ft, Qt,
                       eps_t);
     HFGPU_LAUNCH_CHECK();
 }

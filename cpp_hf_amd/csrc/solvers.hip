@@ -952,6 +952,284 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
 }
 
 // ---------------------------------------------------------------------------
+// Direct minimisation of several solves of one shape at once
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Pinned host scalars for a batch: grows per host thread, kept for the
+// thread's lifetime (as pinned_scalars).
+double* pinned_batch_scalars(size_t doubles) {
+    thread_local double* p = nullptr;
+    thread_local size_t cap = 0;
+    if (doubles > cap) {
+        void* q = nullptr;
+        const size_t want = std::max<size_t>(doubles, 4096);
+        HFGPU_CHECK(hipHostMalloc(&q, want * sizeof(double), hipHostMallocDefault));
+        p = static_cast<double*>(q);  // the previous block stays allocated
+        cap = want;
+    }
+    return p;
+}
+
+}  // namespace
+
+// S independent direct-minimisation solves (one Fock kernel each) on one
+// mesh, band count and temperature, advanced together: every per-k kernel of
+// an iteration (density, orbital gradient, preconditioner, CG combination,
+// eigh of the direction, line search, retraction, occupations) is one launch
+// over all S solves, with workgroup row s working on solve s and its own
+// scalar slots and partials; the Fock builds, energies and chemical
+// potentials are one launch sequence per solve.  Each solve keeps the
+// reference's stop test (solver_dm.hpp:476-490) on its own history: a solve
+// that stops is masked out of the launches that change its state
+// (retraction, occupations, chemical potential), so its orbitals stay those
+// of its last iteration, and the finalisation runs for all solves at the
+// end.  Every per-solve value is computed by the same kernels with the same
+// workgroup decomposition as a single solve, so each result equals
+// solve_dm_gpu's bit for bit.  Iteration k + 1 is enqueued before the
+// scalars of iteration k are read back, as in solve_dm_gpu.
+void solve_dm_batch_gpu(const std::vector<GpuKernel*>& Ks, const std::vector<const double*>& P0s,
+                        const std::vector<double>& n_es, const hf_dm_config& cfg,
+                        hf_dm_result* outs) {
+    const int S = static_cast<int>(Ks.size());
+    if (S < 1 || S > kMaxBatch) throw std::invalid_argument("batch of 1..32 solves");
+    GpuKernel& K0 = *Ks[0];
+    for (GpuKernel* Kp : Ks) {
+        const GpuKernel& K = *Kp;
+        if (K.nk1 != K0.nk1 || K.nk2 != K0.nk2 || K.nb != K0.nb || K.T != K0.T)
+            throw std::invalid_argument(
+                "a batch needs one mesh, band count and temperature for every kernel");
+    }
+    if (cfg.n_blocks > 0 || cfg.symmetry || cfg.exchange_fp32 || cfg.optimizer == 1 ||
+        cfg.n_deflation > 0)
+        throw std::invalid_argument(
+            "batched solves take the CG optimizer without blocks, symmetry, deflation or FP32 "
+            "builds");
+    if (cfg.bt_max > static_cast<size_t>(kMaxBt))
+        throw std::invalid_argument("bt_max above the GPU line-search limit of 16");
+    const int nk = static_cast<int>(K0.nk), nb = K0.nb;
+    const size_t nd = K0.nd, nkb = K0.nk * K0.nb, nkk = K0.nk;
+    const size_t bytes = nd * sizeof(cd), rb = nkb * sizeof(double);
+    const size_t SS = static_cast<size_t>(S);
+    const double T_r = std::max(K0.T, 1.0e-12);
+    hipStream_t st = K0.stream;
+
+    std::vector<double> n_target(SS);
+    std::vector<double> wn(SS * nkk);
+    for (int s = 0; s < S; ++s) {
+        const GpuKernel& K = *Ks[s];
+        validate_electron_count(K.w_host, static_cast<size_t>(nb), n_es[s]);
+        const double wsum = std::max(K.weight_sum, TINY_REAL);
+        for (size_t i = 0; i < nkk; ++i) wn[s * nkk + i] = K.w_host[i] / wsum;
+        n_target[s] = n_es[s] / wsum;
+    }
+    DevBuf w_norm(SS * nkk * sizeof(double));
+    h2d(w_norm.get(), wn.data(), SS * nkk * sizeof(double), st);
+    const double* wd = w_norm.as<double>();
+
+    DevBuf Q(SS * bytes), P(SS * bytes), Sigma(SS * bytes), F(SS * bytes), G(SS * bytes),
+        Gp(SS * bytes), dQ(SS * bytes), dQp(SS * bytes), Vd(SS * bytes), Q_alt(SS * bytes);
+    DevBuf p(SS * rb), p_alt(SS * rb), eps(SS * rb), g_p(SS * rb), g_pp(SS * rb), h_p(SS * rb),
+        d_p(SS * rb), d_pp(SS * rb), lam(SS * rb), eps_w(SS * rb),
+        hdiag(SS * std::max(nb, 1) * sizeof(double));
+    DevBuf mus(SS * sizeof(MuState)), mus_alt(SS * sizeof(MuState)), mu_ws(kMuPartialDoubles * sizeof(double));
+    DevBuf part(SS * kLoopPartialDoubles * sizeof(double));
+    DevBuf scb[2] = {DevBuf(SS * SC_COUNT * sizeof(double)), DevBuf(SS * SC_COUNT * sizeof(double))};
+    fill_zero(Gp.get(), SS * bytes, st);
+    fill_zero(dQp.get(), SS * bytes, st);
+    fill_zero(g_pp.get(), SS * rb, st);
+    fill_zero(d_pp.get(), SS * rb, st);
+    fill_zero(mus.get(), SS * sizeof(MuState), st);
+    fill_zero(mus_alt.get(), SS * sizeof(MuState), st);
+    fill_zero(scb[0].get(), SS * SC_COUNT * sizeof(double), st);
+    fill_zero(scb[1].get(), SS * SC_COUNT * sizeof(double), st);
+
+    auto Pd = [&](DevBuf& b, int s) { return b.as<cd>() + static_cast<size_t>(s) * nd; };
+    // the per-solve launch sequences (Fock build and energy, chemical
+    // potential), one solve after another on the batch stream: running them
+    // on forked per-solve streams joined by events measured slower (the
+    // 48-kernel sweep at 60 against 90-100 solves/s)
+    auto mu_solve = [&](const double* e, MuState* out, const MuState* warm, bool use_warm,
+                        unsigned act) {
+        for (int s = 0; s < S; ++s)
+            if ((act >> s) & 1u)
+                launch_mu_newton(e + static_cast<size_t>(s) * nkb, wd + static_cast<size_t>(s) * nkk,
+                                 nk, nb, n_target[s], 0.0, use_warm, T_r, cfg.mu_maxiter, out + s,
+                                 mu_ws.as<double>(), st, warm + s);
+    };
+    auto builds = [&](unsigned act, bool energy, double* sc) {
+        for (int s = 0; s < S; ++s) {
+            if (!((act >> s) & 1u)) continue;
+            GpuKernel& K = *Ks[s];
+            double* hd = hdiag.as<double>() + static_cast<size_t>(s) * nb;
+            K.build_fock(Pd(P, s), Pd(Sigma, s), Pd(F, s), hd, true, st, true, false);
+            if (energy) K.energy(Pd(P, s), Pd(Sigma, s), hd, sc + s * SC_COUNT + SC_E, st);
+        }
+    };
+    const unsigned all = S == 32 ? ~0u : ((1u << S) - 1u);
+    LoopScratch L;
+    L.partial = part.as<double>();
+    L.S = S;
+
+    // ---- init (solver_dm.hpp:436-459), every solve
+    for (int s = 0; s < S; ++s) h2d(Pd(P, s), P0s[s], bytes, st);
+    launch_hermitize(S * nk, nb, P.as<cd>(), st);
+    builds(all, false, nullptr);
+    launch_eigh(S * nk, nb, 0, F.as<cd>(), eps.as<double>(), Q.as<cd>(), st);
+    mu_solve(eps.as<double>(), mus.as<MuState>(), mus.as<MuState>(), false, all);
+    launch_occupations_batch(eps.as<double>(), nkb, S, all, mus.as<MuState>(), T_r, 1,
+                             p.as<double>(), st);
+
+    // per-solve host state
+    std::vector<std::vector<double>> hist_E(SS), hist_G(SS);
+    std::vector<size_t> ls_trials(SS, 0), k_s(SS, 0);
+    std::vector<double> grad_norm(SS, std::numeric_limits<double>::infinity());
+    std::vector<double> E_prev(SS, std::numeric_limits<double>::infinity());
+    std::vector<double> dE(SS, std::numeric_limits<double>::infinity());
+    auto keep_going = [&](int s, size_t k) {
+        bool e_nc;
+        const std::vector<double>& h = hist_E[s];
+        if (cfg.plateau_window > 0 && k > cfg.plateau_window)
+            e_nc = (h[k - 1 - cfg.plateau_window] - h[k - 1]) > cfg.tol_E;
+        else
+            e_nc = dE[s] > cfg.tol_E;
+        const bool g_nc = (cfg.tol_grad > 0.0) && (grad_norm[s] > cfg.tol_grad);
+        return e_nc || g_nc;
+    };
+    auto record = [&](int s, const double* h) {
+        const double E = h[SC_E];
+        grad_norm[s] = std::sqrt(std::max(0.0, h[SC_GN2]));
+        hist_E[s].push_back(E);
+        hist_G[s].push_back(grad_norm[s]);
+        if (h[SC_ACCEPTED] > 0.5 && h[SC_TAU0] > 0.0 && cfg.bt_shrink > 0.0 &&
+            cfg.bt_shrink < 1.0) {
+            const double j = std::log(h[SC_TAU] / h[SC_TAU0]) / std::log(cfg.bt_shrink);
+            ls_trials[s] += static_cast<size_t>(std::llround(std::max(0.0, j))) + 1;
+        } else {
+            ls_trials[s] += cfg.bt_max;
+        }
+        dE[s] = std::fabs(E - E_prev[s]);
+        E_prev[s] = E;
+    };
+
+    cd* const Qs[2] = {Q.as<cd>(), Q_alt.as<cd>()};
+    double* const ps[2] = {p.as<double>(), p_alt.as<double>()};
+    MuState* const ms[2] = {mus.as<MuState>(), mus_alt.as<MuState>()};
+    double* const hs = pinned_batch_scalars(2 * SS * SC_COUNT);
+    StreamEvent ev[2];
+    ev[0].create();
+    ev[1].create();
+    // iteration j over the solves in act: reads state j & 1, writes (j + 1) & 1
+    auto enqueue = [&](size_t j, unsigned act) {
+        const bool restart = (j == 0) || (cfg.cg_restart > 0 && j % cfg.cg_restart == 0);
+        const int s0 = static_cast<int>(j & 1), s1 = s0 ^ 1;
+        L.sc = scb[s0].as<double>();
+        L.active = act;
+        launch_density(S * nk, nb, Qs[s0], ps[s0], P.as<cd>(), true, st);
+        builds(act, true, L.sc);
+        launch_orbital_grad(nk, nb, Qs[s0], F.as<cd>(), ps[s0], wd, ms[s0], T_r, true,
+                            eps.as<double>(), G.as<cd>(), g_p.as<double>(), L, st);
+        launch_precond(nk, nb, G.as<cd>(), Gp.as<cd>(), eps.as<double>(), ps[s0],
+                       g_p.as<double>(), g_pp.as<double>(), wd, T_r, cfg.denom_scale, dQ.as<cd>(),
+                       h_p.as<double>(), L, st);
+        launch_cg_combine(nk, nb, restart, dQ.as<cd>(), dQp.as<cd>(), h_p.as<double>(),
+                          d_pp.as<double>(), d_p.as<double>(), eps.as<double>(), ps[s0], wd, T_r,
+                          L, st);
+        launch_eigh(S * nk, nb, 1, dQ.as<cd>(), lam.as<double>(), Vd.as<cd>(), st);
+        launch_linesearch(nk, nb, Vd.as<cd>(), lam.as<double>(), F.as<cd>(), Sigma.as<cd>(), ps[s0],
+                          d_p.as<double>(), wd, T_r, cfg.max_step, cfg.bt_shrink,
+                          static_cast<int>(cfg.bt_max), L, st);
+        launch_retract(nk, nb, Qs[s0], Vd.as<cd>(), lam.as<double>(), Sigma.as<cd>(),
+                       eps_w.as<double>(), true, L, st, Qs[s1]);
+        mu_solve(eps_w.as<double>(), ms[s1], ms[s0], true, act);
+        launch_occupations_batch(eps_w.as<double>(), nkb, S, act, ms[s1], T_r, 1, ps[s1], st);
+        d2h(hs + static_cast<size_t>(s0) * SS * SC_COUNT, L.sc, SS * SC_COUNT * sizeof(double),
+            st);
+        HFGPU_CHECK(hipEventRecord(ev[s0].e, st));
+        std::swap(dQ, dQp);
+        std::swap(d_p, d_pp);
+        std::swap(G, Gp);
+        std::swap(g_p, g_pp);
+    };
+
+    unsigned act = 0;
+    for (int s = 0; s < S; ++s)
+        if (cfg.max_iter > 0 && keep_going(s, 0)) act |= 1u << s;
+    size_t k = 0;
+    if (act) {
+        enqueue(0, act);
+        for (;;) {
+            const bool ahead = k + 1 < cfg.max_iter;
+            // the solves active at the top of k run iteration k + 1 at once;
+            // those that stop at its top are masked from k + 2 on
+            if (ahead) enqueue(k + 1, act);
+            HFGPU_CHECK(hipEventSynchronize(ev[k & 1].e));
+            const double* h = hs + static_cast<size_t>(k & 1) * SS * SC_COUNT;
+            for (int s = 0; s < S; ++s) {
+                if (!((act >> s) & 1u)) continue;
+                record(s, h + static_cast<size_t>(s) * SC_COUNT);
+                k_s[s] = k + 1;
+                if (!ahead || !keep_going(s, k + 1)) act &= ~(1u << s);
+            }
+            ++k;
+            if (!ahead || !act) break;
+        }
+    }
+
+    // each solve's state after its k_s iterations is state k_s & 1: gather
+    // the odd ones into buffer 0
+    for (int s = 0; s < S; ++s) {
+        if (!(k_s[s] & 1)) continue;
+        d2d(Q.as<cd>() + static_cast<size_t>(s) * nd, Q_alt.as<cd>() + static_cast<size_t>(s) * nd,
+            bytes, st);
+        d2d(p.as<double>() + static_cast<size_t>(s) * nkb,
+            p_alt.as<double>() + static_cast<size_t>(s) * nkb, rb, st);
+        d2d(mus.as<MuState>() + s, mus_alt.as<MuState>() + s, sizeof(MuState), st);
+    }
+
+    // ---- finalise (solver_dm.hpp:615-652), every solve
+    L.sc = scb[0].as<double>();
+    L.active = all;
+    launch_density(S * nk, nb, Q.as<cd>(), p.as<double>(), P.as<cd>(), true, st);
+    builds(all, false, nullptr);
+    launch_orbital_grad(nk, nb, Q.as<cd>(), F.as<cd>(), nullptr, wd, mus.as<MuState>(), T_r,
+                        false, nullptr, nullptr, nullptr, L, st);
+    launch_eigh(S * nk, nb, 0, F.as<cd>(), eps_w.as<double>(), Vd.as<cd>(), st);
+    launch_retract(nk, nb, Q.as<cd>(), Vd.as<cd>(), nullptr, nullptr, nullptr, false, L, st);
+    mu_solve(eps_w.as<double>(), mus.as<MuState>(), mus.as<MuState>(), true, all);
+    launch_occupations_batch(eps_w.as<double>(), nkb, S, all, mus.as<MuState>(), T_r, 1,
+                             p.as<double>(), st);
+    launch_density(S * nk, nb, Q.as<cd>(), p.as<double>(), P.as<cd>(), true, st);
+    builds(all, true, L.sc);
+
+    std::vector<double> hsc(SS * SC_COUNT);
+    std::vector<MuState> hmu(SS);
+    d2h(hsc.data(), L.sc, SS * SC_COUNT * sizeof(double), st);
+    d2h(hmu.data(), mus.get(), SS * sizeof(MuState), st);
+    for (int s = 0; s < S; ++s) {
+        hf_dm_result* out = outs + s;
+        if (out->Q) d2h(out->Q, Q.as<cd>() + static_cast<size_t>(s) * nd, bytes, st);
+        if (out->p) d2h(out->p, p.as<double>() + static_cast<size_t>(s) * nkb, rb, st);
+        if (out->density) d2h(out->density, P.as<cd>() + static_cast<size_t>(s) * nd, bytes, st);
+        if (out->fock) d2h(out->fock, F.as<cd>() + static_cast<size_t>(s) * nd, bytes, st);
+    }
+    HFGPU_CHECK(hipStreamSynchronize(st));
+    for (int s = 0; s < S; ++s) {
+        hf_dm_result* out = outs + s;
+        out->mu = hmu[s].mu;
+        out->energy = hsc[static_cast<size_t>(s) * SC_COUNT + SC_E];
+        out->n_iter = k_s[s];
+        out->converged = k_s[s] < cfg.max_iter ? 1 : 0;
+        out->n_ls_trials = ls_trials[s];
+        for (size_t i = 0; i < k_s[s]; ++i) {
+            if (out->hist_E) out->hist_E[i] = hist_E[s][i];
+            if (out->hist_grad) out->hist_grad[i] = hist_G[s][i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Trust-region Newton (solver_rtr.hpp:162-459)
 // ---------------------------------------------------------------------------
 
@@ -1260,6 +1538,25 @@ int hfgpu_solve_dm(hfgpu_kernel* k, const double* P0, double n_e, const hf_dm_co
             solve_rtr_gpu(K, P0, n_e, *cfg, pr, out);
         else
             solve_dm_gpu(K, P0, n_e, *cfg, pr, out);
+    });
+}
+
+int hfgpu_solve_dm_batch(hfgpu_kernel* const* ks, size_t n, const double* const* P0s,
+                         const double* n_es, const hf_dm_config* cfg, hf_dm_result* outs) {
+    return guarded([&] {
+        if (!ks || !P0s || !n_es || !cfg || !outs) throw std::invalid_argument("null argument");
+        if (n < 1 || n > static_cast<size_t>(kMaxBatch))
+            throw std::invalid_argument("hfgpu_solve_dm_batch takes 1 to 32 solves");
+        std::vector<GpuKernel*> Ks(n);
+        std::vector<const double*> P0(n);
+        std::vector<double> ne(n);
+        for (size_t i = 0; i < n; ++i) {
+            if (!ks[i] || !P0s[i] || !outs[i].p) throw std::invalid_argument("null argument");
+            Ks[i] = ks[i]->impl.get();
+            P0[i] = P0s[i];
+            ne[i] = n_es[i];
+        }
+        solve_dm_batch_gpu(Ks, P0, ne, *cfg, outs);
     });
 }
 

@@ -258,6 +258,19 @@ int hfgpu_solve_scf(hfgpu_kernel* k, const double* P0, double n_e, const hf_scf_
 int hfgpu_solve_dm(hfgpu_kernel* k, const double* P0, double n_e, const hf_dm_config* cfg,
                    hf_project_fn project, void* user, hf_dm_result* out);
 
+/* n (1..32) direct-minimisation solves advanced together, one per kernel:
+ * every kernel on the same (nk1, nk2, nb) mesh and temperature, one shared
+ * configuration (CG optimizer; no blocks, symmetry, deflation or FP32
+ * builds).  Each iteration's per-k work is one launch over all solves and
+ * each solve stops on its own test; outs[i] equals what hfgpu_solve_dm
+ * returns for kernel i, P0s[i], n_es[i] bit for bit.  No counterpart in the
+ * reference, which runs sweeps one solve after another or in separate
+ * processes (examples/tetralayer_quartermetal/deprecated/quartermetal_grid.py:
+ * 26): for parameter sweeps, where one small solve leaves most of the GPU
+ * idle. */
+int hfgpu_solve_dm_batch(hfgpu_kernel* const* ks, size_t n, const double* const* P0s,
+                         const double* n_es, const hf_dm_config* cfg, hf_dm_result* outs);
+
 /* density_from_fock on device pointers that allocates nothing: F (nk, nb,
  * nb) complex and weights (nk) in, P (nk, nb, nb) complex and mu (one
  * double) out, all device memory, on `stream` (a hipStream_t; NULL: the null

@@ -162,19 +162,26 @@ def test_sweep_subset_covers_every_band_count():
     assert max(ps[i]["n"] for i in SWEEP_SUBSET) <= 72
 
 
-def test_sweep_48_concurrent_equals_sequential(gpu):
+def test_sweep_48_grouped_equals_sequential(gpu):
+    """All 48 sweep kernels: grouped by shape and run concurrently, and
+    concurrent one solve per stream, against one solve after another."""
     from cpp_hf_amd import SolverConfig, solve_many
 
     wls = _sweep()
     ks, P0s, nes = [w.kernel for w in wls], [w.P0 for w in wls], [w.n_electrons for w in wls]
     print("sweep 48 sequential", flush=True)
     seq = solve_many(ks, P0s, nes, config=SolverConfig(), concurrency=1)
-    print("sweep 48 concurrent", flush=True)
+    print("sweep 48 grouped and concurrent", flush=True)
+    grouped = solve_many(ks, P0s, nes, config=SolverConfig(), concurrency=16, batch=True)
+    print("sweep 48 concurrent streams", flush=True)
     par = solve_many(ks, P0s, nes, config=SolverConfig(), concurrency=16)
-    for a, b in zip(seq, par):
-        assert int(a.n_iter) == int(b.n_iter)
-        assert float(a.energy) == float(b.energy)
-        np.testing.assert_array_equal(a.density, b.density)
+    for a, b, c in zip(seq, grouped, par):
+        for x in (b, c):
+            assert int(a.n_iter) == int(x.n_iter)
+            assert float(a.energy) == float(x.energy)
+            np.testing.assert_array_equal(a.density, x.density)
+            np.testing.assert_array_equal(a.Q, x.Q)
+            np.testing.assert_array_equal(a.history["E"], x.history["E"])
 
 
 # -------------------------------------------- 192 x 192, 16 layered bands --

@@ -32,3 +32,58 @@ def test_solve_many_matches_sequential(gpu, method):
             assert a.iterations == b.iterations
             assert a.energy == b.energy
             np.testing.assert_array_equal(a.density_matrix, b.density_matrix)
+
+
+def _assert_same_dm(a, b):
+    assert int(a.n_iter) == int(b.n_iter) and bool(a.converged) == bool(b.converged)
+    assert float(a.energy) == float(b.energy) and float(a.mu) == float(b.mu)
+    for x, y in ((a.Q, b.Q), (a.p, b.p), (a.density, b.density), (a.fock, b.fock),
+                 (a.history["E"], b.history["E"]),
+                 (a.history["grad_norm"], b.history["grad_norm"])):
+        np.testing.assert_array_equal(np.asarray(x), np.asarray(y))
+
+
+@pytest.mark.parametrize("max_batch", [16, 2])
+def test_grouped_solves_equal_single_solves(gpu, max_batch):
+    """Direct-minimisation solves advanced together (one launch sequence per
+    iteration for solves of one mesh, band count and temperature) equal the
+    same solves run alone, bit for bit, including solves that stop many
+    iterations before the rest of their group and groups split into chunks.
+    The kernels are the 48-kernel sweep's groups (36 x 36, 4 and 8 bands;
+    72 x 72, 4 bands: 9 to 200 iterations within a group)."""
+    from cpp_hf_amd import SolverConfig, solve, solve_many
+    from cpp_hf_amd.workloads import sweep_params, sweep_workload
+
+    ps = sweep_params(48, seed=1)
+    pick = [i for i, p in enumerate(ps) if (p["n"], p["nb"]) in ((36, 4), (36, 8), (72, 4))]
+    assert len(pick) == 13
+    wls = [sweep_workload(ps[i]) for i in pick]
+    ks, P0s, nes = [w.kernel for w in wls], [w.P0 for w in wls], [w.n_electrons for w in wls]
+    cfg = SolverConfig()
+    print(f"grouped solves, max_batch={max_batch}", flush=True)
+    grouped = solve_many(ks, P0s, nes, config=cfg, concurrency=3, batch=True,
+                         max_batch=max_batch)
+    print("the same solves one by one", flush=True)
+    single = [solve(k, P, n, config=cfg) for k, P, n in zip(ks, P0s, nes)]
+    iters = sorted(int(r.n_iter) for r in single)
+    assert iters[0] < 20 and iters[-1] >= 100  # stops far apart inside the groups
+    for a, b in zip(grouped, single):
+        _assert_same_dm(a, b)
+
+
+def test_grouped_solve_argument_checks(gpu):
+    from cpp_hf_amd._compat import _native
+    from cpp_hf_amd.solver import _kernel_args_for_native
+    from cpp_hf_amd.workloads import layered_kernel
+
+    a = layered_kernel(12, 1, T=0.05)
+    b = layered_kernel(16, 1, T=0.05)
+    c = layered_kernel(12, 1, T=0.1)
+    args = (50, 1e-7, 0.0, 0.6, 0.5, 1e-3, 8, 10, 5, 25, False, False, False)
+    for other in (b, c):  # another mesh, another temperature
+        with pytest.raises(ValueError):
+            _native.solve_dm_batch([_kernel_args_for_native(a.kernel),
+                                    _kernel_args_for_native(other.kernel)],
+                                   [a.P0, other.P0], [a.n_electrons, other.n_electrons], *args)
+    with pytest.raises(ValueError):
+        _native.solve_dm_batch([], [], [], *args)
