@@ -25,6 +25,16 @@ enum RtrUpdateSlot : int {
     RU_COUNT = 8
 };
 
+// Slots written by launch_tcg_step (step[0..5)).
+enum TcgStepSlot : int {
+    TS_MODE = 0,   // 1: interior step alpha, 2: step t to the trust-region boundary
+    TS_A = 1,      // alpha or t
+    TS_SH = 2,     // the particle-conserving shift s_H of this direction
+    TS_RZ = 3,     // <r, z> the step used
+    TS_BOUND = 4,  // 1 when the step reached the boundary
+    TS_COUNT = 5
+};
+
 // Slots written by launch_rtr_hvp (out[0..7)).
 enum RtrHvpSlot : int {
     RH_DHDX = 0,  // sum Re <dX, HdX>
@@ -64,7 +74,18 @@ void launch_rtr_update(int nk, int nb, int mode, double a, double s_H, double la
                        double T_r, const double* eps, const double* p, const cd* G,
                        const double* w, const cd* dX, const double* dp, const cd* HdX,
                        const double* Hp_raw, cd* vX, double* vp, cd* HvX, double* Hvp, cd* rX,
-                       double* rp, LoopScratch& ls, double* out, hipStream_t st);
+                       double* rp, LoopScratch& ls, double* out, hipStream_t st,
+                       const double* step = nullptr);
+
+// One step decision of Steihaug's truncated CG (solver_rtr.hpp:296-330) on
+// the device, from the HVP slots H (RtrHvpSlot) of direction d and, on the
+// first step, <r, z> from the update slots U; rz_host otherwise.  Writes
+// step[TcgStepSlot], which launch_rtr_update reads when given step: the
+// interior step alpha = <r, z> / <d, Hd> unless the curvature is not
+// positive or |v + alpha d| >= Delta, else the step t to the boundary.
+// The arithmetic is the host's, uncontracted.
+void launch_tcg_step(const double* H, const double* U, int first, double rz_host,
+                     double occ_den, double Delta, double* step, hipStream_t st);
 
 // Trial point (solver_rtr.hpp:366-381): U = V diag(c(lam, tau)) V^H,
 // Qt = Q U, eps_t = diag(U^H Ft U).

@@ -182,9 +182,14 @@ k_rtr_update(int nk, int nb_arg, int mode, double a, double s_H, double lam_pc, 
              const double* __restrict__ dp, const cd* __restrict__ HdX,
              const double* __restrict__ Hp_raw, cd* __restrict__ vX, double* __restrict__ vp,
              cd* __restrict__ HvX, double* __restrict__ Hvp, cd* __restrict__ rX,
-             double* __restrict__ rp, double* partial) {
+             double* __restrict__ rp, double* partial, const double* __restrict__ step) {
     const int nb = NB > 0 ? NB : nb_arg;
     __shared__ double red[W * RU_COUNT];
+    if (step) {  // the device's step decision (uniform)
+        mode = static_cast<int>(step[TS_MODE]);
+        a = step[TS_A];
+        s_H = step[TS_SH];
+    }
     const int nb2 = nb * nb;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const double lam2 = lam_pc * lam_pc;
@@ -397,13 +402,58 @@ void launch_rtr_update(int nk, int nb, int mode, double a, double s_H, double la
                        double T_r, const double* eps, const double* p, const cd* G,
                        const double* w, const cd* dX, const double* dp, const cd* HdX,
                        const double* Hp_raw, cd* vX, double* vp, cd* HvX, double* Hvp, cd* rX,
-                       double* rp, LoopScratch& ls, double* out, hipStream_t st) {
+                       double* rp, LoopScratch& ls, double* out, hipStream_t st,
+                       const double* step) {
     check_nb(nb);
     if (mode < 0 || mode > 2) throw std::invalid_argument("rtr update mode must be 0, 1 or 2");
     HFGPU_LOOP_LAUNCH(k_rtr_update, nb, nk, 0, st, 1, nk, nb, mode, a, s_H, lam_pc, T_r, eps, p, G,
-                      w, dX, dp, HdX, Hp_raw, vX, vp, HvX, Hvp, rX, rp, ls.partial);
+                      w, dX, dp, HdX, Hp_raw, vX, vp, HvX, Hvp, rX, rp, ls.partial, step);
     HFGPU_LAUNCH_CHECK();
     launch_finalize_sums(ls.partial, loop_blocks(nk, waves_for(nb)), RU_COUNT, out, st);
+}
+
+namespace {
+
+// std::max's selection
+__device__ __forceinline__ double smax(double a, double b) { return a < b ? b : a; }
+
+__global__ void k_tcg_step(const double* __restrict__ H, const double* __restrict__ U, int first,
+                           double rz_host, double occ_den, double Delta, double* step) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double rz = first ? U[RU_RZX] : rz_host;
+    const double s_H = H[RH_HP] / occ_den;
+    const double dHd = H[RH_DHDX] + (H[RH_DHP] - s_H * H[RH_DP]);
+    const double dd = H[RH_DD], vd = H[RH_VD], vv0 = H[RH_VV];
+    bool boundary = !(dHd > 1e-14 * dd);
+    double alpha = 0.0;
+    if (!boundary) {
+        alpha = rz / dHd;
+        const double s2 = vv0 + 2.0 * alpha * vd + alpha * alpha * dd;
+        boundary = sqrt(smax(0.0, s2)) >= Delta;
+    }
+    if (boundary) {
+        const double a = dd, b = 2.0 * vd, c2 = smax(0.0, vv0);
+        const double cc = c2 - Delta * Delta;
+        const double t = (-b + sqrt(smax(b * b - 4.0 * a * cc, 0.0))) / (2.0 * a + 1e-30);
+        step[TS_MODE] = 2.0;
+        step[TS_A] = t;
+    } else {
+        step[TS_MODE] = 1.0;
+        step[TS_A] = alpha;
+    }
+    step[TS_SH] = s_H;
+    step[TS_RZ] = rz;
+    step[TS_BOUND] = boundary ? 1.0 : 0.0;
+}
+
+}  // namespace
+
+void launch_tcg_step(const double* H, const double* U, int first, double rz_host,
+                     double occ_den, double Delta, double* step, hipStream_t st) {
+    hipLaunchKernelGGL(k_tcg_step, dim3(1), dim3(64), 0, st, H, U, first, rz_host, occ_den,
+                       Delta, step);
+    HFGPU_LAUNCH_CHECK();
 }
 
 void launch_rtr_retract(int nk, int nb, double tau, const cd* V, const double* lam,

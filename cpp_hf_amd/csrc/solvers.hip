@@ -1246,7 +1246,17 @@ struct RtrPoint {
 };
 
 // Slots of the solver's own scalar array.
-enum RtrSlot : int { RS_UPD = 0, RS_HVP = 8, RS_DEFL = 16, RS_D2 = 18, RS_FIXED = 18 };
+enum RtrSlot : int {
+    RS_UPD = 0,
+    RS_HVP = 8,
+    RS_DEFL = 16,
+    RS_STEP = 18,  // TcgStepSlot
+    RS_D2 = 24,
+    RS_FIXED = 24
+};
+static_assert(RS_UPD + RU_COUNT <= RS_HVP && RS_HVP + RH_COUNT <= RS_DEFL &&
+                  RS_STEP + TS_COUNT <= RS_D2,
+              "trust-region slot layout");
 
 }  // namespace
 
@@ -1365,11 +1375,13 @@ void solve_rtr_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_confi
         const double* pc = cur->p.as<double>();
         const cd* Ftc = cur->Ft.as<cd>();
         const double* epc = cur->eps.as<double>();
-        auto update = [&](int mode, double a, double s_H) {
+        // step: the device's decision in rs + RS_STEP (mode, a and s_H are
+        // then read there), or nullptr with the host's values
+        auto update = [&](int mode, double a, double s_H, const double* step) {
             launch_rtr_update(nk, nb, mode, a, s_H, lam_pc, T_r, epc, pc, cur->G.as<cd>(), w2d,
                               dX.as<cd>(), dpv.as<double>(), HdX.as<cd>(), Hdp.as<double>(),
                               vX.as<cd>(), vp.as<double>(), HvX.as<cd>(), Hvp.as<double>(),
-                              rX.as<cd>(), rp.as<double>(), W.ls, rs + RS_UPD, st);
+                              rX.as<cd>(), rp.as<double>(), W.ls, rs + RS_UPD, st, step);
         };
         auto hvp = [&](bool init, double beta, double s_z) {
             launch_rtr_direction(nk, nb, init, beta, s_z, lam_pc, T_r, Qc, pc, epc, rX.as<cd>(),
@@ -1383,43 +1395,26 @@ void solve_rtr_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_confi
                            rs + RS_HVP, st);
         };
 
-        // ---- Steihaug truncated CG (solver_rtr.hpp:283-346)
-        update(0, 0.0, 0.0);  // v = Hv = 0, r = -G
+        // ---- Steihaug truncated CG (solver_rtr.hpp:283-346).  Each step's
+        // decision (interior alpha or the boundary step t) is taken on the
+        // device from the HVP's sums, with the host's arithmetic, and the
+        // update applies it at once: one host read-back per CG step.
+        update(0, 0.0, 0.0, nullptr);  // v = Hv = 0, r = -G
         double rz = 0.0, gv = 0.0, vhv = 0.0, vv = 0.0;
         double beta = 0.0, s_z = 0.0;
         for (size_t it = 0; it < cfg.tr_cg_max; ++it) {
             hvp(it == 0, beta, s_z);
-            read_back();
-            const double* H = hrs.data() + RS_HVP;
-            if (it == 0) rz = hrs[RS_UPD + RU_RZX];  // <r, z> with r = (-G, 0)
-            const double s_H = H[RH_HP] / occ_den;
-            const double dHd = H[RH_DHDX] + (H[RH_DHP] - s_H * H[RH_DP]);
-            const double dd = H[RH_DD], vd = H[RH_VD], vv0 = H[RH_VV];
-            bool boundary = !(dHd > 1e-14 * dd);
-            double alpha = 0.0;
-            if (!boundary) {
-                alpha = rz / dHd;
-                const double s2 = vv0 + 2.0 * alpha * vd + alpha * alpha * dd;
-                boundary = std::sqrt(std::max(0.0, s2)) >= Delta;
-            }
-            if (boundary) {
-                const double a = dd, b = 2.0 * vd, c2 = std::max(0.0, vv0);
-                const double cc = c2 - Delta * Delta;
-                const double t =
-                    (-b + std::sqrt(std::max(b * b - 4.0 * a * cc, 0.0))) / (2.0 * a + 1e-30);
-                update(2, t, s_H);
-                read_back();
-                gv = hrs[RS_UPD + RU_GV];
-                vhv = hrs[RS_UPD + RU_VHV];
-                vv = hrs[RS_UPD + RU_VV];
-                break;
-            }
-            update(1, alpha, s_H);
+            // on the first step <r, z> (r = (-G, 0)) comes from update 0
+            launch_tcg_step(rs + RS_HVP, rs + RS_UPD, it == 0 ? 1 : 0, rz, occ_den, Delta,
+                            rs + RS_STEP, st);
+            update(1, 0.0, 0.0, rs + RS_STEP);
             read_back();
             const double* U = hrs.data() + RS_UPD;
+            rz = hrs[RS_STEP + TS_RZ];
             gv = U[RU_GV];
             vhv = U[RU_VHV];
             vv = U[RU_VV];
+            if (hrs[RS_STEP + TS_BOUND] != 0.0) break;
             if (std::sqrt(std::max(0.0, U[RU_RN2])) < 0.05 * now.gn) break;
             s_z = U[RU_ZP] / occ_den;
             const double rz_new = U[RU_RZX] + (U[RU_RZP] - s_z * U[RU_RP]);
