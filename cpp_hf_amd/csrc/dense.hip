@@ -30,16 +30,29 @@ namespace {
 // own X_ij once; the mirrored X_ji comes from its lane through LDS (the same
 // sum, so the result equals the hermitisation of X bit for bit, and P comes
 // out exactly Hermitian).
+// k point of wavefront index t: t itself, or in a batch (seg) the t-th k
+// point of solve blockIdx.y (seg[s] + t; the solve's mu state is mus + s),
+// mapped to nk (no k point) past the solve's end or for a solve whose bit in
+// active is clear; nk is then the batch's total.
+__device__ __forceinline__ int seg_k(int t, int nk, const int* seg, unsigned active) {
+    if (!seg) return t;
+    const int sv = blockIdx.y;
+    const bool on = ((active >> sv) & 1u) && t < seg[sv + 1] - seg[sv];
+    return on ? seg[sv] + t : nk;
+}
+
 template <int NB>
 __global__ void __launch_bounds__(256)
 k_density(int nk, int nb_arg, const cd* __restrict__ V, const double* __restrict__ f,
           const double* __restrict__ eps, const MuState* __restrict__ mus, double T,
-          int occ_mode, cd* __restrict__ P, int hermitize) {
+          int occ_mode, cd* __restrict__ P, int hermitize, const int* __restrict__ seg,
+          unsigned active) {
     const int nb = NB > 0 ? NB : nb_arg;
     extern __shared__ cd smem[];
     const int nb2 = nb * nb, ld = nb + 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int k = blockIdx.x * 4 + wave;
+    if (seg && mus) mus += blockIdx.y;
+    const int k = seg_k(blockIdx.x * 4 + wave, nk, seg, active);
     cd* sv = smem + wave * (nb * ld + nb + nb2);
     double* sf = reinterpret_cast<double*>(sv + nb * ld);
     cd* sx = sv + nb * ld + nb;
@@ -97,11 +110,13 @@ template <int NB>
 __global__ void __launch_bounds__(256)
 k_density_mfma(int nk, const cd* __restrict__ V, const double* __restrict__ f,
                const double* __restrict__ eps, const MuState* __restrict__ mus, double T,
-               int occ_mode, cd* __restrict__ P, int hermitize) {
+               int occ_mode, cd* __restrict__ P, int hermitize, const int* __restrict__ seg,
+               unsigned active) {
     static_assert(NB > 8 && NB <= 16, "matrix-core density: 9..16 bands");
     __shared__ double sx[4][2][16][17];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int k = blockIdx.x * 4 + wave;
+    if (seg && mus) mus += blockIdx.y;
+    const int k = seg_k(blockIdx.x * 4 + wave, nk, seg, active);
     if (k >= nk) return;  // no workgroup barrier below: the LDS block is the wavefront's
     const int i = lane & 15, q = lane >> 4;
     const size_t kb = static_cast<size_t>(k) * NB * NB;
@@ -184,33 +199,36 @@ static bool density_mfma_on() {
     return on;
 }
 
+// nk k points, or with seg the S solves' k points seg[s] .. seg[s + 1] - 1
+// (nk the batch's total, nk_grid the largest solve's count).
 static void density_launch(int nk, int nb, const cd* V, const double* f, const double* eps,
                            const MuState* mus, double T, int mode, cd* P, bool hermitize,
-                           hipStream_t st) {
+                           hipStream_t st, const int* seg = nullptr, int S = 1,
+                           int nk_grid = 0, unsigned active = ~0u) {
     const size_t lds = 4 * (sizeof(cd) * (nb * (nb + 1) + nb + nb * nb));
-    const dim3 grid(cdiv(nk, 4)), block(256);
+    const dim3 grid(cdiv(seg ? nk_grid : nk, 4), seg ? S : 1), block(256);
     const int h = hermitize ? 1 : 0;
     if (nb > 8 && nb <= 16 && density_mfma_on()) {
         switch (nb) {
-            case 9: hipLaunchKernelGGL(k_density_mfma<9>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
-            case 10: hipLaunchKernelGGL(k_density_mfma<10>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
-            case 11: hipLaunchKernelGGL(k_density_mfma<11>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
-            case 12: hipLaunchKernelGGL(k_density_mfma<12>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
-            case 13: hipLaunchKernelGGL(k_density_mfma<13>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
-            case 14: hipLaunchKernelGGL(k_density_mfma<14>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
-            case 15: hipLaunchKernelGGL(k_density_mfma<15>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
-            default: hipLaunchKernelGGL(k_density_mfma<16>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h); break;
+            case 9: hipLaunchKernelGGL(k_density_mfma<9>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h, seg, active); break;
+            case 10: hipLaunchKernelGGL(k_density_mfma<10>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h, seg, active); break;
+            case 11: hipLaunchKernelGGL(k_density_mfma<11>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h, seg, active); break;
+            case 12: hipLaunchKernelGGL(k_density_mfma<12>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h, seg, active); break;
+            case 13: hipLaunchKernelGGL(k_density_mfma<13>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h, seg, active); break;
+            case 14: hipLaunchKernelGGL(k_density_mfma<14>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h, seg, active); break;
+            case 15: hipLaunchKernelGGL(k_density_mfma<15>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h, seg, active); break;
+            default: hipLaunchKernelGGL(k_density_mfma<16>, grid, block, 0, st, nk, V, f, eps, mus, T, mode, P, h, seg, active); break;
         }
         HFGPU_LAUNCH_CHECK();
         return;
     }
     switch (nb) {
-        case 2: hipLaunchKernelGGL(k_density<2>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
-        case 4: hipLaunchKernelGGL(k_density<4>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
-        case 8: hipLaunchKernelGGL(k_density<8>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
-        case 12: hipLaunchKernelGGL(k_density<12>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
-        case 16: hipLaunchKernelGGL(k_density<16>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
-        default: hipLaunchKernelGGL(k_density<0>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h); break;
+        case 2: hipLaunchKernelGGL(k_density<2>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h, seg, active); break;
+        case 4: hipLaunchKernelGGL(k_density<4>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h, seg, active); break;
+        case 8: hipLaunchKernelGGL(k_density<8>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h, seg, active); break;
+        case 12: hipLaunchKernelGGL(k_density<12>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h, seg, active); break;
+        case 16: hipLaunchKernelGGL(k_density<16>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h, seg, active); break;
+        default: hipLaunchKernelGGL(k_density<0>, grid, block, lds, st, nk, nb, V, f, eps, mus, T, mode, P, h, seg, active); break;
     }
     HFGPU_LAUNCH_CHECK();
 }
@@ -223,6 +241,13 @@ void launch_density(int nk, int nb, const cd* V, const double* f, cd* P, bool he
 void launch_density_occ(int nk, int nb, const cd* V, const double* eps, const MuState* mus,
                         double T, int mode, cd* P, bool hermitize, hipStream_t st) {
     density_launch(nk, nb, V, nullptr, eps, mus, T, mode, P, hermitize, st);
+}
+
+void launch_density_seg(const int* seg, int S, int nk_max, int nk_total, unsigned active,
+                        int nb, const cd* V, const double* f, cd* P, bool hermitize,
+                        hipStream_t st) {
+    density_launch(nk_total, nb, V, f, nullptr, nullptr, 0.0, 0, P, hermitize, st, seg, S,
+                   nk_max, active);
 }
 
 void launch_hermitize(int nk, int nb, cd* M, hipStream_t st) {

@@ -258,7 +258,7 @@ template <int NP, int NBC>
 __global__ void __launch_bounds__(256)
 k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __restrict__ w_out,
            cd* __restrict__ V_out, BlockMap bm, int sort, const double* __restrict__ cond,
-           int mpw) {
+           int mpw, const int* __restrict__ seg, unsigned active) {
     // the block structure applies unless cond = (off-block max, max) says the
     // batch is not block-diagonal (solver_scf.hpp:111-120): then a dense eigh
     const bool ub = !cond || cond[0] <= 1.0e-10 + 1.0e-10 * cond[1];
@@ -279,8 +279,17 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
     // mpw <= G matrices per wavefront: fewer than G leaves lane groups idle
     // but puts more wavefronts on each SIMD to hide the latency of the
     // serial QR recurrence
-    const int mat = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * mpw + lane / NP;
-    const bool live = lane / NP < mpw && mat < nmat;
+    int mat = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * mpw + lane / NP;
+    bool live = lane / NP < mpw && mat < nmat;
+    if (seg) {  // solve blockIdx.y of a batch: its matrices seg[s] .. seg[s + 1] - 1
+        const int sv = blockIdx.y;
+        live = ((active >> sv) & 1u) && lane / NP < mpw && mat < seg[sv + 1] - seg[sv];
+        mat += seg[sv];
+        // the launch is sized for the largest solve: a wavefront with no
+        // matrix of its solve leaves at once (no workgroup barrier below,
+        // and the lane-group shuffles stay within the wavefront)
+        if (!__any(live)) return;
+    }
     const int nb2 = nb * nb;
     const cd* Mk = M + static_cast<size_t>(live ? mat : 0) * nb2;
     constexpr bool kZLds = NP == 32;
@@ -501,27 +510,32 @@ static int eigh_mpw(int nmat, int g) {
     return std::min(p, g);
 }
 
-void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V,
-                        const BlockMap& bm, bool sort, hipStream_t st, const double* cond) {
+// The eigh launches: nmat matrices, or with seg the S solves' matrices
+// seg[s] .. seg[s + 1] - 1 (nmat the largest count; solves whose bit in
+// active is clear are skipped).
+static void eigh_launch(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V,
+                        const BlockMap& bm, bool sort, hipStream_t st, const double* cond,
+                        const int* seg, int S, unsigned active) {
     if (nb < 1 || nb > kMaxNb)
         throw std::invalid_argument("GPU eigh supports 1 <= nb <= 32");
     if (nmat == 0) return;
     const int so = sort ? 1 : 0;
+    const unsigned gy = seg ? static_cast<unsigned>(S) : 1u;
     if (nb <= 16) {
         const int np = nb <= 2 ? 2 : (nb <= 4 ? 4 : (nb <= 8 ? 8 : 16));
         const int mpw = eigh_mpw(nmat, 64 / np);
         const int per_block = 4 * mpw;
-        const dim3 grid(cdiv(nmat, per_block));
+        const dim3 grid(cdiv(nmat, per_block), gy);
         if (np == 2)
-            hipLaunchKernelGGL((k_eigh_tri<2, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw);
+            hipLaunchKernelGGL((k_eigh_tri<2, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw, seg, active);
         else if (np == 4)
-            hipLaunchKernelGGL((k_eigh_tri<4, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw);
+            hipLaunchKernelGGL((k_eigh_tri<4, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw, seg, active);
         else if (np == 8)
-            hipLaunchKernelGGL((k_eigh_tri<8, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw);
+            hipLaunchKernelGGL((k_eigh_tri<8, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw, seg, active);
         else if (nb == 12)  // the 12-band models: loops sized to 12, not 16
-            hipLaunchKernelGGL((k_eigh_tri<16, 12>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw);
+            hipLaunchKernelGGL((k_eigh_tri<16, 12>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw, seg, active);
         else
-            hipLaunchKernelGGL((k_eigh_tri<16, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw);
+            hipLaunchKernelGGL((k_eigh_tri<16, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw, seg, active);
         HFGPU_LAUNCH_CHECK();
         return;
     }
@@ -532,13 +546,14 @@ void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, c
     if (!use_jacobi) {
         // 32 lanes per matrix, two matrices per wavefront, one wavefront per
         // workgroup: Z rows in LDS (64 rows of 33 complex)
-        const dim3 grid(cdiv(nmat, 2));
+        const dim3 grid(cdiv(nmat, 2), gy);
         const size_t zl = sizeof(cd) * 64 * 33;
         hipLaunchKernelGGL((k_eigh_tri<32, 0>), grid, dim3(64), zl, st, nmat, nb, in_mode, M, w,
-                           V, bm, so, cond, 2);
+                           V, bm, so, cond, 2, seg, active);
         HFGPU_LAUNCH_CHECK();
         return;
     }
+    if (seg) throw std::logic_error("the Jacobi eigh takes no solve segments");
     const int nb2 = nb * nb;
     const size_t nbs = static_cast<size_t>(nb) + 1;
     const size_t lds = sizeof(cd) * (2 * static_cast<size_t>(nb2) + nbs) +
@@ -550,9 +565,20 @@ void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, c
     HFGPU_LAUNCH_CHECK();
 }
 
+void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V,
+                        const BlockMap& bm, bool sort, hipStream_t st, const double* cond) {
+    eigh_launch(nmat, nb, in_mode, M, w, V, bm, sort, st, cond, nullptr, 1, ~0u);
+}
+
 void launch_eigh(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V, hipStream_t st) {
     BlockMap bm{};  // one block
     launch_eigh_blocks(nmat, nb, in_mode, M, w, V, bm, true, st);
+}
+
+void launch_eigh_seg(const int* seg, int S, int nk_max, unsigned active, int nb, int in_mode,
+                     const cd* M, double* w, cd* V, hipStream_t st) {
+    BlockMap bm{};
+    eigh_launch(nk_max, nb, in_mode, M, w, V, bm, true, st, nullptr, seg, S, active);
 }
 
 namespace {

@@ -118,6 +118,27 @@ struct ExchangeArgs {
     const cf* VR32 = nullptr;   // VRb or scalar VR in FP32
 };
 
+// The operands of one solve in a batch of Fock builds on one geometry: the
+// ExchangeArgs fields that differ from solve to solve.  The two-step passes
+// take a device array of them, entry s for workgroup row s (blockIdx.y).
+struct ExchBatchEntry {
+    int sv = 0;  // the solve's bit in the launch's active mask
+    const cd* P = nullptr;
+    const cd* refP = nullptr;
+    const cd* refP0 = nullptr;
+    const double* wrow = nullptr;
+    cd* T = nullptr;
+    cd* partial = nullptr;
+    const double* w2d = nullptr;
+    const cd* VR = nullptr;
+    SmallTerms small;
+    cd* Sigma = nullptr;
+    cd* F = nullptr;
+    const cd* hh = nullptr;
+    const cd* SC = nullptr;
+    const double* hdiag = nullptr;
+};
+
 // Pass A without the transform: weighted row sums (+ small terms) only, for
 // kernels with exchange off but Hartree/contact terms on.
 void launch_row_sums(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st);
@@ -126,6 +147,17 @@ void launch_row_sums(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t s
 void launch_exchange_fwd(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st);
 // Pass C (rows inv + epilogue).
 void launch_exchange_inv(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st);
+// The same passes for S solves on one two-step geometry (fast_rows and
+// fast_cols, FP64 workspace) in one launch each: a gives the flags
+// (vr_scalar, small.enabled, epilogue, ifftshift), tab (device, S entries)
+// each solve's operands; entries whose bit sv in active is clear are
+// skipped.  Every solve's outputs equal its single call's.
+void launch_exchange_fwd_batch(const ExchangeGeom& g, const ExchangeArgs& a,
+                               const ExchBatchEntry* tab, int S, unsigned active,
+                               hipStream_t st);
+void launch_exchange_inv_batch(const ExchangeGeom& g, const ExchangeArgs& a,
+                               const ExchBatchEntry* tab, int S, unsigned active,
+                               hipStream_t st);
 
 // Tile-major operands of a geometry:
 //   VRb from a (nk, nb*nb) interaction (channel (a, b) <- VR[q][a * nb + b]);
@@ -142,6 +174,23 @@ void launch_fock_assemble(int nk, int nb, const cd* h, cd* Sigma, cd* F, const c
 void launch_energy(int nk, int nb, const cd* h, const cd* P, const cd* refP, const cd* Sigma,
                    const double* hdiag, const double* w2d, double* partial, double* E_out,
                    hipStream_t st);
+// launch_energy for the S solves of a batch in two launches: tab (device)
+// holds each solve's operands, nk and output; nk_max is the largest nk.
+// Solves whose bit in active is clear are skipped.  Each energy equals its
+// single call's bit for bit.
+struct EnergyBatchEntry {
+    const cd* h = nullptr;
+    const cd* P = nullptr;
+    const cd* refP = nullptr;
+    const cd* Sigma = nullptr;
+    const double* hdiag = nullptr;
+    const double* w2d = nullptr;
+    double* partial = nullptr;  // cdiv(nk, 4) doubles
+    double* E = nullptr;
+    int nk = 0;
+};
+void launch_energy_batch(const EnergyBatchEntry* tab, int S, int nk_max, int nb,
+                         unsigned active, hipStream_t st);
 // Same with a full Hartree matrix Hm (k stride hk_stride, 0 = broadcast).
 void launch_energy_fullH(int nk, int nb, const cd* h, const cd* P, const cd* refP,
                          const cd* Sigma, const cd* Hm, size_t hk_stride, const double* w2d,

@@ -668,9 +668,20 @@ k_rows_fwd_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, i
                 const cd* __restrict__ P, const cd* __restrict__ refP,
                 const cd* __restrict__ refP0, const double* __restrict__ wrow,
                 TS* __restrict__ T, cd* __restrict__ partial, const double* __restrict__ w2d,
-                int rpw) {
+                int rpw, const ExchBatchEntry* __restrict__ bt, unsigned bt_active) {
     using Sh = FastShape<N1, N2, S>;
     constexpr int N = Sh::N, L = Sh::L;
+    if (bt) {  // solve blockIdx.y of a batch (FP64 workspace)
+        const ExchBatchEntry& e = bt[blockIdx.y];
+        if (!((bt_active >> e.sv) & 1u)) return;
+        P = e.P;
+        refP = e.refP;
+        refP0 = e.refP0;
+        wrow = e.wrow;
+        T = reinterpret_cast<TS*>(e.T);
+        partial = e.partial;
+        w2d = e.w2d;
+    }
     extern __shared__ cd smem[];
     cd* s_tw = smem;
     const int nw = (blockDim.x >> 6) / S;  // tiles per workgroup
@@ -779,9 +790,18 @@ __global__ void __launch_bounds__(64 * kFastWaves)
 k_cols_fast(const cd* __restrict__ tw, int nk2, int nch, int nblk, TS* __restrict__ T,
             const TS* __restrict__ VR, int vr_scalar, int has_small,
             const cd* __restrict__ partial, const int* __restrict__ chan_a,
-            const int* __restrict__ chan_b, int nb, int packed, SmallTerms st) {
+            const int* __restrict__ chan_b, int nb, int packed, SmallTerms st,
+            const ExchBatchEntry* __restrict__ bt, unsigned bt_active) {
     using Sh = FastShape<N1, N2, S>;
     constexpr int N = Sh::N, L = Sh::L;
+    if (bt) {  // solve blockIdx.y of a batch (FP64 workspace)
+        const ExchBatchEntry& e = bt[blockIdx.y];
+        if (!((bt_active >> e.sv) & 1u)) return;
+        T = reinterpret_cast<TS*>(e.T);
+        VR = reinterpret_cast<const TS*>(e.VR);
+        partial = e.partial;
+        st = e.small;
+    }
     extern __shared__ cd smem[];
     if (has_small && blockIdx.x == 0) {
         small_terms_block(partial, N, nch, chan_a, chan_b, nb, packed, st, smem);
@@ -889,9 +909,20 @@ __global__ void __launch_bounds__(64 * kFastRowWavesMax)
 k_rows_inv_fast(const cd* __restrict__ tw, int nk1, int nb, int nch, int nblk, int packed,
                 const TS* __restrict__ T, int epilogue, int shift, double inv_n,
                 cd* __restrict__ Sigma, cd* __restrict__ F, const cd* __restrict__ hh,
-                const cd* __restrict__ SC, const double* __restrict__ hdiag, int rpw) {
+                const cd* __restrict__ SC, const double* __restrict__ hdiag, int rpw,
+                const ExchBatchEntry* __restrict__ bt, unsigned bt_active) {
     using Sh = FastShape<N1, N2, S>;
     constexpr int N = Sh::N, L = Sh::L;
+    if (bt) {  // solve blockIdx.y of a batch (FP64 workspace)
+        const ExchBatchEntry& e = bt[blockIdx.y];
+        if (!((bt_active >> e.sv) & 1u)) return;
+        T = reinterpret_cast<const TS*>(e.T);
+        Sigma = e.Sigma;
+        F = e.F;
+        hh = e.hh;
+        SC = e.SC;
+        hdiag = e.hdiag;
+    }
     extern __shared__ cd smem[];
     cd* s_tw = smem;
     const int nw = (blockDim.x >> 6) / S;  // tiles per workgroup
@@ -1089,12 +1120,11 @@ __global__ void k_fock_assemble(int nk, int nb, const cd* __restrict__ h,
 
 // One wave per k point, four per block; partial[block] = sum over its k.
 // Hm (nullable): full Hartree matrix with k stride hk_stride (0: broadcast).
-__global__ void k_energy_partial(int nk, int nb, const cd* __restrict__ h,
-                                 const cd* __restrict__ P, const cd* __restrict__ refP,
-                                 const cd* __restrict__ Sigma, const double* __restrict__ hdiag,
-                                 const cd* __restrict__ Hm, size_t hk_stride,
-                                 const double* __restrict__ w2d, double* __restrict__ partial) {
-    __shared__ double red[4];
+__device__ __forceinline__ void energy_partial_body(
+    int nk, int nb, const cd* __restrict__ h, const cd* __restrict__ P,
+    const cd* __restrict__ refP, const cd* __restrict__ Sigma, const double* __restrict__ hdiag,
+    const cd* __restrict__ Hm, size_t hk_stride, const double* __restrict__ w2d,
+    double* __restrict__ partial, double* red) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = blockIdx.x * 4 + wave;
     const int nb2 = nb * nb;
@@ -1123,12 +1153,46 @@ __global__ void k_energy_partial(int nk, int nb, const cd* __restrict__ h,
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ void k_sum_partials(const double* __restrict__ partial, int n, double* out) {
+__global__ void k_energy_partial(int nk, int nb, const cd* __restrict__ h,
+                                 const cd* __restrict__ P, const cd* __restrict__ refP,
+                                 const cd* __restrict__ Sigma, const double* __restrict__ hdiag,
+                                 const cd* __restrict__ Hm, size_t hk_stride,
+                                 const double* __restrict__ w2d, double* __restrict__ partial) {
     __shared__ double red[4];
+    energy_partial_body(nk, nb, h, P, refP, Sigma, hdiag, Hm, hk_stride, w2d, partial, red);
+}
+
+__device__ __forceinline__ void sum_partials_body(const double* __restrict__ partial, int n,
+                                                  double* out, double* red) {
     double s = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
     s = block_sum<256>(s, red);
     if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partial, int n, double* out) {
+    __shared__ double red[4];
+    sum_partials_body(partial, n, out, red);
+}
+
+// Energies of a batch: solve blockIdx.y (first launch), blockIdx.x (second),
+// each with the decomposition of its single launch_energy.
+__global__ void k_energy_partial_batch(const EnergyBatchEntry* __restrict__ tab, int nb,
+                                       const unsigned active) {
+    __shared__ double red[4];
+    if (!((active >> blockIdx.y) & 1u)) return;
+    const EnergyBatchEntry& e = tab[blockIdx.y];
+    if (static_cast<int>(blockIdx.x) >= (e.nk + 3) / 4) return;
+    energy_partial_body(e.nk, nb, e.h, e.P, e.refP, e.Sigma, e.hdiag, nullptr, 0, e.w2d,
+                        e.partial, red);
+}
+
+__global__ void k_sum_partials_batch(const EnergyBatchEntry* __restrict__ tab,
+                                     const unsigned active) {
+    __shared__ double red[4];
+    if (!((active >> blockIdx.x) & 1u)) return;
+    const EnergyBatchEntry& e = tab[blockIdx.x];
+    sum_partials_body(e.partial, (e.nk + 3) / 4, e.E, red);
 }
 
 
@@ -1545,28 +1609,30 @@ static int row_waves(const ExchangeGeom& g, int n1, bool row_sums, int split) {
     return w;
 }
 
-static void launch_rows_fwd_fast(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
+static void launch_rows_fwd_fast(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st,
+                                 const ExchBatchEntry* bt = nullptr, int nS = 1,
+                                 unsigned bt_active = ~0u) {
 #define HFGPU_FAST_CASE(N, A, B)                                                              \
     if (g.nk2 == N) {                                                                         \
         const int w = row_waves(g, A, true, g.split_fwd), rpw = (g.nblk + w - 1) / w;         \
-        const dim3 grid(row_grid(g.nk1, rpw)), block(64 * w * g.split_fwd);                   \
+        const dim3 grid(row_grid(g.nk1, rpw), nS), block(64 * w * g.split_fwd);               \
         const size_t lds = fast_lds(N, A, w, true);                                           \
         if (a.ws32 && g.split_fwd == 2)                                                       \
             hipLaunchKernelGGL((k_rows_fwd_fast<A, B, cf, 2>), grid, block, lds, st,          \
                                g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, \
-                               a.refP, a.refP0, a.wrow, a.T32, a.partial, a.w2d, rpw);        \
+                               a.refP, a.refP0, a.wrow, a.T32, a.partial, a.w2d, rpw, bt, bt_active);    \
         else if (a.ws32)                                                                      \
             hipLaunchKernelGGL((k_rows_fwd_fast<A, B, cf, 1>), grid, block, lds, st,          \
                                g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, \
-                               a.refP, a.refP0, a.wrow, a.T32, a.partial, a.w2d, rpw);        \
+                               a.refP, a.refP0, a.wrow, a.T32, a.partial, a.w2d, rpw, bt, bt_active);    \
         else if (g.split_fwd == 2)                                                            \
             hipLaunchKernelGGL((k_rows_fwd_fast<A, B, cd, 2>), grid, block, lds, st,          \
                                g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, \
-                               a.refP, a.refP0, a.wrow, a.T, a.partial, a.w2d, rpw);          \
+                               a.refP, a.refP0, a.wrow, a.T, a.partial, a.w2d, rpw, bt, bt_active);      \
         else                                                                                  \
             hipLaunchKernelGGL((k_rows_fwd_fast<A, B, cd, 1>), grid, block, lds, st,          \
                                g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.P, \
-                               a.refP, a.refP0, a.wrow, a.T, a.partial, a.w2d, rpw);          \
+                               a.refP, a.refP0, a.wrow, a.T, a.partial, a.w2d, rpw, bt, bt_active);      \
         HFGPU_LAUNCH_CHECK();                                                                 \
         return;                                                                               \
     }
@@ -1576,9 +1642,10 @@ static void launch_rows_fwd_fast(const ExchangeGeom& g, const ExchangeArgs& a, h
 }
 
 static void launch_cols_fast(const ExchangeGeom& g, const ExchangeArgs& a, int hs,
-                             hipStream_t st) {
+                             hipStream_t st, const ExchBatchEntry* bt = nullptr, int nS = 1,
+                             unsigned bt_active = ~0u) {
     const int cw = std::max(1, std::min(g.col_waves, kFastWaves / g.split_col));
-    const dim3 grid(fast_grid(g.nk2, g.nblk, cw) + hs), block(64 * cw * g.split_col);
+    const dim3 grid(fast_grid(g.nk2, g.nblk, cw) + hs, nS), block(64 * cw * g.split_col);
 #define HFGPU_FAST_CASE(N, A, B)                                                              \
     if (g.nk1 == N) {                                                                         \
         const size_t lds = std::max(fast_lds(N, A, cw, false),                                \
@@ -1587,22 +1654,22 @@ static void launch_cols_fast(const ExchangeGeom& g, const ExchangeArgs& a, int h
             hipLaunchKernelGGL((k_cols_fast<A, B, cf, 2>), grid, block, lds, st, g.ax1.d_tw,  \
                                g.nk2, g.nch, g.nblk, a.T32, a.VR32, a.vr_scalar ? 1 : 0, hs,  \
                                a.partial, g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0,     \
-                               a.small);                                                      \
+                               a.small, bt, bt_active);                                                  \
         else if (a.ws32)                                                                      \
             hipLaunchKernelGGL((k_cols_fast<A, B, cf, 1>), grid, block, lds, st, g.ax1.d_tw,  \
                                g.nk2, g.nch, g.nblk, a.T32, a.VR32, a.vr_scalar ? 1 : 0, hs,  \
                                a.partial, g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0,     \
-                               a.small);                                                      \
+                               a.small, bt, bt_active);                                                  \
         else if (g.split_col == 2)                                                            \
             hipLaunchKernelGGL((k_cols_fast<A, B, cd, 2>), grid, block, lds, st, g.ax1.d_tw,  \
                                g.nk2, g.nch, g.nblk, a.T, a.VR, a.vr_scalar ? 1 : 0, hs,      \
                                a.partial, g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0,     \
-                               a.small);                                                      \
+                               a.small, bt, bt_active);                                                  \
         else                                                                                  \
             hipLaunchKernelGGL((k_cols_fast<A, B, cd, 1>), grid, block, lds, st, g.ax1.d_tw,  \
                                g.nk2, g.nch, g.nblk, a.T, a.VR, a.vr_scalar ? 1 : 0, hs,      \
                                a.partial, g.d_chan_a, g.d_chan_b, g.nb, g.packed ? 1 : 0,     \
-                               a.small);                                                      \
+                               a.small, bt, bt_active);                                                  \
         HFGPU_LAUNCH_CHECK();                                                                 \
         return;                                                                               \
     }
@@ -1612,32 +1679,33 @@ static void launch_cols_fast(const ExchangeGeom& g, const ExchangeArgs& a, int h
 }
 
 static void launch_rows_inv_fast(const ExchangeGeom& g, const ExchangeArgs& a, double inv_n,
-                                 hipStream_t st) {
+                                 hipStream_t st, const ExchBatchEntry* bt = nullptr,
+                                 int nS = 1, unsigned bt_active = ~0u) {
 #define HFGPU_FAST_CASE(N, A, B)                                                              \
     if (g.nk2 == N) {                                                                         \
         const int w = row_waves(g, A, false, g.split_inv), rpw = (g.nblk + w - 1) / w;        \
-        const dim3 grid(row_grid(g.nk1, rpw)), block(64 * w * g.split_inv);                   \
+        const dim3 grid(row_grid(g.nk1, rpw), nS), block(64 * w * g.split_inv);               \
         const size_t lds = fast_lds(N, A, w, false);                                          \
         if (a.ws32 && g.split_inv == 2)                                                       \
             hipLaunchKernelGGL((k_rows_inv_fast<A, B, cf, 2>), grid, block, lds, st,          \
                                g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0,      \
                                a.T32, a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F,   \
-                               a.hh, a.SC, a.hdiag, rpw);                                     \
+                               a.hh, a.SC, a.hdiag, rpw, bt, bt_active);                                 \
         else if (a.ws32)                                                                      \
             hipLaunchKernelGGL((k_rows_inv_fast<A, B, cf, 1>), grid, block, lds, st,          \
                                g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0,      \
                                a.T32, a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F,   \
-                               a.hh, a.SC, a.hdiag, rpw);                                     \
+                               a.hh, a.SC, a.hdiag, rpw, bt, bt_active);                                 \
         else if (g.split_inv == 2)                                                            \
             hipLaunchKernelGGL((k_rows_inv_fast<A, B, cd, 2>), grid, block, lds, st,          \
                                g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.T, \
                                a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh,    \
-                               a.SC, a.hdiag, rpw);                                           \
+                               a.SC, a.hdiag, rpw, bt, bt_active);                                       \
         else                                                                                  \
             hipLaunchKernelGGL((k_rows_inv_fast<A, B, cd, 1>), grid, block, lds, st,          \
                                g.ax2.d_tw, g.nk1, g.nb, g.nch, g.nblk, g.packed ? 1 : 0, a.T, \
                                a.epilogue, a.ifftshift ? 1 : 0, inv_n, a.Sigma, a.F, a.hh,    \
-                               a.SC, a.hdiag, rpw);                                           \
+                               a.SC, a.hdiag, rpw, bt, bt_active);                                       \
         HFGPU_LAUNCH_CHECK();                                                                 \
         return;                                                                               \
     }
@@ -1682,6 +1750,28 @@ void launch_exchange_fwd(const ExchangeGeom& g, const ExchangeArgs& a, hipStream
                       g.packed ? 1 : 0, a.small);
 }
 
+void launch_exchange_fwd_batch(const ExchangeGeom& g, const ExchangeArgs& a,
+                               const ExchBatchEntry* tab, int S, unsigned active,
+                               hipStream_t st) {
+    if (!(g.fast_rows && g.fast_cols) || a.ws32)
+        throw std::logic_error("batched exchange needs the two-step transforms, FP64 workspace");
+    launch_rows_fwd_fast(g, a, st, tab, S, active);
+    launch_cols_fast(g, a, a.small.enabled ? 1 : 0, st, tab, S, active);
+}
+
+void launch_exchange_inv_batch(const ExchangeGeom& g, const ExchangeArgs& a,
+                               const ExchBatchEntry* tab, int S, unsigned active,
+                               hipStream_t st) {
+    if (!(g.fast_rows && g.fast_cols) || a.ws32)
+        throw std::logic_error("batched exchange needs the two-step transforms, FP64 workspace");
+    if (a.epilogue != EPI_RAW_FULL && !g.packed)
+        throw std::logic_error("packed epilogue on a full-channel geometry");
+    if (a.epilogue == EPI_RAW_FULL && g.packed)
+        throw std::logic_error("full epilogue on a packed geometry");
+    const double inv_n = 1.0 / static_cast<double>(static_cast<size_t>(g.nk1) * g.nk2);
+    launch_rows_inv_fast(g, a, inv_n, st, tab, S, active);
+}
+
 void launch_exchange_inv(const ExchangeGeom& g, const ExchangeArgs& a, hipStream_t st) {
     if (a.epilogue != EPI_RAW_FULL && !g.packed)
         throw std::logic_error("packed epilogue on a full-channel geometry");
@@ -1723,6 +1813,15 @@ void launch_fock_assemble(int nk, int nb, const cd* h, cd* Sigma, cd* F, const c
     const unsigned blocks = std::min<unsigned>(cdiv(total, 256), 4096u);
     hipLaunchKernelGGL(k_fock_assemble, dim3(blocks), dim3(256), 0, st, nk, nb, h, Sigma, F,
                        SC, hdiag);
+    HFGPU_LAUNCH_CHECK();
+}
+
+void launch_energy_batch(const EnergyBatchEntry* tab, int S, int nk_max, int nb,
+                         unsigned active, hipStream_t st) {
+    hipLaunchKernelGGL(k_energy_partial_batch, dim3(cdiv(nk_max, 4), S), dim3(256), 0, st, tab,
+                       nb, active);
+    HFGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_partials_batch, dim3(S), dim3(256), 0, st, tab, active);
     HFGPU_LAUNCH_CHECK();
 }
 

@@ -150,6 +150,41 @@ struct GpuKernel {
     std::vector<GraphEntry> graphs;
 };
 
+// build_fock (+ energy) of several kernels in one launch sequence: the
+// exchange passes of solves on one two-step geometry take one launch each
+// (their operands through a device table, ExchBatchEntry) and the energies
+// one pair of launches for all solves.  Solves whose kernels cannot share
+// the passes (another mesh, packing, interaction or reference-density kind,
+// superlattice, Stockham meshes) are grouped by geometry; a kernel outside
+// every group builds on its own.  Each solve's Sigma, F, hdiag and energy
+// equal its single build_fock / energy bit for bit.
+struct FockBatch {
+    std::vector<GpuKernel*> Ks;
+    std::vector<const cd*> P;
+    std::vector<cd*> Sigma, F;
+    std::vector<double*> hdiag, E;
+    struct Group {
+        std::vector<int> members;  // solve indices
+        ExchangeArgs a;            // flags of the group's passes
+        const ExchangeGeom* g = nullptr;
+        DevBuf tab;                // members.size() ExchBatchEntry
+        bool zero_hdiag = false;   // no small terms: hdiag is zero
+    };
+    std::vector<Group> groups;
+    std::vector<int> single;       // solves built one by one
+    DevBuf etab;                   // S EnergyBatchEntry
+    int nk_max = 0;
+    // Device pointers per solve; E[s] receives solve s's energy.  Uploads
+    // the tables on st (and waits for them).
+    void setup(const std::vector<GpuKernel*>& ks, const std::vector<const cd*>& P_,
+               const std::vector<cd*>& Sigma_, const std::vector<cd*>& F_,
+               const std::vector<double*>& hdiag_, const std::vector<double*>& E_,
+               hipStream_t st);
+    // Fock builds of the solves whose bit in active is set (Hermitian P, the
+    // reference density applied), then their energies when energy.
+    void build(unsigned active, bool energy, hipStream_t st);
+};
+
 // Vector-Jacobian product of build_fock (fock.hpp:34-321) at density P for
 // cotangents on its outputs Sigma, Hh and F (nullable = zero), in the
 // convention dL = Re sum conj(g) dX: gP (nd), gVq (nk * dv * dv, the

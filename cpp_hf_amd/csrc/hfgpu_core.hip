@@ -721,6 +721,155 @@ void GpuKernel::build_fock_graph(const cd* P, cd* Sigma, cd* F, double* hdiag_ou
     HFGPU_CHECK(hipGraphLaunch(exec, st));
 }
 
+// ---------------------------------------------------------------------------
+// batched Fock builds
+// ---------------------------------------------------------------------------
+
+void FockBatch::setup(const std::vector<GpuKernel*>& ks, const std::vector<const cd*>& P_,
+                      const std::vector<cd*>& Sigma_, const std::vector<cd*>& F_,
+                      const std::vector<double*>& hdiag_, const std::vector<double*>& E_,
+                      hipStream_t st) {
+    const int S = static_cast<int>(ks.size());
+    if (S < 1 || S > 32) throw std::invalid_argument("FockBatch takes 1 to 32 kernels");
+    Ks = ks;
+    P = P_;
+    Sigma = Sigma_;
+    F = F_;
+    hdiag = hdiag_;
+    E = E_;
+    groups.clear();
+    single.clear();
+    nk_max = 0;
+    for (GpuKernel* K : Ks) nk_max = std::max(nk_max, static_cast<int>(K->nk));
+    // the kind of a kernel's build (build_fock's branches with a Hermitian
+    // P): solves of one kind and mesh share the passes
+    auto kind_of = [](GpuKernel& K) {
+        const bool need_partials = K.hartree || K.contact;
+        return std::make_tuple(K.nk1, K.nk2, K.nb, K.dv > 1, K.has_refP, K.refP_const,
+                               K.hartree, K.contact, need_partials);
+    };
+    std::vector<std::pair<decltype(kind_of(*Ks[0])), std::vector<int>>> kinds;
+    for (int s = 0; s < S; ++s) {
+        GpuKernel& K = *Ks[s];
+        bool ok = K.exchange && !K.superlattice() && K.packed_for(true);
+        if (ok) {
+            const ExchangeGeom& g = *K.geom(true).g;
+            ok = g.fast_rows && g.fast_cols;
+        }
+        if (!ok) {
+            single.push_back(s);
+            continue;
+        }
+        const auto key = kind_of(K);
+        bool placed = false;
+        for (auto& kv : kinds)
+            if (kv.first == key) {
+                kv.second.push_back(s);
+                placed = true;
+                break;
+            }
+        if (!placed) kinds.emplace_back(key, std::vector<int>{s});
+    }
+    for (auto& kv : kinds) {
+        Group gr;
+        gr.members = kv.second;
+        GpuKernel& K0 = *Ks[gr.members[0]];
+        GpuKernel::Geom& G0 = K0.geom(true);
+        gr.g = G0.g.get();
+        const bool need_partials = K0.hartree || K0.contact;
+        gr.zero_hdiag = !need_partials;
+        ExchangeArgs& a = gr.a;
+        a.vr_scalar = K0.dv <= 1;
+        a.small.enabled = need_partials ? 1 : 0;
+        a.epilogue = EPI_FOCK_PACKED;
+        a.ifftshift = false;
+        a.hh = G0.hhb.as<cd>();  // non-null marker; each solve's comes from the table
+        std::vector<ExchBatchEntry> tab(gr.members.size());
+        for (size_t i = 0; i < gr.members.size(); ++i) {
+            const int s = gr.members[i];
+            GpuKernel& K = *Ks[s];
+            GpuKernel::Geom& G = K.geom(true);
+            ExchBatchEntry& e = tab[i];
+            e.sv = s;
+            e.P = P[s];
+            if (K.has_refP) {
+                if (K.refP_const) {
+                    e.refP0 = K.refP0.as<cd>();
+                    e.wrow = K.wrow.as<double>();
+                } else {
+                    e.refP = K.refP.as<cd>();
+                }
+            }
+            e.T = K.T_ws.as<cd>();
+            e.partial = need_partials ? K.partial.as<cd>() : nullptr;
+            e.w2d = K.w2d.as<double>();
+            e.VR = K.dv > 1 ? G.VRb.as<cd>() : K.VR_full.as<cd>();
+            if (need_partials) {
+                SmallTerms& sm = e.small;
+                sm.enabled = 1;
+                sm.include_hartree = K.hartree ? 1 : 0;
+                sm.has_contact = K.contact ? 1 : 0;
+                sm.HH = K.HH.as<double>();
+                sm.n_contact = K.n_contact;
+                sm.contact_g = K.cg.as<double>();
+                sm.Oi = K.Oi.as<cd>();
+                sm.Oj = K.Oj.as<cd>();
+                sm.hdiag = hdiag[s];
+                sm.SC = K.SC.as<cd>();
+                sm.rho_out = nullptr;
+            }
+            e.Sigma = Sigma[s];
+            e.F = F[s];
+            e.hh = G.hhb.as<cd>();
+            e.SC = K.contact ? K.SC.as<cd>() : nullptr;
+            e.hdiag = K.hartree ? hdiag[s] : nullptr;
+        }
+        gr.tab.alloc(tab.size() * sizeof(ExchBatchEntry));
+        h2d(gr.tab.get(), tab.data(), tab.size() * sizeof(ExchBatchEntry), st);
+        HFGPU_CHECK(hipStreamSynchronize(st));  // tab is a stack vector
+        groups.push_back(std::move(gr));
+    }
+    std::vector<EnergyBatchEntry> et(static_cast<size_t>(S));
+    for (int s = 0; s < S; ++s) {
+        GpuKernel& K = *Ks[s];
+        EnergyBatchEntry& e = et[s];
+        e.h = K.h.as<cd>();
+        e.P = P[s];
+        e.refP = K.has_refP ? K.refP.as<cd>() : nullptr;
+        e.Sigma = Sigma[s];
+        e.hdiag = K.hartree ? hdiag[s] : nullptr;
+        e.w2d = K.w2d.as<double>();
+        e.partial = K.e_partial.as<double>();
+        e.E = E[s];
+        e.nk = static_cast<int>(K.nk);
+    }
+    etab.alloc(et.size() * sizeof(EnergyBatchEntry));
+    h2d(etab.get(), et.data(), et.size() * sizeof(EnergyBatchEntry), st);
+    HFGPU_CHECK(hipStreamSynchronize(st));
+}
+
+void FockBatch::build(unsigned active, bool energy, hipStream_t st) {
+    const int S = static_cast<int>(Ks.size());
+    for (const Group& gr : groups) {
+        // a group's launches cover its members; inactive ones return at once
+        bool any = false;
+        for (int s : gr.members) any = any || ((active >> s) & 1u);
+        if (!any) continue;
+        const int n = static_cast<int>(gr.members.size());
+        if (gr.zero_hdiag)
+            for (int s : gr.members)
+                if ((active >> s) & 1u) fill_zero(hdiag[s], Ks[s]->nb * sizeof(double), st);
+        const ExchBatchEntry* tab = gr.tab.as<ExchBatchEntry>();
+        launch_exchange_fwd_batch(*gr.g, gr.a, tab, n, active, st);
+        launch_exchange_inv_batch(*gr.g, gr.a, tab, n, active, st);
+    }
+    for (int s : single)
+        if ((active >> s) & 1u)
+            Ks[s]->build_fock(P[s], Sigma[s], F[s], hdiag[s], true, st, true, false);
+    if (energy)
+        launch_energy_batch(etab.as<EnergyBatchEntry>(), S, nk_max, Ks[0]->nb, active, st);
+}
+
 void GpuKernel::energy(const cd* P, const cd* Sigma, const double* hdiag_in, double* E,
                        hipStream_t st) {
     launch_energy(static_cast<int>(nk), nb, h.as<cd>(), P, has_refP ? refP.as<cd>() : nullptr,

@@ -10,6 +10,12 @@ namespace hfgpu {
 // in_mode 0: eigh(herm(M)); in_mode 1: eigh(herm(i M)) (Cayley setup).
 void launch_eigh(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V, hipStream_t st);
 
+// launch_eigh over the solves of a batch: solve s owns the matrices
+// seg[s] .. seg[s + 1] - 1 (seg on the device, nk_max the largest count);
+// solves whose bit in active is clear are skipped.  Same results per matrix.
+void launch_eigh_seg(const int* seg, int S, int nk_max, unsigned active, int nb, int in_mode,
+                     const cd* M, double* w, cd* V, hipStream_t st);
+
 // Block-diagonal structure of an nb x nb matrix (nb <= 32): block of each
 // index and the first index of that block.  One block: the plain eigh.
 struct BlockMap {
@@ -60,6 +66,20 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
                       int maxiter, MuState* state, double* ws, hipStream_t st,
                       const MuState* warm = nullptr);
 
+// launch_mu_newton for the S solves of a batch in two launches: solve s has
+// the nb eigenvalues of its k points seg[s] .. seg[s + 1] - 1 (seg on the
+// device, nk_max the largest count), weights w_norm + seg[s], target
+// n_target[s] (device), warm start warm[s] (mu0 = 0 without
+// mu0_from_state), result state[s], workspace ws + s kMuPartialDoubles;
+// solves whose bit in active is clear are skipped.  Each mu equals the
+// single call's bit for bit.  Only where mu_newton_multi_ok holds (the
+// one-launch Newton paths); elsewhere call launch_mu_newton per solve.
+bool mu_newton_multi_ok(int nk_max, int nb, int maxiter);
+void launch_mu_newton_multi(const double* eps, const double* w_norm, const int* seg, int S,
+                            int nk_max, int nb, const double* n_target, double T, int maxiter,
+                            bool mu0_from_state, const MuState* warm, MuState* state,
+                            double* ws, unsigned active, hipStream_t st);
+
 // The Python surface's Newton (src/cpp_hf/utils.py:311-360) on raw weights:
 // one workgroup, result in *mu_out.
 void launch_mu_newton_py(const double* eps, const double* w, int nk, int nb, double n_e,
@@ -71,10 +91,12 @@ void launch_mu_newton_py(const double* eps, const double* w, int nk, int nb, dou
 void launch_occupations(const double* eps, size_t n, const MuState* state, double T, int mode,
                         double* out, hipStream_t st);
 // launch_occupations for S solves of n values each stored one after another,
-// each with its own state (solves whose bit in active is clear are skipped).
+// each with its own state (solves whose bit in active is clear are skipped);
+// with seg (device, S + 1 k offsets) solve s has the nb values of its k
+// points seg[s] .. seg[s + 1] - 1 and n is the largest solve's count.
 void launch_occupations_batch(const double* eps, size_t n, int S, unsigned active,
                               const MuState* states, double T, int mode, double* out,
-                              hipStream_t st);
+                              hipStream_t st, const int* seg = nullptr, int nb = 0);
 
 // level shift: out = eps + shift * (1 - expit(-(eps - mu)/(T + 1e-12)))
 void launch_level_shift(const double* eps, size_t n, const MuState* state, double T,
@@ -88,6 +110,12 @@ void launch_density(int nk, int nb, const cd* V, const double* f, cd* P, bool he
 // launch_occupations does (mode 0 SCF, mode 1 DM), without storing them.
 void launch_density_occ(int nk, int nb, const cd* V, const double* eps, const MuState* mus,
                         double T, int mode, cd* P, bool hermitize, hipStream_t st);
+// launch_density over the solves of a batch: solve s owns the k points
+// seg[s] .. seg[s + 1] - 1 of nk_total (seg on the device, nk_max the
+// largest count); solves whose bit in active is clear are skipped.
+void launch_density_seg(const int* seg, int S, int nk_max, int nk_total, unsigned active,
+                        int nb, const cd* V, const double* f, cd* P, bool hermitize,
+                        hipStream_t st);
 // in-place hermitisation of (nk, nb, nb)
 void launch_hermitize(int nk, int nb, cd* M, hipStream_t st);
 

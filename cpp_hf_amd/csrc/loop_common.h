@@ -79,6 +79,29 @@ inline unsigned loop_blocks(int nk, int W) {
     return std::max(1u, std::min<unsigned>(cdiv(nk, W), 1024u));
 }
 
+// The k range of solve blockIdx.y: offset ko (k points before it), count nk
+// and the workgroups gx its launch would have alone (loop_blocks).  With
+// seg == nullptr every solve has nk_uniform points and the grid is its own.
+struct SolveSeg {
+    size_t ko;
+    int nk, gx;
+};
+template <int W>
+__device__ __forceinline__ SolveSeg solve_seg(const int* seg, int nk_uniform) {
+    const int sv = blockIdx.y;
+    SolveSeg s;
+    if (seg) {
+        s.ko = static_cast<size_t>(seg[sv]);
+        s.nk = seg[sv + 1] - seg[sv];
+        s.gx = max(1, min((s.nk + W - 1) / W, 1024));
+    } else {
+        s.ko = static_cast<size_t>(sv) * nk_uniform;
+        s.nk = nk_uniform;
+        s.gx = static_cast<int>(gridDim.x);
+    }
+    return s;
+}
+
 inline void check_nb(int nb) {
     if (nb < 1 || nb > 32) throw std::invalid_argument("solver kernels support 1 <= nb <= 32");
 }

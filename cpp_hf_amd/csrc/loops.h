@@ -39,6 +39,12 @@ struct LoopScratch {
     // skipped.  S = 1 is a single solve.
     int S = 1;
     unsigned active = ~0u;
+    // Solves of different k counts: seg (device, S + 1 ints) holds the
+    // prefix offsets of the solves' k ranges, solve s owning k points
+    // seg[s] .. seg[s + 1] - 1 of arrays stored back to back; every launch is
+    // sized for the largest solve and each solve keeps the workgroup
+    // decomposition it has alone.  nullptr: S solves of nk points each.
+    const int* seg = nullptr;
 };
 constexpr size_t kLoopPartialDoubles = 1024 * 2 * kMaxBt;
 constexpr int kMaxBatch = 32;  // bits of LoopScratch::active

@@ -36,11 +36,13 @@ template <int MODE>
 __global__ void __launch_bounds__(256)
 k_finalize(const double* __restrict__ partial, int nparts, int ns, double* __restrict__ out,
            double* sc, int nbt, double shrink, double T_r, double max_step, int j_lo, int j_hi,
-           int final_stage, int gate, unsigned active) {
+           int final_stage, int gate, unsigned active, const int* seg, int wv) {
     // one workgroup per solve of a batch (its partials, slots and outputs
-    // one per-solve stride apart)
+    // one per-solve stride apart); with seg, each solve's partials are those
+    // of its own grid (wv wavefronts per workgroup of the producing launch)
     const int sv = blockIdx.x;
     if (!solve_active(active, sv)) return;
+    if (seg) nparts = max(1, min((seg[sv + 1] - seg[sv] + wv - 1) / wv, 1024));
     partial += sv * kLoopPartialDoubles;
     out += sv * SC_COUNT;
     if (sc) sc += sv * SC_COUNT;
@@ -97,19 +99,23 @@ __global__ void __launch_bounds__(W * 64)
 k_orbital_grad(int nk, int nb_arg, const cd* __restrict__ Q, cd* __restrict__ F,
                const double* __restrict__ p, const double* __restrict__ w, const MuState* mus,
                double T_r, int grad, double* __restrict__ eps, cd* __restrict__ G,
-               double* __restrict__ g_p, double* partial, double* sc, unsigned active) {
+               double* __restrict__ g_p, double* partial, double* sc, unsigned active,
+               const int* seg) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     extern __shared__ cd smem[];
     __shared__ double red[W * 2];
     const int nb2 = nb * nb;
+    const SolveSeg sg = solve_seg<W>(seg, nk);
+    const int gx = sg.gx;
     {
         const int sv = blockIdx.y;
-        if (!solve_active(active, sv)) return;
-        const size_t od = static_cast<size_t>(sv) * nk * nb2, ob = static_cast<size_t>(sv) * nk * nb;
+        if (!solve_active(active, sv) || static_cast<int>(blockIdx.x) >= gx) return;
+        nk = sg.nk;
+        const size_t od = sg.ko * nb2, ob = sg.ko * nb;
         Q += od;
         F += od;
         if (p) p += ob;
-        w += static_cast<size_t>(sv) * nk;
+        w += sg.ko;
         mus += sv;
         if (eps) eps += ob;
         if (G) G += od;
@@ -122,9 +128,9 @@ k_orbital_grad(int nk, int nb_arg, const cd* __restrict__ Q, cd* __restrict__ F,
     cd* sT = sF + nb2;
     const double mu = grad ? mus->mu : 0.0;
     double acc[2] = {0.0, 0.0};
-    const int nsteps = (nk + gridDim.x * W - 1) / (gridDim.x * W);
+    const int nsteps = (nk + gx * W - 1) / (gx * W);
     for (int step = 0; step < nsteps; ++step) {
-        const int k = (step * gridDim.x + blockIdx.x) * W + wave;
+        const int k = (step * gx + blockIdx.x) * W + wave;
         const bool act = k < nk;
         const size_t kb = static_cast<size_t>(k) * nb2;
         if (act)
@@ -180,21 +186,25 @@ k_precond(int nk, int nb_arg, const cd* __restrict__ G, const cd* __restrict__ G
           const double* __restrict__ eps, const double* __restrict__ p,
           const double* __restrict__ g_p, const double* __restrict__ g_pp,
           const double* __restrict__ w, double T_r, double denom_scale, cd* __restrict__ dQ,
-          double* __restrict__ h_p, double* partial, const double* sc, unsigned active) {
+          double* __restrict__ h_p, double* partial, const double* sc, unsigned active,
+          const int* seg) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     __shared__ double red[W * 6];
     const int nb2 = nb * nb;
+    const SolveSeg sg = solve_seg<W>(seg, nk);
+    const int gx = sg.gx;
     {
         const int sv = blockIdx.y;
-        if (!solve_active(active, sv)) return;
-        const size_t od = static_cast<size_t>(sv) * nk * nb2, ob = static_cast<size_t>(sv) * nk * nb;
+        if (!solve_active(active, sv) || static_cast<int>(blockIdx.x) >= gx) return;
+        nk = sg.nk;
+        const size_t od = sg.ko * nb2, ob = sg.ko * nb;
         G += od;
         Gp += od;
         eps += ob;
         p += ob;
         g_p += ob;
         g_pp += ob;
-        w += static_cast<size_t>(sv) * nk;
+        w += sg.ko;
         dQ += od;
         h_p += ob;
         partial += sv * kLoopPartialDoubles;
@@ -205,7 +215,7 @@ k_precond(int nk, int nb_arg, const cd* __restrict__ G, const cd* __restrict__ G
     const double eps_scale = sqrt(sc[SC_EPS_SQ] / static_cast<double>(nkb > 0 ? nkb : 1) + TINY_REAL);
     const double lam = fmax(T_r, denom_scale * eps_scale);
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int k = blockIdx.x * W + wave; k < nk; k += gridDim.x * W) {
+    for (int k = blockIdx.x * W + wave; k < nk; k += gx * W) {
         const size_t kb = static_cast<size_t>(k) * nb2;
         const double* ek = eps + static_cast<size_t>(k) * nb;
         const double wk = w[k];
@@ -251,14 +261,17 @@ k_cg_combine(int nk, int nb_arg, int restart, cd* __restrict__ dQ, const cd* __r
              const double* __restrict__ h_p, const double* __restrict__ d_pp,
              double* __restrict__ d_p, const double* __restrict__ eps,
              const double* __restrict__ p, const double* __restrict__ w, double T_r,
-             double* partial, double* sc, unsigned active) {
+             double* partial, double* sc, unsigned active, const int* seg) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     __shared__ double red[W * 3];
     const int nb2 = nb * nb;
+    const SolveSeg sg = solve_seg<W>(seg, nk);
+    const int gx = sg.gx;
     {
         const int sv = blockIdx.y;
-        if (!solve_active(active, sv)) return;
-        const size_t od = static_cast<size_t>(sv) * nk * nb2, ob = static_cast<size_t>(sv) * nk * nb;
+        if (!solve_active(active, sv) || static_cast<int>(blockIdx.x) >= gx) return;
+        nk = sg.nk;
+        const size_t od = sg.ko * nb2, ob = sg.ko * nb;
         dQ += od;
         dQp += od;
         h_p += ob;
@@ -266,7 +279,7 @@ k_cg_combine(int nk, int nb_arg, int restart, cd* __restrict__ dQ, const cd* __r
         d_p += ob;
         eps += ob;
         p += ob;
-        w += static_cast<size_t>(sv) * nk;
+        w += sg.ko;
         partial += sv * kLoopPartialDoubles;
         sc += sv * SC_COUNT;
     }
@@ -277,7 +290,7 @@ k_cg_combine(int nk, int nb_arg, int restart, cd* __restrict__ dQ, const cd* __r
     beta = fmin(beta, 5.0);
     if (restart) beta = 0.0;
     double acc[3] = {0, 0, 0};
-    for (int k = blockIdx.x * W + wave; k < nk; k += gridDim.x * W) {
+    for (int k = blockIdx.x * W + wave; k < nk; k += gx * W) {
         const size_t kb = static_cast<size_t>(k) * nb2;
         const double wk = w[k];
         double sq = 0.0;
@@ -339,20 +352,23 @@ k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restr
              const cd* __restrict__ Ft, cd* __restrict__ Fe, const double* __restrict__ p,
              const double* __restrict__ d_p, const double* __restrict__ w, double T_r,
              double max_step, double shrink, int j_lo, int j_hi, int stage2,
-             double* partial, const double* sc, unsigned active) {
+             double* partial, const double* sc, unsigned active, const int* seg) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
+    const SolveSeg sg = solve_seg<W>(seg, nk);
+    const int gx = sg.gx;
     {
         const int sv = blockIdx.y;
-        if (!solve_active(active, sv)) return;
-        const size_t od = static_cast<size_t>(sv) * nk * nb * nb;
-        const size_t ob = static_cast<size_t>(sv) * nk * nb;
+        if (!solve_active(active, sv) || static_cast<int>(blockIdx.x) >= gx) return;
+        nk = sg.nk;
+        const size_t od = sg.ko * nb * nb;
+        const size_t ob = sg.ko * nb;
         V += od;
         lam += ob;
         Ft += od;
         Fe += od;
         p += ob;
         d_p += ob;
-        w += static_cast<size_t>(sv) * nk;
+        w += sg.ko;
         partial += sv * kLoopPartialDoubles;
         sc += sv * SC_COUNT;
     }
@@ -373,9 +389,9 @@ k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restr
     const double tau0 = fmin(1.0, max_step / fmax(dqn, TINY_REAL));
     double tau_lo = tau0;  // tau0 shrink^j_lo by repeated products, as the reference steps
     for (int j = 0; j < j_lo; ++j) tau_lo *= shrink;
-    const int nsteps = (nk + gridDim.x * W - 1) / (gridDim.x * W);
+    const int nsteps = (nk + gx * W - 1) / (gx * W);
     for (int step = 0; step < nsteps; ++step) {
-        const int k = (step * gridDim.x + blockIdx.x) * W + wave;
+        const int k = (step * gx + blockIdx.x) * W + wave;
         const bool act = k < nk;
         const size_t kb = static_cast<size_t>(k) * nb2;
         if (act) {
@@ -435,13 +451,16 @@ __global__ void __launch_bounds__(W * 64)
 k_retract(int nk, int nb_arg, const cd* Q, const cd* __restrict__ V,
           const double* __restrict__ lam, const cd* __restrict__ Fe,
           double* __restrict__ eps_work, int cayley, const double* sc, cd* Q_out,
-          unsigned active) {
+          unsigned active, const int* seg) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
+    const SolveSeg sg = solve_seg<W>(seg, nk);
+    const int gx = sg.gx;
     {
         const int sv = blockIdx.y;
-        if (!solve_active(active, sv)) return;
-        const size_t od = static_cast<size_t>(sv) * nk * nb * nb;
-        const size_t ob = static_cast<size_t>(sv) * nk * nb;
+        if (!solve_active(active, sv) || static_cast<int>(blockIdx.x) >= gx) return;
+        nk = sg.nk;
+        const size_t od = sg.ko * nb * nb;
+        const size_t ob = sg.ko * nb;
         Q += od;
         Q_out += od;
         V += od;
@@ -462,9 +481,9 @@ k_retract(int nk, int nb_arg, const cd* Q, const cd* __restrict__ V,
     double* slam = srow + nb2;
     cd* sFe = sU;  // reused once U has been applied
     const double tau = cayley ? sc[SC_TAU] : 0.0;
-    const int nsteps = (nk + gridDim.x * W - 1) / (gridDim.x * W);
+    const int nsteps = (nk + gx * W - 1) / (gx * W);
     for (int step = 0; step < nsteps; ++step) {
-        const int k = (step * gridDim.x + blockIdx.x) * W + wave;
+        const int k = (step * gx + blockIdx.x) * W + wave;
         const bool act = k < nk;
         const size_t kb = static_cast<size_t>(k) * nb2;
         if (act) {
@@ -664,14 +683,16 @@ k_lincomb(size_t n, CdPtrs X, DotCoefs c, int nx, int accumulate, double a, doub
 void launch_finalize_sums(const double* partial, int nparts, int ns, double* out,
                           hipStream_t st) {
     hipLaunchKernelGGL(k_finalize<0>, dim3(1), dim3(256), 0, st, partial, nparts, ns, out,
-                       nullptr, 0, 0.0, 0.0, 0.0, 0, 0, 0, 0, ~0u);
+                       nullptr, 0, 0.0, 0.0, 0.0, 0, 0, 0, 0, ~0u, nullptr, 1);
     HFGPU_LAUNCH_CHECK();
 }
 
-// per solve of the scratch's batch: out is solve 0's slot
-static void finalize(LoopScratch& ls, int nparts, int ns, double* out, hipStream_t st) {
+// per solve of the scratch's batch: out is solve 0's slot; wv wavefronts per
+// workgroup of the launch that wrote the partials
+static void finalize(LoopScratch& ls, int nparts, int ns, double* out, hipStream_t st,
+                     int wv) {
     hipLaunchKernelGGL(k_finalize<0>, dim3(ls.S), dim3(256), 0, st, ls.partial, nparts, ns, out,
-                       nullptr, 0, 0.0, 0.0, 0.0, 0, 0, 0, 0, ls.active);
+                       nullptr, 0, 0.0, 0.0, 0.0, 0, 0, 0, 0, ls.active, ls.seg, wv);
     HFGPU_LAUNCH_CHECK();
 }
 
@@ -681,9 +702,9 @@ void launch_orbital_grad(int nk, int nb, const cd* Q, cd* F, const double* p, co
     check_nb(nb);
     const size_t per_wave = sizeof(cd) * 3 * nb * nb;
     HFGPU_LOOP_LAUNCH(k_orbital_grad, nb, nk, per_wave, st, ls.S, nk, nb, Q, F, p, w, mus, T_r,
-                      grad ? 1 : 0, eps, G, g_p, ls.partial, ls.sc, ls.active);
+                      grad ? 1 : 0, eps, G, g_p, ls.partial, ls.sc, ls.active, ls.seg);
     HFGPU_LAUNCH_CHECK();
-    if (grad) finalize(ls, loop_blocks(nk, waves_for(nb)), 2, ls.sc + SC_GN2, st);
+    if (grad) finalize(ls, loop_blocks(nk, waves_for(nb)), 2, ls.sc + SC_GN2, st, waves_for(nb));
 }
 
 void launch_precond(int nk, int nb, const cd* G, const cd* Gp, const double* eps,
@@ -692,9 +713,9 @@ void launch_precond(int nk, int nb, const cd* G, const cd* Gp, const double* eps
                     hipStream_t st) {
     check_nb(nb);
     HFGPU_LOOP_LAUNCH(k_precond, nb, nk, 0, st, ls.S, nk, nb, G, Gp, eps, p, g_p, g_pp, w, T_r,
-                      denom_scale, dQ, h_p, ls.partial, ls.sc, ls.active);
+                      denom_scale, dQ, h_p, ls.partial, ls.sc, ls.active, ls.seg);
     HFGPU_LAUNCH_CHECK();
-    finalize(ls, loop_blocks(nk, waves_for(nb)), 6, ls.sc + SC_PR1, st);
+    finalize(ls, loop_blocks(nk, waves_for(nb)), 6, ls.sc + SC_PR1, st, waves_for(nb));
 }
 
 void launch_cg_combine(int nk, int nb, bool restart, cd* dQ, const cd* dQp, const double* h_p,
@@ -702,9 +723,9 @@ void launch_cg_combine(int nk, int nb, bool restart, cd* dQ, const cd* dQp, cons
                        const double* w, double T_r, LoopScratch& ls, hipStream_t st) {
     check_nb(nb);
     HFGPU_LOOP_LAUNCH(k_cg_combine, nb, nk, 0, st, ls.S, nk, nb, restart ? 1 : 0, dQ, dQp, h_p,
-                      d_pp, d_p, eps, p, w, T_r, ls.partial, ls.sc, ls.active);
+                      d_pp, d_p, eps, p, w, T_r, ls.partial, ls.sc, ls.active, ls.seg);
     HFGPU_LAUNCH_CHECK();
-    finalize(ls, loop_blocks(nk, waves_for(nb)), 3, ls.sc + SC_DQN2, st);
+    finalize(ls, loop_blocks(nk, waves_for(nb)), 3, ls.sc + SC_DQN2, st, waves_for(nb));
 }
 
 static size_t ls_wave_bytes(int nb) { return sizeof(cd) * static_cast<size_t>(ls_wave_cd(nb)); }
@@ -726,11 +747,11 @@ void launch_linesearch(int nk, int nb, const cd* V, const double* lam, const cd*
         const int last = stage == 1 || nbt <= 1;
         HFGPU_LOOP_LAUNCH(k_linesearch, nb, nk, per_wave, st, ls.S, nk, nb, V, lam, Ft, Fe, p,
                           d_p, w, T_r, max_step, shrink, j_lo, j_hi, stage, ls.partial, ls.sc,
-                          ls.active);
+                          ls.active, ls.seg);
         HFGPU_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_finalize<1>, dim3(ls.S), dim3(256), 0, st, ls.partial, nparts,
                            2 * kMaxBt, ls.sc + SC_TRIAL, ls.sc, nbt, shrink, T_r, max_step, j_lo,
-                           j_hi, last, stage, ls.active);
+                           j_hi, last, stage, ls.active, ls.seg, waves_for(nb));
         HFGPU_LAUNCH_CHECK();
     }
 }
@@ -740,7 +761,7 @@ void launch_retract(int nk, int nb, cd* Q, const cd* V, const double* lam, const
     check_nb(nb);
     const size_t per_wave = ls_wave_bytes(nb);
     HFGPU_LOOP_LAUNCH(k_retract, nb, nk, per_wave, st, ls.S, nk, nb, Q, V, lam, Fe, eps_work,
-                      cayley ? 1 : 0, ls.sc, Q_out ? Q_out : Q, ls.active);
+                      cayley ? 1 : 0, ls.sc, Q_out ? Q_out : Q, ls.active, ls.seg);
     HFGPU_LAUNCH_CHECK();
 }
 
@@ -748,11 +769,12 @@ void launch_scf_residuals(int nk, int nb, const cd* F, const cd* D, const cd* Pn
                           cd* comm, const double* w, LoopScratch& ls, double* out,
                           hipStream_t st) {
     check_nb(nb);
+    if (ls.seg) throw std::logic_error("launch_scf_residuals takes solves of one k count");
     const size_t per_wave = sizeof(cd) * 2 * nb * nb;
     HFGPU_LOOP_LAUNCH(k_scf_residuals, nb, nk, per_wave, st, ls.S, nk, nb, F, D, Pn, delta,
                       comm, w, ls.partial, ls.active);
     HFGPU_LAUNCH_CHECK();
-    finalize(ls, loop_blocks(nk, waves_for(nb)), 2, out, st);
+    finalize(ls, loop_blocks(nk, waves_for(nb)), 2, out, st, waves_for(nb));
 }
 
 void launch_wdot(int nk, int m, const cd* X, const cd* Y, const double* w, LoopScratch& ls,
@@ -760,7 +782,8 @@ void launch_wdot(int nk, int m, const cd* X, const cd* Y, const double* w, LoopS
     hipLaunchKernelGGL(k_wdot, dim3(loop_blocks(nk, 4)), dim3(256), 0, st, nk, m, X, Y, w,
                        ls.partial);
     HFGPU_LAUNCH_CHECK();
-    finalize(ls, loop_blocks(nk, 4), 1, out, st);
+    if (ls.seg) throw std::logic_error("launch_wdot takes solves of one k count");
+    finalize(ls, loop_blocks(nk, 4), 1, out, st, 4);
 }
 
 void launch_wdot_multi(int nk, int m, const CdPtrs& X, int nd, const cd* Y, const double* w,
@@ -769,7 +792,8 @@ void launch_wdot_multi(int nk, int m, const CdPtrs& X, int nd, const cd* Y, cons
     hipLaunchKernelGGL(k_wdot_multi, dim3(loop_blocks(nk, 4)), dim3(256), 0, st, nk, m, X, nd,
                        Y, w, ls.partial);
     HFGPU_LAUNCH_CHECK();
-    finalize(ls, loop_blocks(nk, 4), kMaxDots, out, st);
+    if (ls.seg) throw std::logic_error("launch_wdot_multi takes solves of one k count");
+    finalize(ls, loop_blocks(nk, 4), kMaxDots, out, st, 4);
 }
 
 void launch_lincomb(size_t n, const CdPtrs& X, const DotCoefs& c, int nx, bool accumulate,

@@ -58,12 +58,13 @@ __device__ __forceinline__ double* pp_part(double* ws, int k) {
 
 // ---- bracket ---------------------------------------------------------------
 
-__global__ void __launch_bounds__(kBlock)
-k_minmax(const double* __restrict__ eps, size_t n, double* __restrict__ partial) {
-    __shared__ double red[kBlock / 64];
+// Workgroup blockIdx.x of nblocks: its partial extremes of eps.
+__device__ __forceinline__ void minmax_body(const double* __restrict__ eps, size_t n,
+                                            double* __restrict__ partial, int nblocks,
+                                            double* red) {
     double lo = INFINITY, hi = -INFINITY;
     for (size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; i < n;
-         i += static_cast<size_t>(gridDim.x) * kBlock) {
+         i += static_cast<size_t>(nblocks) * kBlock) {
         lo = fmin(lo, eps[i]);
         hi = fmax(hi, eps[i]);
     }
@@ -73,6 +74,12 @@ k_minmax(const double* __restrict__ eps, size_t n, double* __restrict__ partial)
         partial[2 * blockIdx.x] = lo;
         partial[2 * blockIdx.x + 1] = hi;
     }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_minmax(const double* __restrict__ eps, size_t n, double* __restrict__ partial) {
+    __shared__ double red[kBlock / 64];
+    minmax_body(eps, n, partial, static_cast<int>(gridDim.x), red);
 }
 
 // Bracket from the min/max partials (identical in every workgroup).
@@ -347,13 +354,10 @@ k_newton_step(const double* __restrict__ eps, const double* __restrict__ w_norm,
 // path runs every launch and lets the converged ones return at once).
 constexpr size_t kNewtonSoloMax = 16384;
 
-__global__ void __launch_bounds__(kBlock)
-k_newton_solo(const double* __restrict__ eps, const double* __restrict__ w_norm, int nb,
-              size_t n, double n_target, double T, double mu0, int mu0_from_state,
-              const MuState* warm, int maxiter, int mm_parts, double* ws,
-              MuState* state) {  // warm may be state: read before the first barrier
-    __shared__ double red[kBlock / 64];
-    __shared__ double part[2];
+__device__ __forceinline__ void newton_solo_body(
+    const double* __restrict__ eps, const double* __restrict__ w_norm, int nb, size_t n,
+    double n_target, double T, double mu0, int mu0_from_state, const MuState* warm, int maxiter,
+    int mm_parts, double* ws, MuState* state, double* red, double* part) {
     MuState st = bracket_from(pp_part(ws, 1), mm_parts, 1, T,
                               mu0_from_state ? warm->mu : mu0, red);
     const double Tj = fmax(T, 1.0e-12);
@@ -377,6 +381,17 @@ k_newton_solo(const double* __restrict__ eps, const double* __restrict__ w_norm,
         __syncthreads();
     }
     if (threadIdx.x == 0) *state = st;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_newton_solo(const double* __restrict__ eps, const double* __restrict__ w_norm, int nb,
+              size_t n, double n_target, double T, double mu0, int mu0_from_state,
+              const MuState* warm, int maxiter, int mm_parts, double* ws,
+              MuState* state) {  // warm may be state: read before the first barrier
+    __shared__ double red[kBlock / 64];
+    __shared__ double part[2];
+    newton_solo_body(eps, w_norm, nb, n, n_target, T, mu0, mu0_from_state, warm, maxiter,
+                     mm_parts, ws, state, red, part);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -607,15 +622,23 @@ k_bisect_persist(const double* __restrict__ eps, const double* __restrict__ w, i
 
 // solve_mu_inloop (utils.hpp:79-116): Newton with the bisection safeguard,
 // stopping at an exact fixed point as the multi-launch path does.
-__global__ void __launch_bounds__(kBlock)
-k_newton_persist(const double* __restrict__ eps, const double* __restrict__ w_norm, int nb,
-                 size_t n, double n_target, double T, double mu0, int mu0_from_state,
-                 const MuState* warm, int maxiter, double* slots, MuState* state) {
-    __shared__ double red[kBlock / 64];
-    __shared__ double s_mine[kPersistK], s_sum[kPersistK];
-    __shared__ double s_all[kPersistMaxG * 2];
-    __shared__ int s_flag;
-    const int G = gridDim.x;
+struct PersistLds {
+    double red[kBlock / 64];
+    double s_mine[kPersistK], s_sum[kPersistK];
+    double s_all[kPersistMaxG * 2];
+    int s_flag;
+};
+
+// Workgroup blockIdx.x of the G that run one solve.
+__device__ __forceinline__ void newton_persist_body(
+    const double* __restrict__ eps, const double* __restrict__ w_norm, int nb, size_t n,
+    double n_target, double T, double mu0, int mu0_from_state, const MuState* warm, int maxiter,
+    double* slots, MuState* state, int G, PersistLds& L) {
+    double* red = L.red;
+    double* s_mine = L.s_mine;
+    double* s_sum = L.s_sum;
+    double* s_all = L.s_all;
+    int& s_flag = L.s_flag;
     // warm may be state: every workgroup reads it before block 0 writes it
     const double m0 = mu0_from_state ? warm->mu : mu0;
     double emin, emax;
@@ -680,6 +703,81 @@ k_newton_persist(const double* __restrict__ eps, const double* __restrict__ w_no
     if (blockIdx.x == 0 && threadIdx.x == 0) *state = st;
 }
 
+__global__ void __launch_bounds__(kBlock)
+k_newton_persist(const double* __restrict__ eps, const double* __restrict__ w_norm, int nb,
+                 size_t n, double n_target, double T, double mu0, int mu0_from_state,
+                 const MuState* warm, int maxiter, double* slots, MuState* state) {
+    __shared__ PersistLds L;
+    newton_persist_body(eps, w_norm, nb, n, n_target, T, mu0, mu0_from_state, warm, maxiter,
+                        slots, state, static_cast<int>(gridDim.x), L);
+}
+
+// ---- one launch pair for the Newton solves of a batch -----------------------
+// Solve s = blockIdx.y owns the nb eigenvalues of its k points seg[s] ..
+// seg[s + 1] - 1 (weights w_norm + seg[s]), state + s, warm + s and the
+// workspace ws + s kMuPartialDoubles.  Each solve runs the path
+// launch_mu_newton takes for its size (one workgroup up to kNewtonSoloMax
+// eigenvalues, else persist_grid(n) workgroups exchanging through its own
+// slots) with the same decomposition, so every mu equals the single call's.
+
+__device__ __forceinline__ int newton_grid_of(size_t n) {
+    return static_cast<int>(n > kNewtonSoloMax
+                                ? (n + kBlock - 1) / kBlock < static_cast<size_t>(kPersistMaxG)
+                                      ? (n + kBlock - 1) / kBlock
+                                      : static_cast<size_t>(kPersistMaxG)
+                                : 1);
+}
+__device__ __forceinline__ int mm_blocks_of(size_t n) {
+    const size_t b = (n + kBlock - 1) / kBlock;
+    return static_cast<int>(b < 1 ? 1 : (b < static_cast<size_t>(kMuBlocks) ? b : kMuBlocks));
+}
+
+// Before k_newton_multi: the extremes' partials of the one-workgroup solves,
+// the empty pattern in the slots of the persistent ones (rounds of G slots).
+__global__ void __launch_bounds__(kBlock)
+k_newton_multi_prep(const double* __restrict__ eps, int nb, const int* __restrict__ seg,
+                    int rounds, double* ws, unsigned active) {
+    __shared__ double red[kBlock / 64];
+    const int sv = blockIdx.y;
+    if (!((active >> sv) & 1u)) return;
+    const size_t n = static_cast<size_t>(seg[sv + 1] - seg[sv]) * nb;
+    double* w = ws + static_cast<size_t>(sv) * kMuPartialDoubles;
+    if (n <= kNewtonSoloMax) {
+        const int nbk = mm_blocks_of(n);
+        if (static_cast<int>(blockIdx.x) >= nbk) return;
+        minmax_body(eps + static_cast<size_t>(seg[sv]) * nb, n, pp_part(w, 1), nbk, red);
+    } else {
+        const int cnt = rounds * newton_grid_of(n) * kPersistK;
+        unsigned long long* p = reinterpret_cast<unsigned long long*>(w + kPersistSlotBase);
+        for (int i = blockIdx.x * kBlock + threadIdx.x; i < cnt; i += gridDim.x * kBlock)
+            p[i] = kEmptySlot;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_newton_multi(const double* __restrict__ eps, const double* __restrict__ w_norm, int nb,
+               const int* __restrict__ seg, const double* __restrict__ n_target, double T,
+               int mu0_from_state, const MuState* warm, int maxiter, double* ws,
+               MuState* state, unsigned active) {
+    __shared__ PersistLds L;
+    __shared__ double part[2];
+    const int sv = blockIdx.y;
+    if (!((active >> sv) & 1u)) return;
+    const size_t ko = static_cast<size_t>(seg[sv]);
+    const size_t n = static_cast<size_t>(seg[sv + 1] - seg[sv]) * nb;
+    const int G = newton_grid_of(n);
+    if (static_cast<int>(blockIdx.x) >= G) return;
+    double* w = ws + static_cast<size_t>(sv) * kMuPartialDoubles;
+    if (n <= kNewtonSoloMax)
+        newton_solo_body(eps + ko * nb, w_norm + ko, nb, n, n_target[sv], T, 0.0,
+                         mu0_from_state, warm + sv, maxiter, mm_blocks_of(n), w, state + sv,
+                         L.red, part);
+    else
+        newton_persist_body(eps + ko * nb, w_norm + ko, nb, n, n_target[sv], T, 0.0,
+                            mu0_from_state, warm + sv, maxiter, w + kPersistSlotBase,
+                            state + sv, G, L);
+}
+
 // ---- the Python surface's Newton (utils.py:311-360) -------------------------
 // One workgroup runs every step: the bracket from the extremes, maxiter
 // Newton steps with the bisection safeguard on raw weights, then the final
@@ -728,15 +826,24 @@ k_mu_newton_py(const double* __restrict__ eps, const double* __restrict__ w, int
 
 // ---- occupations -----------------------------------------------------------
 
-// Row s = blockIdx.y of the grid: solve s of a batch (n values each, its
-// own state), skipped when its bit in active is clear.
+// Row s = blockIdx.y of the grid: solve s of a batch (n values each, or
+// with seg the nb values of its k points seg[s] .. seg[s + 1] - 1; its own
+// state), skipped when its bit in active is clear.
 __global__ void k_occupations(const double* __restrict__ eps, size_t n,
                               const MuState* __restrict__ state, double T, int mode,
-                              double* __restrict__ out, unsigned active) {
+                              double* __restrict__ out, unsigned active, const int* seg,
+                              int nb) {
     const int sv = blockIdx.y;
     if (!((active >> sv) & 1u)) return;
-    eps += sv * n;
-    out += sv * n;
+    if (seg) {
+        const size_t o = static_cast<size_t>(seg[sv]) * nb;
+        n = static_cast<size_t>(seg[sv + 1] - seg[sv]) * nb;
+        eps += o;
+        out += o;
+    } else {
+        eps += sv * n;
+        out += sv * n;
+    }
     const double mu = state[sv].mu;
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * blockDim.x) {
@@ -894,6 +1001,36 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
     }
 }
 
+bool mu_newton_multi_ok(int nk_max, int nb, int maxiter) {
+    if (maxiter <= 0 || !persist_on()) return false;
+    const size_t n = static_cast<size_t>(nk_max) * nb;
+    return n <= kNewtonSoloMax || persist_fits(1 + maxiter, persist_grid(n));
+}
+
+void launch_mu_newton_multi(const double* eps, const double* w_norm, const int* seg, int S,
+                            int nk_max, int nb, const double* n_target, double T, int maxiter,
+                            bool mu0_from_state, const MuState* warm, MuState* state,
+                            double* ws, unsigned active, hipStream_t st) {
+    if (!mu_newton_multi_ok(nk_max, nb, maxiter))
+        throw std::logic_error("launch_mu_newton_multi outside the one-launch Newton paths");
+    const size_t n_max = static_cast<size_t>(nk_max) * nb;
+    const int rounds = 1 + maxiter;
+    // the largest prep grid: extremes of the one-workgroup solves, slot fills
+    const unsigned gp = std::max<unsigned>(
+        reduce_blocks(std::min<size_t>(n_max, kNewtonSoloMax), 1),
+        n_max > kNewtonSoloMax ? static_cast<unsigned>(std::min(64, (rounds * persist_grid(n_max) *
+                                                                     kPersistK + 255) / 256))
+                               : 1u);
+    hipLaunchKernelGGL(k_newton_multi_prep, dim3(gp, static_cast<unsigned>(S)), dim3(kBlock), 0,
+                       st, eps, nb, seg, rounds, ws, active);
+    HFGPU_LAUNCH_CHECK();
+    const unsigned gm = n_max > kNewtonSoloMax ? static_cast<unsigned>(persist_grid(n_max)) : 1u;
+    hipLaunchKernelGGL(k_newton_multi, dim3(gm, static_cast<unsigned>(S)), dim3(kBlock), 0, st,
+                       eps, w_norm, nb, seg, n_target, T, mu0_from_state ? 1 : 0, warm, maxiter,
+                       ws, state, active);
+    HFGPU_LAUNCH_CHECK();
+}
+
 void launch_mu_newton_py(const double* eps, const double* w, int nk, int nb, double n_e,
                          double T, int maxiter, double* mu_out, hipStream_t st) {
     const size_t n = static_cast<size_t>(nk) * nb;
@@ -905,16 +1042,17 @@ void launch_mu_newton_py(const double* eps, const double* w, int nk, int nb, dou
 void launch_occupations(const double* eps, size_t n, const MuState* state, double T, int mode,
                         double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_occupations, dim3(std::min<unsigned>(cdiv(n, 256), 2048u)), dim3(256),
-                       0, st, eps, n, state, T, mode, out, ~0u);
+                       0, st, eps, n, state, T, mode, out, ~0u, static_cast<const int*>(nullptr),
+                       0);
     HFGPU_LAUNCH_CHECK();
 }
 
 void launch_occupations_batch(const double* eps, size_t n, int S, unsigned active,
                               const MuState* states, double T, int mode, double* out,
-                              hipStream_t st) {
+                              hipStream_t st, const int* seg, int nb) {
     hipLaunchKernelGGL(k_occupations,
                        dim3(std::min<unsigned>(cdiv(n, 256), 2048u), static_cast<unsigned>(S)),
-                       dim3(256), 0, st, eps, n, states, T, mode, out, active);
+                       dim3(256), 0, st, eps, n, states, T, mode, out, active, seg, nb);
     HFGPU_LAUNCH_CHECK();
 }
 

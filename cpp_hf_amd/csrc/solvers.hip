@@ -974,21 +974,24 @@ double* pinned_batch_scalars(size_t doubles) {
 
 }  // namespace
 
-// S independent direct-minimisation solves (one Fock kernel each) on one
-// mesh, band count and temperature, advanced together: every per-k kernel of
-// an iteration (density, orbital gradient, preconditioner, CG combination,
-// eigh of the direction, line search, retraction, occupations) is one launch
-// over all S solves, with workgroup row s working on solve s and its own
-// scalar slots and partials; the Fock builds, energies and chemical
-// potentials are one launch sequence per solve.  Each solve keeps the
-// reference's stop test (solver_dm.hpp:476-490) on its own history: a solve
-// that stops is masked out of the launches that change its state
-// (retraction, occupations, chemical potential), so its orbitals stay those
-// of its last iteration, and the finalisation runs for all solves at the
-// end.  Every per-solve value is computed by the same kernels with the same
-// workgroup decomposition as a single solve, so each result equals
-// solve_dm_gpu's bit for bit.  Iteration k + 1 is enqueued before the
-// scalars of iteration k are read back, as in solve_dm_gpu.
+// S independent direct-minimisation solves (one Fock kernel each) with one
+// band count and temperature, on any meshes, advanced together: every launch
+// of an iteration covers all S solves.  The solves' k points are stored back
+// to back (solve s owns k points seg[s] .. seg[s + 1] - 1 of every array);
+// the per-k kernels (density, orbital gradient, preconditioner, CG
+// combination, eigh of the direction, line search, retraction, occupations)
+// give solve s workgroup row s with the workgroup decomposition it has alone,
+// the Fock builds of solves on one mesh share one launch per exchange pass
+// (FockBatch), the energies one launch pair and the chemical potentials one
+// launch pair (launch_mu_newton_multi).  Each solve keeps the reference's
+// stop test (solver_dm.hpp:476-490) on its own history: a solve that stops
+// is masked out of the launches that change its state (retraction,
+// occupations, chemical potential), so its orbitals stay those of its last
+// iteration, and the finalisation runs for all solves at the end.  Every
+// per-solve value is computed by the same kernels with the same
+// decomposition as a single solve, so each result equals solve_dm_gpu's bit
+// for bit.  Iteration k + 1 is enqueued before the scalars of iteration k
+// are read back, as in solve_dm_gpu.
 void solve_dm_batch_gpu(const std::vector<GpuKernel*>& Ks, const std::vector<const double*>& P0s,
                         const std::vector<double>& n_es, const hf_dm_config& cfg,
                         hf_dm_result* outs) {
@@ -997,9 +1000,9 @@ void solve_dm_batch_gpu(const std::vector<GpuKernel*>& Ks, const std::vector<con
     GpuKernel& K0 = *Ks[0];
     for (GpuKernel* Kp : Ks) {
         const GpuKernel& K = *Kp;
-        if (K.nk1 != K0.nk1 || K.nk2 != K0.nk2 || K.nb != K0.nb || K.T != K0.T)
+        if (K.nb != K0.nb || K.T != K0.T)
             throw std::invalid_argument(
-                "a batch needs one mesh, band count and temperature for every kernel");
+                "a batch needs one band count and temperature for every kernel");
     }
     if (cfg.n_blocks > 0 || cfg.symmetry || cfg.exchange_fp32 || cfg.optimizer == 1 ||
         cfg.n_deflation > 0)
@@ -1008,78 +1011,112 @@ void solve_dm_batch_gpu(const std::vector<GpuKernel*>& Ks, const std::vector<con
             "builds");
     if (cfg.bt_max > static_cast<size_t>(kMaxBt))
         throw std::invalid_argument("bt_max above the GPU line-search limit of 16");
-    const int nk = static_cast<int>(K0.nk), nb = K0.nb;
-    const size_t nd = K0.nd, nkb = K0.nk * K0.nb, nkk = K0.nk;
-    const size_t bytes = nd * sizeof(cd), rb = nkb * sizeof(double);
+    const int nb = K0.nb;
     const size_t SS = static_cast<size_t>(S);
     const double T_r = std::max(K0.T, 1.0e-12);
     hipStream_t st = K0.stream;
 
+    // the solves' k ranges
+    std::vector<int> koff(SS + 1, 0);
+    int nk_max = 0;
+    for (int s = 0; s < S; ++s) {
+        const size_t nks = Ks[s]->nk;
+        if (static_cast<size_t>(koff[s]) + nks > static_cast<size_t>(std::numeric_limits<int>::max() / 64))
+            throw std::invalid_argument("batched solves hold too many k points");
+        koff[s + 1] = koff[s] + static_cast<int>(nks);
+        nk_max = std::max(nk_max, static_cast<int>(nks));
+    }
+    const size_t nk_tot = static_cast<size_t>(koff[S]);
+    const size_t nb2 = static_cast<size_t>(nb) * nb;
+    const size_t bytes_all = nk_tot * nb2 * sizeof(cd), rb_all = nk_tot * nb * sizeof(double);
+    auto nd_of = [&](int s) { return static_cast<size_t>(koff[s + 1] - koff[s]) * nb2; };
+    auto nkb_of = [&](int s) { return static_cast<size_t>(koff[s + 1] - koff[s]) * nb; };
+
     std::vector<double> n_target(SS);
-    std::vector<double> wn(SS * nkk);
+    std::vector<double> wn(nk_tot);
     for (int s = 0; s < S; ++s) {
         const GpuKernel& K = *Ks[s];
         validate_electron_count(K.w_host, static_cast<size_t>(nb), n_es[s]);
         const double wsum = std::max(K.weight_sum, TINY_REAL);
-        for (size_t i = 0; i < nkk; ++i) wn[s * nkk + i] = K.w_host[i] / wsum;
+        for (size_t i = 0; i < K.nk; ++i) wn[koff[s] + i] = K.w_host[i] / wsum;
         n_target[s] = n_es[s] / wsum;
     }
-    DevBuf w_norm(SS * nkk * sizeof(double));
-    h2d(w_norm.get(), wn.data(), SS * nkk * sizeof(double), st);
+    DevBuf w_norm(nk_tot * sizeof(double)), seg_d((SS + 1) * sizeof(int)),
+        nt_d(SS * sizeof(double));
+    h2d(w_norm.get(), wn.data(), nk_tot * sizeof(double), st);
+    h2d(seg_d.get(), koff.data(), (SS + 1) * sizeof(int), st);
+    h2d(nt_d.get(), n_target.data(), SS * sizeof(double), st);
     const double* wd = w_norm.as<double>();
 
-    DevBuf Q(SS * bytes), P(SS * bytes), Sigma(SS * bytes), F(SS * bytes), G(SS * bytes),
-        Gp(SS * bytes), dQ(SS * bytes), dQp(SS * bytes), Vd(SS * bytes), Q_alt(SS * bytes);
-    DevBuf p(SS * rb), p_alt(SS * rb), eps(SS * rb), g_p(SS * rb), g_pp(SS * rb), h_p(SS * rb),
-        d_p(SS * rb), d_pp(SS * rb), lam(SS * rb), eps_w(SS * rb),
+    DevBuf Q(bytes_all), P(bytes_all), Sigma(bytes_all), F(bytes_all), G(bytes_all),
+        Gp(bytes_all), dQ(bytes_all), dQp(bytes_all), Vd(bytes_all), Q_alt(bytes_all);
+    DevBuf p(rb_all), p_alt(rb_all), eps(rb_all), g_p(rb_all), g_pp(rb_all), h_p(rb_all),
+        d_p(rb_all), d_pp(rb_all), lam(rb_all), eps_w(rb_all),
         hdiag(SS * std::max(nb, 1) * sizeof(double));
-    DevBuf mus(SS * sizeof(MuState)), mus_alt(SS * sizeof(MuState)), mu_ws(kMuPartialDoubles * sizeof(double));
+    DevBuf mus(SS * sizeof(MuState)), mus_alt(SS * sizeof(MuState));
+    const bool mu_multi = mu_newton_multi_ok(nk_max, nb, cfg.mu_maxiter);
+    DevBuf mu_ws((mu_multi ? SS : 1) * kMuPartialDoubles * sizeof(double));
     DevBuf part(SS * kLoopPartialDoubles * sizeof(double));
     DevBuf scb[2] = {DevBuf(SS * SC_COUNT * sizeof(double)), DevBuf(SS * SC_COUNT * sizeof(double))};
-    fill_zero(Gp.get(), SS * bytes, st);
-    fill_zero(dQp.get(), SS * bytes, st);
-    fill_zero(g_pp.get(), SS * rb, st);
-    fill_zero(d_pp.get(), SS * rb, st);
+    fill_zero(Gp.get(), bytes_all, st);
+    fill_zero(dQp.get(), bytes_all, st);
+    fill_zero(g_pp.get(), rb_all, st);
+    fill_zero(d_pp.get(), rb_all, st);
     fill_zero(mus.get(), SS * sizeof(MuState), st);
     fill_zero(mus_alt.get(), SS * sizeof(MuState), st);
     fill_zero(scb[0].get(), SS * SC_COUNT * sizeof(double), st);
     fill_zero(scb[1].get(), SS * SC_COUNT * sizeof(double), st);
 
-    auto Pd = [&](DevBuf& b, int s) { return b.as<cd>() + static_cast<size_t>(s) * nd; };
-    // the per-solve launch sequences (Fock build and energy, chemical
-    // potential), one solve after another on the batch stream: running them
-    // on forked per-solve streams joined by events measured slower (the
-    // 48-kernel sweep at 60 against 90-100 solves/s)
+    auto Pd = [&](const DevBuf& b, int s) { return b.as<cd>() + static_cast<size_t>(koff[s]) * nb2; };
+    auto Rd = [&](const DevBuf& b, int s) {
+        return b.as<double>() + static_cast<size_t>(koff[s]) * nb;
+    };
+    // chemical potentials: one launch pair for the batch where the one-launch
+    // Newton paths apply, else one launch sequence per solve
     auto mu_solve = [&](const double* e, MuState* out, const MuState* warm, bool use_warm,
                         unsigned act) {
+        if (mu_multi) {
+            launch_mu_newton_multi(e, wd, seg_d.as<int>(), S, nk_max, nb, nt_d.as<double>(), T_r,
+                                   cfg.mu_maxiter, use_warm, warm, out, mu_ws.as<double>(), act,
+                                   st);
+            return;
+        }
         for (int s = 0; s < S; ++s)
             if ((act >> s) & 1u)
-                launch_mu_newton(e + static_cast<size_t>(s) * nkb, wd + static_cast<size_t>(s) * nkk,
-                                 nk, nb, n_target[s], 0.0, use_warm, T_r, cfg.mu_maxiter, out + s,
-                                 mu_ws.as<double>(), st, warm + s);
+                launch_mu_newton(e + static_cast<size_t>(koff[s]) * nb, wd + koff[s],
+                                 koff[s + 1] - koff[s], nb, n_target[s], 0.0, use_warm, T_r,
+                                 cfg.mu_maxiter, out + s, mu_ws.as<double>(), st, warm + s);
     };
-    auto builds = [&](unsigned act, bool energy, double* sc) {
+    // Fock builds and energies: the scalar slot array decides where the
+    // energies land, so there is one FockBatch per slot array
+    FockBatch fb[2];
+    for (int b = 0; b < 2; ++b) {
+        std::vector<const cd*> Pp(SS);
+        std::vector<cd*> Sp(SS), Fp(SS);
+        std::vector<double*> hp(SS), Ep(SS);
         for (int s = 0; s < S; ++s) {
-            if (!((act >> s) & 1u)) continue;
-            GpuKernel& K = *Ks[s];
-            double* hd = hdiag.as<double>() + static_cast<size_t>(s) * nb;
-            K.build_fock(Pd(P, s), Pd(Sigma, s), Pd(F, s), hd, true, st, true, false);
-            if (energy) K.energy(Pd(P, s), Pd(Sigma, s), hd, sc + s * SC_COUNT + SC_E, st);
+            Pp[s] = Pd(P, s);
+            Sp[s] = Pd(Sigma, s);
+            Fp[s] = Pd(F, s);
+            hp[s] = hdiag.as<double>() + static_cast<size_t>(s) * nb;
+            Ep[s] = scb[b].as<double>() + static_cast<size_t>(s) * SC_COUNT + SC_E;
         }
-    };
+        fb[b].setup(Ks, Pp, Sp, Fp, hp, Ep, st);
+    }
     const unsigned all = S == 32 ? ~0u : ((1u << S) - 1u);
     LoopScratch L;
     L.partial = part.as<double>();
     L.S = S;
+    L.seg = seg_d.as<int>();
 
     // ---- init (solver_dm.hpp:436-459), every solve
-    for (int s = 0; s < S; ++s) h2d(Pd(P, s), P0s[s], bytes, st);
-    launch_hermitize(S * nk, nb, P.as<cd>(), st);
-    builds(all, false, nullptr);
-    launch_eigh(S * nk, nb, 0, F.as<cd>(), eps.as<double>(), Q.as<cd>(), st);
+    for (int s = 0; s < S; ++s) h2d(Pd(P, s), P0s[s], nd_of(s) * sizeof(cd), st);
+    launch_hermitize(static_cast<int>(nk_tot), nb, P.as<cd>(), st);
+    fb[0].build(all, false, st);
+    launch_eigh(static_cast<int>(nk_tot), nb, 0, F.as<cd>(), eps.as<double>(), Q.as<cd>(), st);
     mu_solve(eps.as<double>(), mus.as<MuState>(), mus.as<MuState>(), false, all);
-    launch_occupations_batch(eps.as<double>(), nkb, S, all, mus.as<MuState>(), T_r, 1,
-                             p.as<double>(), st);
+    launch_occupations_batch(eps.as<double>(), static_cast<size_t>(nk_max) * nb, S, all,
+                             mus.as<MuState>(), T_r, 1, p.as<double>(), st, L.seg, nb);
 
     // per-solve host state
     std::vector<std::vector<double>> hist_E(SS), hist_G(SS);
@@ -1120,30 +1157,34 @@ void solve_dm_batch_gpu(const std::vector<GpuKernel*>& Ks, const std::vector<con
     StreamEvent ev[2];
     ev[0].create();
     ev[1].create();
+    const int nkm = nk_max;
     // iteration j over the solves in act: reads state j & 1, writes (j + 1) & 1
     auto enqueue = [&](size_t j, unsigned act) {
         const bool restart = (j == 0) || (cfg.cg_restart > 0 && j % cfg.cg_restart == 0);
         const int s0 = static_cast<int>(j & 1), s1 = s0 ^ 1;
         L.sc = scb[s0].as<double>();
         L.active = act;
-        launch_density(S * nk, nb, Qs[s0], ps[s0], P.as<cd>(), true, st);
-        builds(act, true, L.sc);
-        launch_orbital_grad(nk, nb, Qs[s0], F.as<cd>(), ps[s0], wd, ms[s0], T_r, true,
+        launch_density_seg(L.seg, S, nk_max, static_cast<int>(nk_tot), act, nb, Qs[s0], ps[s0],
+                           P.as<cd>(), true, st);
+        fb[s0].build(act, true, st);
+        launch_orbital_grad(nkm, nb, Qs[s0], F.as<cd>(), ps[s0], wd, ms[s0], T_r, true,
                             eps.as<double>(), G.as<cd>(), g_p.as<double>(), L, st);
-        launch_precond(nk, nb, G.as<cd>(), Gp.as<cd>(), eps.as<double>(), ps[s0],
+        launch_precond(nkm, nb, G.as<cd>(), Gp.as<cd>(), eps.as<double>(), ps[s0],
                        g_p.as<double>(), g_pp.as<double>(), wd, T_r, cfg.denom_scale, dQ.as<cd>(),
                        h_p.as<double>(), L, st);
-        launch_cg_combine(nk, nb, restart, dQ.as<cd>(), dQp.as<cd>(), h_p.as<double>(),
+        launch_cg_combine(nkm, nb, restart, dQ.as<cd>(), dQp.as<cd>(), h_p.as<double>(),
                           d_pp.as<double>(), d_p.as<double>(), eps.as<double>(), ps[s0], wd, T_r,
                           L, st);
-        launch_eigh(S * nk, nb, 1, dQ.as<cd>(), lam.as<double>(), Vd.as<cd>(), st);
-        launch_linesearch(nk, nb, Vd.as<cd>(), lam.as<double>(), F.as<cd>(), Sigma.as<cd>(), ps[s0],
-                          d_p.as<double>(), wd, T_r, cfg.max_step, cfg.bt_shrink,
+        launch_eigh_seg(L.seg, S, nk_max, act, nb, 1, dQ.as<cd>(), lam.as<double>(), Vd.as<cd>(),
+                        st);
+        launch_linesearch(nkm, nb, Vd.as<cd>(), lam.as<double>(), F.as<cd>(), Sigma.as<cd>(),
+                          ps[s0], d_p.as<double>(), wd, T_r, cfg.max_step, cfg.bt_shrink,
                           static_cast<int>(cfg.bt_max), L, st);
-        launch_retract(nk, nb, Qs[s0], Vd.as<cd>(), lam.as<double>(), Sigma.as<cd>(),
+        launch_retract(nkm, nb, Qs[s0], Vd.as<cd>(), lam.as<double>(), Sigma.as<cd>(),
                        eps_w.as<double>(), true, L, st, Qs[s1]);
         mu_solve(eps_w.as<double>(), ms[s1], ms[s0], true, act);
-        launch_occupations_batch(eps_w.as<double>(), nkb, S, act, ms[s1], T_r, 1, ps[s1], st);
+        launch_occupations_batch(eps_w.as<double>(), static_cast<size_t>(nk_max) * nb, S, act,
+                                 ms[s1], T_r, 1, ps[s1], st, L.seg, nb);
         d2h(hs + static_cast<size_t>(s0) * SS * SC_COUNT, L.sc, SS * SC_COUNT * sizeof(double),
             st);
         HFGPU_CHECK(hipEventRecord(ev[s0].e, st));
@@ -1181,27 +1222,25 @@ void solve_dm_batch_gpu(const std::vector<GpuKernel*>& Ks, const std::vector<con
     // the odd ones into buffer 0
     for (int s = 0; s < S; ++s) {
         if (!(k_s[s] & 1)) continue;
-        d2d(Q.as<cd>() + static_cast<size_t>(s) * nd, Q_alt.as<cd>() + static_cast<size_t>(s) * nd,
-            bytes, st);
-        d2d(p.as<double>() + static_cast<size_t>(s) * nkb,
-            p_alt.as<double>() + static_cast<size_t>(s) * nkb, rb, st);
+        d2d(Pd(Q, s), Pd(Q_alt, s), nd_of(s) * sizeof(cd), st);
+        d2d(Rd(p, s), Rd(p_alt, s), nkb_of(s) * sizeof(double), st);
         d2d(mus.as<MuState>() + s, mus_alt.as<MuState>() + s, sizeof(MuState), st);
     }
 
     // ---- finalise (solver_dm.hpp:615-652), every solve
     L.sc = scb[0].as<double>();
     L.active = all;
-    launch_density(S * nk, nb, Q.as<cd>(), p.as<double>(), P.as<cd>(), true, st);
-    builds(all, false, nullptr);
-    launch_orbital_grad(nk, nb, Q.as<cd>(), F.as<cd>(), nullptr, wd, mus.as<MuState>(), T_r,
+    launch_density(static_cast<int>(nk_tot), nb, Q.as<cd>(), p.as<double>(), P.as<cd>(), true, st);
+    fb[0].build(all, false, st);
+    launch_orbital_grad(nkm, nb, Q.as<cd>(), F.as<cd>(), nullptr, wd, mus.as<MuState>(), T_r,
                         false, nullptr, nullptr, nullptr, L, st);
-    launch_eigh(S * nk, nb, 0, F.as<cd>(), eps_w.as<double>(), Vd.as<cd>(), st);
-    launch_retract(nk, nb, Q.as<cd>(), Vd.as<cd>(), nullptr, nullptr, nullptr, false, L, st);
+    launch_eigh(static_cast<int>(nk_tot), nb, 0, F.as<cd>(), eps_w.as<double>(), Vd.as<cd>(), st);
+    launch_retract(nkm, nb, Q.as<cd>(), Vd.as<cd>(), nullptr, nullptr, nullptr, false, L, st);
     mu_solve(eps_w.as<double>(), mus.as<MuState>(), mus.as<MuState>(), true, all);
-    launch_occupations_batch(eps_w.as<double>(), nkb, S, all, mus.as<MuState>(), T_r, 1,
-                             p.as<double>(), st);
-    launch_density(S * nk, nb, Q.as<cd>(), p.as<double>(), P.as<cd>(), true, st);
-    builds(all, true, L.sc);
+    launch_occupations_batch(eps_w.as<double>(), static_cast<size_t>(nk_max) * nb, S, all,
+                             mus.as<MuState>(), T_r, 1, p.as<double>(), st, L.seg, nb);
+    launch_density(static_cast<int>(nk_tot), nb, Q.as<cd>(), p.as<double>(), P.as<cd>(), true, st);
+    fb[0].build(all, true, st);
 
     std::vector<double> hsc(SS * SC_COUNT);
     std::vector<MuState> hmu(SS);
@@ -1209,10 +1248,10 @@ void solve_dm_batch_gpu(const std::vector<GpuKernel*>& Ks, const std::vector<con
     d2h(hmu.data(), mus.get(), SS * sizeof(MuState), st);
     for (int s = 0; s < S; ++s) {
         hf_dm_result* out = outs + s;
-        if (out->Q) d2h(out->Q, Q.as<cd>() + static_cast<size_t>(s) * nd, bytes, st);
-        if (out->p) d2h(out->p, p.as<double>() + static_cast<size_t>(s) * nkb, rb, st);
-        if (out->density) d2h(out->density, P.as<cd>() + static_cast<size_t>(s) * nd, bytes, st);
-        if (out->fock) d2h(out->fock, F.as<cd>() + static_cast<size_t>(s) * nd, bytes, st);
+        if (out->Q) d2h(out->Q, Pd(Q, s), nd_of(s) * sizeof(cd), st);
+        if (out->p) d2h(out->p, Rd(p, s), nkb_of(s) * sizeof(double), st);
+        if (out->density) d2h(out->density, Pd(P, s), nd_of(s) * sizeof(cd), st);
+        if (out->fock) d2h(out->fock, Pd(F, s), nd_of(s) * sizeof(cd), st);
     }
     HFGPU_CHECK(hipStreamSynchronize(st));
     for (int s = 0; s < S; ++s) {

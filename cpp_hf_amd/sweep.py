@@ -9,16 +9,14 @@ bands) leaves most of the GPU idle: its kernels are short and launch-bound.
 
 * solves run concurrently, one HIP stream and host thread each (the native
   loops release the GIL), with ``concurrency`` in flight;
-* optionally (``batch=True``), direct-minimisation solves that share a mesh,
-  band count and temperature are advanced together
-  (``_native.solve_dm_batch``): each iteration's per-k work is one launch
-  over the whole group, every solve keeps its own stop test, and each
-  result equals the solve run alone bit for bit.  On MI355X this measured
-  slower than independent streams for the 48-kernel sweep (87-100 against
-  ~113 solves/s): a group's per-solve Fock builds and chemical-potential
-  solves stay one launch sequence per solve on the group's stream, which
-  lengthens its critical path more than the shared launches save, so it
-  is off by default.
+* with ``batch=True``, direct-minimisation solves that share a band count
+  and temperature are advanced together (``_native.solve_dm_batch``), on
+  whatever meshes: every launch of an iteration covers the whole group (the
+  exchange passes one launch per mesh, the chemical potentials and energies
+  one launch pair), every solve keeps its own stop test, and each result
+  equals the solve run alone bit for bit.  A single small solve is a chain
+  of latency-bound kernels; a group turns the chains of its members into
+  one chain of wider kernels.
 
 Reductions are fixed-order, so results equal those of :func:`solve` /
 :func:`solve_scf` called one by one.
@@ -36,8 +34,8 @@ from .reference_scf import SCFConfig, solve_scf
 from .solver import SolveResult, SolverConfig, _kernel_args_for_native, solve
 from .utils import validate_electron_count
 
-#: solves advanced by one launch sequence at most (the native limit is 32)
-MAX_BATCH = 16
+#: solves advanced by one launch sequence at most (the native limit)
+MAX_BATCH = 32
 
 
 def _batchable(config: SolverConfig) -> bool:
@@ -49,11 +47,11 @@ def _batchable(config: SolverConfig) -> bool:
 
 
 def _groups(kernels: Sequence, max_batch: int) -> list[list[int]]:
-    """Indices grouped by (mesh, bands, temperature), in input order, split
-    into chunks of at most max_batch."""
+    """Indices grouped by (bands, temperature), in input order, split into
+    chunks of at most max_batch."""
     by: dict[tuple, list[int]] = {}
     for i, k in enumerate(kernels):
-        by.setdefault((tuple(k.h.shape), float(k.T)), []).append(i)
+        by.setdefault((int(k.h.shape[-1]), float(k.T)), []).append(i)
     out = []
     for idx in by.values():
         for c in range(0, len(idx), max_batch):
@@ -92,9 +90,9 @@ def solve_many(kernels: Sequence, P0s: Sequence, n_electrons: Sequence[float], *
     "scf" runs :func:`solve_scf` (``config``: :class:`SCFConfig`).
     ``concurrency`` solves (or groups) run at once (1 runs them one after
     another).  With ``batch=True`` (direct minimisation with the CG optimizer
-    and no projector, blocks, deflation or mixed precision), solves on one
-    mesh, band count and temperature are advanced together in groups of up
-    to ``max_batch`` (results unchanged, bit for bit).
+    and no projector, blocks, deflation or mixed precision), solves with one
+    band count and temperature are advanced together in groups of up to
+    ``max_batch`` (results unchanged, bit for bit).
     """
     if not (len(kernels) == len(P0s) == len(n_electrons)):
         raise ValueError("kernels, P0s and n_electrons must have the same length")

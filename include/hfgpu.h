@@ -259,10 +259,11 @@ int hfgpu_solve_dm(hfgpu_kernel* k, const double* P0, double n_e, const hf_dm_co
                    hf_project_fn project, void* user, hf_dm_result* out);
 
 /* n (1..32) direct-minimisation solves advanced together, one per kernel:
- * every kernel on the same (nk1, nk2, nb) mesh and temperature, one shared
- * configuration (CG optimizer; no blocks, symmetry, deflation or FP32
- * builds).  Each iteration's per-k work is one launch over all solves and
- * each solve stops on its own test; outs[i] equals what hfgpu_solve_dm
+ * every kernel with the same band count and temperature (meshes may
+ * differ), one shared configuration (CG optimizer; no blocks, symmetry,
+ * deflation or FP32 builds).  Each launch of an iteration covers all solves
+ * (the exchange passes one launch per mesh) and each solve stops on its own
+ * test; outs[i] equals what hfgpu_solve_dm
  * returns for kernel i, P0s[i], n_es[i] bit for bit.  No counterpart in the
  * reference, which runs sweeps one solve after another or in separate
  * processes (examples/tetralayer_quartermetal/deprecated/quartermetal_grid.py:

@@ -163,7 +163,7 @@ def test_sweep_subset_covers_every_band_count():
 
 
 def test_sweep_48_grouped_equals_sequential(gpu):
-    """All 48 sweep kernels: grouped by shape and run concurrently, and
+    """All 48 sweep kernels: grouped by band count and run concurrently, and
     concurrent one solve per stream, against one solve after another."""
     from cpp_hf_amd import SolverConfig, solve_many
 

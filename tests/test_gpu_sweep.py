@@ -46,11 +46,12 @@ def _assert_same_dm(a, b):
 @pytest.mark.parametrize("max_batch", [16, 2])
 def test_grouped_solves_equal_single_solves(gpu, max_batch):
     """Direct-minimisation solves advanced together (one launch sequence per
-    iteration for solves of one mesh, band count and temperature) equal the
-    same solves run alone, bit for bit, including solves that stop many
-    iterations before the rest of their group and groups split into chunks.
-    The kernels are the 48-kernel sweep's groups (36 x 36, 4 and 8 bands;
-    72 x 72, 4 bands: 9 to 200 iterations within a group)."""
+    iteration for solves of one band count and temperature, meshes mixed)
+    equal the same solves run alone, bit for bit, including solves that stop
+    many iterations before the rest of their group and groups split into
+    chunks.  The kernels are 13 of the 48-kernel sweep's (36 x 36 and
+    72 x 72 with 4 bands, 36 x 36 with 8 bands: 9 to 200 iterations within
+    a group)."""
     from cpp_hf_amd import SolverConfig, solve, solve_many
     from cpp_hf_amd.workloads import sweep_params, sweep_workload
 
@@ -77,13 +78,39 @@ def test_grouped_solve_argument_checks(gpu):
     from cpp_hf_amd.workloads import layered_kernel
 
     a = layered_kernel(12, 1, T=0.05)
-    b = layered_kernel(16, 1, T=0.05)
+    b = layered_kernel(12, 2, T=0.05)
     c = layered_kernel(12, 1, T=0.1)
     args = (50, 1e-7, 0.0, 0.6, 0.5, 1e-3, 8, 10, 5, 25, False, False, False)
-    for other in (b, c):  # another mesh, another temperature
+    for other in (b, c):  # another band count, another temperature
         with pytest.raises(ValueError):
             _native.solve_dm_batch([_kernel_args_for_native(a.kernel),
                                     _kernel_args_for_native(other.kernel)],
                                    [a.P0, other.P0], [a.n_electrons, other.n_electrons], *args)
     with pytest.raises(ValueError):
         _native.solve_dm_batch([], [], [], *args)
+
+
+def test_grouped_mixed_paths_equal_single_solves(gpu):
+    """One group whose solves take every per-solve path of the grouped
+    iteration: two-step meshes sharing the exchange launches (12 x 12 twice,
+    48 x 48), a Stockham mesh built on its own (20 x 20), the one-workgroup
+    Newton mu (up to 16384 eigenvalues) and the multi-workgroup one
+    (48 x 48 x 8 = 18432), with solves that stop at different iterations;
+    each result equals the solve run alone, bit for bit."""
+    from cpp_hf_amd import SolverConfig, solve, solve_many
+    from cpp_hf_amd.workloads import sweep_workload
+
+    params = [dict(n=12, nb=8, u=0.10, scale=1.0, gate=3.0, offset=0.00),
+              dict(n=20, nb=8, u=0.20, scale=0.8, gate=5.0, offset=0.10),
+              dict(n=48, nb=8, u=0.05, scale=1.2, gate=4.0, offset=-0.10),
+              dict(n=12, nb=8, u=0.25, scale=0.6, gate=7.0, offset=0.20)]
+    wls = [sweep_workload(p) for p in params]
+    ks, P0s, nes = [w.kernel for w in wls], [w.P0 for w in wls], [w.n_electrons for w in wls]
+    cfg = SolverConfig(max_iter=60)
+    print("mixed-path group", flush=True)
+    grouped = solve_many(ks, P0s, nes, config=cfg, concurrency=1, batch=True)
+    print("the same solves one by one", flush=True)
+    single = [solve(k, P, n, config=cfg) for k, P, n in zip(ks, P0s, nes)]
+    assert len({int(r.n_iter) for r in single}) > 1
+    for a, b in zip(grouped, single):
+        _assert_same_dm(a, b)
