@@ -739,12 +739,17 @@ void launch_linesearch(int nk, int nb, const cd* V, const double* lam, const cd*
         throw std::invalid_argument("bt_max above the GPU line-search limit of 16");
     const size_t per_wave = ls_wave_bytes(nb);
     const int nparts = static_cast<int>(loop_blocks(nk, waves_for(nb)));
-    // stage 1: Fe and tau0, which the reference's backtracking usually accepts;
-    // stage 2 (device-gated): the remaining trials only when tau0 was rejected
-    const int first = std::min(nbt, 1);
-    for (int stage = 0; stage < (nbt > 1 ? 2 : 1); ++stage) {
-        const int j_lo = stage == 0 ? 0 : 1, j_hi = stage == 0 ? first : nbt;
-        const int last = stage == 1 || nbt <= 1;
+    // stage 1: Fe and tau0, which the reference's backtracking usually
+    // accepts; the later stages (device-gated) only run while no trial has
+    // been accepted: trials 1-3, which take most of the rest, then the others
+    int bounds[4], nst = 0;
+    bounds[nst++] = 0;
+    for (int b : {1, 4, nbt})
+        if (b > bounds[nst - 1] && b <= nbt) bounds[nst++] = b;
+    if (nst == 1) bounds[nst++] = 0;  // nbt = 0: one empty stage applies the no-trial rule
+    for (int stage = 0; stage + 1 < nst; ++stage) {
+        const int j_lo = bounds[stage], j_hi = bounds[stage + 1];
+        const int last = stage + 2 == nst;
         HFGPU_LOOP_LAUNCH(k_linesearch, nb, nk, per_wave, st, ls.S, nk, nb, V, lam, Ft, Fe, p,
                           d_p, w, T_r, max_step, shrink, j_lo, j_hi, stage, ls.partial, ls.sc,
                           ls.active, ls.seg);

@@ -23,7 +23,7 @@
 // L = 1); HFGPU_BISECT_LEVELS overrides it.  Brackets are bit-identical to the
 // sequential loop's for every L.  Newton (solve_mu_inloop) takes one
 // launch per iteration and stops computing at an exact fixed point; for at
-// most 16384 eigenvalues one workgroup runs all iterations in one launch.
+// most 4096 eigenvalues one workgroup runs all iterations in one launch.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -352,7 +352,11 @@ k_newton_step(const double* __restrict__ eps, const double* __restrict__ w_norm,
 // (n <= kNewtonSoloMax): the iterations of k_newton_step / k_newton_last
 // without a launch each, ending at the exact fixed point (the multi-launch
 // path runs every launch and lets the converged ones return at once).
-constexpr size_t kNewtonSoloMax = 16384;
+// Above 4096 eigenvalues the persistent multi-workgroup solve is faster: one
+// workgroup's four wavefronts already fill its CU's SIMDs with the FP64
+// exp work (a 36 x 36 x 8 solve took 87 us in one workgroup against about
+// 30 us spread over 41).
+constexpr size_t kNewtonSoloMax = 4096;
 
 __device__ __forceinline__ void newton_solo_body(
     const double* __restrict__ eps, const double* __restrict__ w_norm, int nb, size_t n,

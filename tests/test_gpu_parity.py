@@ -199,14 +199,15 @@ def test_mu_newton_multi_launch_stays_at_converged_mu(gpu, oracle):
 
 
 def test_mu_newton_solo_and_multi_launch_agree(gpu):
-    """16384 eigenvalues run in one workgroup, 16385 in one launch per
-    iteration; the extra eigenvalue sits far above the Fermi level, where its
-    occupation and slope are exactly zero, so both solve the same problem."""
+    """4096 eigenvalues run in one workgroup, 4097 on several workgroups
+    exchanging partial sums; the extra eigenvalue sits far above the Fermi
+    level, where its occupation and slope are exactly zero, so both solve the
+    same problem."""
     rng = np.random.RandomState(12)
-    eps = rng.randn(128, 128, 1)
-    wn = np.full((128, 128), 1.0 / 16384)
-    eps2 = np.concatenate([eps.reshape(-1), [1.0e6]]).reshape(1, 16385, 1)
-    wn2 = np.concatenate([wn.reshape(-1), [1.0 / 16384]]).reshape(1, 16385)
+    eps = rng.randn(64, 64, 1)
+    wn = np.full((64, 64), 1.0 / 4096)
+    eps2 = np.concatenate([eps.reshape(-1), [1.0e6]]).reshape(1, 4097, 1)
+    wn2 = np.concatenate([wn.reshape(-1), [1.0 / 4096]]).reshape(1, 4097)
     for n_t, mu0 in ((0.3, 0.0), (0.77, 1.5)):
         for it in (1, 3, 25):
             a = gpu.solve_mu_inloop(eps, wn, n_t, mu0, 0.05, it)

@@ -94,8 +94,9 @@ def test_grouped_mixed_paths_equal_single_solves(gpu):
     """One group whose solves take every per-solve path of the grouped
     iteration: two-step meshes sharing the exchange launches (12 x 12 twice,
     48 x 48), a Stockham mesh built on its own (20 x 20), the one-workgroup
-    Newton mu (up to 16384 eigenvalues) and the multi-workgroup one
-    (48 x 48 x 8 = 18432), with solves that stop at different iterations;
+    Newton mu (up to 4096 eigenvalues: 12 x 12 x 8, 20 x 20 x 8) and the
+    multi-workgroup one (48 x 48 x 8), with solves that stop at different
+    iterations;
     each result equals the solve run alone, bit for bit."""
     from cpp_hf_amd import SolverConfig, solve, solve_many
     from cpp_hf_amd.workloads import sweep_workload
