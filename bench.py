@@ -545,7 +545,27 @@ def main() -> int:
     ap.add_argument("--kernel-sample", action="store_true",
                     help="only run the solver kernels a few times each for a profiler: eigh "
                          "and density_from_fock at 96 x 96 x 8 and 192 x 192 x 16")
+    ap.add_argument("--sweep-sample", choices=("sequential", "grouped"), default=None,
+                    help="only run the 48-kernel sweep once after one warm-up pass, one solve "
+                         "after another or as band-count groups one after another (no "
+                         "concurrent host threads), for a profiler")
     args = ap.parse_args()
+
+    if args.sweep_sample:
+        from cpp_hf_amd import SolverConfig, solve_many
+        from cpp_hf_amd.workloads import sweep_params, sweep_workload
+
+        wls = [sweep_workload(p) for p in sweep_params(48, seed=1)]
+        kw = dict(config=SolverConfig(), concurrency=1, batch=args.sweep_sample == "grouped",
+                  max_batch=32)
+        ks, P0s, nes = [w.kernel for w in wls], [w.P0 for w in wls], [w.n_electrons for w in wls]
+        solve_many(ks, P0s, nes, **kw)
+        t = time.perf_counter()
+        rs = solve_many(ks, P0s, nes, **kw)
+        dt = time.perf_counter() - t
+        print(json.dumps({"sweep_sample": args.sweep_sample, "seconds": dt,
+                          "total_iterations": int(sum(int(r.n_iter) for r in rs))}), flush=True)
+        return 0
 
     if args.kernel_sample:
         from cpp_hf_amd._compat import _native
