@@ -115,3 +115,51 @@ def test_grouped_mixed_paths_equal_single_solves(gpu):
     assert len({int(r.n_iter) for r in single}) > 1
     for a, b in zip(grouped, single):
         _assert_same_dm(a, b)
+
+
+def _exchange_off(wl):
+    """The workload's kernel with the exchange off (Hartree only)."""
+    from cpp_hf_amd import HartreeFockKernel
+
+    k = wl.kernel
+    nb = k.h.shape[-1]
+    refP = np.broadcast_to(0.5 * np.eye(nb, dtype=np.complex128), k.h.shape).copy()
+    return HartreeFockKernel(k.w2d, k.h, np.zeros(k.h.shape[:2] + (1, 1)), T=k.T,
+                             include_hartree=True, include_exchange=False,
+                             reference_density=refP, hartree_matrix=k.HH)
+
+
+@pytest.mark.parametrize("case", ["nb16", "nb20", "kinds"])
+def test_grouped_band_counts_and_build_kinds_equal_single_solves(gpu, case):
+    """Groups on the less common paths, each against its solves run alone,
+    bit for bit: 16 bands (matrix-core density, 16-lane eigh) on two-step and
+    Stockham meshes; 20 bands (one-wavefront loop kernels, 32-lane eigh with
+    its rows in LDS); and one mesh with three kinds of Fock build side by
+    side (Hartree + exchange, exchange alone, Hartree alone), which the
+    grouped build runs as separate exchange launches or on its own."""
+    from cpp_hf_amd import SolverConfig, solve, solve_many
+    from cpp_hf_amd.workloads import layered_kernel
+
+    if case == "nb16":
+        wls = [layered_kernel(n, 8, T=0.05, u=0.1 + 0.02 * i, seed=i)
+               for i, n in enumerate((12, 16, 20))]
+        ks = [w.kernel for w in wls]
+        cfg = SolverConfig(max_iter=25)
+    elif case == "nb20":
+        wls = [layered_kernel(n, 10, T=0.05, u=0.1 + 0.03 * i, seed=i)
+               for i, n in enumerate((8, 12))]
+        ks = [w.kernel for w in wls]
+        cfg = SolverConfig(max_iter=15)
+    else:
+        wls = [layered_kernel(12, 4, T=0.05, u=0.1, seed=1),
+               layered_kernel(12, 4, T=0.05, u=0.2, seed=2, hartree=False),
+               layered_kernel(12, 4, T=0.05, u=0.15, seed=3)]
+        ks = [wls[0].kernel, wls[1].kernel, _exchange_off(wls[2])]
+        cfg = SolverConfig(max_iter=40)
+    P0s, nes = [w.P0 for w in wls], [w.n_electrons for w in wls]
+    print(f"grouped {case}", flush=True)
+    grouped = solve_many(ks, P0s, nes, config=cfg, concurrency=1, batch=True)
+    print(f"single {case}", flush=True)
+    single = [solve(k, P, n, config=cfg) for k, P, n in zip(ks, P0s, nes)]
+    for a, b in zip(grouped, single):
+        _assert_same_dm(a, b)
