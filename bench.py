@@ -483,8 +483,8 @@ def run_sweep(concurrency: int) -> dict:
     cfg = SolverConfig()
     # untimed passes page in the code objects and fill the memory pool
     # grouped: solves of one band count advanced together in groups of up to
-    # 4, 8 groups in flight (the best of the shapes measured on MI355X)
-    gkw = dict(concurrency=8, batch=True, max_batch=4)
+    # 3, 8 groups in flight (the best of the shapes measured on MI355X)
+    gkw = dict(concurrency=8, batch=True, max_batch=3)
     solve_many(kernels, P0s, n_es, config=cfg, concurrency=concurrency)
     solve_many(kernels, P0s, n_es, config=cfg, **gkw)
     t = time.perf_counter()
@@ -511,7 +511,7 @@ def run_sweep(concurrency: int) -> dict:
         "grouped_solves_per_s": len(wls) / t_grp,
         "grouped_speedup": t_seq / t_grp,
         "grouped_method": "solves of one band count (any mesh) advanced by one launch sequence "
-                          "per iteration in groups of up to 4, 8 groups concurrent",
+                          "per iteration in groups of up to 3, 8 groups concurrent",
         "converged": int(sum(bool(r.converged) for r in par)),
         "total_iterations": int(sum(int(r.n_iter) for r in par)),
         "batched_equals_sequential": bool(same(par)),
@@ -531,7 +531,8 @@ def main() -> int:
     ap.add_argument("--no-solves", action="store_true")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the 48-kernel sweep (sequential vs concurrent solves)")
-    ap.add_argument("--sweep-concurrency", type=int, default=16)
+    ap.add_argument("--sweep-concurrency", type=int, default=8,
+                    help="solves in flight in the sweep (8 measured fastest on MI355X)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the 192 x 192 mixed-precision and the 144 x 144 adjoint sections")
     ap.add_argument("--no-superlattice", action="store_true",
