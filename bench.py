@@ -501,6 +501,18 @@ def run_sweep(concurrency: int) -> dict:
         return all(int(a.n_iter) == int(b.n_iter) and float(a.energy) == float(b.energy)
                    and np.array_equal(a.density, b.density) for a, b in zip(seq, xs))
 
+    # roofline of the sweep: every solve's reference dataflow bytes,
+    # (30 n_iter + 2 line-search trials) A, over the HBM peak (the trials
+    # counted by the solver itself, in an untimed pass)
+    from cpp_hf_amd._compat import _native
+
+    sweep_bytes = 0
+    for w in wls:
+        r = _native.solve_dm(*_dm_args(w.kernel, w.P0, w.n_electrons), return_stats=True)
+        A = 16 * w.kernel.h.shape[0] * w.kernel.h.shape[1] * w.kernel.h.shape[-1] ** 2
+        sweep_bytes += (30 * int(r[6]) + 2 * int(r[10]["n_ls_trials"])) * A
+    roof_s = sweep_bytes / (HBM_PEAK_GBS * 1e9)
+
     return {
         "kernels": len(wls),
         "sequential_solves_per_s": len(wls) / t_seq,
@@ -514,6 +526,11 @@ def run_sweep(concurrency: int) -> dict:
                           "per iteration in groups of up to 3, 8 groups concurrent",
         "converged": int(sum(bool(r.converged) for r in par)),
         "total_iterations": int(sum(int(r.n_iter) for r in par)),
+        "roofline_seconds": roof_s,
+        "roofline_bytes": sweep_bytes,
+        "roofline_frac_sequential": roof_s / t_seq,
+        "roofline_frac_batched": roof_s / t_par,
+        "roofline_frac_grouped": roof_s / t_grp,
         "batched_equals_sequential": bool(same(par)),
         "grouped_equals_sequential": bool(same(grp)),
     }
