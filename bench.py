@@ -19,8 +19,9 @@ The JSON line also carries:
                 be built here), its one-thread rate, and the 48 sweep
                 kernels solved one after another by the oracle.
   solves        wall time of full direct-minimisation (CG and trust-region
-                Newton) and SCF (mixing 0.5) solves at C1 (QWZ 24 x 24) and
-                C2, on the GPU (Newton: GPU only) and in the
+                Newton) and SCF (mixing 0.5) solves of the two-band QWZ model
+                at 24 x 24 and the eight-band layered model at 96 x 96, on the
+                GPU (Newton: GPU only) and in the
                 oracle on the host, with each solve's roofline time: the
                 reference dataflow's bytes per iteration (DM (30 + 2 trials) A
                 with the line-search trials counted, SCF 21 A) over 8 TB/s.
@@ -287,11 +288,12 @@ def run_superlattice(torch, dev, cpu: bool) -> dict:
 
 
 def run_symmetry() -> dict:
-    """SCF at C2 with a projector: device averaging vs the host callback."""
+    """SCF of the 96 x 96 eight-band layered model with a projector: device
+    averaging vs the host callback."""
     from cpp_hf_amd import SCFConfig, make_project_fn, solve_scf
-    from cpp_hf_amd.workloads import config2
+    from cpp_hf_amd.workloads import layered_nb8
 
-    wl = config2(96)
+    wl = layered_nb8(96)
     nb = wl.kernel.h.shape[-1]
     proj = make_project_fn(time_reversal_U=np.eye(nb))
     out = {"workload": "layered QWZ 96x96 k, nb=8, SCF (DIIS, 40 iterations at most), "
@@ -334,13 +336,13 @@ def _scf_args(kern, P0, n_e, mixing=0.5):
 
 
 def run_solves(cpu: bool) -> dict:
-    """DM and SCF solves at C1 and C2 on the GPU (and in the oracle), with
-    their roofline times."""
+    """DM and SCF solves of the 24 x 24 QWZ and the 96 x 96 layered models on
+    the GPU (and in the oracle), with their roofline times."""
     from cpp_hf_amd._compat import _native
-    from cpp_hf_amd.workloads import config2, qwz_kernel
+    from cpp_hf_amd.workloads import layered_nb8, qwz_kernel
 
     out = {}
-    for name, wl in (("c1_qwz_24x24_nb2", qwz_kernel(24)), ("c2_layered_96x96_nb8", config2(96))):
+    for name, wl in (("qwz_24x24_nb2", qwz_kernel(24)), ("layered_96x96_nb8", layered_nb8(96))):
         kern = wl.kernel
         nk = kern.h.shape[0] * kern.h.shape[1]
         nb = kern.h.shape[-1]
@@ -391,16 +393,17 @@ def run_solves(cpu: bool) -> dict:
 
 
 def cpu_one_thread_builds(seconds: float) -> dict:
-    """The oracle's C2 Fock builds on one host thread (a child process with
-    OMP_NUM_THREADS=1: the thread count is fixed when OpenMP starts)."""
+    """The oracle's 96 x 96 x 8 Fock builds on one host thread (a child
+    process with OMP_NUM_THREADS=1: the thread count is fixed when OpenMP
+    starts)."""
     import subprocess
 
     code = (
         "import sys, time, json, numpy as np; sys.path.insert(0, %r)\n"
-        "from cpp_hf_amd.workloads import config2\n"
+        "from cpp_hf_amd.workloads import layered_nb8\n"
         "from cpp_hf_amd.solver import _kernel_args_for_native\n"
         "from oracle import hfcpu\n"
-        "wl = config2(96); ka = _kernel_args_for_native(wl.kernel)\n"
+        "wl = layered_nb8(96); ka = _kernel_args_for_native(wl.kernel)\n"
         "P = 0.5 * (wl.P0 + np.conj(np.swapaxes(wl.P0, -1, -2)))\n"
         "hfcpu.build_fock_apply(ka, P); n = 0; t = time.perf_counter()\n"
         "while time.perf_counter() - t < %r:\n"
@@ -623,13 +626,13 @@ def main() -> int:
 
     from cpp_hf_amd._compat import _native, native_required
     from cpp_hf_amd.solver import _kernel_args_for_native
-    from cpp_hf_amd.workloads import config2
+    from cpp_hf_amd.workloads import layered_nb8
 
     native_required()
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
 
-    wl = config2(args.n)
+    wl = layered_nb8(args.n)
     kern = wl.kernel
     ka = _kernel_args_for_native(kern)
     nk1, nk2, nb = kern.h.shape[0], kern.h.shape[1], kern.h.shape[-1]

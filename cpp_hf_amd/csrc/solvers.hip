@@ -57,9 +57,9 @@ struct Projector {
 // hipGraphs with HFGPU_SOLVER_GRAPH=1: one host call per iteration instead of
 // one per kernel.  Off by default: with the next iteration enqueued ahead of
 // the host's tests the launches are off the critical path, and a replay
-// measured slower than the same kernels launched one by one (C1 direct
-// minimisation 3.0 against 2.4 ms, C2 14.5 against 13.8 ms; the 48-solve
-// sweep unchanged).  The captured work allocates nothing and clears memory
+// measured slower than the same kernels launched one by one (direct
+// minimisation of the 24 x 24 QWZ model 3.0 against 2.4 ms, of the 96 x 96
+// layered model 14.5 against 13.8 ms; the 48-solve sweep unchanged).  The captured work allocates nothing and clears memory
 // only with fill kernels (no memset nodes).
 bool solver_graphs_on() {
     const char* e = std::getenv("HFGPU_SOLVER_GRAPH");  // read per solve

@@ -161,8 +161,9 @@ def layered_tangents(n: int = 96, n_layers: int = 4, *, mass: float = 0.5, u: fl
     }
 
 
-def config2(n: int = 96) -> Workload:
-    """8 bands (4 layers), T = 0.05, 4.1 electrons per k, Hartree on."""
+def layered_nb8(n: int = 96) -> Workload:
+    """The layered model with 8 bands (4 layers), T = 0.05, 4.1 electrons per
+    k, Hartree on (the 96 x 96 Fock-build and solve workload)."""
     return layered_kernel(n, 4, T=0.05, filling=4.1)
 
 

@@ -111,9 +111,9 @@ def test_qwz24_scf_matches_oracle(gpu, oracle):
 
 @pytest.fixture(scope="module")
 def c2():
-    from cpp_hf_amd.workloads import config2
+    from cpp_hf_amd.workloads import layered_nb8
 
-    return config2(96)
+    return layered_nb8(96)
 
 
 def test_layered96_dm_solve_matches_oracle(gpu, oracle, c2):
