@@ -1581,6 +1581,13 @@ int hfgpu_solve_dm_batch(hfgpu_kernel* const* ks, size_t n, const double* const*
         if (!ks || !P0s || !n_es || !cfg || !outs) throw std::invalid_argument("null argument");
         if (n < 1 || n > static_cast<size_t>(kMaxBatch))
             throw std::invalid_argument("hfgpu_solve_dm_batch takes 1 to 32 solves");
+        // each solve builds in its kernel's workspaces, so one handle cannot
+        // serve two solves of a batch
+        for (size_t i = 0; i < n; ++i)
+            for (size_t j = 0; j < i; ++j)
+                if (ks[i] && ks[i] == ks[j])
+                    throw std::invalid_argument(
+                        "hfgpu_solve_dm_batch: a kernel handle appears twice in the batch");
         std::vector<GpuKernel*> Ks(n);
         std::vector<const double*> P0(n);
         std::vector<double> ne(n);

@@ -258,7 +258,8 @@ int hfgpu_solve_scf(hfgpu_kernel* k, const double* P0, double n_e, const hf_scf_
 int hfgpu_solve_dm(hfgpu_kernel* k, const double* P0, double n_e, const hf_dm_config* cfg,
                    hf_project_fn project, void* user, hf_dm_result* out);
 
-/* n (1..32) direct-minimisation solves advanced together, one per kernel:
+/* n (1..32) direct-minimisation solves advanced together, one per kernel
+ * (n distinct handles: a handle appearing twice is HF_EINVAL):
  * every kernel with the same band count and temperature (meshes may
  * differ), one shared configuration (CG optimizer; no blocks, symmetry,
  * deflation or FP32 builds).  Each launch of an iteration covers all solves

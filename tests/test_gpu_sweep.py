@@ -163,3 +163,33 @@ def test_grouped_band_counts_and_build_kinds_equal_single_solves(gpu, case):
     single = [solve(k, P, n, config=cfg) for k, P, n in zip(ks, P0s, nes)]
     for a, b in zip(grouped, single):
         _assert_same_dm(a, b)
+
+
+def test_grouped_solve_rejects_a_repeated_handle(gpu, oracle):
+    """Through the C ABI (ctypes): one kernel handle passed twice in a batch
+    is refused before any work, since both solves would build in the one
+    handle's workspaces."""
+    import ctypes as C
+    from pathlib import Path
+
+    from cpp_hf_amd.solver import _kernel_args_for_native
+    from cpp_hf_amd.workloads import layered_kernel
+
+    lib = C.CDLL(str(Path(__file__).resolve().parents[1] / "cpp_hf_amd" / "libhfgpu.so"))
+    lib.hfgpu_last_error.restype = C.c_char_p
+    wl = layered_kernel(12, 1, T=0.05)
+    desc = oracle._Desc(_kernel_args_for_native(wl.kernel))
+    handle = C.c_void_p()
+    assert lib.hfgpu_kernel_create(C.byref(desc.desc), C.byref(handle)) == 0
+    try:
+        P0 = np.ascontiguousarray(wl.P0, dtype=np.complex128)
+        ks = (C.c_void_p * 2)(handle, handle)
+        p0s = (C.c_void_p * 2)(P0.ctypes.data, P0.ctypes.data)
+        nes = (C.c_double * 2)(wl.n_electrons, wl.n_electrons)
+        cfg = C.create_string_buffer(1024)
+        outs = C.create_string_buffer(2048)
+        rc = lib.hfgpu_solve_dm_batch(ks, C.c_size_t(2), p0s, nes, cfg, outs)
+        assert rc != 0
+        assert b"twice" in lib.hfgpu_last_error()
+    finally:
+        lib.hfgpu_kernel_destroy(handle)
