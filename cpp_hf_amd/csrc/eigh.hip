@@ -254,8 +254,12 @@ __device__ __forceinline__ cd group_bcast(cd v, int src) {
     return cd{__shfl(v.x, src, 64), __shfl(v.y, src, 64)};
 }
 
+// Two wavefronts per SIMD for nb = 12 (its register rows need 279 registers
+// at one, a few spills outside the QR loop at two): 9216 matrices 309 -> 240
+// us, 14400 418 -> 315 us on MI355X.  Three (8 and 12 bands) and two for the
+// 16-lane generic rows (heavy spills) measured slower.
 template <int NP, int NBC>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NBC == 12 ? 2 : 1)))
 k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __restrict__ w_out,
            cd* __restrict__ V_out, BlockMap bm, int sort, const double* __restrict__ cond,
            int mpw, const int* __restrict__ seg, unsigned active) {
