@@ -31,25 +31,83 @@ __device__ __forceinline__ cd cayley_c(double lam, double tau) {
     return cd{(1.0 - a * a) / den, (a + a) / den};
 }
 
-// C = op(A) op(B) on nb x nb LDS blocks by one wavefront.  With tb the
-// lanes of a row start the sum at n = j (the terms in rotated order), so
-// their reads of B's rows B[j][n] fall on distinct LDS banks instead of a
-// stride of nb complex (all one bank group at nb = 16).
-__device__ __forceinline__ void wave_mm(const cd* A, const cd* B, cd* C, int nb, bool ta,
-                                        bool tb, int lane) {
+// Orders the LDS accesses of one wavefront around it (a lane may then read
+// what another lane of its wavefront wrote before it).  The LDS performs the
+// accesses of one wavefront in order, so this costs no instruction: the
+// fences only keep the compiler from moving accesses across it.  The per-k
+// kernels keep each wavefront's blocks in its own LDS region and
+// synchronise with it instead of the workgroup barrier, so the wavefronts of
+// a workgroup (different k points) do not wait for each other.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Elements e = base + lane + 64 t (t < T) of C = op(A) op(B), nb x nb LDS
+// blocks, each summed over n in its own fixed order and handed to
+// store(e, value).  The T sums of a lane are independent chains run side by
+// side (latency hidden within the lane); lanes past nb^2 compute on row 0
+// and store nothing.  With tb the sum for (i, j) starts at n = j (the terms
+// in rotated order), so the lanes of a row read B's rows B[j][n] on distinct
+// LDS banks instead of a stride of nb complex (one bank group at nb = 16).
+template <int T, class Store>
+__device__ __forceinline__ void wave_mm_part(const cd* A, const cd* B, int nb, bool ta, bool tb,
+                                             int lane, int base, Store&& store) {
     const int nb2 = nb * nb;
-    for (int e = lane; e < nb2; e += 64) {
-        const int i = e / nb, j = e % nb;
-        cd s{0.0, 0.0};
-        int n = tb ? j : 0;
-        for (int t = 0; t < nb; ++t) {
+    int ii[T], jj[T], nn[T];
+    cd s[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int e = base + lane + 64 * t;
+        const bool ok = e < nb2;
+        ii[t] = ok ? e / nb : 0;
+        jj[t] = ok ? e % nb : 0;
+        nn[t] = tb ? jj[t] : 0;
+        s[t] = cd{0.0, 0.0};
+    }
+    // (unrolled by two only: a full unroll at a compile-time nb hoists every
+    // LDS load and costs occupancy)
+#pragma unroll 2
+    for (int q = 0; q < nb; ++q) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int i = ii[t], j = jj[t], n = nn[t];
             const cd a = ta ? cconj(A[n * nb + i]) : A[i * nb + n];
             const cd b = tb ? cconj(B[j * nb + n]) : B[n * nb + j];
-            s = cadd(s, cmul(a, b));
-            n = n + 1 == nb ? 0 : n + 1;
+            s[t] = cadd(s[t], cmul(a, b));
+            nn[t] = n + 1 == nb ? 0 : n + 1;
         }
-        C[e] = s;
     }
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int e = base + lane + 64 * t;
+        if (e < nb2) store(e, s[t]);
+    }
+}
+
+// store(e, C_e) for every element of C = op(A) op(B) by one wavefront: up to
+// four elements per lane at a time.
+template <class Store>
+__device__ __forceinline__ void wave_mm_each(const cd* A, const cd* B, int nb, bool ta, bool tb,
+                                             int lane, Store&& store) {
+    const int nb2 = nb * nb;
+    if (nb2 <= 64) {
+        wave_mm_part<1>(A, B, nb, ta, tb, lane, 0, store);
+    } else if (nb2 <= 128) {
+        wave_mm_part<2>(A, B, nb, ta, tb, lane, 0, store);
+    } else if (nb2 <= 192) {
+        wave_mm_part<3>(A, B, nb, ta, tb, lane, 0, store);
+    } else {
+        for (int base = 0; base < nb2; base += 256)
+            wave_mm_part<4>(A, B, nb, ta, tb, lane, base, store);
+    }
+}
+
+// C = op(A) op(B) on nb x nb LDS blocks by one wavefront.
+__device__ __forceinline__ void wave_mm(const cd* A, const cd* B, cd* C, int nb, bool ta,
+                                        bool tb, int lane) {
+    wave_mm_each(A, B, nb, ta, tb, lane, [C](int e, cd v) { C[e] = v; });
 }
 
 // Sum NS per-thread values over the block and write partial[block * NS + s].

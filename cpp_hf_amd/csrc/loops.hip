@@ -138,11 +138,11 @@ k_orbital_grad(int nk, int nb_arg, const cd* __restrict__ Q, cd* __restrict__ F,
                 sQ[e] = Q[kb + e];
                 sF[e] = F[kb + e];
             }
-        __syncthreads();
+        wave_lds_sync();
         if (act) wave_mm(sQ, sF, sT, nb, true, false, lane);
-        __syncthreads();
+        wave_lds_sync();
         if (act) wave_mm(sT, sQ, sF, nb, false, false, lane);
-        __syncthreads();
+        wave_lds_sync();
         if (act) {
             for (int e = lane; e < nb2; e += 64) F[kb + e] = sF[e];
             const double* pk = p + static_cast<size_t>(k) * nb;
@@ -172,7 +172,7 @@ k_orbital_grad(int nk, int nb_arg, const cd* __restrict__ Q, cd* __restrict__ F,
                 acc[0] += wk * gs;
             }
         }
-        __syncthreads();
+        wave_lds_sync();
     }
     if (!grad) return;
     store_block_partials<2, W>(acc, red, partial);
@@ -324,20 +324,17 @@ __device__ __forceinline__ void diag_at_tau(const cd* sV, const cd* sFe, const d
                                             int lane) {
     const int nb2 = nb * nb;
     for (int n = lane; n < nb; n += 64) sc_[n] = cayley_c(slam[n], tau);
-    __syncthreads();
+    wave_lds_sync();
     for (int e = lane; e < nb2; e += 64) {
         const int n = e % nb;
         sVC[e] = cmulc(sV[e], sc_[n]);  // V_in conj(c_n)
     }
-    __syncthreads();
-    for (int e = lane; e < nb2; e += 64) {
-        const int i = e / nb, j = e % nb;
-        cd a{0.0, 0.0};
-        for (int n = 0; n < nb; ++n) a = cadd(a, cmul(sVC[i * nb + n], sFe[n * nb + j]));
-        a = cmul(a, sc_[j]);
+    wave_lds_sync();
+    wave_mm_each(sVC, sFe, nb, false, false, lane, [&](int e, cd a) {
+        a = cmul(a, sc_[e % nb]);
         srow[e] = cmulc(a, sV[e]).x;
-    }
-    __syncthreads();
+    });
+    wave_lds_sync();
 }
 
 // Backtracking trials j in [j_lo, j_hi) of the spectral line search
@@ -401,12 +398,12 @@ k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restr
             }
             for (int i = lane; i < nb; i += 64) slam[i] = lam[static_cast<size_t>(k) * nb + i];
         }
-        __syncthreads();
+        wave_lds_sync();
         if (!stage2) {
             if (act) wave_mm(sV, sF, sT, nb, true, false, lane);
-            __syncthreads();
+            wave_lds_sync();
             if (act) wave_mm(sT, sV, sF, nb, false, false, lane);  // sF <- Fe
-            __syncthreads();
+            wave_lds_sync();
             if (act)
                 for (int e = lane; e < nb2; e += 64) Fe[kb + e] = sF[e];
         }
@@ -431,7 +428,7 @@ k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restr
                 racc[wave][2 * j] += ej;
                 racc[wave][2 * j + 1] += sj;
             }
-            __syncthreads();
+            wave_lds_sync();
             tau *= shrink;
         }
     }
@@ -496,33 +493,29 @@ k_retract(int nk, int nb_arg, const cd* Q, const cd* __restrict__ V,
                 scf[i] = cayley ? cayley_c(slam[i], tau) : cd{1.0, 0.0};
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         if (act) {
             if (cayley) {
                 // U = (V diag(c)) V^H
                 for (int e = lane; e < nb2; e += 64) sVC[e] = cmul(sV[e], scf[e % nb]);
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         if (act) {
             if (cayley) wave_mm(sVC, sV, sU, nb, false, true, lane);
         }
-        __syncthreads();
+        wave_lds_sync();
         if (act) {
             // Q_new = Q U (cayley) or Q V (finalise)
-            for (int e = lane; e < nb2; e += 64) {
-                const int i = e / nb, j = e % nb;
-                cd s{0.0, 0.0};
-                const cd* R = cayley ? sU : sV;
-                for (int n = 0; n < nb; ++n) s = cadd(s, cmul(sQ[i * nb + n], R[n * nb + j]));
-                Q_out[kb + e] = s;
-            }
+            const cd* R = cayley ? sU : sV;
+            cd* Qk = Q_out + kb;
+            wave_mm_each(sQ, R, nb, false, false, lane, [Qk](int e, cd v) { Qk[e] = v; });
         }
-        __syncthreads();
+        wave_lds_sync();
         if (cayley) {
             if (act)
                 for (int e = lane; e < nb2; e += 64) sFe[e] = Fe[kb + e];
-            __syncthreads();
+            wave_lds_sync();
             diag_at_tau(sV, sFe, slam, sVC, scf, srow, nb, tau, lane);
             if (act)
                 for (int i = lane; i < nb; i += 64) {
@@ -530,7 +523,7 @@ k_retract(int nk, int nb_arg, const cd* Q, const cd* __restrict__ V,
                     for (int jj = 0; jj < nb; ++jj) di += srow[i * nb + jj];
                     eps_work[static_cast<size_t>(k) * nb + i] = di;
                 }
-            __syncthreads();
+            wave_lds_sync();
         }
     }
 }
@@ -576,7 +569,7 @@ k_scf_residuals(int nk, int nb_arg, const cd* __restrict__ F, const cd* __restri
                 sF[e] = F[kb + e];
                 sD[e] = D[kb + e];
             }
-        __syncthreads();
+        wave_lds_sync();
         if (act) {
             const double wk = w[k];
             double sd = 0.0, scm = 0.0;
@@ -597,7 +590,7 @@ k_scf_residuals(int nk, int nb_arg, const cd* __restrict__ F, const cd* __restri
             acc[0] += wk * sd;
             acc[1] += wk * scm;
         }
-        __syncthreads();
+        wave_lds_sync();
     }
     store_block_partials<2, W>(acc, red, partial);
 }
