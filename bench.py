@@ -740,6 +740,9 @@ def main() -> int:
     }
 
     out["autotune"] = {"default_us_per_build": default_us, **tuned}
+    # matrices per wavefront of the solvers' eigh at this kernel's nk x nb
+    # (results do not depend on it)
+    out["autotune"]["eigh"] = {"nmat": nk, "nb": nb, **_native.eigh_autotune(nk, nb)}
 
     if not args.no_solves:
         out["solves"] = run_solves(not args.no_cpu_baseline)

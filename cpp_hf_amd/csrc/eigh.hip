@@ -19,8 +19,12 @@
 // off-diagonal mass falls to (4 nb eps)^2 ||A||_F^2.
 #include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 #include <stdexcept>
+#include <utility>
 
+#include "hfgpu_core.h"
 #include "kernels.h"
 
 namespace hfgpu {
@@ -497,21 +501,33 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
 
 }  // namespace
 
-// Matrices per wavefront of the register kernels: a full wavefront (g).
-// Fewer leave lane groups idle to put more wavefronts on each SIMD; that
-// measured slower (96 x 96 x 8: 72 us at 8 per wavefront, 99 us at 4, 151 us
-// at 2): the kernel is bound by VALU issue, not by the latency of its serial
-// recurrences.  HFGPU_EIGH_MPW overrides for experiments.
-static int eigh_mpw(int nmat, int g) {
-    (void)nmat;
+// Matrices per wavefront of the register kernels: a full wavefront (g) unless
+// the autotuner below kept another count for this (nb, nmat).  Fewer leave
+// lane groups idle to put more wavefronts on each SIMD; that measured slower
+// (96 x 96 x 8: 72 us at 8 per wavefront, 99 us at 4, 151 us at 2): the
+// kernel is bound by VALU issue, not by the latency of its serial
+// recurrences.  HFGPU_EIGH_MPW overrides both.  The count only decides which
+// lane groups hold a matrix, so results do not depend on it.
+namespace {
+std::mutex g_mpw_mu;
+std::map<std::pair<int, int>, int> g_mpw;  // (nb, nmat) -> matrices per wavefront
+}  // namespace
+
+static int pow2_at_most(int want, int g) {
+    int p = 1;
+    while (p < want && p < g) p <<= 1;
+    return std::min(p, g);
+}
+
+static int eigh_mpw(int nmat, int nb, int g) {
     static const int env = [] {
         const char* e = std::getenv("HFGPU_EIGH_MPW");
         return e ? std::atoi(e) : 0;
     }();
-    if (env <= 0) return g;
-    int p = 1;
-    while (p < env && p < g) p <<= 1;
-    return std::min(p, g);
+    if (env > 0) return pow2_at_most(env, g);
+    std::lock_guard<std::mutex> lock(g_mpw_mu);
+    const auto it = g_mpw.find({nb, nmat});
+    return it == g_mpw.end() ? g : pow2_at_most(it->second, g);
 }
 
 // The eigh launches: nmat matrices, or with seg the S solves' matrices
@@ -519,7 +535,7 @@ static int eigh_mpw(int nmat, int g) {
 // active is clear are skipped).
 static void eigh_launch(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V,
                         const BlockMap& bm, bool sort, hipStream_t st, const double* cond,
-                        const int* seg, int S, unsigned active) {
+                        const int* seg, int S, unsigned active, int mpw_force = 0) {
     if (nb < 1 || nb > kMaxNb)
         throw std::invalid_argument("GPU eigh supports 1 <= nb <= 32");
     if (nmat == 0) return;
@@ -527,7 +543,7 @@ static void eigh_launch(int nmat, int nb, int in_mode, const cd* M, double* w, c
     const unsigned gy = seg ? static_cast<unsigned>(S) : 1u;
     if (nb <= 16) {
         const int np = nb <= 2 ? 2 : (nb <= 4 ? 4 : (nb <= 8 ? 8 : 16));
-        const int mpw = eigh_mpw(nmat, 64 / np);
+        const int mpw = mpw_force > 0 ? pow2_at_most(mpw_force, 64 / np) : eigh_mpw(nmat, nb, 64 / np);
         const int per_block = 4 * mpw;
         const dim3 grid(cdiv(nmat, per_block), gy);
         if (np == 2)
@@ -583,6 +599,111 @@ void launch_eigh_seg(const int* seg, int S, int nk_max, unsigned active, int nb,
                      const cd* M, double* w, cd* V, hipStream_t st) {
     BlockMap bm{};
     eigh_launch(nk_max, nb, in_mode, M, w, V, bm, true, st, nullptr, seg, S, active);
+}
+
+namespace {
+
+// Hermitian test matrices for the autotuner: entry (a, b), a <= b, of matrix
+// k from an integer hash (uniform in [-1, 1)), the lower triangle its
+// conjugate, a real diagonal.
+__device__ __forceinline__ double hash_unit(unsigned long long x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return static_cast<double>(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+__global__ void k_fill_hermitian(size_t n, int nb, cd* __restrict__ M) {
+    const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const size_t nb2 = static_cast<size_t>(nb) * nb;
+    const size_t k = t / nb2;
+    const int e = static_cast<int>(t % nb2), i = e / nb, j = e % nb;
+    const int a = min(i, j), b = max(i, j);
+    const unsigned long long key = (k * 64 + a) * 64 + b;
+    const double re = hash_unit(2 * key), im = a == b ? 0.0 : hash_unit(2 * key + 1);
+    M[t] = cd{re, i <= j ? im : -im};
+}
+
+}  // namespace
+
+double eigh_autotune(int nmat, int nb, hipStream_t st, int* mpw_out) {
+    if (nb < 1 || nb > kMaxNb) throw std::invalid_argument("GPU eigh supports 1 <= nb <= 32");
+    if (nmat < 1) throw std::invalid_argument("eigh autotune needs at least one matrix");
+    if (nb > 16) {  // the 17..32-band kernel: two matrices per wavefront, fixed
+        *mpw_out = 2;
+        return 0.0;
+    }
+    const int np = nb <= 2 ? 2 : (nb <= 4 ? 4 : (nb <= 8 ? 8 : 16));
+    const int g = 64 / np;
+    const size_t nd = static_cast<size_t>(nmat) * nb * nb;
+    DevBuf M(nd * sizeof(cd)), V(nd * sizeof(cd)),
+        w(static_cast<size_t>(nmat) * nb * sizeof(double));
+    hipLaunchKernelGGL(k_fill_hermitian, dim3(static_cast<unsigned>(cdiv(nd, 256))), dim3(256), 0,
+                       st, nd, nb, M.as<cd>());
+    HFGPU_LAUNCH_CHECK();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HFGPU_CHECK(hipEventCreate(&e0));
+    HFGPU_CHECK(hipEventCreate(&e1));
+    struct EventGuard {
+        hipEvent_t a, b;
+        hipStream_t s;
+        ~EventGuard() {
+            (void)hipStreamSynchronize(s);
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{e0, e1, st};
+    BlockMap bm{};
+    int reps = 3;
+    auto time_mpw = [&](int m) {
+        for (int i = 0; i < 2; ++i)
+            eigh_launch(nmat, nb, 0, M.as<cd>(), w.as<double>(), V.as<cd>(), bm, true, st,
+                        nullptr, nullptr, 1, ~0u, m);
+        float best = 1e30f;
+        for (int t = 0; t < 3; ++t) {
+            HFGPU_CHECK(hipEventRecord(e0, st));
+            for (int i = 0; i < reps; ++i)
+                eigh_launch(nmat, nb, 0, M.as<cd>(), w.as<double>(), V.as<cd>(), bm, true, st,
+                            nullptr, nullptr, 1, ~0u, m);
+            HFGPU_CHECK(hipEventRecord(e1, st));
+            HFGPU_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HFGPU_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, ms);
+        }
+        return 1e3 * static_cast<double>(best) / reps;
+    };
+    int best_m = g;
+    double best_us = time_mpw(g);
+    // about 1 ms of timed calls per candidate
+    reps = std::max(3, std::min(30, static_cast<int>(1000.0 / std::max(best_us, 1.0))));
+    best_us = time_mpw(g);
+    for (int m = g / 2; m >= 1 && m >= g / 8; m /= 2) {
+        const double us = time_mpw(m);
+        if (us < best_us) {
+            best_us = us;
+            best_m = m;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lock(g_mpw_mu);
+        g_mpw[{nb, nmat}] = best_m;
+    }
+    *mpw_out = best_m;
+    return best_us;
+}
+
+void eigh_autotune_once(int nmat, int nb, hipStream_t st) {
+    if (nmat < 1 || nb < 1 || nb > 16) return;
+    {
+        std::lock_guard<std::mutex> lock(g_mpw_mu);
+        if (g_mpw.count({nb, nmat})) return;
+    }
+    int m = 0;
+    (void)eigh_autotune(nmat, nb, st, &m);
 }
 
 namespace {

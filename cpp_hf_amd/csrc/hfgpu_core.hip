@@ -1,6 +1,7 @@
 // libhfgpu core: device-resident HF kernel, Fock build orchestration and the
 // C ABI entry points that are not solver loops.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -469,6 +470,8 @@ GpuKernel::Geom& GpuKernel::geom(bool packed) {
         }
         HFGPU_CHECK(hipStreamSynchronize(stream));
         G.g = std::move(g);
+        // the solvers' eigh of this kernel's nk matrices (matrices per wavefront)
+        if (autotune_env) eigh_autotune_once(static_cast<int>(nk), nb, stream);
         if (autotune_env && exchange && !superlattice() && packed == packed_for(packed)) {
             const TuneKey key{nk1, nk2, nb, packed, dv > 1, hartree || contact};
             bool cached = false;
@@ -1004,6 +1007,17 @@ int hfgpu_kernel_autotune(hfgpu_kernel* k, int p_hermitian, int* shape, double* 
     return guarded([&] {
         if (!k || !shape || !best_us) throw std::invalid_argument("null argument");
         *best_us = k->impl->autotune(p_hermitian != 0, shape);
+    });
+}
+
+int hfgpu_eigh_autotune(size_t nmat, size_t nb, int* mpw, double* best_us) {
+    return guarded([&] {
+        if (!mpw || !best_us) throw std::invalid_argument("null argument");
+        if (nmat == 0 || nmat > static_cast<size_t>(INT_MAX))
+            throw std::invalid_argument("hfgpu_eigh_autotune: nmat must be in 1 .. 2^31 - 1");
+        Stream st;
+        *best_us = eigh_autotune(static_cast<int>(nmat), static_cast<int>(nb), st.s, mpw);
+        st.sync();
     });
 }
 

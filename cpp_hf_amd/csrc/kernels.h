@@ -32,6 +32,15 @@ struct BlockMap {
 void launch_eigh_blocks(int nmat, int nb, int in_mode, const cd* M, double* w, cd* V,
                         const BlockMap& bm, bool sort, hipStream_t st,
                         const double* cond = nullptr);
+// Matrices-per-wavefront autotuner of the nb <= 16 eigh kernels: times nmat
+// Hermitian test matrices of nb bands with 64/NP, 32/NP, ... matrices per
+// wavefront (down to an eighth), keeps the fastest for (nb, nmat) in the
+// process and returns microseconds per call; results of later calls do not
+// change.  nb 17..32: the fixed two per wavefront, 0 us.
+double eigh_autotune(int nmat, int nb, hipStream_t st, int* mpw_out);
+// eigh_autotune unless (nb, nmat) is already tuned in this process (nb <= 16).
+void eigh_autotune_once(int nmat, int nb, hipStream_t st);
+
 // out[0] = max |M_ij| over the off-block entries of all nmat matrices
 // (linalg.hpp:123-140, max_offblock_sizes), out[1] = max |M_ij| over all
 // entries (linalg.hpp:142-146, max_abs); partial needs

@@ -441,6 +441,25 @@ PYBIND11_MODULE(_native, m) {
         },
         py::arg("M"));
 
+    // matrices-per-wavefront autotuner of the batched eigh (an extension)
+    m.def(
+        "eigh_autotune",
+        [](size_t nmat, size_t nb) {
+            int mpw = 0;
+            double us = 0.0;
+            int rc;
+            {
+                py::gil_scoped_release rel;
+                rc = hfgpu_eigh_autotune(nmat, nb, &mpw, &us);
+            }
+            check(rc);
+            py::dict d;
+            d["matrices_per_wavefront"] = mpw;
+            d["us_per_call"] = us;
+            return d;
+        },
+        py::arg("nmat"), py::arg("nb"));
+
     // --- block-diagonal eigh (cpp_hf_native.cpp:314-358) ---
     m.def(
         "eigh_block_sizes",

@@ -64,6 +64,15 @@ int hfgpu_kernel_packable(const hfgpu_kernel* k);
  * column and inverse-row passes); best_us: microseconds per build (0 when
  * the mesh has no two-step transform).  Results of builds do not change. */
 int hfgpu_kernel_autotune(hfgpu_kernel* k, int p_hermitian, int* shape, double* best_us);
+/* Matrices-per-wavefront autotuner of the batched eigh (no counterpart in the
+ * reference): times nmat Hermitian test matrices of nb bands (1..16) with
+ * full and with half, quarter and eighth wavefronts of matrices, keeps the
+ * fastest for later eigh calls of that (nb, nmat) in the process (the
+ * solvers' included; HFGPU_AUTOTUNE=1 does this for every new kernel's nk),
+ * mpw: matrices per wavefront kept, best_us: microseconds per call.  nb
+ * 17..32: the fixed two per wavefront, 0 us.  Results of later calls do not
+ * change. */
+int hfgpu_eigh_autotune(size_t nmat, size_t nb, int* mpw, double* best_us);
 
 /* ------------------------------------------------------------------------
  * Exchange self-energy.

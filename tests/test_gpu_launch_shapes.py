@@ -82,3 +82,67 @@ def test_autotune_keeps_results(gpu, oracle, monkeypatch):
     assert before[3] == after[3]
     Fr = oracle.build_fock_apply(ka, P)[2]
     assert float(np.max(np.abs(after[2] - Fr))) / float(np.max(np.abs(Fr))) < 1e-12
+
+
+_EIGH_CHILD = r"""
+import hashlib, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from cpp_hf_amd._compat import _native
+rng = np.random.default_rng(5)
+for nb, nmat in ((4, 700), (8, 500), (12, 300), (16, 200)):
+    X = rng.standard_normal((nmat, nb, nb)) + 1j * rng.standard_normal((nmat, nb, nb))
+    M = 0.5 * (X + np.conj(np.swapaxes(X, 1, 2)))
+    w, V = _native.eigh_batched(M)
+    print(nb, hashlib.sha1(w.tobytes() + V.tobytes()).hexdigest(), flush=True)
+"""
+
+
+def test_eigh_matrices_per_wavefront_keep_results(gpu):
+    """The eigh with 4, 2 or 1 matrices per wavefront (HFGPU_EIGH_MPW: the
+    autotuner's candidates) gives the full-wavefront results bit for bit, at
+    4, 8, 12 and 16 bands (one child process per count, the variable being
+    read once per process)."""
+    import os
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = str(Path(__file__).resolve().parents[1])
+    outs = []
+    for mpw in (None, "4", "2", "1"):
+        env = dict(os.environ)
+        env.pop("HFGPU_EIGH_MPW", None)
+        if mpw:
+            env["HFGPU_EIGH_MPW"] = mpw
+        print(f"eigh child HFGPU_EIGH_MPW={mpw}", flush=True)
+        r = subprocess.run([sys.executable, "-c", _EIGH_CHILD, root], env=env,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr[-2000:]
+        outs.append(r.stdout)
+    assert len(outs[0].splitlines()) == 4
+    assert all(o == outs[0] for o in outs[1:])
+
+
+@pytest.mark.parametrize("nb,nmat", [(4, 3000), (12, 1200), (20, 64)])
+def test_eigh_autotune(gpu, nb, nmat):
+    """The matrices-per-wavefront autotuner returns one of its candidates and
+    later calls of that shape give the same results."""
+    rng = np.random.default_rng(nb)
+    M = herm(rand_c(rng, (nmat, nb, nb)))
+    w0, V0 = gpu.eigh_batched(M)
+    r = gpu.eigh_autotune(nmat, nb)
+    print(f"eigh autotune nb={nb} nmat={nmat}: {r}", flush=True)
+    if nb > 16:
+        assert r["matrices_per_wavefront"] == 2 and r["us_per_call"] == 0.0
+    else:
+        g = 64 // (4 if nb <= 4 else 16)
+        assert r["matrices_per_wavefront"] in (g, g // 2, g // 4, g // 8)
+        assert r["us_per_call"] > 0.0
+    w1, V1 = gpu.eigh_batched(M)
+    np.testing.assert_array_equal(w0, w1)
+    np.testing.assert_array_equal(V0, V1)
+    with pytest.raises(ValueError):
+        gpu.eigh_autotune(0, nb)
+    with pytest.raises(ValueError):
+        gpu.eigh_autotune(nmat, 33)
