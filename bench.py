@@ -592,15 +592,16 @@ def main() -> int:
         from cpp_hf_amd._compat import _native
 
         rng = np.random.default_rng(0)
-        for n, nb, n_e in ((96, 8, 4.1), (192, 16, 8.1)):
+        for n, nb, n_e in ((96, 8, 4.1), (96, 12, 6.1), (120, 12, 6.1), (192, 16, 8.1)):
             X = rng.standard_normal((n, n, nb, nb)) + 1j * rng.standard_normal((n, n, nb, nb))
             F = 0.5 * (X + np.conj(np.swapaxes(X, -1, -2)))
             w = np.full((n, n), 1.0 / (n * n))
             for _ in range(5):
                 _native.eigh_batched(F)
                 _native.density_from_fock(F, w, n_e, 0.05, 0)
-        print(json.dumps({"kernel_sample": "eigh and density_from_fock at 96x96x8 and "
-                                           "192x192x16, 5 calls each"}), flush=True)
+        print(json.dumps({"kernel_sample": "eigh and density_from_fock at 96x96x8, "
+                                           "96x96x12, 120x120x12 and 192x192x16, "
+                                           "5 calls each"}), flush=True)
         return 0
 
     if args.copy_calibration:
