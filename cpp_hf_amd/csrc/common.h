@@ -86,6 +86,19 @@ struct DeviceError : std::runtime_error {
 void* dev_alloc(size_t bytes);
 void dev_free(void* p, size_t bytes);
 
+// Non-blocking HIP streams (default priority) from a per-device pool:
+// creating and destroying a stream costs ~100 + 60 us of host time, which
+// was a large share of a small solve's set-up (about ten per solve).
+// stream_acquire takes a stream of the current device; stream_release takes
+// back an idle stream, one the caller has synchronised without error (a
+// stream whose work failed is destroyed instead).  Pooled streams live until
+// the process ends.  Thread-safe.
+hipStream_t stream_acquire();
+void stream_release(hipStream_t s);
+// Synchronise s, then return it to the pool (or destroy it if the wait
+// failed); the wait's status is returned.
+hipError_t stream_sync_release(hipStream_t s);
+
 inline void hip_check(hipError_t e, const char* what, const char* file, int line) {
     if (e != hipSuccess)
         throw DeviceError(std::string(what) + " failed: " + hipGetErrorString(e) + " (" +

@@ -45,16 +45,15 @@ FftAxis AxisPlan::axis() const {
     return a;
 }
 
-// Synchronous upload on a private non-blocking stream, never the null
-// stream: a null-stream copy orders against the process's other streams,
-// which breaks hipGraph captures running in other host threads.
+// Synchronous upload on a private non-blocking (pooled) stream, never the
+// null stream: a null-stream copy orders against the process's other
+// streams, which breaks hipGraph captures running in other host threads.
 static void upload_now(void* dst, const void* src, size_t bytes) {
-    hipStream_t s = nullptr;
-    HFGPU_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipStreamDestroy(s);
+    hipStream_t s = stream_acquire();
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    const hipError_t w = stream_sync_release(s);
     HFGPU_CHECK(e);
+    HFGPU_CHECK(w);
 }
 
 void AxisPlan::init(int n_) {

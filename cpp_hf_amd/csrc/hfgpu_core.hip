@@ -150,6 +150,60 @@ void* dev_alloc(size_t bytes) {
     return p;
 }
 
+namespace {
+
+struct StreamPool {
+    std::mutex mu;
+    std::map<int, std::vector<hipStream_t>> idle;  // device -> streams
+};
+
+StreamPool& stream_pool() {
+    static StreamPool* p = new StreamPool;  // streams outlive static destructors
+    return *p;
+}
+
+}  // namespace
+
+hipStream_t stream_acquire() {
+    int dev = 0;
+    HFGPU_CHECK(hipGetDevice(&dev));
+    StreamPool& pool = stream_pool();
+    {
+        std::lock_guard<std::mutex> lock(pool.mu);
+        auto it = pool.idle.find(dev);
+        if (it != pool.idle.end() && !it->second.empty()) {
+            hipStream_t s = it->second.back();
+            it->second.pop_back();
+            return s;
+        }
+    }
+    hipStream_t s = nullptr;
+    HFGPU_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+void stream_release(hipStream_t s) {
+    if (!s) return;
+    int dev = 0;
+    if (hipStreamGetDevice(s, &dev) != hipSuccess) {
+        (void)hipStreamDestroy(s);
+        return;
+    }
+    StreamPool& pool = stream_pool();
+    std::lock_guard<std::mutex> lock(pool.mu);
+    pool.idle[dev].push_back(s);
+}
+
+hipError_t stream_sync_release(hipStream_t s) {
+    if (!s) return hipSuccess;
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess)
+        stream_release(s);
+    else
+        (void)hipStreamDestroy(s);
+    return e;
+}
+
 void dev_free(void* p, size_t bytes) {
     if (!p) return;
     int dev = 0;
@@ -311,7 +365,7 @@ GpuKernel::GpuKernel(const hf_kernel_desc& d) {
     // hipGraph of the same launches.  HFGPU_GRAPH=1 selects the graph replay.
     if (const char* e = std::getenv("HFGPU_GRAPH")) use_graph = e[0] == '1';
     if (const char* e = std::getenv("HFGPU_AUTOTUNE")) autotune_env = e[0] == '1';
-    HFGPU_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    stream = stream_acquire();
     const size_t cz = sizeof(cd);
     h.alloc(nd * cz);
     h2d(h.get(), d.h, nd * cz, stream);
@@ -405,8 +459,7 @@ GpuKernel::GpuKernel(const hf_kernel_desc& d) {
                 rho_bar.alloc(nb2 * cz);
             }
         } catch (...) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
+            (void)stream_sync_release(stream);
             stream = nullptr;
             throw;
         }
@@ -416,10 +469,7 @@ GpuKernel::GpuKernel(const hf_kernel_desc& d) {
 }
 
 GpuKernel::~GpuKernel() {
-    if (stream) {
-        (void)hipStreamSynchronize(stream);
-        (void)hipStreamDestroy(stream);
-    }
+    if (stream) (void)stream_sync_release(stream);
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (geo_full.g) geo_full.g->release();
     if (geo_packed.g) geo_packed.g->release();
@@ -926,13 +976,10 @@ GeomCache& geom_cache() {
 
 struct Stream {
     hipStream_t s = nullptr;
-    Stream() { HFGPU_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
-    ~Stream() {
-        if (s) {
-            (void)hipStreamSynchronize(s);
-            (void)hipStreamDestroy(s);
-        }
-    }
+    Stream() : s(stream_acquire()) {}
+    ~Stream() { (void)stream_sync_release(s); }
+    Stream(const Stream&) = delete;
+    Stream& operator=(const Stream&) = delete;
     void sync() { HFGPU_CHECK(hipStreamSynchronize(s)); }
 };
 

@@ -380,14 +380,10 @@ extern "C" int hfgpu_selfenergy_superlattice_streamed(
         in.orb = dim_orb;
         in.hermitian = hermitian_rho != 0;
         check_layout(in);  // before any device call: bad layouts fail on any host
-        hipStream_t st = nullptr;
-        HFGPU_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        hipStream_t st = stream_acquire();
         struct StreamGuard {
             hipStream_t s;
-            ~StreamGuard() {
-                (void)hipStreamSynchronize(s);
-                (void)hipStreamDestroy(s);
-            }
+            ~StreamGuard() { (void)stream_sync_release(s); }
         } guard{st};
         SlPlan plan;
         plan.init(in, st);

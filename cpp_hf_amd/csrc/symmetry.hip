@@ -210,14 +210,10 @@ extern "C" int hfgpu_project_symmetry(const hf_symmetry_desc* sym, double* M, si
         if (!sym || !M) throw std::invalid_argument("null argument");
         if (sym->nb != nb) throw std::invalid_argument("symmetry matrices must be (nb, nb)");
         if (nk1 == 0 || nk2 == 0) return;
-        hipStream_t st = nullptr;
-        HFGPU_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        hipStream_t st = stream_acquire();
         struct StreamGuard {
             hipStream_t s;
-            ~StreamGuard() {
-                (void)hipStreamSynchronize(s);
-                (void)hipStreamDestroy(s);
-            }
+            ~StreamGuard() { (void)stream_sync_release(s); }
         } guard{st};
         SymPlan plan;
         plan.init(*sym, static_cast<int>(nk1), static_cast<int>(nk2), st);
