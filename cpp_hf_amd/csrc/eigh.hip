@@ -8,6 +8,11 @@
 // 9216 matrices of 8 x 8 it takes 72 us against 168 us for a register-row
 // parallel Jacobi: the QR phase needs ~17 shifted steps of at most 7
 // rotations, cyclic Jacobi ~56 serial rounds of full-row updates.
+// At exactly 16 bands the eigenvector rows also sit in LDS (one wavefront
+// per workgroup): 256 VGPRs and two wavefronts per SIMD instead of 256 +
+// 102 AGPRs and one, 1701 -> 1473 us at 36864 matrices and 548 -> 505 us at
+// 9216, bit-identical (the same arithmetic); at 12 bands the same layout was
+// slower (239 -> 288 us at 9216).
 // nb 17..32: the same kernel with NP = 32 and the eigenvector rows in LDS
 // (registers for both A and Z would spill): at 576 matrices of 26 x 26 it
 // takes 542 us against 1281 us for the parallel cyclic Jacobi below, at
@@ -277,9 +282,10 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
     constexpr int NL = NBC > 0 ? NBC : NP;
     constexpr double kEps = 2.220446049250313e-16;
     // one TriLds per wavefront of the workgroup: 4 (256 threads), or 1 for
-    // NP = 32, whose launches are one wavefront wide (its Z rows take the
-    // dynamic LDS)
-    __shared__ TriLds<NP> lds[NP == 32 ? 1 : 4];
+    // the launches one wavefront wide whose Z rows take the dynamic LDS
+    // (NP = 32, and nb = 16 exactly)
+    constexpr bool kZLds = NP == 32 || NBC == 16;
+    __shared__ TriLds<NP> lds[kZLds ? 1 : 4];
     const int lane = threadIdx.x & 63;
     TriLds<NP>& X = lds[threadIdx.x >> 6];
     const int i = lane % NP;
@@ -300,7 +306,6 @@ k_eigh_tri(int nmat, int nb, int in_mode, const cd* __restrict__ M, double* __re
     }
     const int nb2 = nb * nb;
     const cd* Mk = M + static_cast<size_t>(live ? mat : 0) * nb2;
-    constexpr bool kZLds = NP == 32;
     cd A[NL];
     ZRow<NL, kZLds> Z;
     if constexpr (kZLds) {
@@ -554,6 +559,8 @@ static void eigh_launch(int nmat, int nb, int in_mode, const cd* M, double* w, c
             hipLaunchKernelGGL((k_eigh_tri<8, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw, seg, active);
         else if (nb == 12)  // the 12-band models: loops sized to 12, not 16
             hipLaunchKernelGGL((k_eigh_tri<16, 12>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw, seg, active);
+        else if (nb == 16)  // one wavefront per workgroup, Z rows in LDS (16 rows of 17)
+            hipLaunchKernelGGL((k_eigh_tri<16, 16>), dim3(cdiv(nmat, mpw), gy), dim3(64), sizeof(cd) * 64 * 17, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw, seg, active);
         else
             hipLaunchKernelGGL((k_eigh_tri<16, 0>), grid, dim3(256), 0, st, nmat, nb, in_mode, M, w, V, bm, so, cond, mpw, seg, active);
         HFGPU_LAUNCH_CHECK();
