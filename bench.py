@@ -741,6 +741,18 @@ def main() -> int:
     }
 
     out["autotune"] = {"default_us_per_build": default_us, **tuned}
+    # the same passes' bytes moved by three plain streaming launches (no
+    # transform): what the three-pass dataflow costs by itself on this box
+    try:
+        floor = gk.dataflow_floor(herm, 200)
+        out["roofline"]["dataflow_floor"] = {
+            "us_per_build": floor["us_per_build"],
+            "bytes_per_build": floor["bytes_per_build"],
+            "build_over_floor": per_build_s * 1e6 / floor["us_per_build"],
+            "frac_at_floor": (alg_bytes / (floor["us_per_build"] * 1e-6)) / 1e9 / HBM_PEAK_GBS,
+        }
+    except ValueError:  # a mesh without the two-step transform
+        out["roofline"]["dataflow_floor"] = None
     # matrices per wavefront of the solvers' eigh at this kernel's nk x nb
     # (results do not depend on it)
     out["autotune"]["eigh"] = {"nmat": nk, "nb": nb, **_native.eigh_autotune(nk, nb)}

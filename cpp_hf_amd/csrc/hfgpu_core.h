@@ -120,6 +120,10 @@ struct GpuKernel {
     // Launch shapes only move tasks between lanes: results are unchanged.
     // Returns the best microseconds per build (0 when nothing is tunable).
     double autotune(bool p_hermitian, int shape_out[5]);
+    // Three dependent streaming launches with the byte volumes of this
+    // kernel's two-step Fock build (no arithmetic): microseconds per build
+    // over reps builds, and the bytes one build moves.
+    double dataflow_floor(bool p_hermitian, int reps, double* bytes_per_build);
     bool autotune_env = false;  // HFGPU_AUTOTUNE=1: tune each new geometry
 
     bool packed_for(bool p_hermitian) const { return hcp || (dense_packable && p_hermitian); }

@@ -543,6 +543,82 @@ GpuKernel::Geom& GpuKernel::geom(bool packed) {
 }
 
 // ---------------------------------------------------------------------------
+// dataflow floor: the two-step build's passes as plain streaming launches
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Reads a[0..na) and b[0..nb) and writes o1[0..n1) and o2[0..n2) once each,
+// 16 B per access, coalesced; the stores depend on the loads so that
+// neither is dropped.  No transform, no LDS, no gather.
+__global__ void __launch_bounds__(256)
+k_stream_pass(const cd* __restrict__ a, size_t na, const cd* __restrict__ b, size_t nbb,
+              cd* __restrict__ o1, size_t n1, cd* __restrict__ o2, size_t n2) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const size_t t0 = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    cd acc{0.0, 0.0};
+    for (size_t i = t0; i < na; i += stride) acc = cadd(acc, a[i]);
+    for (size_t i = t0; i < nbb; i += stride) acc = cadd(acc, b[i]);
+    for (size_t i = t0; i < n1; i += stride) o1[i] = cd{acc.x + static_cast<double>(i), acc.y};
+    for (size_t i = t0; i < n2; i += stride) o2[i] = cd{acc.y, acc.x - static_cast<double>(i)};
+}
+
+}  // namespace
+
+double GpuKernel::dataflow_floor(bool p_hermitian, int reps, double* bytes_per_build) {
+    if (reps < 1) throw std::invalid_argument("dataflow_floor needs reps >= 1");
+    const bool packed = exchange && packed_for(p_hermitian);
+    Geom& G = geom(packed);
+    if (!exchange || superlattice() || !(G.g->fast_rows && G.g->fast_cols))
+        throw std::invalid_argument("dataflow_floor needs the two-step exchange");
+    // pass A: P -> T; pass B: T, V -> T; pass C: T, h -> Sigma, F (the
+    // k-independent reference density and the small terms are not read)
+    const size_t nt = G.g->tile_elems();
+    DevBuf P(nd * sizeof(cd)), T(nt * sizeof(cd)), T2(nt * sizeof(cd)), V(nt * sizeof(cd)),
+        Hh(nt * sizeof(cd)), S(nd * sizeof(cd)), F(nd * sizeof(cd));
+    fill_zero(P.get(), nd * sizeof(cd), stream);
+    fill_zero(V.get(), nt * sizeof(cd), stream);
+    fill_zero(Hh.get(), nt * sizeof(cd), stream);
+    *bytes_per_build = static_cast<double>(sizeof(cd)) *
+                       static_cast<double>(nd + nt + 2 * nt + nt + 2 * nt + 2 * nd);
+    const dim3 grid(2048), block(256);
+    auto build = [&] {
+        hipLaunchKernelGGL(k_stream_pass, grid, block, 0, stream, P.as<cd>(), nd,
+                           static_cast<const cd*>(nullptr), size_t(0), T.as<cd>(), nt,
+                           static_cast<cd*>(nullptr), size_t(0));
+        hipLaunchKernelGGL(k_stream_pass, grid, block, 0, stream, T.as<cd>(), nt, V.as<cd>(), nt,
+                           T2.as<cd>(), nt, static_cast<cd*>(nullptr), size_t(0));
+        hipLaunchKernelGGL(k_stream_pass, grid, block, 0, stream, T2.as<cd>(), nt, Hh.as<cd>(),
+                           nt, S.as<cd>(), nd, F.as<cd>(), nd);
+        HFGPU_LAUNCH_CHECK();
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HFGPU_CHECK(hipEventCreate(&e0));
+    HFGPU_CHECK(hipEventCreate(&e1));
+    struct EventGuard {
+        hipEvent_t a, b;
+        hipStream_t s;
+        ~EventGuard() {
+            (void)hipStreamSynchronize(s);
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{e0, e1, stream};
+    for (int i = 0; i < 10; ++i) build();
+    float best = 1e30f;
+    for (int t = 0; t < 3; ++t) {
+        HFGPU_CHECK(hipEventRecord(e0, stream));
+        for (int i = 0; i < reps; ++i) build();
+        HFGPU_CHECK(hipEventRecord(e1, stream));
+        HFGPU_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HFGPU_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms);
+    }
+    return 1e3 * static_cast<double>(best) / reps;
+}
+
+// ---------------------------------------------------------------------------
 // launch-shape autotuner
 // ---------------------------------------------------------------------------
 
@@ -1054,6 +1130,14 @@ int hfgpu_kernel_autotune(hfgpu_kernel* k, int p_hermitian, int* shape, double* 
     return guarded([&] {
         if (!k || !shape || !best_us) throw std::invalid_argument("null argument");
         *best_us = k->impl->autotune(p_hermitian != 0, shape);
+    });
+}
+
+int hfgpu_kernel_dataflow_floor(hfgpu_kernel* k, int p_hermitian, int reps, double* us,
+                                double* bytes) {
+    return guarded([&] {
+        if (!k || !us || !bytes) throw std::invalid_argument("null argument");
+        *us = k->impl->dataflow_floor(p_hermitian != 0, reps, bytes);
     });
 }
 

@@ -1095,6 +1095,21 @@ PYBIND11_MODULE(_native, m) {
                  return d;
              },
              py::arg("p_hermitian") = true)
+        .def("dataflow_floor",
+             [](PyKernel& k, bool p_hermitian, int reps) {
+                 double us = 0.0, bytes = 0.0;
+                 int rc;
+                 {
+                     py::gil_scoped_release rel;
+                     rc = hfgpu_kernel_dataflow_floor(k.k, p_hermitian ? 1 : 0, reps, &us, &bytes);
+                 }
+                 check(rc);
+                 py::dict d;
+                 d["us_per_build"] = us;
+                 d["bytes_per_build"] = bytes;
+                 return d;
+             },
+             py::arg("p_hermitian") = true, py::arg("reps") = 200)
         .def("mixed_capable",
              [](PyKernel& k, bool p_hermitian) {
                  return hfgpu_kernel_mixed_capable(k.k, p_hermitian ? 1 : 0) != 0;

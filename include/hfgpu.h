@@ -73,6 +73,15 @@ int hfgpu_kernel_autotune(hfgpu_kernel* k, int p_hermitian, int* shape, double* 
  * 17..32: the fixed two per wavefront, 0 us.  Results of later calls do not
  * change. */
 int hfgpu_eigh_autotune(size_t nmat, size_t nb, int* mpw, double* best_us);
+/* Measurement of the Fock build's dataflow floor (no counterpart in the
+ * reference): three dependent streaming launches per build that read and
+ * write exactly the bytes of this kernel's two-step exchange passes (P into
+ * the tile-major workspace; workspace and interaction into the workspace;
+ * workspace and packed h into Sigma and F) with no arithmetic; us:
+ * microseconds per build (best of three runs of reps builds), bytes: bytes
+ * one build moves.  HF_EINVAL when the mesh has no two-step transform. */
+int hfgpu_kernel_dataflow_floor(hfgpu_kernel* k, int p_hermitian, int reps, double* us,
+                                double* bytes);
 
 /* ------------------------------------------------------------------------
  * Exchange self-energy.

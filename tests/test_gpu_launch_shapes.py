@@ -146,3 +146,23 @@ def test_eigh_autotune(gpu, nb, nmat):
         gpu.eigh_autotune(0, nb)
     with pytest.raises(ValueError):
         gpu.eigh_autotune(nmat, 33)
+
+
+def test_dataflow_floor(gpu):
+    """The build's dataflow as three plain streaming launches: the bytes of
+    the two-step passes (P and the packed tiles in, Sigma and F out) and a
+    time below the build's own."""
+    from cpp_hf_amd.workloads import layered_kernel, qwz_kernel
+
+    wl = layered_kernel(96, 4)
+    gk = gpu.GpuKernel(kernel_args_dict(wl.kernel))
+    r = gk.dataflow_floor(True, 50)
+    print(f"dataflow floor 96x96 nb8: {r}", flush=True)
+    nd, nt = 96 * 96 * 64, 5 * 8 * 96 * 96  # 5 tiles of 8 packed channels
+    assert r["bytes_per_build"] == 16.0 * (3 * nd + 6 * nt)
+    t = gk.autotune(True)
+    assert 0.0 < r["us_per_build"] < t["us_per_build"]
+    # a mesh without the two-step transform has no such floor
+    gk13 = gpu.GpuKernel(kernel_args_dict(qwz_kernel(13).kernel))
+    with pytest.raises(ValueError):
+        gk13.dataflow_floor(True, 5)
