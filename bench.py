@@ -490,15 +490,20 @@ def run_sweep(concurrency: int) -> dict:
     gkw = dict(concurrency=8, batch=True, max_batch=3)
     solve_many(kernels, P0s, n_es, config=cfg, concurrency=concurrency)
     solve_many(kernels, P0s, n_es, config=cfg, **gkw)
-    t = time.perf_counter()
-    seq = solve_many(kernels, P0s, n_es, config=cfg, concurrency=1)
-    t_seq = time.perf_counter() - t
-    t = time.perf_counter()
-    par = solve_many(kernels, P0s, n_es, config=cfg, concurrency=concurrency)
-    t_par = time.perf_counter() - t
-    t = time.perf_counter()
-    grp = solve_many(kernels, P0s, n_es, config=cfg, **gkw)
-    t_grp = time.perf_counter() - t
+    # three timed passes of each mode, interleaved; medians reported (single
+    # passes of the concurrent modes vary by ~10 % from run to run)
+    times = {"seq": [], "par": [], "grp": []}
+    for _ in range(3):
+        t = time.perf_counter()
+        seq = solve_many(kernels, P0s, n_es, config=cfg, concurrency=1)
+        times["seq"].append(time.perf_counter() - t)
+        t = time.perf_counter()
+        par = solve_many(kernels, P0s, n_es, config=cfg, concurrency=concurrency)
+        times["par"].append(time.perf_counter() - t)
+        t = time.perf_counter()
+        grp = solve_many(kernels, P0s, n_es, config=cfg, **gkw)
+        times["grp"].append(time.perf_counter() - t)
+    t_seq, t_par, t_grp = (float(np.median(times[k])) for k in ("seq", "par", "grp"))
 
     def same(xs):
         return all(int(a.n_iter) == int(b.n_iter) and float(a.energy) == float(b.energy)
@@ -518,6 +523,7 @@ def run_sweep(concurrency: int) -> dict:
 
     return {
         "kernels": len(wls),
+        "timing": "median of 3 timed passes per mode",
         "sequential_solves_per_s": len(wls) / t_seq,
         "batched_solves_per_s": len(wls) / t_par,
         "speedup": t_seq / t_par,

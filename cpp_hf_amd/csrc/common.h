@@ -88,7 +88,9 @@ void dev_free(void* p, size_t bytes);
 
 // Non-blocking HIP streams (default priority) from a per-device pool:
 // creating and destroying a stream costs ~100 + 60 us of host time, which
-// was a large share of a small solve's set-up (about ten per solve).
+// was a large share of a small solve's set-up (about ten per solve).  Used
+// for the short-lived streams (entry points, synchronous uploads); a kernel
+// handle creates its own.
 // stream_acquire takes a stream of the current device; stream_release takes
 // back an idle stream, one the caller has synchronised without error (a
 // stream whose work failed is destroyed instead).  Pooled streams live until

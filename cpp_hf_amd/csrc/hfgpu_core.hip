@@ -365,7 +365,11 @@ GpuKernel::GpuKernel(const hf_kernel_desc& d) {
     // hipGraph of the same launches.  HFGPU_GRAPH=1 selects the graph replay.
     if (const char* e = std::getenv("HFGPU_GRAPH")) use_graph = e[0] == '1';
     if (const char* e = std::getenv("HFGPU_AUTOTUNE")) autotune_env = e[0] == '1';
-    stream = stream_acquire();
+    // the kernel's own stream (a solve's) is created, not pooled: HIP deals
+    // new streams to its hardware queues in turn, so solves running at
+    // the same time spread over the queues; a pooled stream keeps whatever
+    // queue it was created on
+    HFGPU_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     const size_t cz = sizeof(cd);
     h.alloc(nd * cz);
     h2d(h.get(), d.h, nd * cz, stream);
@@ -459,7 +463,8 @@ GpuKernel::GpuKernel(const hf_kernel_desc& d) {
                 rho_bar.alloc(nb2 * cz);
             }
         } catch (...) {
-            (void)stream_sync_release(stream);
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
             stream = nullptr;
             throw;
         }
@@ -469,7 +474,10 @@ GpuKernel::GpuKernel(const hf_kernel_desc& d) {
 }
 
 GpuKernel::~GpuKernel() {
-    if (stream) (void)stream_sync_release(stream);
+    if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+    }
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (geo_full.g) geo_full.g->release();
     if (geo_packed.g) geo_packed.g->release();
