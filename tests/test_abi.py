@@ -40,6 +40,30 @@ def test_library_reports_errors_without_a_gpu_call():
     assert b"null" in lib.hfgpu_last_error()
 
 
+def test_extension_entry_points_check_arguments_before_any_gpu_call():
+    """The autotuner and dataflow-floor entry points refuse null pointers,
+    an empty batch and a repeated kernel handle before touching the GPU."""
+    lib = C.CDLL(str(LIB))
+    lib.hfgpu_last_error.restype = C.c_char_p
+    mpw, us, nbytes = C.c_int(), C.c_double(), C.c_double()
+    assert lib.hfgpu_eigh_autotune(C.c_size_t(10), C.c_size_t(8), None, C.byref(us)) == 1
+    assert b"null" in lib.hfgpu_last_error()
+    assert lib.hfgpu_eigh_autotune(C.c_size_t(0), C.c_size_t(8), C.byref(mpw), C.byref(us)) == 1
+    assert b"nmat" in lib.hfgpu_last_error()
+    assert lib.hfgpu_kernel_dataflow_floor(None, 1, 10, C.byref(us), C.byref(nbytes)) == 1
+    assert b"null" in lib.hfgpu_last_error()
+    assert lib.hfgpu_kernel_autotune(None, 1, None, None) == 1
+    # a handle passed twice to the grouped solve (never dereferenced)
+    fake = C.c_void_p(0x1000)
+    ks = (C.c_void_p * 2)(fake, fake)
+    p0 = (C.c_void_p * 2)(C.c_void_p(0x2000), C.c_void_p(0x2000))
+    ne = (C.c_double * 2)(1.0, 1.0)
+    cfg = C.create_string_buffer(1024)
+    outs = C.create_string_buffer(2048)
+    assert lib.hfgpu_solve_dm_batch(ks, C.c_size_t(2), p0, ne, cfg, outs) == 1
+    assert b"twice" in lib.hfgpu_last_error()
+
+
 def test_native_module_exports_reference_names():
     import cpp_hf_amd._native as nat
 
