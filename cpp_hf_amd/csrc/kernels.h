@@ -70,10 +70,13 @@ void launch_mu_bisection(const double* eps, const double* w, int nk, int nb, dou
                          double T, int maxiter, MuState* state, double* ws, hipStream_t st);
 // solve_mu_inloop (utils.hpp:79-116); warm start read from warm->mu (warm
 // defaults to state) when mu0_from_state, else mu0; the result goes to state.
+// With occ, also the occupations occ = expit((mu - eps) / T) of
+// launch_occupations (mode 1), formed by the Newton launch itself where it
+// runs in one launch.
 void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
                       double n_target_norm, double mu0, bool mu0_from_state, double T,
                       int maxiter, MuState* state, double* ws, hipStream_t st,
-                      const MuState* warm = nullptr);
+                      const MuState* warm = nullptr, double* occ = nullptr);
 
 // launch_mu_newton for the S solves of a batch in two launches: solve s has
 // the nb eigenvalues of its k points seg[s] .. seg[s + 1] - 1 (seg on the
@@ -84,10 +87,11 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
 // single call's bit for bit.  Only where mu_newton_multi_ok holds (the
 // one-launch Newton paths); elsewhere call launch_mu_newton per solve.
 bool mu_newton_multi_ok(int nk_max, int nb, int maxiter);
+// With occ, also each active solve's occupations (mode 1, occ + seg[s] nb).
 void launch_mu_newton_multi(const double* eps, const double* w_norm, const int* seg, int S,
                             int nk_max, int nb, const double* n_target, double T, int maxiter,
                             bool mu0_from_state, const MuState* warm, MuState* state,
-                            double* ws, unsigned active, hipStream_t st);
+                            double* ws, unsigned active, hipStream_t st, double* occ = nullptr);
 
 // The Python surface's Newton (src/cpp_hf/utils.py:311-360) on raw weights:
 // one workgroup, result in *mu_out.

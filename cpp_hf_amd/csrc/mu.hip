@@ -361,7 +361,7 @@ constexpr size_t kNewtonSoloMax = 4096;
 __device__ __forceinline__ void newton_solo_body(
     const double* __restrict__ eps, const double* __restrict__ w_norm, int nb, size_t n,
     double n_target, double T, double mu0, int mu0_from_state, const MuState* warm, int maxiter,
-    int mm_parts, double* ws, MuState* state, double* red, double* part) {
+    int mm_parts, double* ws, MuState* state, double* red, double* part, double* occ) {
     MuState st = bracket_from(pp_part(ws, 1), mm_parts, 1, T,
                               mu0_from_state ? warm->mu : mu0, red);
     const double Tj = fmax(T, 1.0e-12);
@@ -385,17 +385,20 @@ __device__ __forceinline__ void newton_solo_body(
         __syncthreads();
     }
     if (threadIdx.x == 0) *state = st;
+    // launch_occupations (mode 1) from the final mu
+    if (occ)
+        for (size_t i = threadIdx.x; i < n; i += kBlock) occ[i] = expit_d((st.mu - eps[i]) / T);
 }
 
 __global__ void __launch_bounds__(kBlock)
 k_newton_solo(const double* __restrict__ eps, const double* __restrict__ w_norm, int nb,
               size_t n, double n_target, double T, double mu0, int mu0_from_state,
               const MuState* warm, int maxiter, int mm_parts, double* ws,
-              MuState* state) {  // warm may be state: read before the first barrier
+              MuState* state, double* occ) {  // warm may be state: read before the first barrier
     __shared__ double red[kBlock / 64];
     __shared__ double part[2];
     newton_solo_body(eps, w_norm, nb, n, n_target, T, mu0, mu0_from_state, warm, maxiter,
-                     mm_parts, ws, state, red, part);
+                     mm_parts, ws, state, red, part, occ);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -637,7 +640,7 @@ struct PersistLds {
 __device__ __forceinline__ void newton_persist_body(
     const double* __restrict__ eps, const double* __restrict__ w_norm, int nb, size_t n,
     double n_target, double T, double mu0, int mu0_from_state, const MuState* warm, int maxiter,
-    double* slots, MuState* state, int G, PersistLds& L) {
+    double* slots, MuState* state, int G, PersistLds& L, double* occ) {
     double* red = L.red;
     double* s_mine = L.s_mine;
     double* s_sum = L.s_sum;
@@ -705,15 +708,22 @@ __device__ __forceinline__ void newton_persist_body(
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *state = st;
+    // launch_occupations (mode 1) from the final mu, which every workgroup
+    // holds (the same sums and updates in each; a workgroup whose exchange
+    // timed out has NaN)
+    if (occ)
+        for (size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; i < n;
+             i += static_cast<size_t>(G) * kBlock)
+            occ[i] = expit_d((st.mu - eps[i]) / T);
 }
 
 __global__ void __launch_bounds__(kBlock)
 k_newton_persist(const double* __restrict__ eps, const double* __restrict__ w_norm, int nb,
                  size_t n, double n_target, double T, double mu0, int mu0_from_state,
-                 const MuState* warm, int maxiter, double* slots, MuState* state) {
+                 const MuState* warm, int maxiter, double* slots, MuState* state, double* occ) {
     __shared__ PersistLds L;
     newton_persist_body(eps, w_norm, nb, n, n_target, T, mu0, mu0_from_state, warm, maxiter,
-                        slots, state, static_cast<int>(gridDim.x), L);
+                        slots, state, static_cast<int>(gridDim.x), L, occ);
 }
 
 // ---- one launch pair for the Newton solves of a batch -----------------------
@@ -762,7 +772,7 @@ __global__ void __launch_bounds__(kBlock)
 k_newton_multi(const double* __restrict__ eps, const double* __restrict__ w_norm, int nb,
                const int* __restrict__ seg, const double* __restrict__ n_target, double T,
                int mu0_from_state, const MuState* warm, int maxiter, double* ws,
-               MuState* state, unsigned active) {
+               MuState* state, unsigned active, double* occ) {
     __shared__ PersistLds L;
     __shared__ double part[2];
     const int sv = blockIdx.y;
@@ -772,14 +782,15 @@ k_newton_multi(const double* __restrict__ eps, const double* __restrict__ w_norm
     const int G = newton_grid_of(n);
     if (static_cast<int>(blockIdx.x) >= G) return;
     double* w = ws + static_cast<size_t>(sv) * kMuPartialDoubles;
+    double* o = occ ? occ + ko * nb : nullptr;
     if (n <= kNewtonSoloMax)
         newton_solo_body(eps + ko * nb, w_norm + ko, nb, n, n_target[sv], T, 0.0,
                          mu0_from_state, warm + sv, maxiter, mm_blocks_of(n), w, state + sv,
-                         L.red, part);
+                         L.red, part, o);
     else
         newton_persist_body(eps + ko * nb, w_norm + ko, nb, n, n_target[sv], T, 0.0,
                             mu0_from_state, warm + sv, maxiter, w + kPersistSlotBase,
-                            state + sv, G, L);
+                            state + sv, G, L, o);
 }
 
 // ---- the Python surface's Newton (utils.py:311-360) -------------------------
@@ -954,7 +965,7 @@ void launch_mu_bisection(const double* eps, const double* w, int nk, int nb, dou
 void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
                       double n_target_norm, double mu0, bool mu0_from_state, double T,
                       int maxiter, MuState* state, double* ws, hipStream_t st,
-                      const MuState* warm) {
+                      const MuState* warm, double* occ) {
     if (!warm) warm = state;
     const size_t n = static_cast<size_t>(nk) * nb;
     // a few eigenvalues per thread: each step also re-reduces the previous
@@ -967,7 +978,7 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
             fill_slots(ws, 1 + maxiter, G, st);
             hipLaunchKernelGGL(k_newton_persist, dim3(G), dim3(kBlock), 0, st, eps, w_norm, nb, n,
                                n_target_norm, T, mu0, mu0_from_state ? 1 : 0, warm, maxiter,
-                               ws + kPersistSlotBase, state);
+                               ws + kPersistSlotBase, state, occ);
             HFGPU_LAUNCH_CHECK();
             return;
         }
@@ -977,7 +988,7 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
     if (maxiter > 0 && n <= kNewtonSoloMax) {
         hipLaunchKernelGGL(k_newton_solo, dim3(1), dim3(kBlock), 0, st, eps, w_norm, nb, n,
                            n_target_norm, T, mu0, mu0_from_state ? 1 : 0, warm, maxiter,
-                           static_cast<int>(mm_blocks), ws, state);
+                           static_cast<int>(mm_blocks), ws, state, occ);
         HFGPU_LAUNCH_CHECK();
         return;
     }
@@ -1003,6 +1014,7 @@ void launch_mu_newton(const double* eps, const double* w_norm, int nk, int nb,
                            0, 0, ws, state);
         HFGPU_LAUNCH_CHECK();
     }
+    if (occ) launch_occupations(eps, n, state, T, 1, occ, st);
 }
 
 bool mu_newton_multi_ok(int nk_max, int nb, int maxiter) {
@@ -1014,7 +1026,7 @@ bool mu_newton_multi_ok(int nk_max, int nb, int maxiter) {
 void launch_mu_newton_multi(const double* eps, const double* w_norm, const int* seg, int S,
                             int nk_max, int nb, const double* n_target, double T, int maxiter,
                             bool mu0_from_state, const MuState* warm, MuState* state,
-                            double* ws, unsigned active, hipStream_t st) {
+                            double* ws, unsigned active, hipStream_t st, double* occ) {
     if (!mu_newton_multi_ok(nk_max, nb, maxiter))
         throw std::logic_error("launch_mu_newton_multi outside the one-launch Newton paths");
     const size_t n_max = static_cast<size_t>(nk_max) * nb;
@@ -1031,7 +1043,7 @@ void launch_mu_newton_multi(const double* eps, const double* w_norm, const int* 
     const unsigned gm = n_max > kNewtonSoloMax ? static_cast<unsigned>(persist_grid(n_max)) : 1u;
     hipLaunchKernelGGL(k_newton_multi, dim3(gm, static_cast<unsigned>(S)), dim3(kBlock), 0, st,
                        eps, w_norm, nb, seg, n_target, T, mu0_from_state ? 1 : 0, warm, maxiter,
-                       ws, state, active);
+                       ws, state, active, occ);
     HFGPU_LAUNCH_CHECK();
 }
 

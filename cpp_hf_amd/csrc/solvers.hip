@@ -692,8 +692,7 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
 
     auto mu_solve = [&](const double* e, bool warm, double mu0) {
         launch_mu_newton(e, wd, nk, nb, n_target, mu0, warm, T_r, cfg.mu_maxiter, W.mu(),
-                         W.mu_partial.as<double>(), st);
-        launch_occupations(e, nkb, W.mu(), T_r, 1, p.as<double>(), st);
+                         W.mu_partial.as<double>(), st, nullptr, p.as<double>());
     };
     auto density_step = [&]() {
         // the hermitising density kernel equals X followed by a hermitisation
@@ -835,8 +834,8 @@ void solve_dm_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_config
             launch_retract(nk, nb, Qs[s0], Vd.as<cd>(), lam.as<double>(), Sigma.as<cd>(),
                            eps_w.as<double>(), true, L, st, Qs[s1]);
             launch_mu_newton(eps_w.as<double>(), wd, nk, nb, n_target, 0.0, true, T_r,
-                             cfg.mu_maxiter, ms[s1], W.mu_partial.as<double>(), st, ms[s0]);
-            launch_occupations(eps_w.as<double>(), nkb, ms[s1], T_r, 1, ps[s1], st);
+                             cfg.mu_maxiter, ms[s1], W.mu_partial.as<double>(), st, ms[s0],
+                             ps[s1]);
         };
         auto enqueue = [&](size_t j) {
             const bool restart = (j == 0) || (cfg.cg_restart > 0 && j % cfg.cg_restart == 0);
@@ -1073,19 +1072,22 @@ void solve_dm_batch_gpu(const std::vector<GpuKernel*>& Ks, const std::vector<con
     };
     // chemical potentials: one launch pair for the batch where the one-launch
     // Newton paths apply, else one launch sequence per solve
+    // (with the occupations of the active solves in occ, as launch_occupations
+    // mode 1)
     auto mu_solve = [&](const double* e, MuState* out, const MuState* warm, bool use_warm,
-                        unsigned act) {
+                        unsigned act, double* occ) {
         if (mu_multi) {
             launch_mu_newton_multi(e, wd, seg_d.as<int>(), S, nk_max, nb, nt_d.as<double>(), T_r,
                                    cfg.mu_maxiter, use_warm, warm, out, mu_ws.as<double>(), act,
-                                   st);
+                                   st, occ);
             return;
         }
         for (int s = 0; s < S; ++s)
             if ((act >> s) & 1u)
                 launch_mu_newton(e + static_cast<size_t>(koff[s]) * nb, wd + koff[s],
                                  koff[s + 1] - koff[s], nb, n_target[s], 0.0, use_warm, T_r,
-                                 cfg.mu_maxiter, out + s, mu_ws.as<double>(), st, warm + s);
+                                 cfg.mu_maxiter, out + s, mu_ws.as<double>(), st, warm + s,
+                                 occ + static_cast<size_t>(koff[s]) * nb);
     };
     // Fock builds and energies: the scalar slot array decides where the
     // energies land, so there is one FockBatch per slot array
@@ -1114,9 +1116,7 @@ void solve_dm_batch_gpu(const std::vector<GpuKernel*>& Ks, const std::vector<con
     launch_hermitize(static_cast<int>(nk_tot), nb, P.as<cd>(), st);
     fb[0].build(all, false, st);
     launch_eigh(static_cast<int>(nk_tot), nb, 0, F.as<cd>(), eps.as<double>(), Q.as<cd>(), st);
-    mu_solve(eps.as<double>(), mus.as<MuState>(), mus.as<MuState>(), false, all);
-    launch_occupations_batch(eps.as<double>(), static_cast<size_t>(nk_max) * nb, S, all,
-                             mus.as<MuState>(), T_r, 1, p.as<double>(), st, L.seg, nb);
+    mu_solve(eps.as<double>(), mus.as<MuState>(), mus.as<MuState>(), false, all, p.as<double>());
 
     // per-solve host state
     std::vector<std::vector<double>> hist_E(SS), hist_G(SS);
@@ -1182,9 +1182,7 @@ void solve_dm_batch_gpu(const std::vector<GpuKernel*>& Ks, const std::vector<con
                           static_cast<int>(cfg.bt_max), L, st);
         launch_retract(nkm, nb, Qs[s0], Vd.as<cd>(), lam.as<double>(), Sigma.as<cd>(),
                        eps_w.as<double>(), true, L, st, Qs[s1]);
-        mu_solve(eps_w.as<double>(), ms[s1], ms[s0], true, act);
-        launch_occupations_batch(eps_w.as<double>(), static_cast<size_t>(nk_max) * nb, S, act,
-                                 ms[s1], T_r, 1, ps[s1], st, L.seg, nb);
+        mu_solve(eps_w.as<double>(), ms[s1], ms[s0], true, act, ps[s1]);
         d2h(hs + static_cast<size_t>(s0) * SS * SC_COUNT, L.sc, SS * SC_COUNT * sizeof(double),
             st);
         HFGPU_CHECK(hipEventRecord(ev[s0].e, st));
@@ -1236,9 +1234,8 @@ void solve_dm_batch_gpu(const std::vector<GpuKernel*>& Ks, const std::vector<con
                         false, nullptr, nullptr, nullptr, L, st);
     launch_eigh(static_cast<int>(nk_tot), nb, 0, F.as<cd>(), eps_w.as<double>(), Vd.as<cd>(), st);
     launch_retract(nkm, nb, Q.as<cd>(), Vd.as<cd>(), nullptr, nullptr, nullptr, false, L, st);
-    mu_solve(eps_w.as<double>(), mus.as<MuState>(), mus.as<MuState>(), true, all);
-    launch_occupations_batch(eps_w.as<double>(), static_cast<size_t>(nk_max) * nb, S, all,
-                             mus.as<MuState>(), T_r, 1, p.as<double>(), st, L.seg, nb);
+    mu_solve(eps_w.as<double>(), mus.as<MuState>(), mus.as<MuState>(), true, all,
+             p.as<double>());
     launch_density(static_cast<int>(nk_tot), nb, Q.as<cd>(), p.as<double>(), P.as<cd>(), true, st);
     fb[0].build(all, true, st);
 
@@ -1388,9 +1385,8 @@ void solve_rtr_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_confi
     launch_eigh(nk, nb, 0, Fb.as<cd>(), cur->eps.as<double>(), cur->Q.as<cd>(), st);
     fill_zero(cur->mus.get(), sizeof(MuState), st);
     launch_mu_newton(cur->eps.as<double>(), wd, nk, nb, n_target, 0.0, false, T_r,
-                     cfg.mu_maxiter, cur->mus.as<MuState>(), W.mu_partial.as<double>(), st);
-    launch_occupations(cur->eps.as<double>(), nkb, cur->mus.as<MuState>(), T_r, 1,
-                       cur->p.as<double>(), st);
+                     cfg.mu_maxiter, cur->mus.as<MuState>(), W.mu_partial.as<double>(), st,
+                     nullptr, cur->p.as<double>());
     eval_state(*cur);
     EvalOut now = eval_result();
 
@@ -1472,9 +1468,8 @@ void solve_rtr_gpu(GpuKernel& K, const double* P0, double n_e, const hf_dm_confi
                            trial->Q.as<cd>(), eps_t.as<double>(), st);
         d2d(trial->mus.get(), cur->mus.get(), sizeof(MuState), st);
         launch_mu_newton(eps_t.as<double>(), wd, nk, nb, n_target, 0.0, true, T_r,
-                         cfg.mu_maxiter, trial->mus.as<MuState>(), W.mu_partial.as<double>(), st);
-        launch_occupations(eps_t.as<double>(), nkb, trial->mus.as<MuState>(), T_r, 1,
-                           trial->p.as<double>(), st);
+                         cfg.mu_maxiter, trial->mus.as<MuState>(), W.mu_partial.as<double>(), st,
+                         nullptr, trial->p.as<double>());
         eval_state(*trial);
         const EvalOut tr = eval_result();
 
