@@ -426,7 +426,11 @@ k_newton_last(double n_target, int j, int prev_parts, int apply, double* ws, MuS
 
 constexpr int kPersistSlotBase = 16384;  // after the multi-launch layout
 constexpr int kPersistK = 8;             // slot doubles per workgroup per round
-constexpr int kPersistMaxG = 256;
+// At most 64 workgroups per persistent solve: a persistent grid only makes
+// progress once all its workgroups are resident, and solves run side by side
+// on streams (16 of them at 64 x 4 wavefronts fill half the workgroup slots
+// of the GPU, so they can all be resident); 256 measured no faster.
+constexpr int kPersistMaxG = 64;
 constexpr unsigned long long kEmptySlot = 0x7FF4DEADBEEF0001ull;
 constexpr long long kSpinMax = 1ll << 24;
 
