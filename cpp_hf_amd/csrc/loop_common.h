@@ -86,13 +86,51 @@ __device__ __forceinline__ void wave_mm_part(const cd* A, const cd* B, int nb, b
     }
 }
 
-// store(e, C_e) for every element of C = op(A) op(B) by one wavefront: up to
-// four elements per lane at a time.
+// C = op(A) op(B) for 9..16 bands on the FP64 matrix cores: the blocks
+// padded to 16 x 16 with zeros, Re C = Ar Br - Ai Bi and Im C = Ar Bi + Ai Br
+// as four v_mfma_f64_16x16x4f64 steps over n each.  Operand lanes (gfx950):
+// A lane l holds A[l % 16][l / 16] of a 16 x 4 step, B lane l holds
+// B[l / 16][l % 16], result value r of lane l is C[4 r + l / 16][l % 16];
+// every lane reads its 2 x 4 operands from LDS once.  All 64 lanes of the
+// wavefront must be active.
+typedef double loop_d4_t __attribute__((ext_vector_type(4)));
+
+template <class Store>
+__device__ __forceinline__ void wave_mm_mfma(const cd* A, const cd* B, int nb, bool ta, bool tb,
+                                             int lane, Store&& store) {
+    const int i = lane & 15, q = lane >> 4;
+    loop_d4_t cre = {0.0, 0.0, 0.0, 0.0}, cim = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int n = 4 * c + q;
+        const bool ok = i < nb && n < nb;
+        cd a{0.0, 0.0}, b{0.0, 0.0};
+        if (ok) {
+            a = ta ? cconj(A[n * nb + i]) : A[i * nb + n];
+            b = tb ? cconj(B[i * nb + n]) : B[n * nb + i];
+        }
+        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, cre, 0, 0, 0);
+        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, cre, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, cim, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, cim, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = 4 * r + q;
+        if (row < nb && i < nb) store(row * nb + i, cd{cre[r], cim[r]});
+    }
+}
+
+// store(e, C_e) for every element of C = op(A) op(B) by one wavefront: on the
+// matrix cores for 9..16 bands when mfma, else up to four elements per lane
+// at a time.
 template <class Store>
 __device__ __forceinline__ void wave_mm_each(const cd* A, const cd* B, int nb, bool ta, bool tb,
-                                             int lane, Store&& store) {
+                                             int lane, Store&& store, bool mfma = false) {
     const int nb2 = nb * nb;
-    if (nb2 <= 64) {
+    if (mfma && nb > 8 && nb <= 16) {
+        wave_mm_mfma(A, B, nb, ta, tb, lane, store);
+    } else if (nb2 <= 64) {
         wave_mm_part<1>(A, B, nb, ta, tb, lane, 0, store);
     } else if (nb2 <= 128) {
         wave_mm_part<2>(A, B, nb, ta, tb, lane, 0, store);
@@ -106,8 +144,8 @@ __device__ __forceinline__ void wave_mm_each(const cd* A, const cd* B, int nb, b
 
 // C = op(A) op(B) on nb x nb LDS blocks by one wavefront.
 __device__ __forceinline__ void wave_mm(const cd* A, const cd* B, cd* C, int nb, bool ta,
-                                        bool tb, int lane) {
-    wave_mm_each(A, B, nb, ta, tb, lane, [C](int e, cd v) { C[e] = v; });
+                                        bool tb, int lane, bool mfma = false) {
+    wave_mm_each(A, B, nb, ta, tb, lane, [C](int e, cd v) { C[e] = v; }, mfma);
 }
 
 // Sum NS per-thread values over the block and write partial[block * NS + s].

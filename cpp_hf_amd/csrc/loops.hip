@@ -20,6 +20,8 @@
 // launch shape, and no workgroup needs a device-scope fence.
 #include <cmath>
 
+#include <cstdlib>
+
 #include "loop_common.h"
 
 namespace hfgpu {
@@ -100,7 +102,7 @@ k_orbital_grad(int nk, int nb_arg, const cd* __restrict__ Q, cd* __restrict__ F,
                const double* __restrict__ p, const double* __restrict__ w, const MuState* mus,
                double T_r, int grad, double* __restrict__ eps, cd* __restrict__ G,
                double* __restrict__ g_p, double* partial, double* sc, unsigned active,
-               const int* seg) {
+               const int* seg, int mfma) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     extern __shared__ cd smem[];
     __shared__ double red[W * 2];
@@ -139,9 +141,9 @@ k_orbital_grad(int nk, int nb_arg, const cd* __restrict__ Q, cd* __restrict__ F,
                 sF[e] = F[kb + e];
             }
         wave_lds_sync();
-        if (act) wave_mm(sQ, sF, sT, nb, true, false, lane);
+        if (act) wave_mm(sQ, sF, sT, nb, true, false, lane, mfma);
         wave_lds_sync();
-        if (act) wave_mm(sT, sQ, sF, nb, false, false, lane);
+        if (act) wave_mm(sT, sQ, sF, nb, false, false, lane, mfma);
         wave_lds_sync();
         if (act) {
             for (int e = lane; e < nb2; e += 64) F[kb + e] = sF[e];
@@ -321,7 +323,7 @@ k_cg_combine(int nk, int nb_arg, int restart, cd* __restrict__ dQ, const cd* __r
 // spectrum, as in diag_UFU_from_spectrum.  Result for row i in out[i].
 __device__ __forceinline__ void diag_at_tau(const cd* sV, const cd* sFe, const double* slam,
                                             cd* sVC, cd* sc_, double* srow, int nb, double tau,
-                                            int lane) {
+                                            int lane, bool mfma) {
     const int nb2 = nb * nb;
     for (int n = lane; n < nb; n += 64) sc_[n] = cayley_c(slam[n], tau);
     wave_lds_sync();
@@ -330,10 +332,13 @@ __device__ __forceinline__ void diag_at_tau(const cd* sV, const cd* sFe, const d
         sVC[e] = cmulc(sV[e], sc_[n]);  // V_in conj(c_n)
     }
     wave_lds_sync();
-    wave_mm_each(sVC, sFe, nb, false, false, lane, [&](int e, cd a) {
-        a = cmul(a, sc_[e % nb]);
-        srow[e] = cmulc(a, sV[e]).x;
-    });
+    wave_mm_each(
+        sVC, sFe, nb, false, false, lane,
+        [&](int e, cd a) {
+            a = cmul(a, sc_[e % nb]);
+            srow[e] = cmulc(a, sV[e]).x;
+        },
+        mfma);
     wave_lds_sync();
 }
 
@@ -349,7 +354,7 @@ k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restr
              const cd* __restrict__ Ft, cd* __restrict__ Fe, const double* __restrict__ p,
              const double* __restrict__ d_p, const double* __restrict__ w, double T_r,
              double max_step, double shrink, int j_lo, int j_hi, int stage2,
-             double* partial, const double* sc, unsigned active, const int* seg) {
+             double* partial, const double* sc, unsigned active, const int* seg, int mfma) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     const SolveSeg sg = solve_seg<W>(seg, nk);
     const int gx = sg.gx;
@@ -400,16 +405,16 @@ k_linesearch(int nk, int nb_arg, const cd* __restrict__ V, const double* __restr
         }
         wave_lds_sync();
         if (!stage2) {
-            if (act) wave_mm(sV, sF, sT, nb, true, false, lane);
+            if (act) wave_mm(sV, sF, sT, nb, true, false, lane, mfma);
             wave_lds_sync();
-            if (act) wave_mm(sT, sV, sF, nb, false, false, lane);  // sF <- Fe
+            if (act) wave_mm(sT, sV, sF, nb, false, false, lane, mfma);  // sF <- Fe
             wave_lds_sync();
             if (act)
                 for (int e = lane; e < nb2; e += 64) Fe[kb + e] = sF[e];
         }
         double tau = tau_lo;
         for (int j = j_lo; j < j_hi; ++j) {
-            diag_at_tau(sV, sF, slam, sVC, scf, srow, nb, tau, lane);
+            diag_at_tau(sV, sF, slam, sVC, scf, srow, nb, tau, lane, mfma);
             double ej = 0.0, sj = 0.0;
             if (act) {
                 const double wk = w[k];
@@ -448,7 +453,7 @@ __global__ void __launch_bounds__(W * 64)
 k_retract(int nk, int nb_arg, const cd* Q, const cd* __restrict__ V,
           const double* __restrict__ lam, const cd* __restrict__ Fe,
           double* __restrict__ eps_work, int cayley, const double* sc, cd* Q_out,
-          unsigned active, const int* seg) {
+          unsigned active, const int* seg, int mfma) {
     const int nb = NB > 0 ? NB : nb_arg;  // compile-time for the common sizes
     const SolveSeg sg = solve_seg<W>(seg, nk);
     const int gx = sg.gx;
@@ -502,21 +507,21 @@ k_retract(int nk, int nb_arg, const cd* Q, const cd* __restrict__ V,
         }
         wave_lds_sync();
         if (act) {
-            if (cayley) wave_mm(sVC, sV, sU, nb, false, true, lane);
+            if (cayley) wave_mm(sVC, sV, sU, nb, false, true, lane, mfma);
         }
         wave_lds_sync();
         if (act) {
             // Q_new = Q U (cayley) or Q V (finalise)
             const cd* R = cayley ? sU : sV;
             cd* Qk = Q_out + kb;
-            wave_mm_each(sQ, R, nb, false, false, lane, [Qk](int e, cd v) { Qk[e] = v; });
+            wave_mm_each(sQ, R, nb, false, false, lane, [Qk](int e, cd v) { Qk[e] = v; }, mfma);
         }
         wave_lds_sync();
         if (cayley) {
             if (act)
                 for (int e = lane; e < nb2; e += 64) sFe[e] = Fe[kb + e];
             wave_lds_sync();
-            diag_at_tau(sV, sFe, slam, sVC, scf, srow, nb, tau, lane);
+            diag_at_tau(sV, sFe, slam, sVC, scf, srow, nb, tau, lane, mfma);
             if (act)
                 for (int i = lane; i < nb; i += 64) {
                     double di = 0.0;
@@ -673,6 +678,20 @@ k_lincomb(size_t n, CdPtrs X, DotCoefs c, int nx, int accumulate, double a, doub
 // launchers
 // ---------------------------------------------------------------------------
 
+// The per-k kernels' products run on the matrix cores at 9..16 bands unless
+// HFGPU_LOOP_MFMA=0 (read once, on the host; a launch argument).
+static bool loop_mfma_on() {
+    static const bool on = [] {
+        const char* e = std::getenv("HFGPU_LOOP_MFMA");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+// the launch argument of the per-k kernels' products: matrix cores at 9..16
+// bands
+static int loop_mfma(int nb) { return loop_mfma_on() && nb > 8 && nb <= 16 ? 1 : 0; }
+
 void launch_finalize_sums(const double* partial, int nparts, int ns, double* out,
                           hipStream_t st) {
     hipLaunchKernelGGL(k_finalize<0>, dim3(1), dim3(256), 0, st, partial, nparts, ns, out,
@@ -695,7 +714,8 @@ void launch_orbital_grad(int nk, int nb, const cd* Q, cd* F, const double* p, co
     check_nb(nb);
     const size_t per_wave = sizeof(cd) * 3 * nb * nb;
     HFGPU_LOOP_LAUNCH(k_orbital_grad, nb, nk, per_wave, st, ls.S, nk, nb, Q, F, p, w, mus, T_r,
-                      grad ? 1 : 0, eps, G, g_p, ls.partial, ls.sc, ls.active, ls.seg);
+                      grad ? 1 : 0, eps, G, g_p, ls.partial, ls.sc, ls.active, ls.seg,
+                      loop_mfma(nb));
     HFGPU_LAUNCH_CHECK();
     if (grad) finalize(ls, loop_blocks(nk, waves_for(nb)), 2, ls.sc + SC_GN2, st, waves_for(nb));
 }
@@ -745,7 +765,7 @@ void launch_linesearch(int nk, int nb, const cd* V, const double* lam, const cd*
         const int last = stage + 2 == nst;
         HFGPU_LOOP_LAUNCH(k_linesearch, nb, nk, per_wave, st, ls.S, nk, nb, V, lam, Ft, Fe, p,
                           d_p, w, T_r, max_step, shrink, j_lo, j_hi, stage, ls.partial, ls.sc,
-                          ls.active, ls.seg);
+                          ls.active, ls.seg, loop_mfma(nb));
         HFGPU_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_finalize<1>, dim3(ls.S), dim3(256), 0, st, ls.partial, nparts,
                            2 * kMaxBt, ls.sc + SC_TRIAL, ls.sc, nbt, shrink, T_r, max_step, j_lo,
@@ -759,7 +779,7 @@ void launch_retract(int nk, int nb, cd* Q, const cd* V, const double* lam, const
     check_nb(nb);
     const size_t per_wave = ls_wave_bytes(nb);
     HFGPU_LOOP_LAUNCH(k_retract, nb, nk, per_wave, st, ls.S, nk, nb, Q, V, lam, Fe, eps_work,
-                      cayley ? 1 : 0, ls.sc, Q_out ? Q_out : Q, ls.active, ls.seg);
+                      cayley ? 1 : 0, ls.sc, Q_out ? Q_out : Q, ls.active, ls.seg, loop_mfma(nb));
     HFGPU_LAUNCH_CHECK();
 }
 
