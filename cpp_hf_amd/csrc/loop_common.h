@@ -91,8 +91,9 @@ __device__ __forceinline__ void wave_mm_part(const cd* A, const cd* B, int nb, b
 // as four v_mfma_f64_16x16x4f64 steps over n each.  Operand lanes (gfx950):
 // A lane l holds A[l % 16][l / 16] of a 16 x 4 step, B lane l holds
 // B[l / 16][l % 16], result value r of lane l is C[4 r + l / 16][l % 16];
-// every lane reads its 2 x 4 operands from LDS once.  All 64 lanes of the
-// wavefront must be active.
+// every lane reads its 2 x 4 operands from LDS once; steps whose n range lies
+// wholly in the padding are skipped.  All 64 lanes of the wavefront must be
+// active.
 typedef double loop_d4_t __attribute__((ext_vector_type(4)));
 
 template <class Store>
@@ -102,6 +103,7 @@ __device__ __forceinline__ void wave_mm_mfma(const cd* A, const cd* B, int nb, b
     loop_d4_t cre = {0.0, 0.0, 0.0, 0.0}, cim = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
+        if (4 * c >= nb) continue;  // an all-zero step (uniform over the wavefront)
         const int n = 4 * c + q;
         const bool ok = i < nb && n < nb;
         cd a{0.0, 0.0}, b{0.0, 0.0};
