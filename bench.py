@@ -572,6 +572,9 @@ def main() -> int:
     ap.add_argument("--kernel-sample", action="store_true",
                     help="only run the solver kernels a few times each for a profiler: eigh "
                          "and density_from_fock at 96 x 96 x 8 and 192 x 192 x 16")
+    ap.add_argument("--loop-sample", action="store_true",
+                    help="only run one DM solve of a 96 x 96 k, 12-band sweep kernel three "
+                         "times after one warm-up solve, for a profiler of the per-k loop kernels")
     ap.add_argument("--sweep-sample", choices=("sequential", "grouped"), default=None,
                     help="only run the 48-kernel sweep once after one warm-up pass, one solve "
                          "after another or as band-count groups one after another (no "
@@ -592,6 +595,22 @@ def main() -> int:
         dt = time.perf_counter() - t
         print(json.dumps({"sweep_sample": args.sweep_sample, "seconds": dt,
                           "total_iterations": int(sum(int(r.n_iter) for r in rs))}), flush=True)
+        return 0
+
+    if args.loop_sample:
+        from cpp_hf_amd import SolverConfig, solve
+        from cpp_hf_amd.workloads import sweep_params, sweep_workload
+
+        wl = sweep_workload(sweep_params(48, seed=1)[7])  # 96 x 96 k, 12 bands
+        solve(wl.kernel, wl.P0, wl.n_electrons, config=SolverConfig())
+        ts, r = [], None
+        for _ in range(3):
+            t = time.perf_counter()
+            r = solve(wl.kernel, wl.P0, wl.n_electrons, config=SolverConfig())
+            ts.append(time.perf_counter() - t)
+        print(json.dumps({"loop_sample": "DM solve, 96x96 k, 12 bands (sweep kernel 7 of the "
+                                         "seed-1 set)", "seconds": ts, "n_iter": int(r.n_iter),
+                          "energy": float(r.energy)}), flush=True)
         return 0
 
     if args.kernel_sample:

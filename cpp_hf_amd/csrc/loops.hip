@@ -30,8 +30,10 @@ namespace {
 
 using namespace loopk;
 
-// Finalising launch: out[s] = sum over workgroups of partial[b * ns + s], in
-// workgroup order within each lane, then a fixed wave tree.  MODE 1 also
+// Finalising launch: out[s] = sum over workgroups of partial[b * ns + s].
+// Thread t sums the rows b = t, t + 256, ... of up to kFinSlots adjacent
+// slots at once (every load of a pass in flight together), then a fixed
+// wave tree per slot and the four wavefronts in a fixed order.  MODE 1 also
 // selects the line-search step from the trial free energies (the reference's
 // sequential acceptance test, solver_dm.hpp:561-589).
 template <int MODE>
@@ -52,21 +54,41 @@ k_finalize(const double* __restrict__ partial, int nparts, int ns, double* __res
     // 575-587); the last stage applies the no-acceptance rule; gate: a
     // trial accepted by an earlier stage stands
     if (MODE == 1 && gate && sc[SC_ACCEPTED] > 0.5) return;
+    constexpr int kFinSlots = 8;
     __shared__ double s_sum[2 * kMaxBt];
+    __shared__ double s_red[4][kFinSlots];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // MODE 1 needs only the slots of the trials it tests
     const int s_lo = MODE == 1 ? 2 * j_lo : 0, s_hi = MODE == 1 ? 2 * j_hi : ns;
-    for (int slot = s_lo + wave; slot < s_hi; slot += 4) {
-        double t = 0.0;
-        for (int b = lane; b < nparts; b += 64) t += partial[b * ns + slot];
-        t = wave_sum(t);
-        if (lane == 0) {
-            out[slot] = t;
-            if (MODE == 1) s_sum[slot] = t;
+    for (int s0 = s_lo; s0 < s_hi; s0 += kFinSlots) {
+        const int nsl = min(kFinSlots, s_hi - s0);  // uniform over the workgroup
+        double acc[kFinSlots];
+#pragma unroll
+        for (int j = 0; j < kFinSlots; ++j) acc[j] = 0.0;
+#pragma unroll 4
+        for (int b = threadIdx.x; b < nparts; b += 256) {
+            const double* row = partial + static_cast<size_t>(b) * ns + s0;
+#pragma unroll
+            for (int j = 0; j < kFinSlots; ++j)
+                if (j < nsl) acc[j] += row[j];
         }
+#pragma unroll
+        for (int j = 0; j < kFinSlots; ++j) {
+            if (j < nsl) {
+                const double t = wave_sum(acc[j]);
+                if (lane == 0) s_red[wave][j] = t;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < nsl) {
+            const int j = threadIdx.x;
+            const double t = (s_red[0][j] + s_red[1][j]) + (s_red[2][j] + s_red[3][j]);
+            out[s0 + j] = t;
+            if (MODE == 1) s_sum[s0 + j] = t;
+        }
+        __syncthreads();
     }
     if (MODE != 1) return;
-    __syncthreads();
     if (threadIdx.x != 0) return;
     const double dqn = sqrt(fmax(0.0, sc[SC_DQN2]));
     const double tau0 = fmin(1.0, max_step / fmax(dqn, TINY_REAL));
